@@ -1,0 +1,319 @@
+"""ctypes binding of the CPU FP64 oracle -- test infrastructure only.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py``
+import this module, and only as the checker.  The product package
+``cardiacfoam_amd`` never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
+_lib = None
+
+BO, TNNP, ORD, AP = 0, 1, 2, 3
+EPI, MCELL, ENDO, MYOCYTE = 1, 2, 3, 4
+MAX_STATES, MAX_CONSTS = 41, 160
+NSTATES = {BO: 4, TNNP: 17, ORD: 41, AP: 2}
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class Stim(C.Structure):
+    _fields_ = [("start", C.c_double), ("period_s1", C.c_double), ("duration", C.c_double),
+                ("amplitude", C.c_double), ("nstim1", C.c_int), ("period_s2", C.c_double),
+                ("nstim2", C.c_int)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("hx", C.c_double),
+                ("hy", C.c_double), ("hz", C.c_double), ("n_cells", C.c_long),
+                ("n_faces", C.c_long), ("owner", C.POINTER(C.c_int32)),
+                ("neighbour", C.POINTER(C.c_int32)), ("face_dir", C.POINTER(C.c_int8))]
+
+
+class Boxes(C.Structure):
+    _fields_ = [("n_boxes", C.c_int), ("box_min", C.POINTER(C.c_double)),
+                ("box_max", C.POINTER(C.c_double)), ("t_start", C.POINTER(C.c_double)),
+                ("duration", C.POINTER(C.c_double)), ("intensity", C.POINTER(C.c_double))]
+
+
+class TissueCfg(C.Structure):
+    _fields_ = [("model", C.c_int), ("tissue", C.c_int), ("integrator", C.c_int),
+                ("n_sub", C.c_int), ("abs_tol", C.c_double), ("rel_tol", C.c_double),
+                ("ddt", C.c_int), ("implicit", C.c_int), ("pcg_tol", C.c_double),
+                ("pcg_max_iter", C.c_int), ("chi", C.c_double), ("cm", C.c_double),
+                ("dt", C.c_double), ("nthreads", C.c_int)]
+
+
+class Tissue(C.Structure):
+    _fields_ = [("mesh", Mesh), ("cfg", TissueCfg), ("nstates", C.c_int),
+                ("consts", C.c_double * MAX_CONSTS), ("states", C.POINTER(C.c_double)),
+                ("step", C.POINTER(C.c_double)), ("sigma", C.POINTER(C.c_double)),
+                ("V", C.POINTER(C.c_double)), ("V0", C.POINTER(C.c_double)),
+                ("V00", C.POINTER(C.c_double)), ("Iion", C.POINTER(C.c_double)),
+                ("src", C.POINTER(C.c_double)), ("tact", C.POINTER(C.c_double)),
+                ("flag", C.POINTER(C.c_uint8)), ("work", C.POINTER(C.c_double)),
+                ("t", C.c_double), ("step_index", C.c_long), ("pcg_iters_total", C.c_long)]
+
+
+class Beat(C.Structure):
+    _fields_ = [("t_up", C.c_double), ("v_base", C.c_double), ("v_peak", C.c_double),
+                ("v90", C.c_double), ("t_repol90", C.c_double), ("repol_found", C.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_model_nstates.argtypes = [C.c_int]
+        L.orc_model_init.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_model_const_index.argtypes = [C.c_int, C.c_char_p]
+        L.orc_model_eval.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_int,
+                                     C.POINTER(Stim), _dp, _ip, _dp, _dp,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_stimulus.argtypes = [C.c_double, C.POINTER(Stim)]
+        L.orc_stimulus.restype = C.c_double
+        L.orc_rl_step.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_int,
+                                  C.POINTER(Stim)]
+        L.orc_rkf45_solve.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.POINTER(Stim),
+                                      C.c_double, C.c_double, C.POINTER(C.c_double), C.c_double,
+                                      C.c_double, C.c_long]
+        L.orc_rkf45_solve.restype = C.c_long
+        L.orc_ionic_tissue_step.argtypes = [C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long,
+                                            C.c_double, C.c_double, _dp, _dp, C.c_int, C.c_int,
+                                            _dp, C.c_double, C.c_double, C.c_int]
+        L.orc_single_cell_run.argtypes = [C.c_int, C.c_int, _dp, _dp, C.POINTER(Stim), C.c_double,
+                                          C.c_long, C.c_int, C.c_double, C.c_double, _dp, _dp]
+        L.orc_model_vm_mV.argtypes = [C.c_int, _dp]
+        L.orc_model_vm_mV.restype = C.c_double
+        L.orc_mesh_build.argtypes = [C.POINTER(Mesh), C.c_int, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, C.c_double]
+        L.orc_mesh_free.argtypes = [C.POINTER(Mesh)]
+        L.orc_laplacian.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, C.c_int]
+        L.orc_correction_div.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, C.c_int]
+        L.orc_assemble_implicit.argtypes = [C.POINTER(Mesh), _dp, C.c_double, C.c_double,
+                                            C.c_double, _dp, _dp]
+        L.orc_ldu_amul.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, _dp]
+        L.orc_pcg.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, _dp, C.c_double, C.c_int,
+                              C.POINTER(C.c_double)]
+        L.orc_box_stimulus.argtypes = [C.POINTER(Mesh), C.POINTER(Boxes), C.c_double, _dp]
+        L.orc_activation.argtypes = [C.c_long, _dp, _dp, _u8, _dp, C.c_double, C.c_double]
+        L.orc_explicit_dt.argtypes = [C.POINTER(Mesh), _dp, C.c_double, C.c_double]
+        L.orc_explicit_dt.restype = C.c_double
+        L.orc_tissue_create.argtypes = [C.POINTER(Tissue), C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.c_double, C.c_double, C.POINTER(TissueCfg), _dp]
+        L.orc_tissue_free.argtypes = [C.POINTER(Tissue)]
+        L.orc_tissue_step.argtypes = [C.POINTER(Tissue), C.POINTER(Boxes)]
+        L.orc_detect_beats.argtypes = [_dp, _dp, C.c_long, C.POINTER(Beat), C.c_int, C.c_double,
+                                       C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+# ----------------------------------------------------------------- helpers
+def make_stim(start=0.0, period_s1=0.0, duration=0.0, amplitude=0.0, nstim1=None,
+              period_s2=0.0, nstim2=0):
+    """stimulusIO.C:142-194 defaults: nstim1 = 1 if a positive S1 period is given."""
+    if nstim1 is None:
+        nstim1 = 1 if period_s1 > 1e-15 else 0
+    return Stim(start, period_s1, duration, amplitude, nstim1, period_s2, nstim2)
+
+
+def model_init(model, tissue):
+    c = np.zeros(MAX_CONSTS)
+    y = np.zeros(MAX_STATES)
+    if lib().orc_model_init(model, tissue, c, y) != 0:
+        raise ValueError("unsupported tissue %r for model %r" % (tissue, model))
+    return c, y[:NSTATES[model]].copy()
+
+
+def const_index(model, name):
+    return lib().orc_model_const_index(model, name.encode())
+
+
+def model_eval(model, tissue, t, c, y, solve_vm=False, stim=None):
+    n = NSTATES[model]
+    yy = np.zeros(MAX_STATES)
+    yy[:n] = y
+    r = np.zeros(MAX_STATES)
+    g = np.zeros(MAX_STATES, dtype=np.int32)
+    xi = np.zeros(MAX_STATES)
+    ta = np.zeros(MAX_STATES)
+    iion = C.c_double()
+    ist = C.c_double()
+    lib().orc_model_eval(model, tissue, t, np.ascontiguousarray(c, dtype=np.float64), yy,
+                         int(solve_vm), C.byref(stim) if stim else None, r, g, xi, ta,
+                         C.byref(iion), C.byref(ist))
+    return dict(rates=r[:n], gate=g[:n].astype(bool), xinf=xi[:n], tau=ta[:n],
+                iion=iion.value, istim=ist.value)
+
+
+def single_cell_run(model, tissue, stim, dt_s, n_steps, integrator=1, abs_tol=1e-6,
+                    rel_tol=1e-4, consts=None, y0=None):
+    c, y = model_init(model, tissue)
+    if consts is not None:
+        c = np.array(consts, dtype=np.float64)
+    if y0 is not None:
+        y = np.array(y0, dtype=np.float64)
+    yy = np.zeros(MAX_STATES)
+    yy[:len(y)] = y
+    t = np.zeros(n_steps)
+    v = np.zeros(n_steps)
+    rc = lib().orc_single_cell_run(model, tissue, c, yy, C.byref(stim), dt_s, n_steps,
+                                   integrator, abs_tol, rel_tol, t, v)
+    if rc != 0:
+        raise RuntimeError("single-cell integration failed")
+    return t, v, yy[:NSTATES[model]]
+
+
+def ionic_tissue_step(model, tissue, consts, states, t0, dt, Vm, integrator=0, n_sub=1,
+                      step=None, abs_tol=1e-6, rel_tol=1e-4, c_stride=0, nthreads=8):
+    """states: [nstates, n] float64 (modified in place); returns Im (V/s)."""
+    n = states.shape[1]
+    Im = np.zeros(n)
+    if step is None:
+        step = np.full(n, dt)
+    lib().orc_ionic_tissue_step(model, tissue, np.ascontiguousarray(consts, dtype=np.float64),
+                                c_stride, states, n, t0, dt,
+                                np.ascontiguousarray(Vm, dtype=np.float64), Im, integrator,
+                                n_sub, step, abs_tol, rel_tol, nthreads)
+    return Im
+
+
+class OracleMesh:
+    def __init__(self, nx, ny, nz, hx, hy, hz):
+        self.m = Mesh()
+        if lib().orc_mesh_build(C.byref(self.m), nx, ny, nz, hx, hy, hz) != 0:
+            raise MemoryError
+        self.n = nx * ny * nz
+        nf = self.m.n_faces
+        self.owner = np.ctypeslib.as_array(self.m.owner, shape=(nf,)).copy()
+        self.neighbour = np.ctypeslib.as_array(self.m.neighbour, shape=(nf,)).copy()
+        self.face_dir = np.ctypeslib.as_array(
+            C.cast(self.m.face_dir, C.POINTER(C.c_int8)), shape=(nf,)).copy()
+
+    def __del__(self):
+        try:
+            lib().orc_mesh_free(C.byref(self.m))
+        except Exception:
+            pass
+
+    def laplacian(self, sigma9, V, nthreads=8):
+        out = np.zeros(self.n)
+        lib().orc_laplacian(C.byref(self.m), np.ascontiguousarray(sigma9, dtype=np.float64),
+                            np.ascontiguousarray(V, dtype=np.float64), out, nthreads)
+        return out
+
+    def correction_div(self, sigma9, V, nthreads=8):
+        out = np.zeros(self.n)
+        lib().orc_correction_div(C.byref(self.m), np.ascontiguousarray(sigma9, dtype=np.float64),
+                                 np.ascontiguousarray(V, dtype=np.float64), out, nthreads)
+        return out
+
+    def assemble_implicit(self, sigma9, chiCm, dt, coefft=1.0):
+        diag = np.zeros(self.n)
+        upper = np.zeros(self.m.n_faces)
+        lib().orc_assemble_implicit(C.byref(self.m), np.ascontiguousarray(sigma9, dtype=np.float64),
+                                    chiCm, dt, coefft, diag, upper)
+        return diag, upper
+
+    def amul(self, diag, upper, x):
+        y = np.zeros(self.n)
+        lib().orc_ldu_amul(C.byref(self.m), diag, upper, np.ascontiguousarray(x, dtype=np.float64), y)
+        return y
+
+    def pcg(self, diag, upper, b, x0, tol=1e-10, max_iter=10000):
+        x = np.array(x0, dtype=np.float64)
+        res = C.c_double()
+        it = lib().orc_pcg(C.byref(self.m), diag, upper, np.ascontiguousarray(b, dtype=np.float64),
+                           x, tol, max_iter, C.byref(res))
+        return x, it, res.value
+
+    def explicit_dt(self, sigma9, chiCm, maxCo):
+        return lib().orc_explicit_dt(C.byref(self.m), np.ascontiguousarray(sigma9, dtype=np.float64),
+                                     chiCm, maxCo)
+
+
+def make_boxes(box_min, box_max, t_start, duration, intensity):
+    arrs = [np.ascontiguousarray(np.ravel(a), dtype=np.float64)
+            for a in (box_min, box_max, t_start, duration, intensity)]
+    b = Boxes(len(arrs[2]), *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+    b._keep = arrs
+    return b
+
+
+class OracleTissue:
+    """FP64 tissue run of the reference's step order (myocardiumDomain.C:487-529)."""
+
+    def __init__(self, nx, ny, nz, h, sigma9, model, tissue, dt, chi, cm, integrator=0,
+                 n_sub=1, ddt=0, implicit=0, pcg_tol=1e-10, pcg_max_iter=20000,
+                 abs_tol=1e-6, rel_tol=1e-4, nthreads=8):
+        hx, hy, hz = (h, h, h) if np.isscalar(h) else h
+        self.T = Tissue()
+        cfg = TissueCfg(model, tissue, integrator, n_sub, abs_tol, rel_tol, ddt, implicit,
+                        pcg_tol, pcg_max_iter, chi, cm, dt, nthreads)
+        sig = np.ascontiguousarray(sigma9, dtype=np.float64)
+        if sig.size == 9:
+            sig = np.tile(sig.reshape(9), nx * ny * nz)
+        rc = lib().orc_tissue_create(C.byref(self.T), nx, ny, nz, hx, hy, hz, C.byref(cfg), sig)
+        if rc != 0:
+            raise ValueError("orc_tissue_create failed (%d)" % rc)
+        self.n = nx * ny * nz
+        self.nstates = NSTATES[model]
+
+    def __del__(self):
+        try:
+            lib().orc_tissue_free(C.byref(self.T))
+        except Exception:
+            pass
+
+    def step(self, boxes=None):
+        lib().orc_tissue_step(C.byref(self.T), C.byref(boxes) if boxes is not None else None)
+
+    def _arr(self, ptr, n):
+        return np.ctypeslib.as_array(ptr, shape=(n,))
+
+    @property
+    def V(self):
+        return self._arr(self.T.V, self.n)
+
+    @property
+    def Iion(self):
+        return self._arr(self.T.Iion, self.n)
+
+    @property
+    def tact(self):
+        return self._arr(self.T.tact, self.n)
+
+    @property
+    def flag(self):
+        return np.ctypeslib.as_array(self.T.flag, shape=(self.n,))
+
+    @property
+    def states(self):
+        return self._arr(self.T.states, self.n * self.nstates).reshape(self.nstates, self.n)
+
+    @property
+    def t(self):
+        return self.T.t
+
+
+def detect_beats(t, vm, max_beats=256, dvdt_threshold=5000.0, baseline_s=0.002, peak_s=2.0):
+    beats = (Beat * max_beats)()
+    n = lib().orc_detect_beats(np.ascontiguousarray(t, dtype=np.float64),
+                               np.ascontiguousarray(vm, dtype=np.float64), len(t), beats,
+                               max_beats, dvdt_threshold, baseline_s, peak_s)
+    return [dict(t_up=b.t_up, v_base=b.v_base, v_peak=b.v_peak, v90=b.v90,
+                 t_repol90=b.t_repol90, repol_found=bool(b.repol_found)) for b in beats[:n]]

@@ -1,0 +1,713 @@
+/*
+ * Oracle (test infrastructure): FP64 restatement of the four ionic models on the
+ * tissue path.  Each model exposes one evaluation that returns the rates, the
+ * Rush-Larsen classification (xinf, tau) of each state, the total ionic current
+ * and the single-cell stimulus.
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   BuenoOrovio  - singleCell.reference golden (RKF45 mode), NiedererEtAl2012.reference
+ *                  golden (tissue), hand-computed RHS values.
+ *   AlievPanfilov- closed-form RHS (cubic), hand-computed values.
+ *   TNNP, ORd    - no golden in the reference; pinned by invariants: rest stability,
+ *                  RL exactness, published action-potential shape (APD90, peak,
+ *                  resting potential), charge/concentration conservation.  The exact
+ *                  generated expressions remain "parity unpinned" beyond that.
+ */
+#include <math.h>
+#include <string.h>
+#include "oracle.h"
+
+/* Heaviside of heaviside.H:39-49: strictly positive argument -> 1 */
+static inline double H(double x) { return x > 0.0 ? 1.0 : 0.0; }
+
+double orc_stimulus(double t, const orc_stim_t* s)
+{
+    /* stimulusIO.C:238-286 */
+    if (!s) return 0.0;
+    double I = 0.0;
+    if (s->period_s1 > 0.0 && s->nstim1 > 0) {
+        double tp = t - s->start;
+        if (tp >= 0 && tp <= s->period_s1 * s->nstim1) {
+            double ph = tp - floor(tp / s->period_s1) * s->period_s1;
+            if (ph >= 0 && ph <= s->duration) I = -s->amplitude;
+        }
+    }
+    if (I == 0.0 && s->period_s2 > 0.0 && s->nstim2 > 0) {
+        double t1 = s->start + s->period_s1 * s->nstim1;
+        double tp = t - t1;
+        if (tp >= 0 && tp <= s->period_s2 * s->nstim2) {
+            double ph = tp - floor(tp / s->period_s2) * s->period_s2;
+            if (ph >= 0 && ph <= s->duration) I = -s->amplitude;
+        }
+    }
+    return I;
+}
+
+/* =============================================================== BuenoOrovio
+ * BuenoOrovio_2008.H:116-225; constant order follows BuenoOrovio_2008Names.H. */
+enum { BO_uo, BO_uu, BO_thv, BO_thw, BO_thvm, BO_tho, BO_tv1m, BO_tv2m, BO_tvp,
+       BO_tw1m, BO_tw2m, BO_kwm, BO_uwm, BO_twp, BO_tfi, BO_to1, BO_to2, BO_tso1,
+       BO_tso2, BO_kso, BO_uso, BO_ts1, BO_ts2, BO_ks, BO_us, BO_tsi, BO_twinf,
+       BO_winfstar, BO_NC };
+
+static void bo_init(int tf, double* c, double* y)
+{
+    /* tissue flag: 1 epi, 2 M, 3 endo (anything else takes the endo column) */
+    int e = tf == 1, m = tf == 2;
+    c[BO_uo] = 0.0;
+    c[BO_uu] = e ? 1.55 : m ? 1.61 : 1.56;
+    c[BO_thv] = 0.3;
+    c[BO_thw] = 0.13;
+    c[BO_thvm] = e ? 0.006 : m ? 0.1 : 0.2;
+    c[BO_tho] = m ? 0.005 : 0.006;
+    c[BO_tv1m] = e ? 60 : m ? 80 : 75;
+    c[BO_tv2m] = e ? 1150 : m ? 1.4506 : 10;
+    c[BO_tvp] = 1.4506;
+    c[BO_tw1m] = e ? 60 : m ? 70 : 6;
+    c[BO_tw2m] = e ? 15 : m ? 8 : 140;
+    c[BO_kwm] = e ? 65 : 200;
+    c[BO_uwm] = e ? 0.03 : 0.016;
+    c[BO_twp] = e ? 200 : 280;
+    c[BO_tfi] = e ? 0.11 : m ? 0.078 : 0.1;
+    c[BO_to1] = e ? 400 : m ? 410 : 470;
+    c[BO_to2] = m ? 7 : 6;
+    c[BO_tso1] = e ? 30.0181 : m ? 91 : 40;
+    c[BO_tso2] = e ? 0.9957 : m ? 0.8 : 1.2;
+    c[BO_kso] = e ? 2.0458 : m ? 2.1 : 2;
+    c[BO_uso] = m ? 0.6 : 0.65;
+    c[BO_ts1] = 2.7342;
+    c[BO_ts2] = e ? 16 : m ? 4 : 2;
+    c[BO_ks] = 2.0994;
+    c[BO_us] = 0.9087;
+    c[BO_tsi] = e ? 1.8875 : m ? 3.3849 : 2.9013;
+    c[BO_twinf] = e ? 0.07 : m ? 0.01 : 0.0273;
+    c[BO_winfstar] = e ? 0.94 : m ? 0.5 : 0.78;
+    y[0] = 0.0; y[1] = 1.0; y[2] = 1.0; y[3] = 0.0;
+}
+
+static void bo_eval(double t, const double* c, const double* y, int solve_vm,
+                    const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                    double* iion, double* istim)
+{
+    const double u = y[0], v = y[1], w = y[2], s = y[3];
+    const double Hv = H(u - c[BO_thv]), Hw = H(u - c[BO_thw]);
+    const double Ho = H(u - c[BO_tho]), Hvm = H(u - c[BO_thvm]);
+    const double tso = c[BO_tso1] + (c[BO_tso2] - c[BO_tso1]) *
+                       (1.0 + tanh(c[BO_kso] * (u - c[BO_uso]))) / 2.0;
+    const double to = (1.0 - Ho) * c[BO_to1] + Ho * c[BO_to2];
+    const double Jfi = -v * Hv * (u - c[BO_thv]) * (c[BO_uu] - u) / c[BO_tfi];
+    const double Jso = (u - c[BO_uo]) * (1.0 - Hw) / to + Hw / tso;
+    const double Jsi = -Hw * w * s / c[BO_tsi];
+    const double Jion = Jfi + Jso + Jsi;
+    const double tvm = (1.0 - Hvm) * c[BO_tv1m] + Hvm * c[BO_tv2m];
+    const double twm = c[BO_tw1m] + (c[BO_tw2m] - c[BO_tw1m]) *
+                       (1.0 + tanh(c[BO_kwm] * (u - c[BO_uwm]))) / 2.0;
+    const double tS = (1.0 - Hw) * c[BO_ts1] + Hw * c[BO_ts2];
+    const double vinf = H(c[BO_thvm] - u);
+    const double winf = (1.0 - Ho) * (1.0 - u / c[BO_twinf]) + Ho * c[BO_winfstar];
+    const double sinf = (1.0 + tanh(c[BO_ks] * (u - c[BO_us]))) / 2.0;
+
+    r[1] = (1.0 - Hv) * (vinf - v) / tvm - Hv * v / c[BO_tvp];
+    r[2] = (1.0 - Hw) * (winf - w) / twm - Hw * w / c[BO_twp];
+    r[3] = (sinf - s) / tS;
+    /* the same rates written as (xinf - x)/tau for Rush-Larsen */
+    g[0] = 0;
+    g[1] = 1; xi[1] = Hv > 0 ? 0.0 : vinf; ta[1] = Hv > 0 ? c[BO_tvp] : tvm;
+    g[2] = 1; xi[2] = Hw > 0 ? 0.0 : winf; ta[2] = Hw > 0 ? c[BO_twp] : twm;
+    g[3] = 1; xi[3] = sinf; ta[3] = tS;
+
+    const double Ist = orc_stimulus(t, st);
+    r[0] = solve_vm ? -Jion - Ist : 0.0;
+    *iion = Jion; *istim = Ist;
+}
+
+/* ===================================================================== TNNP
+ * ten Tusscher-Noble-Noble-Panfilov 2004, 17 states: TNNP_2004.H:309-509.
+ * State order of TNNP_2004Names.H: V K_i Na_i Ca_i Xr1 Xr2 Xs m h j d f fCa s r
+ * Ca_SR g. */
+enum { TN_V, TN_Ki, TN_Nai, TN_Cai, TN_Xr1, TN_Xr2, TN_Xs, TN_m, TN_h, TN_j, TN_d,
+       TN_f, TN_fCa, TN_s, TN_r, TN_CaSR, TN_g, TN_NS };
+enum { TC_R, TC_T, TC_F, TC_Cm, TC_Vc, TC_Pkna, TC_Ko, TC_Nao, TC_Cao, TC_gK1, TC_gKr,
+       TC_gKs, TC_gNa, TC_gbNa, TC_gCaL, TC_gbCa, TC_gto, TC_PNaK, TC_Kmk, TC_KmNa,
+       TC_KNaCa, TC_Ksat, TC_alpha, TC_gamma, TC_KmCa, TC_KmNai, TC_gpCa, TC_KpCa,
+       TC_gpK, TC_taug, TC_arel, TC_brel, TC_crel, TC_Kup, TC_Vleak, TC_Vmaxup,
+       TC_Bufc, TC_Kbufc, TC_Bufsr, TC_Kbufsr, TC_Vsr, TC_taufCa, TC_NC };
+
+static void tnnp_init(int tf, double* c, double* y)
+{
+    c[TC_R] = 8.314; c[TC_T] = 310; c[TC_F] = 96.485; c[TC_Cm] = 185; c[TC_Vc] = 16404;
+    c[TC_Pkna] = 0.03; c[TC_Ko] = 5.4; c[TC_Nao] = 140; c[TC_Cao] = 2;
+    c[TC_gK1] = 5.405; c[TC_gKr] = 0.096;
+    c[TC_gKs] = tf == 2 ? 0.062 : 0.245;
+    c[TC_gNa] = 14.838; c[TC_gbNa] = 0.00029; c[TC_gCaL] = 0.175; c[TC_gbCa] = 0.000592;
+    c[TC_gto] = tf == 3 ? 0.073 : 0.294;
+    c[TC_PNaK] = 1.362; c[TC_Kmk] = 1; c[TC_KmNa] = 40; c[TC_KNaCa] = 1000;
+    c[TC_Ksat] = 0.1; c[TC_alpha] = 2.5; c[TC_gamma] = 0.35; c[TC_KmCa] = 1.38;
+    c[TC_KmNai] = 87.5; c[TC_gpCa] = 0.825; c[TC_KpCa] = 0.0005; c[TC_gpK] = 0.0146;
+    c[TC_taug] = 2; c[TC_arel] = 0.016464; c[TC_brel] = 0.25; c[TC_crel] = 0.008232;
+    c[TC_Kup] = 0.00025; c[TC_Vleak] = 8e-5; c[TC_Vmaxup] = 0.000425;
+    c[TC_Bufc] = 0.15; c[TC_Kbufc] = 0.001; c[TC_Bufsr] = 10; c[TC_Kbufsr] = 0.3;
+    c[TC_Vsr] = 1094; c[TC_taufCa] = 2.0;
+
+    y[TN_V] = -86.2; y[TN_Ki] = 138.3; y[TN_Nai] = 11.6; y[TN_Cai] = 0.0002;
+    y[TN_Xr1] = 0; y[TN_Xr2] = 1; y[TN_Xs] = 0; y[TN_m] = 0; y[TN_h] = 0.75;
+    y[TN_j] = 0.75; y[TN_d] = 0; y[TN_f] = 1; y[TN_fCa] = 1; y[TN_s] = 1; y[TN_r] = 0;
+    y[TN_CaSR] = 0.2; y[TN_g] = 1;
+}
+
+static void tnnp_eval(int tf, double t, const double* c, const double* y, int solve_vm,
+                      const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                      double* iion, double* istim)
+{
+    const double V = y[TN_V], Cai = y[TN_Cai], Nai = y[TN_Nai], Ki = y[TN_Ki];
+    const double RTF = c[TC_R] * c[TC_T] / c[TC_F];
+    const double FRT = c[TC_F] / (c[TC_R] * c[TC_T]);
+    for (int i = 0; i < TN_NS; ++i) g[i] = 0;
+
+    /* INa gates */
+    double minf = 1.0 / pow(1.0 + exp((-56.86 - V) / 9.03), 2.0);
+    double am = 1.0 / (1.0 + exp((-60.0 - V) / 5.0));
+    double bm = 0.1 / (1.0 + exp((V + 35.0) / 5.0)) + 0.1 / (1.0 + exp((V - 50.0) / 200.0));
+    double tm = 1.0 * am * bm;
+    double hinf = 1.0 / pow(1.0 + exp((V + 71.55) / 7.43), 2.0);
+    double ah = V < -40.0 ? 0.057 * exp(-(V + 80.0) / 6.8) : 0.0;
+    double bh = V < -40.0 ? 2.7 * exp(0.079 * V) + 310000.0 * exp(0.3485 * V)
+                          : 0.77 / (0.13 * (1.0 + exp((V + 10.66) / -11.1)));
+    double th = 1.0 / (ah + bh);
+    double jinf = hinf;
+    double aj = V < -40.0 ? ((-25428.0 * exp(0.2444 * V) - 6.948e-06 * exp(-0.04391 * V)) *
+                             (V + 37.78) / 1.0) / (1.0 + exp(0.311 * (V + 79.23)))
+                          : 0.0;
+    double bj = V < -40.0 ? 0.02424 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)))
+                          : 0.6 * exp(0.057 * V) / (1.0 + exp(-0.1 * (V + 32.0)));
+    double tj = 1.0 / (aj + bj);
+    /* ICaL gates */
+    double dinf = 1.0 / (1.0 + exp((-5.0 - V) / 7.5));
+    double ad = 1.4 / (1.0 + exp((-35.0 - V) / 13.0)) + 0.25;
+    double bd_ = 1.4 / (1.0 + exp((V + 5.0) / 5.0));
+    double gd = 1.0 / (1.0 + exp((50.0 - V) / 20.0));
+    double td = 1.0 * ad * bd_ + gd;
+    double finf = 1.0 / (1.0 + exp((V + 20.0) / 7.0));
+    double tf_ = 1125.0 * exp(-pow(V + 27.0, 2.0) / 240.0) + 80.0 +
+                 165.0 / (1.0 + exp((25.0 - V) / 10.0));
+    double afca = 1.0 / (1.0 + pow(Cai / 0.000325, 8.0));
+    double bfca = 0.1 / (1.0 + exp((Cai - 0.0005) / 0.0001));
+    double cfca = 0.2 / (1.0 + exp((Cai - 0.00075) / 0.0008));
+    double fcainf = (afca + bfca + cfca + 0.23) / 1.46;
+    /* Ito gates (tissue-conditional) */
+    double sinf, ts;
+    if (tf == 1) {
+        sinf = 1.0 / (1.0 + exp((V + 20.0) / 5.0));
+        ts = 85.0 * exp(-pow(V + 45.0, 2.0) / 320.0) + 5.0 / (1.0 + exp((V - 20.0) / 5.0)) + 3.0;
+    } else {
+        sinf = 1.1 / (1.0 + exp((V + 28.0) / 6.0));
+        ts = 1000.0 * exp(-pow(V + 67.0, 2.0) / 1000.0) + 8.0;
+    }
+    double rinf = 1.0 / (1.0 + exp((20.0 - V) / 6.0));
+    double tr = 9.5 * exp(-pow(V + 40.0, 2.0) / 1800.0) + 0.8;
+    /* IKr / IKs gates */
+    double xr1inf = 1.0 / (1.0 + exp((-26.0 - V) / 7.0));
+    double axr1 = 450.0 / (1.0 + exp((-45.0 - V) / 10.0));
+    double bxr1 = 6.0 / (1.0 + exp((V + 30.0) / 11.5));
+    double txr1 = 1.0 * axr1 * bxr1;
+    double xr2inf = 1.0 / (1.0 + exp((V + 88.0) / 24.0));
+    double axr2 = 3.0 / (1.0 + exp((-60.0 - V) / 20.0));
+    double bxr2 = 1.12 / (1.0 + exp((V - 60.0) / 20.0));
+    double txr2 = 1.0 * axr2 * bxr2;
+    double xsinf = 1.0 / (1.0 + exp((-5.0 - V) / 14.0));
+    double axs = 1100.0 / pow(1.0 + exp((-10.0 - V) / 6.0), 0.5);
+    double bxs = 1.0 / (1.0 + exp((V - 60.0) / 20.0));
+    double txs = 1.0 * axs * bxs;
+    /* RyR gate g */
+    double ginf = Cai < 0.00035 ? 1.0 / (1.0 + pow(Cai / 0.00035, 6.0))
+                                : 1.0 / (1.0 + pow(Cai / 0.00035, 16.0));
+
+#define TN_GATE(idx, inf, tau) do { r[idx] = ((inf) - y[idx]) / (tau); g[idx] = 1; \
+                                    xi[idx] = (inf); ta[idx] = (tau); } while (0)
+    TN_GATE(TN_m, minf, tm);
+    TN_GATE(TN_h, hinf, th);
+    TN_GATE(TN_j, jinf, tj);
+    TN_GATE(TN_d, dinf, td);
+    TN_GATE(TN_f, finf, tf_);
+    TN_GATE(TN_s, sinf, ts);
+    TN_GATE(TN_r, rinf, tr);
+    TN_GATE(TN_Xr1, xr1inf, txr1);
+    TN_GATE(TN_Xr2, xr2inf, txr2);
+    TN_GATE(TN_Xs, xsinf, txs);
+#undef TN_GATE
+    /* fCa and g: the rate is frozen at 0 while the gate would open above -60 mV
+     * (TNNP_2004.H:422,459); otherwise first-order relaxation.  RL keeps the frozen
+     * value by setting xinf = current value. */
+    {
+        int frz = fcainf > y[TN_fCa] && V > -60.0;
+        r[TN_fCa] = frz ? 0.0 : (fcainf - y[TN_fCa]) / c[TC_taufCa];
+        g[TN_fCa] = 1; xi[TN_fCa] = frz ? y[TN_fCa] : fcainf; ta[TN_fCa] = c[TC_taufCa];
+        frz = ginf > y[TN_g] && V > -60.0;
+        r[TN_g] = frz ? 0.0 : (ginf - y[TN_g]) / c[TC_taug];
+        g[TN_g] = 1; xi[TN_g] = frz ? y[TN_g] : ginf; ta[TN_g] = c[TC_taug];
+    }
+
+    /* currents */
+    double INaK = ((c[TC_PNaK] * c[TC_Ko] / (c[TC_Ko] + c[TC_Kmk])) * Nai / (Nai + c[TC_KmNa])) /
+                  (1.0 + 0.1245 * exp(-0.1 * V * FRT) + 0.0353 * exp(-V * FRT));
+    double ENa = RTF * log(c[TC_Nao] / Nai);
+    double INa = c[TC_gNa] * pow(y[TN_m], 3.0) * y[TN_h] * y[TN_j] * (V - ENa);
+    double IbNa = c[TC_gbNa] * (V - ENa);
+    double INaCa = c[TC_KNaCa] *
+                   (exp(c[TC_gamma] * V * FRT) * pow(Nai, 3.0) * c[TC_Cao] -
+                    exp((c[TC_gamma] - 1.0) * V * FRT) * pow(c[TC_Nao], 3.0) * Cai * c[TC_alpha]) /
+                   ((pow(c[TC_KmNai], 3.0) + pow(c[TC_Nao], 3.0)) * (c[TC_KmCa] + c[TC_Cao]) *
+                    (1.0 + c[TC_Ksat] * exp((c[TC_gamma] - 1.0) * V * FRT)));
+    double EK = RTF * log(c[TC_Ko] / Ki);
+    double aK1 = 0.1 / (1.0 + exp(0.06 * ((V - EK) - 200.0)));
+    double bK1 = (3.0 * exp(0.0002 * ((V - EK) + 100.0)) + 1.0 * exp(0.1 * ((V - EK) - 10.0))) /
+                 (1.0 + exp(-0.5 * (V - EK)));
+    double xK1 = aK1 / (aK1 + bK1);
+    double IK1 = c[TC_gK1] * xK1 * pow(c[TC_Ko] / 5.4, 0.5) * (V - EK);
+    double Ito = c[TC_gto] * y[TN_r] * y[TN_s] * (V - EK);
+    double IKr = c[TC_gKr] * pow(c[TC_Ko] / 5.4, 0.5) * y[TN_Xr1] * y[TN_Xr2] * (V - EK);
+    double EKs = RTF * log((c[TC_Ko] + c[TC_Pkna] * c[TC_Nao]) / (Ki + c[TC_Pkna] * Nai));
+    double IKs = c[TC_gKs] * pow(y[TN_Xs], 2.0) * (V - EKs);
+    double e2 = exp(2.0 * V * FRT);
+    double ICaL = (c[TC_gCaL] * y[TN_d] * y[TN_f] * y[TN_fCa] * 4.0 * V * pow(c[TC_F], 2.0) /
+                   (c[TC_R] * c[TC_T])) * (Cai * e2 - 0.341 * c[TC_Cao]) / (e2 - 1.0);
+    double ECa = 0.5 * RTF * log(c[TC_Cao] / Cai);
+    double IbCa = c[TC_gbCa] * (V - ECa);
+    double IpK = c[TC_gpK] * (V - EK) / (1.0 + exp((25.0 - V) / 5.98));
+    double IpCa = c[TC_gpCa] * Cai / (Cai + c[TC_KpCa]);
+    double Iion = IK1 + Ito + IKr + IKs + ICaL + INaK + INa + IbNa + INaCa + IbCa + IpK + IpCa;
+    double Ist = orc_stimulus(t, st);
+
+    r[TN_V] = solve_vm ? -Iion - Ist : 0.0;
+    r[TN_Nai] = -(INa + IbNa + 3.0 * INaK + 3.0 * INaCa) * c[TC_Cm] / (c[TC_Vc] * c[TC_F]);
+    /* TNNP_2004.H:495: the potassium balance includes the stimulus current */
+    r[TN_Ki] = -((IK1 + Ito + IKr + IKs + IpK + Ist) - 2.0 * INaK) * c[TC_Cm] / (c[TC_Vc] * c[TC_F]);
+    double Irel = (c[TC_arel] * pow(y[TN_CaSR], 2.0) / (pow(c[TC_brel], 2.0) + pow(y[TN_CaSR], 2.0)) +
+                   c[TC_crel]) * y[TN_d] * y[TN_g];
+    double Iup = c[TC_Vmaxup] / (1.0 + pow(c[TC_Kup], 2.0) / pow(Cai, 2.0));
+    double Ileak = c[TC_Vleak] * (y[TN_CaSR] - Cai);
+    double dCai = ((-((ICaL + IbCa + IpCa) - 2.0 * INaCa) / (2.0 * c[TC_Vc] * c[TC_F])) * c[TC_Cm] +
+                   Ileak - Iup) + Irel;
+    double fCai = 1.0 / (1.0 + c[TC_Bufc] * c[TC_Kbufc] / pow(Cai + c[TC_Kbufc], 2.0));
+    r[TN_Cai] = dCai * fCai;
+    double dCasr = (c[TC_Vc] / c[TC_Vsr]) * (Iup - (Irel + Ileak));
+    double fCasr = 1.0 / (1.0 + c[TC_Bufsr] * c[TC_Kbufsr] / pow(y[TN_CaSR] + c[TC_Kbufsr], 2.0));
+    r[TN_CaSR] = dCasr * fCasr;
+    *iion = Iion; *istim = Ist;
+}
+
+/* ====================================================================== ORd
+ * O'Hara-Rudy 2011, 41 states: ORd_2011.H:493-989, state order of ORd_2011Names.H.
+ *
+ * Deviation: ORd_2011.H's ORdinitConsts never assigns 27 constants of INaK and SERCA
+ * (Knai0, Knao0, delta, Kki, Kko, MgADP, MgATP, Kmgatp, H, eP, Khp, Knap, Kxkur,
+ * k1p..k4m, a2, a4, b1, Pnak, upScale; CONSTANTS_ is zero-filled, ORd.C:54), which
+ * makes INaK = 0*NaN = NaN in the reference.  The published ORd-2011 values
+ * (O'Hara et al., PLoS Comput Biol 2011, endocardial column) are used here.  The
+ * K_i rate omits the stimulus, as the reference does inside derivatives()
+ * (ALGEBRAIC[Istim] is read before it is assigned, ORd_2011.H:962 vs :982, on a
+ * zeroed scratch, ORd.C:165). */
+enum { OR_V, OR_CaMKt, OR_Nai, OR_Nass, OR_Ki, OR_Kss, OR_Cass, OR_Cansr, OR_Cajsr, OR_Cai,
+       OR_m, OR_hf, OR_hs, OR_j, OR_hsp, OR_jp, OR_mL, OR_hL, OR_hLp, OR_a, OR_iF, OR_iS,
+       OR_ap, OR_iFp, OR_iSp, OR_d, OR_ff, OR_fs, OR_fcaf, OR_fcas, OR_jca, OR_ffp,
+       OR_fcafp, OR_nca, OR_xrf, OR_xrs, OR_xs1, OR_xs2, OR_xk1, OR_Jrelnp, OR_Jrelp,
+       OR_NS };
+enum { OC_GNa, OC_GNaL, OC_Gto, OC_PCa, OC_GKr, OC_GKs, OC_GK1, OC_GKb, OC_Gncx, OC_Pnak,
+       OC_upScale, OC_cmdnmax, OC_PCab, OC_PNab, OC_GpCa, OC_NC };
+
+static void ord_init(int tf, double* c, double* y)
+{
+    (void)tf; /* ORd accepts "myocyte" only (ORd.C:93-96): endocardial column */
+    c[OC_GNa] = 75.0; c[OC_GNaL] = 0.0075; c[OC_Gto] = 0.02; c[OC_PCa] = 0.0001;
+    c[OC_GKr] = 0.046; c[OC_GKs] = 0.0034; c[OC_GK1] = 0.1908; c[OC_GKb] = 0.003;
+    c[OC_Gncx] = 0.0008; c[OC_Pnak] = 30.0; c[OC_upScale] = 1.0; c[OC_cmdnmax] = 0.05;
+    c[OC_PCab] = 2.5e-8; c[OC_PNab] = 3.75e-10; c[OC_GpCa] = 0.0005;
+    static const double y0[OR_NS] = {
+        -87.0, 0.0, 7.0, 7.0, 145.0, 145.0, 1e-4, 1.2, 1.2, 1e-4,
+        0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0,
+        0.0, 1.0, 1.0, 0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0,
+        1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0 };
+    memcpy(y, y0, sizeof y0);
+}
+
+static void ord_eval(double t, const double* c, const double* y, int solve_vm,
+                     const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                     double* iion, double* istim)
+{
+    /* fixed constants of ORdinitConsts (ORd_2011.H:497-692) */
+    const double L = 0.01, rad = 0.0011;
+    const double Ageo = 2.0 * 3.14 * rad * rad + 2.0 * 3.14 * rad * L;
+    const double vcell = 1000.0 * 3.14 * rad * rad * L;
+    const double Acap = 2.0 * Ageo;
+    const double vjsr = 0.0048 * vcell, vmyo = 0.68 * vcell, vnsr = 0.0552 * vcell,
+                 vss = 0.02 * vcell;
+    const double cao = 1.8, ko = 5.4, nao = 140.0;
+    const double F = 96485.0, R = 8314.0, T = 310.0;
+    const double KmCaMK = 0.15, aCaMK = 0.05, bCaMK = 0.00068, CaMKo = 0.05, KmCaM = 0.0015;
+    const double KmCap = 0.0005;
+    const double cm = 1.0;
+
+    const double V = y[OR_V], Cai = y[OR_Cai], Cass = y[OR_Cass], Nai = y[OR_Nai],
+                 Nass = y[OR_Nass], Ki = y[OR_Ki], Kss = y[OR_Kss];
+    for (int i = 0; i < OR_NS; ++i) g[i] = 0;
+
+    /* CaMK */
+    double CaMKb = CaMKo * (1.0 - y[OR_CaMKt]) / (1.0 + KmCaM / Cass);
+    double CaMKa = CaMKb + y[OR_CaMKt];
+    r[OR_CaMKt] = aCaMK * CaMKb * (CaMKb + y[OR_CaMKt]) - bCaMK * y[OR_CaMKt];
+    double IpCa = c[OC_GpCa] * Cai / (KmCap + Cai);
+    double Jdiff = (Cass - Cai) / 0.2, JdiffK = (Kss - Ki) / 2.0, JdiffNa = (Nass - Nai) / 2.0;
+    double Jtr = (y[OR_Cansr] - y[OR_Cajsr]) / 100.0;
+
+    /* INaCa (i and ss compartments) */
+    const double KmCaAct = 0.00015, kasymm = 12.5, kcaoff = 5000.0, kcaon = 1500000.0,
+                 kna1 = 15.0, kna2 = 5.0, kna3 = 88.12, qca = 0.167, qna = 0.5224,
+                 wca = 60000.0, wna = 60000.0, wnaca = 5000.0;
+    const double h10 = kasymm + 1.0 + nao / kna1 * (1.0 + nao / kna2);
+    const double h11 = nao * nao / (h10 * kna1 * kna2);
+    const double h12 = 1.0 / h10;
+    const double k1 = h12 * cao * kcaon, k2 = kcaoff, k5 = kcaoff;
+    double hca = exp(qca * V * F / (R * T));
+    double hna = exp(qna * V * F / (R * T));
+    double INaCa[2];
+    for (int comp = 0; comp < 2; ++comp) {
+        double na = comp == 0 ? Nai : Nass, ca = comp == 0 ? Cai : Cass;
+        double allo = 1.0 / (1.0 + pow(KmCaAct / ca, 2.0));
+        double h4 = 1.0 + na / kna1 * (1.0 + na / kna2);
+        double h1 = 1.0 + na / kna3 * (1.0 + hna);
+        double h5 = na * na / (h4 * kna1 * kna2);
+        double h6 = 1.0 / h4;
+        double h7 = 1.0 + nao / kna3 * (1.0 + 1.0 / hna);
+        double h2 = na * hna / (kna3 * h1);
+        double h3 = 1.0 / h1;
+        double h8 = nao / (kna3 * hna * h7);
+        double h9 = 1.0 / h7;
+        double k6 = h6 * ca * kcaon;
+        double k3p = h9 * wca, k3pp = h8 * wnaca, k4p = h3 * wca / hca, k4pp = h2 * wnaca;
+        double k7 = h5 * h2 * wna, k8 = h8 * h11 * wna;
+        double k3 = k3p + k3pp, k4 = k4p + k4pp;
+        double x1 = k2 * k4 * (k7 + k6) + k5 * k7 * (k2 + k3);
+        double x2 = k1 * k7 * (k4 + k5) + k4 * k6 * (k1 + k8);
+        double x3 = k1 * k3 * (k7 + k6) + k8 * k6 * (k2 + k3);
+        double x4 = k2 * k8 * (k4 + k5) + k3 * k5 * (k1 + k8);
+        double E1 = x1 / (x1 + x2 + x3 + x4), E2 = x2 / (x1 + x2 + x3 + x4),
+               E3 = x3 / (x1 + x2 + x3 + x4), E4 = x4 / (x1 + x2 + x3 + x4);
+        double JCa = E2 * k2 - E1 * k1;
+        double JNa = 3.0 * (E4 * k7 - E1 * k8) + E3 * k4pp - E2 * k3pp;
+        INaCa[comp] = (comp == 0 ? 0.8 : 0.2) * c[OC_Gncx] * allo * (1.0 * JNa + 2.0 * JCa);
+    }
+
+    /* INaK (published constants) */
+    const double k1p = 949.5, k1m = 182.4, k2p = 687.2, k2m = 39.4, k3p_ = 1899.0,
+                 k3m = 79300.0, k4p_ = 639.0, k4m = 40.0, Knai0 = 9.073, Knao0 = 27.78,
+                 delta = -0.155, Kki = 0.5, Kko = 0.3582, MgADP = 0.05, MgATP = 9.8,
+                 Kmgatp = 1.698e-7, Hp = 1.0e-7, eP = 4.2, Khp = 1.698e-7, Knap = 224.0,
+                 Kxkur = 292.0;
+    double Knai = Knai0 * exp(delta * V * F / (3.0 * R * T));
+    double Knao = Knao0 * exp((1.0 - delta) * V * F / (3.0 * R * T));
+    double P = eP / (1.0 + Hp / Khp + Nai / Knap + Ki / Kxkur);
+    double a1 = k1p * pow(Nai / Knai, 3.0) /
+                (pow(1.0 + Nai / Knai, 3.0) + pow(1.0 + Ki / Kki, 2.0) - 1.0);
+    double b1 = k1m * MgADP, a2 = k2p;
+    double b2 = k2m * pow(nao / Knao, 3.0) /
+                (pow(1.0 + nao / Knao, 3.0) + pow(1.0 + ko / Kko, 2.0) - 1.0);
+    double a3 = k3p_ * pow(ko / Kko, 2.0) /
+                (pow(1.0 + nao / Knao, 3.0) + pow(1.0 + ko / Kko, 2.0) - 1.0);
+    double b3 = k3m * P * Hp / (1.0 + MgATP / Kmgatp);
+    double a4 = k4p_ * MgATP / Kmgatp / (1.0 + MgATP / Kmgatp);
+    double b4 = k4m * pow(Ki / Kki, 2.0) /
+                (pow(1.0 + Nai / Knai, 3.0) + pow(1.0 + Ki / Kki, 2.0) - 1.0);
+    double x1 = a4 * a1 * a2 + b2 * b4 * b3 + a2 * b4 * b3 + b3 * a1 * a2;
+    double x2 = b2 * b1 * b4 + a1 * a2 * a3 + a3 * b1 * b4 + a2 * a3 * b4;
+    double x3 = a2 * a3 * a4 + b3 * b2 * b1 + b2 * b1 * a4 + a3 * a4 * b1;
+    double x4 = b4 * b3 * b2 + a3 * a4 * a1 + b2 * a4 * a1 + b3 * b2 * a1;
+    double sx = x1 + x2 + x3 + x4;
+    double E1 = x1 / sx, E2 = x2 / sx, E3 = x3 / sx, E4 = x4 / sx;
+    double JnakK = 2.0 * (E4 * b1 - E3 * a1);
+    double JnakNa = 3.0 * (E1 * a3 - E2 * b3);
+    double INaK = c[OC_Pnak] * (1.0 * JnakNa + 1.0 * JnakK);
+
+    /* SERCA */
+    double Jleak = 0.0039375 * y[OR_Cansr] / 15.0;
+    double fJupp = 1.0 / (1.0 + KmCaMK / CaMKa);
+    double Jupnp = c[OC_upScale] * 0.004375 * Cai / (Cai + 0.00092);
+    double Jupp = c[OC_upScale] * 2.75 * 0.004375 * Cai / (Cai + 0.00092 - 0.00017);
+    double Jup = (1.0 - fJupp) * Jupnp + fJupp * Jupp - Jleak;
+
+    /* reversal potentials */
+    const double PKNa = 0.01833;
+    double EK = R * T / F * log(ko / Ki);
+    double ENa = R * T / F * log(nao / Nai);
+    double EKs = R * T / F * log((ko + PKNa * nao) / (Ki + PKNa * Nai));
+
+#define OR_GATE(idx, inf, tau) do { double _i = (inf), _t = (tau); \
+        r[idx] = (_i - y[idx]) / _t; g[idx] = 1; xi[idx] = _i; ta[idx] = _t; } while (0)
+    /* IK1 */
+    double rk1 = 1.0 / (1.0 + exp((V + 105.8 - 2.6 * ko) / 9.493));
+    double txk1 = 122.2 / (exp(-(V + 127.2) / 20.36) + exp((V + 236.8) / 69.33));
+    double xk1ss = 1.0 / (1.0 + exp(-(V + 2.5538 * ko + 144.59) / (1.5692 * ko + 3.8115)));
+    OR_GATE(OR_xk1, xk1ss, txk1);
+    double IK1 = c[OC_GK1] * sqrt(ko) * rk1 * y[OR_xk1] * (V - EK);
+    /* IKb */
+    double xkb = 1.0 / (1.0 + exp(-(V - 14.48) / 18.34));
+    double IKb = c[OC_GKb] * xkb * (V - EK);
+    /* IKr */
+    double Axrf = 1.0 / (1.0 + exp((V + 54.81) / 38.21));
+    double rkr = 1.0 / (1.0 + exp((V + 55.0) / 75.0)) * 1.0 / (1.0 + exp((V - 10.0) / 30.0));
+    double txrf = 12.98 + 1.0 / (0.3652 * exp((V - 31.66) / 3.869) + 4.123e-05 * exp(-(V - 47.78) / 20.38));
+    double txrs = 1.865 + 1.0 / (0.06629 * exp((V - 34.7) / 7.355) + 1.128e-05 * exp(-(V - 29.74) / 25.94));
+    double xrss = 1.0 / (1.0 + exp(-(V + 8.337) / 6.789));
+    double Axrs = 1.0 - Axrf;
+    OR_GATE(OR_xrf, xrss, txrf);
+    OR_GATE(OR_xrs, xrss, txrs);
+    double xr = Axrf * y[OR_xrf] + Axrs * y[OR_xrs];
+    double IKr = c[OC_GKr] * sqrt(ko / 5.4) * xr * rkr * (V - EK);
+    /* IKs */
+    double KsCa = 1.0 + 0.6 / (1.0 + pow(3.8e-05 / Cai, 1.4));
+    double txs1 = 817.3 + 1.0 / (0.0002326 * exp((V + 48.28) / 17.8) + 0.001292 * exp(-(V + 210.0) / 230.0));
+    double txs2 = 1.0 / (0.01 * exp((V - 50.0) / 20.0) + 0.0193 * exp(-(V + 66.54) / 31.0));
+    double xs1ss = 1.0 / (1.0 + exp(-(V + 11.6) / 8.932));
+    OR_GATE(OR_xs1, xs1ss, txs1);
+    OR_GATE(OR_xs2, xs1ss, txs2);
+    double IKs = c[OC_GKs] * KsCa * y[OR_xs1] * y[OR_xs2] * (V - EKs);
+    /* INa */
+    double fINap = 1.0 / (1.0 + KmCaMK / CaMKa);
+    double hssp = 1.0 / (1.0 + exp((V + 89.1) / 6.086));
+    double thf = 1.0 / (1.432e-05 * exp(-(V + 1.196) / 6.285) + 6.149 * exp((V + 0.5096) / 20.27));
+    double ths = 1.0 / (0.009794 * exp(-(V + 17.95) / 28.05) + 0.3343 * exp((V + 5.73) / 56.66));
+    double tj = 2.038 + 1.0 / (0.02136 * exp(-(V + 100.6) / 8.281) + 0.3052 * exp((V + 0.9941) / 38.45));
+    double hss = 1.0 / (1.0 + exp((V + 82.9) / 6.086));
+    double mss = 1.0 / (1.0 + exp(-(V + 39.57) / 9.871));
+    double tm = 1.0 / (6.765 * exp((V + 11.64) / 34.77) + 8.552 * exp(-(V + 77.42) / 5.955));
+    const double Ahf = 0.99, Ahs = 1.0 - Ahf;
+    double h = Ahf * y[OR_hf] + Ahs * y[OR_hs];
+    double hp = Ahf * y[OR_hf] + Ahs * y[OR_hsp];
+    OR_GATE(OR_hf, hss, thf);
+    OR_GATE(OR_hs, hss, ths);
+    OR_GATE(OR_hsp, hssp, 3.0 * ths);
+    OR_GATE(OR_m, mss, tm);
+    double INa = c[OC_GNa] * (V - ENa) * pow(y[OR_m], 3.0) *
+                 ((1.0 - fINap) * h * y[OR_j] + fINap * hp * y[OR_jp]);
+    OR_GATE(OR_j, hss, tj);
+    OR_GATE(OR_jp, hss, 1.46 * tj);
+    /* Ito */
+    double AiF = 1.0 / (1.0 + exp((V - 213.6) / 151.2));
+    double ass = 1.0 / (1.0 + exp(-(V - 14.34) / 14.82));
+    double assp = 1.0 / (1.0 + exp(-(V - 24.34) / 14.82));
+    double dti_dev = 1.354 + 0.0001 / (exp((V - 167.4) / 15.89) + exp(-(V - 12.23) / 0.2154));
+    double dti_rec = 1.0 - 0.5 / (1.0 + exp((V + 70.0) / 20.0));
+    double fItop = 1.0 / (1.0 + KmCaMK / CaMKa);
+    double iss = 1.0 / (1.0 + exp((V + 43.94) / 5.711));
+    double ta_ = 1.0515 / (1.0 / (1.2089 * (1.0 + exp(-(V - 18.4099) / 29.3814))) +
+                           3.5 / (1.0 + exp((V + 100.0) / 29.3814)));
+    double tiF = 4.562 + 1.0 / (0.3933 * exp(-(V + 100.0) / 100.0) + 0.08004 * exp((V + 50.0) / 16.59));
+    double tiS = 23.62 + 1.0 / (0.001416 * exp(-(V + 96.52) / 59.05) + 1.78e-08 * exp((V + 114.1) / 8.079));
+    double AiS = 1.0 - AiF;
+    /* delta_epi = 1 outside the epicardial column */
+    OR_GATE(OR_a, ass, ta_);
+    OR_GATE(OR_ap, assp, ta_);
+    double ii = AiF * y[OR_iF] + AiS * y[OR_iS];
+    double ip = AiF * y[OR_iFp] + AiS * y[OR_iSp];
+    double tiFp = dti_dev * dti_rec * tiF, tiSp = dti_dev * dti_rec * tiS;
+    OR_GATE(OR_iF, iss, tiF);
+    OR_GATE(OR_iS, iss, tiS);
+    double Ito = c[OC_Gto] * (V - EK) * ((1.0 - fItop) * y[OR_a] * ii + fItop * y[OR_ap] * ip);
+    OR_GATE(OR_iFp, iss, tiFp);
+    OR_GATE(OR_iSp, iss, tiSp);
+    /* INaL */
+    double fINaLp = 1.0 / (1.0 + KmCaMK / CaMKa);
+    double hLss = 1.0 / (1.0 + exp((V + 87.61) / 7.488));
+    double hLssp = 1.0 / (1.0 + exp((V + 93.81) / 7.488));
+    double mLss = 1.0 / (1.0 + exp(-(V + 42.85) / 5.264));
+    const double thL = 200.0, thLp = 3.0 * thL;
+    OR_GATE(OR_hL, hLss, thL);
+    OR_GATE(OR_mL, mLss, tm);
+    double INaL = c[OC_GNaL] * (V - ENa) * y[OR_mL] * ((1.0 - fINaLp) * y[OR_hL] + fINaLp * y[OR_hLp]);
+    OR_GATE(OR_hLp, hLssp, thLp);
+    /* ICaL */
+    double Afcaf = 0.3 + 0.6 / (1.0 + exp((V - 10.0) / 10.0));
+    double dss = 1.0 / (1.0 + exp(-(V + 3.94) / 4.23));
+    double fICaLp = 1.0 / (1.0 + KmCaMK / CaMKa);
+    double fss = 1.0 / (1.0 + exp((V + 19.58) / 3.696));
+    double km2n = y[OR_jca] * 1.0;
+    double td = 0.6 + 1.0 / (exp(-0.05 * (V + 6.0)) + exp(0.09 * (V + 14.0)));
+    double tfcaf = 7.0 + 1.0 / (0.04 * exp(-(V - 4.0) / 7.0) + 0.04 * exp((V - 4.0) / 7.0));
+    double tfcas = 100.0 + 1.0 / (0.00012 * exp(-V / 3.0) + 0.00012 * exp(V / 7.0));
+    double tff = 7.0 + 1.0 / (0.0045 * exp(-(V + 20.0) / 10.0) + 0.0045 * exp((V + 20.0) / 10.0));
+    double tfs = 1000.0 + 1.0 / (3.5e-05 * exp(-(V + 5.0) / 4.0) + 3.5e-05 * exp((V + 5.0) / 6.0));
+    double Afcas = 1.0 - Afcaf;
+    const double k2n = 1000.0, Kmn = 0.002, tjca = 75.0, Aff = 0.6, Afs = 1.0 - Aff;
+    double anca = 1.0 / (k2n / km2n + pow(1.0 + Kmn / Cass, 4.0));
+    double fcass = fss;
+    OR_GATE(OR_d, dss, td);
+    OR_GATE(OR_ff, fss, tff);
+    OR_GATE(OR_fs, fss, tfs);
+    double f = Aff * y[OR_ff] + Afs * y[OR_fs];
+    double fca = Afcaf * y[OR_fcaf] + Afcas * y[OR_fcas];
+    double fcap = Afcaf * y[OR_fcafp] + Afcas * y[OR_fcas];
+    double fp = Aff * y[OR_ffp] + Afs * y[OR_fs];
+    OR_GATE(OR_fcaf, fcass, tfcaf);
+    OR_GATE(OR_fcafp, fcass, 2.5 * tfcaf);
+    OR_GATE(OR_fcas, fcass, tfcas);
+    OR_GATE(OR_ffp, fss, 2.5 * tff);
+    OR_GATE(OR_jca, fcass, tjca);
+    /* nca: dn/dt = anca*k2n - n*km2n is linear in n -> xinf = anca*k2n/km2n, tau = 1/km2n */
+    r[OR_nca] = anca * k2n - y[OR_nca] * km2n;
+    g[OR_nca] = 1; xi[OR_nca] = anca * k2n / km2n; ta[OR_nca] = 1.0 / km2n;
+    /* buffers */
+    const double BSRmax = 0.047, KmBSR = 0.00087, BSLmax = 1.124, KmBSL = 0.0087,
+                 csqnmax = 10.0, kmcsqn = 0.8, kmcmdn = 0.00238, trpnmax = 0.07, kmtrpn = 0.0005;
+    r[OR_Cansr] = Jup - Jtr * vjsr / vnsr;
+    double Bcajsr = 1.0 / (1.0 + csqnmax * kmcsqn / pow(kmcsqn + y[OR_Cajsr], 2.0));
+    double Bcass = 1.0 / (1.0 + BSRmax * KmBSR / pow(KmBSR + Cass, 2.0) +
+                          BSLmax * KmBSL / pow(KmBSL + Cass, 2.0));
+    double Bcai = 1.0 / (1.0 + c[OC_cmdnmax] * kmcmdn / pow(kmcmdn + Cai, 2.0) +
+                         trpnmax * kmtrpn / pow(kmtrpn + Cai, 2.0));
+    double vffrt = V * F * F / (R * T), vfrt = V * F / (R * T);
+    /* RyR */
+    double fJrelp = 1.0 / (1.0 + KmCaMK / CaMKa);
+    double Jrel = (1.0 - fJrelp) * y[OR_Jrelnp] + fJrelp * y[OR_Jrelp];
+    const double bt = 4.75, a_rel = 0.5 * bt, btp = 1.25 * bt, a_relp = 0.5 * btp;
+    double trel = bt / (1.0 + 0.0123 / y[OR_Cajsr]);
+    if (trel < 0.001) trel = 0.001;
+    double trelp = btp / (1.0 + 0.0123 / y[OR_Cajsr]);
+    if (trelp < 0.001) trelp = 0.001;
+    /* GHK fluxes */
+    double e1 = exp(1.0 * vfrt), e2 = exp(2.0 * vfrt);
+    double PhiCaK = 1.0 * vffrt * (0.75 * Kss * e1 - 0.75 * ko) / (e1 - 1.0);
+    double PhiCaL = 4.0 * vffrt * (Cass * e2 - 0.341 * cao) / (e2 - 1.0);
+    double PhiCaNa = 1.0 * vffrt * (0.75 * Nass * e1 - 0.75 * nao) / (e1 - 1.0);
+    double ICab = c[OC_PCab] * 4.0 * vffrt * (Cai * e2 - 0.341 * cao) / (e2 - 1.0);
+    double INab = c[OC_PNab] * vffrt * (Nai * exp(vfrt) - nao) / (exp(vfrt) - 1.0);
+    r[OR_Cajsr] = Bcajsr * (Jtr - Jrel);
+    r[OR_Ki] = -(Ito + IKr + IKs + IK1 + IKb - 2.0 * INaK) * cm * Acap / (F * vmyo) +
+               JdiffK * vss / vmyo;
+    const double PCa = c[OC_PCa], PCap = 1.1 * PCa, PCaK = 0.0003574 * PCa,
+                 PCaNa = 0.00125 * PCa, PCaKp = 0.0003574 * PCap, PCaNap = 0.00125 * PCap;
+    double gnp = f * (1.0 - y[OR_nca]) + y[OR_jca] * fca * y[OR_nca];
+    double gp = fp * (1.0 - y[OR_nca]) + y[OR_jca] * fcap * y[OR_nca];
+    double ICaK = (1.0 - fICaLp) * PCaK * PhiCaK * y[OR_d] * gnp + fICaLp * PCaKp * PhiCaK * y[OR_d] * gp;
+    double ICaL = (1.0 - fICaLp) * PCa * PhiCaL * y[OR_d] * gnp + fICaLp * PCap * PhiCaL * y[OR_d] * gp;
+    double ICaNa = (1.0 - fICaLp) * PCaNa * PhiCaNa * y[OR_d] * gnp + fICaLp * PCaNap * PhiCaNa * y[OR_d] * gp;
+    r[OR_Cai] = Bcai * (-(IpCa + ICab - 2.0 * INaCa[0]) * cm * Acap / (2.0 * F * vmyo) -
+                        Jup * vnsr / vmyo + Jdiff * vss / vmyo);
+    r[OR_Nai] = -(INa + INaL + 3.0 * INaCa[0] + 3.0 * INaK + INab) * Acap * cm / (F * vmyo) +
+                JdiffNa * vss / vmyo;
+    r[OR_Cass] = Bcass * (-(ICaL - 2.0 * INaCa[1]) * cm * Acap / (2.0 * F * vss) +
+                          Jrel * vjsr / vss - Jdiff);
+    r[OR_Kss] = -ICaK * cm * Acap / (F * vss) - JdiffK;
+    r[OR_Nass] = -(ICaNa + 3.0 * INaCa[1]) * cm * Acap / (F * vss) - JdiffNa;
+    double Jrel_inf = a_rel * -ICaL / (1.0 + 1.0 * pow(1.5 / y[OR_Cajsr], 8.0));
+    double Jrel_infp = a_relp * -ICaL / (1.0 + pow(1.5 / y[OR_Cajsr], 8.0));
+    OR_GATE(OR_Jrelnp, Jrel_inf, trel);
+    OR_GATE(OR_Jrelp, Jrel_infp, trelp);
+#undef OR_GATE
+
+    double Iion = INa + INaL + Ito + ICaL + ICaNa + ICaK + IKr + IKs + IK1 + INaCa[0] +
+                  INaCa[1] + INaK + INab + IKb + IpCa + ICab;
+    double Ist = orc_stimulus(t, st);
+    r[OR_V] = solve_vm ? -Iion - Ist : 0.0;
+    *iion = Iion; *istim = Ist;
+}
+
+/* ============================================================ AlievPanfilov
+ * AlievPanfilov_1996.H:72-120 (time in units of 12.9 ms, u dimensionless). */
+static void ap_init(double* c, double* y)
+{
+    c[0] = 20.0;   /* Vp */
+    c[1] = -80.0;  /* Vr */
+    c[2] = -70.0;  /* Vth */
+    c[3] = 0.01;   /* epsilon */
+    c[4] = 8.0;    /* k */
+    c[5] = 0.2;    /* mu1 */
+    c[6] = 0.3;    /* mu2 */
+    c[7] = (c[2] - c[1]) / (c[0] - c[1]); /* a */
+    y[0] = 0.05; y[1] = 0.0;
+}
+
+static void ap_eval(double t, const double* c, const double* y, int solve_vm,
+                    const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                    double* iion, double* istim)
+{
+    (void)xi; (void)ta;
+    const double u = y[0], rr = y[1], k = c[4], a = c[7];
+    double eps = c[3] + c[5] * rr / (c[6] + u);
+    r[1] = eps * (-rr - k * u * (u - (a + 1.0)));
+    double Iion = k * u * (u - a) * (u - 1.0) + rr * u;
+    double Ist = orc_stimulus(t, st);
+    r[0] = solve_vm ? -Iion - Ist : 0.0;
+    g[0] = 0; g[1] = 0;
+    *iion = Iion; *istim = Ist;
+}
+
+/* ============================================================== dispatch */
+int orc_model_nstates(int model)
+{
+    switch (model) {
+    case ORC_BO: return 4;
+    case ORC_TNNP: return TN_NS;
+    case ORC_ORD: return OR_NS;
+    case ORC_AP: return 2;
+    default: return -1;
+    }
+}
+
+int orc_model_init(int model, int tissue, double* c, double* y)
+{
+    memset(c, 0, sizeof(double) * ORC_MAX_CONSTS);
+    switch (model) {
+    case ORC_BO:
+        if (tissue < 1 || tissue > 3) return -1;
+        bo_init(tissue, c, y); return 0;
+    case ORC_TNNP:
+        if (tissue < 1 || tissue > 3) return -1;
+        tnnp_init(tissue, c, y); return 0;
+    case ORC_ORD:
+        if (tissue != ORC_MYOCYTE) return -1;
+        ord_init(tissue, c, y); return 0;
+    case ORC_AP:
+        if (tissue != ORC_MYOCYTE) return -1;
+        ap_init(c, y); return 0;
+    default: return -1;
+    }
+}
+
+int orc_model_const_index(int model, const char* name)
+{
+    if (model == ORC_TNNP) {
+        if (!strcmp(name, "g_Na")) return TC_gNa;
+        if (!strcmp(name, "g_Kr")) return TC_gKr;
+        if (!strcmp(name, "g_CaL")) return TC_gCaL;
+        if (!strcmp(name, "g_Ks")) return TC_gKs;
+    } else if (model == ORC_ORD) {
+        if (!strcmp(name, "GNa")) return OC_GNa;
+        if (!strcmp(name, "GKr")) return OC_GKr;
+        if (!strcmp(name, "PCa")) return OC_PCa;
+        if (!strcmp(name, "GKs")) return OC_GKs;
+    } else if (model == ORC_BO) {
+        if (!strcmp(name, "tauFi")) return BO_tfi;
+        if (!strcmp(name, "tauSi")) return BO_tsi;
+    }
+    return -1;
+}
+
+void orc_model_eval(int model, int tissue, double t, const double* c, const double* y,
+                    int solve_vm, const orc_stim_t* st, double* r, int* g, double* xi,
+                    double* ta, double* iion, double* istim)
+{
+    switch (model) {
+    case ORC_BO: bo_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_TNNP: tnnp_eval(tissue, t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_ORD: ord_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_AP: ap_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    default: break;
+    }
+}
+
+double orc_model_vm_mV(int model, const double* y)
+{
+    /* BuenoOrovio.H:78-81, AlievPanfilov.H:78-81; TNNP/ORd store mV directly */
+    if (model == ORC_BO) return y[0] * 85.7 - 84.0;
+    if (model == ORC_AP) return y[0] * 100 - 80;
+    return y[0];
+}

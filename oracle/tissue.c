@@ -1,0 +1,345 @@
+/*
+ * Oracle (test infrastructure): finite-volume monodomain operators on a uniform hex
+ * block, restating monodomainSolver.C:103-150 (fvc::laplacian / fvm::laplacian with
+ * a tensor conductivity, "Gauss linear corrected") and myocardiumDomain.C:307-374,
+ * 487-550.  The OpenFOAM operators (ESI v2312..v2512 gaussLaplacianScheme with tensor
+ * gamma) are restated from their documented discretisation:
+ *   SfGamma     = Sf . sigma_f            (sigma_f linear-interpolated, w = 1/2)
+ *   SfGammaSn   = SfGamma . n             (orthogonal coefficient)
+ *   SfGammaCorr = SfGamma - SfGammaSn n   (tangential part, explicit)
+ *   flux_f      = SfGammaSn * (V_N - V_P)/h + SfGammaCorr . interp(grad V)
+ * grad V is the Gauss-linear cell gradient with zeroGradient walls (boundary face
+ * value = cell value); its wall value is the cell gradient with the normal
+ * component replaced by snGrad = 0.  Wall faces carry only the correction flux.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+int orc_mesh_build(orc_mesh_t* m, int nx, int ny, int nz, double hx, double hy, double hz)
+{
+    memset(m, 0, sizeof *m);
+    m->nx = nx; m->ny = ny; m->nz = nz; m->hx = hx; m->hy = hy; m->hz = hz;
+    m->n_cells = (long)nx * ny * nz;
+    m->n_faces = (long)(nx - 1) * ny * nz + (long)nx * (ny - 1) * nz + (long)nx * ny * (nz - 1);
+    m->owner = malloc(sizeof(int32_t) * m->n_faces);
+    m->neighbour = malloc(sizeof(int32_t) * m->n_faces);
+    m->face_dir = malloc(m->n_faces);
+    if (!m->owner || !m->neighbour || !m->face_dir) return -1;
+    long f = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                int32_t c = i + nx * (j + ny * k);
+                if (i < nx - 1) { m->owner[f] = c; m->neighbour[f] = c + 1; m->face_dir[f++] = 0; }
+                if (j < ny - 1) { m->owner[f] = c; m->neighbour[f] = c + nx; m->face_dir[f++] = 1; }
+                if (k < nz - 1) { m->owner[f] = c; m->neighbour[f] = c + nx * ny; m->face_dir[f++] = 2; }
+            }
+    return 0;
+}
+
+void orc_mesh_free(orc_mesh_t* m)
+{
+    free(m->owner); free(m->neighbour); free(m->face_dir);
+    memset(m, 0, sizeof *m);
+}
+
+static inline double face_area(const orc_mesh_t* m, int d)
+{
+    return d == 0 ? m->hy * m->hz : d == 1 ? m->hx * m->hz : m->hx * m->hy;
+}
+static inline double cell_h(const orc_mesh_t* m, int d)
+{
+    return d == 0 ? m->hx : d == 1 ? m->hy : m->hz;
+}
+
+/* Gauss linear gradient, zeroGradient walls: grad[3n] */
+static void gauss_grad(const orc_mesh_t* m, const double* V, double* grad, int nthreads)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const double vol = m->hx * m->hy * m->hz;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long c = 0; c < m->n_cells; ++c) {
+        int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
+        const long st[3] = {1, nx, (long)nx * ny};
+        const int idx[3] = {i, j, k}, nn[3] = {nx, ny, nz};
+        for (int d = 0; d < 3; ++d) {
+            double A = face_area(m, d);
+            double vp = V[c];
+            double fplus = idx[d] < nn[d] - 1 ? 0.5 * (vp + V[c + st[d]]) : vp;
+            double fminus = idx[d] > 0 ? 0.5 * (vp + V[c - st[d]]) : vp;
+            grad[3 * c + d] = A * (fplus - fminus) / vol;
+        }
+    }
+}
+
+/* correction flux through the face with normal d between P (owner side) and the
+ * interpolated gradient gf, with face tensor s (row-major 3x3): A * sum_{e!=d} s[d][e] gf[e] */
+static inline double corr_flux(double A, int d, const double* s, const double* gf)
+{
+    double q = 0.0;
+    for (int e = 0; e < 3; ++e)
+        if (e != d) q += s[3 * d + e] * gf[e];
+    return A * q;
+}
+
+static void div_flux(const orc_mesh_t* m, const double* sigma, const double* V,
+                     const double* grad, int with_orth, int with_corr, double* out,
+                     int nthreads)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const double vol = m->hx * m->hy * m->hz;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long c = 0; c < m->n_cells; ++c) {
+        int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
+        const long st[3] = {1, nx, (long)nx * ny};
+        const int idx[3] = {i, j, k}, nn[3] = {nx, ny, nz};
+        double acc = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double A = face_area(m, d), h = cell_h(m, d);
+            for (int side = 0; side < 2; ++side) {
+                const int interior = side == 0 ? idx[d] < nn[d] - 1 : idx[d] > 0;
+                const double sgn = side == 0 ? 1.0 : -1.0;
+                if (interior) {
+                    long nb = side == 0 ? c + st[d] : c - st[d];
+                    double sf[9], gf[3];
+                    for (int q = 0; q < 9; ++q) sf[q] = 0.5 * (sigma[9 * c + q] + sigma[9 * nb + q]);
+                    for (int q = 0; q < 3; ++q) gf[q] = 0.5 * (grad ? grad[3 * c + q] + grad[3 * nb + q] : 0.0);
+                    /* flux out of c through this face; Sf = sgn*A*e_d */
+                    double fl = 0.0;
+                    if (with_orth) fl += A * sf[4 * d] * (V[nb] - V[c]) / h;
+                    if (with_corr) fl += sgn * corr_flux(A, d, sf, gf);
+                    acc += fl;
+                } else if (with_corr) {
+                    /* wall: sigma_b = sigma_P; grad_b = grad_P with normal part = snGrad = 0 */
+                    double gb[3] = {grad[3 * c], grad[3 * c + 1], grad[3 * c + 2]};
+                    gb[d] = 0.0;
+                    acc += sgn * corr_flux(A, d, sigma + 9 * c, gb);
+                }
+            }
+        }
+        out[c] = acc / vol;
+    }
+}
+
+void orc_laplacian(const orc_mesh_t* m, const double* sigma, const double* V, double* out,
+                   int nthreads)
+{
+    double* grad = malloc(sizeof(double) * 3 * m->n_cells);
+    gauss_grad(m, V, grad, nthreads);
+    div_flux(m, sigma, V, grad, 1, 1, out, nthreads);
+    free(grad);
+}
+
+void orc_correction_div(const orc_mesh_t* m, const double* sigma, const double* V,
+                        double* out, int nthreads)
+{
+    double* grad = malloc(sizeof(double) * 3 * m->n_cells);
+    gauss_grad(m, V, grad, nthreads);
+    div_flux(m, sigma, V, grad, 0, 1, out, nthreads);
+    free(grad);
+}
+
+void orc_assemble_implicit(const orc_mesh_t* m, const double* sigma, double chiCm,
+                           double dt, double coefft, double* diag, double* upper)
+{
+    /* per unit volume: (chiCm*coefft/dt + sum a_f) V_P - sum a_f V_N,
+     * a_f = SfGammaSn_f * deltaCoeff_f / vol; upper[f] holds -a_f */
+    const double vol = m->hx * m->hy * m->hz;
+    for (long c = 0; c < m->n_cells; ++c) diag[c] = chiCm * coefft / dt;
+    for (long f = 0; f < m->n_faces; ++f) {
+        int d = m->face_dir[f];
+        long P = m->owner[f], N = m->neighbour[f];
+        double snn = 0.5 * (sigma[9 * P + 4 * d] + sigma[9 * N + 4 * d]);
+        double a = face_area(m, d) * snn / cell_h(m, d) / vol;
+        upper[f] = -a;
+        diag[P] += a;
+        diag[N] += a;
+    }
+}
+
+void orc_ldu_amul(const orc_mesh_t* m, const double* diag, const double* upper,
+                  const double* x, double* y)
+{
+    for (long c = 0; c < m->n_cells; ++c) y[c] = diag[c] * x[c];
+    for (long f = 0; f < m->n_faces; ++f) {
+        long P = m->owner[f], N = m->neighbour[f];
+        y[P] += upper[f] * x[N];
+        y[N] += upper[f] * x[P];
+    }
+}
+
+int orc_pcg(const orc_mesh_t* m, const double* diag, const double* upper, const double* b,
+            double* x, double tol, int max_iter, double* res)
+{
+    const long n = m->n_cells;
+    double* r = malloc(sizeof(double) * n);
+    double* z = malloc(sizeof(double) * n);
+    double* p = malloc(sizeof(double) * n);
+    double* q = malloc(sizeof(double) * n);
+    orc_ldu_amul(m, diag, upper, x, q);
+    double bn = 0.0, rr = 0.0, rz = 0.0;
+    for (long i = 0; i < n; ++i) {
+        r[i] = b[i] - q[i];
+        z[i] = r[i] / diag[i];
+        p[i] = z[i];
+        bn += b[i] * b[i]; rr += r[i] * r[i]; rz += r[i] * z[i];
+    }
+    bn = sqrt(bn);
+    if (bn == 0.0) bn = 1.0;
+    int it = 0;
+    while (sqrt(rr) > tol * bn && it < max_iter) {
+        orc_ldu_amul(m, diag, upper, p, q);
+        double pq = 0.0;
+        for (long i = 0; i < n; ++i) pq += p[i] * q[i];
+        double alpha = rz / pq;
+        double rz_new = 0.0;
+        rr = 0.0;
+        for (long i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+            z[i] = r[i] / diag[i];
+            rz_new += r[i] * z[i];
+            rr += r[i] * r[i];
+        }
+        double beta = rz_new / rz;
+        rz = rz_new;
+        for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        ++it;
+    }
+    /* true residual */
+    orc_ldu_amul(m, diag, upper, x, q);
+    rr = 0.0;
+    for (long i = 0; i < n; ++i) rr += (b[i] - q[i]) * (b[i] - q[i]);
+    if (res) *res = sqrt(rr) / bn;
+    free(r); free(z); free(p); free(q);
+    return it;
+}
+
+void orc_box_stimulus(const orc_mesh_t* m, const orc_boxes_t* b, double t0, double* src)
+{
+    const int nx = m->nx, ny = m->ny;
+    for (long c = 0; c < m->n_cells; ++c) src[c] = 0.0;
+    if (!b) return;
+    for (int s = 0; s < b->n_boxes; ++s) {
+        const double tS = b->t_start[s], tE = tS + b->duration[s];
+        if (t0 < tS || t0 > tE) continue;
+        for (long c = 0; c < m->n_cells; ++c) {
+            int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
+            double p[3] = {(i + 0.5) * m->hx, (j + 0.5) * m->hy, (k + 0.5) * m->hz};
+            int in = 1;
+            for (int d = 0; d < 3; ++d)
+                if (p[d] < b->box_min[3 * s + d] || p[d] > b->box_max[3 * s + d]) in = 0;
+            if (in) src[c] += b->intensity[s];
+        }
+    }
+}
+
+void orc_activation(long n, const double* V, const double* Vold, uint8_t* flag, double* tact,
+                    double t_old, double dt)
+{
+    const double SMALL = 1e-15;
+    for (long c = 0; c < n; ++c) {
+        if (flag[c] && V[c] > SMALL) {
+            flag[c] = 0;
+            double w = (0.0 - Vold[c]) / (V[c] - Vold[c]);
+            tact[c] = t_old + w * dt;
+        }
+    }
+}
+
+double orc_explicit_dt(const orc_mesh_t* m, const double* sigma, double chiCm, double maxCo)
+{
+    double best = INFINITY;
+    for (long f = 0; f < m->n_faces; ++f) {
+        int d = m->face_dir[f];
+        long P = m->owner[f], N = m->neighbour[f];
+        double snn = 0.5 * (sigma[9 * P + 4 * d] + sigma[9 * N + 4 * d]);
+        double Df = snn / chiCm;
+        double h = cell_h(m, d);
+        double q = h * h / Df;
+        if (q < best) best = q;
+    }
+    return maxCo * best;
+}
+
+/* ----------------------------------------------------------- orchestration */
+int orc_tissue_create(orc_tissue_t* T, int nx, int ny, int nz, double hx, double hy,
+                      double hz, const orc_tissue_cfg_t* cfg, const double* sigma9)
+{
+    memset(T, 0, sizeof *T);
+    if (orc_mesh_build(&T->mesh, nx, ny, nz, hx, hy, hz)) return -1;
+    T->cfg = *cfg;
+    const long n = T->mesh.n_cells;
+    T->nstates = orc_model_nstates(cfg->model);
+    double y[ORC_MAX_STATES];
+    if (orc_model_init(cfg->model, cfg->tissue, T->consts, y)) return -2;
+    T->states = malloc(sizeof(double) * T->nstates * n);
+    for (int i = 0; i < T->nstates; ++i)
+        for (long c = 0; c < n; ++c) T->states[(long)i * n + c] = y[i];
+    T->step = malloc(sizeof(double) * n);
+    for (long c = 0; c < n; ++c) T->step[c] = cfg->dt; /* seconds used as ms, ionicModel.C:38 */
+    T->sigma = malloc(sizeof(double) * 9 * n);
+    memcpy(T->sigma, sigma9, sizeof(double) * 9 * n);
+    T->V = malloc(sizeof(double) * n); T->V0 = malloc(sizeof(double) * n);
+    T->V00 = malloc(sizeof(double) * n); T->Iion = calloc(n, sizeof(double));
+    T->src = calloc(n, sizeof(double)); T->tact = calloc(n, sizeof(double));
+    T->flag = malloc(n); T->work = malloc(sizeof(double) * 4 * n);
+    for (long c = 0; c < n; ++c) {
+        T->V[c] = T->V0[c] = T->V00[c] = -0.084; /* myocardiumDomain.C:207 */
+        T->flag[c] = 1;
+    }
+    return 0;
+}
+
+void orc_tissue_free(orc_tissue_t* T)
+{
+    orc_mesh_free(&T->mesh);
+    free(T->states); free(T->step); free(T->sigma); free(T->V); free(T->V0); free(T->V00);
+    free(T->Iion); free(T->src); free(T->tact); free(T->flag); free(T->work);
+    memset(T, 0, sizeof *T);
+}
+
+int orc_tissue_step(orc_tissue_t* T, const orc_boxes_t* boxes)
+{
+    const orc_tissue_cfg_t* k = &T->cfg;
+    const long n = T->mesh.n_cells;
+    const double dt = k->dt, chiCm = k->chi * k->cm;
+    T->t = T->t + dt;
+    const double t0 = T->t - dt;
+    /* BDF2 with Euler start-up: deltaT0 = GREAT while fewer than 2 old levels exist */
+    double coefft = 1.0, coefft0 = 1.0, coefft00 = 0.0;
+    if (k->ddt == 1 && T->step_index >= 1) { coefft = 1.5; coefft0 = 2.0; coefft00 = 0.5; }
+    /* store old times: V00 <- V0 <- V */
+    memcpy(T->V00, T->V0, sizeof(double) * n);
+    memcpy(T->V0, T->V, sizeof(double) * n);
+
+    orc_box_stimulus(&T->mesh, boxes, t0, T->src);
+    orc_ionic_tissue_step(k->model, k->tissue, T->consts, 0, T->states, n, t0, dt, T->V,
+                          T->Iion, k->integrator, k->n_sub, T->step, k->abs_tol, k->rel_tol,
+                          k->nthreads);
+    double* lap = T->work;
+    if (!k->implicit) {
+        orc_laplacian(&T->mesh, T->sigma, T->V0, lap, k->nthreads);
+        for (long c = 0; c < n; ++c)
+            T->V[c] = (coefft0 * T->V0[c] - coefft00 * T->V00[c] +
+                       dt / chiCm * (lap[c] - chiCm * T->Iion[c] + T->src[c])) / coefft;
+    } else {
+        double* diag = T->work + n;
+        double* rhs = T->work + 2 * n;
+        double* upper = malloc(sizeof(double) * T->mesh.n_faces);
+        orc_assemble_implicit(&T->mesh, T->sigma, chiCm, dt, coefft, diag, upper);
+        orc_correction_div(&T->mesh, T->sigma, T->V0, lap, k->nthreads);
+        for (long c = 0; c < n; ++c)
+            rhs[c] = chiCm / dt * (coefft0 * T->V0[c] - coefft00 * T->V00[c]) + lap[c] -
+                     chiCm * T->Iion[c] + T->src[c];
+        double res;
+        int it = orc_pcg(&T->mesh, diag, upper, rhs, T->V, k->pcg_tol, k->pcg_max_iter, &res);
+        T->pcg_iters_total += it;
+        free(upper);
+    }
+    orc_activation(n, T->V, T->V0, T->flag, T->tact, T->t - dt, dt);
+    T->step_index++;
+    return 0;
+}
