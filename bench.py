@@ -1,0 +1,237 @@
+"""Benchmark of the monodomain tissue step on one MI355X.
+
+Workload (default "ord_implicit_2M"): implicit monodomain + ORd on a 256x128x64 hex
+block (2,097,152 cells, h = 0.2 mm), fibre angle rotating -60..+60 deg through z,
+sigma_l 0.17 / sigma_t 0.019 S/m, chi 1400 /cm, Cm 1 uF/cm^2, backward-Euler diffusion
+at dt = 0.02 ms, corner stimulus 50,000 uA/cm^3 for 2 ms in a 1.5 mm cube.  The step is
+run in the plain FP64 path and in the mixed path (FP32 ionic states and inner CG, FP64
+residual/refinement/dots/activation) from the same initial state; the mixed figure is
+reported when its Vm L-inf error <= 0.5 mV and activation error <= 0.05 ms against FP64.
+
+One "step" = one tissue time step over all cells (ionic + diffusion solve + activation).
+value = mixed-path cell-steps/s; the FP64 figure, speedup and errors ride along.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "ord_implicit_2M": dict(shape=(256, 128, 64), h=2e-4, model="ORd", tissue="myocyte",
+                            algorithm="implicit", dt=2e-5, rotating=True, tol=1e-8),
+    "bo_explicit_147k": dict(shape=(128, 48, 24), h=1.5e-4, model="BuenoOrovio",
+                             tissue="epicardialCells", algorithm="explicit", dt=5e-6,
+                             rotating=False, tol=0.0),
+}
+
+# algorithmic bytes per cell (row) of one launch, by kernel and precision
+NSTATES = {"ORd": 41, "BuenoOrovio": 4, "TNNP": 17, "AlievPanfilov": 2}
+
+
+def algo_bytes_per_cell(kernel, prec, model, algorithm):
+    w = 8 if prec == "fp64" else 4
+    s = NSTATES[model] - 1
+    if kernel == "ionic":
+        if algorithm == "implicit":  # V^n read, states r/w, Iion + rhs written
+            return 8 + 2 * s * w + 8 + 8
+        return 8 + 2 * s * w + 8 + 8 + 1  # V^n, states, Iion, V^{n+1}, flag
+    if kernel == "spmv_dot":         # p, 3 face coefficients, q
+        return 5 * w
+    if kernel == "cg_update":        # x, p, r, q, 1/diag read; x, r written
+        return 7 * w
+    if kernel == "p_update":         # 1/diag, r, p read; p written
+        return 4 * w
+    if kernel == "residual":         # x, b fp64; 3 fp64 coefficients; invd; r, p (, e) out
+        return (8 + 8 + 24 + w + 2 * w + (w if prec == "mixed" else 0))
+    if kernel == "activation":
+        return 8 + 8 + 1
+    if kernel == "tensor_correction":  # V, grad (3) out; grad (3), 9 sigma in, out
+        return 8 + 24 + 24 + 72 + 8
+    if kernel == "add_correction":
+        return 8 + 4 + 8
+    return None
+
+
+def make_sigma(shape, rotating, sl=0.17, st=0.019):
+    nx, ny, nz = shape
+    if not rotating:
+        return np.array([sl, 0, 0, 0, st, 0, 0, 0, st], dtype=np.float64)
+    theta = np.deg2rad(-60.0 + 120.0 * (np.arange(nz) + 0.5) / nz)
+    per_plane = np.zeros((nz, 9))
+    for k, t in enumerate(theta):
+        f = np.array([np.cos(t), np.sin(t), 0.0])
+        per_plane[k] = (st * np.eye(3) + (sl - st) * np.outer(f, f)).ravel()
+    return np.ascontiguousarray(np.repeat(per_plane, nx * ny, axis=0))
+
+
+def coeffs_for(w, sigma):
+    return {"ionicModel": w["model"], "tissue": w["tissue"], "chi": 1.4e5, "cm": 0.01,
+            "solutionAlgorithm": w["algorithm"], "conductivity": sigma,
+            "externalStimulus": {"stimulusLocationMin": [0.0, 0.0, 0.0],
+                                 "stimulusLocationMax": [1.5e-3, 1.5e-3, 1.5e-3],
+                                 "stimulusDuration": 2e-3, "stimulusIntensity": 5e4,
+                                 "stimulusStartTime": 0.0}}
+
+
+def run_precision(cf, torch, w, sigma, prec, steps, warmup, profile):
+    mesh = cf.HexMesh(*w["shape"], w["h"])
+    dom = cf.Monodomain(mesh, coeffs_for(w, sigma), dt=w["dt"], precision=prec,
+                        tol=w["tol"] or 1e-8, max_iter=5000)
+    dom.step(warmup)
+    dom.sync()
+    torch.cuda.synchronize()
+    if profile:
+        dom.set_profiling(True)
+    it0 = dom.solver_stats()[0] if w["algorithm"] == "implicit" else 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dom.step(steps)
+    dom.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = dom.kernel_times() if profile else {}
+    if profile:
+        dom.set_profiling(False)
+    it1 = dom.solver_stats()[0] if w["algorithm"] == "implicit" else 0
+    return dict(dom=dom, mesh=mesh, elapsed=elapsed, kernels=kt, iters=it1 - it0)
+
+
+def cpu_baseline(w, sigma, sample_steps):
+    """The CPU FP64 oracle (OpenMP over all host cores) on the same workload: the full
+    mesh for `sample_steps` steps after a 2-step start, timed as cell-steps/s."""
+    import oracle
+    nx, ny, nz = w["shape"]
+    cores = os.cpu_count() or 1
+    model = {"ORd": oracle.ORD, "BuenoOrovio": oracle.BO}[w["model"]]
+    tissue = {"myocyte": oracle.MYOCYTE, "epicardialCells": oracle.EPI}[w["tissue"]]
+    T = oracle.OracleTissue(nx, ny, nz, w["h"], sigma, model, tissue, w["dt"], 1.4e5, 0.01,
+                            integrator=0, ddt=0, implicit=int(w["algorithm"] == "implicit"),
+                            pcg_tol=w["tol"] or 1e-8, nthreads=cores)
+    bx = oracle.make_boxes([0, 0, 0], [1.5e-3] * 3, [0.0], [2e-3], [5e4])
+    T.step(bx)
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        T.step(bx)
+    el = time.perf_counter() - t0
+    n = nx * ny * nz
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": n * sample_steps / el, "unit": "cell-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": "CPU FP64 oracle (C, OpenMP, %d threads, %s): full %dx%dx%d mesh, %d steps "
+                      "after 1 warm-up step, %.1f s" % (cores, cpu, nx, ny, nz, sample_steps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=250)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="ord_implicit_2M", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the tissue step is not sharded across GPUs in this "
+              "build (see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    import cardiacfoam_amd as cf
+
+    w = WORKLOADS[args.workload]
+    sigma = make_sigma(w["shape"], w["rotating"])
+    n = int(np.prod(w["shape"]))
+    res = {}
+    for prec in ("fp64", "mixed"):
+        res[prec] = run_precision(cf, torch, w, sigma, prec, args.steps, args.warmup, True)
+        print("[bench] %s: %.3f s for %d steps" % (prec, res[prec]["elapsed"], args.steps),
+              file=sys.stderr, flush=True)
+    V64, V32 = res["fp64"]["dom"].Vm, res["mixed"]["dom"].Vm
+    a64, a32 = res["fp64"]["dom"].activationTime, res["mixed"]["dom"].activationTime
+    vm_err_mv = float(np.max(np.abs(V64 - V32)) * 1e3)
+    both = (a64 > 0) & (a32 > 0)
+    act_err_ms = float(np.max(np.abs(a64[both] - a32[both])) * 1e3) if both.any() else 0.0
+    act_mismatch = int(np.count_nonzero((a64 > 0) != (a32 > 0)))
+    ok = vm_err_mv <= 0.5 and act_err_ms <= 0.05
+    thr = {p: n * args.steps / res[p]["elapsed"] for p in res}
+
+    # roofline of the dominant kernel of the mixed path (HIP-event times, same stream)
+    kt = res["mixed"]["kernels"]
+    dom_k = max(kt, key=lambda k: kt[k][0]) if kt else None
+    roof = None
+    kernels_out = {}
+    for prec in ("fp64", "mixed"):
+        for k, (ms, cnt) in res[prec]["kernels"].items():
+            bpc = algo_bytes_per_cell(k, prec, w["model"], w["algorithm"])
+            avg = ms / cnt if cnt else 0.0
+            gbs = bpc * n / (avg * 1e-3) / 1e9 if (bpc and avg > 0) else None
+            kernels_out["%s/%s" % (prec, k)] = {"avg_us": round(avg * 1e3, 2), "launches": cnt,
+                                                "bytes_per_cell": bpc,
+                                                "GBps": round(gbs, 1) if gbs else None,
+                                                "share": round(ms / sum(v[0] for v in res[prec]["kernels"].values()), 3)}
+    if dom_k:
+        ms, cnt = kt[dom_k]
+        bpc = algo_bytes_per_cell(dom_k, "mixed", w["model"], w["algorithm"])
+        avg_s = ms / cnt * 1e-3
+        ach = bpc * n / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom_k, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": bpc * n, "avg_launch_us": round(avg_s * 1e6, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(w, sigma, args.cpu_steps)
+        except Exception as e:  # reported, never silently replaced
+            cpu = {"error": repr(e)}
+
+    line = {
+        "metric": "cell-steps/s mixed vs FP64 at <=0.5 mV Vm Linf error; adjoint/forward cost",
+        "value": thr["mixed"] if ok else thr["fp64"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": (res["mixed"] if ok else res["fp64"])["elapsed"] / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64" if ok else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "cells": n, "shape": w["shape"], "h_m": w["h"],
+                   "model": w["model"], "algorithm": w["algorithm"], "dt_s": w["dt"],
+                   "tol": w["tol"]},
+        "fp64_cell_steps_per_s": thr["fp64"],
+        "mixed_cell_steps_per_s": thr["mixed"],
+        "mixed_over_fp64": thr["mixed"] / thr["fp64"],
+        "mixed_vm_linf_mV": vm_err_mv, "mixed_activation_linf_ms": act_err_ms,
+        "mixed_activation_set_mismatch": act_mismatch, "mixed_gate_passed": ok,
+        "cg_iterations_per_step": {p: res[p]["iters"] / args.steps for p in res},
+        "roofline": roof,
+        "kernels": kernels_out,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,730 @@
+// C-ABI of include/cardiacgpu.h: handles, argument checks, host-side time stepping,
+// and launches of the gfx950 kernels.  No torch types, no global state except the
+// thread-local error message.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <cmath>
+
+#include "../../include/cardiacgpu.h"
+#include "cg_internal.hpp"
+#include "cg_implicit.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(CG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));           \
+    } while (0)
+
+const char* const kModelKeys[4] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov"};
+
+// state names, order of the *_STATES_NAMES tables / Names.H enums
+const char* const kBONames[4] = {"u", "v", "w", "s"};
+const char* const kAPNames[2] = {"u", "recovery_r"};
+const char* const kTNNPNames[17] = {"V", "K_i", "Na_i", "Ca_i", "Xr1", "Xr2", "Xs", "m", "h",
+                                    "j", "d", "f", "fCa", "s", "r", "Ca_SR", "g"};
+const char* const kORdNames[41] = {
+    "membrane_V", "CaMK_CaMKt", "Na_i", "Na_ss", "K_i", "K_ss", "Ca_ss", "Ca_nsr", "Ca_jsr",
+    "Ca_i", "INa_m", "INa_hf", "INa_hs", "INa_j", "INa_hsp", "INa_jp", "INaL_mL", "INaL_hL",
+    "INaL_hLp", "Ito_a", "Ito_iF", "Ito_iS", "Ito_ap", "Ito_iFp", "Ito_iSp", "ICaL_d",
+    "ICaL_ff", "ICaL_fs", "ICaL_fcaf", "ICaL_fcas", "ICaL_jca", "ICaL_ffp", "ICaL_fcafp",
+    "ICaL_nca", "IKr_xrf", "IKr_xrs", "IKs_xs1", "IKs_xs2", "IK1_xk1", "ryr_Jrelnp",
+    "ryr_Jrelp"};
+
+struct ModelInfo {
+    int ns, np;
+    double vscale, voff, iscale, tscale;
+};
+
+bool model_info(int model, ModelInfo* mi)
+{
+    switch (model) {
+    case cg::MODEL_BO:
+        *mi = {cg::BuenoOrovio::NS, cg::BuenoOrovio::NP, cg::BuenoOrovio::VSCALE,
+               cg::BuenoOrovio::VOFF, cg::BuenoOrovio::ISCALE, cg::BuenoOrovio::TSCALE};
+        return true;
+    case cg::MODEL_TNNP:
+        *mi = {cg::TNNP::NS, cg::TNNP::NP, 1.0, 0.0, 1.0, 1000.0};
+        return true;
+    case cg::MODEL_ORD:
+        *mi = {cg::ORd::NS, cg::ORd::NP, 1.0, 0.0, 1.0, 1000.0};
+        return true;
+    case cg::MODEL_AP:
+        *mi = {cg::AlievPanfilov::NS, cg::AlievPanfilov::NP, cg::AlievPanfilov::VSCALE,
+               cg::AlievPanfilov::VOFF, cg::AlievPanfilov::ISCALE, cg::AlievPanfilov::TSCALE};
+        return true;
+    default: return false;
+    }
+}
+
+bool model_host_init(int model, int tissue, double* p, double* y)
+{
+    switch (model) {
+    case cg::MODEL_BO: return cg::BuenoOrovio::host_init(tissue, p, y);
+    case cg::MODEL_TNNP: return cg::TNNP::host_init(tissue, p, y);
+    case cg::MODEL_ORD: return cg::ORd::host_init(tissue, p, y);
+    case cg::MODEL_AP: return cg::AlievPanfilov::host_init(tissue, p, y);
+    default: return false;
+    }
+}
+
+// model time of a PDE time in seconds, as each wrapper's solveODE scales it
+// (BuenoOrovio.C:107-108, AlievPanfilov.C:107-108)
+double model_time(int model, double t_s)
+{
+    return model == cg::MODEL_AP ? t_s * 1000.0 / 12.9 : t_s * 1000.0;
+}
+
+} // namespace
+
+struct cg_ionic {
+    int model, tissue, prec, solve_vm;
+    ModelInfo mi;
+    int64_t n;
+    cg::ModelParams P;
+    size_t elem;
+    void* d_states = nullptr;
+    void* d_gscale = nullptr;
+    double* d_vm = nullptr;
+    double* d_im = nullptr;
+    hipStream_t stream = nullptr;
+    cg::Stimulus stim{};
+};
+
+struct cg_mesh {
+    cg::HexGeom g;
+    int64_t n;
+    bool uniform = true;
+    bool has_corr = false;
+    double sig[9] = {0};
+    double* d_sigma9 = nullptr;
+    double* d_fx = nullptr;
+    double* d_fy = nullptr;
+    double* d_fz = nullptr;
+    std::vector<double> h_sigma9;  // host copy for dt and implicit assembly
+};
+
+struct cg_tissue {
+    cg_mesh* mesh;
+    cg_ionic* ion;
+    cg_tissue_config cfg;
+    double* d_V[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;  // index of V^n; (cur+2)%3 holds V^{n-1}
+    double* d_Iion = nullptr;
+    uint8_t* d_flag = nullptr;
+    double* d_tact = nullptr;
+    double* d_corr = nullptr;
+    double* d_grad = nullptr;
+    int n_boxes = 0;
+    double box_lo[cg::MAX_BOXES][3], box_hi[cg::MAX_BOXES][3];
+    double box_t0[cg::MAX_BOXES], box_dur[cg::MAX_BOXES], box_I[cg::MAX_BOXES];
+    double t = 0.0;
+    int64_t steps = 0;
+    cg::ImplicitSolver* implicit = nullptr;
+    cg::Profiler prof;
+};
+
+extern "C" {
+
+const char* cg_last_error(void) { return g_err.c_str(); }
+const char* cg_version(void) { return "cardiacgpu 0.1 gfx950"; }
+
+int cg_model_id(const char* key)
+{
+    if (!key) return -1;
+    for (int i = 0; i < 4; ++i)
+        if (!strcmp(key, kModelKeys[i])) return i;
+    return -1;
+}
+
+int cg_tissue_flag(const char* t)
+{
+    if (!t) return -1;
+    if (!strcmp(t, "epicardialCells")) return 1;
+    if (!strcmp(t, "mCells")) return 2;
+    if (!strcmp(t, "endocardialCells")) return 3;
+    if (!strcmp(t, "myocyte")) return 4;
+    return -1;
+}
+
+int cg_model_nstates(int model)
+{
+    ModelInfo mi;
+    return model_info(model, &mi) ? mi.ns : -1;
+}
+
+int cg_model_state_name(int model, int i, char* buf, size_t len)
+{
+    ModelInfo mi;
+    if (!model_info(model, &mi)) return fail(CG_ERR_MODEL, "unknown model id %d", model);
+    if (i < 0 || i >= mi.ns || !buf || len == 0) return fail(CG_ERR_ARG, "bad state index %d", i);
+    const char* nm = model == 0 ? kBONames[i] : model == 1 ? kTNNPNames[i]
+                   : model == 2 ? kORdNames[i] : kAPNames[i];
+    snprintf(buf, len, "%s", nm);
+    return CG_OK;
+}
+
+int cg_ionic_create(int model, int tissue, int64_t n, int solve_vm, int precision,
+                    cg_ionic** out)
+{
+    if (!out) return fail(CG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    ModelInfo mi;
+    if (!model_info(model, &mi)) return fail(CG_ERR_MODEL, "unknown ionicModel id %d", model);
+    if (n <= 0) return fail(CG_ERR_ARG, "n_cells must be positive");
+    if (precision != CG_FP64 && precision != CG_MIXED)
+        return fail(CG_ERR_ARG, "unknown precision %d", precision);
+    double p[cg::MAX_PARAMS] = {0}, y0[64] = {0};
+    if (!model_host_init(model, tissue, p, y0))
+        return fail(CG_ERR_TISSUE, "Unsupported tissue flag %d for ionicModel %s", tissue,
+                    kModelKeys[model]);
+    cg_ionic* h = new cg_ionic();
+    h->model = model; h->tissue = tissue; h->prec = precision; h->solve_vm = solve_vm ? 1 : 0;
+    h->mi = mi; h->n = n;
+    memcpy(h->P.p, p, sizeof p);
+    h->elem = precision == CG_FP64 ? sizeof(double) : sizeof(float);
+    const size_t bytes = h->elem * (size_t)mi.ns * (size_t)n;
+    if (hipMalloc(&h->d_states, bytes) != hipSuccess) {
+        delete h;
+        return fail(CG_ERR_NOMEM, "hipMalloc of %zu bytes for states failed", bytes);
+    }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        hipFree(h->d_states);
+        delete h;
+        return fail(CG_ERR_HIP, "stream creation failed");
+    }
+    // initial states, one SoA plane per variable
+    std::vector<char> plane(h->elem * (size_t)n);
+    for (int k = 0; k < mi.ns; ++k) {
+        if (precision == CG_FP64) {
+            double* q = (double*)plane.data();
+            for (int64_t i = 0; i < n; ++i) q[i] = y0[k];
+        } else {
+            float* q = (float*)plane.data();
+            for (int64_t i = 0; i < n; ++i) q[i] = (float)y0[k];
+        }
+        hipMemcpy((char*)h->d_states + h->elem * (size_t)k * n, plane.data(), plane.size(),
+                  hipMemcpyHostToDevice);
+    }
+    *out = h;
+    return CG_OK;
+}
+
+int cg_ionic_destroy(cg_ionic* h)
+{
+    if (!h) return CG_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    hipFree(h->d_states);
+    if (h->d_gscale) hipFree(h->d_gscale);
+    if (h->d_vm) hipFree(h->d_vm);
+    if (h->d_im) hipFree(h->d_im);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return CG_OK;
+}
+
+int cg_ionic_nstates(const cg_ionic* h) { return h ? h->mi.ns : -1; }
+
+int cg_ionic_set_constant(cg_ionic* h, int k, double v)
+{
+    if (!h) return fail(CG_ERR_ARG, "NULL handle");
+    if (k < 0 || k >= h->mi.np) return fail(CG_ERR_ARG, "constant index %d out of range", k);
+    h->P.p[k] = v;
+    return CG_OK;
+}
+
+int cg_ionic_set_conductance_scale(cg_ionic* h, const double* s)
+{
+    if (!h) return fail(CG_ERR_ARG, "NULL handle");
+    if (h->model != cg::MODEL_TNNP && h->model != cg::MODEL_ORD && s)
+        return fail(CG_ERR_STATE, "conductance scaling is defined for TNNP and ORd only");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!s) {
+        if (h->d_gscale) hipFree(h->d_gscale);
+        h->d_gscale = nullptr;
+        return CG_OK;
+    }
+    const size_t cnt = 4 * (size_t)h->n;
+    if (!h->d_gscale && hipMalloc(&h->d_gscale, h->elem * cnt) != hipSuccess)
+        return fail(CG_ERR_NOMEM, "hipMalloc for conductance scales failed");
+    if (h->prec == CG_FP64) {
+        HIPCHK(hipMemcpy(h->d_gscale, s, sizeof(double) * cnt, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(cnt);
+        for (size_t i = 0; i < cnt; ++i) f[i] = (float)s[i];
+        HIPCHK(hipMemcpy(h->d_gscale, f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice));
+    }
+    return CG_OK;
+}
+
+int cg_ionic_set_stimulus(cg_ionic* h, const cg_stimulus* s)
+{
+    if (!h || !s) return fail(CG_ERR_ARG, "NULL argument");
+    if (!h->solve_vm)
+        return fail(CG_ERR_STATE,
+                    "Detected active ionic-model stimulus protocol while using a "
+                    "reaction-diffusion myocardium domain; use externalStimulus");
+    if (s->stim_duration < 0.0) return fail(CG_ERR_STIMULUS, "stim_duration must be non-negative");
+    h->stim = {s->stim_start, s->stim_period_s1, s->stim_duration, s->stim_amplitude,
+               s->nstim1, s->stim_period_s2, s->nstim2};
+    return CG_OK;
+}
+
+int cg_ionic_step(cg_ionic* h, double t0, double dt, const double* Vm, double* Im, int flags)
+{
+    if (!h || !Vm || !Im) return fail(CG_ERR_ARG, "NULL argument");
+    if (h->solve_vm) return fail(CG_ERR_STATE, "cg_ionic_step needs a tissue handle");
+    if (!(dt > 0.0)) return fail(CG_ERR_ARG, "deltaT must be positive");
+    (void)t0;  // no time-dependent term in tissue mode (stimulus lives in the PDE)
+    const double* dv = Vm;
+    double* di = Im;
+    if (!(flags & CG_DEVICE_PTRS)) {
+        if (!h->d_vm) {
+            if (hipMalloc(&h->d_vm, sizeof(double) * h->n) != hipSuccess ||
+                hipMalloc(&h->d_im, sizeof(double) * h->n) != hipSuccess)
+                return fail(CG_ERR_NOMEM, "staging allocation failed");
+        }
+        HIPCHK(hipMemcpyAsync(h->d_vm, Vm, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+        dv = h->d_vm;
+        di = h->d_im;
+    }
+    const double dtm = h->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
+    HIPCHK(cg::launch_ionic_tissue(h->model, h->prec, h->tissue, h->d_states, h->n, dv, di, dtm,
+                                   h->P, h->d_gscale, h->stream));
+    if (!(flags & CG_DEVICE_PTRS))
+        HIPCHK(hipMemcpyAsync(Im, di, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CG_OK;
+}
+
+int cg_ionic_run_cells(cg_ionic* h, double t0, double dt, int n_steps)
+{
+    if (!h) return fail(CG_ERR_ARG, "NULL handle");
+    if (!h->solve_vm) return fail(CG_ERR_STATE, "cg_ionic_run_cells needs a single-cell handle");
+    if (n_steps < 0 || !(dt > 0.0)) return fail(CG_ERR_ARG, "bad step arguments");
+    const double dtm = h->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
+    HIPCHK(cg::launch_ionic_cells(h->model, h->prec, h->tissue, h->d_states, h->n,
+                                  model_time(h->model, t0), dtm, n_steps, h->P, h->d_gscale,
+                                  h->stim, h->stream));
+    return CG_OK;
+}
+
+static int states_io(cg_ionic* h, int first, int nv, double* host, const double* hin)
+{
+    if (!h || (!host && !hin)) return fail(CG_ERR_ARG, "NULL argument");
+    if (first < 0 || nv < 0 || first + nv > h->mi.ns) return fail(CG_ERR_ARG, "state range");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t cnt = (size_t)nv * h->n;
+    char* dsrc = (char*)h->d_states + h->elem * (size_t)first * h->n;
+    if (h->prec == CG_FP64) {
+        if (host) HIPCHK(hipMemcpy(host, dsrc, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+        else HIPCHK(hipMemcpy(dsrc, hin, sizeof(double) * cnt, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(cnt);
+        if (host) {
+            HIPCHK(hipMemcpy(f.data(), dsrc, sizeof(float) * cnt, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < cnt; ++i) host[i] = f[i];
+        } else {
+            for (size_t i = 0; i < cnt; ++i) f[i] = (float)hin[i];
+            HIPCHK(hipMemcpy(dsrc, f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice));
+        }
+    }
+    return CG_OK;
+}
+
+int cg_ionic_get_states(cg_ionic* h, int first, int nv, double* host)
+{
+    return states_io(h, first, nv, host, nullptr);
+}
+
+int cg_ionic_set_states(cg_ionic* h, int first, int nv, const double* host)
+{
+    return states_io(h, first, nv, nullptr, host);
+}
+
+// ------------------------------------------------------------------------ mesh
+int cg_mesh_create_hex(int nx, int ny, int nz, double hx, double hy, double hz, cg_mesh** out)
+{
+    if (!out) return fail(CG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1 || !(hx > 0) || !(hy > 0) || !(hz > 0))
+        return fail(CG_ERR_ARG, "bad hex block dimensions");
+    cg_mesh* m = new cg_mesh();
+    m->g = {nx, ny, nz, hx, hy, hz};
+    m->n = (int64_t)nx * ny * nz;
+    *out = m;
+    return CG_OK;
+}
+
+int cg_mesh_match_ldu(int nx, int ny, int nz, int64_t nf, const int32_t* owner,
+                      const int32_t* neighbour)
+{
+    if (!owner || !neighbour) return fail(CG_ERR_ARG, "NULL addressing");
+    int64_t f = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int32_t c = i + nx * (j + ny * k);
+                const int32_t nb[3] = {c + 1, c + nx, c + nx * ny};
+                const bool has[3] = {i < nx - 1, j < ny - 1, k < nz - 1};
+                for (int d = 0; d < 3; ++d) {
+                    if (!has[d]) continue;
+                    if (f >= nf || owner[f] != c || neighbour[f] != nb[d])
+                        return fail(CG_ERR_ARG, "addressing differs from the hex block at face %lld",
+                                    (long long)f);
+                    ++f;
+                }
+            }
+    if (f != nf) return fail(CG_ERR_ARG, "face count %lld != %lld", (long long)nf, (long long)f);
+    return CG_OK;
+}
+
+int cg_mesh_destroy(cg_mesh* m)
+{
+    if (!m) return CG_OK;
+    hipFree(m->d_sigma9); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
+    delete m;
+    return CG_OK;
+}
+
+int64_t cg_mesh_ncells(const cg_mesh* m) { return m ? m->n : -1; }
+
+int cg_mesh_set_conductivity(cg_mesh* m, const double* s, int uniform)
+{
+    if (!m || !s) return fail(CG_ERR_ARG, "NULL argument");
+    const cg::HexGeom& g = m->g;
+    const int64_t n = m->n;
+    hipFree(m->d_sigma9); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
+    m->d_sigma9 = m->d_fx = m->d_fy = m->d_fz = nullptr;
+    m->h_sigma9.assign(9 * (size_t)n, 0.0);
+    for (int64_t c = 0; c < n; ++c)
+        for (int q = 0; q < 9; ++q) m->h_sigma9[9 * c + q] = uniform ? s[q] : s[9 * c + q];
+    bool offdiag = false;
+    for (int64_t c = 0; c < n && !offdiag; ++c)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                if (a != b && m->h_sigma9[9 * c + 3 * a + b] != 0.0) offdiag = true;
+    m->has_corr = offdiag;
+    m->uniform = uniform != 0;
+    if (uniform) {
+        memcpy(m->sig, s, sizeof m->sig);
+    } else {
+        // per-face orthogonal coefficients: a_d = sigma_dd,f / h_d^2 (face area/(h V))
+        std::vector<double> fx(n, 0.0), fy(n, 0.0), fz(n, 0.0);
+        const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+        for (int64_t c = 0; c < n; ++c) {
+            const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / sz);
+            const double* a = &m->h_sigma9[9 * c];
+            if (i < g.nx - 1) fx[c] = 0.5 * (a[0] + m->h_sigma9[9 * (c + 1) + 0]) / (g.hx * g.hx);
+            if (j < g.ny - 1) fy[c] = 0.5 * (a[4] + m->h_sigma9[9 * (c + sy) + 4]) / (g.hy * g.hy);
+            if (k < g.nz - 1) fz[c] = 0.5 * (a[8] + m->h_sigma9[9 * (c + sz) + 8]) / (g.hz * g.hz);
+        }
+        if (hipMalloc(&m->d_fx, sizeof(double) * n) != hipSuccess ||
+            hipMalloc(&m->d_fy, sizeof(double) * n) != hipSuccess ||
+            hipMalloc(&m->d_fz, sizeof(double) * n) != hipSuccess)
+            return fail(CG_ERR_NOMEM, "face coefficient allocation failed");
+        HIPCHK(hipMemcpy(m->d_fx, fx.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(m->d_fy, fy.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(m->d_fz, fz.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    if (offdiag) {
+        if (hipMalloc(&m->d_sigma9, sizeof(double) * 9 * n) != hipSuccess)
+            return fail(CG_ERR_NOMEM, "conductivity allocation failed");
+        HIPCHK(hipMemcpy(m->d_sigma9, m->h_sigma9.data(), sizeof(double) * 9 * n,
+                         hipMemcpyHostToDevice));
+    }
+    return CG_OK;
+}
+
+int cg_mesh_explicit_dt(const cg_mesh* m, double chi, double cm, double maxco, double* dt)
+{
+    if (!m || !dt) return fail(CG_ERR_ARG, "NULL argument");
+    if (m->h_sigma9.empty()) return fail(CG_ERR_STATE, "conductivity not set");
+    const cg::HexGeom& g = m->g;
+    const double chiCm = chi * cm;
+    double best = INFINITY;
+    const int64_t st[3] = {1, g.nx, (int64_t)g.nx * g.ny};
+    const double h[3] = {g.hx, g.hy, g.hz};
+    for (int64_t c = 0; c < m->n; ++c) {
+        const int idx[3] = {(int)(c % g.nx), (int)((c / g.nx) % g.ny), (int)(c / st[2])};
+        const int nn[3] = {g.nx, g.ny, g.nz};
+        for (int d = 0; d < 3; ++d) {
+            if (idx[d] >= nn[d] - 1) continue;
+            const double snn = 0.5 * (m->h_sigma9[9 * c + 4 * d] + m->h_sigma9[9 * (c + st[d]) + 4 * d]);
+            const double q = h[d] * h[d] / (snn / chiCm);
+            if (q < best) best = q;
+        }
+    }
+    if (!std::isfinite(best)) return fail(CG_ERR_STATE, "no internal faces");
+    *dt = maxco * best;
+    return CG_OK;
+}
+
+static cg::HexCoefs mesh_coefs(const cg_mesh* m)
+{
+    cg::HexCoefs k{};
+    if (m->uniform) {
+        k.ax = m->sig[0] / (m->g.hx * m->g.hx);
+        k.ay = m->sig[4] / (m->g.hy * m->g.hy);
+        k.az = m->sig[8] / (m->g.hz * m->g.hz);
+    } else {
+        k.fx = m->d_fx; k.fy = m->d_fy; k.fz = m->d_fz;
+    }
+    return k;
+}
+
+int cg_mesh_laplacian(cg_mesh* m, const double* V, double* out)
+{
+    if (!m || !V || !out) return fail(CG_ERR_ARG, "NULL argument");
+    if (m->h_sigma9.empty()) return fail(CG_ERR_STATE, "conductivity not set");
+    double* corr = nullptr;
+    double* grad = nullptr;
+    if (m->has_corr) {
+        if (hipMalloc(&corr, sizeof(double) * m->n) != hipSuccess ||
+            hipMalloc(&grad, sizeof(double) * 3 * m->n) != hipSuccess)
+            return fail(CG_ERR_NOMEM, "scratch allocation failed");
+        HIPCHK(cg::launch_hex_correction(m->g, m->d_sigma9, V, grad, corr, nullptr));
+    }
+    HIPCHK(cg::launch_hex_laplacian(m->g, mesh_coefs(m), V, corr, out, nullptr));
+    HIPCHK(hipDeviceSynchronize());
+    hipFree(corr);
+    hipFree(grad);
+    return CG_OK;
+}
+
+// ---------------------------------------------------------------------- tissue
+int cg_tissue_create(cg_mesh* m, cg_ionic* ion, const cg_tissue_config* cfg, cg_tissue** out)
+{
+    if (!out || !m || !ion || !cfg) return fail(CG_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (ion->solve_vm) return fail(CG_ERR_STATE, "tissue needs an ionic handle with solve_vm = 0");
+    if (ion->n != m->n) return fail(CG_ERR_ARG, "ionic cells %lld != mesh cells %lld",
+                                    (long long)ion->n, (long long)m->n);
+    if (m->h_sigma9.empty()) return fail(CG_ERR_STATE, "conductivity not set");
+    if (!(cfg->dt > 0) || !(cfg->chi > 0) || !(cfg->cm > 0)) return fail(CG_ERR_ARG, "bad dt/chi/cm");
+    if (cfg->algorithm != CG_EXPLICIT && cfg->algorithm != CG_IMPLICIT)
+        return fail(CG_ERR_ARG, "unknown solutionAlgorithm %d", cfg->algorithm);
+    if (cfg->ddt != CG_DDT_EULER && cfg->ddt != CG_DDT_BACKWARD)
+        return fail(CG_ERR_ARG, "unknown ddt scheme %d", cfg->ddt);
+    cg_tissue* t = new cg_tissue();
+    t->mesh = m; t->ion = ion; t->cfg = *cfg;
+    t->prof.stream = ion->stream;
+    const int64_t n = m->n;
+    bool ok = true;
+    for (int q = 0; q < 3; ++q) ok &= hipMalloc(&t->d_V[q], sizeof(double) * n) == hipSuccess;
+    ok &= hipMalloc(&t->d_Iion, sizeof(double) * n) == hipSuccess;
+    ok &= hipMalloc(&t->d_flag, n) == hipSuccess;
+    ok &= hipMalloc(&t->d_tact, sizeof(double) * n) == hipSuccess;
+    if (m->has_corr) {
+        ok &= hipMalloc(&t->d_corr, sizeof(double) * n) == hipSuccess;
+        ok &= hipMalloc(&t->d_grad, sizeof(double) * 3 * n) == hipSuccess;
+    }
+    if (!ok) {
+        cg_tissue_destroy(t);
+        return fail(CG_ERR_NOMEM, "tissue allocation failed");
+    }
+    std::vector<double> v(n, cfg->vm0);
+    for (int q = 0; q < 3; ++q)
+        HIPCHK(hipMemcpy(t->d_V[q], v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(t->d_Iion, 0, sizeof(double) * n));
+    HIPCHK(hipMemset(t->d_flag, 1, n));
+    HIPCHK(hipMemset(t->d_tact, 0, sizeof(double) * n));
+    if (cfg->algorithm == CG_IMPLICIT) {
+        int rc = cg::implicit_create(&t->implicit, m->g, m->h_sigma9.data(), cfg->chi * cfg->cm,
+                                     cfg->dt, ion->prec, ion->stream, &g_err);
+        if (rc != 0) {
+            cg_tissue_destroy(t);
+            return CG_ERR_NOMEM;
+        }
+    }
+    *out = t;
+    return CG_OK;
+}
+
+int cg_tissue_destroy(cg_tissue* t)
+{
+    if (!t) return CG_OK;
+    if (t->ion && t->ion->stream) hipStreamSynchronize(t->ion->stream);
+    for (int q = 0; q < 3; ++q) hipFree(t->d_V[q]);
+    hipFree(t->d_Iion); hipFree(t->d_flag); hipFree(t->d_tact);
+    hipFree(t->d_corr); hipFree(t->d_grad);
+    if (t->implicit) cg::implicit_destroy(t->implicit);
+    delete t;
+    return CG_OK;
+}
+
+int cg_tissue_set_stimulus_boxes(cg_tissue* t, int n, const double* lo, const double* hi,
+                                 const double* t0, const double* dur, const double* I)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (n < 0 || n > cg::MAX_BOXES) return fail(CG_ERR_ARG, "at most %d stimulus boxes", cg::MAX_BOXES);
+    if (n > 0 && (!lo || !hi || !t0 || !dur || !I)) return fail(CG_ERR_ARG, "NULL box array");
+    for (int b = 0; b < n; ++b) {
+        if (dur[b] < 0.0)
+            return fail(CG_ERR_STIMULUS, "stimulusDuration must be non-negative. Found value %g at index %d.",
+                        dur[b], b);
+        for (int d = 0; d < 3; ++d) { t->box_lo[b][d] = lo[3 * b + d]; t->box_hi[b][d] = hi[3 * b + d]; }
+        t->box_t0[b] = t0[b]; t->box_dur[b] = dur[b]; t->box_I[b] = I[b];
+    }
+    t->n_boxes = n;
+    return CG_OK;
+}
+
+static cg::ActiveBoxes active_boxes(const cg_tissue* t, double t0)
+{
+    cg::ActiveBoxes a{};
+    for (int b = 0; b < t->n_boxes; ++b) {
+        const double tS = t->box_t0[b], tE = tS + t->box_dur[b];
+        if (t0 < tS || t0 > tE) continue;
+        for (int d = 0; d < 3; ++d) { a.lo[a.n][d] = t->box_lo[b][d]; a.hi[a.n][d] = t->box_hi[b][d]; }
+        a.intensity[a.n] = t->box_I[b];
+        ++a.n;
+    }
+    return a;
+}
+
+int cg_tissue_step(cg_tissue* t, int n_steps)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    cg_ionic* ion = t->ion;
+    cg_mesh* m = t->mesh;
+    const double dt = t->cfg.dt, chiCm = t->cfg.chi * t->cfg.cm;
+    const double dtm = ion->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
+    const cg::HexCoefs coefs = mesh_coefs(m);
+    for (int s = 0; s < n_steps; ++s) {
+        const double tn = t->t + dt;
+        const double t0 = tn - dt;
+        const bool bdf2 = t->cfg.ddt == CG_DDT_BACKWARD && t->steps >= 1;
+        cg::ExplicitStep st{};
+        st.dt = dt; st.dt_model = dtm; st.chiCm = chiCm;
+        st.c = bdf2 ? 1.5 : 1.0; st.c0 = bdf2 ? 2.0 : 1.0; st.c00 = bdf2 ? 0.5 : 0.0;
+        st.t_old = t0;
+        st.boxes = active_boxes(t, t0);
+        double* Vn = t->d_V[t->cur];
+        double* Vm1 = t->d_V[(t->cur + 2) % 3];
+        double* Vnew = t->d_V[(t->cur + 1) % 3];
+        cg::Profiler* prof = t->prof.enabled ? &t->prof : nullptr;
+        if (m->has_corr) {
+            const int tk = prof ? prof->begin(cg::K_CORR) : -1;
+            HIPCHK(cg::launch_hex_correction(m->g, m->d_sigma9, Vn, t->d_grad, t->d_corr, ion->stream));
+            if (prof) prof->end(tk);
+            st.corr = t->d_corr;
+        }
+        if (t->cfg.algorithm == CG_EXPLICIT) {
+            const int tk = prof ? prof->begin(cg::K_IONIC) : -1;
+            HIPCHK(cg::launch_explicit_hex(ion->model, ion->prec, ion->tissue, m->g, coefs,
+                                           ion->d_states, Vn, Vm1, Vnew, t->d_Iion, t->d_flag,
+                                           t->d_tact, st, ion->P, ion->stream));
+            if (prof) prof->end(tk);
+        } else {
+            int rc = cg::implicit_step(t->implicit, ion->model, ion->prec, ion->tissue,
+                                       ion->d_states, ion->P, Vn, Vm1, Vnew, t->d_Iion,
+                                       t->d_flag, t->d_tact, st, t->cfg.tol, t->cfg.max_iter,
+                                       prof, &g_err);
+            if (rc != 0) return rc;
+        }
+        t->cur = (t->cur + 1) % 3;
+        t->t = tn;
+        t->steps++;
+    }
+    return CG_OK;
+}
+
+int cg_tissue_sync(cg_tissue* t)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    HIPCHK(hipStreamSynchronize(t->ion->stream));
+    return CG_OK;
+}
+
+static double* field_ptr(cg_tissue* t, int which)
+{
+    switch (which) {
+    case CG_FIELD_VM: return t->d_V[t->cur];
+    case CG_FIELD_IION: return t->d_Iion;
+    case CG_FIELD_ACTIVATION: return t->d_tact;
+    case CG_FIELD_VM_OLD: return t->d_V[(t->cur + 2) % 3];
+    default: return nullptr;
+    }
+}
+
+int cg_tissue_get_field(cg_tissue* t, int which, double* host)
+{
+    if (!t || !host) return fail(CG_ERR_ARG, "NULL argument");
+    double* d = field_ptr(t, which);
+    if (!d) return fail(CG_ERR_ARG, "unknown field %d", which);
+    HIPCHK(hipStreamSynchronize(t->ion->stream));
+    HIPCHK(hipMemcpy(host, d, sizeof(double) * t->mesh->n, hipMemcpyDeviceToHost));
+    return CG_OK;
+}
+
+int cg_tissue_set_field(cg_tissue* t, int which, const double* host)
+{
+    if (!t || !host) return fail(CG_ERR_ARG, "NULL argument");
+    double* d = field_ptr(t, which);
+    if (!d) return fail(CG_ERR_ARG, "unknown field %d", which);
+    HIPCHK(hipStreamSynchronize(t->ion->stream));
+    HIPCHK(hipMemcpy(d, host, sizeof(double) * t->mesh->n, hipMemcpyHostToDevice));
+    return CG_OK;
+}
+
+int cg_tissue_time(const cg_tissue* t, double* time_s, int64_t* steps)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (time_s) *time_s = t->t;
+    if (steps) *steps = t->steps;
+    return CG_OK;
+}
+
+int cg_tissue_set_profiling(cg_tissue* t, int on)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    t->prof.reset();
+    t->prof.enabled = on != 0;
+    return CG_OK;
+}
+
+int cg_tissue_kernel_times(cg_tissue* t, int max, char* names, double* total_ms,
+                           int64_t* launches, int* n_out)
+{
+    if (!t || !n_out) return fail(CG_ERR_ARG, "NULL argument");
+    t->prof.resolve(true);
+    int k = 0;
+    for (int id = 0; id < cg::K_COUNT && k < max; ++id) {
+        if (t->prof.launches[id] == 0) continue;
+        if (names) snprintf(names + 32 * k, 32, "%s", cg::kernel_name(id));
+        if (total_ms) total_ms[k] = t->prof.total_ms[id];
+        if (launches) launches[k] = t->prof.launches[id];
+        ++k;
+    }
+    *n_out = k;
+    return CG_OK;
+}
+
+int cg_tissue_solver_stats(const cg_tissue* t, int64_t* iters, double* res)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
+    cg::implicit_stats(t->implicit, iters, res);
+    return CG_OK;
+}
+
+} // extern "C"

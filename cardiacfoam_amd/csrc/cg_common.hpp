@@ -1,0 +1,64 @@
+// Shared definitions for the gfx950 kernels of the monodomain tissue step.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cmath>
+
+#define CG_DEV __device__ __forceinline__
+#define CG_HD __host__ __device__ __forceinline__
+
+namespace cg {
+
+// Heaviside of src/ionicModels/BuenoOrovio/heaviside.H:39-49 (strict x > 0).
+template <class T> CG_DEV bool pos(T x) { return x > T(0); }
+
+// Precision-generic math: the FP64 path calls the double routines, the FP32 path
+// the single ones (hardware v_exp_f32 based), never mixing the two in one formula.
+CG_DEV double cexp(double x) { return exp(x); }
+CG_DEV float cexp(float x) { return expf(x); }
+CG_DEV double clog(double x) { return log(x); }
+CG_DEV float clog(float x) { return logf(x); }
+CG_DEV double ctanh(double x) { return tanh(x); }
+CG_DEV float ctanh(float x) { return tanhf(x); }
+CG_DEV double csqrt(double x) { return sqrt(x); }
+CG_DEV float csqrt(float x) { return sqrtf(x); }
+CG_DEV double cpow(double x, double y) { return pow(x, y); }
+CG_DEV float cpow(float x, float y) { return powf(x, y); }
+
+template <class T> CG_DEV T sq(T x) { return x * x; }
+template <class T> CG_DEV T cube(T x) { return x * x * x; }
+
+// Rush-Larsen update of one gate over dt: xinf + (x - xinf) exp(-dt/tau)
+// (batchedKernelExecution.H:590-596).
+template <class T> CG_DEV T rl(T x, T xinf, T tau, T dt) { return xinf + (x - xinf) * cexp(-dt / tau); }
+
+// Single-cell stimulus protocol of stimulusIO.C:238-286, evaluated in FP64 time.
+struct Stimulus {
+    double start, period_s1, duration, amplitude;
+    int nstim1;
+    double period_s2;
+    int nstim2;
+};
+
+CG_HD double stimulus_current(double t, const Stimulus& s)
+{
+    double I = 0.0;
+    if (s.period_s1 > 0.0 && s.nstim1 > 0) {
+        double tp = t - s.start;
+        if (tp >= 0 && tp <= s.period_s1 * s.nstim1) {
+            double ph = tp - floor(tp / s.period_s1) * s.period_s1;
+            if (ph >= 0 && ph <= s.duration) I = -s.amplitude;
+        }
+    }
+    if (I == 0.0 && s.period_s2 > 0.0 && s.nstim2 > 0) {
+        double t1 = s.start + s.period_s1 * s.nstim1;
+        double tp = t - t1;
+        if (tp >= 0 && tp <= s.period_s2 * s.nstim2) {
+            double ph = tp - floor(tp / s.period_s2) * s.period_s2;
+            if (ph >= 0 && ph <= s.duration) I = -s.amplitude;
+        }
+    }
+    return I;
+}
+
+} // namespace cg

@@ -1,0 +1,26 @@
+// Implicit monodomain step on the hex block (solveDiffusionImplicit,
+// monodomainSolver.C:121-150): orthogonal part of fvm::laplacian as a symmetric
+// 7-point operator, tensor correction explicit in the source, Jacobi-PCG.
+#pragma once
+#include <string>
+#include "cg_internal.hpp"
+
+namespace cg {
+
+struct ImplicitSolver;
+
+int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* h_sigma9,
+                    double chiCm, double dt, int prec, hipStream_t s, std::string* err);
+void implicit_destroy(ImplicitSolver* S);
+// one step: ionic (V^n) + right-hand side, solve for Vnew, activation update
+int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* states,
+                  const ModelParams& P, const double* Vn, const double* Vm1, double* Vnew,
+                  double* Iion, uint8_t* flag, double* tact, const ExplicitStep& st, double tol,
+                  int max_iter, Profiler* prof, std::string* err);
+void implicit_stats(const ImplicitSolver* S, int64_t* iters, double* res);
+
+// standalone pieces used by tests through the C-ABI
+hipError_t launch_hex_laplacian(const HexGeom& g, const HexCoefs& k, const double* V,
+                                const double* corr, double* out, hipStream_t s);
+
+} // namespace cg

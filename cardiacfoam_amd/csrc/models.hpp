@@ -1,0 +1,104 @@
+// Uniform device interface over the four ionic models of the tissue path.
+//   tissue_step: V held at the PDE value for the whole step (solveVmWithinODESolver
+//                false); advances the other states by dt and returns the total ionic
+//                current at the step end (model units), as the reference's solveODE
+//                does with its final computeVariables call (BuenoOrovio.C:134-152,
+//                TNNP.C:146-174, ORd.C:134-150, AlievPanfilov.C:134-152).
+//   cell_step:   single-cell step with V integrated inside (singleCellSolver.C:186-192).
+#pragma once
+#include "model_bo.hpp"
+#include "model_tnnp.hpp"
+#include "model_ord.hpp"
+
+namespace cg {
+
+enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3 };
+
+constexpr int MAX_PARAMS = 48;
+struct ModelParams {
+    double p[MAX_PARAMS];
+};
+
+template <class M> struct Ops;
+
+template <> struct Ops<BuenoOrovio> {
+    template <class T>
+    CG_DEV static T tissue_step(T u, T* y, T dt, const T* p, int, const T*)
+    {
+        y[0] = u;
+        BuenoOrovio::gates(y, dt, p);
+        return BuenoOrovio::jion(u, y, p);
+    }
+    template <class T> CG_DEV static void cell_step(T* y, T dt, T istim, const T* p, int, const T*)
+    {
+        BuenoOrovio::cell_step(y, dt, istim, p);
+    }
+    template <class T> CG_DEV static T iion(const T* y, const T* p, int, const T*)
+    {
+        return BuenoOrovio::jion(y[0], y, p);
+    }
+};
+
+template <> struct Ops<AlievPanfilov> {
+    template <class T>
+    CG_DEV static T tissue_step(T u, T* y, T dt, const T* p, int, const T*)
+    {
+        y[0] = u;
+        AlievPanfilov::gates(y, dt, p);
+        return AlievPanfilov::jion(u, y, p);
+    }
+    template <class T> CG_DEV static void cell_step(T* y, T dt, T istim, const T* p, int, const T*)
+    {
+        AlievPanfilov::cell_step(y, dt, istim, p);
+    }
+    template <class T> CG_DEV static T iion(const T* y, const T* p, int, const T*)
+    {
+        return AlievPanfilov::jion(y[0], y, p);
+    }
+};
+
+template <> struct Ops<TNNP> {
+    template <class T>
+    CG_DEV static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
+    {
+        y[0] = v;
+        const auto q = TNNP::vterms(v, p);
+        TNNP::step(y, dt, p, tissue, gs, false, T(0), q);
+        return TNNP::currents(v, y, p, q, gs).total();
+    }
+    template <class T>
+    CG_DEV static void cell_step(T* y, T dt, T istim, const T* p, int tissue, const T* gs)
+    {
+        const auto q = TNNP::vterms(y[0], p);
+        TNNP::step(y, dt, p, tissue, gs, true, istim, q);
+    }
+    template <class T> CG_DEV static T iion(const T* y, const T* p, int, const T* gs)
+    {
+        const auto q = TNNP::vterms(y[0], p);
+        return TNNP::currents(y[0], y, p, q, gs).total();
+    }
+};
+
+template <> struct Ops<ORd> {
+    template <class T>
+    CG_DEV static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
+    {
+        y[0] = v;
+        const auto q = ORd::vterms(v);
+        ORd::step(y, dt, p, tissue, gs, false, T(0), q);
+        return ORd::currents(v, y, p, q, gs).total();
+    }
+    template <class T>
+    CG_DEV static void cell_step(T* y, T dt, T istim, const T* p, int tissue, const T* gs)
+    {
+        const auto q = ORd::vterms(y[0]);
+        ORd::step(y, dt, p, tissue, gs, true, istim, q);
+    }
+    template <class T> CG_DEV static T iion(const T* y, const T* p, int, const T* gs)
+    {
+        const auto q = ORd::vterms(y[0]);
+        return ORd::currents(y[0], y, p, q, gs).total();
+    }
+};
+
+} // namespace cg

@@ -1,0 +1,157 @@
+"""Host mirror of cardiacFoam's ionicModel runtime selection (the GPU is the engine).
+
+``IonicModel.New(dict, nIntegrationPoints, initialDeltaT, solveVmWithinODESolver)``
+follows ionicModel::New (src/ionicModels/ionicModel/ionicModel.C:76-94): the model is
+looked up by the ``ionicModel`` key, the tissue by ``tissue``
+(ionicSelector.C:32-66), and the same errors are raised for unknown names.
+``solveODE(stepStartTime, deltaT, Vm, Im)`` has the meaning of ionicModel.H:250-256
+(seconds, volts, V/s) and runs on the device through the C-ABI.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+
+MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov"]
+SUPPORTED_TISSUES = {
+    "BuenoOrovio": ["endocardialCells", "mCells", "epicardialCells"],
+    "TNNP": ["endocardialCells", "mCells", "epicardialCells"],
+    "ORd": ["myocyte"],
+    "AlievPanfilov": ["myocyte"],
+}
+
+
+class FatalIOError(KeyError):
+    """Raised where the reference calls FatalIOErrorInLookup / FatalErrorInFunction."""
+
+
+def _ptr(a):
+    """Host numpy array -> (pointer, flags); torch device tensor -> device pointer."""
+    if isinstance(a, np.ndarray):
+        if a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise TypeError("fields must be C-contiguous float64 arrays")
+        return a.ctypes.data_as(C.c_void_p), 0
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        torch = None
+    if torch is not None and isinstance(a, torch.Tensor):
+        if a.dtype != torch.float64 or not a.is_contiguous():
+            raise TypeError("tensors must be contiguous float64")
+        if a.is_cuda:
+            return C.c_void_p(a.data_ptr()), _capi.DEVICE_PTRS
+        return C.c_void_p(a.data_ptr()), 0
+    raise TypeError("unsupported field type %r" % type(a))
+
+
+class IonicModel:
+    """Device-resident ionic model for nIntegrationPoints cells (SoA states)."""
+
+    def __init__(self, model_key, tissue, n, solve_vm=False, precision="fp64"):
+        L = lib()
+        self.model_key = model_key
+        self.model = L.cg_model_id(model_key.encode())
+        if self.model < 0:
+            raise FatalIOError("Unknown ionicModel type %s\nValid ionicModel types: %s"
+                               % (model_key, MODEL_KEYS))
+        if tissue not in SUPPORTED_TISSUES[model_key]:
+            raise FatalIOError("Unsupported tissue '%s' allowed: %s"
+                               % (tissue, SUPPORTED_TISSUES[model_key]))
+        self.tissue_name = tissue
+        self.tissue = L.cg_tissue_flag(tissue.encode())
+        self.n = int(n)
+        self.solve_vm = bool(solve_vm)
+        self.precision = precision
+        prec = {"fp64": _capi.FP64, "mixed": _capi.MIXED}[precision]
+        h = C.c_void_p()
+        check(L.cg_ionic_create(self.model, self.tissue, self.n, int(solve_vm), prec, C.byref(h)))
+        self.h = h
+        self.nstates = L.cg_ionic_nstates(h)
+
+    @staticmethod
+    def New(dict_, nIntegrationPoints, initialDeltaT, solveVmWithinODESolver=False,
+            precision=None):
+        """ionicModel::New.  ``precision`` may also come from the dictionary key
+        ``precision`` (fp64 | mixed), an addition of this backend."""
+        if "ionicModel" not in dict_:
+            raise FatalIOError("keyword ionicModel is undefined")
+        if "tissue" not in dict_:
+            raise FatalIOError("Normal model requires 'tissue' entry")
+        key = dict_["ionicModel"]
+        if key not in MODEL_KEYS:
+            raise FatalIOError("Unknown ionicModel type %s\nValid ionicModel types: %s"
+                               % (key, MODEL_KEYS))
+        prec = precision or dict_.get("precision", "fp64")
+        m = IonicModel(key, dict_["tissue"], nIntegrationPoints, solveVmWithinODESolver, prec)
+        m.initialDeltaT = initialDeltaT
+        stim = dict_.get("singleCellStimulus")
+        if stim:
+            m.set_stimulus_protocol(stim)
+        return m
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            try:
+                lib().cg_ionic_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    # ------------------------------------------------------------------ API
+    def solveODE(self, stepStartTime, deltaT, Vm, Im):
+        pv, fv = _ptr(Vm)
+        pi, fi = _ptr(Im)
+        if fv != fi:
+            raise TypeError("Vm and Im must both live on the host or both on the device")
+        if len(Vm) != self.n or len(Im) != self.n:
+            raise ValueError("field size %d/%d != nIntegrationPoints %d" % (len(Vm), len(Im), self.n))
+        check(lib().cg_ionic_step(self.h, float(stepStartTime), float(deltaT), pv, pi, fv))
+
+    def run_cells(self, t0, dt, n_steps):
+        check(lib().cg_ionic_run_cells(self.h, float(t0), float(dt), int(n_steps)))
+
+    def set_stimulus_protocol(self, d):
+        """singleCellStimulus sub-dictionary (stimulusIO.C:142-194)."""
+        keys = ("stim_start", "stim_period_S1", "stim_duration", "stim_amplitude")
+        if any(k in d for k in keys + ("nstim1", "stim_period_S2", "nstim2")) and \
+                not all(k in d for k in keys):
+            raise FatalIOError("Incomplete single-cell stimulus protocol. Required keys are: "
+                               "(stim_start, stim_period_S1, stim_duration, stim_amplitude).")
+        s1 = float(d.get("stim_period_S1", 0.0))
+        s = _capi.Stimulus(float(d.get("stim_start", 0.0)), s1, float(d.get("stim_duration", 0.0)),
+                           float(d.get("stim_amplitude", 0.0)),
+                           int(d.get("nstim1", 1 if s1 > 1e-15 else 0)),
+                           float(d.get("stim_period_S2", 0.0)), int(d.get("nstim2", 0)))
+        check(lib().cg_ionic_set_stimulus(self.h, C.byref(s)))
+
+    def set_constant(self, k, value):
+        check(lib().cg_ionic_set_constant(self.h, int(k), float(value)))
+
+    def set_conductance_scale(self, scale):
+        if scale is None:
+            check(lib().cg_ionic_set_conductance_scale(self.h, None))
+            return
+        a = np.ascontiguousarray(scale, dtype=np.float64).reshape(4, self.n)
+        check(lib().cg_ionic_set_conductance_scale(self.h, a.ctypes.data_as(C.c_void_p)))
+
+    def states(self, first=0, count=None):
+        count = self.nstates - first if count is None else count
+        out = np.empty((count, self.n))
+        check(lib().cg_ionic_get_states(self.h, first, count, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_states(self, values, first=0):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        a = a.reshape(-1, self.n)
+        check(lib().cg_ionic_set_states(self.h, first, a.shape[0], a.ctypes.data_as(C.c_void_p)))
+
+    def stateVariableNames(self):
+        buf = C.create_string_buffer(64)
+        names = []
+        for i in range(self.nstates):
+            check(lib().cg_model_state_name(self.model, i, buf, 64))
+            names.append(buf.value.decode())
+        return names

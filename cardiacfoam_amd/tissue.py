@@ -1,0 +1,197 @@
+"""Host mirror of MyocardiumDomain + MonodomainSolver on a uniform hex block.
+
+A ``Monodomain`` owns the device mesh, the ionic model and the tissue state and runs
+the reference's step (myocardiumDomain.C:487-529): box stimulus at t0, solveODE,
+solveDiffusionExplicit/Implicit, updateActivationTime.  Construction reads the same
+electroProperties keys as MyocardiumDomain / MonodomainSolver
+(myocardiumDomain.C:180-277, monodomainSolver.C:48-100, stimulusIO.C:305-390).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+from .foam_dict import symm_to_full
+from .ionic import IonicModel, FatalIOError
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class HexMesh:
+    """Uniform nx*ny*nz hex block, blockMesh cell order c = i + nx*(j + ny*k)."""
+
+    def __init__(self, nx, ny, nz, h):
+        hx, hy, hz = (h, h, h) if np.isscalar(h) else h
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.h = (float(hx), float(hy), float(hz))
+        self.n = self.nx * self.ny * self.nz
+        m = C.c_void_p()
+        check(lib().cg_mesh_create_hex(self.nx, self.ny, self.nz, *self.h, C.byref(m)))
+        self.m = m
+
+    def __del__(self):
+        if getattr(self, "m", None):
+            try:
+                lib().cg_mesh_destroy(self.m)
+            except Exception:
+                pass
+            self.m = None
+
+    def cell_centres(self):
+        i = np.arange(self.nx)
+        j = np.arange(self.ny)
+        k = np.arange(self.nz)
+        K, J, I = np.meshgrid(k, j, i, indexing="ij")
+        return np.stack([(I.ravel() + 0.5) * self.h[0], (J.ravel() + 0.5) * self.h[1],
+                         (K.ravel() + 0.5) * self.h[2]], axis=1)
+
+    def set_conductivity(self, sigma):
+        """sigma: 9 values (uniform tensor), 6 (symmTensor) or [n, 9]/[n, 3, 3] per cell."""
+        s = np.asarray(sigma, dtype=np.float64)
+        if s.size == 6:
+            s = np.asarray(symm_to_full(s.ravel()))
+        if s.size == 9:
+            s = np.ascontiguousarray(s.ravel())
+            check(lib().cg_mesh_set_conductivity(self.m, _dp(s), 1))
+        else:
+            s = np.ascontiguousarray(s.reshape(self.n, 9))
+            check(lib().cg_mesh_set_conductivity(self.m, _dp(s), 0))
+        self._sigma = s
+
+    def explicit_dt(self, chi, cm, max_co):
+        dt = C.c_double()
+        check(lib().cg_mesh_explicit_dt(self.m, chi, cm, max_co, C.byref(dt)))
+        return dt.value
+
+
+class Monodomain:
+    """Device monodomain tissue: MyocardiumDomain state + MonodomainSolver operator."""
+
+    def __init__(self, mesh, coeffs, dt=None, max_co=None, precision=None, tol=1e-8,
+                 max_iter=2000, ddt=None, vm0=-0.084):
+        self.mesh = mesh
+        self.coeffs = coeffs
+        if "conductivity" not in coeffs:
+            raise FatalIOError("keyword conductivity is undefined")
+        cond = coeffs["conductivity"]
+        mesh.set_conductivity(cond if not isinstance(cond, np.ndarray) else cond)
+        self.chi = float(coeffs["chi"])
+        self.cm = float(coeffs["cm"])
+        alg = coeffs.get("solutionAlgorithm", "implicit")
+        if alg not in ("explicit", "implicit"):
+            raise FatalIOError("Unrecognised solution algorithm %s" % alg)
+        self.algorithm = alg
+        if alg == "explicit" and dt is None:
+            # electroModel::setDeltaT (electroModel.C:238-257): dt from maxCo, once
+            dt = mesh.explicit_dt(self.chi, self.cm, 0.1 if max_co is None else max_co)
+        if dt is None:
+            raise ValueError("dt is required for the implicit algorithm")
+        self.dt = float(dt)
+        self.ionic = IonicModel.New(coeffs, mesh.n, self.dt, False, precision=precision)
+        if coeffs.get("singleCellStimulus"):
+            raise FatalIOError("Detected active ionic-model stimulus protocol while using a "
+                               "reaction-diffusion myocardium domain.")
+        ddt_name = ddt or coeffs.get("ddtScheme", "Euler")
+        cfg = _capi.TissueConfig(
+            _capi.EXPLICIT if alg == "explicit" else _capi.IMPLICIT,
+            _capi.DDT_BACKWARD if ddt_name == "backward" else _capi.DDT_EULER,
+            self.chi, self.cm, self.dt, vm0, tol, max_iter)
+        t = C.c_void_p()
+        check(lib().cg_tissue_create(mesh.m, self.ionic.h, C.byref(cfg), C.byref(t)))
+        self.t = t
+        stim = coeffs.get("externalStimulus")
+        if stim:
+            self.set_external_stimulus(stim)
+
+    def __del__(self):
+        if getattr(self, "t", None):
+            try:
+                lib().cg_tissue_destroy(self.t)
+            except Exception:
+                pass
+            self.t = None
+
+    def set_external_stimulus(self, d):
+        """externalStimulus sub-dictionary (stimulusIO.C:305-390)."""
+        if "stimulusLocationMinList" in d:
+            lo = np.asarray(d["stimulusLocationMinList"], dtype=np.float64).reshape(-1, 3)
+            hi = np.asarray(d["stimulusLocationMaxList"], dtype=np.float64).reshape(-1, 3)
+        else:
+            lo = np.asarray([d["stimulusLocationMin"]], dtype=np.float64)
+            hi = np.asarray([d["stimulusLocationMax"]], dtype=np.float64)
+        nb = len(lo)
+        t0 = np.asarray(d.get("stimulusStartTimeList", [d.get("stimulusStartTime", 0.0)] * nb),
+                        dtype=np.float64)
+        dur = np.asarray(d.get("stimulusDurationList", [d.get("stimulusDuration", 0.0)] * nb),
+                         dtype=np.float64)
+        inten = np.asarray(d.get("stimulusIntensityList", [d.get("stimulusIntensity", 0.0)] * nb),
+                           dtype=np.float64)
+        for name, a in (("stimulusStartTimeList", t0), ("stimulusDurationList", dur),
+                        ("stimulusIntensityList", inten)):
+            if len(a) != nb:
+                raise FatalIOError("%s must have the same size as the stimulus box list." % name)
+        self.set_boxes(lo, hi, t0, dur, inten)
+
+    def set_boxes(self, lo, hi, t0, dur, inten):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (lo, hi, t0, dur, inten)]
+        check(lib().cg_tissue_set_stimulus_boxes(self.t, len(arrs[2]), *[_dp(a) for a in arrs]))
+        self._boxes = arrs
+
+    def step(self, n=1):
+        check(lib().cg_tissue_step(self.t, int(n)))
+
+    def sync(self):
+        check(lib().cg_tissue_sync(self.t))
+
+    def field(self, which):
+        out = np.empty(self.mesh.n)
+        check(lib().cg_tissue_get_field(self.t, which, _dp(out)))
+        return out
+
+    def set_field(self, which, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().cg_tissue_set_field(self.t, which, _dp(a)))
+
+    @property
+    def Vm(self):
+        return self.field(_capi.FIELD_VM)
+
+    @property
+    def Iion(self):
+        return self.field(_capi.FIELD_IION)
+
+    @property
+    def activationTime(self):
+        return self.field(_capi.FIELD_ACTIVATION)
+
+    @property
+    def time(self):
+        t = C.c_double()
+        s = C.c_int64()
+        check(lib().cg_tissue_time(self.t, C.byref(t), C.byref(s)))
+        return t.value, s.value
+
+    def set_profiling(self, on=True):
+        check(lib().cg_tissue_set_profiling(self.t, int(on)))
+
+    def kernel_times(self):
+        """{kernel: (total ms, launches)} of the HIP-event timer."""
+        names = C.create_string_buffer(32 * 16)
+        ms = (C.c_double * 16)()
+        cnt = (C.c_int64 * 16)()
+        n = C.c_int()
+        check(lib().cg_tissue_kernel_times(self.t, 16, names, ms, cnt, C.byref(n)))
+        out = {}
+        for k in range(n.value):
+            nm = names.raw[32 * k:32 * (k + 1)].split(b"\0")[0].decode()
+            out[nm] = (ms[k], cnt[k])
+        return out
+
+    def solver_stats(self):
+        it = C.c_int64()
+        r = C.c_double()
+        check(lib().cg_tissue_solver_stats(self.t, C.byref(it), C.byref(r)))
+        return it.value, r.value

@@ -1,0 +1,198 @@
+/*
+ * cardiacgpu.h -- C-ABI of the MI355X (gfx950) monodomain tissue step.
+ *
+ * Drop-in boundary for cardiacFoam's MonoDomainSolver hot path.  An OpenFOAM-side
+ * adapter (an ionicModel subclass and a myocardiumSolver subclass registered under the
+ * existing runtime-selection keys) forwards to these functions; Python reaches the same
+ * symbols through ctypes.  Only plain pointers, sizes and PODs cross the boundary.
+ *
+ * Units at the boundary follow the reference (src/electroModels/core/dimVoltage.H,
+ * myocardiumDomain.C:196-251): time s, Vm V, Im/Iion V/s, sigma S/m, chi 1/m,
+ * Cm F/m^2, stimulus A/m^3.  Model-internal units (ms, mV) never cross it.
+ *
+ * Every entry returns CG_OK (0) or an error code; cg_last_error() gives a thread-local
+ * message the adapter turns into FatalErrorInFunction.  Each handle owns its device
+ * memory and one HIP stream; calls on one handle must not overlap.
+ */
+#ifndef CARDIACGPU_H
+#define CARDIACGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ codes */
+enum {
+    CG_OK = 0,
+    CG_ERR_ARG = 1,        /* invalid argument / size */
+    CG_ERR_MODEL = 2,      /* unknown ionicModel (ionicModel.C:83-88) */
+    CG_ERR_TISSUE = 3,     /* unsupported tissue (ionicSelector.C:32-52) */
+    CG_ERR_HIP = 4,        /* HIP runtime failure */
+    CG_ERR_NOMEM = 5,      /* device allocation failed */
+    CG_ERR_NONFINITE = 6,  /* NaN/Inf detected in a checked output */
+    CG_ERR_NOT_CONVERGED = 7, /* linear solver hit max iterations */
+    CG_ERR_STIMULUS = 8,   /* malformed stimulus (stimulusIO.C:123-177) */
+    CG_ERR_STATE = 9       /* call not valid in the handle's configuration */
+};
+
+/* model ids: the ionicModel runtime-selection keys */
+enum { CG_MODEL_BUENO_OROVIO = 0, CG_MODEL_TNNP = 1, CG_MODEL_ORD = 2, CG_MODEL_ALIEV_PANFILOV = 3 };
+/* tissue flags of ionicSelector::tissueFlag (ionicSelector.C:60-66) */
+enum { CG_TISSUE_EPI = 1, CG_TISSUE_MCELL = 2, CG_TISSUE_ENDO = 3, CG_TISSUE_MYOCYTE = 4 };
+/* arithmetic: CG_FP64 = plain FP64 path; CG_MIXED = FP32 states/arithmetic, FP64 time,
+ * reductions, residuals and activation times */
+enum { CG_FP64 = 0, CG_MIXED = 1 };
+/* flags */
+enum { CG_DEVICE_PTRS = 1 };
+
+const char* cg_last_error(void);
+/* library version string */
+const char* cg_version(void);
+
+/* ------------------------------------------------------------- ionic model */
+typedef struct cg_ionic cg_ionic;
+
+/* Model lookup by ionicModel key ("BuenoOrovio", "TNNP", "ORd", "AlievPanfilov") and
+ * tissue word ("epicardialCells", "mCells", "endocardialCells", "myocyte").
+ * Replaces ionicModel::New's table lookup (ionicModel.C:76-94) and
+ * ionicSelector::tissueFlag (ionicSelector.C:60-66).  Return -1 when unknown. */
+int cg_model_id(const char* ionic_model_key);
+int cg_tissue_flag(const char* tissue_word);
+/* number of states of a model (NUM_STATES of the generated headers), or -1 */
+int cg_model_nstates(int model);
+/* name of state i (the *_STATES_NAMES tables, e.g. BuenoOrovio_2008.H:24-30) */
+int cg_model_state_name(int model, int i, char* buf, size_t len);
+
+/* Replaces <Model>::<Model>(dict, nIntegrationPoints, initialDeltaT, solveVm)
+ * (ionicModel.H:184-196; BuenoOrovio.C:45-81, TNNP.C:44-78, ORd.C:45-80,
+ * AlievPanfilov.C:45-80): allocates SoA states for n_cells integration points
+ * initialised from the model's initConsts.  solve_vm = 1 is the single-cell mode
+ * (singleCellSolver.C:48-57). */
+int cg_ionic_create(int model, int tissue, int64_t n_cells, int solve_vm, int precision,
+                    cg_ionic** out);
+int cg_ionic_destroy(cg_ionic* h);
+int cg_ionic_nstates(const cg_ionic* h);
+
+/* Override model constant k (index in the constant order of the model's Names.H file)
+ * for all cells.  Replaces editing CONSTANTS_ after initConsts. */
+int cg_ionic_set_constant(cg_ionic* h, int k, double value);
+/* Per-cell multipliers of four conductances, SoA [4][n_cells] (NULL clears):
+ * TNNP g_Na g_Kr g_CaL g_Ks; ORd GNa GKr PCa GKs.  No reference counterpart
+ * (constants are per model instance there, TNNP.H:61). */
+int cg_ionic_set_conductance_scale(cg_ionic* h, const double* scale4xn);
+
+/* Single-cell stimulus protocol (stimulusIO.C:142-194 keys, ms and model units).
+ * Rejected for tissue handles, as myocardiumDomain.C:377-394 does. */
+typedef struct {
+    double stim_start, stim_period_s1, stim_duration, stim_amplitude;
+    int nstim1;
+    double stim_period_s2;
+    int nstim2;
+} cg_stimulus;
+int cg_ionic_set_stimulus(cg_ionic* h, const cg_stimulus* s);
+
+/* Replaces ionicModel::solveODE(stepStartTime, deltaT, Vm, Im) (ionicModel.H:250-256)
+ * for a tissue handle: Vm [n] in V, Im [n] out in V/s.  Pointers are host memory unless
+ * flags has CG_DEVICE_PTRS.  Synchronous. */
+int cg_ionic_step(cg_ionic* h, double t0_s, double dt_s, const double* Vm, double* Im,
+                  int flags);
+
+/* Single-cell handles: n_steps Rush-Larsen steps of dt_s from t0_s with Vm inside the
+ * ODE (SingleCellSolver::evolve, singleCellSolver.C:179-229), all cells in one launch
+ * with the states kept in registers.  Asynchronous; cg_ionic_get_states synchronises. */
+int cg_ionic_run_cells(cg_ionic* h, double t0_s, double dt_s, int n_steps);
+
+/* States as host SoA doubles [n_vars][n_cells], starting at state first_var.
+ * Replaces the io*Ptr hooks / exportStates (ionicModel.H:115-168, 584-638). */
+int cg_ionic_get_states(cg_ionic* h, int first_var, int n_vars, double* host_soa);
+int cg_ionic_set_states(cg_ionic* h, int first_var, int n_vars, const double* host_soa);
+
+/* -------------------------------------------------------------- hex mesh */
+typedef struct cg_mesh cg_mesh;
+
+/* Uniform nx*ny*nz hex block in blockMesh cell order (c = i + nx*(j + ny*k)), spacing
+ * h (m).  Replaces the fvMesh the myocardiumSolver receives (myocardiumSolver.H:34-106)
+ * for single-block hex meshes. */
+int cg_mesh_create_hex(int nx, int ny, int nz, double hx, double hy, double hz,
+                       cg_mesh** out);
+/* Checks that an lduAddressing (owner/neighbour, upper-triangular order) is the
+ * blockMesh ordering of an nx*ny*nz block; 0 if it is. */
+int cg_mesh_match_ldu(int nx, int ny, int nz, int64_t n_faces, const int32_t* owner,
+                      const int32_t* neighbour);
+int cg_mesh_destroy(cg_mesh* m);
+int64_t cg_mesh_ncells(const cg_mesh* m);
+
+/* Conductivity, row-major 3x3 per cell [9*n] (uniform == 1: one tensor for all cells).
+ * Replaces MonodomainSolver::initialiseConductivity (monodomainSolver.C:48-100). */
+int cg_mesh_set_conductivity(cg_mesh* m, const double* sigma9, int uniform);
+/* maxCo * min_f dx^2/(n.sigma_f.n/(chi Cm)) (myocardiumDomain.C:532-550) */
+int cg_mesh_explicit_dt(const cg_mesh* m, double chi, double cm, double max_co, double* dt);
+
+/* -------------------------------------------------------- tissue (fused step) */
+typedef struct cg_tissue cg_tissue;
+
+enum { CG_EXPLICIT = 0, CG_IMPLICIT = 1 };      /* solutionAlgorithm */
+enum { CG_DDT_EULER = 0, CG_DDT_BACKWARD = 1 }; /* ddtSchemes */
+
+typedef struct {
+    int algorithm;     /* CG_EXPLICIT | CG_IMPLICIT */
+    int ddt;           /* CG_DDT_EULER | CG_DDT_BACKWARD */
+    double chi;        /* 1/m */
+    double cm;         /* F/m^2 */
+    double dt;         /* s */
+    double vm0;        /* initial Vm (V), -0.084 in myocardiumDomain.C:207 */
+    double tol;        /* implicit: relative L2 residual tolerance */
+    int max_iter;      /* implicit: max inner iterations per solve */
+} cg_tissue_config;
+
+/* MyocardiumDomain + MonodomainSolver state on the device: Vm (and old levels),
+ * Iion, activation time and flag, the ionic handle's states.  The ionic handle must be
+ * a tissue handle with n_cells == mesh cells; it stays owned by the caller. */
+int cg_tissue_create(cg_mesh* m, cg_ionic* ionic, const cg_tissue_config* cfg,
+                     cg_tissue** out);
+int cg_tissue_destroy(cg_tissue* t);
+
+/* externalStimulus boxes (stimulusIO.C:305-390): [3n] min/max corners (m), start (s),
+ * duration (s), intensity (A/m^3).  At most 8 boxes. */
+int cg_tissue_set_stimulus_boxes(cg_tissue* t, int n, const double* box_min,
+                                 const double* box_max, const double* t_start,
+                                 const double* duration, const double* intensity);
+
+/* n_steps of the reference's step: t += dt; t0 = t - dt; prepareTimeStep(t0) (box
+ * stimulus); advance(t0, dt): solveODE -> solveDiffusion{Explicit,Implicit} ->
+ * updateActivationTime (myocardiumDomain.C:487-529).  Asynchronous on the handle's
+ * stream; cg_tissue_sync waits. */
+int cg_tissue_step(cg_tissue* t, int n_steps);
+int cg_tissue_sync(cg_tissue* t);
+
+enum { CG_FIELD_VM = 0, CG_FIELD_IION = 1, CG_FIELD_ACTIVATION = 2, CG_FIELD_VM_OLD = 3 };
+/* copies a field [n] to host doubles (synchronous) */
+int cg_tissue_get_field(cg_tissue* t, int which, double* host);
+int cg_tissue_set_field(cg_tissue* t, int which, const double* host);
+/* accumulated time (s) and step count */
+int cg_tissue_time(const cg_tissue* t, double* time_s, int64_t* steps);
+/* implicit path: total inner iterations and last relative residual */
+int cg_tissue_solver_stats(const cg_tissue* t, int64_t* iterations, double* last_residual);
+
+/* Per-kernel HIP-event timing on the tissue's stream (the equivalent of the clockTime
+ * phases staggeredElectrophysicsAdvanceScheme.C:54-74 stores in
+ * electrophysicsAdvanceTimings).  Enabling resets the totals. */
+int cg_tissue_set_profiling(cg_tissue* t, int on);
+/* Up to max kernels: names (32 chars each), summed milliseconds and launch counts;
+ * launches that exit early on a converged solver are not counted.  Synchronises. */
+int cg_tissue_kernel_times(cg_tissue* t, int max, char* names, double* total_ms,
+                           int64_t* launches, int* n_out);
+
+/* ------------------------------------------------------- standalone operators */
+/* Explicit FV Laplacian div(sigma grad V) per unit volume (A/m^3) of the "Gauss linear
+ * corrected" scheme with the mesh's conductivity (fvc::laplacian, monodomainSolver.C:111);
+ * V and out are device pointers of n doubles. */
+int cg_mesh_laplacian(cg_mesh* m, const double* V_dev, double* out_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

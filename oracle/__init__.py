@@ -283,7 +283,8 @@ class OracleTissue:
         lib().orc_tissue_step(C.byref(self.T), C.byref(boxes) if boxes is not None else None)
 
     def _arr(self, ptr, n):
-        return np.ctypeslib.as_array(ptr, shape=(n,))
+        # copies: the C buffers are freed with the tissue object
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
 
     @property
     def V(self):
@@ -299,7 +300,7 @@ class OracleTissue:
 
     @property
     def flag(self):
-        return np.ctypeslib.as_array(self.T.flag, shape=(self.n,))
+        return np.ctypeslib.as_array(self.T.flag, shape=(self.n,)).copy()
 
     @property
     def states(self):

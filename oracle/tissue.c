@@ -16,6 +16,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include "oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 int orc_mesh_build(orc_mesh_t* m, int nx, int ny, int nz, double hx, double hy, double hz)
 {
@@ -170,16 +173,51 @@ void orc_ldu_amul(const orc_mesh_t* m, const double* diag, const double* upper,
     }
 }
 
+/* the hex block's faces in cell-gather form: for cell c, the index of its face to
+ * +x/+y/+z (or -1) in the owner-ordered face list, so y = A x runs in parallel */
+static void hex_face_index(const orc_mesh_t* m, long* fidx)
+{
+    for (long c = 0; c < 3 * m->n_cells; ++c) fidx[c] = -1;
+    for (long f = 0; f < m->n_faces; ++f) fidx[3 * (long)m->owner[f] + m->face_dir[f]] = f;
+}
+
+static void hex_amul(const orc_mesh_t* m, const long* fidx, const double* diag,
+                     const double* upper, const double* x, double* y, int nthreads)
+{
+    const long st[3] = {1, m->nx, (long)m->nx * m->ny};
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (long c = 0; c < m->n_cells; ++c) {
+        double acc = diag[c] * x[c];
+        for (int d = 0; d < 3; ++d) {
+            long f = fidx[3 * c + d];
+            if (f >= 0) acc += upper[f] * x[c + st[d]];
+            long b = c - st[d];
+            if (b >= 0) {
+                long g = fidx[3 * b + d];
+                if (g >= 0 && m->neighbour[g] == c) acc += upper[g] * x[b];
+            }
+        }
+        y[c] = acc;
+    }
+}
+
 int orc_pcg(const orc_mesh_t* m, const double* diag, const double* upper, const double* b,
             double* x, double tol, int max_iter, double* res)
 {
     const long n = m->n_cells;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
     double* r = malloc(sizeof(double) * n);
     double* z = malloc(sizeof(double) * n);
     double* p = malloc(sizeof(double) * n);
     double* q = malloc(sizeof(double) * n);
-    orc_ldu_amul(m, diag, upper, x, q);
+    long* fidx = malloc(sizeof(long) * 3 * n);
+    hex_face_index(m, fidx);
+    hex_amul(m, fidx, diag, upper, x, q, nt);
     double bn = 0.0, rr = 0.0, rz = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(+ : bn, rr, rz)
     for (long i = 0; i < n; ++i) {
         r[i] = b[i] - q[i];
         z[i] = r[i] / diag[i];
@@ -190,12 +228,14 @@ int orc_pcg(const orc_mesh_t* m, const double* diag, const double* upper, const 
     if (bn == 0.0) bn = 1.0;
     int it = 0;
     while (sqrt(rr) > tol * bn && it < max_iter) {
-        orc_ldu_amul(m, diag, upper, p, q);
+        hex_amul(m, fidx, diag, upper, p, q, nt);
         double pq = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(+ : pq)
         for (long i = 0; i < n; ++i) pq += p[i] * q[i];
-        double alpha = rz / pq;
+        const double alpha = rz / pq;
         double rz_new = 0.0;
         rr = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(+ : rz_new, rr)
         for (long i = 0; i < n; ++i) {
             x[i] += alpha * p[i];
             r[i] -= alpha * q[i];
@@ -203,17 +243,19 @@ int orc_pcg(const orc_mesh_t* m, const double* diag, const double* upper, const 
             rz_new += r[i] * z[i];
             rr += r[i] * r[i];
         }
-        double beta = rz_new / rz;
+        const double beta = rz_new / rz;
         rz = rz_new;
+#pragma omp parallel for schedule(static) num_threads(nt)
         for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
         ++it;
     }
     /* true residual */
-    orc_ldu_amul(m, diag, upper, x, q);
+    hex_amul(m, fidx, diag, upper, x, q, nt);
     rr = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(+ : rr)
     for (long i = 0; i < n; ++i) rr += (b[i] - q[i]) * (b[i] - q[i]);
     if (res) *res = sqrt(rr) / bn;
-    free(r); free(z); free(p); free(q);
+    free(r); free(z); free(p); free(q); free(fidx);
     return it;
 }
 
@@ -322,6 +364,7 @@ int orc_tissue_step(orc_tissue_t* T, const orc_boxes_t* boxes)
     double* lap = T->work;
     if (!k->implicit) {
         orc_laplacian(&T->mesh, T->sigma, T->V0, lap, k->nthreads);
+#pragma omp parallel for schedule(static) num_threads(k->nthreads > 0 ? k->nthreads : 1)
         for (long c = 0; c < n; ++c)
             T->V[c] = (coefft0 * T->V0[c] - coefft00 * T->V00[c] +
                        dt / chiCm * (lap[c] - chiCm * T->Iion[c] + T->src[c])) / coefft;

@@ -1,0 +1,55 @@
+"""Seeded inputs shared by the GPU parity tests and the smoke check."""
+import numpy as np
+
+MODELS = {  # name -> (oracle id, tissue flag, library key, tissue word, stimulus amplitude)
+    "BO_epi": (0, 1, "BuenoOrovio", "epicardialCells", 0.4),
+    "BO_endo": (0, 3, "BuenoOrovio", "endocardialCells", 0.4),
+    "TNNP_epi": (1, 1, "TNNP", "epicardialCells", 52.0),
+    "TNNP_M": (1, 2, "TNNP", "mCells", 52.0),
+    "TNNP_endo": (1, 3, "TNNP", "endocardialCells", 52.0),
+    "ORd": (2, 4, "ORd", "myocyte", 80.0),
+    "AP": (3, 4, "AlievPanfilov", "myocyte", 0.5),
+}
+
+
+def physiological_states(orc, name, n, seed=0, dt_ms=0.01, span_ms=400.0):
+    """n cell states sampled along one paced action potential of the oracle, plus
+    Vm (V) per cell with small seeded jitter."""
+    m, tis, _, _, amp = MODELS[name]
+    c, y = orc.model_init(m, tis)
+    st = orc.make_stim(start=2.0, period_s1=1000.0, duration=1.0, amplitude=amp, nstim1=1)
+    tscale = 1.0 / 12.9 if m == 3 else 1.0
+    nsamp = 64
+    every = max(1, int(span_ms / dt_ms / nsamp))
+    samples = []
+    yy = np.zeros(41)
+    yy[:len(y)] = y
+    for k in range(nsamp * every):
+        orc.lib().orc_rl_step(m, tis, k * dt_ms * tscale, dt_ms * tscale, c, yy, 1, st)
+        if k % every == 0:
+            samples.append(yy[:len(y)].copy())
+    S = np.array(samples)
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, len(S), n)
+    states = S[idx].T.copy()                    # [ns, n]
+    vm_model = states[0]
+    if m == 0:
+        vm_mv = vm_model * 85.7 - 84.0
+    elif m == 3:
+        vm_mv = vm_model * 100.0 - 80.0
+    else:
+        vm_mv = vm_model
+    Vm = (vm_mv + rng.normal(0.0, 0.5, n)) * 1e-3
+    return c, np.ascontiguousarray(states), Vm
+
+
+def rotating_fibre_sigma(nx, ny, nz, sl=0.17, st=0.019, deg=60.0):
+    """Per-cell tensor sigma_t I + (sigma_l - sigma_t) f f^T with f = (cos t, sin t, 0),
+    t rising linearly from -deg to +deg through z."""
+    k = np.arange(nz)
+    theta = np.deg2rad(-deg + 2 * deg * (k + 0.5) / nz)
+    sig = np.zeros((nz, ny, nx, 9))
+    for kk, t in enumerate(theta):
+        f = np.array([np.cos(t), np.sin(t), 0.0])
+        sig[kk, :, :, :] = (st * np.eye(3) + (sl - st) * np.outer(f, f)).ravel()
+    return sig.reshape(-1, 9)

@@ -1,0 +1,58 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function the header
+declares, and its pure-host lookups follow the reference's selection tables
+(ionicModel.C:76-94, ionicSelector.C:60-66, BuenoOrovio_2008.H:24-30)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cardiacfoam_amd import _capi
+
+
+def test_library_exports_header_symbols():
+    assert _capi.missing_symbols() == []
+    assert b"gfx950" in _capi.lib().cg_version()
+
+
+def test_model_and_tissue_lookup():
+    L = _capi.lib()
+    assert [L.cg_model_id(k) for k in (b"BuenoOrovio", b"TNNP", b"ORd", b"AlievPanfilov")] == [0, 1, 2, 3]
+    assert L.cg_model_id(b"Courtemanche") == -1
+    assert [L.cg_tissue_flag(t) for t in (b"epicardialCells", b"mCells", b"endocardialCells",
+                                          b"myocyte")] == [1, 2, 3, 4]
+    assert L.cg_tissue_flag(b"purkinje") == -1
+    assert [L.cg_model_nstates(m) for m in range(4)] == [4, 17, 41, 2]
+    buf = C.create_string_buffer(32)
+    names = []
+    for i in range(4):
+        _capi.check(L.cg_model_state_name(0, i, buf, 32))
+        names.append(buf.value)
+    assert names == [b"u", b"v", b"w", b"s"]
+    _capi.check(L.cg_model_state_name(2, 40, buf, 32))
+    assert buf.value == b"ryr_Jrelp"
+    with pytest.raises(_capi.CgError):
+        _capi.check(L.cg_model_state_name(1, 17, buf, 32))
+
+
+def test_mesh_ldu_matching(orc):
+    L = _capi.lib()
+    m = orc.OracleMesh(5, 4, 3, 1e-4, 1e-4, 1e-4)
+    own = np.ascontiguousarray(m.owner, dtype=np.int32)
+    nei = np.ascontiguousarray(m.neighbour, dtype=np.int32)
+    _capi.check(L.cg_mesh_match_ldu(5, 4, 3, len(own), own.ctypes.data_as(C.c_void_p),
+                                    nei.ctypes.data_as(C.c_void_p)))
+    nei2 = nei.copy()
+    nei2[[3, 4]] = nei2[[4, 3]]
+    with pytest.raises(_capi.CgError):
+        _capi.check(L.cg_mesh_match_ldu(5, 4, 3, len(own), own.ctypes.data_as(C.c_void_p),
+                                        nei2.ctypes.data_as(C.c_void_p)))
+
+
+def test_ionic_new_errors_without_device():
+    import cardiacfoam_amd as cf
+    with pytest.raises(cf.FatalIOError):
+        cf.IonicModel.New({"ionicModel": "Courtemanche", "tissue": "myocyte"}, 10, 1e-5)
+    with pytest.raises(cf.FatalIOError):
+        cf.IonicModel.New({"ionicModel": "ORd", "tissue": "epicardialCells"}, 10, 1e-5)
+    with pytest.raises(cf.FatalIOError):
+        cf.IonicModel.New({"ionicModel": "TNNP"}, 10, 1e-5)

@@ -1,0 +1,156 @@
+"""GPU monodomain tissue step (cg_tissue_step) against the oracle's FP64 tissue run on
+the same seeded inputs: plain FP64 path Vm L-inf <= 1e-9 relative and identical
+activation times; mixed path Vm L-inf <= 0.5 mV and activation <= 0.05 ms."""
+import sys
+
+import numpy as np
+import pytest
+
+from tests.golden import cases
+from tests.gpu_cases import MODELS, rotating_fibre_sigma
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _coeffs(key, word, alg, sigma=None, box=None):
+    d = {"ionicModel": key, "tissue": word, "chi": 1400.0 * 100.0, "cm": 0.01,
+         "solutionAlgorithm": alg,
+         "conductivity": sigma if sigma is not None else [0.17, 0, 0, 0, 0.019, 0, 0, 0, 0.019]}
+    if box:
+        d["externalStimulus"] = box
+    return d
+
+
+BOX = {"stimulusLocationMin": [0.0, 0.0, 0.0], "stimulusLocationMax": [6e-4, 6e-4, 6e-4],
+       "stimulusDuration": 2e-3, "stimulusIntensity": 5e4, "stimulusStartTime": 0.0}
+
+
+def _oracle_run(orc, shape, h, sigma, model, tissue, dt, n_steps, ddt, implicit, box,
+                pcg_tol=1e-13):
+    nx, ny, nz = shape
+    T = orc.OracleTissue(nx, ny, nz, h, sigma, model, tissue, dt, 1.4e5, 0.01, integrator=0,
+                         ddt=ddt, implicit=implicit, pcg_tol=pcg_tol, nthreads=8)
+    bx = orc.make_boxes(box["stimulusLocationMin"], box["stimulusLocationMax"],
+                        [box["stimulusStartTime"]], [box["stimulusDuration"]],
+                        [box["stimulusIntensity"]])
+    for _ in range(n_steps):
+        T.step(bx)
+    return T
+
+
+def _compare(gpu_dom, T, vtol_rel=1e-9, ttol=1e-12):
+    V, Vr = gpu_dom.Vm, T.V
+    err = np.max(np.abs(V - Vr)) / np.max(np.abs(Vr))
+    ta, tr = gpu_dom.activationTime, T.tact
+    act = tr > 0
+    _say("  Vm rel err %.3e, activated %d/%d, max dt_act %.3e" %
+         (err, act.sum(), len(tr), np.max(np.abs(ta - tr)) if len(tr) else 0))
+    assert err <= vtol_rel
+    assert np.array_equal(ta > 0, act)
+    assert np.max(np.abs(ta - tr)) <= ttol
+    return err
+
+
+@pytest.mark.parametrize("ddt", ["Euler", "backward"])
+@pytest.mark.parametrize("name", ["BO_epi", "TNNP_epi", "ORd", "AP"])
+def test_explicit_parity_fp64(gpu, orc, name, ddt):
+    _say("explicit fp64", name, ddt)
+    m, tis, key, word, _ = MODELS[name]
+    shape, h = (24, 12, 8), 1.5e-4
+    dt = 5e-6 if m in (0, 3) else 1e-5
+    n_steps = 1600 if m in (0, 3) else 800
+    mesh = gpu.HexMesh(*shape, h)
+    dom = gpu.Monodomain(mesh, _coeffs(key, word, "explicit", box=BOX), dt=dt, ddt=ddt)
+    dom.step(n_steps)
+    T = _oracle_run(orc, shape, h, np.array([0.17, 0, 0, 0, 0.019, 0, 0, 0, 0.019]), m, tis,
+                    dt, n_steps, 1 if ddt == "backward" else 0, 0, BOX)
+    assert abs(dom.time[0] - T.t) == 0.0
+    _compare(dom, T)
+
+
+def test_niederer_case_gpu(gpu, orc):
+    """The reference's tissue regression case on the GPU path (RL, backward ddt):
+    same probes as the oracle and the golden's tolerances (probe 4 as in the oracle)."""
+    _say("niederer gpu")
+    c = cases.NIEDERER
+    nx, ny, nz = c["shape"]
+    mesh = gpu.HexMesh(nx, ny, nz, c["h"])
+    b = c["box"]
+    coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": c["chi"],
+              "cm": c["cm"], "solutionAlgorithm": "explicit", "conductivity": c["sigma9"],
+              "externalStimulus": {"stimulusLocationMin": b["min"], "stimulusLocationMax": b["max"],
+                                   "stimulusDuration": b["duration"],
+                                   "stimulusIntensity": b["intensity"], "stimulusStartTime": 0.0}}
+    dom = gpu.Monodomain(mesh, coeffs, max_co=c["max_co"], ddt="backward")
+    assert abs(dom.dt - 4.1973434541077816e-05) < 1e-15
+    steps = 0
+    while steps * dom.dt < c["end_time"] - 0.5 * dom.dt:
+        steps += 1
+    dom.step(steps)
+    ta = dom.activationTime.reshape(nz, ny, nx)
+    for p in c["probes"]:
+        i, j, k = p["cell"]
+        assert abs(ta[k, j, i] - p["expected"]) <= p.get("tol_here", p["tol"]), p
+
+
+@pytest.mark.parametrize("name", ["BO_epi", "ORd"])
+def test_implicit_parity_fp64_rotating_fibres(gpu, orc, name):
+    _say("implicit fp64", name)
+    m, tis, key, word, _ = MODELS[name]
+    shape, h = (20, 14, 10), 2e-4
+    sig = rotating_fibre_sigma(*shape)
+    dt, n_steps = 2e-5, 300
+    mesh = gpu.HexMesh(*shape, h)
+    dom = gpu.Monodomain(mesh, _coeffs(key, word, "implicit", sigma=sig, box=BOX), dt=dt,
+                         tol=1e-13, max_iter=5000)
+    dom.step(n_steps)
+    T = _oracle_run(orc, shape, h, sig, m, tis, dt, n_steps, 0, 1, BOX, pcg_tol=1e-13)
+    _compare(dom, T, vtol_rel=1e-9, ttol=1e-10)
+    iters, res = dom.solver_stats()
+    _say("  iterations", iters, "last residual", res)
+    assert res <= 1e-13
+
+
+@pytest.mark.parametrize("name", ["BO_epi", "ORd"])
+def test_implicit_mixed_vs_fp64(gpu, orc, name):
+    _say("implicit mixed", name)
+    m, tis, key, word, _ = MODELS[name]
+    shape, h = (20, 14, 10), 2e-4
+    sig = rotating_fibre_sigma(*shape)
+    dt, n_steps = 2e-5, 400
+    doms = {}
+    for prec in ("fp64", "mixed"):
+        mesh = gpu.HexMesh(*shape, h)
+        dom = gpu.Monodomain(mesh, _coeffs(key, word, "implicit", sigma=sig, box=BOX), dt=dt,
+                             precision=prec, tol=1e-8, max_iter=5000)
+        dom.step(n_steps)
+        doms[prec] = (mesh, dom)
+    V64, V32 = doms["fp64"][1].Vm, doms["mixed"][1].Vm
+    t64, t32 = doms["fp64"][1].activationTime, doms["mixed"][1].activationTime
+    dv = np.max(np.abs(V64 - V32)) * 1e3
+    both = (t64 > 0) & (t32 > 0)
+    dact = np.max(np.abs(t64[both] - t32[both])) * 1e3 if both.any() else 0.0
+    _say("  Vm Linf %.4f mV, activation Linf %.5f ms" % (dv, dact))
+    assert dv <= 0.5
+    assert dact <= 0.05
+    assert np.array_equal(t64 > 0, t32 > 0)
+
+
+def test_explicit_mixed_vs_fp64(gpu, orc):
+    _say("explicit mixed BO")
+    shape, h = (24, 12, 8), 1.5e-4
+    doms = {}
+    for prec in ("fp64", "mixed"):
+        mesh = gpu.HexMesh(*shape, h)
+        dom = gpu.Monodomain(mesh, _coeffs("BuenoOrovio", "epicardialCells", "explicit", box=BOX),
+                             dt=5e-6, precision=prec)
+        dom.step(2000)
+        doms[prec] = (mesh, dom)
+    dv = np.max(np.abs(doms["fp64"][1].Vm - doms["mixed"][1].Vm)) * 1e3
+    _say("  Vm Linf %.4f mV" % dv)
+    assert dv <= 0.5

@@ -1,0 +1,60 @@
+"""The oracle against the reference's own golden vectors.
+
+- tutorials/regressionTests/singleCell/singleCell.reference:2-6 (BuenoOrovio endo,
+  S1 1000 ms, RKF45 absTol 1e-6 relTol 1e-4, dt 5e-5 s; Vm in mV, tol 5e-3)
+- tutorials/regressionTests/NiedererEtAl2012/NiedererEtAl2012.reference:9-14 and the
+  passing log log.Alltest-regression:6-11 (activation times at probes, BO epi,
+  100x15x35 hex of 0.2 mm, explicit, backward ddt, dt from maxCo 0.1).
+"""
+import numpy as np
+import pytest
+
+from tests.golden import cases
+
+
+def test_single_cell_golden_rkf45(orc):
+    s = cases.SINGLE_CELL
+    st = orc.make_stim(**s["stimulus"])
+    t, v, _ = orc.single_cell_run(orc.BO, orc.ENDO, st, s["dt"], s["n_steps"], integrator=1,
+                                  abs_tol=s["abs_tol"], rel_tol=s["rel_tol"])
+    for tt, expected, tol in s["reference"]:
+        i = int(np.argmin(np.abs(t - tt)))
+        assert abs(t[i] - tt) <= 1e-9
+        assert abs(v[i] - expected) <= tol, (tt, v[i], expected)
+
+
+def test_single_cell_golden_rush_larsen_close(orc):
+    """The plain-path integrator (RL gates + forward-Euler u) at the same dt stays
+    within a few mV of the golden trace (first-order in dt during the upstroke)."""
+    s = cases.SINGLE_CELL
+    st = orc.make_stim(**s["stimulus"])
+    t, v, _ = orc.single_cell_run(orc.BO, orc.ENDO, st, s["dt"], s["n_steps"], integrator=0)
+    for tt, expected, _ in s["reference"]:
+        i = int(np.argmin(np.abs(t - tt)))
+        assert abs(v[i] - expected) < 4.0
+
+
+def _niederer(orc, integrator):
+    c = cases.NIEDERER
+    nx, ny, nz = c["shape"]
+    m = orc.OracleMesh(nx, ny, nz, c["h"], c["h"], c["h"])
+    sig = np.asarray(c["sigma9"])
+    dt = m.explicit_dt(np.tile(sig, nx * ny * nz), c["chi"] * c["cm"], c["max_co"])
+    T = orc.OracleTissue(nx, ny, nz, c["h"], sig, orc.BO, orc.EPI, dt, c["chi"], c["cm"],
+                         integrator=integrator, ddt=1, abs_tol=1e-15, rel_tol=1e-4, nthreads=8)
+    b = c["box"]
+    bx = orc.make_boxes(b["min"], b["max"], [b["t_start"]], [b["duration"]], [b["intensity"]])
+    while T.t < c["end_time"] - 0.5 * dt:
+        T.step(bx)
+    return T.tact.reshape(nz, ny, nx), dt
+
+
+@pytest.mark.slow
+def test_niederer_golden(orc):
+    ta, dt = _niederer(orc, integrator=0)
+    assert abs(dt - 4.1973434541077816e-05) < 1e-15
+    for probe in cases.NIEDERER["probes"]:
+        i, j, k = probe["cell"]
+        got = ta[k, j, i]
+        tol = probe.get("tol_here", probe["tol"])
+        assert abs(got - probe["expected"]) <= tol, (probe, got)

@@ -1,0 +1,176 @@
+"""Pins of the oracle's ionic models beyond the BuenoOrovio goldens.
+
+TNNP and ORd have no golden in the reference.  They are pinned by properties the
+published models fix and a transcription slip would break:
+  - charge conservation: the concentration balances (with rapid buffering) carry
+    exactly the charge of the summed ionic current, F sum z V dc_total = -Cm Iion;
+  - published action-potential shape at 1 Hz (resting potential, peak, APD90);
+  - Rush-Larsen exactness for a gate with constant (xinf, tau).
+AlievPanfilov: the cubic's roots 0, a, 1 and the recovery nullcline.
+Stimulus: stimulusIO.C:238-286 cases, including the S2 placement at the end of S1.
+"""
+import numpy as np
+import pytest
+
+
+def _states_along_ap(orc, model, tissue, amp, n=12):
+    st = orc.make_stim(start=5.0, period_s1=1000.0, duration=1.0, amplitude=amp, nstim1=1)
+    out = []
+    c, y = orc.model_init(model, tissue)
+    dt = 0.01
+    for k in range(int(400 / dt)):
+        yy = _pad(y)
+        orc.lib().orc_rl_step(model, tissue, k * dt, dt, c, yy, 1, st)
+        y = yy[:len(y)]
+        if k % int(400 / dt / n) == 0:
+            out.append(y.copy())
+    return c, out
+
+
+def _pad(y):
+    a = np.zeros(41)
+    a[:len(y)] = y
+    return a
+
+
+def test_tnnp_charge_conservation(orc):
+    # TNNP_2004.H constants: F, V_c, V_sr, Cm, Buf_c, K_buf_c, Buf_sr, K_buf_sr
+    F, Vc, Vsr, Cm = 96.485, 16404.0, 1094.0, 185.0
+    Bufc, Kbufc, Bufsr, Kbufsr = 0.15, 0.001, 10.0, 0.3
+    for tissue in (orc.EPI, orc.MCELL, orc.ENDO):
+        c, states = _states_along_ap(orc, orc.TNNP, tissue, 52.0)
+        st = orc.make_stim(start=0.0, period_s1=1000.0, duration=2.0, amplitude=30.0, nstim1=1)
+        for y in states:
+            for t, stim in ((500.0, None), (1.0, st)):
+                r = orc.model_eval(orc.TNNP, tissue, t, c, y, solve_vm=True, stim=stim)
+                d = r["rates"]
+                Nai, Ki, Cai, CaSR = 2, 1, 3, 15
+                dCai_t = d[Cai] * (1 + Bufc * Kbufc / (y[Cai] + Kbufc) ** 2)
+                dCaSR_t = d[CaSR] * (1 + Bufsr * Kbufsr / (y[CaSR] + Kbufsr) ** 2)
+                lhs = F * (Vc * (d[Nai] + d[Ki] + 2 * dCai_t) + 2 * Vsr * dCaSR_t)
+                rhs = -Cm * (r["iion"] + r["istim"])
+                assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-9 * Cm)
+                # dV/dt carries the same current
+                assert d[0] == pytest.approx(-(r["iion"] + r["istim"]), rel=1e-14)
+
+
+def test_ord_charge_conservation(orc):
+    # ORd_2011.H:497-692 geometry and buffer constants
+    L, rad = 0.01, 0.0011
+    Ageo = 2 * 3.14 * rad * rad + 2 * 3.14 * rad * L
+    vcell = 1000 * 3.14 * rad * rad * L
+    Acap, vmyo, vss, vnsr, vjsr = 2 * Ageo, 0.68 * vcell, 0.02 * vcell, 0.0552 * vcell, 0.0048 * vcell
+    F = 96485.0
+    cmdnmax, kmcmdn, trpnmax, kmtrpn = 0.05, 0.00238, 0.07, 0.0005
+    BSRmax, KmBSR, BSLmax, KmBSL = 0.047, 0.00087, 1.124, 0.0087
+    csqnmax, kmcsqn = 10.0, 0.8
+    c, states = _states_along_ap(orc, orc.ORD, orc.MYOCYTE, 80.0)
+    st = orc.make_stim(start=0.0, period_s1=1000.0, duration=2.0, amplitude=40.0, nstim1=1)
+    I = dict(V=0, Nai=2, Nass=3, Ki=4, Kss=5, Cass=6, Cansr=7, Cajsr=8, Cai=9)
+    for y in states:
+        for t, stim in ((500.0, None), (1.0, st)):
+            r = orc.model_eval(orc.ORD, orc.MYOCYTE, t, c, y, solve_vm=True, stim=stim)
+            d = r["rates"]
+            Cai, Cass, Cajsr = y[I["Cai"]], y[I["Cass"]], y[I["Cajsr"]]
+            dCai_t = d[I["Cai"]] * (1 + cmdnmax * kmcmdn / (kmcmdn + Cai) ** 2 + trpnmax * kmtrpn / (kmtrpn + Cai) ** 2)
+            dCass_t = d[I["Cass"]] * (1 + BSRmax * KmBSR / (KmBSR + Cass) ** 2 + BSLmax * KmBSL / (KmBSL + Cass) ** 2)
+            dCajsr_t = d[I["Cajsr"]] * (1 + csqnmax * kmcsqn / (kmcsqn + Cajsr) ** 2)
+            lhs = F * (vmyo * (d[I["Nai"]] + d[I["Ki"]]) + vss * (d[I["Nass"]] + d[I["Kss"]]) +
+                       2 * (vmyo * dCai_t + vss * dCass_t + vnsr * d[I["Cansr"]] + vjsr * dCajsr_t))
+            # the K_i balance omits the stimulus (ORd_2011.H:962 reads Istim before :982)
+            rhs = -Acap * r["iion"]
+            assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-12)
+            assert d[0] == pytest.approx(-(r["iion"] + r["istim"]), rel=1e-14)
+
+
+@pytest.mark.parametrize("model,tissue,amp,rest,peak,apd", [
+    # published ranges at 1 Hz pacing after 5 beats
+    ("TNNP", "EPI", 52.0, (-87.5, -85.0), (25.0, 45.0), (250.0, 310.0)),
+    ("TNNP", "ENDO", 52.0, (-87.5, -85.0), (25.0, 45.0), (250.0, 310.0)),
+    ("TNNP", "MCELL", 52.0, (-87.5, -85.0), (25.0, 45.0), (300.0, 420.0)),
+    ("ORD", "MYOCYTE", 80.0, (-89.0, -87.0), (35.0, 55.0), (240.0, 290.0)),
+])
+def test_published_action_potential(orc, model, tissue, amp, rest, peak, apd):
+    m, t_ = getattr(orc, model), getattr(orc, tissue)
+    st = orc.make_stim(start=10.0, period_s1=1000.0, duration=1.0, amplitude=amp, nstim1=5)
+    t, v, _ = orc.single_cell_run(m, t_, st, 1e-5, 500000, integrator=0)
+    beats = orc.detect_beats(t, v)
+    assert len(beats) == 5
+    b = beats[-1]
+    assert rest[0] < b["v_base"] < rest[1]
+    assert peak[0] < b["v_peak"] < peak[1]
+    assert apd[0] < (b["t_repol90"] - b["t_up"]) * 1e3 < apd[1]
+    # RL and adaptive RKF45 agree on the last beat's APD90 within 1 ms
+    t2, v2, _ = orc.single_cell_run(m, t_, st, 1e-5, 500000, integrator=1)
+    b2 = orc.detect_beats(t2, v2)[-1]
+    assert abs((b2["t_repol90"] - b2["t_up"]) - (b["t_repol90"] - b["t_up"])) < 1e-3
+
+
+def test_rush_larsen_exact_for_frozen_voltage(orc):
+    """BuenoOrovio gates have (xinf, tau) depending on u only: with u frozen (tissue
+    mode) one RL step of dt equals the exact solution, i.e. two RL steps of dt/2."""
+    c, y0 = orc.model_init(orc.BO, orc.EPI)
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        y = np.array([rng.uniform(-0.1, 1.4), rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0, 1)])
+        a = _pad(y)
+        orc.lib().orc_rl_step(orc.BO, orc.EPI, 0.0, 0.5, c, a, 0, None)
+        b = _pad(y)
+        orc.lib().orc_rl_step(orc.BO, orc.EPI, 0.0, 0.25, c, b, 0, None)
+        orc.lib().orc_rl_step(orc.BO, orc.EPI, 0.25, 0.25, c, b, 0, None)
+        np.testing.assert_allclose(a[:4], b[:4], rtol=1e-13, atol=1e-15)
+
+
+def test_aliev_panfilov_cubic(orc):
+    c, y = orc.model_init(orc.AP, orc.MYOCYTE)
+    a = 0.1  # (Vth - Vr)/(Vp - Vr) = 10/100
+    for u in (0.0, a, 1.0):
+        r = orc.model_eval(orc.AP, orc.MYOCYTE, 0.0, c, [u, 0.0], solve_vm=True)
+        assert abs(r["iion"]) < 1e-15
+    # recovery nullcline: dr/dt = 0 on r = -k u (u - a - 1)
+    for u in (0.2, 0.5, 0.9):
+        rr = -8.0 * u * (u - a - 1.0)
+        r = orc.model_eval(orc.AP, orc.MYOCYTE, 0.0, c, [u, rr], solve_vm=True)
+        assert abs(r["rates"][1]) < 1e-12
+    r = orc.model_eval(orc.AP, orc.MYOCYTE, 0.0, c, [0.05, 0.0], solve_vm=True)
+    assert r["iion"] == pytest.approx(8 * 0.05 * (0.05 - a) * (0.05 - 1.0), rel=1e-14)
+
+
+def test_tissue_support_matches_reference(orc):
+    # BuenoOrovio.C:92-96, TNNP.C:90-94, ORd.C:93-96, AlievPanfilov.C:92-95
+    for tissue in (1, 2, 3):
+        orc.model_init(orc.BO, tissue)
+        orc.model_init(orc.TNNP, tissue)
+        with pytest.raises(ValueError):
+            orc.model_init(orc.ORD, tissue)
+    orc.model_init(orc.ORD, 4)
+    orc.model_init(orc.AP, 4)
+    with pytest.raises(ValueError):
+        orc.model_init(orc.BO, 4)
+
+
+@pytest.mark.parametrize("t,expected", [
+    (19.999, 0.0), (20.0, -0.4), (20.5, -0.4), (21.0, -0.4), (21.0001, 0.0),
+    (1020.0, -0.4), (2020.5, -0.4), (3020.0, -0.4),   # tp == S1*n1 is inside (<=)
+    (3020.5, 0.0),
+])
+def test_stimulus_s1(orc, t, expected):
+    st = orc.make_stim(start=20.0, period_s1=1000.0, duration=1.0, amplitude=0.4, nstim1=3)
+    assert orc.lib().orc_stimulus(t, st) == expected
+
+
+def test_stimulus_s2_placement(orc):
+    # first S2 pulse at tS1End = start + S1*n1 (already covered by S1's closed end),
+    # then S2 later; the tutorials therefore use nstim2 = 2
+    st = orc.make_stim(start=10.0, period_s1=1000.0, duration=1.0, amplitude=1.0, nstim1=2,
+                       period_s2=300.0, nstim2=2)
+    f = lambda t: orc.lib().orc_stimulus(t, st)
+    assert f(2010.5) == -1.0          # S2 pulse at tS1End (=2010)
+    assert f(2310.5) == -1.0          # next S2 pulse
+    assert f(2610.0) == -1.0          # tp == S2*n2 is still inside (<=)
+    assert f(2610.5) == 0.0           # tp > S2*n2: the train is over
+    assert f(2911.0) == 0.0
+    assert f(2200.0) == 0.0
+    # defaults: nstim1 = 1 when a positive S1 period is given (stimulusIO.C:179-183)
+    assert orc.make_stim(period_s1=10.0).nstim1 == 1
+    assert orc.make_stim().nstim1 == 0
