@@ -47,20 +47,17 @@ def algo_bytes_per_cell(kernel, prec, model, algorithm):
         if algorithm == "implicit":  # V^n read, states r/w, Iion + rhs written
             return 8 + 2 * s * w + 8 + 8
         return 8 + 2 * s * w + 8 + 8 + 1  # V^n, states, Iion, V^{n+1}, flag
-    if kernel == "spmv_dot":         # p, 3 face coefficients, q
+    if kernel == "spmv_dot":         # u, 3 face coefficients read; w written
         return 5 * w
-    if kernel == "cg_update":        # x, p, r, q, 1/diag read; x, r written
-        return 7 * w
-    if kernel == "p_update":         # 1/diag, r, p read; p written
-        return 4 * w
-    if kernel == "residual":         # x, b fp64; 3 fp64 coefficients; invd; r, p (, e) out
-        return (8 + 8 + 24 + w + 2 * w + (w if prec == "mixed" else 0))
-    if kernel == "activation":
-        return 8 + 8 + 1
-    if kernel == "tensor_correction":  # V, grad (3) out; grad (3), 9 sigma in, out
-        return 8 + 24 + 24 + 72 + 8
-    if kernel == "add_correction":
-        return 8 + 4 + 8
+    if kernel == "cg_update":        # u, p, w, s, r, 1/diag, x read; p, s, x, r, u written
+        return 12 * w
+    if kernel == "residual_first":   # V^n, b, 3 fp64 coefficients, 1/diag, (b, V^{n+1}) ; r, u
+        corr = 24 + 8                # off-diagonal sigma read, corrected b written
+        return 8 + 8 + 24 + w + 2 * w + corr + (8 if prec == "fp64" else 0)
+    if kernel == "residual":         # x, e, b, 3 fp64 coefficients, 1/diag; x', r, u written
+        return 8 + 4 + 8 + 24 + 4 + 8 + 2 * 4
+    if kernel == "activation":       # solution, flag read (V^n on crossing only)
+        return 8 + 1
     return None
 
 
@@ -85,27 +82,31 @@ def coeffs_for(w, sigma):
                                  "stimulusStartTime": 0.0}}
 
 
-def run_precision(cf, torch, w, sigma, prec, steps, warmup, profile):
+def run_precision(cf, torch, w, sigma, prec, steps, warmup, prof_steps):
+    """warmup steps, `steps` timed steps (no per-kernel events in the timed region), then
+    `prof_steps` more steps with HIP-event timing of every kernel on the library stream."""
     mesh = cf.HexMesh(*w["shape"], w["h"])
     dom = cf.Monodomain(mesh, coeffs_for(w, sigma), dt=w["dt"], precision=prec,
                         tol=w["tol"] or 1e-8, max_iter=5000)
     dom.step(warmup)
     dom.sync()
     torch.cuda.synchronize()
-    if profile:
-        dom.set_profiling(True)
     it0 = dom.solver_stats()[0] if w["algorithm"] == "implicit" else 0
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     dom.step(steps)
     dom.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kt = dom.kernel_times() if profile else {}
-    if profile:
-        dom.set_profiling(False)
     it1 = dom.solver_stats()[0] if w["algorithm"] == "implicit" else 0
-    return dict(dom=dom, mesh=mesh, elapsed=elapsed, kernels=kt, iters=it1 - it0)
+    V, act = dom.Vm, dom.activationTime
+    kt = {}
+    if prof_steps > 0:
+        dom.set_profiling(True)
+        dom.step(prof_steps)
+        dom.sync()
+        kt = dom.kernel_times()
+        dom.set_profiling(False)
+    return dict(dom=dom, mesh=mesh, elapsed=elapsed, kernels=kt, iters=it1 - it0, V=V, act=act)
 
 
 def cpu_baseline(w, sigma, sample_steps):
@@ -147,6 +148,7 @@ def main():
     ap.add_argument("--steps", type=int, default=250)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="ord_implicit_2M", choices=sorted(WORKLOADS))
+    ap.add_argument("--prof-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -166,11 +168,12 @@ def main():
     n = int(np.prod(w["shape"]))
     res = {}
     for prec in ("fp64", "mixed"):
-        res[prec] = run_precision(cf, torch, w, sigma, prec, args.steps, args.warmup, True)
+        res[prec] = run_precision(cf, torch, w, sigma, prec, args.steps, args.warmup,
+                                  args.prof_steps)
         print("[bench] %s: %.3f s for %d steps" % (prec, res[prec]["elapsed"], args.steps),
               file=sys.stderr, flush=True)
-    V64, V32 = res["fp64"]["dom"].Vm, res["mixed"]["dom"].Vm
-    a64, a32 = res["fp64"]["dom"].activationTime, res["mixed"]["dom"].activationTime
+    V64, V32 = res["fp64"]["V"], res["mixed"]["V"]
+    a64, a32 = res["fp64"]["act"], res["mixed"]["act"]
     vm_err_mv = float(np.max(np.abs(V64 - V32)) * 1e3)
     both = (a64 > 0) & (a32 > 0)
     act_err_ms = float(np.max(np.abs(a64[both] - a32[both])) * 1e3) if both.any() else 0.0

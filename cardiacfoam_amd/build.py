@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcardiacgpu.so")
-SOURCES = ["ionic.hip", "tissue_explicit.hip", "tissue_implicit.hip", "batched.hip",
+SOURCES = ["model_fp64.hip", "model_fp32.hip", "tissue_implicit.hip", "batched.hip",
            "adjoint.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -23,10 +23,19 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
+# Single-precision model kernels: reciprocal-based division, hardware exp2/log2 and
+# fp32 denormals flushed to zero (fp64 keeps IEEE denormals; no finite-math assumption,
+# so inf/NaN propagate as in IEEE arithmetic).
+FAST_F32 = [
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fno-math-errno", "-freciprocal-math",
+    "-fapprox-func", "-fno-signed-zeros", "-fgpu-flush-denormals-to-zero",
+]
+EXTRA = {"model_fp32.hip": FAST_F32}
+
 
 def _compile(src):
     obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr))

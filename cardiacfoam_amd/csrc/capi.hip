@@ -115,7 +115,8 @@ struct cg_mesh {
     bool uniform = true;
     bool has_corr = false;
     double sig[9] = {0};
-    double* d_sigma9 = nullptr;
+    double* d_od = nullptr;  // off-diagonal components (xy, xz, yz) [3][n]
+    unsigned od_mask = 0;
     double* d_fx = nullptr;
     double* d_fy = nullptr;
     double* d_fz = nullptr;
@@ -131,8 +132,6 @@ struct cg_tissue {
     double* d_Iion = nullptr;
     uint8_t* d_flag = nullptr;
     double* d_tact = nullptr;
-    double* d_corr = nullptr;
-    double* d_grad = nullptr;
     int n_boxes = 0;
     double box_lo[cg::MAX_BOXES][3], box_hi[cg::MAX_BOXES][3];
     double box_t0[cg::MAX_BOXES], box_dur[cg::MAX_BOXES], box_I[cg::MAX_BOXES];
@@ -367,6 +366,8 @@ int cg_mesh_create_hex(int nx, int ny, int nz, double hx, double hy, double hz, 
     *out = nullptr;
     if (nx < 1 || ny < 1 || nz < 1 || !(hx > 0) || !(hy > 0) || !(hz > 0))
         return fail(CG_ERR_ARG, "bad hex block dimensions");
+    if ((int64_t)nx * ny * nz >= ((int64_t)1 << 31))
+        return fail(CG_ERR_ARG, "hex block of %lld cells exceeds 2^31 - 1", (long long)nx * ny * nz);
     cg_mesh* m = new cg_mesh();
     m->g = {nx, ny, nz, hx, hy, hz};
     m->n = (int64_t)nx * ny * nz;
@@ -400,7 +401,7 @@ int cg_mesh_match_ldu(int nx, int ny, int nz, int64_t nf, const int32_t* owner,
 int cg_mesh_destroy(cg_mesh* m)
 {
     if (!m) return CG_OK;
-    hipFree(m->d_sigma9); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
+    hipFree(m->d_od); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
     delete m;
     return CG_OK;
 }
@@ -412,8 +413,8 @@ int cg_mesh_set_conductivity(cg_mesh* m, const double* s, int uniform)
     if (!m || !s) return fail(CG_ERR_ARG, "NULL argument");
     const cg::HexGeom& g = m->g;
     const int64_t n = m->n;
-    hipFree(m->d_sigma9); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
-    m->d_sigma9 = m->d_fx = m->d_fy = m->d_fz = nullptr;
+    hipFree(m->d_od); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
+    m->d_od = m->d_fx = m->d_fy = m->d_fz = nullptr;
     m->h_sigma9.assign(9 * (size_t)n, 0.0);
     for (int64_t c = 0; c < n; ++c)
         for (int q = 0; q < 9; ++q) m->h_sigma9[9 * c + q] = uniform ? s[q] : s[9 * c + q];
@@ -446,10 +447,18 @@ int cg_mesh_set_conductivity(cg_mesh* m, const double* s, int uniform)
         HIPCHK(hipMemcpy(m->d_fz, fz.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     }
     if (offdiag) {
-        if (hipMalloc(&m->d_sigma9, sizeof(double) * 9 * n) != hipSuccess)
+        std::vector<double> od(3 * (size_t)n);
+        m->od_mask = 0;
+        for (int64_t c = 0; c < n; ++c) {
+            od[c] = m->h_sigma9[9 * c + 1];
+            od[n + c] = m->h_sigma9[9 * c + 2];
+            od[2 * n + c] = m->h_sigma9[9 * c + 5];
+            for (int q = 0; q < 3; ++q)
+                if (od[q * n + c] != 0.0) m->od_mask |= 1u << q;
+        }
+        if (hipMalloc(&m->d_od, sizeof(double) * 3 * n) != hipSuccess)
             return fail(CG_ERR_NOMEM, "conductivity allocation failed");
-        HIPCHK(hipMemcpy(m->d_sigma9, m->h_sigma9.data(), sizeof(double) * 9 * n,
-                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(m->d_od, od.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     }
     return CG_OK;
 }
@@ -495,18 +504,8 @@ int cg_mesh_laplacian(cg_mesh* m, const double* V, double* out)
 {
     if (!m || !V || !out) return fail(CG_ERR_ARG, "NULL argument");
     if (m->h_sigma9.empty()) return fail(CG_ERR_STATE, "conductivity not set");
-    double* corr = nullptr;
-    double* grad = nullptr;
-    if (m->has_corr) {
-        if (hipMalloc(&corr, sizeof(double) * m->n) != hipSuccess ||
-            hipMalloc(&grad, sizeof(double) * 3 * m->n) != hipSuccess)
-            return fail(CG_ERR_NOMEM, "scratch allocation failed");
-        HIPCHK(cg::launch_hex_correction(m->g, m->d_sigma9, V, grad, corr, nullptr));
-    }
-    HIPCHK(cg::launch_hex_laplacian(m->g, mesh_coefs(m), V, corr, out, nullptr));
+    HIPCHK(cg::launch_hex_laplacian(m->g, mesh_coefs(m), V, m->d_od, m->od_mask, out, nullptr));
     HIPCHK(hipDeviceSynchronize());
-    hipFree(corr);
-    hipFree(grad);
     return CG_OK;
 }
 
@@ -533,10 +532,6 @@ int cg_tissue_create(cg_mesh* m, cg_ionic* ion, const cg_tissue_config* cfg, cg_
     ok &= hipMalloc(&t->d_Iion, sizeof(double) * n) == hipSuccess;
     ok &= hipMalloc(&t->d_flag, n) == hipSuccess;
     ok &= hipMalloc(&t->d_tact, sizeof(double) * n) == hipSuccess;
-    if (m->has_corr) {
-        ok &= hipMalloc(&t->d_corr, sizeof(double) * n) == hipSuccess;
-        ok &= hipMalloc(&t->d_grad, sizeof(double) * 3 * n) == hipSuccess;
-    }
     if (!ok) {
         cg_tissue_destroy(t);
         return fail(CG_ERR_NOMEM, "tissue allocation failed");
@@ -565,7 +560,6 @@ int cg_tissue_destroy(cg_tissue* t)
     if (t->ion && t->ion->stream) hipStreamSynchronize(t->ion->stream);
     for (int q = 0; q < 3; ++q) hipFree(t->d_V[q]);
     hipFree(t->d_Iion); hipFree(t->d_flag); hipFree(t->d_tact);
-    hipFree(t->d_corr); hipFree(t->d_grad);
     if (t->implicit) cg::implicit_destroy(t->implicit);
     delete t;
     return CG_OK;
@@ -622,12 +616,8 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
         double* Vm1 = t->d_V[(t->cur + 2) % 3];
         double* Vnew = t->d_V[(t->cur + 1) % 3];
         cg::Profiler* prof = t->prof.enabled ? &t->prof : nullptr;
-        if (m->has_corr) {
-            const int tk = prof ? prof->begin(cg::K_CORR) : -1;
-            HIPCHK(cg::launch_hex_correction(m->g, m->d_sigma9, Vn, t->d_grad, t->d_corr, ion->stream));
-            if (prof) prof->end(tk);
-            st.corr = t->d_corr;
-        }
+        st.od = m->d_od;
+        st.od_mask = m->od_mask;
         if (t->cfg.algorithm == CG_EXPLICIT) {
             const int tk = prof ? prof->begin(cg::K_IONIC) : -1;
             HIPCHK(cg::launch_explicit_hex(ion->model, ion->prec, ion->tissue, m->g, coefs,

@@ -21,6 +21,6 @@ void implicit_stats(const ImplicitSolver* S, int64_t* iters, double* res);
 
 // standalone pieces used by tests through the C-ABI
 hipError_t launch_hex_laplacian(const HexGeom& g, const HexCoefs& k, const double* V,
-                                const double* corr, double* out, hipStream_t s);
+                                const double* od, unsigned od_mask, double* out, hipStream_t s);
 
 } // namespace cg

@@ -39,31 +39,78 @@ struct ExplicitStep {
     double c, c0, c00; // ddt coefficients (Euler: 1,1,0; backward: 1.5,2,0.5)
     double t_old;      // time at the step start (activation interpolation)
     ActiveBoxes boxes;
-    const double* corr;  // explicit tensor correction div (per unit volume) or null
+    const double* od;  // off-diagonal conductivity (xy, xz, yz) [3][n], or null
+    unsigned od_mask;  // which of xy, xz, yz are non-zero somewhere (bits 0..2)
 };
 
-// ionic step (tissue mode) on SoA states; Vm in V, Im in V/s; prec 0 = fp64, 1 = fp32
-hipError_t launch_ionic_tissue(int model, int prec, int tissue, void* states, int64_t n,
-                               const double* Vm, double* Im, double dt_model,
-                               const ModelParams& P, const void* gscale, hipStream_t s);
+// Model kernels, one set per arithmetic type: _fp64 (IEEE double, model_fp64.hip) and
+// _fp32 (single precision with hardware reciprocal/exp2/log2, model_fp32.hip).
+#define CG_DECLARE_MODEL_ENTRIES(SFX)                                                      \
+    /* ionic step (tissue mode) on SoA states; Vm in V, Im in V/s */                       \
+    hipError_t launch_ionic_tissue_##SFX(int model, int tissue, void* states, int64_t n,   \
+                                         const double* Vm, double* Im, double dt_model,    \
+                                         const ModelParams& P, const void* gscale,         \
+                                         hipStream_t s);                                   \
+    /* n independent single cells (V inside the ODE), n_steps steps from model time t0 */  \
+    hipError_t launch_ionic_cells_##SFX(int model, int tissue, void* states, int64_t n,    \
+                                        double t0, double dt, int n_steps,                 \
+                                        const ModelParams& P, const void* gscale,          \
+                                        const Stimulus& stim, hipStream_t s);              \
+    /* fused explicit monodomain step on the hex block: ionic + diffusion + activation */  \
+    hipError_t launch_explicit_hex_##SFX(int model, int tissue, const HexGeom& g,          \
+                                         const HexCoefs& a, void* states, const double* V, \
+                                         const double* Vold, double* Vnew, double* Iion,   \
+                                         uint8_t* flag, double* tact,                      \
+                                         const ExplicitStep& st, const ModelParams& P,     \
+                                         hipStream_t s);                                   \
+    /* ionic step + right-hand side of the implicit monodomain system */                   \
+    hipError_t launch_ionic_rhs_##SFX(int model, int tissue, const HexGeom& g,             \
+                                      void* states, const double* Vn, const double* Vm1,   \
+                                      const double* corr, double* Iion, double* b,         \
+                                      const ExplicitStep& st, const ModelParams& P,        \
+                                      hipStream_t s);
 
-// single-cell steps (V inside the ODE) for n independent cells, n_steps steps of
-// dt_model starting at model time t0; V trace of the last step not stored.
-hipError_t launch_ionic_cells(int model, int prec, int tissue, void* states, int64_t n,
-                              double t0_model, double dt_model, int n_steps,
-                              const ModelParams& P, const void* gscale, const Stimulus& stim,
-                              hipStream_t s);
+CG_DECLARE_MODEL_ENTRIES(fp64)
+CG_DECLARE_MODEL_ENTRIES(fp32)
 
-// fused explicit monodomain step on the hex block: ionic + diffusion + activation
-hipError_t launch_explicit_hex(int model, int prec, int tissue, const HexGeom& g,
-                               const HexCoefs& k, void* states, const double* V,
-                               const double* Vold, double* Vnew, double* Iion, uint8_t* flag,
-                               double* tact, const ExplicitStep& st, const ModelParams& P,
-                               hipStream_t s);
+// precision dispatch: prec 0 = fp64, 1 = fp32
+inline hipError_t launch_ionic_tissue(int model, int prec, int tissue, void* states, int64_t n,
+                                      const double* Vm, double* Im, double dt_model,
+                                      const ModelParams& P, const void* gscale, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    return (prec ? launch_ionic_tissue_fp32 : launch_ionic_tissue_fp64)(
+        model, tissue, states, n, Vm, Im, dt_model, P, gscale, s);
+}
 
-// tensor correction div(SfGammaCorr . interp(grad V)) on the hex block (sigma9 per cell)
-hipError_t launch_hex_correction(const HexGeom& g, const double* sigma9, const double* V,
-                                 double* grad3, double* out, hipStream_t s);
+inline hipError_t launch_ionic_cells(int model, int prec, int tissue, void* states, int64_t n,
+                                     double t0_model, double dt_model, int n_steps,
+                                     const ModelParams& P, const void* gscale,
+                                     const Stimulus& stim, hipStream_t s)
+{
+    if (n <= 0 || n_steps <= 0) return hipSuccess;
+    return (prec ? launch_ionic_cells_fp32 : launch_ionic_cells_fp64)(
+        model, tissue, states, n, t0_model, dt_model, n_steps, P, gscale, stim, s);
+}
+
+inline hipError_t launch_explicit_hex(int model, int prec, int tissue, const HexGeom& g,
+                                      const HexCoefs& a, void* states, const double* V,
+                                      const double* Vold, double* Vnew, double* Iion,
+                                      uint8_t* flag, double* tact, const ExplicitStep& st,
+                                      const ModelParams& P, hipStream_t s)
+{
+    return (prec ? launch_explicit_hex_fp32 : launch_explicit_hex_fp64)(
+        model, tissue, g, a, states, V, Vold, Vnew, Iion, flag, tact, st, P, s);
+}
+
+inline hipError_t launch_ionic_rhs(int model, int prec, int tissue, const HexGeom& g,
+                                   void* states, const double* Vn, const double* Vm1,
+                                   const double* corr, double* Iion, double* b,
+                                   const ExplicitStep& st, const ModelParams& P, hipStream_t s)
+{
+    return (prec ? launch_ionic_rhs_fp32 : launch_ionic_rhs_fp64)(
+        model, tissue, g, states, Vn, Vm1, corr, Iion, b, st, P, s);
+}
 
 } // namespace cg
 
@@ -74,8 +121,8 @@ namespace cg {
 
 // Per-kernel HIP-event timing on the handle's stream (the trace equivalent of the
 // clockTime phases of staggeredElectrophysicsAdvanceScheme.C:54-74).
-enum KernelId { K_IONIC = 0, K_SPMV, K_UPDATE, K_PUPDATE, K_RESIDUAL, K_CORR, K_ACTIVATION,
-                K_INVDIAG, K_ADDCORR, K_COUNT };
+enum KernelId { K_IONIC = 0, K_SPMV, K_UPDATE, K_RESIDUAL0, K_RESIDUAL, K_ACTIVATION,
+                K_INVDIAG, K_COUNT };
 const char* kernel_name(int id);
 
 struct Profiler {
@@ -111,10 +158,10 @@ struct Profiler {
         if (tok >= 0) hipEventRecord(recs[tok].b, stream);
     }
     size_t mark() const { return recs.size(); }
-    // drop records from index i on (launches that exited early on a converged flag)
-    void invalidate_from(size_t i)
+    // drop records [i, j) (launches that exited early on a converged flag)
+    void invalidate_range(size_t i, size_t j)
     {
-        for (size_t k = i; k < recs.size(); ++k) recs[k].id = -1;
+        for (size_t k = i; k < j && k < recs.size(); ++k) recs[k].id = -1;
     }
     // accumulate recorded intervals; sync first when asked
     void resolve(bool sync)

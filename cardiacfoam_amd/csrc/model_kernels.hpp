@@ -1,0 +1,290 @@
+// Kernels that run an ionic model, templated on the model M and the arithmetic type T.
+// They are instantiated in two translation units: model_fp64.hip (IEEE FP64, the plain
+// path) and model_fp32.hip (FP32 with approximate reciprocal and exponential, the mixed
+// path).  Launch entry points of both are declared in cg_internal.hpp.
+//
+// Memory layout: ionic states structure-of-arrays, state q of cell c at q*n + c, so each
+// state load/store of a wavefront is one contiguous segment; one cell per lane with all
+// states of the cell in registers for the whole step.
+#pragma once
+#include "cg_internal.hpp"
+#include "cg_hexops.hpp"
+
+namespace cg {
+
+constexpr int MBS = 256;  // threads per block of the model kernels
+
+// model constants already in the kernel's arithmetic type (no per-lane conversions)
+template <class T> struct KParams {
+    T p[MAX_PARAMS];
+};
+template <class T> inline KParams<T> kparams(const ModelParams& P)
+{
+    KParams<T> k;
+    for (int i = 0; i < MAX_PARAMS; ++i) k.p[i] = T(P.p[i]);
+    return k;
+}
+
+// states of one cell: state q of cell c at st[q*n + c], walked with a pointer step of n
+template <int NS, class T>
+CG_DEV void load_states(T (&y)[NS], const T* __restrict__ st, int64_t n, int64_t c, int first)
+{
+    const T* sp = st + c + first * n;
+#pragma unroll
+    for (int q = first; q < NS; ++q, sp += n) y[q] = *sp;
+}
+template <int NS, class T>
+CG_DEV void store_states(const T (&y)[NS], T* __restrict__ st, int64_t n, int64_t c, int first)
+{
+    T* sp = st + c + first * n;
+#pragma unroll
+    for (int q = first; q < NS; ++q, sp += n) *sp = y[q];
+}
+
+// ---------------------------------------------------------------- ionic only
+template <class M, class T>
+__global__ __launch_bounds__(MBS) void k_ionic_tissue(T* __restrict__ st, int64_t n,
+                                                      const double* __restrict__ Vm,
+                                                      double* __restrict__ Im, T dt,
+                                                      KParams<T> P,
+                                                      const T* __restrict__ gsc, int tissue)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T p[M::NP];
+#pragma unroll
+    for (int k = 0; k < M::NP; ++k) p[k] = P.p[k];
+    T gs[4] = {T(1), T(1), T(1), T(1)};
+    if (gsc) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gs[k] = gsc[k * n + i];
+    }
+    T y[M::NS];
+    load_states(y, st, n, i, 1);
+    const T v = T((Vm[i] * 1000.0 + M::VOFF) / M::VSCALE);
+    const T I = Ops<M>::tissue_step(v, y, dt, p, tissue, gs);
+    store_states(y, st, n, i, 1);
+    Im[i] = double(I) * M::ISCALE;
+}
+
+template <class M, class T>
+__global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t n, double t0,
+                                                     T dt, int n_steps, KParams<T> P,
+                                                     const T* __restrict__ gsc, int tissue,
+                                                     Stimulus stim)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T p[M::NP];
+#pragma unroll
+    for (int k = 0; k < M::NP; ++k) p[k] = P.p[k];
+    T gs[4] = {T(1), T(1), T(1), T(1)};
+    if (gsc) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gs[k] = gsc[k * n + i];
+    }
+    T y[M::NS];
+    load_states(y, st, n, i, 0);
+    for (int s = 0; s < n_steps; ++s) {
+        const double t = t0 + s * double(dt);
+        Ops<M>::cell_step(y, dt, T(stimulus_current(t, stim)), p, tissue, gs);
+    }
+    store_states(y, st, n, i, 0);
+}
+
+// ------------------------------------------------------- fused explicit step
+// Box stimulus of myocardiumDomain.C:307-342: summed intensities of the boxes whose
+// closed extent contains the cell centre.
+CG_DEV double box_source(int i, int j, int k, const HexGeom& g, const ActiveBoxes& b)
+{
+    double src = 0.0;
+    const double p0 = (i + 0.5) * g.hx, p1 = (j + 0.5) * g.hy, p2 = (k + 0.5) * g.hz;
+    for (int q = 0; q < b.n; ++q) {
+        if (p0 >= b.lo[q][0] && p0 <= b.hi[q][0] && p1 >= b.lo[q][1] && p1 <= b.hi[q][1] &&
+            p2 >= b.lo[q][2] && p2 <= b.hi[q][2])
+            src += b.intensity[q];
+    }
+    return src;
+}
+
+// Fused explicit monodomain step on the hex block (myocardiumDomain.C:487-529,
+// monodomainSolver.C:103-118): box stimulus -> ionic step at V^n -> diagonal solve
+//   V^{n+1} = (c0 V^n - c00 V^{n-1} + dt/chiCm (lap - chiCm Iion + src)) / c
+// -> activation time where V first exceeds 1e-15 V (myocardiumDomain.C:345-374).
+// Bytes per cell-step (fp64, BuenoOrovio): V 8 + gates 24 r/w + V^{n+1} 8 + Iion 8 +
+// flag 1 (+ V^{n-1} 8 for backward, + off-diagonal sigma 24 with a tensor correction).
+template <class M, class T, bool UNIFORM>
+__global__ __launch_bounds__(MBS) void k_explicit_hex(
+    HexGeom g, HexCoefs a, T* __restrict__ st, const double* __restrict__ V,
+    const double* __restrict__ Vold, double* __restrict__ Vnew, double* __restrict__ Iion,
+    uint8_t* __restrict__ flag, double* __restrict__ tact, ExplicitStep s, KParams<T> P,
+    int tissue)
+{
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const IJK ix = ijk_of((uint32_t)c, g);
+    T p[M::NP];
+#pragma unroll
+    for (int q = 0; q < M::NP; ++q) p[q] = P.p[q];
+    const T one[4] = {T(1), T(1), T(1), T(1)};
+    T y[M::NS];
+    load_states(y, st, n, c, 1);
+    const double vn = V[c];
+    const T v = T((vn * 1000.0 + M::VOFF) / M::VSCALE);
+    const T I = Ops<M>::tissue_step(v, y, T(s.dt_model), p, tissue, one);
+    store_states(y, st, n, c, 1);
+    const double iion = double(I) * M::ISCALE;
+    if (Iion) Iion[c] = iion;
+
+    double lap = hex_laplacian<UNIFORM>(V, c, ix, g, a);
+    if (s.od) lap += hex_tensor_correction(V, s.od, n, c, ix.i, ix.j, ix.k, g, s.od_mask);
+    const double src = box_source(ix.i, ix.j, ix.k, g, s.boxes);
+    double rhs = s.c0 * vn + s.dt / s.chiCm * (lap - s.chiCm * iion + src);
+    if (s.c00 != 0.0) rhs -= s.c00 * Vold[c];
+    const double vnew = rhs / s.c;
+    Vnew[c] = vnew;
+    if (flag[c] && vnew > 1e-15) {
+        flag[c] = 0;
+        tact[c] = s.t_old + (0.0 - vn) / (vnew - vn) * s.dt;
+    }
+}
+
+// ------------------------------------------ ionic step + implicit right-hand side
+// rhs = chiCm/dt (c0 V^n - c00 V^{n-1}) + corr - chiCm Iion + source, where corr is the
+// explicit tensor correction of the Laplacian (null when sigma is diagonal).
+template <class M, class T>
+__global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st,
+                                                   const double* __restrict__ Vn,
+                                                   const double* __restrict__ Vm1,
+                                                   const double* __restrict__ corr,
+                                                   double* __restrict__ Iion,
+                                                   double* __restrict__ b, ExplicitStep s,
+                                                   KParams<T> P, int tissue)
+{
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * MBS + threadIdx.x;
+    if (c >= n) return;
+    const IJK cc = ijk_of((uint32_t)c, g);
+    T p[M::NP];
+#pragma unroll
+    for (int q = 0; q < M::NP; ++q) p[q] = P.p[q];
+    const T one[4] = {T(1), T(1), T(1), T(1)};
+    T y[M::NS];
+    load_states(y, st, n, c, 1);
+    const double vn = Vn[c];
+    const T v = T((vn * 1000.0 + M::VOFF) / M::VSCALE);
+    const T I = Ops<M>::tissue_step(v, y, T(s.dt_model), p, tissue, one);
+    store_states(y, st, n, c, 1);
+    const double iion = double(I) * M::ISCALE;
+    Iion[c] = iion;
+    const double src = box_source(cc.i, cc.j, cc.k, g, s.boxes);
+    double rhs = s.chiCm / s.dt * (s.c0 * vn - s.c00 * (s.c00 != 0.0 ? Vm1[c] : 0.0)) +
+                 src - s.chiCm * iion;
+    if (corr) rhs += corr[c];
+    b[c] = rhs;
+}
+
+// ---------------------------------------------------------------- launchers
+inline unsigned grid_8(int64_t n, int bs)
+{
+    int64_t nb = (n + bs - 1) / bs;
+    return (unsigned)((nb + 7) / 8 * 8);  // multiple of 8 for the XCD remap
+}
+
+template <class T> struct ModelLaunch {
+    template <class M>
+    static hipError_t tissue(int tissue, void* states, int64_t n, const double* Vm, double* Im,
+                             double dt_model, const ModelParams& P, const void* gscale,
+                             hipStream_t s)
+    {
+        const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
+        hipLaunchKernelGGL((k_ionic_tissue<M, T>), dim3(grid), dim3(MBS), 0, s, (T*)states, n,
+                           Vm, Im, T(dt_model), kparams<T>(P), (const T*)gscale, tissue);
+        return hipGetLastError();
+    }
+    template <class M>
+    static hipError_t cells(int tissue, void* states, int64_t n, double t0, double dt,
+                            int n_steps, const ModelParams& P, const void* gscale,
+                            const Stimulus& stim, hipStream_t s)
+    {
+        const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
+        hipLaunchKernelGGL((k_ionic_cells<M, T>), dim3(grid), dim3(MBS), 0, s, (T*)states, n,
+                           t0, T(dt), n_steps, kparams<T>(P), (const T*)gscale, tissue, stim);
+        return hipGetLastError();
+    }
+    template <class M>
+    static hipError_t explicit_hex(int tissue, const HexGeom& g, const HexCoefs& a,
+                                   void* states, const double* V, const double* Vold,
+                                   double* Vnew, double* Iion, uint8_t* flag, double* tact,
+                                   const ExplicitStep& st, const ModelParams& P, hipStream_t s)
+    {
+        const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+        const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
+        if (a.fx)
+            hipLaunchKernelGGL((k_explicit_hex<M, T, false>), dim3(grid), dim3(MBS), 0, s, g,
+                               a, (T*)states, V, Vold, Vnew, Iion, flag, tact, st, kparams<T>(P), tissue);
+        else
+            hipLaunchKernelGGL((k_explicit_hex<M, T, true>), dim3(grid), dim3(MBS), 0, s, g, a,
+                               (T*)states, V, Vold, Vnew, Iion, flag, tact, st, kparams<T>(P), tissue);
+        return hipGetLastError();
+    }
+    template <class M>
+    static hipError_t ionic_rhs(int tissue, const HexGeom& g, void* states, const double* Vn,
+                                const double* Vm1, const double* corr, double* Iion,
+                                double* b, const ExplicitStep& st, const ModelParams& P,
+                                hipStream_t s)
+    {
+        const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+        hipLaunchKernelGGL((k_ionic_rhs<M, T>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,
+                           (T*)states, Vn, Vm1, corr, Iion, b, st, kparams<T>(P), tissue);
+        return hipGetLastError();
+    }
+};
+
+// switch over the four models for one arithmetic type
+#define CG_MODEL_SWITCH(T, FN, ...)                                                        \
+    switch (model) {                                                                       \
+    case MODEL_BO: return ModelLaunch<T>::template FN<BuenoOrovio>(__VA_ARGS__);           \
+    case MODEL_TNNP: return ModelLaunch<T>::template FN<TNNP>(__VA_ARGS__);                \
+    case MODEL_ORD: return ModelLaunch<T>::template FN<ORd>(__VA_ARGS__);                  \
+    case MODEL_AP: return ModelLaunch<T>::template FN<AlievPanfilov>(__VA_ARGS__);         \
+    default: return hipErrorInvalidValue;                                                  \
+    }
+
+// defines the four per-precision entry points of cg_internal.hpp for type T, suffix SFX
+#define CG_DEFINE_MODEL_ENTRIES(T, SFX)                                                    \
+    hipError_t launch_ionic_tissue_##SFX(int model, int tissue, void* states, int64_t n,   \
+                                         const double* Vm, double* Im, double dt_model,    \
+                                         const ModelParams& P, const void* gscale,         \
+                                         hipStream_t s)                                    \
+    {                                                                                      \
+        CG_MODEL_SWITCH(T, tissue, tissue, states, n, Vm, Im, dt_model, P, gscale, s)      \
+    }                                                                                      \
+    hipError_t launch_ionic_cells_##SFX(int model, int tissue, void* states, int64_t n,    \
+                                        double t0, double dt, int n_steps,                 \
+                                        const ModelParams& P, const void* gscale,          \
+                                        const Stimulus& stim, hipStream_t s)               \
+    {                                                                                      \
+        CG_MODEL_SWITCH(T, cells, tissue, states, n, t0, dt, n_steps, P, gscale, stim, s)  \
+    }                                                                                      \
+    hipError_t launch_explicit_hex_##SFX(int model, int tissue, const HexGeom& g,          \
+                                         const HexCoefs& a, void* states, const double* V, \
+                                         const double* Vold, double* Vnew, double* Iion,   \
+                                         uint8_t* flag, double* tact,                      \
+                                         const ExplicitStep& st, const ModelParams& P,     \
+                                         hipStream_t s)                                    \
+    {                                                                                      \
+        CG_MODEL_SWITCH(T, explicit_hex, tissue, g, a, states, V, Vold, Vnew, Iion, flag,  \
+                        tact, st, P, s)                                                    \
+    }                                                                                      \
+    hipError_t launch_ionic_rhs_##SFX(int model, int tissue, const HexGeom& g,             \
+                                      void* states, const double* Vn, const double* Vm1,   \
+                                      const double* corr, double* Iion, double* b,         \
+                                      const ExplicitStep& st, const ModelParams& P,        \
+                                      hipStream_t s)                                       \
+    {                                                                                      \
+        CG_MODEL_SWITCH(T, ionic_rhs, tissue, g, states, Vn, Vm1, corr, Iion, b, st, P, s) \
+    }
+
+} // namespace cg

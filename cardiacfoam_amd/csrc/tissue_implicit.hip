@@ -1,49 +1,68 @@
-// Implicit monodomain step on the hex block.
+// Implicit monodomain step on the hex block (solveDiffusionImplicit,
+// monodomainSolver.C:121-150).
 //
-// Matrix (per unit volume, monodomainSolver.C:129-135 with fvm::laplacian):
-//   (c chiCm/dt + sum_f a_f) V_P - sum_f a_f V_N = rhs,   a_f = sigma_nn,f / h^2
+// Matrix per unit volume (fvm::laplacian, orthogonal part; fvm::ddt with coefficient c):
+//   (c chiCm/dt + sum_f a_f) V_P - sum_f a_f V_N = rhs,   a_f = sigma_nn,f / h_d^2
 // stored as three face-coefficient planes fx, fy, fz (face to +x/+y/+z, 0 on the wall)
-// plus 1/diag; the operator is applied as a 7-point stencil (no index arrays):
-//   bytes per row and product: 3 coefficients + x + y  (fp64 40 B, fp32 20 B).
-// Right-hand side (computed in the ionic kernel, so Iion never makes an extra pass):
-//   rhs = chiCm/dt (c0 V^n - c00 V^{n-1}) + div(corr) - chiCm Iion + source
-// Solver: Jacobi-preconditioned CG.  fp64 path: CG in fp64 to ||b-Ax||/||b|| <= tol.
-// mixed path: iterative refinement -- residual and solution update in fp64, correction
-// from an fp32 CG (fp32 operator and vectors, fp64 dot products).
-// Reductions are deterministic: per-block partials, the last block to finish sums them
-// in a fixed order and updates the scalars of the iteration (alpha, beta, convergence).
+// and 1/diag, applied as a 7-point stencil without index arrays.
+// Right-hand side, built by the ionic kernel so Iion never takes a pass of its own:
+//   rhs = chiCm/dt (c0 V^n - c00 V^{n-1}) - chiCm Iion + source
+// to which the first residual pass adds the explicit tensor correction
+//   div(SfGammaCorr . interp grad V^n)   (19-point stencil over V^n and sigma xy, xz, yz).
+//
+// Solver: Jacobi-preconditioned conjugate gradients in the single-reduction form of
+// Chronopoulos and Gear (J. Comput. Appl. Math. 25, 1989): the same iterates as the
+// textbook recurrence, with both dot products of an iteration taken in one pass.
+//   fp64 path : CG in fp64 to ||b - A x||_2 <= tol ||b||_2.
+//   mixed path: iterative refinement; residual and solution in fp64, each correction from
+//               an fp32 CG (fp32 operator and vectors, fp64 dot products) to 1e-4 relative.
+// Kernels of one step:
+//   residual (first)  x0 = V^n, b += tensor correction, r = b - A x0, u = M^-1 r
+//   per iteration k   S: convergence test on r_k; w = A u, partial (w,u)
+//                     U: alpha, beta; p = u + beta p, s = w + beta s, x += alpha p,
+//                        r -= alpha s, u = M^-1 r, partials (r,u), (r,r)
+//   residual (mixed)  x' = x + e (ping-pong buffers), r = b - A x', next correction start
+//   activation        first crossing of V > 1e-15 V, copy of the solution into V^{n+1}
+// Scalars are never stored for readers of the same launch: every block recomputes them
+// from per-block partial sums (fixed summation order, so all blocks agree); partial slots
+// read and written in one launch are double-buffered by iteration parity.  Each CG solve
+// carries a generation number, so "solve g is finished" needs no reset.  No atomics, no
+// device-scope fences.
+#include <cmath>
 #include <cstring>
 #include <vector>
 #include "cg_implicit.hpp"
+#include "cg_hexops.hpp"
 
 namespace cg {
 
-constexpr int BS = 256;
-constexpr int NWAVE = BS / 64;
+constexpr int BS = 256;    // elementwise kernels
+constexpr int RBS = 512;   // reducing kernels
+constexpr int RG = 1024;   // blocks of every reducing kernel
 
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD owns a
-// contiguous run of cells and the +-nx, +-nx*ny neighbour rows stay in its own L2.
-CG_DEV int64_t xcd_block(int64_t b, int64_t nb)
-{
-    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
-}
+// partial-sum slots (RG doubles each)
+enum { P_G = 0,      // (r,u), parity sets 0 and 1
+       P_RR = 2,     // (r,r) in the CG precision, parity sets 2 and 3
+       P_D = 4,      // (w,u)
+       P_RR0 = 5,    // (r,r) of the fp64 residual
+       P_BB = 6,     // (b,b)
+       P_NSLOT = 7 };
 
-enum { S_RZ, S_RZ_OLD, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL2, S_R0, S_NSCAL };
+enum { S_RR, S_BB, S_R0, S_NSCAL };
 struct DevState {
-    double s[S_NSCAL];
-    int iters;
-    int done;
-    int max_iter;
+    double s[S_NSCAL];        // last (r,r), (b,b), fp64 (r0,r0)
+    double alpha[2], gamma[2];  // by iteration parity, written by block 0 of the update
+    long long done_gen;       // generation of the last finished CG solve
+    long long outer_gen;      // generation whose start already met the outer tolerance
+    int iters;                // iterations of the current solve
     int pad;
-    unsigned counter[4];
+    long long iters_sum;      // iterations of all solves, added by each residual pass
 };
 
-template <int NV>
-CG_DEV bool reduce_last(double (&v)[NV], double* __restrict__ partials, unsigned* counter,
-                        int nblocks, int64_t bid, double (&out)[NV])
+template <int NT, int NV> CG_DEV void block_sum(double (&v)[NV])
 {
-    __shared__ double sm[NV][NWAVE];
-    __shared__ int last;
+    constexpr int NW = NT / 64;
+    __shared__ double sm[NV][NW];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
         for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
@@ -52,36 +71,27 @@ CG_DEV bool reduce_last(double (&v)[NV], double* __restrict__ partials, unsigned
 #pragma unroll
         for (int k = 0; k < NV; ++k) sm[k][w] = v[k];
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double s = 0.0;
-            for (int q = 0; q < NWAVE; ++q) s += sm[k][q];
-            partials[k * nblocks + bid] = s;
-        }
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        last = prev == (unsigned)(nblocks - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int q = threadIdx.x; q < nblocks; q += BS) s += partials[k * nblocks + q];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if (lane == 0) sm[k][w] = s;
+        double t = 0.0;
+        for (int q = 0; q < NW; ++q) t += sm[k][q];
+        v[k] = t;
     }
     __syncthreads();
+}
+
+// fixed-order sums of the RG partials of NV slots: the same values in every block
+template <int NV>
+CG_DEV void sum_slots(const double* __restrict__ part, const int (&slot)[NV], double (&v)[NV])
+{
+    static_assert(RG % RBS == 0, "partials must split evenly over the block");
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int q = 0; q < NWAVE; ++q) s += sm[k][q];
-        out[k] = s;
+        v[k] = 0.0;
+#pragma unroll
+        for (int q = 0; q < RG / RBS; ++q) v[k] += part[slot[k] * RG + q * RBS + threadIdx.x];
     }
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-    return true;
+    block_sum<RBS, NV>(v);
 }
 
 template <class T> struct HexOp {
@@ -92,204 +102,243 @@ template <class T> struct HexOp {
     T mass;  // c chiCm / dt
 };
 
-template <class T, class TX>
-CG_DEV T apply_row(const HexOp<T>& A, const TX* __restrict__ x, int64_t c, int i, int j, int k,
-                   const HexGeom& g)
+// (A x)_c = mass x_c + sum_f a_f (x_c - x_nb), x read through a loader (plain array, or
+// fp64 x plus fp32 correction).  Branch-free: wall neighbours are clamped to c, so their
+// difference is 0 whatever the coefficient, and all 13 loads of a row issue together.
+template <class T, class LD>
+CG_DEV T apply_row(const HexOp<T>& A, const LD& x, int64_t c, const IJK& q, const HexGeom& g)
 {
-    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    const T xc = T(x[c]);
-    T diag = A.mass, off = T(0);
-    if (i < g.nx - 1) { const T a = A.fx[c]; diag += a; off += a * T(x[c + 1]); }
-    if (i > 0) { const T a = A.fx[c - 1]; diag += a; off += a * T(x[c - 1]); }
-    if (j < g.ny - 1) { const T a = A.fy[c]; diag += a; off += a * T(x[c + sy]); }
-    if (j > 0) { const T a = A.fy[c - sy]; diag += a; off += a * T(x[c - sy]); }
-    if (k < g.nz - 1) { const T a = A.fz[c]; diag += a; off += a * T(x[c + sz]); }
-    if (k > 0) { const T a = A.fz[c - sz]; diag += a; off += a * T(x[c - sz]); }
-    return diag * xc - off;
+    const Nbr b = nbr_of(c, q, g);
+    const T xc = T(x(c));
+    const T fl = A.fx[c] * (xc - T(x(b.xp))) + A.fx[b.xm] * (xc - T(x(b.xm))) +
+                 A.fy[c] * (xc - T(x(b.yp))) + A.fy[b.ym] * (xc - T(x(b.ym))) +
+                 A.fz[c] * (xc - T(x(b.zp))) + A.fz[b.zm] * (xc - T(x(b.zm)));
+    return A.mass * xc + fl;
 }
 
+template <class T> struct PlainLoad {
+    const T* __restrict__ x;
+    CG_DEV T operator()(int64_t c) const { return x[c]; }
+};
+struct CorrectedLoad {  // fp64 x plus the fp32 correction e
+    const double* __restrict__ x;
+    const float* __restrict__ e;
+    CG_DEV double operator()(int64_t c) const { return x[c] + double(e[c]); }
+};
+
+// one cell per lane (elementwise kernels)
 #define CELL_INDEX                                                                         \
-    const int64_t nb_ = gridDim.x;                                                         \
-    const int64_t bid = xcd_block(blockIdx.x, nb_);                                        \
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);                                 \
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;                                         \
-    const int64_t c = bid * BS + threadIdx.x;                                              \
-    const bool act = c < n;                                                                \
-    const int ci = act ? (int)(c % g.nx) : 0;                                              \
-    const int cj = act ? (int)((c / g.nx) % g.ny) : 0;                                     \
-    const int ck = act ? (int)(c / ((int64_t)g.nx * g.ny)) : 0;
+    const int64_t c = (int64_t)bid * BS + threadIdx.x;                                     \
+    if (c >= n) return;                                                                    \
+    const IJK cc = ijk_of((uint32_t)c, g);
+
+// contiguous chunk of cells per block (reducing kernels on RG blocks of RBS threads)
+#define RANGE_LOOP(g)                                                                      \
+    const int64_t n = (int64_t)(g).nx * (g).ny * (g).nz;                                   \
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);                                 \
+    const int64_t chunk_ = ((n + RG - 1) / RG + RBS - 1) / RBS * RBS;                      \
+    const int64_t lo_ = (int64_t)bid * chunk_;                                             \
+    const int64_t hi_ = lo_ + chunk_ < n ? lo_ + chunk_ : n;                               \
+    for (int64_t c = lo_ + threadIdx.x; c < hi_; c += RBS)
 
 // ---- 1/diag for a given mass coefficient
 template <class T>
 __global__ __launch_bounds__(BS) void k_invdiag(HexGeom g, HexOp<T> A, T* __restrict__ invd)
 {
     CELL_INDEX
-    if (!act) return;
-    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    T d = A.mass;
-    if (ci < g.nx - 1) d += A.fx[c];
-    if (ci > 0) d += A.fx[c - 1];
-    if (cj < g.ny - 1) d += A.fy[c];
-    if (cj > 0) d += A.fy[c - sy];
-    if (ck < g.nz - 1) d += A.fz[c];
-    if (ck > 0) d += A.fz[c - sz];
+    const Nbr b = nbr_of(c, cc, g);
+    const T fxm = A.fx[b.xm], fym = A.fy[b.ym], fzm = A.fz[b.zm];
+    const T d = A.mass + ((A.fx[c] + (cc.i > 0 ? fxm : T(0))) +
+                          (A.fy[c] + (cc.j > 0 ? fym : T(0))) +
+                          (A.fz[c] + (cc.k > 0 ? fzm : T(0))));
     invd[c] = T(1) / d;
 }
 
-// ---- ionic step + right-hand side
-template <class M, class T>
-__global__ __launch_bounds__(BS) void k_ionic_rhs(HexGeom g, T* __restrict__ st,
-                                                  const double* __restrict__ Vn,
-                                                  const double* __restrict__ Vm1,
-                                                  double* __restrict__ Iion,
-                                                  double* __restrict__ b, ExplicitStep s,
-                                                  ModelParams P, int tissue)
+// ---- fp64 residual that starts CG solve `gen` in precision TP.
+//   x = xin (+ e when e != null and solve gen-1 did not stop at its outer test);
+//   xout = x when xout != null (xout never aliases xin or e);
+//   b += tensor correction of xin when od != null (first pass of a step);
+//   r = b - A x;  rt = TP(r), ut = M^-1 rt.
+//   partials: (rt,ut) -> G set 0, (rt,rt) -> RR set 0, (r,r) -> RR0, (b,b) -> BB.
+//   Block 0 adds the iterations of the previous solve to the running sum.
+template <class TP, class LD, bool CORR>
+CG_DEV void residual_range(const HexGeom& g, const HexOp<double>& A, const TP* __restrict__ invd_t,
+                           const LD& X, const double* __restrict__ xin, double* __restrict__ xout,
+                           double* __restrict__ b, const double* __restrict__ od,
+                           unsigned od_mask, TP* __restrict__ rt, TP* __restrict__ ut,
+                           double (&v)[4])
 {
-    CELL_INDEX
-    if (!act) return;
-    T p[M::NP];
-#pragma unroll
-    for (int q = 0; q < M::NP; ++q) p[q] = T(P.p[q]);
-    const T one[4] = {T(1), T(1), T(1), T(1)};
-    T y[M::NS];
-#pragma unroll
-    for (int q = 1; q < M::NS; ++q) y[q] = st[q * n + c];
-    const double vn = Vn[c];
-    const T v = T((vn * 1000.0 + M::VOFF) / M::VSCALE);
-    const T I = Ops<M>::tissue_step(v, y, T(s.dt_model), p, tissue, one);
-#pragma unroll
-    for (int q = 1; q < M::NS; ++q) st[q * n + c] = y[q];
-    const double iion = double(I) * M::ISCALE;
-    Iion[c] = iion;
-    double src = 0.0;
+    RANGE_LOOP(g)
     {
-        const double p0 = (ci + 0.5) * g.hx, p1 = (cj + 0.5) * g.hy, p2 = (ck + 0.5) * g.hz;
-        for (int q = 0; q < s.boxes.n; ++q)
-            if (p0 >= s.boxes.lo[q][0] && p0 <= s.boxes.hi[q][0] && p1 >= s.boxes.lo[q][1] &&
-                p1 <= s.boxes.hi[q][1] && p2 >= s.boxes.lo[q][2] && p2 <= s.boxes.hi[q][2])
-                src += s.boxes.intensity[q];
-    }
-    double rhs = s.chiCm / s.dt * (s.c0 * vn - s.c00 * (s.c00 != 0.0 ? Vm1[c] : 0.0)) +
-                 src - s.chiCm * iion;
-    if (s.corr) rhs += s.corr[c];
-    b[c] = rhs;
-}
-
-// ---- fp64 residual r = b - A x, with ||r||^2 and ||b||^2; also starts a CG in TP:
-//      rt = r (TP), p = z = invd rt, rz = r.z.  init_state: set tolerance/counters.
-template <class TP>
-__global__ __launch_bounds__(BS) void k_residual(HexGeom g, HexOp<double> A, HexOp<TP> Ap,
-                                                 const double* __restrict__ x,
-                                                 const double* __restrict__ b,
-                                                 TP* __restrict__ rt, TP* __restrict__ p,
-                                                 TP* __restrict__ xt, double* partials,
-                                                 DevState* S, int which)
-{
-    CELL_INDEX
-    double v[3] = {0.0, 0.0, 0.0};
-    if (act) {
-        const double r = b[c] - apply_row(A, x, c, ci, cj, ck, g);
+        const IJK q = ijk_of((uint32_t)c, g);
+        double bc = b[c];
+        if (CORR) {
+            bc += hex_tensor_correction(xin, od, n, c, q.i, q.j, q.k, g, od_mask);
+            b[c] = bc;
+        }
+        if (xout) xout[c] = X(c);
+        const double r = bc - apply_row(A, X, c, q, g);
         const TP rr = TP(r);
-        const TP z = Ap.invd[c] * rr;
+        const TP u = invd_t[c] * rr;
         rt[c] = rr;
-        p[c] = z;
-        if (xt) xt[c] = TP(0);
-        v[0] = r * r;
-        v[1] = b[c] * b[c];
-        v[2] = double(rr) * double(z);
-    }
-    double out[3];
-    if (reduce_last<3>(v, partials, &S->counter[which], (int)nb_, bid, out) && threadIdx.x == 0) {
-        S->s[S_RR] = out[0];
-        S->s[S_BB] = out[1];
-        S->s[S_RZ] = out[2];
-        S->s[S_R0] = out[0];
+        ut[c] = u;
+        v[0] += double(rr) * double(u);
+        v[1] += double(rr) * double(rr);
+        v[2] += r * r;
+        v[3] += bc * bc;
     }
 }
 
-// ---- q = A p ; alpha = rz / (p.q)
-template <class T>
-__global__ __launch_bounds__(BS) void k_spmv_dot(HexGeom g, HexOp<T> A, const T* __restrict__ p,
-                                                 T* __restrict__ q, double* partials,
-                                                 DevState* S)
+template <class TP>
+__global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
+                                                  const TP* __restrict__ invd_t,
+                                                  const double* __restrict__ xin,
+                                                  const float* __restrict__ e,
+                                                  double* __restrict__ xout,
+                                                  double* __restrict__ b,
+                                                  const double* __restrict__ od, unsigned od_mask,
+                                                  TP* __restrict__ rt, TP* __restrict__ ut,
+                                                  double* __restrict__ part, DevState* S,
+                                                  long long gen)
 {
-    if (S->done) return;
-    CELL_INDEX
+    const bool add_e = e != nullptr && S->outer_gen != gen - 1;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const PlainLoad<double> X0{xin};
+    if (od)
+        residual_range<TP, PlainLoad<double>, true>(g, A, invd_t, X0, xin, xout, b, od, od_mask,
+                                                    rt, ut, v);
+    else if (add_e)
+        residual_range<TP, CorrectedLoad, false>(g, A, invd_t, CorrectedLoad{xin, e}, xin, xout,
+                                                 b, od, od_mask, rt, ut, v);
+    else
+        residual_range<TP, PlainLoad<double>, false>(g, A, invd_t, X0, xin, xout, b, od,
+                                                     od_mask, rt, ut, v);
+    block_sum<RBS, 4>(v);
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    if (threadIdx.x == 0) {
+        part[(P_G + 0) * RG + bid] = v[0];
+        part[(P_RR + 0) * RG + bid] = v[1];
+        part[P_RR0 * RG + bid] = v[2];
+        part[P_BB * RG + bid] = v[3];
+        if (blockIdx.x == 0) {
+            S->iters_sum += S->iters;
+            S->iters = 0;
+        }
+    }
+}
+
+// ---- S-kernel of iteration k of solve gen: convergence test on r_k, then w = A u_k and
+// the partial (w,u).  Converged when (r,r) <= tol2 * ref, ref = (r0,r0) (relative_to_r0)
+// or (b,b); at k == 0 with outer_tol2 > 0 also when the fp64 residual already meets
+// (r0,r0) <= outer_tol2 (b,b) (the refinement is finished).
+template <class T>
+__global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __restrict__ u,
+                                              T* __restrict__ w, double* __restrict__ part,
+                                              DevState* S, long long gen, int k, double tol2,
+                                              int relative_to_r0, double outer_tol2,
+                                              int max_iter)
+{
+    if (S->done_gen == gen) return;
+    double v3[3];
+    sum_slots<3>(part, {P_RR + (k & 1), P_RR0, P_BB}, v3);
+    const double rr = v3[0], rr0 = v3[1], bb = v3[2];
+    const bool outer = k == 0 && outer_tol2 > 0.0 && rr0 <= outer_tol2 * bb;
+    const bool conv = rr <= tol2 * (relative_to_r0 ? rr0 : bb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (k == 0) {
+            S->s[S_R0] = rr0;
+            S->s[S_BB] = bb;
+        }
+        if (outer || conv || k >= max_iter) {
+            S->s[S_RR] = rr;
+            S->iters = k;
+            if (outer) S->outer_gen = gen;
+            S->done_gen = gen;
+        }
+    }
+    if (outer || conv || k >= max_iter) return;
     double v[1] = {0.0};
-    if (act) {
-        const T y = apply_row(A, p, c, ci, cj, ck, g);
-        q[c] = y;
-        v[0] = double(p[c]) * double(y);
+    const PlainLoad<T> U{u};
+    RANGE_LOOP(g)
+    {
+        const IJK ix = ijk_of((uint32_t)c, g);
+        const T y = apply_row(A, U, c, ix, g);
+        w[c] = y;
+        v[0] += double(u[c]) * double(y);
     }
-    double out[1];
-    if (reduce_last<1>(v, partials, &S->counter[0], (int)nb_, bid, out) && threadIdx.x == 0) {
-        S->s[S_PQ] = out[0];
-        S->s[S_ALPHA] = S->s[S_RZ] / out[0];
-    }
+    block_sum<RBS, 1>(v);
+    if (threadIdx.x == 0) part[P_D * RG + bid] = v[0];
 }
 
-// ---- x += alpha p ; r -= alpha q ; z = invd r ; rz' = r.z, rr = r.r ; beta ; convergence
+// ---- U-kernel of iteration k: gamma_k = (r,u), delta_k = (w,u);
+//   beta = gamma_k / gamma_{k-1} (0 at k = 0),
+//   alpha = gamma_k / (delta_k - beta gamma_k / alpha_{k-1})   (gamma/delta at k = 0);
+//   p = u + beta p, s = w + beta s, x += alpha p (x = alpha p at k = 0 when x_from_zero),
+//   r -= alpha s, u = M^-1 r; partials (r,u), (r,r) of iteration k+1.
 template <class T, class TX>
-__global__ __launch_bounds__(BS) void k_update(HexGeom g, const T* __restrict__ invd,
-                                               TX* __restrict__ x, T* __restrict__ r,
-                                               const T* __restrict__ p,
-                                               const T* __restrict__ q, double* partials,
-                                               DevState* S)
+__global__ __launch_bounds__(RBS) void k_cg_u(HexGeom g, const T* __restrict__ invd,
+                                              TX* __restrict__ x, T* __restrict__ r,
+                                              T* __restrict__ u, T* __restrict__ p,
+                                              T* __restrict__ s, const T* __restrict__ w,
+                                              double* __restrict__ part, DevState* S,
+                                              long long gen, int k, int x_from_zero)
 {
-    if (S->done) return;
-    CELL_INDEX
-    const T alpha = T(S->s[S_ALPHA]);
+    if (S->done_gen == gen) return;
+    double v2[2];
+    sum_slots<2>(part, {P_G + (k & 1), P_D}, v2);
+    const double gam = v2[0], del = v2[1];
+    double alpha, beta = 0.0;
+    if (k == 0) {
+        alpha = gam / del;
+    } else {
+        const double gp = S->gamma[(k - 1) & 1], ap = S->alpha[(k - 1) & 1];
+        beta = gam / gp;
+        alpha = gam / (del - beta * gam / ap);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->alpha[k & 1] = alpha;
+        S->gamma[k & 1] = gam;
+        S->iters = k + 1;
+    }
+    const T a = T(alpha), bt = T(beta);
+    const bool first = k == 0;
+    const bool xz = first && x_from_zero;
     double v[2] = {0.0, 0.0};
-    if (act) {
-        x[c] = x[c] + TX(alpha * p[c]);
-        const T rn = r[c] - alpha * q[c];
+    RANGE_LOOP(g)
+    {
+        const T pc = first ? u[c] : u[c] + bt * p[c];
+        const T sc = first ? w[c] : w[c] + bt * s[c];
+        p[c] = pc;
+        s[c] = sc;
+        x[c] = xz ? TX(a * pc) : TX(x[c] + TX(a * pc));
+        const T rn = r[c] - a * sc;
+        const T un = invd[c] * rn;
         r[c] = rn;
-        const T z = invd[c] * rn;
-        v[0] = double(rn) * double(z);
-        v[1] = double(rn) * double(rn);
+        u[c] = un;
+        v[0] += double(rn) * double(un);
+        v[1] += double(rn) * double(rn);
     }
-    double out[2];
-    if (reduce_last<2>(v, partials, &S->counter[1], (int)nb_, bid, out) && threadIdx.x == 0) {
-        S->s[S_BETA] = out[0] / S->s[S_RZ];
-        S->s[S_RZ_OLD] = S->s[S_RZ];
-        S->s[S_RZ] = out[0];
-        S->s[S_RR] = out[1];
-        S->iters += 1;
-        if (out[1] <= S->s[S_TOL2] || S->iters >= S->max_iter) S->done = 1;
+    block_sum<RBS, 2>(v);
+    if (threadIdx.x == 0) {
+        part[(P_G + ((k + 1) & 1)) * RG + bid] = v[0];
+        part[(P_RR + ((k + 1) & 1)) * RG + bid] = v[1];
     }
 }
 
-// ---- p = invd r + beta p
-template <class T>
-__global__ __launch_bounds__(BS) void k_pupdate(HexGeom g, const T* __restrict__ invd,
-                                                const T* __restrict__ r, T* __restrict__ p,
-                                                DevState* S)
-{
-    if (S->done) return;
-    CELL_INDEX
-    if (!act) return;
-    const T beta = T(S->s[S_BETA]);
-    p[c] = invd[c] * r[c] + beta * p[c];
-}
-
-// ---- x64 += e32
-__global__ __launch_bounds__(BS) void k_add_correction(HexGeom g, double* __restrict__ x,
-                                                       const float* __restrict__ e)
-{
-    CELL_INDEX
-    if (!act) return;
-    x[c] += double(e[c]);
-}
-
-// ---- activation time (myocardiumDomain.C:345-374)
+// ---- activation time (myocardiumDomain.C:345-374) and V^{n+1} = x when x is elsewhere
 __global__ __launch_bounds__(BS) void k_activation(HexGeom g, const double* __restrict__ Vn,
-                                                   const double* __restrict__ Vnew,
+                                                   const double* __restrict__ x,
+                                                   double* __restrict__ Vnew,
                                                    uint8_t* __restrict__ flag,
                                                    double* __restrict__ tact, double t_old,
                                                    double dt)
 {
     CELL_INDEX
-    if (!act) return;
-    const double vnew = Vnew[c];
+    (void)cc;
+    const double vnew = x[c];
+    if (x != Vnew) Vnew[c] = vnew;
     if (flag[c] && vnew > 1e-15) {
         const double vn = Vn[c];
         flag[c] = 0;
@@ -297,33 +346,23 @@ __global__ __launch_bounds__(BS) void k_activation(HexGeom g, const double* __re
     }
 }
 
-// ---- explicit Laplacian (operator check): lap = -(K V) + corr
+// ---- explicit Laplacian (operator check)
 __global__ __launch_bounds__(BS) void k_laplacian(HexGeom g, HexCoefs a,
                                                   const double* __restrict__ V,
-                                                  const double* __restrict__ corr,
-                                                  double* __restrict__ out)
+                                                  const double* __restrict__ od,
+                                                  unsigned od_mask, double* __restrict__ out)
 {
     CELL_INDEX
-    if (!act) return;
-    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    const double vc = V[c];
-    double acc = 0.0;
-    const bool u = a.fx == nullptr;
-    if (ci < g.nx - 1) acc += (u ? a.ax : a.fx[c]) * (V[c + 1] - vc);
-    if (ci > 0) acc += (u ? a.ax : a.fx[c - 1]) * (V[c - 1] - vc);
-    if (cj < g.ny - 1) acc += (u ? a.ay : a.fy[c]) * (V[c + sy] - vc);
-    if (cj > 0) acc += (u ? a.ay : a.fy[c - sy]) * (V[c - sy] - vc);
-    if (ck < g.nz - 1) acc += (u ? a.az : a.fz[c]) * (V[c + sz] - vc);
-    if (ck > 0) acc += (u ? a.az : a.fz[c - sz]) * (V[c - sz] - vc);
-    if (corr) acc += corr[c];
+    const double acc0 = a.fx ? hex_laplacian<false>(V, c, cc, g, a) : hex_laplacian<true>(V, c, cc, g, a);
+    double acc = acc0;
+    if (od) acc += hex_tensor_correction(V, od, n, c, cc.i, cc.j, cc.k, g, od_mask);
     out[c] = acc;
 }
 
 const char* kernel_name(int id)
 {
-    static const char* names[K_COUNT] = {"ionic", "spmv_dot", "cg_update", "p_update",
-                                         "residual", "tensor_correction", "activation",
-                                         "invdiag", "add_correction"};
+    static const char* names[K_COUNT] = {"ionic",    "spmv_dot",   "cg_update", "residual_first",
+                                         "residual", "activation", "invdiag"};
     return id >= 0 && id < K_COUNT ? names[id] : "?";
 }
 
@@ -335,14 +374,18 @@ static unsigned grid_of(int64_t n)
 }
 
 hipError_t launch_hex_laplacian(const HexGeom& g, const HexCoefs& k, const double* V,
-                                const double* corr, double* out, hipStream_t s)
+                                const double* od, unsigned od_mask, double* out, hipStream_t s)
 {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_laplacian, dim3(grid_of(n)), dim3(BS), 0, s, g, k, V, corr, out);
+    hipLaunchKernelGGL(k_laplacian, dim3(grid_of(n)), dim3(BS), 0, s, g, k, V, od, od_mask, out);
     return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ solver object
+template <class T> struct CgVectors {
+    T *r = nullptr, *u = nullptr, *p = nullptr, *s = nullptr, *w = nullptr;
+};
+
 struct ImplicitSolver {
     HexGeom g;
     int64_t n;
@@ -351,15 +394,22 @@ struct ImplicitSolver {
     hipStream_t stream;
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *invd = nullptr;  // fp64 operator
     float *fx32 = nullptr, *fy32 = nullptr, *fz32 = nullptr, *invd32 = nullptr;
-    double *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;        // fp64 CG
-    float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *e32 = nullptr; // fp32 inner CG
+    double* od = nullptr;                                                  // sigma xy xz yz
+    unsigned od_mask = 0;
+    double* b = nullptr;
+    double* xalt = nullptr;  // second fp64 solution buffer of the refinement (mixed)
+    CgVectors<double> v64;
+    CgVectors<float> v32;
+    float* e32 = nullptr;
     double* partials = nullptr;
     DevState* S = nullptr;
     DevState* hS = nullptr;  // pinned mirror
     double mass_c = -1.0;    // coefft of the current invd
+    long long gen = 0;       // generation of the last CG solve started
     int64_t total_iters = 0;
     double last_res = 0.0;
     int last_inner = 16;
+    long long iters_sum_seen = 0;  // DevState::iters_sum at the previous read
 };
 
 template <class T> static HexOp<T> op_of(const ImplicitSolver* S, double coefft);
@@ -380,26 +430,37 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
     S->dt = dt; S->stream = s;
     const int64_t n = S->n;
     const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    std::vector<double> fx(n, 0.0), fy(n, 0.0), fz(n, 0.0);
+    std::vector<double> fx(n, 0.0), fy(n, 0.0), fz(n, 0.0), od(3 * n, 0.0);
     for (int64_t c = 0; c < n; ++c) {
         const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / sz);
         if (i < g.nx - 1) fx[c] = 0.5 * (sig[9 * c + 0] + sig[9 * (c + 1) + 0]) / (g.hx * g.hx);
         if (j < g.ny - 1) fy[c] = 0.5 * (sig[9 * c + 4] + sig[9 * (c + sy) + 4]) / (g.hy * g.hy);
         if (k < g.nz - 1) fz[c] = 0.5 * (sig[9 * c + 8] + sig[9 * (c + sz) + 8]) / (g.hz * g.hz);
+        od[c] = sig[9 * c + 1];
+        od[n + c] = sig[9 * c + 2];
+        od[2 * n + c] = sig[9 * c + 5];
+        for (int q = 0; q < 3; ++q)
+            if (od[q * n + c] != 0.0) S->od_mask |= 1u << q;
     }
-    const int64_t nblk = grid_of(n);
+    const bool offdiag = S->od_mask != 0;
     bool ok = true;
     auto A = [&](void** ptr, size_t bytes) { ok &= hipMalloc(ptr, bytes) == hipSuccess; };
     A((void**)&S->fx, 8 * n); A((void**)&S->fy, 8 * n); A((void**)&S->fz, 8 * n);
     A((void**)&S->invd, 8 * n);
-    A((void**)&S->b, 8 * n); A((void**)&S->r, 8 * n); A((void**)&S->p, 8 * n);
-    A((void**)&S->q, 8 * n);
-    A((void**)&S->partials, 8 * 4 * nblk);
+    A((void**)&S->b, 8 * n);
+    A((void**)&S->partials, 8 * RG * P_NSLOT);
     A((void**)&S->S, sizeof(DevState));
+    if (offdiag) A((void**)&S->od, 8 * 3 * n);
     if (prec == 1) {
         A((void**)&S->fx32, 4 * n); A((void**)&S->fy32, 4 * n); A((void**)&S->fz32, 4 * n);
-        A((void**)&S->invd32, 4 * n); A((void**)&S->r32, 4 * n); A((void**)&S->p32, 4 * n);
-        A((void**)&S->q32, 4 * n); A((void**)&S->e32, 4 * n);
+        A((void**)&S->invd32, 4 * n);
+        A((void**)&S->v32.r, 4 * n); A((void**)&S->v32.u, 4 * n); A((void**)&S->v32.p, 4 * n);
+        A((void**)&S->v32.s, 4 * n); A((void**)&S->v32.w, 4 * n);
+        A((void**)&S->e32, 4 * n);
+        A((void**)&S->xalt, 8 * n);
+    } else {
+        A((void**)&S->v64.r, 8 * n); A((void**)&S->v64.u, 8 * n); A((void**)&S->v64.p, 8 * n);
+        A((void**)&S->v64.s, 8 * n); A((void**)&S->v64.w, 8 * n);
     }
     ok &= hipHostMalloc((void**)&S->hS, sizeof(DevState), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
@@ -410,6 +471,7 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
     hipMemcpy(S->fx, fx.data(), 8 * n, hipMemcpyHostToDevice);
     hipMemcpy(S->fy, fy.data(), 8 * n, hipMemcpyHostToDevice);
     hipMemcpy(S->fz, fz.data(), 8 * n, hipMemcpyHostToDevice);
+    if (offdiag) hipMemcpy(S->od, od.data(), 8 * 3 * n, hipMemcpyHostToDevice);
     if (prec == 1) {
         std::vector<float> f(n);
         for (int64_t c = 0; c < n; ++c) f[c] = (float)fx[c];
@@ -420,6 +482,8 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
         hipMemcpy(S->fz32, f.data(), 4 * n, hipMemcpyHostToDevice);
     }
     DevState z{};
+    z.done_gen = -1;
+    z.outer_gen = -1;
     hipMemcpy(S->S, &z, sizeof z, hipMemcpyHostToDevice);
     *out = S;
     return 0;
@@ -428,8 +492,10 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
 void implicit_destroy(ImplicitSolver* S)
 {
     if (!S) return;
-    void* ptrs[] = {S->fx, S->fy, S->fz, S->invd, S->b, S->r, S->p, S->q, S->partials, S->S,
-                    S->fx32, S->fy32, S->fz32, S->invd32, S->r32, S->p32, S->q32, S->e32};
+    void* ptrs[] = {S->fx,     S->fy,     S->fz,     S->invd,     S->b,        S->xalt,
+                    S->partials, S->S,    S->od,     S->fx32,     S->fy32,     S->fz32,
+                    S->invd32, S->v32.r,  S->v32.u,  S->v32.p,    S->v32.s,    S->v32.w,
+                    S->e32,    S->v64.r,  S->v64.u,  S->v64.p,    S->v64.s,    S->v64.w};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (S->hS) hipHostFree(S->hS);
@@ -442,85 +508,67 @@ void implicit_stats(const ImplicitSolver* S, int64_t* iters, double* res)
     if (res) *res = S->last_res;
 }
 
-// set scalars of a fresh CG: tolerance^2 on ||r||^2, iteration budget
-__global__ void k_cg_begin(DevState* S, double tol, int max_iter, int relative_to_r0)
-{
-    const double ref = relative_to_r0 ? S->s[S_R0] : S->s[S_BB];
-    S->s[S_TOL2] = tol * tol * ref;
-    S->iters = 0;
-    S->max_iter = max_iter;
-    S->done = S->s[S_RR] <= S->s[S_TOL2] ? 1 : 0;
-}
+namespace {
 
-template <class T, class TX>
-static hipError_t cg_iterations(ImplicitSolver* S, const HexOp<T>& A, TX* x, T* r, T* p, T* q,
-                                int count, Profiler* prof, std::vector<size_t>* marks)
-{
-    const unsigned grid = grid_of(S->n);
-    for (int it = 0; it < count; ++it) {
-        if (marks) marks->push_back(prof->mark());
-        int tk = prof ? prof->begin(K_SPMV) : -1;
-        hipLaunchKernelGGL((k_spmv_dot<T>), dim3(grid), dim3(BS), 0, S->stream, S->g, A, p, q,
-                           S->partials, S->S);
-        if (prof) { prof->end(tk); tk = prof->begin(K_UPDATE); }
-        hipLaunchKernelGGL((k_update<T, TX>), dim3(grid), dim3(BS), 0, S->stream, S->g, A.invd,
-                           x, r, p, q, S->partials, S->S);
-        if (prof) { prof->end(tk); tk = prof->begin(K_PUPDATE); }
-        hipLaunchKernelGGL((k_pupdate<T>), dim3(grid), dim3(BS), 0, S->stream, S->g, A.invd, r,
-                           p, S->S);
+// launch bookkeeping for the per-kernel timings: launches that exited on a finished
+// solve are dropped once the host knows how many iterations ran
+struct CgLaunch {
+    size_t rec;
+    long long gen;
+    int k;
+    bool s_kernel;
+};
+
+template <class T> struct CgRun {
+    ImplicitSolver* S;
+    HexOp<T> A;
+    CgVectors<T> v;
+    Profiler* prof;
+    std::vector<CgLaunch>* log;
+
+    void s_kernel(long long gen, int k, double tol2, int rel, double outer_tol2, int max_iter)
+    {
+        const int tk = prof ? prof->begin(K_SPMV) : -1;
+        hipLaunchKernelGGL((k_cg_s<T>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,
+                           (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel, outer_tol2,
+                           max_iter);
+        if (prof) {
+            prof->end(tk);
+            log->push_back({(size_t)tk, gen, k, true});
+        }
+    }
+    template <class TX> void u_kernel(long long gen, int k, TX* x, int x_from_zero)
+    {
+        const int tk = prof ? prof->begin(K_UPDATE) : -1;
+        hipLaunchKernelGGL((k_cg_u<T, TX>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A.invd, x,
+                           v.r, v.u, v.p, v.s, (const T*)v.w, S->partials, S->S, gen, k,
+                           x_from_zero);
+        if (prof) {
+            prof->end(tk);
+            log->push_back({(size_t)tk, gen, k, false});
+        }
+    }
+    void residual(int kid, long long gen, const double* xin, const float* e, double* xout,
+                  const double* od, const HexOp<double>& A64)
+    {
+        const int tk = prof ? prof->begin(kid) : -1;
+        hipLaunchKernelGGL((k_residual<T>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A64,
+                           (const T*)A.invd, xin, e, xout, S->b, od, S->od_mask, v.r, v.u,
+                           S->partials, S->S, gen);
         if (prof) prof->end(tk);
     }
-    return hipGetLastError();
+};
+
+// drop the timing records of launches of solve `gen` beyond its `iters` iterations
+// (S-kernels with k >= iters only tested convergence, U-kernels with k >= iters did nothing)
+void drop_idle(Profiler* prof, std::vector<CgLaunch>& log, long long gen, int iters)
+{
+    if (!prof) return;
+    for (auto& l : log)
+        if (l.gen == gen && l.k >= iters) prof->invalidate_range(l.rec, l.rec + 1);
 }
 
-// runs CG chunks until the device flag says done; returns iterations used
-template <class T, class TX>
-static int cg_solve(ImplicitSolver* S, const HexOp<T>& A, TX* x, T* r, T* p, T* q, int max_iter,
-                    Profiler* prof, std::string* err)
-{
-    int chunk = S->last_inner > 2 ? S->last_inner : 2;
-    int launched = 0;
-    int done_before = 0;
-    std::vector<size_t> marks;
-    while (true) {
-        if (launched + chunk > max_iter) chunk = max_iter - launched;
-        marks.clear();
-        if (chunk > 0) {
-            if (cg_iterations<T, TX>(S, A, x, r, p, q, chunk, prof, prof ? &marks : nullptr) !=
-                hipSuccess) {
-                *err = "CG kernel launch failed";
-                return -1;
-            }
-            launched += chunk;
-        }
-        hipMemcpyAsync(S->hS, S->S, sizeof(DevState), hipMemcpyDeviceToHost, S->stream);
-        if (hipStreamSynchronize(S->stream) != hipSuccess) {
-            *err = "CG synchronisation failed";
-            return -1;
-        }
-        if (prof) {
-            // iterations launched after convergence exit at once and are not timed
-            const size_t useful = (size_t)(S->hS->iters - done_before);
-            if (useful < marks.size()) prof->invalidate_from(marks[useful]);
-            prof->resolve(false);
-        }
-        done_before = S->hS->iters;
-        if (S->hS->done || launched >= max_iter) break;
-        chunk = 4;
-    }
-    S->last_inner = S->hS->iters;
-    return S->hS->iters;
-}
-
-template <class M, class T>
-static hipError_t ionic_rhs_t(ImplicitSolver* S, int tissue, void* st, const double* Vn,
-                              const double* Vm1, double* Iion, const ExplicitStep& s,
-                              const ModelParams& P)
-{
-    hipLaunchKernelGGL((k_ionic_rhs<M, T>), dim3(grid_of(S->n)), dim3(BS), 0, S->stream, S->g,
-                       (T*)st, Vn, Vm1, Iion, S->b, s, P, tissue);
-    return hipGetLastError();
-}
+} // namespace
 
 int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* states,
                   const ModelParams& P, const double* Vn, const double* Vm1, double* Vnew,
@@ -528,19 +576,9 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
                   int max_iter, Profiler* prof, std::string* err)
 {
     const unsigned grid = grid_of(S->n);
-    hipError_t e;
     int tk = prof ? prof->begin(K_IONIC) : -1;
-    switch (model * 2 + prec) {
-    case 0: e = ionic_rhs_t<BuenoOrovio, double>(S, tissue, states, Vn, Vm1, Iion, st, P); break;
-    case 1: e = ionic_rhs_t<BuenoOrovio, float>(S, tissue, states, Vn, Vm1, Iion, st, P); break;
-    case 2: e = ionic_rhs_t<TNNP, double>(S, tissue, states, Vn, Vm1, Iion, st, P); break;
-    case 3: e = ionic_rhs_t<TNNP, float>(S, tissue, states, Vn, Vm1, Iion, st, P); break;
-    case 4: e = ionic_rhs_t<ORd, double>(S, tissue, states, Vn, Vm1, Iion, st, P); break;
-    case 5: e = ionic_rhs_t<ORd, float>(S, tissue, states, Vn, Vm1, Iion, st, P); break;
-    case 6: e = ionic_rhs_t<AlievPanfilov, double>(S, tissue, states, Vn, Vm1, Iion, st, P); break;
-    case 7: e = ionic_rhs_t<AlievPanfilov, float>(S, tissue, states, Vn, Vm1, Iion, st, P); break;
-    default: e = hipErrorInvalidValue;
-    }
+    hipError_t e = launch_ionic_rhs(model, prec, tissue, S->g, states, Vn, Vm1, nullptr, Iion,
+                                    S->b, st, P, S->stream);
     if (prof) prof->end(tk);
     if (e != hipSuccess) { *err = std::string("ionic/rhs launch: ") + hipGetErrorString(e); return 4; }
     // 1/diag for this step's ddt coefficient
@@ -553,61 +591,95 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
         }
         S->mass_c = st.c;
     }
-    hipMemcpyAsync(Vnew, Vn, 8 * S->n, hipMemcpyDeviceToDevice, S->stream);
     const HexOp<double> A = op_of<double>(S, st.c);
-    int used = 0;
+    const double inner_tol = 1e-4;
+    std::vector<CgLaunch> log;
+    double* x_final = Vnew;
+    int used_total = 0;
+
+    auto sync_state = [&]() -> bool {
+        hipMemcpyAsync(S->hS, S->S, sizeof(DevState), hipMemcpyDeviceToHost, S->stream);
+        return hipStreamSynchronize(S->stream) == hipSuccess;
+    };
+
     if (prec == 0) {
-        tk = prof ? prof->begin(K_RESIDUAL) : -1;
-        hipLaunchKernelGGL((k_residual<double>), dim3(grid), dim3(BS), 0, S->stream, S->g, A, A,
-                           (const double*)Vnew, (const double*)S->b, S->r, S->p, (double*)nullptr,
-                           S->partials, S->S, 2);
-        if (prof) prof->end(tk);
-        hipLaunchKernelGGL(k_cg_begin, dim3(1), dim3(1), 0, S->stream, S->S, tol, max_iter, 0);
-        used = cg_solve<double, double>(S, A, Vnew, S->r, S->p, S->q, max_iter, prof, err);
-        if (used < 0) return 4;
-        S->total_iters += used;
-        S->last_res = std::sqrt(S->hS->s[S_RR] / (S->hS->s[S_BB] > 0 ? S->hS->s[S_BB] : 1.0));
-        if (!S->hS->done || S->hS->s[S_RR] > S->hS->s[S_TOL2]) {
-            if (used >= max_iter) { *err = "PCG did not converge within max_iter"; return 7; }
+        // one fp64 CG on the system, x0 = V^n copied into V^{n+1} by the first residual
+        CgRun<double> cg{S, A, S->v64, prof, &log};
+        const long long gen = ++S->gen;
+        cg.residual(K_RESIDUAL0, gen, Vn, nullptr, Vnew, S->od, A);
+        cg.s_kernel(gen, 0, tol * tol, 0, 0.0, max_iter);
+        int k = 0;
+        int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
+        for (;;) {
+            for (int q = 0; q < chunk; ++q, ++k) {
+                cg.u_kernel(gen, k, Vnew, 0);
+                cg.s_kernel(gen, k + 1, tol * tol, 0, 0.0, max_iter);
+            }
+            if (hipGetLastError() != hipSuccess) { *err = "CG launch failed"; return 4; }
+            if (!sync_state()) { *err = "synchronisation failed"; return 4; }
+            const DevState& h = *S->hS;
+            if (h.done_gen == gen) {
+                drop_idle(prof, log, gen, h.iters);
+                if (prof) prof->resolve(false);
+                used_total = h.iters;
+                S->last_inner = h.iters;
+                const double bb = h.s[S_BB] > 0 ? h.s[S_BB] : 1.0;
+                S->last_res = std::sqrt(h.s[S_RR] / bb);
+                if (h.s[S_RR] > tol * tol * h.s[S_BB]) {
+                    *err = "PCG did not converge within max_iter";
+                    return 7;
+                }
+                break;
+            }
+            chunk = 4;  // not finished: continue the same solve
         }
     } else {
-        const HexOp<float> A32 = op_of<float>(S, st.c);
-        int outer = 0;
-        int total = 0;
-        const double inner_tol = 1e-4;
-        while (true) {
-            // fp64 residual; starts the fp32 correction solve from e = 0
-            tk = prof ? prof->begin(K_RESIDUAL) : -1;
-            hipLaunchKernelGGL((k_residual<float>), dim3(grid), dim3(BS), 0, S->stream, S->g, A,
-                               A32, (const double*)Vnew, (const double*)S->b, S->r32, S->p32,
-                               S->e32, S->partials, S->S, 2);
-            if (prof) prof->end(tk);
-            hipMemcpyAsync(S->hS, S->S, sizeof(DevState), hipMemcpyDeviceToHost, S->stream);
-            hipStreamSynchronize(S->stream);
-            const double rr = S->hS->s[S_RR], bb = S->hS->s[S_BB] > 0 ? S->hS->s[S_BB] : 1.0;
-            S->last_res = std::sqrt(rr / bb);
-            if (rr <= tol * tol * bb) break;
-            if (total >= max_iter || outer >= 20) {
+        // mixed: refinement passes; x lives in V^n (pass 0), then alternates Vnew / xalt
+        CgRun<float> cg{S, op_of<float>(S, st.c), S->v32, prof, &log};
+        long long gen = ++S->gen;
+        cg.residual(K_RESIDUAL0, gen, Vn, nullptr, nullptr, S->od, A);
+        cg.s_kernel(gen, 0, inner_tol * inner_tol, 1, tol * tol, max_iter);
+        const double* cur = Vn;
+        for (int pass = 0;; ++pass) {
+            const int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
+            for (int k = 0; k < chunk; ++k) {
+                cg.u_kernel(gen, k, S->e32, 1);
+                cg.s_kernel(gen, k + 1, inner_tol * inner_tol, 1, tol * tol, max_iter);
+            }
+            // x' = x + e into the other buffer, fp64 residual, start of the next solve
+            double* nxt = cur == Vnew ? S->xalt : Vnew;
+            const long long ngen = ++S->gen;
+            cg.residual(K_RESIDUAL, ngen, cur, S->e32, nxt, nullptr, A);
+            cg.s_kernel(ngen, 0, inner_tol * inner_tol, 1, tol * tol, max_iter);
+            cur = nxt;
+            if (hipGetLastError() != hipSuccess) { *err = "CG launch failed"; return 4; }
+            if (!sync_state()) { *err = "synchronisation failed"; return 4; }
+            const DevState& h = *S->hS;
+            // iterations of solve gen, folded into the running sum by the residual pass
+            const int run = (int)(h.iters_sum - S->iters_sum_seen);
+            S->iters_sum_seen = h.iters_sum;
+            drop_idle(prof, log, gen, run);
+            const bool finished = h.outer_gen == ngen;
+            if (finished) drop_idle(prof, log, ngen, 0);
+            if (prof) prof->resolve(false);
+            log.clear();
+            used_total += run;
+            if (run > 0) S->last_inner = run;
+            const double bb = h.s[S_BB] > 0 ? h.s[S_BB] : 1.0;
+            S->last_res = std::sqrt(h.s[S_R0] / bb);
+            if (finished) break;
+            if (used_total >= max_iter || pass >= 50) {
                 *err = "mixed-precision refinement did not converge";
                 return 7;
             }
-            hipLaunchKernelGGL(k_cg_begin, dim3(1), dim3(1), 0, S->stream, S->S, inner_tol,
-                               max_iter - total, 1);
-            used = cg_solve<float, float>(S, A32, S->e32, S->r32, S->p32, S->q32, max_iter - total,
-                                          prof, err);
-            if (used < 0) return 4;
-            total += used;
-            tk = prof ? prof->begin(K_ADDCORR) : -1;
-            hipLaunchKernelGGL(k_add_correction, dim3(grid), dim3(BS), 0, S->stream, S->g, Vnew,
-                               (const float*)S->e32);
-            if (prof) prof->end(tk);
-            ++outer;
+            gen = ngen;
         }
-        S->total_iters += total;
+        x_final = const_cast<double*>(cur);
     }
+    S->total_iters += used_total;
     tk = prof ? prof->begin(K_ACTIVATION) : -1;
     hipLaunchKernelGGL(k_activation, dim3(grid), dim3(BS), 0, S->stream, S->g, Vn,
-                       (const double*)Vnew, flag, tact, st.t_old, st.dt);
+                       (const double*)x_final, Vnew, flag, tact, st.t_old, st.dt);
     if (prof) prof->end(tk);
     e = hipGetLastError();
     if (e != hipSuccess) { *err = hipGetErrorString(e); return 4; }
