@@ -82,6 +82,34 @@ def coeffs_for(w, sigma):
                                  "stimulusStartTime": 0.0}}
 
 
+# kernel symbol of each timing bucket, as the counter runs name it
+def kernel_symbol(kernel, prec, model, algorithm):
+    t = "float" if prec == "mixed" else "double"
+    if kernel == "ionic":
+        if algorithm == "implicit":
+            return "cg::k_ionic_rhs<cg::%s, %s>" % (model, t)
+        return None
+    return {"spmv_dot": "cg::k_cg_s<%s>" % t, "cg_update": "cg::k_cg_u<%s, %s>" % (t, t),
+            "activation": "cg::k_activation"}.get(kernel)
+
+
+def measured_traffic(workload, prec, symbol):
+    """HBM bytes per launch of `symbol` from the committed FETCH_SIZE/WRITE_SIZE summary of
+    this workload (profiles/tissue_step_<workload>_hbm_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "tissue_step_%s_hbm_traffic.md" % workload)
+    if symbol is None or not os.path.exists(path):
+        return None
+    text = open(path).read()
+    try:
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        rec = json.loads(block).get("%s/%s" % (prec, symbol))
+    except (IndexError, ValueError):
+        return None
+    if not rec:
+        return None
+    return rec["fetch_bytes"] + rec["write_bytes"]
+
+
 def run_precision(cf, torch, w, sigma, prec, steps, warmup, prof_steps):
     """warmup steps, `steps` timed steps (no per-kernel events in the timed region), then
     `prof_steps` more steps with HIP-event timing of every kernel on the library stream."""
@@ -114,7 +142,8 @@ def cpu_baseline(w, sigma, sample_steps):
     mesh for `sample_steps` steps after a 2-step start, timed as cell-steps/s."""
     import oracle
     nx, ny, nz = w["shape"]
-    cores = os.cpu_count() or 1
+    # the host share of one GPU: OMP_NUM_THREADS when set, else every core
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     model = {"ORd": oracle.ORD, "BuenoOrovio": oracle.BO}[w["model"]]
     tissue = {"myocyte": oracle.MYOCYTE, "epicardialCells": oracle.EPI}[w["tissue"]]
     T = oracle.OracleTissue(nx, ny, nz, w["h"], sigma, model, tissue, w["dt"], 1.4e5, 0.01,
@@ -149,8 +178,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="ord_implicit_2M", choices=sorted(WORKLOADS))
     ap.add_argument("--prof-steps", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both",
+                    help="run one precision only (profiling runs; no error gate)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 10 times (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the tissue step is not sharded across GPUs in this "
@@ -161,12 +194,27 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nbytes, "launches": 10}))
+        return
     import cardiacfoam_amd as cf
 
     w = WORKLOADS[args.workload]
     sigma = make_sigma(w["shape"], w["rotating"])
     n = int(np.prod(w["shape"]))
     res = {}
+    if args.only != "both":
+        r = run_precision(cf, torch, w, sigma, args.only, args.steps, args.warmup,
+                          args.prof_steps)
+        print(json.dumps({"only": args.only, "workload": args.workload, "steps": args.steps,
+                          "ms_per_step": r["elapsed"] / args.steps * 1e3}))
+        return
     for prec in ("fp64", "mixed"):
         res[prec] = run_precision(cf, torch, w, sigma, prec, args.steps, args.warmup,
                                   args.prof_steps)
@@ -200,8 +248,10 @@ def main():
         bpc = algo_bytes_per_cell(dom_k, "mixed", w["model"], w["algorithm"])
         avg_s = ms / cnt * 1e-3
         ach = bpc * n / avg_s / 1e9
-        roof = {"bound": "hbm", "kernel": dom_k, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+        sym = kernel_symbol(dom_k, "mixed", w["model"], w["algorithm"])
+        roof = {"bound": "hbm", "kernel": sym or dom_k, "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.workload, "mixed", sym),
                 "bytes_per_launch": bpc * n, "avg_launch_us": round(avg_s * 1e6, 2)}
 
     cpu = None
