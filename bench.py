@@ -137,6 +137,71 @@ def run_precision(cf, torch, w, sigma, prec, steps, warmup, prof_steps):
     return dict(dom=dom, mesh=mesh, elapsed=elapsed, kernels=kt, iters=it1 - it0, V=V, act=act)
 
 
+def measure_spmv(torch, dom, n, prec, launches=200):
+    """The implicit operator y = A x alone (7-point stencil, one row per lane), timed with
+    HIP events on the library stream over `launches` back-to-back applications."""
+    dtype = torch.float64 if prec == "fp64" else torch.float32
+    x = torch.randn(n, dtype=dtype, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(10):
+        dom.apply_operator(x, y, 1.0, prec)
+    dom.sync()
+    dom.set_profiling(True)
+    for _ in range(launches):
+        dom.apply_operator(x, y, 1.0, prec)
+    dom.sync()
+    ms, cnt = dom.kernel_times()["apply"]
+    dom.set_profiling(False)
+    w = 8 if prec == "fp64" else 4
+    bpr = 5 * w  # x, three face planes, y; neighbour rows from cache
+    avg = ms / cnt * 1e-3
+    gbs = bpr * n / avg / 1e9
+    return {"avg_us": round(avg * 1e6, 2), "bytes_per_row": bpr, "GBps": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "rows": n}
+
+
+def run_explicit_c0(cf, torch, steps=20000, warmup=200):
+    """Plain-method workload: BuenoOrovio epi on 128x48x24 (h 0.15 mm), explicit Euler
+    diffusion with Rush-Larsen gates, dt 0.005 ms, 100 ms; FP64 and mixed from rest."""
+    w = WORKLOADS["bo_explicit_147k"]
+    sigma = make_sigma(w["shape"], False)
+    n = int(np.prod(w["shape"]))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        mesh = cf.HexMesh(*w["shape"], w["h"])
+        dom = cf.Monodomain(mesh, coeffs_for(w, sigma), dt=w["dt"], precision=prec)
+        dom.step(warmup)
+        dom.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dom.step(steps - warmup)
+        dom.sync()
+        el = time.perf_counter() - t0
+        dom.set_profiling(True)
+        dom.step(50)
+        dom.sync()
+        ms, cnt = dom.kernel_times()["ionic"]
+        dom.set_profiling(False)
+        avg = ms / cnt * 1e-3
+        bpc = algo_bytes_per_cell("ionic", prec, w["model"], "explicit")
+        out[prec] = {"cell_steps_per_s": n * (steps - warmup) / el,
+                     "ms_per_step": el / (steps - warmup) * 1e3,
+                     "fused_step_kernel_us": round(avg * 1e6, 2),
+                     "fused_step_GBps": round(bpc * n / avg / 1e9, 1),
+                     "fused_step_bytes_per_cell": bpc,
+                     "V": dom.Vm, "act": dom.activationTime}
+    vm_err = float(np.max(np.abs(out["fp64"]["V"] - out["mixed"]["V"])) * 1e3)
+    a64, a32 = out["fp64"]["act"], out["mixed"]["act"]
+    both = (a64 > 0) & (a32 > 0)
+    act_err = float(np.max(np.abs(a64[both] - a32[both])) * 1e3) if both.any() else 0.0
+    for p in out:
+        del out[p]["V"], out[p]["act"]
+    return {"workload": "bo_explicit_147k", "cells": n, "steps": steps,
+            "simulated_ms": steps * w["dt"] * 1e3, **out,
+            "mixed_vm_linf_mV": vm_err, "mixed_activation_linf_ms": act_err,
+            "activated_cells": int(np.count_nonzero(a64 > 0))}
+
+
 def cpu_baseline(w, sigma, sample_steps):
     """The CPU FP64 oracle (OpenMP over all host cores) on the same workload: the full
     mesh for `sample_steps` steps after a 2-step start, timed as cell-steps/s."""
@@ -254,6 +319,10 @@ def main():
                 "traffic": measured_traffic(args.workload, "mixed", sym),
                 "bytes_per_launch": bpc * n, "avg_launch_us": round(avg_s * 1e6, 2)}
 
+    spmv = {p: measure_spmv(torch, res[p]["dom"], n, p) for p in ("fp64", "mixed")} \
+        if w["algorithm"] == "implicit" else None
+    c0 = run_explicit_c0(cf, torch) if args.workload != "bo_explicit_147k" else None
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -281,6 +350,8 @@ def main():
         "cg_iterations_per_step": {p: res[p]["iters"] / args.steps for p in res},
         "roofline": roof,
         "kernels": kernels_out,
+        "spmv": spmv,
+        "explicit_c0": c0,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

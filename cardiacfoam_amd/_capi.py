@@ -86,6 +86,7 @@ def lib():
     L.cg_tissue_set_profiling.argtypes = [vp, C.c_int]
     L.cg_tissue_kernel_times.argtypes = [vp, C.c_int, C.c_char_p, dp, C.POINTER(C.c_int64),
                                          C.POINTER(C.c_int)]
+    L.cg_tissue_apply_operator.argtypes = [vp, C.c_double, C.c_int, vp, vp]
     _lib = L
     return L
 

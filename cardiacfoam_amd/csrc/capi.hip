@@ -717,4 +717,19 @@ int cg_tissue_solver_stats(const cg_tissue* t, int64_t* iters, double* res)
     return CG_OK;
 }
 
+int cg_tissue_apply_operator(cg_tissue* t, double ddt_coeff, int precision, const void* x_dev,
+                             void* y_dev)
+{
+    if (!t || !x_dev || !y_dev) return fail(CG_ERR_ARG, "NULL argument");
+    if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no implicit operator");
+    if (precision != CG_FP64 && precision != CG_MIXED)
+        return fail(CG_ERR_ARG, "unknown precision %d", precision);
+    if (!(ddt_coeff >= 0)) return fail(CG_ERR_ARG, "ddt coefficient must be >= 0");
+    cg::Profiler* prof = t->prof.enabled ? &t->prof : nullptr;
+    const int rc = cg::implicit_apply(t->implicit, precision, ddt_coeff, x_dev, y_dev, prof);
+    if (rc == 9) return fail(CG_ERR_STATE, "fp32 operator exists on a mixed-precision tissue only");
+    if (rc != 0) return fail(CG_ERR_HIP, "operator launch failed");
+    return CG_OK;
+}
+
 } // extern "C"

@@ -18,6 +18,10 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
                   double* Iion, uint8_t* flag, double* tact, const ExplicitStep& st, double tol,
                   int max_iter, Profiler* prof, std::string* err);
 void implicit_stats(const ImplicitSolver* S, int64_t* iters, double* res);
+// y = A x, A = coefft chiCm/dt I + orthogonal Laplacian matrix; prec 0: double, 1: float
+// (device pointers).  Returns 0, 9 when the fp32 operator was not built, 4 on HIP error.
+int implicit_apply(ImplicitSolver* S, int prec, double coefft, const void* x, void* y,
+                   Profiler* prof);
 
 // standalone pieces used by tests through the C-ABI
 hipError_t launch_hex_laplacian(const HexGeom& g, const HexCoefs& k, const double* V,

@@ -346,6 +346,15 @@ __global__ __launch_bounds__(BS) void k_activation(HexGeom g, const double* __re
     }
 }
 
+// ---- y = A x with the implicit operator (one row per lane, no reduction)
+template <class T>
+__global__ __launch_bounds__(BS) void k_apply(HexGeom g, HexOp<T> A, const T* __restrict__ x,
+                                              T* __restrict__ y)
+{
+    CELL_INDEX
+    y[c] = apply_row(A, PlainLoad<T>{x}, c, cc, g);
+}
+
 // ---- explicit Laplacian (operator check)
 __global__ __launch_bounds__(BS) void k_laplacian(HexGeom g, HexCoefs a,
                                                   const double* __restrict__ V,
@@ -362,7 +371,7 @@ __global__ __launch_bounds__(BS) void k_laplacian(HexGeom g, HexCoefs a,
 const char* kernel_name(int id)
 {
     static const char* names[K_COUNT] = {"ionic",    "spmv_dot",   "cg_update", "residual_first",
-                                         "residual", "activation", "invdiag"};
+                                         "residual", "activation", "invdiag",    "apply"};
     return id >= 0 && id < K_COUNT ? names[id] : "?";
 }
 
@@ -506,6 +515,21 @@ void implicit_stats(const ImplicitSolver* S, int64_t* iters, double* res)
 {
     if (iters) *iters = S->total_iters;
     if (res) *res = S->last_res;
+}
+
+int implicit_apply(ImplicitSolver* S, int prec, double coefft, const void* x, void* y,
+                   Profiler* prof)
+{
+    if (prec == 1 && !S->fx32) return 9;
+    const int tk = prof ? prof->begin(K_APPLY) : -1;
+    if (prec == 0)
+        hipLaunchKernelGGL((k_apply<double>), dim3(grid_of(S->n)), dim3(BS), 0, S->stream, S->g,
+                           op_of<double>(S, coefft), (const double*)x, (double*)y);
+    else
+        hipLaunchKernelGGL((k_apply<float>), dim3(grid_of(S->n)), dim3(BS), 0, S->stream, S->g,
+                           op_of<float>(S, coefft), (const float*)x, (float*)y);
+    if (prof) prof->end(tk);
+    return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
 namespace {

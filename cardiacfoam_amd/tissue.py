@@ -190,6 +190,20 @@ class Monodomain:
             out[nm] = (ms[k], cnt[k])
         return out
 
+    def apply_operator(self, x, y, ddt_coeff=1.0, precision="fp64"):
+        """y = A x on the device with the implicit operator (fvm::ddt + fvm::laplacian,
+        orthogonal part, per unit volume).  x, y: contiguous CUDA tensors of float64
+        (precision "fp64") or float32 ("mixed").  Asynchronous on the tissue's stream."""
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        want = 8 if prec == 0 else 4
+        for a in (x, y):
+            if not (a.is_cuda and a.is_contiguous() and a.element_size() == want
+                    and a.numel() == self.mesh.n):
+                raise ValueError("apply_operator needs contiguous device tensors of %d cells "
+                                 "and %d-byte elements" % (self.mesh.n, want))
+        check(lib().cg_tissue_apply_operator(self.t, float(ddt_coeff), prec,
+                                             C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
+
     def solver_stats(self):
         it = C.c_int64()
         r = C.c_double()

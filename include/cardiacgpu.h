@@ -192,6 +192,16 @@ int cg_tissue_kernel_times(cg_tissue* t, int max, char* names, double* total_ms,
  * V and out are device pointers of n doubles. */
 int cg_mesh_laplacian(cg_mesh* m, const double* V_dev, double* out_dev);
 
+/* y = A x on the device with the implicit tissue's matrix (the fvm::laplacian + fvm::ddt
+ * assembly of monodomainSolver.C:129-135, orthogonal part):
+ *   A = ddt_coeff*chi*Cm/dt I + L,  (L x)_P = sum_f sigma_nn,f/h_f^2 (x_P - x_N),
+ * per unit volume; ddt_coeff is 1 for Euler and 1.5 for backward.  precision CG_FP64:
+ * x, y are double; CG_MIXED: float (the operator of the inner solve, mixed tissues only).
+ * Asynchronous on the tissue's stream (cg_tissue_sync waits); timed as kernel "apply"
+ * when profiling is on. */
+int cg_tissue_apply_operator(cg_tissue* t, double ddt_coeff, int precision, const void* x_dev,
+                             void* y_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -195,3 +195,33 @@ def test_laplacian_operator(gpu, orc):
     err = np.max(np.abs(out.cpu().numpy() - ref)) / np.max(np.abs(ref))
     _say("  laplacian rel err", err)
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("coefft", [1.0, 1.5])
+def test_implicit_operator(gpu, orc, precision, coefft):
+    """cg_tissue_apply_operator (the implicit matrix: ddt mass + orthogonal fvm::laplacian)
+    against the oracle's LDU assembly and Amul, per-cell rotating-fibre conductivity."""
+    import torch
+    _say("implicit operator", precision, coefft)
+    shape, h = (19, 13, 11), 2e-4
+    n = int(np.prod(shape))
+    sig = rotating_fibre_sigma(*shape)
+    dt = 2e-5
+    mesh = gpu.HexMesh(*shape, h)
+    dom = gpu.Monodomain(mesh, _coeffs("BuenoOrovio", "epicardialCells", "implicit", sigma=sig),
+                         dt=dt, precision=precision)
+    m = orc.OracleMesh(*shape, h, h, h)
+    diag, upper = m.assemble_implicit(sig, 1.4e5 * 0.01, dt, coefft)
+    x = np.random.default_rng(11).normal(-0.05, 0.02, n)
+    ref = m.amul(diag, upper, x)
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    xd = torch.tensor(x, dtype=dtype, device="cuda")
+    yd = torch.empty(n, dtype=dtype, device="cuda")
+    dom.apply_operator(xd, yd, coefft, precision)
+    dom.sync()
+    y = yd.double().cpu().numpy()
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    _say("  operator rel err %.3e" % err)
+    # fp32: operands rounded to fp32 (2^-24 each), six-term sums with cancellation
+    assert err <= (1e-13 if precision == "fp64" else 2e-5)
