@@ -8,10 +8,10 @@ reference's ionicModel / MyocardiumDomain interface.
 """
 from . import _capi
 from .foam_dict import parse as parse_dictionary, read as read_dictionary
-from .ionic import IonicModel, FatalIOError
+from .ionic import IonicModel, FatalIOError, batch_run, batch_autotune
 from .tissue import HexMesh, Monodomain
 
-__all__ = ["IonicModel", "FatalIOError", "HexMesh", "Monodomain", "parse_dictionary",
+__all__ = ["IonicModel", "FatalIOError", "batch_run", "batch_autotune", "HexMesh", "Monodomain", "parse_dictionary",
            "read_dictionary", "library_path", "version"]
 
 

@@ -87,6 +87,13 @@ def lib():
     L.cg_tissue_kernel_times.argtypes = [vp, C.c_int, C.c_char_p, dp, C.POINTER(C.c_int64),
                                          C.POINTER(C.c_int)]
     L.cg_tissue_apply_operator.argtypes = [vp, C.c_double, C.c_int, vp, vp]
+    L.cg_ionic_set_protocols.argtypes = [vp, vp]
+    L.cg_ionic_set_run_lengths.argtypes = [vp, vp, ip64]
+    L.cg_ionic_enable_beats.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.cg_batch_run.argtypes = [C.POINTER(vp), C.c_int, C.c_double, C.c_int]
+    L.cg_batch_autotune.argtypes = [C.POINTER(vp), C.c_int, C.c_double, C.c_int, C.c_char_p,
+                                    C.c_size_t]
+    L.cg_ionic_get_beats.argtypes = [vp, vp, vp]
     _lib = L
     return L
 

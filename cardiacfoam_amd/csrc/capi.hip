@@ -1,6 +1,7 @@
 // C-ABI of include/cardiacgpu.h: handles, argument checks, host-side time stepping,
 // and launches of the gfx950 kernels.  No torch types, no global state except the
 // thread-local error message.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -107,6 +108,22 @@ struct cg_ionic {
     double* d_im = nullptr;
     hipStream_t stream = nullptr;
     cg::Stimulus stim{};
+    // batched protocols (single-cell handles)
+    cg::Stimulus* d_prot = nullptr;   // [n] per-cell protocol, or null
+    int64_t* d_len = nullptr;         // [n] run length in steps
+    int64_t* d_order = nullptr;       // lane -> cell, longest runs first
+    int64_t max_len = 0;
+    bool lens_set = false;
+    int variant = 0;                  // kernel shape chosen by cg_batch_autotune
+    // beat detection
+    int max_beats = 0;
+    double beat_thr = 0.0, beat_base_s = 0.0, beat_peak_s = 0.0;  // V/s, s, s
+    double* d_det = nullptr;
+    void* d_ring = nullptr;
+    int ring_len = 0;
+    double* d_beats = nullptr;
+    int* d_nbeats = nullptr;
+    double beats_dt = 0.0;            // dt of the run that filled the beat records
 };
 
 struct cg_mesh {
@@ -235,6 +252,10 @@ int cg_ionic_destroy(cg_ionic* h)
     if (h->d_gscale) hipFree(h->d_gscale);
     if (h->d_vm) hipFree(h->d_vm);
     if (h->d_im) hipFree(h->d_im);
+    void* batch[] = {h->d_prot, h->d_len, h->d_order, h->d_det, h->d_ring, h->d_beats,
+                     h->d_nbeats};
+    for (void* q : batch)
+        if (q) hipFree(q);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return CG_OK;
@@ -323,6 +344,286 @@ int cg_ionic_run_cells(cg_ionic* h, double t0, double dt, int n_steps)
     HIPCHK(cg::launch_ionic_cells(h->model, h->prec, h->tissue, h->d_states, h->n,
                                   model_time(h->model, t0), dtm, n_steps, h->P, h->d_gscale,
                                   h->stim, h->stream));
+    return CG_OK;
+}
+
+// ----------------------------------------------------------- batched protocols
+int cg_ionic_set_protocols(cg_ionic* h, const cg_stimulus* per_cell)
+{
+    if (!h) return fail(CG_ERR_ARG, "NULL handle");
+    if (!h->solve_vm) return fail(CG_ERR_STATE, "per-cell protocols need a single-cell handle");
+    if (!per_cell) {
+        if (h->d_prot) hipFree(h->d_prot);
+        h->d_prot = nullptr;
+        return CG_OK;
+    }
+    std::vector<cg::Stimulus> v((size_t)h->n);
+    for (int64_t i = 0; i < h->n; ++i) {
+        const cg_stimulus& q = per_cell[i];
+        if (q.stim_duration < 0.0)
+            return fail(CG_ERR_STIMULUS, "cell %lld: stim_duration must be non-negative",
+                        (long long)i);
+        v[i] = {q.stim_start, q.stim_period_s1, q.stim_duration, q.stim_amplitude, q.nstim1,
+                q.stim_period_s2, q.nstim2};
+    }
+    if (!h->d_prot && hipMalloc(&h->d_prot, sizeof(cg::Stimulus) * h->n) != hipSuccess)
+        return fail(CG_ERR_NOMEM, "protocol allocation failed");
+    HIPCHK(hipMemcpy(h->d_prot, v.data(), sizeof(cg::Stimulus) * h->n, hipMemcpyHostToDevice));
+    return CG_OK;
+}
+
+int cg_ionic_set_run_lengths(cg_ionic* h, const int64_t* steps, int64_t n_steps_all)
+{
+    if (!h) return fail(CG_ERR_ARG, "NULL handle");
+    if (!h->solve_vm) return fail(CG_ERR_STATE, "run lengths need a single-cell handle");
+    std::vector<int64_t> len((size_t)h->n);
+    for (int64_t i = 0; i < h->n; ++i) {
+        len[i] = steps ? steps[i] : n_steps_all;
+        if (len[i] < 0) return fail(CG_ERR_ARG, "negative run length for cell %lld", (long long)i);
+    }
+    // lanes in order of decreasing run length: the lanes of a wave finish together
+    std::vector<int64_t> order((size_t)h->n);
+    for (int64_t i = 0; i < h->n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return len[a] > len[b]; });
+    if (!h->d_len && (hipMalloc(&h->d_len, sizeof(int64_t) * h->n) != hipSuccess ||
+                      hipMalloc(&h->d_order, sizeof(int64_t) * h->n) != hipSuccess))
+        return fail(CG_ERR_NOMEM, "run-length allocation failed");
+    HIPCHK(hipMemcpy(h->d_len, len.data(), sizeof(int64_t) * h->n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_order, order.data(), sizeof(int64_t) * h->n, hipMemcpyHostToDevice));
+    h->max_len = len.empty() ? 0 : len[order[0]];
+    h->lens_set = true;
+    return CG_OK;
+}
+
+int cg_ionic_enable_beats(cg_ionic* h, int max_beats, double dvdt_threshold, double baseline_s,
+                          double peak_window_s)
+{
+    if (!h) return fail(CG_ERR_ARG, "NULL handle");
+    if (!h->solve_vm) return fail(CG_ERR_STATE, "beat detection needs a single-cell handle");
+    if (max_beats < 0 || !(baseline_s > 0) || !(peak_window_s > 0))
+        return fail(CG_ERR_ARG, "bad beat-detection arguments");
+    auto drop = [&]() {
+        void* q[] = {h->d_det, h->d_ring, h->d_beats, h->d_nbeats};
+        for (void* x : q)
+            if (x) hipFree(x);
+        h->d_det = nullptr; h->d_ring = nullptr; h->d_beats = nullptr; h->d_nbeats = nullptr;
+        h->ring_len = 0;
+    };
+    drop();
+    h->max_beats = max_beats;
+    if (max_beats == 0) return CG_OK;
+    h->beat_thr = dvdt_threshold;
+    h->beat_base_s = baseline_s;
+    h->beat_peak_s = peak_window_s;
+    const size_t n = (size_t)h->n;
+    if (hipMalloc(&h->d_det, sizeof(double) * cg::DS_N * n) != hipSuccess ||
+        hipMalloc(&h->d_beats, sizeof(double) * cg::BEAT_REC * max_beats * n) != hipSuccess ||
+        hipMalloc(&h->d_nbeats, sizeof(int) * n) != hipSuccess) {
+        drop();
+        return fail(CG_ERR_NOMEM, "beat-detection allocation failed");
+    }
+    return CG_OK;
+}
+
+namespace {
+
+// one chunk launch of handle h covering steps [s0, s0 + chunk)
+hipError_t batch_launch(cg_ionic* h, double dtm, int64_t s0, int chunk, hipStream_t s)
+{
+    cg::BatchArgs a{};
+    a.n = h->n;
+    a.s0 = s0;
+    a.chunk = chunk;
+    a.dt = dtm;
+    a.order = h->d_order;
+    a.len = h->d_len;
+    a.prot = h->d_prot;
+    a.stim = h->stim;
+    if (h->max_beats > 0) {
+        const double per_s = model_time(h->model, 1.0);  // model time units per second
+        a.det = h->d_det;
+        a.ring = h->d_ring;
+        a.ring_len = h->ring_len;
+        a.beats = h->d_beats;
+        a.nbeats = h->d_nbeats;
+        a.max_beats = h->max_beats;
+        a.thr = h->beat_thr * 1000.0 / per_s;  // V/s -> mV per model time unit
+        a.baseline = h->beat_base_s * per_s;
+        a.peak_window = h->beat_peak_s * per_s;
+        a.vscale = h->mi.vscale;
+        a.voff = h->mi.voff;
+    }
+    return h->prec == CG_FP64
+        ? cg::launch_cells_protocol_fp64(h->model, h->tissue, h->variant, h->d_states, h->P,
+                                         h->d_gscale, a, s)
+        : cg::launch_cells_protocol_fp32(h->model, h->tissue, h->variant, h->d_states, h->P,
+                                         h->d_gscale, a, s);
+}
+
+int batch_check(cg_ionic* const* hs, int nh, double dt_s)
+{
+    if (!hs || nh <= 0) return fail(CG_ERR_ARG, "no handles");
+    if (!(dt_s > 0.0)) return fail(CG_ERR_ARG, "dt must be positive");
+    for (int i = 0; i < nh; ++i) {
+        if (!hs[i]) return fail(CG_ERR_ARG, "NULL handle %d", i);
+        if (!hs[i]->solve_vm) return fail(CG_ERR_STATE, "handle %d is a tissue handle", i);
+        if (!hs[i]->lens_set) return fail(CG_ERR_STATE, "handle %d: run lengths not set", i);
+        for (int j = 0; j < i; ++j)
+            if (hs[j] == hs[i]) return fail(CG_ERR_ARG, "handle %d given twice", i);
+    }
+    return CG_OK;
+}
+
+} // namespace
+
+int cg_batch_run(cg_ionic* const* hs, int nh, double dt_s, int chunk)
+{
+    int rc = batch_check(hs, nh, dt_s);
+    if (rc) return rc;
+    if (chunk <= 0) return fail(CG_ERR_ARG, "chunk must be positive");
+    // beat detection state, cleared outside the graph
+    for (int i = 0; i < nh; ++i) {
+        cg_ionic* h = hs[i];
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (h->max_beats <= 0) continue;
+        const double dtm = h->model == cg::MODEL_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+        const int need = (int)std::ceil(h->beat_base_s * model_time(h->model, 1.0) / dtm) + 4;
+        if (need != h->ring_len) {
+            if (h->d_ring) hipFree(h->d_ring);
+            h->d_ring = nullptr;
+            if (hipMalloc(&h->d_ring, h->elem * (size_t)need * h->n) != hipSuccess)
+                return fail(CG_ERR_NOMEM, "beat ring allocation failed");
+            h->ring_len = need;
+        }
+        HIPCHK(hipMemset(h->d_det, 0, sizeof(double) * cg::DS_N * h->n));
+        HIPCHK(hipMemset(h->d_nbeats, 0, sizeof(int) * h->n));
+        h->beats_dt = dt_s;
+    }
+    // capture: fork every handle's stream off the first, chunk launches per handle, join
+    hipStream_t s0 = hs[0]->stream;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;
+    auto cleanup = [&]() {
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+    };
+    HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    hipEvent_t fork;
+    if ((err = hipEventCreateWithFlags(&fork, hipEventDisableTiming)) == hipSuccess) {
+        ev.push_back(fork);
+        err = hipEventRecord(fork, s0);
+    }
+    for (int i = 1; i < nh && err == hipSuccess; ++i) err = hipStreamWaitEvent(hs[i]->stream, fork, 0);
+    for (int i = 0; i < nh && err == hipSuccess; ++i) {
+        cg_ionic* h = hs[i];
+        const double dtm = h->model == cg::MODEL_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+        for (int64_t st = 0; st < h->max_len && err == hipSuccess; st += chunk)
+            err = batch_launch(h, dtm, st, chunk, h->stream);
+    }
+    for (int i = 1; i < nh && err == hipSuccess; ++i) {
+        hipEvent_t join;
+        if ((err = hipEventCreateWithFlags(&join, hipEventDisableTiming)) != hipSuccess) break;
+        ev.push_back(join);
+        if ((err = hipEventRecord(join, hs[i]->stream)) == hipSuccess)
+            err = hipStreamWaitEvent(s0, join, 0);
+    }
+    const hipError_t ec = hipStreamEndCapture(s0, &graph);
+    if (err == hipSuccess) err = ec;
+    if (err == hipSuccess) err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (err == hipSuccess) err = hipGraphLaunch(exec, s0);
+    if (err == hipSuccess) err = hipStreamSynchronize(s0);
+    cleanup();
+    if (err != hipSuccess) return fail(CG_ERR_HIP, "batched run: %s", hipGetErrorString(err));
+    return CG_OK;
+}
+
+int cg_batch_autotune(cg_ionic* const* hs, int nh, double dt_s, int trial_steps, char* report,
+                      size_t len)
+{
+    int rc = batch_check(hs, nh, dt_s);
+    if (rc) return rc;
+    if (trial_steps <= 0) return fail(CG_ERR_ARG, "trial_steps must be positive");
+    std::string rep;
+    for (int i = 0; i < nh; ++i) {
+        cg_ionic* h = hs[i];
+        const double dtm = h->model == cg::MODEL_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+        const size_t bytes = h->elem * (size_t)h->mi.ns * h->n;
+        void* save = nullptr;
+        if (hipMalloc(&save, bytes) != hipSuccess) return fail(CG_ERR_NOMEM, "autotune scratch");
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipError_t err = hipMemcpyAsync(save, h->d_states, bytes, hipMemcpyDeviceToDevice, h->stream);
+        const int keep_beats = h->max_beats;
+        h->max_beats = 0;  // trials without beat detection
+        float best = 0.f;
+        int best_v = 0;
+        char line[160];
+        for (int v = 0; v < cg::BATCH_VARIANTS && err == hipSuccess; ++v) {
+            h->variant = v;
+            err = batch_launch(h, dtm, 0, trial_steps, h->stream);  // warm-up
+            if (err == hipSuccess) err = hipEventRecord(e0, h->stream);
+            if (err == hipSuccess) err = batch_launch(h, dtm, 0, trial_steps, h->stream);
+            if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if (err != hipSuccess) break;
+            snprintf(line, sizeof line, "%s/%s/%s: %s %.3f ms\n", kModelKeys[h->model],
+                     h->prec == CG_FP64 ? "fp64" : "mixed", "variant", cg::batch_variant_name(v), ms);
+            rep += line;
+            if (v == 0 || ms < best) { best = ms; best_v = v; }
+        }
+        h->max_beats = keep_beats;
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(h->d_states, save, bytes, hipMemcpyDeviceToDevice, h->stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(h->stream);
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipFree(save);
+        if (err != hipSuccess) return fail(CG_ERR_HIP, "autotune: %s", hipGetErrorString(err));
+        h->variant = best_v;
+        snprintf(line, sizeof line, "%s/%s chosen: %s\n", kModelKeys[h->model],
+                 h->prec == CG_FP64 ? "fp64" : "mixed", cg::batch_variant_name(best_v));
+        rep += line;
+    }
+    if (report && len) snprintf(report, len, "%s", rep.c_str());
+    return CG_OK;
+}
+
+int cg_ionic_get_beats(cg_ionic* h, int32_t* counts, double* records)
+{
+    if (!h) return fail(CG_ERR_ARG, "NULL handle");
+    if (h->max_beats <= 0 || !h->d_nbeats) return fail(CG_ERR_STATE, "beat detection is off");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)h->n;
+    std::vector<int> cnt(n);
+    HIPCHK(hipMemcpy(cnt.data(), h->d_nbeats, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (counts)
+        for (size_t i = 0; i < n; ++i) counts[i] = cnt[i];
+    if (records) {
+        const size_t nr = n * h->max_beats * cg::BEAT_REC;
+        HIPCHK(hipMemcpy(records, h->d_beats, sizeof(double) * nr, hipMemcpyDeviceToHost));
+        // model units -> s and V; entries past each cell's count are NaN
+        const double per_s = model_time(h->model, 1.0);
+        for (size_t i = 0; i < n; ++i)
+            for (int b = 0; b < h->max_beats; ++b) {
+                double* r = records + (i * h->max_beats + b) * cg::BEAT_REC;
+                if (b >= cnt[i]) {
+                    for (int k = 0; k < cg::BEAT_REC; ++k) r[k] = NAN;
+                    continue;
+                }
+                r[0] /= per_s;
+                r[1] *= 1e-3;
+                r[2] *= 1e-3;
+                r[3] *= 1e-3;
+                r[4] /= per_s;
+            }
+    }
     return CG_OK;
 }
 

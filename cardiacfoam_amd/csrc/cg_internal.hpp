@@ -73,6 +73,47 @@ struct ExplicitStep {
 CG_DECLARE_MODEL_ENTRIES(fp64)
 CG_DECLARE_MODEL_ENTRIES(fp32)
 
+// detector state, FP64 SoA [field][lane]
+enum {
+    DS_V1, DS_V2, DS_V3, DS_DVP,  // V_{i-1}, V_{i-2}, V_{i-3}, dv_{i-2}
+    DS_CUR,                       // current beat: 8 fields from here
+    DS_OLD = DS_CUR + 8,          // closing beat
+    DS_N = DS_OLD + 8
+};
+enum { BF_TU, BF_TEND, BF_BASE, BF_PEAK, BF_JP, BF_V90, BF_TREP, BF_FLAGS };
+enum { BEAT_OPEN = 1, BEAT_HAS = 2, BEAT_FOUND = 4 };
+constexpr int BEAT_REC = 5;  // t_up, v_base, v_peak, v90, t_repol90 (NaN when not found)
+
+struct BatchArgs {
+    int64_t n;             // lanes (= cells)
+    int64_t s0;            // steps already done
+    int chunk;             // steps of this launch
+    double dt;             // model time units
+    const int64_t* order;  // lane -> cell
+    const int64_t* len;    // [cell] run length in steps
+    const Stimulus* prot;  // [cell] protocol, or null: `stim` for every cell
+    Stimulus stim;
+    // beat detection (det == null: off)
+    double* det;           // [DS_N][n]
+    void* ring;            // [ring][n] samples of T
+    int ring_len;
+    double* beats;         // [cell][max_beats][BEAT_REC]
+    int* nbeats;           // [cell]
+    int max_beats;
+    double thr, baseline, peak_window;  // mV per model time unit, model time units
+    double vscale, voff;                // V (mV) = y[0] * vscale - voff
+};
+
+// batched single-cell protocol launch (batch_kernels.hpp); variant = kernel shape
+hipError_t launch_cells_protocol_fp64(int model, int tissue, int variant, void* states,
+                                      const ModelParams& P, const void* gscale,
+                                      const BatchArgs& a, hipStream_t s);
+hipError_t launch_cells_protocol_fp32(int model, int tissue, int variant, void* states,
+                                      const ModelParams& P, const void* gscale,
+                                      const BatchArgs& a, hipStream_t s);
+constexpr int BATCH_VARIANTS = 6;
+const char* batch_variant_name(int v);
+
 // precision dispatch: prec 0 = fp64, 1 = fp32
 inline hipError_t launch_ionic_tissue(int model, int prec, int tissue, void* states, int64_t n,
                                       const double* Vm, double* Im, double dt_model,

@@ -8,3 +8,8 @@
 namespace cg {
 CG_DEFINE_MODEL_ENTRIES(float, fp32)
 } // namespace cg
+
+#include "batch_kernels.hpp"
+namespace cg {
+CG_DEFINE_BATCH_ENTRY(float, fp32)
+} // namespace cg

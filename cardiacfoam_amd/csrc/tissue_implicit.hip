@@ -126,6 +126,32 @@ struct CorrectedLoad {  // fp64 x plus the fp32 correction e
     CG_DEV double operator()(int64_t c) const { return x[c] + double(e[c]); }
 };
 
+// ---- four consecutive rows (c0..c0+3, one x-row segment; nx % 4 == 0) of A x with
+// 16/32-byte vector loads: 7 vector loads of x and face planes plus two x scalars per quad
+template <class T> using V4 = T __attribute__((ext_vector_type(4)));
+template <class T> CG_DEV V4<T> ld4(const T* p) { return *reinterpret_cast<const V4<T>*>(p); }
+
+template <class T>
+CG_DEV V4<T> apply_quad(const HexOp<T>& A, const T* __restrict__ x, int64_t c0, const IJK& q,
+                        const HexGeom& g)
+{
+    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    const int64_t ym = q.j > 0 ? c0 - sy : c0, yp = q.j < g.ny - 1 ? c0 + sy : c0;
+    const int64_t zm = q.k > 0 ? c0 - sz : c0, zp = q.k < g.nz - 1 ? c0 + sz : c0;
+    const int64_t xl = q.i > 0 ? c0 - 1 : c0, xr = q.i + 4 < g.nx ? c0 + 4 : c0 + 3;
+    const V4<T> xc = ld4(x + c0);
+    const V4<T> xym = ld4(x + ym), xyp = ld4(x + yp), xzm = ld4(x + zm), xzp = ld4(x + zp);
+    const V4<T> fx = ld4(A.fx + c0), fyp = ld4(A.fy + c0), fym = ld4(A.fy + ym);
+    const V4<T> fzp = ld4(A.fz + c0), fzm = ld4(A.fz + zm);
+    const T x_l = x[xl], x_r = x[xr], fx_l = A.fx[xl];
+    // x-direction neighbours of the four rows; fx of the +x wall face is 0
+    const V4<T> xxm = {x_l, xc.x, xc.y, xc.z};
+    const V4<T> xxp = {xc.y, xc.z, xc.w, x_r};
+    const V4<T> fxm = {fx_l, fx.x, fx.y, fx.z};
+    return A.mass * xc + (fx * (xc - xxp) + fxm * (xc - xxm)) + (fyp * (xc - xyp) + fym * (xc - xym)) +
+           (fzp * (xc - xzp) + fzm * (xc - xzm));
+}
+
 // one cell per lane (elementwise kernels)
 #define CELL_INDEX                                                                         \
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);                                 \
@@ -142,6 +168,21 @@ struct CorrectedLoad {  // fp64 x plus the fp32 correction e
     const int64_t lo_ = (int64_t)bid * chunk_;                                             \
     const int64_t hi_ = lo_ + chunk_ < n ? lo_ + chunk_ : n;                               \
     for (int64_t c = lo_ + threadIdx.x; c < hi_; c += RBS)
+
+// the same chunks in quads of 4 consecutive cells (c0 = 4 qi), for nx % 4 == 0
+#define RANGE_LOOP_Q(g)                                                                    \
+    const int64_t nq_ = (int64_t)(g).nx * (g).ny * (g).nz / 4;                             \
+    const int64_t qchunk_ = ((nq_ + RG - 1) / RG + RBS - 1) / RBS * RBS;                   \
+    const int64_t qlo_ = (int64_t)bid * qchunk_;                                           \
+    const int64_t qhi_ = qlo_ + qchunk_ < nq_ ? qlo_ + qchunk_ : nq_;                      \
+    for (int64_t qi = qlo_ + threadIdx.x; qi < qhi_; qi += RBS)
+
+template <class T> CG_DEV double dot4(const V4<T>& a, const V4<T>& b)
+{
+    return (double(a.x) * double(b.x) + double(a.y) * double(b.y)) +
+           (double(a.z) * double(b.z) + double(a.w) * double(b.w));
+}
+template <class T> CG_DEV void st4(T* p, const V4<T>& v) { *reinterpret_cast<V4<T>*>(p) = v; }
 
 // ---- 1/diag for a given mass coefficient
 template <class T>
@@ -233,7 +274,7 @@ __global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
 // the partial (w,u).  Converged when (r,r) <= tol2 * ref, ref = (r0,r0) (relative_to_r0)
 // or (b,b); at k == 0 with outer_tol2 > 0 also when the fp64 residual already meets
 // (r0,r0) <= outer_tol2 (b,b) (the refinement is finished).
-template <class T>
+template <class T, bool QUAD>
 __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __restrict__ u,
                                               T* __restrict__ w, double* __restrict__ part,
                                               DevState* S, long long gen, int k, double tol2,
@@ -260,14 +301,26 @@ __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __
     }
     if (outer || conv || k >= max_iter) return;
     double v[1] = {0.0};
-    const PlainLoad<T> U{u};
-    RANGE_LOOP(g)
-    {
-        const IJK ix = ijk_of((uint32_t)c, g);
-        const T y = apply_row(A, U, c, ix, g);
-        w[c] = y;
-        v[0] += double(u[c]) * double(y);
+    if constexpr (QUAD) {
+        const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+        RANGE_LOOP_Q(g)
+        {
+            const int64_t c0 = qi * 4;
+            const V4<T> y = apply_quad(A, u, c0, ijk_of((uint32_t)c0, g), g);
+            st4(w + c0, y);
+            v[0] += dot4(ld4(u + c0), y);
+        }
+    } else {
+        const PlainLoad<T> U{u};
+        RANGE_LOOP(g)
+        {
+            const IJK ix = ijk_of((uint32_t)c, g);
+            const T y = apply_row(A, U, c, ix, g);
+            w[c] = y;
+            v[0] += double(u[c]) * double(y);
+        }
     }
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
     block_sum<RBS, 1>(v);
     if (threadIdx.x == 0) part[P_D * RG + bid] = v[0];
 }
@@ -277,7 +330,7 @@ __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __
 //   alpha = gamma_k / (delta_k - beta gamma_k / alpha_{k-1})   (gamma/delta at k = 0);
 //   p = u + beta p, s = w + beta s, x += alpha p (x = alpha p at k = 0 when x_from_zero),
 //   r -= alpha s, u = M^-1 r; partials (r,u), (r,r) of iteration k+1.
-template <class T, class TX>
+template <class T, class TX, bool QUAD>
 __global__ __launch_bounds__(RBS) void k_cg_u(HexGeom g, const T* __restrict__ invd,
                                               TX* __restrict__ x, T* __restrict__ r,
                                               T* __restrict__ u, T* __restrict__ p,
@@ -306,21 +359,43 @@ __global__ __launch_bounds__(RBS) void k_cg_u(HexGeom g, const T* __restrict__ i
     const bool first = k == 0;
     const bool xz = first && x_from_zero;
     double v[2] = {0.0, 0.0};
-    RANGE_LOOP(g)
-    {
-        const T pc = first ? u[c] : u[c] + bt * p[c];
-        const T sc = first ? w[c] : w[c] + bt * s[c];
-        p[c] = pc;
-        s[c] = sc;
-        x[c] = xz ? TX(a * pc) : TX(x[c] + TX(a * pc));
-        const T rn = r[c] - a * sc;
-        const T un = invd[c] * rn;
-        r[c] = rn;
-        u[c] = un;
-        v[0] += double(rn) * double(un);
-        v[1] += double(rn) * double(rn);
+    if constexpr (QUAD) {
+        const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+        RANGE_LOOP_Q(g)
+        {
+            const int64_t c0 = qi * 4;
+            const V4<T> u4 = ld4(u + c0), w4 = ld4(w + c0);
+            const V4<T> pc = first ? u4 : u4 + bt * ld4(p + c0);
+            const V4<T> sc = first ? w4 : w4 + bt * ld4(s + c0);
+            st4(p + c0, pc);
+            st4(s + c0, sc);
+            const V4<TX> ap = __builtin_convertvector(a * pc, V4<TX>);
+            st4(x + c0, xz ? ap : ld4(x + c0) + ap);
+            const V4<T> rn = ld4(r + c0) - a * sc;
+            const V4<T> un = ld4(invd + c0) * rn;
+            st4(r + c0, rn);
+            st4(u + c0, un);
+            v[0] += dot4(rn, un);
+            v[1] += dot4(rn, rn);
+        }
+    } else {
+        RANGE_LOOP(g)
+        {
+            const T pc = first ? u[c] : u[c] + bt * p[c];
+            const T sc = first ? w[c] : w[c] + bt * s[c];
+            p[c] = pc;
+            s[c] = sc;
+            x[c] = xz ? TX(a * pc) : TX(x[c] + TX(a * pc));
+            const T rn = r[c] - a * sc;
+            const T un = invd[c] * rn;
+            r[c] = rn;
+            u[c] = un;
+            v[0] += double(rn) * double(un);
+            v[1] += double(rn) * double(rn);
+        }
     }
     block_sum<RBS, 2>(v);
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         part[(P_G + ((k + 1) & 1)) * RG + bid] = v[0];
         part[(P_RR + ((k + 1) & 1)) * RG + bid] = v[1];
@@ -344,6 +419,18 @@ __global__ __launch_bounds__(BS) void k_activation(HexGeom g, const double* __re
         flag[c] = 0;
         tact[c] = t_old + (0.0 - vn) / (vnew - vn) * dt;
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(BS) void k_apply4(HexGeom g, HexOp<T> A, const T* __restrict__ x,
+                                               T* __restrict__ y)
+{
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t c0 = ((int64_t)bid * BS + threadIdx.x) * 4;
+    if (c0 >= n) return;
+    const IJK q = ijk_of((uint32_t)c0, g);
+    *reinterpret_cast<V4<T>*>(y + c0) = apply_quad(A, x, c0, q, g);
 }
 
 // ---- y = A x with the implicit operator (one row per lane, no reduction)
@@ -414,6 +501,7 @@ struct ImplicitSolver {
     DevState* S = nullptr;
     DevState* hS = nullptr;  // pinned mirror
     double mass_c = -1.0;    // coefft of the current invd
+    bool quad = false;       // nx % 4 == 0: CG kernels on quads of cells (vector loads)
     long long gen = 0;       // generation of the last CG solve started
     int64_t total_iters = 0;
     double last_res = 0.0;
@@ -437,6 +525,7 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
     ImplicitSolver* S = new ImplicitSolver();
     S->g = g; S->n = (int64_t)g.nx * g.ny * g.nz; S->prec = prec; S->chiCm = chiCm;
     S->dt = dt; S->stream = s;
+    S->quad = g.nx % 4 == 0;
     const int64_t n = S->n;
     const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
     std::vector<double> fx(n, 0.0), fy(n, 0.0), fz(n, 0.0), od(3 * n, 0.0);
@@ -522,11 +611,22 @@ int implicit_apply(ImplicitSolver* S, int prec, double coefft, const void* x, vo
 {
     if (prec == 1 && !S->fx32) return 9;
     const int tk = prof ? prof->begin(K_APPLY) : -1;
-    if (prec == 0)
-        hipLaunchKernelGGL((k_apply<double>), dim3(grid_of(S->n)), dim3(BS), 0, S->stream, S->g,
+    // quads need whole x-rows of multiples of 4 and 16/32-byte aligned vectors
+    const int elt = prec == 0 ? 8 : 4;
+    const bool quad = S->g.nx % 4 == 0 && ((uintptr_t)x % (4 * elt)) == 0 &&
+                      ((uintptr_t)y % (4 * elt)) == 0;
+    const unsigned grid = quad ? grid_of((S->n + 3) / 4) : grid_of(S->n);
+    if (prec == 0 && quad)
+        hipLaunchKernelGGL((k_apply4<double>), dim3(grid), dim3(BS), 0, S->stream, S->g,
                            op_of<double>(S, coefft), (const double*)x, (double*)y);
+    else if (prec == 0)
+        hipLaunchKernelGGL((k_apply<double>), dim3(grid), dim3(BS), 0, S->stream, S->g,
+                           op_of<double>(S, coefft), (const double*)x, (double*)y);
+    else if (quad)
+        hipLaunchKernelGGL((k_apply4<float>), dim3(grid), dim3(BS), 0, S->stream, S->g,
+                           op_of<float>(S, coefft), (const float*)x, (float*)y);
     else
-        hipLaunchKernelGGL((k_apply<float>), dim3(grid_of(S->n)), dim3(BS), 0, S->stream, S->g,
+        hipLaunchKernelGGL((k_apply<float>), dim3(grid), dim3(BS), 0, S->stream, S->g,
                            op_of<float>(S, coefft), (const float*)x, (float*)y);
     if (prof) prof->end(tk);
     return hipGetLastError() == hipSuccess ? 0 : 4;
@@ -553,9 +653,14 @@ template <class T> struct CgRun {
     void s_kernel(long long gen, int k, double tol2, int rel, double outer_tol2, int max_iter)
     {
         const int tk = prof ? prof->begin(K_SPMV) : -1;
-        hipLaunchKernelGGL((k_cg_s<T>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,
-                           (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel, outer_tol2,
-                           max_iter);
+        if (S->quad)
+            hipLaunchKernelGGL((k_cg_s<T, true>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,
+                               (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel,
+                               outer_tol2, max_iter);
+        else
+            hipLaunchKernelGGL((k_cg_s<T, false>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,
+                               (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel,
+                               outer_tol2, max_iter);
         if (prof) {
             prof->end(tk);
             log->push_back({(size_t)tk, gen, k, true});
@@ -564,9 +669,14 @@ template <class T> struct CgRun {
     template <class TX> void u_kernel(long long gen, int k, TX* x, int x_from_zero)
     {
         const int tk = prof ? prof->begin(K_UPDATE) : -1;
-        hipLaunchKernelGGL((k_cg_u<T, TX>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A.invd, x,
-                           v.r, v.u, v.p, v.s, (const T*)v.w, S->partials, S->S, gen, k,
-                           x_from_zero);
+        if (S->quad)
+            hipLaunchKernelGGL((k_cg_u<T, TX, true>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
+                               A.invd, x, v.r, v.u, v.p, v.s, (const T*)v.w, S->partials, S->S,
+                               gen, k, x_from_zero);
+        else
+            hipLaunchKernelGGL((k_cg_u<T, TX, false>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
+                               A.invd, x, v.r, v.u, v.p, v.s, (const T*)v.w, S->partials, S->S,
+                               gen, k, x_from_zero);
         if (prof) {
             prof->end(tk);
             log->push_back({(size_t)tk, gen, k, false});

@@ -148,6 +148,50 @@ class IonicModel:
         a = a.reshape(-1, self.n)
         check(lib().cg_ionic_set_states(self.h, first, a.shape[0], a.ctypes.data_as(C.c_void_p)))
 
+    # --------------------------------------------------- batched single cells
+    def set_protocols(self, protocols):
+        """Per-cell S1/S2 protocols: a structured sequence of n dicts with the
+        singleCellStimulus keys (model time units), or None for the handle's own protocol."""
+        if protocols is None:
+            check(lib().cg_ionic_set_protocols(self.h, None))
+            return
+        if len(protocols) != self.n:
+            raise ValueError("need one protocol per cell")
+        arr = (_capi.Stimulus * self.n)()
+        for i, d in enumerate(protocols):
+            s1 = float(d.get("stim_period_S1", 0.0))
+            arr[i] = _capi.Stimulus(float(d.get("stim_start", 0.0)), s1,
+                                    float(d.get("stim_duration", 0.0)),
+                                    float(d.get("stim_amplitude", 0.0)),
+                                    int(d.get("nstim1", 1 if s1 > 1e-15 else 0)),
+                                    float(d.get("stim_period_S2", 0.0)), int(d.get("nstim2", 0)))
+        check(lib().cg_ionic_set_protocols(self.h, C.cast(arr, C.c_void_p)))
+
+    def set_run_lengths(self, steps=None, n_steps_all=0):
+        if steps is None:
+            check(lib().cg_ionic_set_run_lengths(self.h, None, int(n_steps_all)))
+            return
+        a = np.ascontiguousarray(steps, dtype=np.int64)
+        if a.size != self.n:
+            raise ValueError("need one run length per cell")
+        check(lib().cg_ionic_set_run_lengths(self.h, a.ctypes.data_as(C.c_void_p), 0))
+
+    def enable_beats(self, max_beats=32, dvdt_threshold=5.0, baseline_s=2e-3, peak_window_s=2.0):
+        """On-device detect_beats (postProcessing_restCurves.py:115-200); threshold in V/s."""
+        check(lib().cg_ionic_enable_beats(self.h, int(max_beats), float(dvdt_threshold),
+                                          float(baseline_s), float(peak_window_s)))
+        self.max_beats = int(max_beats)
+
+    def beats(self):
+        """(counts[n], records[n, max_beats, 5]) of the last batch run: t_up (s), Vm base,
+        peak, V90 (V), t_repol90 (s, NaN if not reached); unused records NaN."""
+        mb = getattr(self, "max_beats", 0)
+        cnt = np.zeros(self.n, dtype=np.int32)
+        rec = np.zeros((self.n, mb, 5))
+        check(lib().cg_ionic_get_beats(self.h, cnt.ctypes.data_as(C.c_void_p),
+                                       rec.ctypes.data_as(C.c_void_p)))
+        return cnt, rec
+
     def stateVariableNames(self):
         buf = C.create_string_buffer(64)
         names = []
@@ -155,3 +199,24 @@ class IonicModel:
             check(lib().cg_model_state_name(self.model, i, buf, 64))
             names.append(buf.value.decode())
         return names
+
+
+def _handles(models):
+    arr = (C.c_void_p * len(models))()
+    for i, m in enumerate(models):
+        arr[i] = m.h
+    return arr
+
+
+def batch_run(models, dt, chunk=20000):
+    """Runs every cell of every model for its run length in one hipGraph launch."""
+    arr = _handles(models)
+    check(lib().cg_batch_run(arr, len(models), float(dt), int(chunk)))
+
+
+def batch_autotune(models, dt, trial_steps=400):
+    """Picks the fastest kernel shape per model on a short trial; returns the report."""
+    arr = _handles(models)
+    buf = C.create_string_buffer(4096)
+    check(lib().cg_batch_autotune(arr, len(models), float(dt), int(trial_steps), buf, 4096))
+    return buf.value.decode()

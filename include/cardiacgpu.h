@@ -105,6 +105,40 @@ int cg_ionic_step(cg_ionic* h, double t0_s, double dt_s, const double* Vm, doubl
  * with the states kept in registers.  Asynchronous; cg_ionic_get_states synchronises. */
 int cg_ionic_run_cells(cg_ionic* h, double t0_s, double dt_s, int n_steps);
 
+/* ---------------------------------------------------- batched single cells
+ * Thousands to millions of independent single cells (solve_vm = 1 handles), each with its
+ * own stimulus protocol, run length and conductance scales, in one hipGraph launch.  No
+ * reference counterpart as one call: the reference runs one cell per SingleCellSolver
+ * (singleCellSolver.C:179-229) and post-processes the traces offline
+ * (postProcessing_restCurves.py:115-200). */
+
+/* Per-cell protocols, n_cells entries with the fields of cg_stimulus (model time units,
+ * ms for BuenoOrovio/TNNP/ORd); NULL returns every cell to the handle's protocol. */
+int cg_ionic_set_protocols(cg_ionic* h, const cg_stimulus* per_cell);
+/* Run lengths in steps per cell (NULL: every cell runs n_steps_all). */
+int cg_ionic_set_run_lengths(cg_ionic* h, const int64_t* steps, int64_t n_steps_all);
+/* On-device beat detection during cg_batch_run, the detect_beats definition of
+ * postProcessing_restCurves.py:115-200 on each cell's trace: upstroke where the centred
+ * dVm/dt crosses dvdt_threshold (V/s; 5 V/s there), baseline = min Vm over baseline_s
+ * before it, peak over peak_window_s (or up to the next upstroke), repolarisation at
+ * 90 %.  Up to max_beats records per cell; 0 switches detection off. */
+int cg_ionic_enable_beats(cg_ionic* h, int max_beats, double dvdt_threshold, double baseline_s,
+                          double peak_window_s);
+/* Runs every cell of every handle from time 0 for its run length with steps of dt_s
+ * (states continue from their current values), chunk steps per launch, all launches of
+ * all handles captured in one hipGraph (handles on their own streams run concurrently).
+ * Synchronous. */
+int cg_batch_run(cg_ionic* const* handles, int n_handles, double dt_s, int chunk);
+/* Times each kernel shape (threads per block x register budget) on trial_steps steps of
+ * every handle, restores the states, keeps the fastest shape per handle; a text report
+ * goes to report (may be NULL). */
+int cg_batch_autotune(cg_ionic* const* handles, int n_handles, double dt_s, int trial_steps,
+                      char* report, size_t report_len);
+/* Beats of the last cg_batch_run: counts[n_cells] (may exceed max_beats) and records
+ * [n_cells][max_beats][5] = t_up (s), Vm base, peak, V90 (V), t_repol90 (s, NaN when the
+ * trace ends first); unused records are NaN. */
+int cg_ionic_get_beats(cg_ionic* h, int32_t* counts, double* records);
+
 /* States as host SoA doubles [n_vars][n_cells], starting at state first_var.
  * Replaces the io*Ptr hooks / exportStates (ionicModel.H:115-168, 584-638). */
 int cg_ionic_get_states(cg_ionic* h, int first_var, int n_vars, double* host_soa);

@@ -96,6 +96,9 @@ def lib():
                                             _dp, C.c_double, C.c_double, C.c_int]
         L.orc_single_cell_run.argtypes = [C.c_int, C.c_int, _dp, _dp, C.POINTER(Stim), C.c_double,
                                           C.c_long, C.c_int, C.c_double, C.c_double, _dp, _dp]
+        L.orc_single_cell_trace.argtypes = [C.c_int, C.c_int, _dp, _dp, C.POINTER(Stim),
+                                            C.c_double, C.c_long, _dp, _dp]
+        L.orc_single_cell_trace.restype = None
         L.orc_model_vm_mV.argtypes = [C.c_int, _dp]
         L.orc_model_vm_mV.restype = C.c_double
         L.orc_mesh_build.argtypes = [C.POINTER(Mesh), C.c_int, C.c_int, C.c_int, C.c_double,
@@ -175,6 +178,22 @@ def single_cell_run(model, tissue, stim, dt_s, n_steps, integrator=1, abs_tol=1e
                                    integrator, abs_tol, rel_tol, t, v)
     if rc != 0:
         raise RuntimeError("single-cell integration failed")
+    return t, v, yy[:NSTATES[model]]
+
+
+def single_cell_trace(model, tissue, stim, dt_model, n_steps, consts=None, y0=None):
+    """Rush-Larsen single cell with the step start at k*dt_model (model units, ms for
+    TNNP/ORd); returns (t, Vm mV) after each step and the final state."""
+    c, y = model_init(model, tissue)
+    if consts is not None:
+        c = np.array(consts, dtype=np.float64)
+    if y0 is not None:
+        y = np.array(y0, dtype=np.float64)
+    yy = np.zeros(MAX_STATES)
+    yy[:len(y)] = y
+    t = np.zeros(n_steps)
+    v = np.zeros(n_steps)
+    lib().orc_single_cell_trace(model, tissue, c, yy, C.byref(stim), dt_model, n_steps, t, v)
     return t, v, yy[:NSTATES[model]]
 
 

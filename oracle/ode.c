@@ -223,3 +223,14 @@ int orc_single_cell_run(int model, int tissue, const double* c, double* y,
     }
     return 0;
 }
+
+void orc_single_cell_trace(int model, int tissue, const double* c, double* y,
+                           const orc_stim_t* st, double dt_model, long n_steps,
+                           double* t_out, double* vm_out)
+{
+    for (long k = 0; k < n_steps; ++k) {
+        orc_rl_step(model, tissue, (double)k * dt_model, dt_model, c, y, 1, st);
+        if (t_out) t_out[k] = (double)(k + 1) * dt_model;
+        if (vm_out) vm_out[k] = orc_model_vm_mV(model, y);
+    }
+}

@@ -99,6 +99,14 @@ int orc_single_cell_run(int model, int tissue, const double* c, double* y,
                         int integrator, double abs_tol, double rel_tol,
                         double* t_out, double* vm_out);
 
+/* Batched-protocol single cell: n_steps Rush-Larsen steps of dt_model (model time units)
+ * with the step start at k*dt_model (time as step count times dt, no accumulation) and
+ * Vm inside the ODE; writes the trace t_out[k] = (k+1)*dt_model, vm_out[k] (mV) after
+ * each step.  Parity: same RL step and stimulus as orc_single_cell_run. */
+void orc_single_cell_trace(int model, int tissue, const double* c, double* y,
+                           const orc_stim_t* stim, double dt_model, long n_steps,
+                           double* t_out, double* vm_out);
+
 /* Vm of the single cell in mV from a model state vector (ioVmTransform) */
 double orc_model_vm_mV(int model, const double* y);
 

@@ -98,11 +98,13 @@ def test_niederer_case_gpu(gpu, orc):
         assert abs(ta[k, j, i] - p["expected"]) <= p.get("tol_here", p["tol"]), p
 
 
-@pytest.mark.parametrize("name", ["BO_epi", "ORd"])
-def test_implicit_parity_fp64_rotating_fibres(gpu, orc, name):
-    _say("implicit fp64", name)
+@pytest.mark.parametrize("name,shape", [("BO_epi", (20, 14, 10)), ("ORd", (20, 14, 10)),
+                                        ("TNNP_epi", (19, 14, 10))])
+def test_implicit_parity_fp64_rotating_fibres(gpu, orc, name, shape):
+    """nx = 20 runs the CG kernels on quads of cells, nx = 19 on single cells."""
+    _say("implicit fp64", name, shape)
     m, tis, key, word, _ = MODELS[name]
-    shape, h = (20, 14, 10), 2e-4
+    h = 2e-4
     sig = rotating_fibre_sigma(*shape)
     dt, n_steps = 2e-5, 300
     mesh = gpu.HexMesh(*shape, h)
@@ -116,11 +118,12 @@ def test_implicit_parity_fp64_rotating_fibres(gpu, orc, name):
     assert res <= 1e-13
 
 
-@pytest.mark.parametrize("name", ["BO_epi", "ORd"])
-def test_implicit_mixed_vs_fp64(gpu, orc, name):
-    _say("implicit mixed", name)
+@pytest.mark.parametrize("name,shape", [("BO_epi", (20, 14, 10)), ("ORd", (20, 14, 10)),
+                                        ("ORd", (21, 14, 10))])
+def test_implicit_mixed_vs_fp64(gpu, orc, name, shape):
+    _say("implicit mixed", name, shape)
     m, tis, key, word, _ = MODELS[name]
-    shape, h = (20, 14, 10), 2e-4
+    h = 2e-4
     sig = rotating_fibre_sigma(*shape)
     dt, n_steps = 2e-5, 400
     doms = {}
@@ -197,14 +200,16 @@ def test_laplacian_operator(gpu, orc):
     assert err < 1e-12
 
 
+@pytest.mark.parametrize("shape", [(19, 13, 11), (20, 13, 11)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("coefft", [1.0, 1.5])
-def test_implicit_operator(gpu, orc, precision, coefft):
+def test_implicit_operator(gpu, orc, precision, coefft, shape):
     """cg_tissue_apply_operator (the implicit matrix: ddt mass + orthogonal fvm::laplacian)
-    against the oracle's LDU assembly and Amul, per-cell rotating-fibre conductivity."""
+    against the oracle's LDU assembly and Amul, per-cell rotating-fibre conductivity; nx = 20
+    takes the four-rows-per-lane vector kernel, nx = 19 the one-row kernel."""
     import torch
-    _say("implicit operator", precision, coefft)
-    shape, h = (19, 13, 11), 2e-4
+    _say("implicit operator", precision, coefft, shape)
+    h = 2e-4
     n = int(np.prod(shape))
     sig = rotating_fibre_sigma(*shape)
     dt = 2e-5
