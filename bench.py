@@ -202,6 +202,154 @@ def run_explicit_c0(cf, torch, steps=20000, warmup=200):
             "activated_cells": int(np.count_nonzero(a64 > 0))}
 
 
+# stimulus amplitudes (model units): TNNP from the restitution tutorial's map
+# (mainRestitutionCurves_s1s2Protocol.py:33-38); ORd is not in that map, 80 uA/uF is a
+# suprathreshold value for it
+STIM_AMP = {"TNNP": 60.0, "ORd": 80.0}
+
+
+def _batch_handle(cf, key, word, prots, lens, scales, prec):
+    n = len(prots)
+    ion = cf.IonicModel(key, word, n, solve_vm=True, precision=prec)
+    col = lambda k, d=0.0: np.array([p.get(k, d) for p in prots])
+    ion.set_protocol_arrays(col("stim_start"), col("stim_period_S1"), col("stim_duration"),
+                            col("stim_amplitude"), col("nstim1", 0).astype(np.int32),
+                            col("stim_period_S2"), col("nstim2", 0).astype(np.int32))
+    ion.set_run_lengths(lens)
+    if scales is not None:
+        ion.set_conductance_scale(scales)
+    ion.enable_beats(max_beats=24, dvdt_threshold=5.0, baseline_s=2e-3, peak_window_s=2.0)
+    return ion
+
+
+def _apd_errors(h64, h32):
+    """Mixed vs FP64 over a batch.  Cells whose beat counts differ (a premature stimulus
+    near refractoriness answered in one precision and not in the other) are counted and
+    left out of the APD90 statistics; final states are compared per state relative to that
+    state's largest magnitude over the batch."""
+    (c64, r64), (c32, r32) = h64.beats(), h32.beats()
+    same = c64 == c32
+    apd64, apd32 = r64[:, :, 4] - r64[:, :, 0], r32[:, :, 4] - r32[:, :, 0]
+    both = ~np.isnan(apd64) & ~np.isnan(apd32) & same[:, None]
+    d = np.abs(apd64[both] - apd32[both]) * 1e3
+    s64, s32 = h64.states(), h32.states()
+    scale = np.max(np.abs(s64), axis=1, keepdims=True) + 1e-300
+    out = {"beats_compared": int(both.sum()),
+           "cells_beat_count_differs": int(np.count_nonzero(~same)),
+           "vm_final_linf_mV": float(np.max(np.abs(s64[0] - s32[0]))),
+           "state_final_max_err_rel_to_range": float(np.max(np.abs(s64 - s32) / scale))}
+    if d.size:
+        out.update({"apd90_median_ms": float(np.median(d)), "apd90_p99_ms": float(np.percentile(d, 99)),
+                    "apd90_p999_ms": float(np.percentile(d, 99.9)), "apd90_max_ms": float(d.max())})
+    return out
+
+
+def _oracle_check(model_id, tissue_id, prots, lens, scales, h64, cells, dt_ms):
+    """FP64 batch vs the oracle's single-cell trace + detect_beats on a few cells."""
+    import oracle
+    names = {1: ["g_Na", "g_Kr", "g_CaL", "g_Ks"], 2: ["GNa", "GKr", "PCa", "GKs"]}[model_id]
+    cnt, rec = h64.beats()
+    states = h64.states()
+    worst_t, worst_s = 0.0, 0.0
+    for i in cells:
+        c, _ = oracle.model_init(model_id, tissue_id)
+        if scales is not None:
+            for k, nm in enumerate(names):
+                c[oracle.const_index(model_id, nm)] *= scales[k, i]
+        p = prots[i]
+        st = oracle.make_stim(start=p["stim_start"], period_s1=p["stim_period_S1"],
+                              duration=p["stim_duration"], amplitude=p["stim_amplitude"],
+                              nstim1=p["nstim1"], period_s2=p.get("stim_period_S2", 0.0),
+                              nstim2=p.get("nstim2", 0))
+        t, v, yf = oracle.single_cell_trace(model_id, tissue_id, st, dt_ms, int(lens[i]),
+                                            consts=c)
+        beats = oracle.detect_beats(t, v, dvdt_threshold=5.0, baseline_s=2.0, peak_s=2000.0)
+        if len(beats) != cnt[i]:
+            return {"cells": len(cells), "error": "beat count differs at cell %d" % i}
+        for b, r in zip(beats, rec[i]):
+            if b["repol_found"]:
+                worst_t = max(worst_t, abs(b["t_repol90"] * 1e-3 - r[4]),
+                              abs(b["t_up"] * 1e-3 - r[0]))
+        worst_s = max(worst_s, float(np.max(np.abs(states[:, i] - yf) / (np.abs(yf) + 1e-30))))
+    return {"cells": len(cells), "t_up_t_repol90_linf_s": worst_t, "state_max_rel": worst_s}
+
+
+def run_batch_c1(cf, n=1048576, window_ms=2000.0, dt_ms=0.01, seed=20240601):
+    """1,048,576 TNNP epi cells, g_Na/g_Kr/g_CaL/g_Ks x U[0.8, 1.2], CL ~ U[300, 1000] ms,
+    5..20 beats, RL dt 0.01 ms, FP64 and mixed, one hipGraph launch each; the simulated
+    window is cut at window_ms (run length = min(protocol, window))."""
+    rng = np.random.default_rng(seed)
+    cl = rng.uniform(300.0, 1000.0, n)
+    beats = rng.integers(5, 21, n)
+    scales = rng.uniform(0.8, 1.2, (4, n))
+    prots = [{"stim_start": 10.0, "stim_period_S1": float(cl[i]), "stim_duration": 1.0,
+              "stim_amplitude": STIM_AMP["TNNP"], "nstim1": int(beats[i])} for i in range(n)]
+    full = (10.0 + cl * beats) / dt_ms
+    lens = np.minimum(full, window_ms / dt_ms).astype(np.int64)
+    out = {"workload": "tnnp_batch_1M", "cells": n, "window_ms": window_ms,
+           "full_protocol_cell_steps": float(full.sum()), "cell_steps": int(lens.sum())}
+    hs = {}
+    for prec in ("fp64", "mixed"):
+        h = _batch_handle(cf, "TNNP", "epicardialCells", prots, lens, scales, prec)
+        rep = cf.batch_autotune([h], dt_ms * 1e-3, trial_steps=200)
+        t0 = time.perf_counter()
+        cf.batch_run([h], dt_ms * 1e-3, chunk=50000)
+        el = time.perf_counter() - t0
+        out[prec] = {"seconds": el, "cell_steps_per_s": lens.sum() / el,
+                     "kernel_shape": rep.strip().splitlines()[-1]}
+        hs[prec] = h
+    out["mixed_vs_fp64"] = _apd_errors(hs["fp64"], hs["mixed"])
+    out["fp64_vs_oracle"] = _oracle_check(1, 1, prots, lens, scales, hs["fp64"],
+                                          [0, 1, n // 2, n - 1], dt_ms)
+    return out
+
+
+def run_batch_c2(cf, n=4096, dt_ms=0.01, seed=20240602):
+    """4096 cells, 2048 ORd + 2048 TNNP epi, S1 = 10 x 1000 ms then S2 at U[200, 600] ms,
+    FP32 states with FP64 time vs FP64; both models in one hipGraph launch; APD90 of the S2
+    beat against the S2 interval.  The S2 train starts at start + 10 x S1 (stimulusIO.C:
+    268-283: its first pulse lands 1000 ms after the last S1 pulse), so nstim2 = 2 puts the
+    premature beat at coupling interval S2 after the 11th; each cell runs 600 ms past it."""
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    res = {"workload": "ord_tnnp_s1s2_4096", "cells": n}
+    groups = {}
+    for key, word, mid, tid in (("ORd", "myocyte", 2, 4), ("TNNP", "epicardialCells", 1, 1)):
+        s2 = rng.uniform(200.0, 600.0, half)
+        prots = [{"stim_start": 10.0, "stim_period_S1": 1000.0, "stim_duration": 1.0,
+                  "stim_amplitude": STIM_AMP[key], "nstim1": 10,
+                  "stim_period_S2": float(s2[i]), "nstim2": 2} for i in range(half)]
+        lens = ((10.0 + 10 * 1000.0 + s2 + 600.0) / dt_ms).astype(np.int64)
+        groups[key] = (word, mid, tid, s2, prots, lens)
+    cell_steps = sum(int(g[5].sum()) for g in groups.values())
+    hs = {}
+    for prec in ("fp64", "mixed"):
+        handles = [_batch_handle(cf, key, g[0], g[4], g[5], None, prec)
+                   for key, g in groups.items()]
+        rep = cf.batch_autotune(handles, dt_ms * 1e-3, trial_steps=200)
+        t0 = time.perf_counter()
+        cf.batch_run(handles, dt_ms * 1e-3, chunk=200000)
+        el = time.perf_counter() - t0
+        res[prec] = {"seconds": el, "cell_steps_per_s": cell_steps / el,
+                     "kernel_shapes": [l for l in rep.splitlines() if "chosen" in l]}
+        hs[prec] = handles
+    for k, key in enumerate(groups):
+        res[key + "_mixed_vs_fp64"] = _apd_errors(hs["fp64"][k], hs["mixed"][k])
+        word, mid, tid, s2, prots, lens = groups[key]
+        res[key + "_fp64_vs_oracle"] = _oracle_check(mid, tid, prots, lens, None, hs["fp64"][k],
+                                                     [0, half - 1], dt_ms)
+        # restitution: APD90 of the S2 beat (the 12th) against the S2 interval
+        cnt, rec = hs["mixed"][k].beats()
+        apd_s2 = (rec[:, 11, 4] - rec[:, 11, 0]) * 1e3 if rec.shape[1] > 11 else None
+        if apd_s2 is not None:
+            order = np.argsort(s2)
+            res[key + "_restitution"] = {"s2_ms": [round(float(s2[i]), 1) for i in order[::256]],
+                                         "apd90_ms": [round(float(apd_s2[i]), 2)
+                                                      for i in order[::256]]}
+    res["cell_steps"] = cell_steps
+    return res
+
+
 def cpu_baseline(w, sigma, sample_steps):
     """The CPU FP64 oracle (OpenMP over all host cores) on the same workload: the full
     mesh for `sample_steps` steps after a 2-step start, timed as cell-steps/s."""
@@ -245,6 +393,7 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched single-cell legs")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both",
                     help="run one precision only (profiling runs; no error gate)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -322,6 +471,8 @@ def main():
     spmv = {p: measure_spmv(torch, res[p]["dom"], n, p) for p in ("fp64", "mixed")} \
         if w["algorithm"] == "implicit" else None
     c0 = run_explicit_c0(cf, torch) if args.workload != "bo_explicit_147k" else None
+    c1 = run_batch_c1(cf) if not args.no_batch else None
+    c2 = run_batch_c2(cf) if not args.no_batch else None
 
     cpu = None
     if not args.no_cpu:
@@ -352,6 +503,8 @@ def main():
         "kernels": kernels_out,
         "spmv": spmv,
         "explicit_c0": c0,
+        "batch_c1": c1,
+        "batch_c2": c2,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -25,6 +25,24 @@ CG_DEV float csqrt(float x) { return sqrtf(x); }
 CG_DEV double cpow(double x, double y) { return pow(x, y); }
 CG_DEV float cpow(float x, float y) { return powf(x, y); }
 
+// GHK flux quotient v / (exp(k v) - 1) of the generated ICaL/ICab/INab expressions
+// (TNNP_2004.H, ORd_2011.H), with ekv = exp(k v) already computed.  FP64 keeps the plain
+// quotient, the reference's expression (singular only at v = 0 exactly).  In FP32,
+// exp(k v) - 1 cancels for small |k v| and is exactly 0 next to v = 0 (a membrane potential
+// crossing 0 mV then gives inf); there the quotient comes from the series
+// x / (e^x - 1) = 1 - x/2 + x^2/12 - x^4/720, x = k v, truncation < 1e-12 for |x| < 0.05.
+CG_DEV double ghk_quot(double v, double, double ekv) { return v / (ekv - 1.0); }
+CG_DEV float ghk_quot(float v, float k, float ekv)
+{
+    const float x = k * v;
+    if (fabsf(x) < 0.05f) {
+        const float x2 = x * x;
+        return (1.0f - 0.5f * x + x2 * (1.0f / 12.0f - x2 * (1.0f / 720.0f))) / k;
+    }
+    return v / (ekv - 1.0f);
+}
+template <class T> constexpr bool is_fp32() { return sizeof(T) == 4; }
+
 template <class T> CG_DEV T sq(T x) { return x * x; }
 template <class T> CG_DEV T cube(T x) { return x * x * x; }
 

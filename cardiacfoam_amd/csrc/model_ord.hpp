@@ -125,6 +125,7 @@ struct ORd {
         // gs: per-cell multipliers of GNa, GKr, PCa, GKs
         Currents<T> c;
         const T RTF = T(Rc * Tc / Fc);
+        const T FRT = T(Fc / (Rc * Tc));
         const T CaMKb = T(0.05) * (T(1) - y[CaMKt]) / (T(1) + T(0.0015) / y[Cass]);
         c.CaMKa = CaMKb + y[CaMKt];
         const T fp = T(1) / (T(1) + T(0.15) / c.CaMKa);  // fINap = fItop = fINaLp = fICaLp
@@ -150,9 +151,17 @@ struct ORd {
         const T fp_ = T(0.6) * y[ffp] + T(0.4) * y[fs];
         const T gnp = f_ * (T(1) - y[nca]) + y[jca] * fca * y[nca];
         const T gpp = fp_ * (T(1) - y[nca]) + y[jca] * fcap * y[nca];
-        const T PhiCaL = T(4) * q.vffrt * (y[Cass] * q.e2 - T(0.341) * T(cao)) / (q.e2 - T(1));
-        const T PhiCaNa = q.vffrt * (T(0.75) * y[Nass] * q.e1 - T(0.75) * T(nao)) / (q.e1 - T(1));
-        const T PhiCaK = q.vffrt * (T(0.75) * y[Kss] * q.e1 - T(0.75) * T(ko)) / (q.e1 - T(1));
+        T PhiCaL, PhiCaNa, PhiCaK;
+        if constexpr (is_fp32<T>()) {
+            const T FF = T(Fc) * FRT, g1 = ghk_quot(Vv, FRT, q.e1), g2 = ghk_quot(Vv, T(2) * FRT, q.e2);
+            PhiCaL = T(4) * FF * (y[Cass] * q.e2 - T(0.341) * T(cao)) * g2;
+            PhiCaNa = FF * (T(0.75) * y[Nass] * q.e1 - T(0.75) * T(nao)) * g1;
+            PhiCaK = FF * (T(0.75) * y[Kss] * q.e1 - T(0.75) * T(ko)) * g1;
+        } else {
+            PhiCaL = T(4) * q.vffrt * (y[Cass] * q.e2 - T(0.341) * T(cao)) / (q.e2 - T(1));
+            PhiCaNa = q.vffrt * (T(0.75) * y[Nass] * q.e1 - T(0.75) * T(nao)) / (q.e1 - T(1));
+            PhiCaK = q.vffrt * (T(0.75) * y[Kss] * q.e1 - T(0.75) * T(ko)) / (q.e1 - T(1));
+        }
         const T pca = gs[2] * p[PCa], pcap = T(1.1) * pca;
         c.ICaL = (T(1) - fp) * pca * PhiCaL * y[d] * gnp + fp * pcap * PhiCaL * y[d] * gpp;
         c.ICaNa = (T(1) - fp) * T(0.00125) * pca * PhiCaNa * y[d] * gnp +
@@ -193,8 +202,15 @@ struct ORd {
             const T JnakNa = T(3) * (x1 / sx * a3 - x2 / sx * b3);
             c.INaK = p[Pnak] * (JnakNa + JnakK);
         }
-        c.INab = p[PNab] * q.vffrt * (y[Nai] * q.e1 - T(nao)) / (q.e1 - T(1));
-        c.ICab = p[PCab] * T(4) * q.vffrt * (y[Cai] * q.e2 - T(0.341) * T(cao)) / (q.e2 - T(1));
+        if constexpr (is_fp32<T>()) {
+            const T FF = T(Fc) * FRT;
+            c.INab = p[PNab] * FF * (y[Nai] * q.e1 - T(nao)) * ghk_quot(Vv, FRT, q.e1);
+            c.ICab = p[PCab] * T(4) * FF * (y[Cai] * q.e2 - T(0.341) * T(cao)) *
+                     ghk_quot(Vv, T(2) * FRT, q.e2);
+        } else {
+            c.INab = p[PNab] * q.vffrt * (y[Nai] * q.e1 - T(nao)) / (q.e1 - T(1));
+            c.ICab = p[PCab] * T(4) * q.vffrt * (y[Cai] * q.e2 - T(0.341) * T(cao)) / (q.e2 - T(1));
+        }
         c.IpCa = p[GpCa] * y[Cai] / (T(0.0005) + y[Cai]);
         return c;
     }

@@ -86,8 +86,12 @@ struct TNNP {
         c.Ito = p[gto] * y[r] * y[s] * dvk;
         c.IKr = gs[1] * p[gKr] * sko * y[Xr1] * y[Xr2] * dvk;
         c.IKs = gs[3] * p[gKs] * sq(y[Xs]) * (Vv - EKs);
-        c.ICaL = gs[2] * (p[gCaL] * y[d] * y[f] * y[fCa] * T(4) * Vv * p[F] * FRT) *
-                 (y[Cai] * q.cal_e2 - T(0.341) * p[Cao]) / (q.cal_e2 - T(1));
+        if constexpr (is_fp32<T>())
+            c.ICaL = gs[2] * (p[gCaL] * y[d] * y[f] * y[fCa] * T(4) * p[F] * FRT) *
+                     (y[Cai] * q.cal_e2 - T(0.341) * p[Cao]) * ghk_quot(Vv, T(2) * FRT, q.cal_e2);
+        else
+            c.ICaL = gs[2] * (p[gCaL] * y[d] * y[f] * y[fCa] * T(4) * Vv * p[F] * FRT) *
+                     (y[Cai] * q.cal_e2 - T(0.341) * p[Cao]) / (q.cal_e2 - T(1));
         c.IbCa = p[gbCa] * (Vv - ECa);
         c.IpK = p[gpK] * dvk * q.ipk;
         c.IpCa = p[gpCa] * y[Cai] / (y[Cai] + p[KpCa]);

@@ -167,6 +167,22 @@ class IonicModel:
                                     float(d.get("stim_period_S2", 0.0)), int(d.get("nstim2", 0)))
         check(lib().cg_ionic_set_protocols(self.h, C.cast(arr, C.c_void_p)))
 
+    # numpy layout of cg_stimulus (4 doubles, int, double, int; natural alignment)
+    PROTOCOL_DTYPE = np.dtype([("stim_start", "f8"), ("stim_period_S1", "f8"),
+                               ("stim_duration", "f8"), ("stim_amplitude", "f8"),
+                               ("nstim1", "i4"), ("stim_period_S2", "f8"), ("nstim2", "i4")],
+                              align=True)
+
+    def set_protocol_arrays(self, stim_start, stim_period_S1, stim_duration, stim_amplitude,
+                            nstim1, stim_period_S2=0.0, nstim2=0):
+        """Per-cell protocols from arrays (or scalars broadcast to every cell)."""
+        a = np.zeros(self.n, dtype=self.PROTOCOL_DTYPE)
+        a["stim_start"], a["stim_period_S1"] = stim_start, stim_period_S1
+        a["stim_duration"], a["stim_amplitude"] = stim_duration, stim_amplitude
+        a["nstim1"], a["stim_period_S2"], a["nstim2"] = nstim1, stim_period_S2, nstim2
+        assert a.itemsize == C.sizeof(_capi.Stimulus)
+        check(lib().cg_ionic_set_protocols(self.h, a.ctypes.data_as(C.c_void_p)))
+
     def set_run_lengths(self, steps=None, n_steps_all=0):
         if steps is None:
             check(lib().cg_ionic_set_run_lengths(self.h, None, int(n_steps_all)))

@@ -150,3 +150,34 @@ def test_batch_autotune_keeps_results(gpu, orc):
     np.testing.assert_array_equal(c_t, c_d)
     ok = ~np.isnan(r_d)
     np.testing.assert_allclose(r_t[ok], r_d[ok], rtol=1e-12, atol=0)
+
+
+def test_batch_mixed_crossing_zero_mv(gpu, orc):
+    """Regression: a TNNP cell whose FP32 Vm passes within 1e-7 mV of 0 mV (seed 20240601,
+    cell 340 of the 1,048,576-cell batch) used to hit exp(2 V F/RT) - 1 == 0 in the ICaL
+    GHK quotient and turn every state to inf/NaN; it now stays finite and tracks FP64."""
+    _say("batch fp32 GHK quotient at 0 mV")
+    n_all = 1048576
+    rng = np.random.default_rng(20240601)
+    cl = rng.uniform(300.0, 1000.0, n_all)
+    nb = rng.integers(5, 21, n_all)
+    sc = rng.uniform(0.8, 1.2, (4, n_all))
+    c = 340
+    steps = 169500
+    out = {}
+    for prec in ("fp64", "mixed"):
+        ion = gpu.IonicModel("TNNP", "epicardialCells", 1, solve_vm=True, precision=prec)
+        ion.set_protocol_arrays(10.0, cl[c:c + 1], 1.0, 60.0, nb[c:c + 1].astype(np.int32))
+        ion.set_run_lengths(None, steps)
+        ion.set_conductance_scale(sc[:, c:c + 1])
+        ion.enable_beats(max_beats=32)
+        gpu.batch_run([ion], DT_MS * 1e-3)
+        out[prec] = (ion.states()[:, 0], ion.beats())
+    s32, s64 = out["mixed"][0], out["fp64"][0]
+    assert np.all(np.isfinite(s32))
+    _say("  Vm fp32 %.4f fp64 %.4f mV" % (s32[0], s64[0]))
+    assert abs(s32[0] - s64[0]) < 1.0
+    (c64, r64), (c32, r32) = out["fp64"][1], out["mixed"][1]
+    assert c64[0] == c32[0]
+    ok = ~np.isnan(r64[0, :c64[0], 4])
+    assert np.max(np.abs(r64[0, :c64[0], 4][ok] - r32[0, :c32[0], 4][ok])) < 1e-3
