@@ -94,6 +94,10 @@ def lib():
     L.cg_batch_autotune.argtypes = [C.POINTER(vp), C.c_int, C.c_double, C.c_int, C.c_char_p,
                                     C.c_size_t]
     L.cg_ionic_get_beats.argtypes = [vp, vp, vp]
+    L.cg_adjoint_create.argtypes = [vp, C.c_int, vp, C.c_int, C.POINTER(vp)]
+    L.cg_adjoint_destroy.argtypes = [vp]
+    L.cg_adjoint_forward.argtypes = [vp, vp, vp, C.c_int, vp]
+    L.cg_adjoint_backward.argtypes = [vp, vp, vp, vp]
     _lib = L
     return L
 

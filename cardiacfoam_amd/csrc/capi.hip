@@ -12,6 +12,7 @@
 #include "../../include/cardiacgpu.h"
 #include "cg_internal.hpp"
 #include "cg_implicit.hpp"
+#include "cg_adjoint.hpp"
 
 namespace {
 
@@ -1015,6 +1016,83 @@ int cg_tissue_solver_stats(const cg_tissue* t, int64_t* iters, double* res)
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
     if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
     cg::implicit_stats(t->implicit, iters, res);
+    return CG_OK;
+}
+
+// ------------------------------------------------------------------ adjoint
+struct cg_adjoint {
+    cg_tissue* t;
+    cg::Adjoint* A;
+    double t0;  // tissue time at creation
+    int n_probes;
+};
+
+int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
+                      int checkpoint_every, cg_adjoint** out)
+{
+    if (!t || !out || (n_probes > 0 && !probe_cells)) return fail(CG_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    cg_ionic* ion = t->ion;
+    cg_mesh* m = t->mesh;
+    if (t->cfg.algorithm != CG_EXPLICIT || t->cfg.ddt != CG_DDT_EULER)
+        return fail(CG_ERR_STATE, "the adjoint covers the explicit Euler tissue step");
+    if (ion->model != cg::MODEL_BO || ion->prec != CG_FP64)
+        return fail(CG_ERR_STATE, "the adjoint covers BuenoOrovio in FP64");
+    if (!m->uniform) return fail(CG_ERR_STATE, "the adjoint scales a uniform conductivity");
+    if (m->sig[1] != 0 || m->sig[2] != 0 || m->sig[5] != 0 || m->sig[3] != 0 ||
+        m->sig[6] != 0 || m->sig[7] != 0)
+        return fail(CG_ERR_STATE, "the adjoint needs a diagonal conductivity");
+    if (n_probes < 1 || checkpoint_every < 1) return fail(CG_ERR_ARG, "bad probe count or interval");
+    HIPCHK(hipStreamSynchronize(ion->stream));
+    const double diag[3] = {m->sig[0], m->sig[4], m->sig[8]};
+    std::string err;
+    cg::Adjoint* A = nullptr;
+    const int rc = cg::adjoint_create(&A, m->g, diag, t->cfg.dt, t->cfg.chi * t->cfg.cm, ion->P,
+                                      t->d_V[t->cur], ion->d_states, checkpoint_every, n_probes,
+                                      probe_cells, ion->stream, &err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    cg_adjoint* a = new cg_adjoint{t, A, t->t, n_probes};
+    *out = a;
+    return CG_OK;
+}
+
+int cg_adjoint_destroy(cg_adjoint* a)
+{
+    if (!a) return CG_OK;
+    hipStreamSynchronize(a->t->ion->stream);
+    cg::adjoint_destroy(a->A);
+    delete a;
+    return CG_OK;
+}
+
+int cg_adjoint_forward(cg_adjoint* a, const double* kappa_dev, const double* theta, int n_steps,
+                       double* probes_dev)
+{
+    if (!a || !kappa_dev || !probes_dev) return fail(CG_ERR_ARG, "NULL argument");
+    if (n_steps < 1) return fail(CG_ERR_ARG, "n_steps must be positive");
+    // box stimulus of each step, with the tissue's time accumulation
+    std::vector<cg::ActiveBoxes> boxes((size_t)n_steps);
+    double tt = a->t0;
+    for (int s = 0; s < n_steps; ++s) {
+        const double tn = tt + a->t->cfg.dt;
+        boxes[s] = active_boxes(a->t, tn - a->t->cfg.dt);
+        tt = tn;
+    }
+    const int idx[2] = {cg::BuenoOrovio::tfi, cg::BuenoOrovio::tsi};
+    std::string err;
+    const int rc = cg::adjoint_forward(a->A, kappa_dev, theta, idx, theta ? 2 : 0, n_steps,
+                                       boxes, probes_dev, &err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    return CG_OK;
+}
+
+int cg_adjoint_backward(cg_adjoint* a, const double* dprobes_dev, double* dkappa_dev,
+                        double* dtheta)
+{
+    if (!a || !dprobes_dev || !dkappa_dev || !dtheta) return fail(CG_ERR_ARG, "NULL argument");
+    std::string err;
+    const int rc = cg::adjoint_backward(a->A, dprobes_dev, dkappa_dev, dtheta, &err);
+    if (rc) return fail(rc, "%s", err.c_str());
     return CG_OK;
 }
 

@@ -226,6 +226,30 @@ int cg_tissue_kernel_times(cg_tissue* t, int max, char* names, double* total_ms,
  * V and out are device pointers of n doubles. */
 int cg_mesh_laplacian(cg_mesh* m, const double* V_dev, double* out_dev);
 
+/* ------------------------------------------------------------------ adjoint
+ * Reverse-mode gradients of Vm at probe cells over n steps of an explicit, Euler-ddt,
+ * BuenoOrovio FP64 tissue with respect to a per-cell scale kappa of the (uniform,
+ * diagonal) conductivity and the model constants tau_fi, tau_si.  No reference
+ * counterpart (cardiacFoam has no derivatives); the forward map is the reference's step
+ * (myocardiumDomain.C:487-529, monodomainSolver.C:103-118, BuenoOrovio.C:107-152) with
+ * sigma_c = kappa_c sigma. */
+typedef struct cg_adjoint cg_adjoint;
+/* Snapshots the tissue's current Vm, ionic states and time as the start of every forward
+ * run; checkpoints every checkpoint_every steps. */
+int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
+                      int checkpoint_every, cg_adjoint** out);
+int cg_adjoint_destroy(cg_adjoint* a);
+/* From the snapshot: kappa_dev [n_cells] (device), theta = {tau_fi, tau_si} (host, NULL
+ * keeps the model's values), Vm (V) after each step at the probes into probes_dev
+ * [n_steps][n_probes] (device).  Synchronous.  The tissue handle itself is not advanced. */
+int cg_adjoint_forward(cg_adjoint* a, const double* kappa_dev, const double* theta, int n_steps,
+                       double* probes_dev);
+/* dLoss/dprobes (device [n_steps][n_probes], same layout) of the last forward run ->
+ * dLoss/dkappa (device [n_cells]) and dLoss/d{tau_fi, tau_si} (host [2]).  Re-runs each
+ * checkpoint segment, then sweeps it in reverse.  Synchronous. */
+int cg_adjoint_backward(cg_adjoint* a, const double* dprobes_dev, double* dkappa_dev,
+                        double* dtheta);
+
 /* y = A x on the device with the implicit tissue's matrix (the fvm::laplacian + fvm::ddt
  * assembly of monodomainSolver.C:129-135, orthogonal part):
  *   A = ddt_coeff*chi*Cm/dt I + L,  (L x)_P = sum_f sigma_nn,f/h_f^2 (x_P - x_N),

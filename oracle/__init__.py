@@ -301,6 +301,10 @@ class OracleTissue:
     def step(self, boxes=None):
         lib().orc_tissue_step(C.byref(self.T), C.byref(boxes) if boxes is not None else None)
 
+    def set_const(self, k, value):
+        """model constant k (the model's constant order) for every cell"""
+        self.T.consts[k] = float(value)
+
     def _arr(self, ptr, n):
         # copies: the C buffers are freed with the tissue object
         return np.ctypeslib.as_array(ptr, shape=(n,)).copy()

@@ -1,0 +1,108 @@
+"""Adjoint of the explicit monodomain step (BuenoOrovio, Euler ddt, FP64): the adjoint's
+forward run equals cg_tissue_step, and its gradients with respect to a per-cell
+conductivity scale and tau_fi / tau_si equal central finite differences of the loss
+computed by the CPU oracle's independent tissue run."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIG0 = np.array([0.17, 0.019, 0.019])
+BOX = {"stimulusLocationMin": [0.0, 0.0, 0.0], "stimulusLocationMax": [6e-4, 6e-4, 6e-4],
+       "stimulusDuration": 2e-3, "stimulusIntensity": 5e4, "stimulusStartTime": 0.0}
+DT = 1e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dom(gpu, shape, h):
+    mesh = gpu.HexMesh(*shape, h)
+    coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": 1.4e5,
+              "cm": 0.01, "solutionAlgorithm": "explicit",
+              "conductivity": [SIG0[0], 0, 0, 0, SIG0[1], 0, 0, 0, SIG0[2]],
+              "externalStimulus": BOX}
+    return mesh, gpu.Monodomain(mesh, coeffs, dt=DT)
+
+
+def test_adjoint_forward_equals_tissue_step(gpu, orc):
+    import torch
+    _say("adjoint forward vs tissue step")
+    shape, h, steps = (20, 12, 8), 2e-4, 400
+    mesh, dom = _dom(gpu, shape, h)
+    n = mesh.n
+    cells = np.random.default_rng(1).choice(n, 16, replace=False)
+    adj = gpu.TissueAdjoint(dom, cells, checkpoint_every=64)
+    kappa = torch.ones(n, dtype=torch.float64, device="cuda")
+    tr = gpu.probe_traces(adj, kappa, None, steps).cpu().numpy()
+    ref = np.empty((steps, len(cells)))
+    for s in range(steps):
+        dom.step(1)
+        ref[s] = dom.Vm[cells]
+    err = np.max(np.abs(tr - ref)) / np.max(np.abs(ref))
+    _say("  trace rel err %.3e" % err)
+    assert err < 1e-12
+
+
+def _oracle_loss(orc, shape, h, kappa, theta, steps, cells, target):
+    nx, ny, nz = shape
+    sig = np.zeros((kappa.size, 9))
+    sig[:, 0], sig[:, 4], sig[:, 8] = kappa * SIG0[0], kappa * SIG0[1], kappa * SIG0[2]
+    T = orc.OracleTissue(nx, ny, nz, h, sig, orc.BO, orc.EPI, DT, 1.4e5, 0.01, integrator=0,
+                         ddt=0, implicit=0, nthreads=8)
+    T.set_const(orc.const_index(orc.BO, "tauFi"), theta[0])
+    T.set_const(orc.const_index(orc.BO, "tauSi"), theta[1])
+    bx = orc.make_boxes(BOX["stimulusLocationMin"], BOX["stimulusLocationMax"], [0.0], [2e-3],
+                        [5e4])
+    loss = 0.0
+    for s in range(steps):
+        T.step(bx)
+        loss += float(np.sum((T.V[cells] - target[s]) ** 2))
+    return loss
+
+
+def test_adjoint_gradient_vs_oracle_finite_differences(gpu, orc):
+    """dL/dkappa (three random directions) and dL/dtau_fi, dL/dtau_si against central
+    differences of the oracle's loss; 300 steps with checkpoints every 64 (partial last
+    segment), L = sum over steps and probes of (Vm - target)^2."""
+    import torch
+    _say("adjoint gradient vs oracle FD")
+    shape, h, steps = (12, 10, 6), 2e-4, 300
+    mesh, dom = _dom(gpu, shape, h)
+    n = mesh.n
+    rng = np.random.default_rng(7)
+    cells = rng.choice(n, 24, replace=False)
+    theta0 = np.array([0.11, 1.8875])                   # epicardial tau_fi, tau_si
+    kappa0 = rng.uniform(0.8, 1.25, n)
+    target = -0.084 + 0.1 * rng.random((steps, cells.size))
+    adj = gpu.TissueAdjoint(dom, cells, checkpoint_every=64)
+    kappa = torch.tensor(kappa0, dtype=torch.float64, device="cuda", requires_grad=True)
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    tr = gpu.probe_traces(adj, kappa, theta, steps)
+    loss = ((tr - torch.tensor(target, device="cuda")) ** 2).sum()
+    loss.backward()
+    gk = kappa.grad.cpu().numpy()
+    gt = theta.grad.numpy()
+    l_or = _oracle_loss(orc, shape, h, kappa0, theta0, steps, cells, target)
+    _say("  loss gpu %.12e oracle %.12e" % (loss.item(), l_or))
+    assert abs(loss.item() - l_or) <= 1e-10 * abs(l_or)
+    eps = 1e-6
+    for k in range(3):
+        d = rng.standard_normal(n)
+        fp = _oracle_loss(orc, shape, h, kappa0 + eps * d, theta0, steps, cells, target)
+        fm = _oracle_loss(orc, shape, h, kappa0 - eps * d, theta0, steps, cells, target)
+        fd, ad = (fp - fm) / (2 * eps), float(gk @ d)
+        _say("  kappa direction %d: adjoint %.9e fd %.9e" % (k, ad, fd))
+        assert abs(ad - fd) <= 1e-5 * abs(fd)
+    for i in range(2):
+        e = np.zeros(2)
+        e[i] = eps * theta0[i]
+        fp = _oracle_loss(orc, shape, h, kappa0, theta0 + e, steps, cells, target)
+        fm = _oracle_loss(orc, shape, h, kappa0, theta0 - e, steps, cells, target)
+        fd = (fp - fm) / (2 * e[i])
+        _say("  theta %d: adjoint %.9e fd %.9e" % (i, gt[i], fd))
+        assert abs(gt[i] - fd) <= 1e-5 * abs(fd)
