@@ -350,6 +350,113 @@ def run_batch_c2(cf, n=4096, dt_ms=0.01, seed=20240602):
     return res
 
 
+def gaussian_field(shape, h, corr_len, rng):
+    """Unit-variance Gaussian random field with a Gaussian covariance of correlation length
+    corr_len (m), by spectral synthesis of seeded white noise (z, y, x order flattened to
+    blockMesh cell order)."""
+    nx, ny, nz = shape
+    noise = rng.standard_normal((nz, ny, nx))
+    kx = np.fft.fftfreq(nx, d=h) * 2 * np.pi
+    ky = np.fft.fftfreq(ny, d=h) * 2 * np.pi
+    kz = np.fft.fftfreq(nz, d=h) * 2 * np.pi
+    K2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    filt = np.exp(-K2 * corr_len ** 2 / 4.0)      # sqrt of the Gaussian spectrum
+    g = np.real(np.fft.ifftn(np.fft.fftn(noise) * filt))
+    g = (g - g.mean()) / g.std()
+    return g.ravel()
+
+
+def run_adjoint_c4(cf, torch, shape=(64, 64, 32), h=2e-4, steps=5000, n_probes=256,
+                   n_dirs=32, fd_steps=500, seed=20240604):
+    """Adjoint workload: BuenoOrovio epi on 64x64x32 (h 0.2 mm), explicit Euler, dt 0.01 ms,
+    50 ms; loss = sum of squared Vm misfits at 256 seeded probes against a target run whose
+    conductivity scale is exp(0.3 g), g a unit-variance Gaussian field with 2 mm correlation
+    length; gradient in kappa (per cell) and tau_fi, tau_si at kappa = 1 with checkpoints
+    every 100 steps, timed against one plain forward run of the tissue step.
+
+    Central differences: the model's Heaviside switches (BuenoOrovio_2008.H:171-205) make
+    the discrete loss piecewise smooth with jumps wherever a perturbation moves a threshold
+    crossing by one step, so finite differences only resolve the derivative while no
+    crossing moves.  The 32-direction check runs on the first fd_steps steps (5 ms, where
+    eps = 1e-6 moves none); for the full 50 ms the spread of one direction's difference
+    quotient over eps is reported beside the adjoint value."""
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": 1.4e5,
+              "cm": 0.01, "solutionAlgorithm": "explicit",
+              "conductivity": [0.17, 0, 0, 0, 0.019, 0, 0, 0, 0.019],
+              "externalStimulus": {"stimulusLocationMin": [0, 0, 0],
+                                   "stimulusLocationMax": [1.5e-3] * 3,
+                                   "stimulusDuration": 2e-3, "stimulusIntensity": 5e4,
+                                   "stimulusStartTime": 0.0}}
+    dt = 1e-5
+    mesh = cf.HexMesh(*shape, h)
+    dom = cf.Monodomain(mesh, coeffs, dt=dt)
+    dom.step(10)
+    dom.sync()
+    dom = cf.Monodomain(mesh, coeffs, dt=dt)
+    t0 = time.perf_counter()
+    dom.step(steps)
+    dom.sync()
+    t_plain = time.perf_counter() - t0
+    dom = cf.Monodomain(mesh, coeffs, dt=dt)
+    cells = rng.choice(n, n_probes, replace=False)
+    adj = cf.TissueAdjoint(dom, cells, checkpoint_every=100)
+    kt = torch.tensor(np.exp(0.3 * gaussian_field(shape, h, 2e-3, rng)), device="cuda")
+    theta0 = torch.tensor([0.11, 1.8875], dtype=torch.float64)
+    with torch.no_grad():
+        target = cf.probe_traces(adj, kt, theta0, steps).clone()
+
+    def grad_at(nst):
+        kappa = torch.ones(n, dtype=torch.float64, device="cuda", requires_grad=True)
+        theta = theta0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        L = ((cf.probe_traces(adj, kappa, theta, nst) - target[:nst]) ** 2).sum()
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        return L.item(), kappa.grad.detach(), theta.grad.clone(), b - a, time.perf_counter() - b
+
+    def loss_at(k, th, nst):
+        with torch.no_grad():
+            return ((cf.probe_traces(adj, k, th, nst) - target[:nst]) ** 2).sum().item()
+
+    grad_at(steps)                                    # warm-up: allocations, first touch
+    L, g, gth, t_fwd, t_bwd = grad_at(steps)
+    ones = torch.ones(n, dtype=torch.float64, device="cuda")
+    # full horizon: one direction, difference quotients over eps
+    d0 = torch.tensor(rng.standard_normal(n), device="cuda")
+    spread = {}
+    for eps in (1e-4, 1e-6, 1e-8):
+        spread["%g" % eps] = (loss_at(ones + eps * d0, theta0, steps) -
+                              loss_at(ones - eps * d0, theta0, steps)) / (2 * eps)
+    # 32 directions on the first fd_steps steps
+    _, gs, gths, _, _ = grad_at(fd_steps)
+    eps, rel = 1e-6, []
+    for _ in range(n_dirs):
+        d = torch.tensor(rng.standard_normal(n), device="cuda")
+        fd = (loss_at(ones + eps * d, theta0, fd_steps) -
+              loss_at(ones - eps * d, theta0, fd_steps)) / (2 * eps)
+        rel.append(abs(float(gs @ d) - fd) / max(abs(fd), 1e-300))
+    th_rel = []
+    for i in range(2):
+        e = torch.zeros(2, dtype=torch.float64)
+        e[i] = eps * theta0[i]
+        fd = (loss_at(ones, theta0 + e, fd_steps) - loss_at(ones, theta0 - e, fd_steps)) / (2 * e[i].item())
+        th_rel.append(abs(gths[i].item() - fd) / max(abs(fd), 1e-300))
+    return {"workload": "bo_adjoint_131k", "cells": n, "steps": steps, "probes": n_probes,
+            "checkpoint_every": 100, "loss": L,
+            "plain_forward_s": t_plain, "adjoint_forward_s": t_fwd, "backward_s": t_bwd,
+            "gradient_over_forward": (t_fwd + t_bwd) / t_plain,
+            "grad_theta": [float(x) for x in gth],
+            "fd_check": {"steps": fd_steps, "directions": n_dirs, "eps": eps,
+                         "kappa_rel_err_max": max(rel), "kappa_rel_err_median": float(np.median(rel)),
+                         "theta_rel_err": th_rel},
+            "full_horizon_one_direction": {"adjoint": float(g @ d0), "difference_quotients": spread}}
+
+
 def cpu_baseline(w, sigma, sample_steps):
     """The CPU FP64 oracle (OpenMP over all host cores) on the same workload: the full
     mesh for `sample_steps` steps after a 2-step start, timed as cell-steps/s."""
@@ -473,6 +580,7 @@ def main():
     c0 = run_explicit_c0(cf, torch) if args.workload != "bo_explicit_147k" else None
     c1 = run_batch_c1(cf) if not args.no_batch else None
     c2 = run_batch_c2(cf) if not args.no_batch else None
+    c4 = run_adjoint_c4(cf, torch) if not args.no_batch else None
 
     cpu = None
     if not args.no_cpu:
@@ -505,6 +613,7 @@ def main():
         "explicit_c0": c0,
         "batch_c1": c1,
         "batch_c2": c2,
+        "adjoint_c4": c4,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -36,49 +36,111 @@ constexpr int AG = 1024;  // blocks of the backward kernel (parameter partials)
 
 using BO = BuenoOrovio;
 
-struct AdjStep {
-    HexGeom g;
-    const double* fx;  // face coefficients a_f (per unit volume), + faces, 0 on walls
-    const double* fy;
-    const double* fz;
-    double dt, dt_model, chiCm;
-    ActiveBoxes boxes;
+// slot layout of the state store: V, v, w, s, then the step's transcendental factors
+// exp(-dt/tau_v), exp(-dt/tau_w), exp(-dt/tau_s), tanh of tau_w-, s_inf and tau_so
+enum { SL_V, SL_v, SL_w, SL_s, SL_Ev, SL_Ew, SL_Es, SL_thw, SL_ths, SL_thso, SL_N };
+
+// stimulus boxes of the tissue (myocardiumDomain.C:307-342), active when the step start
+// t0 lies in [t_start, t_start + duration]
+struct BoxTable {
+    int n;
+    double lo[MAX_BOXES][3], hi[MAX_BOXES][3], t0[MAX_BOXES], dur[MAX_BOXES], I[MAX_BOXES];
 };
 
+struct AdjGeom {
+    HexGeom g;
+    const double* fx;   // face coefficients a_f (per unit volume), + faces, 0 on walls
+    const double* fy;
+    const double* fz;
+    const double* prm;  // BuenoOrovio constants (device)
+    const BoxTable* boxes;
+    double dt, dt_model, chiCm;
+};
+
+CG_DEV double box_src(int i, int j, int k, const HexGeom& g, const BoxTable& b, double t0)
+{
+    double src = 0.0;
+    const double p0 = (i + 0.5) * g.hx, p1 = (j + 0.5) * g.hy, p2 = (k + 0.5) * g.hz;
+    for (int q = 0; q < b.n; ++q) {
+        if (t0 < b.t0[q] || t0 > b.t0[q] + b.dur[q]) continue;
+        if (p0 >= b.lo[q][0] && p0 <= b.hi[q][0] && p1 >= b.lo[q][1] && p1 <= b.hi[q][1] &&
+            p2 >= b.lo[q][2] && p2 <= b.hi[q][2])
+            src += b.I[q];
+    }
+    return src;
+}
+
+// The BuenoOrovio tissue step (u frozen) written out with its transcendental factors,
+// operation for operation as BuenoOrovio::gates and ::jion; returns Jion at the step end.
+struct BoFactors {
+    double Ev, Ew, Es, thw, ths, thso;
+};
+CG_DEV double bo_step(double u, double& v, double& w, double& s, double dt, const double* p,
+                      BoFactors& f)
+{
+    const bool Hv = pos(u - p[BO::thv]), Hw = pos(u - p[BO::thw]), Ho = pos(u - p[BO::tho]),
+               Hvm = pos(u - p[BO::thvm]);
+    const double vinf = Hv ? 0.0 : (pos(p[BO::thvm] - u) ? 1.0 : 0.0);
+    const double tv = Hv ? p[BO::tvp] : (Hvm ? p[BO::tv2m] : p[BO::tv1m]);
+    f.thw = tanh(p[BO::kwm] * (u - p[BO::uwm]));
+    const double twm = p[BO::tw1m] + (p[BO::tw2m] - p[BO::tw1m]) * (1.0 + f.thw) / 2.0;
+    const double winf = Hw ? 0.0 : (Ho ? p[BO::winfstar] : 1.0 - u / p[BO::twinf]);
+    const double tw = Hw ? p[BO::twp] : twm;
+    f.ths = tanh(p[BO::ks] * (u - p[BO::us]));
+    const double sinf = (1.0 + f.ths) / 2.0;
+    const double ts = Hw ? p[BO::ts2] : p[BO::ts1];
+    f.Ev = exp(-dt / tv);
+    f.Ew = exp(-dt / tw);
+    f.Es = exp(-dt / ts);
+    v = vinf + (v - vinf) * f.Ev;
+    w = winf + (w - winf) * f.Ew;
+    s = sinf + (s - sinf) * f.Es;
+    f.thso = tanh(p[BO::kso] * (u - p[BO::uso]));
+    const double tso = p[BO::tso1] + (p[BO::tso2] - p[BO::tso1]) * (1.0 + f.thso) / 2.0;
+    const double to = Ho ? p[BO::to2] : p[BO::to1];
+    const double jfi = Hv ? -v * (u - p[BO::thv]) * (p[BO::uu] - u) / p[BO::tfi] : 0.0;
+    const double jso = Hw ? 1.0 / tso : (u - p[BO::uo]) / to;
+    const double jsi = Hw ? -w * s / p[BO::tsi] : 0.0;
+    return jfi + jso + jsi;
+}
+
 // ------------------------------------------------------------------ forward
-// One step from slot (Vin, yin[3][n]) into (Vout, yout); Vm at probes into probe_row.
-__global__ __launch_bounds__(ABS) void k_adj_forward(AdjStep s, KParams<double> P,
-                                                     const double* __restrict__ Vin,
-                                                     const double* __restrict__ yin,
-                                                     double* __restrict__ Vout,
-                                                     double* __restrict__ yout,
+// One step from slot `in` into slot `out` (planes SL_V..SL_s, [plane][n]); when aux is
+// not null (planes SL_Ev.. of the input slot) the step's factors are stored there; Vm at
+// probes into probe_row.
+__global__ __launch_bounds__(ABS) void k_adj_forward(AdjGeom G, double t0,
+                                                     const double* __restrict__ in,
+                                                     double* __restrict__ out,
+                                                     double* __restrict__ aux,
                                                      const int* __restrict__ probe_slot,
                                                      double* __restrict__ probe_row)
 {
-    const HexGeom& g = s.g;
+    const HexGeom& g = G.g;
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int64_t c = (int64_t)blockIdx.x * ABS + threadIdx.x;
     if (c >= n) return;
     const IJK q = ijk_of((uint32_t)c, g);
-    double p[BO::NP];
-#pragma unroll
-    for (int k = 0; k < BO::NP; ++k) p[k] = P.p[k];
-    const double one[4] = {1, 1, 1, 1};
-    double y[BO::NS];
-    y[1] = yin[c];
-    y[2] = yin[n + c];
-    y[3] = yin[2 * n + c];
-    const double vn = Vin[c];
+    double v = in[SL_v * n + c], w = in[SL_w * n + c], s = in[SL_s * n + c];
+    const double vn = in[SL_V * n + c];
     const double u = (vn * 1000.0 + BO::VOFF) / BO::VSCALE;
-    const double iion = Ops<BO>::tissue_step(u, y, s.dt_model, p, 1, one) * BO::ISCALE;
-    yout[c] = y[1];
-    yout[n + c] = y[2];
-    yout[2 * n + c] = y[3];
-    const HexCoefs a{0, 0, 0, s.fx, s.fy, s.fz};
-    const double lap = hex_laplacian<false>(Vin, c, q, g, a);
-    const double src = box_source(q.i, q.j, q.k, g, s.boxes);
-    const double vnew = (1.0 * vn + s.dt / s.chiCm * (lap - s.chiCm * iion + src)) / 1.0;
-    Vout[c] = vnew;
+    BoFactors f;
+    const double iion = bo_step(u, v, w, s, G.dt_model, G.prm, f) * BO::ISCALE;
+    out[SL_v * n + c] = v;
+    out[SL_w * n + c] = w;
+    out[SL_s * n + c] = s;
+    if (aux) {  // planes SL_Ev.. of the input slot
+        aux[(SL_Ev - SL_Ev) * n + c] = f.Ev;
+        aux[(SL_Ew - SL_Ev) * n + c] = f.Ew;
+        aux[(SL_Es - SL_Ev) * n + c] = f.Es;
+        aux[(SL_thw - SL_Ev) * n + c] = f.thw;
+        aux[(SL_ths - SL_Ev) * n + c] = f.ths;
+        aux[(SL_thso - SL_Ev) * n + c] = f.thso;
+    }
+    const HexCoefs hc{0, 0, 0, G.fx, G.fy, G.fz};
+    const double lap = hex_laplacian<false>(in + SL_V * n, c, q, g, hc);
+    const double src = box_src(q.i, q.j, q.k, g, *G.boxes, t0);
+    const double vnew = (1.0 * vn + G.dt / G.chiCm * (lap - G.chiCm * iion + src)) / 1.0;
+    out[SL_V * n + c] = vnew;
     if (probe_row) {
         const int slot = probe_slot[c];
         if (slot >= 0) probe_row[slot] = vnew;
@@ -108,39 +170,31 @@ struct CellAdj {
     double lv, lw, ls, lu, dtfi, dtsi;
 };
 
-// Vector-Jacobian product of the BuenoOrovio tissue step (u frozen) at (u, v, w, s):
-// cotangents (lv1, lw1, ls1) of the updated gates and lJ of Jion at the step end.
-CG_DEV CellAdj bo_vjp(double u, double v, double w, double s, double dt, const double* p,
-                      double lv1, double lw1, double ls1, double lJ)
+// Vector-Jacobian product of bo_step at (u, v, w, s) with its stored factors f: cotangents
+// (lv1, lw1, ls1) of the updated gates and lJ of Jion.  Derivatives by hand from the
+// model equations; the Heaviside switches are piecewise constant (zero derivative).
+CG_DEV CellAdj bo_vjp(double u, double v, double w, double s, const BoFactors& f, double dt,
+                      const double* p, double lv1, double lw1, double ls1, double lJ)
 {
-    const bool Hv = pos(u - p[BO::thv]), Hw = pos(u - p[BO::thw]), Ho = pos(u - p[BO::tho]),
-               Hvm = pos(u - p[BO::thvm]);
-    // gates, as BuenoOrovio::gates
+    const bool Hv = pos(u - p[BO::thv]), Hw = pos(u - p[BO::thw]), Ho = pos(u - p[BO::tho]);
     const double vinf = Hv ? 0.0 : (pos(p[BO::thvm] - u) ? 1.0 : 0.0);
-    const double tv = Hv ? p[BO::tvp] : (Hvm ? p[BO::tv2m] : p[BO::tv1m]);
-    const double thw_ = tanh(p[BO::kwm] * (u - p[BO::uwm]));
-    const double twm = p[BO::tw1m] + (p[BO::tw2m] - p[BO::tw1m]) * (1.0 + thw_) / 2.0;
-    const double dtwm = (p[BO::tw2m] - p[BO::tw1m]) * p[BO::kwm] * (1.0 - thw_ * thw_) / 2.0;
+    const double twm = p[BO::tw1m] + (p[BO::tw2m] - p[BO::tw1m]) * (1.0 + f.thw) / 2.0;
+    const double dtwm = (p[BO::tw2m] - p[BO::tw1m]) * p[BO::kwm] * (1.0 - f.thw * f.thw) / 2.0;
     const double winf = Hw ? 0.0 : (Ho ? p[BO::winfstar] : 1.0 - u / p[BO::twinf]);
     const double dwinf = (!Hw && !Ho) ? -1.0 / p[BO::twinf] : 0.0;
     const double tw = Hw ? p[BO::twp] : twm;
     const double dtw = Hw ? 0.0 : dtwm;
-    const double ths = tanh(p[BO::ks] * (u - p[BO::us]));
-    const double sinf = (1.0 + ths) / 2.0;
-    const double dsinf = p[BO::ks] * (1.0 - ths * ths) / 2.0;
-    const double ts = Hw ? p[BO::ts2] : p[BO::ts1];
-    const double Ev = exp(-dt / tv), Ew = exp(-dt / tw), Es = exp(-dt / ts);
-    const double v1 = vinf + (v - vinf) * Ev;
-    const double w1 = winf + (w - winf) * Ew;
-    const double s1 = sinf + (s - sinf) * Es;
-    const double dw1du = dwinf * (1.0 - Ew) + (w - winf) * Ew * dt / (tw * tw) * dtw;
-    const double ds1du = dsinf * (1.0 - Es);
-    // currents at (u, v1, w1, s1), as BuenoOrovio::jion
+    const double sinf = (1.0 + f.ths) / 2.0;
+    const double dsinf = p[BO::ks] * (1.0 - f.ths * f.ths) / 2.0;
+    const double v1 = vinf + (v - vinf) * f.Ev;
+    const double w1 = winf + (w - winf) * f.Ew;
+    const double s1 = sinf + (s - sinf) * f.Es;
+    const double dw1du = dwinf * (1.0 - f.Ew) + (w - winf) * f.Ew * dt / (tw * tw) * dtw;
+    const double ds1du = dsinf * (1.0 - f.Es);
     const double tfi = p[BO::tfi], tsi = p[BO::tsi];
     const double a1 = u - p[BO::thv], a2 = p[BO::uu] - u;
-    const double thso = tanh(p[BO::kso] * (u - p[BO::uso]));
-    const double tso = p[BO::tso1] + (p[BO::tso2] - p[BO::tso1]) * (1.0 + thso) / 2.0;
-    const double dtso = (p[BO::tso2] - p[BO::tso1]) * p[BO::kso] * (1.0 - thso * thso) / 2.0;
+    const double tso = p[BO::tso1] + (p[BO::tso2] - p[BO::tso1]) * (1.0 + f.thso) / 2.0;
+    const double dtso = (p[BO::tso2] - p[BO::tso1]) * p[BO::kso] * (1.0 - f.thso * f.thso) / 2.0;
     const double to = Ho ? p[BO::to2] : p[BO::to1];
     const double dJ_dv1 = Hv ? -a1 * a2 / tfi : 0.0;
     const double dJ_dw1 = Hw ? -s1 / tsi : 0.0;
@@ -148,9 +202,9 @@ CG_DEV CellAdj bo_vjp(double u, double v, double w, double s, double dt, const d
     const double dJ_du = (Hv ? -v1 * (a2 - a1) / tfi : 0.0) + (Hw ? -dtso / (tso * tso) : 1.0 / to);
     const double cv = lv1 + lJ * dJ_dv1, cw = lw1 + lJ * dJ_dw1, cs = ls1 + lJ * dJ_ds1;
     CellAdj r;
-    r.lv = cv * Ev;
-    r.lw = cw * Ew;
-    r.ls = cs * Es;
+    r.lv = cv * f.Ev;
+    r.lw = cw * f.Ew;
+    r.ls = cs * f.Es;
     r.lu = lJ * dJ_du + cw * dw1du + cs * ds1du;  // v1 does not depend on u (a.e.)
     r.dtfi = Hv ? lJ * v1 * a1 * a2 / (tfi * tfi) : 0.0;
     r.dtsi = Hw ? lJ * w1 * s1 / (tsi * tsi) : 0.0;
@@ -178,15 +232,13 @@ template <int NV> CG_DEV void block_sum_a(double (&v)[NV])
 }
 
 // ----------------------------------------------------------------- backward
-// Adjoint of step n.  mu = dLoss/dV^{n+1} (includes the probe seed of V^{n+1}),
-// ly1 = dLoss/dy^{n+1}; writes lV = dLoss/dV^n (plus the seed of V^n from seed_row when
-// not null), ly = dLoss/dy^n, adds into gk and into the per-block parameter partials.
-// Grid-stride over cells with AG blocks (fixed block -> cell assignment, so the partial
-// of block b is always accumulated by block b).
-__global__ __launch_bounds__(ABS) void k_adj_backward(AdjStep s, KParams<double> P, double gxh,
-                                                      double gyh, double gzh,
-                                                      const double* __restrict__ Vn,
-                                                      const double* __restrict__ yn,
+// Adjoint of step n from its slot (state and factors of step n).  mu = dLoss/dV^{n+1}
+// (with the probe seed of V^{n+1}), ly1 = dLoss/dy^{n+1}; writes lV = dLoss/dV^n (plus the
+// seed of V^n from seed_row when not null) and ly = dLoss/dy^n, adds into gk and into the
+// per-block parameter partials.  Grid-stride over cells with AG blocks (block b always
+// accumulates partial b).
+__global__ __launch_bounds__(ABS) void k_adj_backward(AdjGeom G, double gxh, double gyh,
+                                                      double gzh, const double* __restrict__ sl,
                                                       const double* __restrict__ mu,
                                                       const double* __restrict__ ly1,
                                                       double* __restrict__ lV,
@@ -196,33 +248,40 @@ __global__ __launch_bounds__(ABS) void k_adj_backward(AdjStep s, KParams<double>
                                                       double* __restrict__ gk,
                                                       double* __restrict__ gtheta)
 {
-    const HexGeom& g = s.g;
+    const HexGeom& g = G.g;
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    double p[BO::NP];
-#pragma unroll
-    for (int k = 0; k < BO::NP; ++k) p[k] = P.p[k];
-    const double a = s.dt / s.chiCm;
-    const double lJ_per_mu = -s.dt * BO::ISCALE;
+    const double* __restrict__ p = G.prm;
+    const double a = G.dt / G.chiCm;
+    const double lJ_per_mu = -G.dt * BO::ISCALE;
     const double du_dV = 1000.0 / BO::VSCALE;
+    const double* __restrict__ Vn = sl + SL_V * n;
     double acc[2] = {0.0, 0.0};
     for (int64_t c = (int64_t)blockIdx.x * ABS + threadIdx.x; c < n; c += (int64_t)gridDim.x * ABS) {
         const IJK q = ijk_of((uint32_t)c, g);
         const Nbr b = nbr_of(c, q, g);
         const double muc = mu[c];
-        const double u = (Vn[c] * 1000.0 + BO::VOFF) / BO::VSCALE;
-        const CellAdj r = bo_vjp(u, yn[c], yn[n + c], yn[2 * n + c], s.dt_model, p, ly1[c],
-                                 ly1[n + c], ly1[2 * n + c], lJ_per_mu * muc);
+        const double vc = Vn[c];
+        const double u = (vc * 1000.0 + BO::VOFF) / BO::VSCALE;
+        BoFactors f;
+        f.Ev = sl[SL_Ev * n + c];
+        f.Ew = sl[SL_Ew * n + c];
+        f.Es = sl[SL_Es * n + c];
+        f.thw = sl[SL_thw * n + c];
+        f.ths = sl[SL_ths * n + c];
+        f.thso = sl[SL_thso * n + c];
+        const CellAdj r = bo_vjp(u, sl[SL_v * n + c], sl[SL_w * n + c], sl[SL_s * n + c], f,
+                                 G.dt_model, p, ly1[c], ly1[n + c], ly1[2 * n + c],
+                                 lJ_per_mu * muc);
         ly[c] = r.lv;
         ly[n + c] = r.lw;
         ly[2 * n + c] = r.ls;
         acc[0] += r.dtfi;
         acc[1] += r.dtsi;
         // diffusion adjoint (L symmetric) and the kappa gradient, one pass over the faces
-        const double fxp = s.fx[c], fxm = s.fx[b.xm], fyp = s.fy[c], fym = s.fy[b.ym];
-        const double fzp = s.fz[c], fzm = s.fz[b.zm];
+        const double fxp = G.fx[c], fxm = G.fx[b.xm], fyp = G.fy[c], fym = G.fy[b.ym];
+        const double fzp = G.fz[c], fzm = G.fz[b.zm];
         const double mxp = mu[b.xp], mxm = mu[b.xm], myp = mu[b.yp], mym = mu[b.ym];
         const double mzp = mu[b.zp], mzm = mu[b.zm];
-        const double vc = Vn[c];
         const double Lmu = fxp * (mxp - muc) + fxm * (mxm - muc) + fyp * (myp - muc) +
                            fym * (mym - muc) + fzp * (mzp - muc) + fzm * (mzm - muc);
         double lv = muc + a * Lmu + du_dV * r.lu;
@@ -277,26 +336,30 @@ struct Adjoint {
     ModelParams P;
     int K;                      // checkpoint interval
     int n_probes;
-    std::vector<int64_t> probes;
+    double* d_prm = nullptr;    // model constants [NP]
+    BoxTable* d_boxes = nullptr;
     int* d_slot = nullptr;      // [n] probe slot or -1
     double* d_V0 = nullptr;     // initial state
     double* d_y0 = nullptr;     // [3][n]
     double *fx = nullptr, *fy = nullptr, *fz = nullptr;
     double* ck = nullptr;       // checkpoints [n_ck][4][n]
     int n_ck = 0;
-    double* seg = nullptr;      // recompute store [K+1][4][n]
+    double* seg = nullptr;      // state store [slots][SL_N][n]
+    int seg_slots = 0;
+    bool store_all = false;     // last forward kept every step's slot (no recompute)
     double *lV[2] = {nullptr, nullptr}, *ly[2] = {nullptr, nullptr};
     double* gtheta = nullptr;   // [AG][2]
     int last_steps = 0;
-    std::vector<ActiveBoxes> boxes;  // per step of the last forward
+    std::vector<double> t0;     // start time of every step of the last forward
     hipStream_t stream;
 };
 
 void adjoint_destroy(Adjoint* A)
 {
     if (!A) return;
-    void* p[] = {A->d_slot, A->d_V0, A->d_y0, A->fx,   A->fy,   A->fz,   A->ck,
-                 A->seg,    A->lV[0], A->lV[1], A->ly[0], A->ly[1], A->gtheta};
+    void* p[] = {A->d_prm, A->d_boxes, A->d_slot, A->d_V0, A->d_y0,  A->fx,    A->fy,
+                 A->fz,    A->ck,      A->seg,    A->lV[0], A->lV[1], A->ly[0], A->ly[1],
+                 A->gtheta};
     for (void* q : p)
         if (q) hipFree(q);
     delete A;
@@ -304,8 +367,8 @@ void adjoint_destroy(Adjoint* A)
 
 int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, double dt,
                    double chiCm, const ModelParams& P, const double* V0, const void* y0_states,
-                   int K, int n_probes, const int64_t* probes, hipStream_t stream,
-                   std::string* err)
+                   const ActiveBoxesAll& boxes, int K, int n_probes, const int64_t* probes,
+                   hipStream_t stream, std::string* err)
 {
     Adjoint* A = new Adjoint();
     A->g = g;
@@ -319,16 +382,30 @@ int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, d
     A->P = P;
     A->K = K;
     A->n_probes = n_probes;
-    A->probes.assign(probes, probes + n_probes);
     A->stream = stream;
     const int64_t n = A->n;
+    std::vector<int> slot((size_t)n, -1);
+    for (int k = 0; k < n_probes; ++k) {
+        if (probes[k] < 0 || probes[k] >= n) {
+            delete A;
+            *err = "probe cell out of range";
+            return 1;
+        }
+        if (slot[probes[k]] >= 0) {
+            delete A;
+            *err = "probe cell listed twice";
+            return 1;
+        }
+        slot[probes[k]] = k;
+    }
     bool ok = true;
     auto M = [&](void** p, size_t b) { ok &= hipMalloc(p, b) == hipSuccess; };
+    M((void**)&A->d_prm, sizeof(double) * MAX_PARAMS);
+    M((void**)&A->d_boxes, sizeof(BoxTable));
     M((void**)&A->d_slot, sizeof(int) * n);
     M((void**)&A->d_V0, 8 * n);
     M((void**)&A->d_y0, 8 * 3 * n);
     M((void**)&A->fx, 8 * n); M((void**)&A->fy, 8 * n); M((void**)&A->fz, 8 * n);
-    M((void**)&A->seg, 8 * 4 * n * (size_t)(K + 1));
     for (int i = 0; i < 2; ++i) { M((void**)&A->lV[i], 8 * n); M((void**)&A->ly[i], 8 * 3 * n); }
     M((void**)&A->gtheta, 8 * 2 * AG);
     if (!ok) {
@@ -336,20 +413,15 @@ int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, d
         *err = "adjoint allocation failed";
         return 5;
     }
-    std::vector<int> slot((size_t)n, -1);
-    for (int k = 0; k < n_probes; ++k) {
-        if (probes[k] < 0 || probes[k] >= n) {
-            adjoint_destroy(A);
-            *err = "probe cell out of range";
-            return 1;
-        }
-        if (slot[probes[k]] >= 0) {
-            adjoint_destroy(A);
-            *err = "probe cell listed twice";
-            return 1;
-        }
-        slot[probes[k]] = k;
+    BoxTable bt{};
+    bt.n = boxes.n;
+    for (int b = 0; b < boxes.n; ++b) {
+        for (int d = 0; d < 3; ++d) { bt.lo[b][d] = boxes.lo[b][d]; bt.hi[b][d] = boxes.hi[b][d]; }
+        bt.t0[b] = boxes.t0[b];
+        bt.dur[b] = boxes.dur[b];
+        bt.I[b] = boxes.I[b];
     }
+    hipMemcpy(A->d_boxes, &bt, sizeof bt, hipMemcpyHostToDevice);
     hipMemcpy(A->d_slot, slot.data(), sizeof(int) * n, hipMemcpyHostToDevice);
     hipMemcpy(A->d_V0, V0, 8 * n, hipMemcpyDeviceToDevice);
     // gates v, w, s: state planes 1..3 of the ionic handle
@@ -358,60 +430,84 @@ int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, d
     return 0;
 }
 
-static AdjStep step_of(const Adjoint* A, const ActiveBoxes& bx)
+static AdjGeom geom_of(const Adjoint* A)
 {
-    AdjStep s;
-    s.g = A->g;
-    s.fx = A->fx;
-    s.fy = A->fy;
-    s.fz = A->fz;
-    s.dt = A->dt;
-    s.dt_model = A->dt_model;
-    s.chiCm = A->chiCm;
-    s.boxes = bx;
-    return s;
+    AdjGeom G;
+    G.g = A->g;
+    G.fx = A->fx;
+    G.fy = A->fy;
+    G.fz = A->fz;
+    G.prm = A->d_prm;
+    G.boxes = A->d_boxes;
+    G.dt = A->dt;
+    G.dt_model = A->dt_model;
+    G.chiCm = A->chiCm;
+    return G;
+}
+
+static bool ensure(double** buf, int* have, int want, size_t per)
+{
+    if (want <= *have) return true;
+    if (*buf) hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    if (hipMalloc((void**)buf, per * (size_t)want) != hipSuccess) return false;
+    *have = want;
+    return true;
 }
 
 int adjoint_forward(Adjoint* A, const double* kappa, const double* theta, const int* theta_index,
-                    int n_theta, int n_steps, const std::vector<ActiveBoxes>& boxes,
-                    double* probes, std::string* err)
+                    int n_theta, int n_steps, const std::vector<double>& t0, double* probes,
+                    size_t store_budget, std::string* err)
 {
     const int64_t n = A->n;
     const unsigned nb = blocks_of(n);
+    const size_t slot_bytes = 8 * (size_t)SL_N * n;
     for (int k = 0; k < n_theta; ++k) A->P.p[theta_index[k]] = theta[k];
-    const int n_ck = (n_steps + A->K - 1) / A->K;
-    if (n_ck > A->n_ck) {
-        if (A->ck) hipFree(A->ck);
-        A->ck = nullptr;
-        if (hipMalloc((void**)&A->ck, 8 * 4 * n * (size_t)n_ck) != hipSuccess) {
-            A->n_ck = 0;
-            *err = "checkpoint allocation failed";
-            return 5;
-        }
-        A->n_ck = n_ck;
+    // every step's slot kept when the interval covers the run and the store fits the budget
+    A->store_all = A->K >= n_steps && slot_bytes * (size_t)(n_steps + 1) <= store_budget;
+    const int slots = A->store_all ? n_steps + 1 : A->K + 1;
+    if (!ensure(&A->seg, &A->seg_slots, slots, slot_bytes)) {
+        *err = "state store allocation failed";
+        return 5;
     }
+    const int n_ck = A->store_all ? 0 : (n_steps + A->K - 1) / A->K;
+    if (!ensure(&A->ck, &A->n_ck, n_ck, 8 * 4 * (size_t)n)) {
+        *err = "checkpoint allocation failed";
+        return 5;
+    }
+    hipMemcpyAsync(A->d_prm, A->P.p, sizeof(double) * MAX_PARAMS, hipMemcpyHostToDevice,
+                   A->stream);
     hipLaunchKernelGGL(k_adj_faces, dim3(nb), dim3(ABS), 0, A->stream, A->g, kappa, A->gx,
                        A->gy, A->gz, A->fx, A->fy, A->fz);
-    // ping-pong between two slots of the segment store
-    double* slot[2] = {A->seg, A->seg + 4 * n};
-    hipMemcpyAsync(slot[0], A->d_V0, 8 * n, hipMemcpyDeviceToDevice, A->stream);
-    hipMemcpyAsync(slot[0] + n, A->d_y0, 8 * 3 * n, hipMemcpyDeviceToDevice, A->stream);
-    const KParams<double> KP = kparams<double>(A->P);
+    hipMemcpyAsync(A->seg + SL_V * n, A->d_V0, 8 * n, hipMemcpyDeviceToDevice, A->stream);
+    hipMemcpyAsync(A->seg + SL_v * n, A->d_y0, 8 * 3 * n, hipMemcpyDeviceToDevice, A->stream);
+    const AdjGeom G = geom_of(A);
     for (int st = 0; st < n_steps; ++st) {
-        double* in = slot[st & 1];
-        double* outp = slot[(st + 1) & 1];
-        if (st % A->K == 0)
-            hipMemcpyAsync(A->ck + (size_t)(st / A->K) * 4 * n, in, 8 * 4 * n,
-                           hipMemcpyDeviceToDevice, A->stream);
-        hipLaunchKernelGGL(k_adj_forward, dim3(nb), dim3(ABS), 0, A->stream, step_of(A, boxes[st]),
-                           KP, in, in + n, outp, outp + n, A->d_slot,
+        double* in;
+        double* outp;
+        double* aux = nullptr;
+        if (A->store_all) {
+            in = A->seg + (size_t)st * SL_N * n;
+            outp = in + (size_t)SL_N * n;
+            aux = in + SL_Ev * n;
+        } else {  // two slots in turn, a checkpoint every K steps
+            in = A->seg + (size_t)(st & 1) * SL_N * n;
+            outp = A->seg + (size_t)((st + 1) & 1) * SL_N * n;
+            if (st % A->K == 0)
+                hipMemcpyAsync(A->ck + (size_t)(st / A->K) * 4 * n, in, 8 * 4 * n,
+                               hipMemcpyDeviceToDevice, A->stream);
+        }
+        hipLaunchKernelGGL(k_adj_forward, dim3(nb), dim3(ABS), 0, A->stream, G, t0[st],
+                           (const double*)in, outp, aux, (const int*)A->d_slot,
                            probes + (size_t)st * A->n_probes);
     }
     A->last_steps = n_steps;
-    A->boxes = boxes;
+    A->t0 = t0;
     const hipError_t e = hipStreamSynchronize(A->stream);
-    if (e != hipSuccess || hipGetLastError() != hipSuccess) {
-        *err = std::string("adjoint forward: ") + hipGetErrorString(e);
+    const hipError_t e2 = hipGetLastError();
+    if (e != hipSuccess || e2 != hipSuccess) {
+        *err = std::string("adjoint forward: ") + hipGetErrorString(e != hipSuccess ? e : e2);
         return 4;
     }
     return 0;
@@ -427,37 +523,41 @@ int adjoint_backward(Adjoint* A, const double* dprobes, double* dkappa, double* 
         return 9;
     }
     const unsigned nb = blocks_of(n);
-    const KParams<double> KP = kparams<double>(A->P);
+    const AdjGeom G = geom_of(A);
     const double gxh = 0.5 * A->gx, gyh = 0.5 * A->gy, gzh = 0.5 * A->gz;
+    const size_t SN = (size_t)SL_N * n;
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(ABS), 0, A->stream, n, dkappa, 0.0);
     hipLaunchKernelGGL(k_fill, dim3((2 * AG + ABS - 1) / ABS), dim3(ABS), 0, A->stream,
                        (int64_t)2 * AG, A->gtheta, 0.0);
     // lV = dLoss/dV^N from the last probe row
     int cur = 0;
-    hipLaunchKernelGGL(k_adj_seed, dim3(nb), dim3(ABS), 0, A->stream, n, A->d_slot,
-                       dprobes + (size_t)(N - 1) * A->n_probes, A->lV[cur], A->ly[cur]);
-    const int n_seg = (N + A->K - 1) / A->K;
+    hipLaunchKernelGGL(k_adj_seed, dim3(nb), dim3(ABS), 0, A->stream, n,
+                       (const int*)A->d_slot, dprobes + (size_t)(N - 1) * A->n_probes,
+                       A->lV[cur], A->ly[cur]);
+    const int K = A->store_all ? N : A->K;
+    const int n_seg = (N + K - 1) / K;
     for (int sgi = n_seg - 1; sgi >= 0; --sgi) {
-        const int s0 = sgi * A->K, s1 = s0 + A->K < N ? s0 + A->K : N;
-        // recompute the segment: slot j holds the state at step s0 + j
-        hipMemcpyAsync(A->seg, A->ck + (size_t)sgi * 4 * n, 8 * 4 * n, hipMemcpyDeviceToDevice,
-                       A->stream);
-        for (int st = s0; st < s1 - 1; ++st) {
-            double* in = A->seg + (size_t)(st - s0) * 4 * n;
-            double* outp = in + 4 * n;
-            hipLaunchKernelGGL(k_adj_forward, dim3(nb), dim3(ABS), 0, A->stream,
-                               step_of(A, A->boxes[st]), KP, in, in + n, outp, outp + n,
-                               A->d_slot, (double*)nullptr);
+        const int s0 = sgi * K, s1 = s0 + K < N ? s0 + K : N;
+        const double* base = A->store_all ? A->seg + (size_t)s0 * SN : A->seg;
+        if (!A->store_all) {
+            // recompute the segment: slot j <- state at step s0 + j, with the factors
+            hipMemcpyAsync(A->seg, A->ck + (size_t)sgi * 4 * n, 8 * 4 * n,
+                           hipMemcpyDeviceToDevice, A->stream);
+            for (int st = s0; st < s1; ++st) {
+                double* in = A->seg + (size_t)(st - s0) * SN;
+                hipLaunchKernelGGL(k_adj_forward, dim3(nb), dim3(ABS), 0, A->stream, G,
+                                   A->t0[st], (const double*)in, in + SN, in + SL_Ev * n,
+                                   (const int*)A->d_slot, (double*)nullptr);
+            }
         }
-        // reverse sweep over the segment's steps
         for (int st = s1 - 1; st >= s0; --st) {
-            const double* Vn = A->seg + (size_t)(st - s0) * 4 * n;
+            const double* sl = base + (size_t)(st - s0) * SN;
             const int nxt = cur ^ 1;
             const double* seed = st >= 1 ? dprobes + (size_t)(st - 1) * A->n_probes : nullptr;
-            hipLaunchKernelGGL(k_adj_backward, dim3(AG), dim3(ABS), 0, A->stream,
-                               step_of(A, A->boxes[st]), KP, gxh, gyh, gzh, Vn, Vn + n,
-                               (const double*)A->lV[cur], (const double*)A->ly[cur], A->lV[nxt],
-                               A->ly[nxt], A->d_slot, seed, dkappa, A->gtheta);
+            hipLaunchKernelGGL(k_adj_backward, dim3(AG), dim3(ABS), 0, A->stream, G, gxh, gyh,
+                               gzh, sl, (const double*)A->lV[cur], (const double*)A->ly[cur],
+                               A->lV[nxt], A->ly[nxt], (const int*)A->d_slot, seed, dkappa,
+                               A->gtheta);
             cur = nxt;
         }
     }
@@ -465,17 +565,18 @@ int adjoint_backward(Adjoint* A, const double* dprobes, double* dkappa, double* 
     hipError_t e = hipMemcpyAsync(gt.data(), A->gtheta, 8 * 2 * AG, hipMemcpyDeviceToHost,
                                   A->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(A->stream);
-    if (e != hipSuccess || hipGetLastError() != hipSuccess) {
-        *err = std::string("adjoint backward: ") + hipGetErrorString(e);
+    const hipError_t e2 = hipGetLastError();
+    if (e != hipSuccess || e2 != hipSuccess) {
+        *err = std::string("adjoint backward: ") + hipGetErrorString(e != hipSuccess ? e : e2);
         return 4;
     }
-    double s0 = 0.0, s1 = 0.0;
+    double a0 = 0.0, a1 = 0.0;
     for (int b = 0; b < AG; ++b) {
-        s0 += gt[2 * b];
-        s1 += gt[2 * b + 1];
+        a0 += gt[2 * b];
+        a1 += gt[2 * b + 1];
     }
-    dtheta[0] = s0;
-    dtheta[1] = s1;
+    dtheta[0] = a0;
+    dtheta[1] = a1;
     return 0;
 }
 

@@ -1045,11 +1045,19 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
     if (n_probes < 1 || checkpoint_every < 1) return fail(CG_ERR_ARG, "bad probe count or interval");
     HIPCHK(hipStreamSynchronize(ion->stream));
     const double diag[3] = {m->sig[0], m->sig[4], m->sig[8]};
+    cg::ActiveBoxesAll bx{};
+    bx.n = t->n_boxes;
+    for (int b = 0; b < t->n_boxes; ++b) {
+        for (int d = 0; d < 3; ++d) { bx.lo[b][d] = t->box_lo[b][d]; bx.hi[b][d] = t->box_hi[b][d]; }
+        bx.t0[b] = t->box_t0[b];
+        bx.dur[b] = t->box_dur[b];
+        bx.I[b] = t->box_I[b];
+    }
     std::string err;
     cg::Adjoint* A = nullptr;
     const int rc = cg::adjoint_create(&A, m->g, diag, t->cfg.dt, t->cfg.chi * t->cfg.cm, ion->P,
-                                      t->d_V[t->cur], ion->d_states, checkpoint_every, n_probes,
-                                      probe_cells, ion->stream, &err);
+                                      t->d_V[t->cur], ion->d_states, bx, checkpoint_every,
+                                      n_probes, probe_cells, ion->stream, &err);
     if (rc) return fail(rc, "%s", err.c_str());
     cg_adjoint* a = new cg_adjoint{t, A, t->t, n_probes};
     *out = a;
@@ -1070,18 +1078,19 @@ int cg_adjoint_forward(cg_adjoint* a, const double* kappa_dev, const double* the
 {
     if (!a || !kappa_dev || !probes_dev) return fail(CG_ERR_ARG, "NULL argument");
     if (n_steps < 1) return fail(CG_ERR_ARG, "n_steps must be positive");
-    // box stimulus of each step, with the tissue's time accumulation
-    std::vector<cg::ActiveBoxes> boxes((size_t)n_steps);
+    // start time of each step, with the tissue's time accumulation
+    std::vector<double> t0((size_t)n_steps);
     double tt = a->t0;
     for (int s = 0; s < n_steps; ++s) {
         const double tn = tt + a->t->cfg.dt;
-        boxes[s] = active_boxes(a->t, tn - a->t->cfg.dt);
+        t0[s] = tn - a->t->cfg.dt;
         tt = tn;
     }
     const int idx[2] = {cg::BuenoOrovio::tfi, cg::BuenoOrovio::tsi};
     std::string err;
-    const int rc = cg::adjoint_forward(a->A, kappa_dev, theta, idx, theta ? 2 : 0, n_steps,
-                                       boxes, probes_dev, &err);
+    const size_t budget = (size_t)64 << 30;  // every step kept up to 64 GiB of HBM
+    const int rc = cg::adjoint_forward(a->A, kappa_dev, theta, idx, theta ? 2 : 0, n_steps, t0,
+                                       probes_dev, budget, &err);
     if (rc) return fail(rc, "%s", err.c_str());
     return CG_OK;
 }

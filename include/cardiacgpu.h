@@ -234,8 +234,11 @@ int cg_mesh_laplacian(cg_mesh* m, const double* V_dev, double* out_dev);
  * (myocardiumDomain.C:487-529, monodomainSolver.C:103-118, BuenoOrovio.C:107-152) with
  * sigma_c = kappa_c sigma. */
 typedef struct cg_adjoint cg_adjoint;
-/* Snapshots the tissue's current Vm, ionic states and time as the start of every forward
- * run; checkpoints every checkpoint_every steps. */
+/* Snapshots the tissue's current Vm, ionic states, stimulus boxes and time as the start of
+ * every forward run.  The forward run checkpoints the state every checkpoint_every steps
+ * and the backward run re-runs each segment; when checkpoint_every >= n_steps and every
+ * step's state (80 B per cell and step) fits in 64 GiB, all states are kept instead and
+ * nothing is re-run. */
 int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
                       int checkpoint_every, cg_adjoint** out);
 int cg_adjoint_destroy(cg_adjoint* a);

@@ -106,3 +106,23 @@ def test_adjoint_gradient_vs_oracle_finite_differences(gpu, orc):
         fd = (fp - fm) / (2 * e[i])
         _say("  theta %d: adjoint %.9e fd %.9e" % (i, gt[i], fd))
         assert abs(gt[i] - fd) <= 1e-5 * abs(fd)
+
+
+def test_adjoint_store_all_equals_checkpointing(gpu, orc):
+    """Keeping every step (interval >= steps) and re-running checkpointed segments give the
+    same gradient."""
+    import torch
+    _say("adjoint store-all vs checkpoints")
+    shape, h, steps = (16, 10, 6), 2e-4, 250
+    res = []
+    for K in (32, 1000):
+        mesh, dom = _dom(gpu, shape, h)
+        cells = np.random.default_rng(3).choice(mesh.n, 12, replace=False)
+        adj = gpu.TissueAdjoint(dom, cells, checkpoint_every=K)
+        kappa = torch.ones(mesh.n, dtype=torch.float64, device="cuda", requires_grad=True)
+        theta = torch.tensor([0.11, 1.8875], dtype=torch.float64, requires_grad=True)
+        tr = gpu.probe_traces(adj, kappa, theta, steps)
+        (tr ** 2).sum().backward()
+        res.append((kappa.grad.cpu().numpy(), theta.grad.numpy()))
+    np.testing.assert_allclose(res[0][0], res[1][0], rtol=1e-13, atol=1e-300)
+    np.testing.assert_allclose(res[0][1], res[1][1], rtol=1e-13)
