@@ -905,21 +905,29 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
     const double dt = t->cfg.dt, chiCm = t->cfg.chi * t->cfg.cm;
     const double dtm = ion->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
     const cg::HexCoefs coefs = mesh_coefs(m);
-    for (int s = 0; s < n_steps; ++s) {
-        const double tn = t->t + dt;
+    // constants of the step that starts at accumulated time t_before after steps_before
+    // steps: t += dt; t0 = t - dt; prepareTimeStep(t0) (myocardiumDomain.C:487-495)
+    auto make_step = [&](double t_before, int64_t steps_before) {
+        const double tn = t_before + dt;
         const double t0 = tn - dt;
-        const bool bdf2 = t->cfg.ddt == CG_DDT_BACKWARD && t->steps >= 1;
+        const bool bdf2 = t->cfg.ddt == CG_DDT_BACKWARD && steps_before >= 1;
         cg::ExplicitStep st{};
         st.dt = dt; st.dt_model = dtm; st.chiCm = chiCm;
         st.c = bdf2 ? 1.5 : 1.0; st.c0 = bdf2 ? 2.0 : 1.0; st.c00 = bdf2 ? 0.5 : 0.0;
         st.t_old = t0;
         st.boxes = active_boxes(t, t0);
+        st.od = m->d_od;
+        st.od_mask = m->od_mask;
+        return st;
+    };
+    bool ionic_queued = false;  // this step's ionic launch was queued by the previous one
+    for (int s = 0; s < n_steps; ++s) {
+        const double tn = t->t + dt;
+        const cg::ExplicitStep st = make_step(t->t, t->steps);
         double* Vn = t->d_V[t->cur];
         double* Vm1 = t->d_V[(t->cur + 2) % 3];
         double* Vnew = t->d_V[(t->cur + 1) % 3];
         cg::Profiler* prof = t->prof.enabled ? &t->prof : nullptr;
-        st.od = m->d_od;
-        st.od_mask = m->od_mask;
         if (t->cfg.algorithm == CG_EXPLICIT) {
             const int tk = prof ? prof->begin(cg::K_IONIC) : -1;
             HIPCHK(cg::launch_explicit_hex(ion->model, ion->prec, ion->tissue, m->g, coefs,
@@ -927,11 +935,16 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
                                            t->d_tact, st, ion->P, ion->stream));
             if (prof) prof->end(tk);
         } else {
+            cg::NextIonic nx;
+            const bool more = s + 1 < n_steps;
+            if (more) nx.st = make_step(tn, t->steps + 1);
             int rc = cg::implicit_step(t->implicit, ion->model, ion->prec, ion->tissue,
                                        ion->d_states, ion->P, Vn, Vm1, Vnew, t->d_Iion,
                                        t->d_flag, t->d_tact, st, t->cfg.tol, t->cfg.max_iter,
-                                       prof, &g_err);
+                                       prof, ionic_queued, more ? &nx : nullptr,
+                                       t->steps >= 1, &g_err);
             if (rc != 0) return rc;
+            ionic_queued = more;
         }
         t->cur = (t->cur + 1) % 3;
         t->t = tn;

@@ -12,11 +12,20 @@ struct ImplicitSolver;
 int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* h_sigma9,
                     double chiCm, double dt, int prec, hipStream_t s, std::string* err);
 void implicit_destroy(ImplicitSolver* S);
-// one step: ionic (V^n) + right-hand side, solve for Vnew, activation update
+// ionic launch of the following step, queued behind the current solve (gated on its
+// convergence): the step constants of that step
+struct NextIonic {
+    ExplicitStep st;
+};
+// one step: ionic (V^n) + right-hand side (unless ionic_queued: the previous call already
+// queued it), solve for Vnew, activation update; with `next`, the next step's ionic kernel
+// (V^{n+1}, V^n) is queued behind the read-back of this step's solver state.  extrapolate
+// (mixed path): initial guess 2 V^n - V^{n-1} instead of V^n.
 int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* states,
                   const ModelParams& P, const double* Vn, const double* Vm1, double* Vnew,
                   double* Iion, uint8_t* flag, double* tact, const ExplicitStep& st, double tol,
-                  int max_iter, Profiler* prof, std::string* err);
+                  int max_iter, Profiler* prof, bool ionic_queued, const NextIonic* next,
+                  bool extrapolate, std::string* err);
 void implicit_stats(const ImplicitSolver* S, int64_t* iters, double* res);
 // y = A x, A = coefft chiCm/dt I + orthogonal Laplacian matrix; prec 0: double, 1: float
 // (device pointers).  Returns 0, 9 when the fp32 operator was not built, 4 on HIP error.

@@ -41,6 +41,10 @@ struct ExplicitStep {
     ActiveBoxes boxes;
     const double* od;  // off-diagonal conductivity (xy, xz, yz) [3][n], or null
     unsigned od_mask;  // which of xy, xz, yz are non-zero somewhere (bits 0..2)
+    // launch gate: when gate != null the kernel runs only if *gate == gate_val (a solver
+    // generation written by an earlier kernel of the stream)
+    const long long* gate;
+    long long gate_val;
 };
 
 // Model kernels, one set per arithmetic type: _fp64 (IEEE double, model_fp64.hip) and
@@ -169,8 +173,7 @@ const char* kernel_name(int id);
 struct Profiler {
     bool enabled = false;
     hipStream_t stream = nullptr;
-    std::vector<hipEvent_t> pool;
-    size_t used = 0;
+    std::vector<hipEvent_t> all, free_;
     struct Rec { int id; hipEvent_t a, b; };
     std::vector<Rec> recs;
     double total_ms[K_COUNT] = {0};
@@ -178,47 +181,56 @@ struct Profiler {
 
     hipEvent_t next()
     {
-        if (used == pool.size()) {
+        if (free_.empty()) {
             hipEvent_t e;
-            hipEventCreate(&e);
-            pool.push_back(e);
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            all.push_back(e);
+            return e;
         }
-        return pool[used++];
+        hipEvent_t e = free_.back();
+        free_.pop_back();
+        return e;
     }
     int begin(int id)
     {
         if (!enabled) return -1;
-        if (used + 2 > 8192) resolve(true);
+        if (recs.size() >= 4096) resolve(true);
         Rec r{id, next(), next()};
-        hipEventRecord(r.a, stream);
+        if (!r.a || !r.b || hipEventRecord(r.a, stream) != hipSuccess) r.id = -1;
         recs.push_back(r);
         return (int)recs.size() - 1;
     }
     void end(int tok)
     {
-        if (tok >= 0) hipEventRecord(recs[tok].b, stream);
+        if (tok >= 0 && hipEventRecord(recs[tok].b, stream) != hipSuccess) recs[tok].id = -1;
     }
     size_t mark() const { return recs.size(); }
-    // drop records [i, j) (launches that exited early on a converged flag)
+    // drop records [i, j) (launches that exited early on a finished solve or a closed gate)
     void invalidate_range(size_t i, size_t j)
     {
         for (size_t k = i; k < j && k < recs.size(); ++k) recs[k].id = -1;
     }
-    // accumulate recorded intervals; sync first when asked
+    // accumulate the records whose launches have completed (all of them after a sync when
+    // asked), keep the others; record indices are valid until the next resolve
     void resolve(bool sync)
     {
         if (recs.empty()) return;
-        if (sync) hipStreamSynchronize(stream);
+        if (sync && hipStreamSynchronize(stream) != hipSuccess) return;
+        std::vector<Rec> keep;
         for (auto& r : recs) {
-            if (r.id < 0) continue;
+            if (hipEventQuery(r.b) == hipErrorNotReady) {
+                keep.push_back(r);
+                continue;
+            }
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            if (r.id >= 0 && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
                 total_ms[r.id] += ms;
                 launches[r.id] += 1;
             }
+            if (r.a) free_.push_back(r.a);
+            if (r.b) free_.push_back(r.b);
         }
-        recs.clear();
-        used = 0;
+        recs.swap(keep);
     }
     void reset()
     {
@@ -227,7 +239,7 @@ struct Profiler {
     }
     ~Profiler()
     {
-        for (auto e : pool) hipEventDestroy(e);
+        for (auto e : all) hipEventDestroy(e);
     }
 };
 

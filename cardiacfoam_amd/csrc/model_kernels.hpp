@@ -152,7 +152,9 @@ __global__ __launch_bounds__(MBS) void k_explicit_hex(
 
 // ------------------------------------------ ionic step + implicit right-hand side
 // rhs = chiCm/dt (c0 V^n - c00 V^{n-1}) + corr - chiCm Iion + source, where corr is the
-// explicit tensor correction of the Laplacian (null when sigma is diagonal).
+// explicit tensor correction of the Laplacian (null when sigma is diagonal).  With a gate
+// (s.gate) the launch may be queued before the previous step's solve is known to have
+// converged: it then does nothing unless the solver recorded the expected generation.
 template <class M, class T>
 __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st,
                                                    const double* __restrict__ Vn,
@@ -162,6 +164,7 @@ __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st
                                                    double* __restrict__ b, ExplicitStep s,
                                                    KParams<T> P, int tissue)
 {
+    if (s.gate && *s.gate != s.gate_val) return;
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * MBS + threadIdx.x;
     if (c >= n) return;

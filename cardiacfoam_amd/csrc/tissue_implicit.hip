@@ -54,6 +54,7 @@ struct DevState {
     double alpha[2], gamma[2];  // by iteration parity, written by block 0 of the update
     long long done_gen;       // generation of the last finished CG solve
     long long outer_gen;      // generation whose start already met the outer tolerance
+    long long conv_gen;       // generation of the last solve that met the step tolerance
     int iters;                // iterations of the current solve
     int pad;
     long long iters_sum;      // iterations of all solves, added by each residual pass
@@ -119,6 +120,11 @@ CG_DEV T apply_row(const HexOp<T>& A, const LD& x, int64_t c, const IJK& q, cons
 template <class T> struct PlainLoad {
     const T* __restrict__ x;
     CG_DEV T operator()(int64_t c) const { return x[c]; }
+};
+struct ExtrapLoad {  // linear extrapolation 2 x - x_prev (initial guess of a step)
+    const double* __restrict__ x;
+    const double* __restrict__ xp;
+    CG_DEV double operator()(int64_t c) const { return 2.0 * x[c] - xp[c]; }
 };
 struct CorrectedLoad {  // fp64 x plus the fp32 correction e
     const double* __restrict__ x;
@@ -198,7 +204,8 @@ __global__ __launch_bounds__(BS) void k_invdiag(HexGeom g, HexOp<T> A, T* __rest
 }
 
 // ---- fp64 residual that starts CG solve `gen` in precision TP.
-//   x = xin (+ e when e != null and solve gen-1 did not stop at its outer test);
+//   x = xin (+ e when e != null and solve gen-1 did not stop at its outer test), or
+//   x = 2 xin - xprev when xprev != null (extrapolated initial guess of a step);
 //   xout = x when xout != null (xout never aliases xin or e);
 //   b += tensor correction of xin when od != null (first pass of a step);
 //   r = b - A x;  rt = TP(r), ut = M^-1 rt.
@@ -236,6 +243,7 @@ template <class TP>
 __global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
                                                   const TP* __restrict__ invd_t,
                                                   const double* __restrict__ xin,
+                                                  const double* __restrict__ xprev,
                                                   const float* __restrict__ e,
                                                   double* __restrict__ xout,
                                                   double* __restrict__ b,
@@ -247,9 +255,16 @@ __global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
     const bool add_e = e != nullptr && S->outer_gen != gen - 1;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const PlainLoad<double> X0{xin};
-    if (od)
+    const ExtrapLoad XE{xin, xprev};
+    if (od && xprev)
+        residual_range<TP, ExtrapLoad, true>(g, A, invd_t, XE, xin, xout, b, od, od_mask, rt,
+                                             ut, v);
+    else if (od)
         residual_range<TP, PlainLoad<double>, true>(g, A, invd_t, X0, xin, xout, b, od, od_mask,
                                                     rt, ut, v);
+    else if (xprev)
+        residual_range<TP, ExtrapLoad, false>(g, A, invd_t, XE, xin, xout, b, od, od_mask, rt,
+                                              ut, v);
     else if (add_e)
         residual_range<TP, CorrectedLoad, false>(g, A, invd_t, CorrectedLoad{xin, e}, xin, xout,
                                                  b, od, od_mask, rt, ut, v);
@@ -272,21 +287,22 @@ __global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
 
 // ---- S-kernel of iteration k of solve gen: convergence test on r_k, then w = A u_k and
 // the partial (w,u).  Converged when (r,r) <= tol2 * ref, ref = (r0,r0) (relative_to_r0)
-// or (b,b); at k == 0 with outer_tol2 > 0 also when the fp64 residual already meets
+// or (b,b), or when (r,r) <= abs2 (b,b) (an inner solve that already meets the step
+// tolerance); at k == 0 with outer_tol2 > 0 also when the fp64 residual already meets
 // (r0,r0) <= outer_tol2 (b,b) (the refinement is finished).
 template <class T, bool QUAD>
 __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __restrict__ u,
                                               T* __restrict__ w, double* __restrict__ part,
                                               DevState* S, long long gen, int k, double tol2,
-                                              int relative_to_r0, double outer_tol2,
-                                              int max_iter)
+                                              int relative_to_r0, double abs2,
+                                              double outer_tol2, int max_iter)
 {
     if (S->done_gen == gen) return;
     double v3[3];
     sum_slots<3>(part, {P_RR + (k & 1), P_RR0, P_BB}, v3);
     const double rr = v3[0], rr0 = v3[1], bb = v3[2];
     const bool outer = k == 0 && outer_tol2 > 0.0 && rr0 <= outer_tol2 * bb;
-    const bool conv = rr <= tol2 * (relative_to_r0 ? rr0 : bb);
+    const bool conv = rr <= tol2 * (relative_to_r0 ? rr0 : bb) || rr <= abs2 * bb;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (k == 0) {
             S->s[S_R0] = rr0;
@@ -296,6 +312,7 @@ __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __
             S->s[S_RR] = rr;
             S->iters = k;
             if (outer) S->outer_gen = gen;
+            if (outer || (conv && !relative_to_r0)) S->conv_gen = gen;
             S->done_gen = gen;
         }
     }
@@ -402,14 +419,17 @@ __global__ __launch_bounds__(RBS) void k_cg_u(HexGeom g, const T* __restrict__ i
     }
 }
 
-// ---- activation time (myocardiumDomain.C:345-374) and V^{n+1} = x when x is elsewhere
+// ---- activation time (myocardiumDomain.C:345-374) and V^{n+1} = x when x is elsewhere;
+// with a gate it runs only when the solve of that generation met the step tolerance
 __global__ __launch_bounds__(BS) void k_activation(HexGeom g, const double* __restrict__ Vn,
-                                                   const double* __restrict__ x,
-                                                   double* __restrict__ Vnew,
+                                                   const double* x,  // may be Vnew
+                                                   double* Vnew,
                                                    uint8_t* __restrict__ flag,
                                                    double* __restrict__ tact, double t_old,
-                                                   double dt)
+                                                   double dt, const long long* gate,
+                                                   long long gate_val)
 {
+    if (gate && *gate != gate_val) return;
     CELL_INDEX
     (void)cc;
     const double vnew = x[c];
@@ -500,6 +520,7 @@ struct ImplicitSolver {
     double* partials = nullptr;
     DevState* S = nullptr;
     DevState* hS = nullptr;  // pinned mirror
+    hipEvent_t read_done = nullptr;  // after the read-back of S
     double mass_c = -1.0;    // coefft of the current invd
     bool quad = false;       // nx % 4 == 0: CG kernels on quads of cells (vector loads)
     long long gen = 0;       // generation of the last CG solve started
@@ -561,6 +582,7 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
         A((void**)&S->v64.s, 8 * n); A((void**)&S->v64.w, 8 * n);
     }
     ok &= hipHostMalloc((void**)&S->hS, sizeof(DevState), hipHostMallocDefault) == hipSuccess;
+    ok &= hipEventCreateWithFlags(&S->read_done, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         implicit_destroy(S);
         *err = "implicit solver allocation failed";
@@ -582,6 +604,7 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
     DevState z{};
     z.done_gen = -1;
     z.outer_gen = -1;
+    z.conv_gen = -1;
     hipMemcpy(S->S, &z, sizeof z, hipMemcpyHostToDevice);
     *out = S;
     return 0;
@@ -597,6 +620,7 @@ void implicit_destroy(ImplicitSolver* S)
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (S->hS) hipHostFree(S->hS);
+    if (S->read_done) hipEventDestroy(S->read_done);
     delete S;
 }
 
@@ -650,17 +674,18 @@ template <class T> struct CgRun {
     Profiler* prof;
     std::vector<CgLaunch>* log;
 
-    void s_kernel(long long gen, int k, double tol2, int rel, double outer_tol2, int max_iter)
+    void s_kernel(long long gen, int k, double tol2, int rel, double abs2, double outer_tol2,
+                  int max_iter)
     {
         const int tk = prof ? prof->begin(K_SPMV) : -1;
         if (S->quad)
             hipLaunchKernelGGL((k_cg_s<T, true>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,
                                (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel,
-                               outer_tol2, max_iter);
+                               abs2, outer_tol2, max_iter);
         else
             hipLaunchKernelGGL((k_cg_s<T, false>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,
                                (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel,
-                               outer_tol2, max_iter);
+                               abs2, outer_tol2, max_iter);
         if (prof) {
             prof->end(tk);
             log->push_back({(size_t)tk, gen, k, true});
@@ -682,13 +707,13 @@ template <class T> struct CgRun {
             log->push_back({(size_t)tk, gen, k, false});
         }
     }
-    void residual(int kid, long long gen, const double* xin, const float* e, double* xout,
-                  const double* od, const HexOp<double>& A64)
+    void residual(int kid, long long gen, const double* xin, const double* xprev,
+                  const float* e, double* xout, const double* od, const HexOp<double>& A64)
     {
         const int tk = prof ? prof->begin(kid) : -1;
         hipLaunchKernelGGL((k_residual<T>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A64,
-                           (const T*)A.invd, xin, e, xout, S->b, od, S->od_mask, v.r, v.u,
-                           S->partials, S->S, gen);
+                           (const T*)A.invd, xin, xprev, e, xout, S->b, od, S->od_mask, v.r,
+                           v.u, S->partials, S->S, gen);
         if (prof) prof->end(tk);
     }
 };
@@ -707,14 +732,19 @@ void drop_idle(Profiler* prof, std::vector<CgLaunch>& log, long long gen, int it
 int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* states,
                   const ModelParams& P, const double* Vn, const double* Vm1, double* Vnew,
                   double* Iion, uint8_t* flag, double* tact, const ExplicitStep& st, double tol,
-                  int max_iter, Profiler* prof, std::string* err)
+                  int max_iter, Profiler* prof, bool ionic_queued, const NextIonic* next,
+                  bool extrapolate, std::string* err)
 {
     const unsigned grid = grid_of(S->n);
-    int tk = prof ? prof->begin(K_IONIC) : -1;
-    hipError_t e = launch_ionic_rhs(model, prec, tissue, S->g, states, Vn, Vm1, nullptr, Iion,
-                                    S->b, st, P, S->stream);
-    if (prof) prof->end(tk);
-    if (e != hipSuccess) { *err = std::string("ionic/rhs launch: ") + hipGetErrorString(e); return 4; }
+    int tk;
+    hipError_t e;
+    if (!ionic_queued) {
+        tk = prof ? prof->begin(K_IONIC) : -1;
+        e = launch_ionic_rhs(model, prec, tissue, S->g, states, Vn, Vm1, nullptr, Iion, S->b, st,
+                             P, S->stream);
+        if (prof) prof->end(tk);
+        if (e != hipSuccess) { *err = std::string("ionic/rhs launch: ") + hipGetErrorString(e); return 4; }
+    }
     // 1/diag for this step's ddt coefficient
     if (st.c != S->mass_c) {
         HexOp<double> A = op_of<double>(S, st.c);
@@ -728,66 +758,97 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
     const HexOp<double> A = op_of<double>(S, st.c);
     const double inner_tol = 1e-4;
     std::vector<CgLaunch> log;
-    double* x_final = Vnew;
     int used_total = 0;
+    const long long* conv_gen = &S->S->conv_gen;
 
-    auto sync_state = [&]() -> bool {
+    // Before each read-back: the step's activation (x into V^{n+1}) and the next step's
+    // ionic kernel, both gated on "solve g met the tolerance", then the 64-byte state.  The
+    // host waits for the read-back only; the queued ionic kernel of the next step keeps the
+    // device busy meanwhile.  Gated launches of a pass that did not converge do nothing and
+    // are dropped from the timings.
+    auto finish_and_read = [&](long long g, const double* x, size_t* mark) -> bool {
+        *mark = prof ? prof->mark() : 0;
+        int t = prof ? prof->begin(K_ACTIVATION) : -1;
+        hipLaunchKernelGGL(k_activation, dim3(grid), dim3(BS), 0, S->stream, S->g, Vn, x, Vnew,
+                           flag, tact, st.t_old, st.dt, conv_gen, g);
+        if (prof) prof->end(t);
         hipMemcpyAsync(S->hS, S->S, sizeof(DevState), hipMemcpyDeviceToHost, S->stream);
-        return hipStreamSynchronize(S->stream) == hipSuccess;
+        hipEventRecord(S->read_done, S->stream);
+        if (next) {
+            ExplicitStep sn = next->st;
+            sn.gate = conv_gen;
+            sn.gate_val = g;
+            t = prof ? prof->begin(K_IONIC) : -1;
+            launch_ionic_rhs(model, prec, tissue, S->g, states, Vnew, Vn, nullptr, Iion, S->b, sn,
+                             P, S->stream);
+            if (prof) prof->end(t);
+        }
+        if (hipGetLastError() != hipSuccess) return false;
+        return hipEventSynchronize(S->read_done) == hipSuccess;
     };
 
     if (prec == 0) {
         // one fp64 CG on the system, x0 = V^n copied into V^{n+1} by the first residual
         CgRun<double> cg{S, A, S->v64, prof, &log};
         const long long gen = ++S->gen;
-        cg.residual(K_RESIDUAL0, gen, Vn, nullptr, Vnew, S->od, A);
-        cg.s_kernel(gen, 0, tol * tol, 0, 0.0, max_iter);
+        cg.residual(K_RESIDUAL0, gen, Vn, nullptr, nullptr, Vnew, S->od, A);
+        cg.s_kernel(gen, 0, tol * tol, 0, 0.0, 0.0, max_iter);
         int k = 0;
         int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
         for (;;) {
             for (int q = 0; q < chunk; ++q, ++k) {
                 cg.u_kernel(gen, k, Vnew, 0);
-                cg.s_kernel(gen, k + 1, tol * tol, 0, 0.0, max_iter);
+                cg.s_kernel(gen, k + 1, tol * tol, 0, 0.0, 0.0, max_iter);
             }
-            if (hipGetLastError() != hipSuccess) { *err = "CG launch failed"; return 4; }
-            if (!sync_state()) { *err = "synchronisation failed"; return 4; }
+            size_t gated;
+            if (!finish_and_read(gen, Vnew, &gated)) { *err = "CG launch or read-back failed"; return 4; }
             const DevState& h = *S->hS;
             if (h.done_gen == gen) {
                 drop_idle(prof, log, gen, h.iters);
-                if (prof) prof->resolve(false);
+                const bool ok = h.conv_gen == gen;
+                if (prof) {
+                    if (!ok) prof->invalidate_range(gated, prof->mark());
+                    prof->resolve(false);
+                }
                 used_total = h.iters;
                 S->last_inner = h.iters;
                 const double bb = h.s[S_BB] > 0 ? h.s[S_BB] : 1.0;
                 S->last_res = std::sqrt(h.s[S_RR] / bb);
-                if (h.s[S_RR] > tol * tol * h.s[S_BB]) {
+                if (!ok) {
                     *err = "PCG did not converge within max_iter";
                     return 7;
                 }
                 break;
             }
+            if (prof) prof->invalidate_range(gated, prof->mark());
             chunk = 4;  // not finished: continue the same solve
         }
     } else {
         // mixed: refinement passes; x lives in V^n (pass 0), then alternates Vnew / xalt
         CgRun<float> cg{S, op_of<float>(S, st.c), S->v32, prof, &log};
         long long gen = ++S->gen;
-        cg.residual(K_RESIDUAL0, gen, Vn, nullptr, nullptr, S->od, A);
-        cg.s_kernel(gen, 0, inner_tol * inner_tol, 1, tol * tol, max_iter);
-        const double* cur = Vn;
+        // x0 = 2 V^n - V^{n-1} (V^n on the first step); an inner solve stops at 1e-4
+        // relative or as soon as its residual meets half the step tolerance
+        const double* xprev = extrapolate ? Vm1 : nullptr;
+        const double abs2 = 0.25 * tol * tol;
+        // x0 goes to xalt, so the first refinement lands in V^{n+1} itself
+        cg.residual(K_RESIDUAL0, gen, Vn, xprev, nullptr, xprev ? S->xalt : nullptr, S->od, A);
+        cg.s_kernel(gen, 0, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter);
+        const double* cur = xprev ? S->xalt : Vn;
         for (int pass = 0;; ++pass) {
             const int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
             for (int k = 0; k < chunk; ++k) {
                 cg.u_kernel(gen, k, S->e32, 1);
-                cg.s_kernel(gen, k + 1, inner_tol * inner_tol, 1, tol * tol, max_iter);
+                cg.s_kernel(gen, k + 1, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter);
             }
             // x' = x + e into the other buffer, fp64 residual, start of the next solve
             double* nxt = cur == Vnew ? S->xalt : Vnew;
             const long long ngen = ++S->gen;
-            cg.residual(K_RESIDUAL, ngen, cur, S->e32, nxt, nullptr, A);
-            cg.s_kernel(ngen, 0, inner_tol * inner_tol, 1, tol * tol, max_iter);
+            cg.residual(K_RESIDUAL, ngen, cur, nullptr, S->e32, nxt, nullptr, A);
+            cg.s_kernel(ngen, 0, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter);
             cur = nxt;
-            if (hipGetLastError() != hipSuccess) { *err = "CG launch failed"; return 4; }
-            if (!sync_state()) { *err = "synchronisation failed"; return 4; }
+            size_t gated;
+            if (!finish_and_read(ngen, cur, &gated)) { *err = "CG launch or read-back failed"; return 4; }
             const DevState& h = *S->hS;
             // iterations of solve gen, folded into the running sum by the residual pass
             const int run = (int)(h.iters_sum - S->iters_sum_seen);
@@ -795,6 +856,7 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
             drop_idle(prof, log, gen, run);
             const bool finished = h.outer_gen == ngen;
             if (finished) drop_idle(prof, log, ngen, 0);
+            else if (prof) prof->invalidate_range(gated, prof->mark());
             if (prof) prof->resolve(false);
             log.clear();
             used_total += run;
@@ -808,13 +870,8 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
             }
             gen = ngen;
         }
-        x_final = const_cast<double*>(cur);
     }
     S->total_iters += used_total;
-    tk = prof ? prof->begin(K_ACTIVATION) : -1;
-    hipLaunchKernelGGL(k_activation, dim3(grid), dim3(BS), 0, S->stream, S->g, Vn,
-                       (const double*)x_final, Vnew, flag, tact, st.t_old, st.dt);
-    if (prof) prof->end(tk);
     e = hipGetLastError();
     if (e != hipSuccess) { *err = hipGetErrorString(e); return 4; }
     return 0;
