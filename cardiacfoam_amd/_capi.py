@@ -11,9 +11,10 @@ LIB_PATH = os.path.join(HERE, "libcardiacgpu.so")
 
 CG_OK = 0
 ERRORS = {1: "ARG", 2: "MODEL", 3: "TISSUE", 4: "HIP", 5: "NOMEM", 6: "NONFINITE",
-          7: "NOT_CONVERGED", 8: "STIMULUS", 9: "STATE"}
+          7: "NOT_CONVERGED", 8: "STIMULUS", 9: "STATE", 10: "ODE"}
 FP64, MIXED = 0, 1
 DEVICE_PTRS = 1
+ODE_RUSH_LARSEN, ODE_RKF45 = 0, 1
 EXPLICIT, IMPLICIT = 0, 1
 DDT_EULER, DDT_BACKWARD = 0, 1
 FIELD_VM, FIELD_IION, FIELD_ACTIVATION, FIELD_VM_OLD = 0, 1, 2, 3
@@ -61,6 +62,8 @@ def lib():
     L.cg_ionic_set_constant.argtypes = [vp, C.c_int, C.c_double]
     L.cg_ionic_set_conductance_scale.argtypes = [vp, vp]
     L.cg_ionic_set_stimulus.argtypes = [vp, C.POINTER(Stimulus)]
+    L.cg_ionic_set_integrator.argtypes = [vp, C.c_int, C.c_double, C.c_double, ip64,
+                                          C.c_double]
     L.cg_ionic_step.argtypes = [vp, C.c_double, C.c_double, vp, vp, C.c_int]
     L.cg_ionic_run_cells.argtypes = [vp, C.c_double, C.c_double, C.c_int]
     L.cg_ionic_get_states.argtypes = [vp, C.c_int, C.c_int, vp]

@@ -209,9 +209,7 @@ void k_cells_protocol(T* __restrict__ st, KParams<T> P, const T* __restrict__ gs
     if (a.s0 >= len) return;
     const int64_t s1 = a.s0 + a.chunk < len ? a.s0 + a.chunk : len;
     const int64_t n = a.n;
-    T p[M::NP];
-#pragma unroll
-    for (int k = 0; k < M::NP; ++k) p[k] = P.p[k];
+    const T* p = P.p;  // model constants read from the kernel arguments
     T gs[4] = {T(1), T(1), T(1), T(1)};
     if (gsc) {
 #pragma unroll

@@ -125,6 +125,14 @@ struct cg_ionic {
     double* d_beats = nullptr;
     int* d_nbeats = nullptr;
     double beats_dt = 0.0;            // dt of the run that filled the beat records
+    // ODE integrator (ionicModel::odeSolver(), ionicModel.H:233-248)
+    int ode_kind = CG_ODE_RUSH_LARSEN;
+    double abs_tol = 1e-15, rel_tol = 1e-4;
+    long long max_steps = 10000;
+    double init_step = 0.0;           // model time units; <= 0: the first deltaT in seconds
+    bool step_set = false;            // d_step holds the kept per-cell trial steps
+    double* d_step = nullptr;         // [n]
+    int* d_fail = nullptr;            // set by a cell whose integration failed
 };
 
 struct cg_mesh {
@@ -158,6 +166,57 @@ struct cg_tissue {
     cg::ImplicitSolver* implicit = nullptr;
     cg::Profiler prof;
 };
+
+namespace {
+
+// Integrator arguments of one ionic step from t0_s over dt_s seconds: tStart, tEnd and the
+// clamp deltaT in model time as each solveODE scales them (BuenoOrovio.C:107-108,127,
+// AlievPanfilov.C:107-108,127).  The first RKF45 step fills the per-cell trial steps with
+// the handle's initial step, which by default is deltaT in seconds read as model time, as
+// ionicModel's step_(num, initialDeltaT) is (ionicModel.C:38).
+int ode_args(cg_ionic* h, double t0_s, double dt_s, cg::OdeArgs* o)
+{
+    *o = cg::OdeArgs{};
+    o->rkf = h->ode_kind == CG_ODE_RKF45;
+    if (!o->rkf) return CG_OK;
+    const bool ap = h->model == cg::MODEL_AP;
+    o->t0 = ap ? t0_s * 1000.0 / 12.9 : t0_s * 1000.0;
+    o->t1 = ap ? (t0_s + dt_s) * 1000.0 / 12.9 : (t0_s + dt_s) * 1000.0;
+    o->dt_clamp = ap ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+    o->abs_tol = h->abs_tol;
+    o->rel_tol = h->rel_tol;
+    o->max_steps = h->max_steps;
+    o->t_s = t0_s;
+    o->dt_s = dt_s;
+    o->ap = ap;
+    if (!h->step_set) {
+        const double s0 = h->init_step > 0.0 ? h->init_step : dt_s;
+        std::vector<double> v((size_t)h->n, s0);
+        HIPCHK(hipMemcpyAsync(h->d_step, v.data(), sizeof(double) * h->n,
+                              hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        h->step_set = true;
+    }
+    o->step = h->d_step;
+    o->fail = h->d_fail;
+    return CG_OK;
+}
+
+// ODESolver's fatal errors (step-size underflow, maxSteps) as an error code, read at the
+// handle's synchronisation points
+int check_ode(cg_ionic* h)
+{
+    if (h->ode_kind != CG_ODE_RKF45 || !h->d_fail) return CG_OK;
+    int f = 0;
+    HIPCHK(hipMemcpyAsync(&f, h->d_fail, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!f) return CG_OK;
+    HIPCHK(hipMemsetAsync(h->d_fail, 0, sizeof(int), h->stream));
+    return fail(CG_ERR_ODE, "RKF45 integration failed in at least one cell (step-size "
+                            "underflow or more than maxSteps = %lld steps)", h->max_steps);
+}
+
+} // namespace
 
 extern "C" {
 
@@ -254,7 +313,7 @@ int cg_ionic_destroy(cg_ionic* h)
     if (h->d_vm) hipFree(h->d_vm);
     if (h->d_im) hipFree(h->d_im);
     void* batch[] = {h->d_prot, h->d_len, h->d_order, h->d_det, h->d_ring, h->d_beats,
-                     h->d_nbeats};
+                     h->d_nbeats, h->d_step, h->d_fail};
     for (void* q : batch)
         if (q) hipFree(q);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -309,12 +368,38 @@ int cg_ionic_set_stimulus(cg_ionic* h, const cg_stimulus* s)
     return CG_OK;
 }
 
+int cg_ionic_set_integrator(cg_ionic* h, int kind, double abs_tol, double rel_tol,
+                            int64_t max_steps, double initial_step)
+{
+    if (!h) return fail(CG_ERR_ARG, "NULL handle");
+    if (kind != CG_ODE_RUSH_LARSEN && kind != CG_ODE_RKF45)
+        return fail(CG_ERR_ARG, "unknown ODE solver %d", kind);
+    if (kind == CG_ODE_RKF45 &&
+        (!(abs_tol >= 0.0) || !(rel_tol >= 0.0) || !(abs_tol + rel_tol > 0.0) || max_steps < 1))
+        return fail(CG_ERR_ARG, "RKF45 needs absTol, relTol >= 0 (not both 0) and maxSteps >= 1");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (kind == CG_ODE_RKF45 && !h->d_step) {
+        if (hipMalloc(&h->d_step, sizeof(double) * h->n) != hipSuccess ||
+            hipMalloc(&h->d_fail, sizeof(int)) != hipSuccess)
+            return fail(CG_ERR_NOMEM, "allocation of the per-cell ODE steps failed");
+        HIPCHK(hipMemset(h->d_fail, 0, sizeof(int)));
+    }
+    h->ode_kind = kind;
+    h->abs_tol = abs_tol;
+    h->rel_tol = rel_tol;
+    h->max_steps = max_steps;
+    h->init_step = initial_step;
+    h->step_set = false;
+    return CG_OK;
+}
+
 int cg_ionic_step(cg_ionic* h, double t0, double dt, const double* Vm, double* Im, int flags)
 {
     if (!h || !Vm || !Im) return fail(CG_ERR_ARG, "NULL argument");
     if (h->solve_vm) return fail(CG_ERR_STATE, "cg_ionic_step needs a tissue handle");
     if (!(dt > 0.0)) return fail(CG_ERR_ARG, "deltaT must be positive");
-    (void)t0;  // no time-dependent term in tissue mode (stimulus lives in the PDE)
+    cg::OdeArgs ode;
+    if (int rc = ode_args(h, t0, dt, &ode)) return rc;
     const double* dv = Vm;
     double* di = Im;
     if (!(flags & CG_DEVICE_PTRS)) {
@@ -329,11 +414,11 @@ int cg_ionic_step(cg_ionic* h, double t0, double dt, const double* Vm, double* I
     }
     const double dtm = h->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
     HIPCHK(cg::launch_ionic_tissue(h->model, h->prec, h->tissue, h->d_states, h->n, dv, di, dtm,
-                                   h->P, h->d_gscale, h->stream));
+                                   h->P, h->d_gscale, ode, h->stream));
     if (!(flags & CG_DEVICE_PTRS))
         HIPCHK(hipMemcpyAsync(Im, di, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    return CG_OK;
+    return check_ode(h);
 }
 
 int cg_ionic_run_cells(cg_ionic* h, double t0, double dt, int n_steps)
@@ -342,9 +427,11 @@ int cg_ionic_run_cells(cg_ionic* h, double t0, double dt, int n_steps)
     if (!h->solve_vm) return fail(CG_ERR_STATE, "cg_ionic_run_cells needs a single-cell handle");
     if (n_steps < 0 || !(dt > 0.0)) return fail(CG_ERR_ARG, "bad step arguments");
     const double dtm = h->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
+    cg::OdeArgs ode;
+    if (int rc = ode_args(h, t0, dt, &ode)) return rc;
     HIPCHK(cg::launch_ionic_cells(h->model, h->prec, h->tissue, h->d_states, h->n,
                                   model_time(h->model, t0), dtm, n_steps, h->P, h->d_gscale,
-                                  h->stim, h->stream));
+                                  h->stim, ode, h->stream));
     return CG_OK;
 }
 
@@ -470,6 +557,9 @@ int batch_check(cg_ionic* const* hs, int nh, double dt_s)
         if (!hs[i]) return fail(CG_ERR_ARG, "NULL handle %d", i);
         if (!hs[i]->solve_vm) return fail(CG_ERR_STATE, "handle %d is a tissue handle", i);
         if (!hs[i]->lens_set) return fail(CG_ERR_STATE, "handle %d: run lengths not set", i);
+        if (hs[i]->ode_kind != CG_ODE_RUSH_LARSEN)
+            return fail(CG_ERR_STATE, "handle %d: batched protocols integrate with Rush-Larsen; "
+                                      "use cg_ionic_run_cells for RKF45", i);
         for (int j = 0; j < i; ++j)
             if (hs[j] == hs[i]) return fail(CG_ERR_ARG, "handle %d given twice", i);
     }
@@ -633,6 +723,7 @@ static int states_io(cg_ionic* h, int first, int nv, double* host, const double*
     if (!h || (!host && !hin)) return fail(CG_ERR_ARG, "NULL argument");
     if (first < 0 || nv < 0 || first + nv > h->mi.ns) return fail(CG_ERR_ARG, "state range");
     HIPCHK(hipStreamSynchronize(h->stream));
+    if (int rc = check_ode(h)) return rc;
     const size_t cnt = (size_t)nv * h->n;
     char* dsrc = (char*)h->d_states + h->elem * (size_t)first * h->n;
     if (h->prec == CG_FP64) {
@@ -907,6 +998,8 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
     const cg::HexCoefs coefs = mesh_coefs(m);
     // constants of the step that starts at accumulated time t_before after steps_before
     // steps: t += dt; t0 = t - dt; prepareTimeStep(t0) (myocardiumDomain.C:487-495)
+    cg::OdeArgs ode0;
+    if (int rc = ode_args(ion, 0.0, dt, &ode0)) return rc;  // fills the trial steps once
     auto make_step = [&](double t_before, int64_t steps_before) {
         const double tn = t_before + dt;
         const double t0 = tn - dt;
@@ -918,6 +1011,11 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
         st.boxes = active_boxes(t, t0);
         st.od = m->d_od;
         st.od_mask = m->od_mask;
+        st.ode = ode0;
+        if (ode0.rkf) {
+            st.ode.t0 = model_time(ion->model, t0);
+            st.ode.t1 = model_time(ion->model, t0 + dt);
+        }
         return st;
     };
     bool ionic_queued = false;  // this step's ionic launch was queued by the previous one
@@ -957,7 +1055,7 @@ int cg_tissue_sync(cg_tissue* t)
 {
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
     HIPCHK(hipStreamSynchronize(t->ion->stream));
-    return CG_OK;
+    return check_ode(t->ion);
 }
 
 static double* field_ptr(cg_tissue* t, int which)
@@ -977,6 +1075,7 @@ int cg_tissue_get_field(cg_tissue* t, int which, double* host)
     double* d = field_ptr(t, which);
     if (!d) return fail(CG_ERR_ARG, "unknown field %d", which);
     HIPCHK(hipStreamSynchronize(t->ion->stream));
+    if (int rc = check_ode(t->ion)) return rc;
     HIPCHK(hipMemcpy(host, d, sizeof(double) * t->mesh->n, hipMemcpyDeviceToHost));
     return CG_OK;
 }
@@ -1051,6 +1150,8 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
         return fail(CG_ERR_STATE, "the adjoint covers the explicit Euler tissue step");
     if (ion->model != cg::MODEL_BO || ion->prec != CG_FP64)
         return fail(CG_ERR_STATE, "the adjoint covers BuenoOrovio in FP64");
+    if (t->ion->ode_kind != CG_ODE_RUSH_LARSEN)
+        return fail(CG_ERR_STATE, "the adjoint differentiates the Rush-Larsen ionic step");
     if (!m->uniform) return fail(CG_ERR_STATE, "the adjoint scales a uniform conductivity");
     if (m->sig[1] != 0 || m->sig[2] != 0 || m->sig[5] != 0 || m->sig[3] != 0 ||
         m->sig[6] != 0 || m->sig[7] != 0)

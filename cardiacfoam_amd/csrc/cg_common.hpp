@@ -10,20 +10,20 @@
 namespace cg {
 
 // Heaviside of src/ionicModels/BuenoOrovio/heaviside.H:39-49 (strict x > 0).
-template <class T> CG_DEV bool pos(T x) { return x > T(0); }
+template <class T> CG_HD bool pos(T x) { return x > T(0); }
 
 // Precision-generic math: the FP64 path calls the double routines, the FP32 path
 // the single ones (hardware v_exp_f32 based), never mixing the two in one formula.
-CG_DEV double cexp(double x) { return exp(x); }
-CG_DEV float cexp(float x) { return expf(x); }
-CG_DEV double clog(double x) { return log(x); }
-CG_DEV float clog(float x) { return logf(x); }
-CG_DEV double ctanh(double x) { return tanh(x); }
-CG_DEV float ctanh(float x) { return tanhf(x); }
-CG_DEV double csqrt(double x) { return sqrt(x); }
-CG_DEV float csqrt(float x) { return sqrtf(x); }
-CG_DEV double cpow(double x, double y) { return pow(x, y); }
-CG_DEV float cpow(float x, float y) { return powf(x, y); }
+CG_HD double cexp(double x) { return exp(x); }
+CG_HD float cexp(float x) { return expf(x); }
+CG_HD double clog(double x) { return log(x); }
+CG_HD float clog(float x) { return logf(x); }
+CG_HD double ctanh(double x) { return tanh(x); }
+CG_HD float ctanh(float x) { return tanhf(x); }
+CG_HD double csqrt(double x) { return sqrt(x); }
+CG_HD float csqrt(float x) { return sqrtf(x); }
+CG_HD double cpow(double x, double y) { return pow(x, y); }
+CG_HD float cpow(float x, float y) { return powf(x, y); }
 
 // GHK flux quotient v / (exp(k v) - 1) of the generated ICaL/ICab/INab expressions
 // (TNNP_2004.H, ORd_2011.H), with ekv = exp(k v) already computed.  FP64 keeps the plain
@@ -31,8 +31,8 @@ CG_DEV float cpow(float x, float y) { return powf(x, y); }
 // exp(k v) - 1 cancels for small |k v| and is exactly 0 next to v = 0 (a membrane potential
 // crossing 0 mV then gives inf); there the quotient comes from the series
 // x / (e^x - 1) = 1 - x/2 + x^2/12 - x^4/720, x = k v, truncation < 1e-12 for |x| < 0.05.
-CG_DEV double ghk_quot(double v, double, double ekv) { return v / (ekv - 1.0); }
-CG_DEV float ghk_quot(float v, float k, float ekv)
+CG_HD double ghk_quot(double v, double, double ekv) { return v / (ekv - 1.0); }
+CG_HD float ghk_quot(float v, float k, float ekv)
 {
     const float x = k * v;
     if (fabsf(x) < 0.05f) {
@@ -43,12 +43,34 @@ CG_DEV float ghk_quot(float v, float k, float ekv)
 }
 template <class T> constexpr bool is_fp32() { return sizeof(T) == 4; }
 
-template <class T> CG_DEV T sq(T x) { return x * x; }
-template <class T> CG_DEV T cube(T x) { return x * x * x; }
+template <class T> CG_HD T sq(T x) { return x * x; }
+template <class T> CG_HD T cube(T x) { return x * x * x; }
 
 // Rush-Larsen update of one gate over dt: xinf + (x - xinf) exp(-dt/tau)
 // (batchedKernelExecution.H:590-596).
-template <class T> CG_DEV T rl(T x, T xinf, T tau, T dt) { return xinf + (x - xinf) * cexp(-dt / tau); }
+template <class T> CG_HD T rl(T x, T xinf, T tau, T dt) { return xinf + (x - xinf) * cexp(-dt / tau); }
+
+// Update policies for the models' right-hand sides.  A model writes each state once,
+// as out.gate(i, xinf, tau) (dx/dt = (xinf - x)/tau), out.lin(i, a, b) (dx/dt = a - b x)
+// or out.rate(i, r) (dx/dt = r), from rates evaluated at the state it was given.
+//   RlStep: the Rush-Larsen / forward-Euler step of dt in place (gates exact for frozen
+//           xinf, tau; the rest forward Euler).
+//   Deriv:  the time derivative, the RATES array an ODE solver asks for
+//           (TNNP.C:181-203, ORd.C:157-180, BuenoOrovio.C / AlievPanfilov.C derivatives).
+template <class T> struct RlStep {
+    T* y;
+    T dt;
+    CG_HD void gate(int i, T xinf, T tau) { y[i] = rl(y[i], xinf, tau, dt); }
+    CG_HD void lin(int i, T a, T b) { y[i] = rl(y[i], a / b, T(1) / b, dt); }
+    CG_HD void rate(int i, T r) { y[i] += dt * r; }
+};
+template <class T> struct Deriv {
+    const T* y;
+    T* d;
+    CG_HD void gate(int i, T xinf, T tau) { d[i] = (xinf - y[i]) / tau; }
+    CG_HD void lin(int i, T a, T b) { d[i] = a - y[i] * b; }
+    CG_HD void rate(int i, T r) { d[i] = r; }
+};
 
 // Single-cell stimulus protocol of stimulusIO.C:238-286, evaluated in FP64 time.
 struct Stimulus {

@@ -32,6 +32,22 @@ struct HexCoefs {
     const double* fz;
 };
 
+// Per-cell ODE integrator of the ionic step: Rush-Larsen (rkf = 0) or adaptive RKF45
+// (rkf = 1) with the reference's tolerances and per-cell trial step.
+struct OdeArgs {
+    int rkf;
+    double t0, t1;     // model time at the step start and end (tStart, tEnd)
+    double dt_clamp;   // deltaT in model units; a cell's trial step is min(step, dt_clamp)
+    double* step;      // [n] per-cell trial step in model time units (RKF45)
+    int* fail;         // set to 1 by a cell whose integration underflowed or ran out of steps
+    double abs_tol, rel_tol;
+    long long max_steps;
+    // single cells over several steps: PDE time t_s (s) advanced by dt_s per step, the
+    // step start taken as (t + dt) - dt like Time's accumulated value; ap: time / 12.9
+    double t_s, dt_s;
+    int ap;
+};
+
 struct ExplicitStep {
     double dt;         // s
     double dt_model;   // model time units
@@ -45,6 +61,7 @@ struct ExplicitStep {
     // generation written by an earlier kernel of the stream)
     const long long* gate;
     long long gate_val;
+    OdeArgs ode;
 };
 
 // Model kernels, one set per arithmetic type: _fp64 (IEEE double, model_fp64.hip) and
@@ -54,12 +71,13 @@ struct ExplicitStep {
     hipError_t launch_ionic_tissue_##SFX(int model, int tissue, void* states, int64_t n,   \
                                          const double* Vm, double* Im, double dt_model,    \
                                          const ModelParams& P, const void* gscale,         \
-                                         hipStream_t s);                                   \
+                                         const OdeArgs& ode, hipStream_t s);               \
     /* n independent single cells (V inside the ODE), n_steps steps from model time t0 */  \
     hipError_t launch_ionic_cells_##SFX(int model, int tissue, void* states, int64_t n,    \
                                         double t0, double dt, int n_steps,                 \
                                         const ModelParams& P, const void* gscale,          \
-                                        const Stimulus& stim, hipStream_t s);              \
+                                        const Stimulus& stim, const OdeArgs& ode,          \
+                                        hipStream_t s);                                    \
     /* fused explicit monodomain step on the hex block: ionic + diffusion + activation */  \
     hipError_t launch_explicit_hex_##SFX(int model, int tissue, const HexGeom& g,          \
                                          const HexCoefs& a, void* states, const double* V, \
@@ -121,21 +139,22 @@ const char* batch_variant_name(int v);
 // precision dispatch: prec 0 = fp64, 1 = fp32
 inline hipError_t launch_ionic_tissue(int model, int prec, int tissue, void* states, int64_t n,
                                       const double* Vm, double* Im, double dt_model,
-                                      const ModelParams& P, const void* gscale, hipStream_t s)
+                                      const ModelParams& P, const void* gscale,
+                                      const OdeArgs& ode, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     return (prec ? launch_ionic_tissue_fp32 : launch_ionic_tissue_fp64)(
-        model, tissue, states, n, Vm, Im, dt_model, P, gscale, s);
+        model, tissue, states, n, Vm, Im, dt_model, P, gscale, ode, s);
 }
 
 inline hipError_t launch_ionic_cells(int model, int prec, int tissue, void* states, int64_t n,
                                      double t0_model, double dt_model, int n_steps,
                                      const ModelParams& P, const void* gscale,
-                                     const Stimulus& stim, hipStream_t s)
+                                     const Stimulus& stim, const OdeArgs& ode, hipStream_t s)
 {
     if (n <= 0 || n_steps <= 0) return hipSuccess;
     return (prec ? launch_ionic_cells_fp32 : launch_ionic_cells_fp64)(
-        model, tissue, states, n, t0_model, dt_model, n_steps, P, gscale, stim, s);
+        model, tissue, states, n, t0_model, dt_model, n_steps, P, gscale, stim, ode, s);
 }
 
 inline hipError_t launch_explicit_hex(int model, int prec, int tissue, const HexGeom& g,

@@ -38,7 +38,7 @@ struct BuenoOrovio {
     }
 
     // Jion at (u, v, w, s): the three currents of BuenoOrovio_2008.H:171-188
-    template <class T> CG_DEV static T jion(T u, const T* y, const T* p)
+    template <class T> CG_HD static T jion(T u, const T* y, const T* p)
     {
         const bool Hv = pos(u - p[thv]), Hw = pos(u - p[thw]), Ho = pos(u - p[tho]);
         const T tso = p[tso1] + (p[tso2] - p[tso1]) * (T(1) + ctanh(p[kso] * (u - p[uso]))) / T(2);
@@ -49,8 +49,8 @@ struct BuenoOrovio {
         return jfi + jso + jsi;
     }
 
-    // Advance v, w, s by dt with u frozen; y[0] is u.
-    template <class T> CG_DEV static void gates(T* y, T dt, const T* p)
+    // Rates of v, w, s at u = y[0] (BuenoOrovio_2008.H:189-220), through an update policy.
+    template <class T, class U> CG_HD static void gate_rates(const T* y, U& out, const T* p)
     {
         const T u = y[0];
         const bool Hv = pos(u - p[thv]), Hw = pos(u - p[thw]), Ho = pos(u - p[tho]),
@@ -65,14 +65,30 @@ struct BuenoOrovio {
         // s
         const T sinf = (T(1) + ctanh(p[ks] * (u - p[us]))) / T(2);
         const T ts = Hw ? p[ts2] : p[ts1];
-        y[1] = rl(y[1], vinf, tv, dt);
-        y[2] = rl(y[2], winf, tw, dt);
-        y[3] = rl(y[3], sinf, ts, dt);
+        out.gate(1, vinf, tv);
+        out.gate(2, winf, tw);
+        out.gate(3, sinf, ts);
+    }
+
+    // Advance v, w, s by dt with u frozen; y[0] is u.
+    template <class T> CG_HD static void gates(T* y, T dt, const T* p)
+    {
+        RlStep<T> out{y, dt};
+        gate_rates(y, out, p);
+    }
+
+    // dy/dt; du/dt = -Jion - Istim when u is integrated with the gates, else 0
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, bool solve_v, T istim)
+    {
+        Deriv<T> out{y, d};
+        gate_rates(y, out, p);
+        d[0] = solve_v ? -jion(y[0], y, p) - istim : T(0);
     }
 
     // Single-cell step (Vm inside the ODE): RL gates, forward-Euler u with
     // du/dt = -Jion - Istim at the step start (BuenoOrovio_2008.H:221-224).
-    template <class T> CG_DEV static void cell_step(T* y, T dt, T istim, const T* p)
+    template <class T> CG_HD static void cell_step(T* y, T dt, T istim, const T* p)
     {
         const T du = -jion(y[0], y, p) - istim;
         gates(y, dt, p);
@@ -100,19 +116,33 @@ struct AlievPanfilov {
         return true;
     }
 
-    template <class T> CG_DEV static T jion(T u, const T* y, const T* p)
+    template <class T> CG_HD static T jion(T u, const T* y, const T* p)
     {
         return p[kk] * u * (u - p[aa]) * (u - T(1)) + y[1] * u;
     }
 
-    template <class T> CG_DEV static void gates(T* y, T dt, const T* p)
+    template <class T, class U> CG_HD static void gate_rates(const T* y, U& out, const T* p)
     {
         const T u = y[0], r = y[1];
         const T e = p[eps0] + p[mu1] * r / (p[mu2] + u);
-        y[1] = r + dt * (e * (-r - p[kk] * u * (u - (p[aa] + T(1)))));
+        out.rate(1, e * (-r - p[kk] * u * (u - (p[aa] + T(1)))));
     }
 
-    template <class T> CG_DEV static void cell_step(T* y, T dt, T istim, const T* p)
+    template <class T> CG_HD static void gates(T* y, T dt, const T* p)
+    {
+        RlStep<T> out{y, dt};
+        gate_rates(y, out, p);
+    }
+
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, bool solve_v, T istim)
+    {
+        Deriv<T> out{y, d};
+        gate_rates(y, out, p);
+        d[0] = solve_v ? -jion(y[0], y, p) - istim : T(0);
+    }
+
+    template <class T> CG_HD static void cell_step(T* y, T dt, T istim, const T* p)
     {
         const T du = -jion(y[0], y, p) - istim;
         gates(y, dt, p);

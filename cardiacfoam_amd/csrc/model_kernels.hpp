@@ -9,6 +9,7 @@
 #pragma once
 #include "cg_internal.hpp"
 #include "cg_hexops.hpp"
+#include "ode_rkf45.hpp"
 
 namespace cg {
 
@@ -41,19 +42,44 @@ CG_DEV void store_states(const T (&y)[NS], T* __restrict__ st, int64_t n, int64_
     for (int q = first; q < NS; ++q, sp += n) *sp = y[q];
 }
 
+// Ionic step of cell c in tissue mode with V held at v over the step; returns the total
+// ionic current at the step end.  INTEG 0: Rush-Larsen; 1: RKF45 over [o.t0, o.t1] from
+// the cell's kept trial step (BuenoOrovio.C:124-132 and its TNNP/ORd/AP twins), the V
+// terms computed once for all stages.
+template <class M, class T, int INTEG>
+CG_DEV T tissue_advance(T v, T (&y)[M::NS], T dt, const T* p, int tissue, const T* gs,
+                        const OdeArgs& o, int64_t c)
+{
+    if constexpr (INTEG == 0) {
+        return Ops<M>::tissue_step(v, y, dt, p, tissue, gs);
+    } else {
+        y[0] = v;
+        const auto q = Ops<M>::vt(v, p);
+        auto f = [&](double, const T* yy, T* d) __attribute__((always_inline)) {
+            Ops<M>::deriv(yy, d, p, tissue, gs, false, T(0), q);
+        };
+        double step = o.step[c];
+        if (!(step < o.dt_clamp)) step = o.dt_clamp;
+        const bool ok = rkf45_solve<M::NS>(f, y, o.t0, o.t1, step, T(o.abs_tol), T(o.rel_tol),
+                                           o.max_steps);
+        o.step[c] = step;
+        if (!ok) *o.fail = 1;
+        return Ops<M>::iion(y, p, tissue, gs);
+    }
+}
+
 // ---------------------------------------------------------------- ionic only
-template <class M, class T>
+template <class M, class T, int INTEG>
 __global__ __launch_bounds__(MBS) void k_ionic_tissue(T* __restrict__ st, int64_t n,
                                                       const double* __restrict__ Vm,
                                                       double* __restrict__ Im, T dt,
                                                       KParams<T> P,
-                                                      const T* __restrict__ gsc, int tissue)
+                                                      const T* __restrict__ gsc, int tissue,
+                                                      OdeArgs ode)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    T p[M::NP];
-#pragma unroll
-    for (int k = 0; k < M::NP; ++k) p[k] = P.p[k];
+    const T* p = P.p;  // model constants read from the kernel arguments
     T gs[4] = {T(1), T(1), T(1), T(1)};
     if (gsc) {
 #pragma unroll
@@ -62,22 +88,22 @@ __global__ __launch_bounds__(MBS) void k_ionic_tissue(T* __restrict__ st, int64_
     T y[M::NS];
     load_states(y, st, n, i, 1);
     const T v = T((Vm[i] * 1000.0 + M::VOFF) / M::VSCALE);
-    const T I = Ops<M>::tissue_step(v, y, dt, p, tissue, gs);
+    const T I = tissue_advance<M, T, INTEG>(v, y, dt, p, tissue, gs, ode, i);
     store_states(y, st, n, i, 1);
     Im[i] = double(I) * M::ISCALE;
 }
 
-template <class M, class T>
+// n single cells with V integrated inside the ODE (singleCellSolver.C:179-229), n_steps
+// steps of dt from model time t0; RKF45 evaluates the stimulus at every stage time.
+template <class M, class T, int INTEG>
 __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t n, double t0,
                                                      T dt, int n_steps, KParams<T> P,
                                                      const T* __restrict__ gsc, int tissue,
-                                                     Stimulus stim)
+                                                     Stimulus stim, OdeArgs ode)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    T p[M::NP];
-#pragma unroll
-    for (int k = 0; k < M::NP; ++k) p[k] = P.p[k];
+    const T* p = P.p;  // model constants read from the kernel arguments
     T gs[4] = {T(1), T(1), T(1), T(1)};
     if (gsc) {
 #pragma unroll
@@ -85,9 +111,31 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
     }
     T y[M::NS];
     load_states(y, st, n, i, 0);
-    for (int s = 0; s < n_steps; ++s) {
-        const double t = t0 + s * double(dt);
-        Ops<M>::cell_step(y, dt, T(stimulus_current(t, stim)), p, tissue, gs);
+    if constexpr (INTEG == 0) {
+        for (int s = 0; s < n_steps; ++s) {
+            const double t = t0 + s * double(dt);
+            Ops<M>::cell_step(y, dt, T(stimulus_current(t, stim)), p, tissue, gs);
+        }
+    } else {
+        auto f = [&](double t, const T* yy, T* d) __attribute__((always_inline)) {
+            const auto q = Ops<M>::vt(yy[0], p);
+            Ops<M>::deriv(yy, d, p, tissue, gs, true, T(stimulus_current(t, stim)), q);
+        };
+        double step = ode.step[i];
+        bool ok = true;
+        double tacc = ode.t_s;
+        for (int s = 0; s < n_steps && ok; ++s) {
+            const double tn = tacc + ode.dt_s;
+            const double ts = tn - ode.dt_s;
+            tacc = tn;
+            const double x0 = ode.ap ? ts * 1000.0 / 12.9 : ts * 1000.0;
+            const double x1 = ode.ap ? (ts + ode.dt_s) * 1000.0 / 12.9 : (ts + ode.dt_s) * 1000.0;
+            if (!(step < ode.dt_clamp)) step = ode.dt_clamp;
+            ok = rkf45_solve<M::NS>(f, y, x0, x1, step, T(ode.abs_tol), T(ode.rel_tol),
+                                    ode.max_steps);
+        }
+        ode.step[i] = step;
+        if (!ok) *ode.fail = 1;
     }
     store_states(y, st, n, i, 0);
 }
@@ -113,7 +161,7 @@ CG_DEV double box_source(int i, int j, int k, const HexGeom& g, const ActiveBoxe
 // -> activation time where V first exceeds 1e-15 V (myocardiumDomain.C:345-374).
 // Bytes per cell-step (fp64, BuenoOrovio): V 8 + gates 24 r/w + V^{n+1} 8 + Iion 8 +
 // flag 1 (+ V^{n-1} 8 for backward, + off-diagonal sigma 24 with a tensor correction).
-template <class M, class T, bool UNIFORM>
+template <class M, class T, bool UNIFORM, int INTEG>
 __global__ __launch_bounds__(MBS) void k_explicit_hex(
     HexGeom g, HexCoefs a, T* __restrict__ st, const double* __restrict__ V,
     const double* __restrict__ Vold, double* __restrict__ Vnew, double* __restrict__ Iion,
@@ -124,15 +172,13 @@ __global__ __launch_bounds__(MBS) void k_explicit_hex(
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const IJK ix = ijk_of((uint32_t)c, g);
-    T p[M::NP];
-#pragma unroll
-    for (int q = 0; q < M::NP; ++q) p[q] = P.p[q];
+    const T* p = P.p;  // model constants read from the kernel arguments
     const T one[4] = {T(1), T(1), T(1), T(1)};
     T y[M::NS];
     load_states(y, st, n, c, 1);
     const double vn = V[c];
     const T v = T((vn * 1000.0 + M::VOFF) / M::VSCALE);
-    const T I = Ops<M>::tissue_step(v, y, T(s.dt_model), p, tissue, one);
+    const T I = tissue_advance<M, T, INTEG>(v, y, T(s.dt_model), p, tissue, one, s.ode, c);
     store_states(y, st, n, c, 1);
     const double iion = double(I) * M::ISCALE;
     if (Iion) Iion[c] = iion;
@@ -155,7 +201,7 @@ __global__ __launch_bounds__(MBS) void k_explicit_hex(
 // explicit tensor correction of the Laplacian (null when sigma is diagonal).  With a gate
 // (s.gate) the launch may be queued before the previous step's solve is known to have
 // converged: it then does nothing unless the solver recorded the expected generation.
-template <class M, class T>
+template <class M, class T, int INTEG>
 __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st,
                                                    const double* __restrict__ Vn,
                                                    const double* __restrict__ Vm1,
@@ -169,15 +215,13 @@ __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st
     const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * MBS + threadIdx.x;
     if (c >= n) return;
     const IJK cc = ijk_of((uint32_t)c, g);
-    T p[M::NP];
-#pragma unroll
-    for (int q = 0; q < M::NP; ++q) p[q] = P.p[q];
+    const T* p = P.p;  // model constants read from the kernel arguments
     const T one[4] = {T(1), T(1), T(1), T(1)};
     T y[M::NS];
     load_states(y, st, n, c, 1);
     const double vn = Vn[c];
     const T v = T((vn * 1000.0 + M::VOFF) / M::VSCALE);
-    const T I = Ops<M>::tissue_step(v, y, T(s.dt_model), p, tissue, one);
+    const T I = tissue_advance<M, T, INTEG>(v, y, T(s.dt_model), p, tissue, one, s.ode, c);
     store_states(y, st, n, c, 1);
     const double iion = double(I) * M::ISCALE;
     Iion[c] = iion;
@@ -199,21 +243,33 @@ template <class T> struct ModelLaunch {
     template <class M>
     static hipError_t tissue(int tissue, void* states, int64_t n, const double* Vm, double* Im,
                              double dt_model, const ModelParams& P, const void* gscale,
-                             hipStream_t s)
+                             const OdeArgs& ode, hipStream_t s)
     {
         const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
-        hipLaunchKernelGGL((k_ionic_tissue<M, T>), dim3(grid), dim3(MBS), 0, s, (T*)states, n,
-                           Vm, Im, T(dt_model), kparams<T>(P), (const T*)gscale, tissue);
+        if (ode.rkf)
+            hipLaunchKernelGGL((k_ionic_tissue<M, T, 1>), dim3(grid), dim3(MBS), 0, s,
+                               (T*)states, n, Vm, Im, T(dt_model), kparams<T>(P),
+                               (const T*)gscale, tissue, ode);
+        else
+            hipLaunchKernelGGL((k_ionic_tissue<M, T, 0>), dim3(grid), dim3(MBS), 0, s,
+                               (T*)states, n, Vm, Im, T(dt_model), kparams<T>(P),
+                               (const T*)gscale, tissue, ode);
         return hipGetLastError();
     }
     template <class M>
     static hipError_t cells(int tissue, void* states, int64_t n, double t0, double dt,
                             int n_steps, const ModelParams& P, const void* gscale,
-                            const Stimulus& stim, hipStream_t s)
+                            const Stimulus& stim, const OdeArgs& ode, hipStream_t s)
     {
         const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
-        hipLaunchKernelGGL((k_ionic_cells<M, T>), dim3(grid), dim3(MBS), 0, s, (T*)states, n,
-                           t0, T(dt), n_steps, kparams<T>(P), (const T*)gscale, tissue, stim);
+        if (ode.rkf)
+            hipLaunchKernelGGL((k_ionic_cells<M, T, 1>), dim3(grid), dim3(MBS), 0, s,
+                               (T*)states, n, t0, T(dt), n_steps, kparams<T>(P),
+                               (const T*)gscale, tissue, stim, ode);
+        else
+            hipLaunchKernelGGL((k_ionic_cells<M, T, 0>), dim3(grid), dim3(MBS), 0, s,
+                               (T*)states, n, t0, T(dt), n_steps, kparams<T>(P),
+                               (const T*)gscale, tissue, stim, ode);
         return hipGetLastError();
     }
     template <class M>
@@ -224,12 +280,17 @@ template <class T> struct ModelLaunch {
     {
         const int64_t n = (int64_t)g.nx * g.ny * g.nz;
         const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
-        if (a.fx)
-            hipLaunchKernelGGL((k_explicit_hex<M, T, false>), dim3(grid), dim3(MBS), 0, s, g,
-                               a, (T*)states, V, Vold, Vnew, Iion, flag, tact, st, kparams<T>(P), tissue);
-        else
-            hipLaunchKernelGGL((k_explicit_hex<M, T, true>), dim3(grid), dim3(MBS), 0, s, g, a,
-                               (T*)states, V, Vold, Vnew, Iion, flag, tact, st, kparams<T>(P), tissue);
+#define CG_EXPLICIT_HEX(U, I)                                                              \
+    hipLaunchKernelGGL((k_explicit_hex<M, T, U, I>), dim3(grid), dim3(MBS), 0, s, g, a,    \
+                       (T*)states, V, Vold, Vnew, Iion, flag, tact, st, kparams<T>(P), tissue)
+        if (st.ode.rkf) {
+            if (a.fx) CG_EXPLICIT_HEX(false, 1);
+            else CG_EXPLICIT_HEX(true, 1);
+        } else {
+            if (a.fx) CG_EXPLICIT_HEX(false, 0);
+            else CG_EXPLICIT_HEX(true, 0);
+        }
+#undef CG_EXPLICIT_HEX
         return hipGetLastError();
     }
     template <class M>
@@ -239,8 +300,12 @@ template <class T> struct ModelLaunch {
                                 hipStream_t s)
     {
         const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-        hipLaunchKernelGGL((k_ionic_rhs<M, T>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,
-                           (T*)states, Vn, Vm1, corr, Iion, b, st, kparams<T>(P), tissue);
+        if (st.ode.rkf)
+            hipLaunchKernelGGL((k_ionic_rhs<M, T, 1>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,
+                               (T*)states, Vn, Vm1, corr, Iion, b, st, kparams<T>(P), tissue);
+        else
+            hipLaunchKernelGGL((k_ionic_rhs<M, T, 0>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,
+                               (T*)states, Vn, Vm1, corr, Iion, b, st, kparams<T>(P), tissue);
         return hipGetLastError();
     }
 };
@@ -260,16 +325,18 @@ template <class T> struct ModelLaunch {
     hipError_t launch_ionic_tissue_##SFX(int model, int tissue, void* states, int64_t n,   \
                                          const double* Vm, double* Im, double dt_model,    \
                                          const ModelParams& P, const void* gscale,         \
-                                         hipStream_t s)                                    \
+                                         const OdeArgs& ode, hipStream_t s)                \
     {                                                                                      \
-        CG_MODEL_SWITCH(T, tissue, tissue, states, n, Vm, Im, dt_model, P, gscale, s)      \
+        CG_MODEL_SWITCH(T, tissue, tissue, states, n, Vm, Im, dt_model, P, gscale, ode, s) \
     }                                                                                      \
     hipError_t launch_ionic_cells_##SFX(int model, int tissue, void* states, int64_t n,    \
                                         double t0, double dt, int n_steps,                 \
                                         const ModelParams& P, const void* gscale,          \
-                                        const Stimulus& stim, hipStream_t s)               \
+                                        const Stimulus& stim, const OdeArgs& ode,          \
+                                        hipStream_t s)                                     \
     {                                                                                      \
-        CG_MODEL_SWITCH(T, cells, tissue, states, n, t0, dt, n_steps, P, gscale, stim, s)  \
+        CG_MODEL_SWITCH(T, cells, tissue, states, n, t0, dt, n_steps, P, gscale, stim,     \
+                        ode, s)                                                            \
     }                                                                                      \
     hipError_t launch_explicit_hex_##SFX(int model, int tissue, const HexGeom& g,          \
                                          const HexCoefs& a, void* states, const double* V, \

@@ -53,7 +53,7 @@ struct ORd {
         T hca, hna, Knai, Knao, e1, e2, vffrt, rk1, xkb, Axrf, rkr, AiF, Afcaf;
     };
 
-    template <class T> CG_DEV static VTerms<T> vterms(T Vv)
+    template <class T> CG_HD static VTerms<T> vterms(T Vv)
     {
         VTerms<T> q;
         const T FRT = T(Fc / (Rc * Tc));
@@ -76,7 +76,7 @@ struct ORd {
 
     // Na/Ca exchanger of one compartment (ORd_2011.H:715-777)
     template <class T>
-    CG_DEV static T ncx(T na, T ca, const VTerms<T>& q, T gfrac)
+    CG_HD static T ncx(T na, T ca, const VTerms<T>& q, T gfrac)
     {
         const T kna1 = T(15), kna2 = T(5), kna3 = T(88.12), kcaon = T(1500000), kcaoff = T(5000);
         const T wca = T(60000), wna = T(60000), wnaca = T(5000);
@@ -111,7 +111,7 @@ struct ORd {
     template <class T> struct Currents {
         T INa, INaL, Ito, ICaL, ICaNa, ICaK, IKr, IKs, IK1, INaCai, INaCass, INaK, INab, IKb,
           IpCa, ICab, CaMKa;
-        CG_DEV T total() const
+        CG_HD T total() const
         {
             return INa + INaL + Ito + ICaL + ICaNa + ICaK + IKr + IKs + IK1 + INaCai + INaCass +
                    INaK + INab + IKb + IpCa + ICab;
@@ -119,7 +119,7 @@ struct ORd {
     };
 
     template <class T>
-    CG_DEV static Currents<T> currents(T Vv, const T* y, const T* p, const VTerms<T>& q,
+    CG_HD static Currents<T> currents(T Vv, const T* y, const T* p, const VTerms<T>& q,
                                        const T* gs)
     {
         // gs: per-cell multipliers of GNa, GKr, PCa, GKs
@@ -215,9 +215,11 @@ struct ORd {
         return c;
     }
 
-    template <class T>
-    CG_DEV static void step(T* y, T dt, const T* p, int tissue, const T* gs, bool solve_v,
-                            T istim, const VTerms<T>& q)
+    // Rates at the state y (y[V] in mV) through an update policy (ORd_2011.H:795-989);
+    // V has a rate only when solve_v (-Iion - Istim).
+    template <class T, class U>
+    CG_HD static void rates(const T* y, U& out, const T* p, int tissue, const T* gs,
+                             bool solve_v, T istim, const VTerms<T>& q)
     {
         (void)tissue;
         const T Vv = y[V];
@@ -270,20 +272,20 @@ struct ORd {
         {
             const T txk1 = T(122.2) / (cexp(-(Vv + T(127.2)) / T(20.36)) + cexp((Vv + T(236.8)) / T(69.33)));
             const T xk1ss = T(1) / (T(1) + cexp(-(Vv + T(2.5538 * ko + 144.59)) / T(1.5692 * ko + 3.8115)));
-            y[xk1] = rl(y[xk1], xk1ss, txk1, dt);
+            out.gate(xk1, xk1ss, txk1);
             const T txrf = T(12.98) + T(1) / (T(0.3652) * cexp((Vv - T(31.66)) / T(3.869)) +
                                               T(4.123e-05) * cexp(-(Vv - T(47.78)) / T(20.38)));
             const T txrs = T(1.865) + T(1) / (T(0.06629) * cexp((Vv - T(34.7)) / T(7.355)) +
                                               T(1.128e-05) * cexp(-(Vv - T(29.74)) / T(25.94)));
             const T xrss = T(1) / (T(1) + cexp(-(Vv + T(8.337)) / T(6.789)));
-            y[xrf] = rl(y[xrf], xrss, txrf, dt);
-            y[xrs] = rl(y[xrs], xrss, txrs, dt);
+            out.gate(xrf, xrss, txrf);
+            out.gate(xrs, xrss, txrs);
             const T txs1 = T(817.3) + T(1) / (T(0.0002326) * cexp((Vv + T(48.28)) / T(17.8)) +
                                               T(0.001292) * cexp(-(Vv + T(210)) / T(230)));
             const T txs2 = T(1) / (T(0.01) * cexp((Vv - T(50)) / T(20)) + T(0.0193) * cexp(-(Vv + T(66.54)) / T(31)));
             const T xs1ss = T(1) / (T(1) + cexp(-(Vv + T(11.6)) / T(8.932)));
-            y[xs1] = rl(y[xs1], xs1ss, txs1, dt);
-            y[xs2] = rl(y[xs2], xs1ss, txs2, dt);
+            out.gate(xs1, xs1ss, txs1);
+            out.gate(xs2, xs1ss, txs2);
             const T hss = T(1) / (T(1) + cexp((Vv + T(82.9)) / T(6.086)));
             const T hssp = T(1) / (T(1) + cexp((Vv + T(89.1)) / T(6.086)));
             const T thf = T(1) / (T(1.432e-05) * cexp(-(Vv + T(1.196)) / T(6.285)) + T(6.149) * cexp((Vv + T(0.5096)) / T(20.27)));
@@ -291,12 +293,12 @@ struct ORd {
             const T tj = T(2.038) + T(1) / (T(0.02136) * cexp(-(Vv + T(100.6)) / T(8.281)) + T(0.3052) * cexp((Vv + T(0.9941)) / T(38.45)));
             const T mss = T(1) / (T(1) + cexp(-(Vv + T(39.57)) / T(9.871)));
             const T tm = T(1) / (T(6.765) * cexp((Vv + T(11.64)) / T(34.77)) + T(8.552) * cexp(-(Vv + T(77.42)) / T(5.955)));
-            y[hf] = rl(y[hf], hss, thf, dt);
-            y[hs] = rl(y[hs], hss, ths, dt);
-            y[hsp] = rl(y[hsp], hssp, T(3) * ths, dt);
-            y[m] = rl(y[m], mss, tm, dt);
-            y[j] = rl(y[j], hss, tj, dt);
-            y[jp] = rl(y[jp], hss, T(1.46) * tj, dt);
+            out.gate(hf, hss, thf);
+            out.gate(hs, hss, ths);
+            out.gate(hsp, hssp, T(3) * ths);
+            out.gate(m, mss, tm);
+            out.gate(j, hss, tj);
+            out.gate(jp, hss, T(1.46) * tj);
             const T ass = T(1) / (T(1) + cexp(-(Vv - T(14.34)) / T(14.82)));
             const T assp = T(1) / (T(1) + cexp(-(Vv - T(24.34)) / T(14.82)));
             const T dti_dev = T(1.354) + T(0.0001) / (cexp((Vv - T(167.4)) / T(15.89)) + cexp(-(Vv - T(12.23)) / T(0.2154)));
@@ -306,18 +308,18 @@ struct ORd {
                                        T(3.5) / (T(1) + cexp((Vv + T(100)) / T(29.3814))));
             const T tiF = T(4.562) + T(1) / (T(0.3933) * cexp(-(Vv + T(100)) / T(100)) + T(0.08004) * cexp((Vv + T(50)) / T(16.59)));
             const T tiS = T(23.62) + T(1) / (T(0.001416) * cexp(-(Vv + T(96.52)) / T(59.05)) + T(1.78e-08) * cexp((Vv + T(114.1)) / T(8.079)));
-            y[a] = rl(y[a], ass, ta_, dt);
-            y[ap] = rl(y[ap], assp, ta_, dt);
-            y[iF] = rl(y[iF], iss, tiF, dt);
-            y[iS] = rl(y[iS], iss, tiS, dt);
-            y[iFp] = rl(y[iFp], iss, dti_dev * dti_rec * tiF, dt);
-            y[iSp] = rl(y[iSp], iss, dti_dev * dti_rec * tiS, dt);
+            out.gate(a, ass, ta_);
+            out.gate(ap, assp, ta_);
+            out.gate(iF, iss, tiF);
+            out.gate(iS, iss, tiS);
+            out.gate(iFp, iss, dti_dev * dti_rec * tiF);
+            out.gate(iSp, iss, dti_dev * dti_rec * tiS);
             const T hLss = T(1) / (T(1) + cexp((Vv + T(87.61)) / T(7.488)));
             const T hLssp = T(1) / (T(1) + cexp((Vv + T(93.81)) / T(7.488)));
             const T mLss = T(1) / (T(1) + cexp(-(Vv + T(42.85)) / T(5.264)));
-            y[hL] = rl(y[hL], hLss, T(200), dt);
-            y[mL] = rl(y[mL], mLss, tm, dt);
-            y[hLp] = rl(y[hLp], hLssp, T(600), dt);
+            out.gate(hL, hLss, T(200));
+            out.gate(mL, mLss, tm);
+            out.gate(hLp, hLssp, T(600));
             const T dss = T(1) / (T(1) + cexp(-(Vv + T(3.94)) / T(4.23)));
             const T fss = T(1) / (T(1) + cexp((Vv + T(19.58)) / T(3.696)));
             const T td = T(0.6) + T(1) / (cexp(T(-0.05) * (Vv + T(6))) + cexp(T(0.09) * (Vv + T(14))));
@@ -325,28 +327,47 @@ struct ORd {
             const T tfcas = T(100) + T(1) / (T(0.00012) * cexp(-Vv / T(3)) + T(0.00012) * cexp(Vv / T(7)));
             const T tff = T(7) + T(1) / (T(0.0045) * cexp(-(Vv + T(20)) / T(10)) + T(0.0045) * cexp((Vv + T(20)) / T(10)));
             const T tfs = T(1000) + T(1) / (T(3.5e-05) * cexp(-(Vv + T(5)) / T(4)) + T(3.5e-05) * cexp((Vv + T(5)) / T(6)));
-            y[d] = rl(y[d], dss, td, dt);
-            y[ff] = rl(y[ff], fss, tff, dt);
-            y[fs] = rl(y[fs], fss, tfs, dt);
-            y[fcaf] = rl(y[fcaf], fss, tfcaf, dt);
-            y[fcafp] = rl(y[fcafp], fss, T(2.5) * tfcaf, dt);
-            y[fcas] = rl(y[fcas], fss, tfcas, dt);
-            y[ffp] = rl(y[ffp], fss, T(2.5) * tff, dt);
-            y[jca] = rl(y[jca], fss, T(75), dt);
+            out.gate(d, dss, td);
+            out.gate(ff, fss, tff);
+            out.gate(fs, fss, tfs);
+            out.gate(fcaf, fss, tfcaf);
+            out.gate(fcafp, fss, T(2.5) * tfcaf);
+            out.gate(fcas, fss, tfcas);
+            out.gate(ffp, fss, T(2.5) * tff);
+            out.gate(jca, fss, T(75));
         }
-        y[nca] = rl(y[nca], anca * T(1000) / km2n, T(1) / km2n, dt);
-        y[Jrelnp] = rl(y[Jrelnp], Jrel_inf, trel, dt);
-        y[Jrelp] = rl(y[Jrelp], Jrel_infp, trelp, dt);
-        y[CaMKt] += dt * dCaMKt;
-        y[Nai] += dt * dNai;
-        y[Nass] += dt * dNass;
-        y[Ki] += dt * dKi;
-        y[Kss] += dt * dKss;
-        y[Cai] += dt * dCai;
-        y[Cass] += dt * dCass;
-        y[Cansr] += dt * dCansr;
-        y[Cajsr] += dt * dCajsr;
-        if (solve_v) y[V] = Vv + dt * (-Iion - istim);
+        out.lin(nca, anca * T(1000), km2n);  // dn/dt = anca k2n - n km2n
+        out.gate(Jrelnp, Jrel_inf, trel);
+        out.gate(Jrelp, Jrel_infp, trelp);
+        out.rate(CaMKt, dCaMKt);
+        out.rate(Nai, dNai);
+        out.rate(Nass, dNass);
+        out.rate(Ki, dKi);
+        out.rate(Kss, dKss);
+        out.rate(Cai, dCai);
+        out.rate(Cass, dCass);
+        out.rate(Cansr, dCansr);
+        out.rate(Cajsr, dCajsr);
+        if (solve_v) out.rate(V, -Iion - istim);
+    }
+
+    // one RL/FE step of dt in place
+    template <class T>
+    CG_HD static void step(T* y, T dt, const T* p, int tissue, const T* gs, bool solve_v,
+                            T istim, const VTerms<T>& q)
+    {
+        RlStep<T> out{y, dt};
+        rates(y, out, p, tissue, gs, solve_v, istim, q);
+    }
+
+    // dy/dt at y (dV/dt = 0 unless V is integrated with the other states)
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int tissue, const T* gs, bool solve_v,
+                             T istim, const VTerms<T>& q)
+    {
+        Deriv<T> out{y, d};
+        d[V] = T(0);
+        rates(y, out, p, tissue, gs, solve_v, istim, q);
     }
 };
 

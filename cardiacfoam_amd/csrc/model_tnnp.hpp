@@ -39,7 +39,7 @@ struct TNNP {
         T nak, ncx_num, ncx_den, cal_e2, ipk;
     };
 
-    template <class T> CG_DEV static VTerms<T> vterms(T Vv, const T* p)
+    template <class T> CG_HD static VTerms<T> vterms(T Vv, const T* p)
     {
         const T FRT = p[F] / (p[R] * p[Tk]);
         VTerms<T> q;
@@ -53,14 +53,14 @@ struct TNNP {
 
     template <class T> struct Currents {
         T INa, IbNa, INaK, INaCa, IK1, Ito, IKr, IKs, ICaL, IbCa, IpK, IpCa;
-        CG_DEV T total() const
+        CG_HD T total() const
         {
             return IK1 + Ito + IKr + IKs + ICaL + INaK + INa + IbNa + INaCa + IbCa + IpK + IpCa;
         }
     };
 
     template <class T>
-    CG_DEV static Currents<T> currents(T Vv, const T* y, const T* p, const VTerms<T>& q,
+    CG_HD static Currents<T> currents(T Vv, const T* y, const T* p, const VTerms<T>& q,
                                        const T* gs)
     {
         // gs: per-cell multipliers of g_Na, g_Kr, g_CaL, g_Ks (1 when unscaled)
@@ -98,11 +98,11 @@ struct TNNP {
         return c;
     }
 
-    // One RL/FE step of dt from the state y (y[V] is the membrane potential, mV).
-    // Returns nothing; when solve_v the potential also advances with -Iion - Istim.
-    template <class T>
-    CG_DEV static void step(T* y, T dt, const T* p, int tissue, const T* gs, bool solve_v,
-                            T istim, const VTerms<T>& q)
+    // Rates at the state y (y[V] is the membrane potential, mV) through an update policy;
+    // V has a rate only when solve_v (-Iion - Istim).
+    template <class T, class U>
+    CG_HD static void rates(const T* y, U& out, const T* p, int tissue, const T* gs,
+                             bool solve_v, T istim, const VTerms<T>& q)
     {
         const T Vv = y[V];
         const Currents<T> c = currents(Vv, y, p, q, gs);
@@ -137,7 +137,7 @@ struct TNNP {
             const T am = T(1) / (T(1) + cexp((T(-60) - Vv) / T(5)));
             const T bm = T(0.1) / (T(1) + cexp((Vv + T(35)) / T(5))) +
                          T(0.1) / (T(1) + cexp((Vv - T(50)) / T(200)));
-            y[m] = rl(y[m], minf, am * bm, dt);
+            out.gate(m, minf, am * bm);
             const T hinf = T(1) / sq(T(1) + cexp((Vv + T(71.55)) / T(7.43)));
             T ah, bh, aj, bj;
             if (Vv < T(-40)) {
@@ -152,17 +152,17 @@ struct TNNP {
                 aj = T(0);
                 bj = T(0.6) * cexp(T(0.057) * Vv) / (T(1) + cexp(T(-0.1) * (Vv + T(32))));
             }
-            y[h] = rl(y[h], hinf, T(1) / (ah + bh), dt);
-            y[j] = rl(y[j], hinf, T(1) / (aj + bj), dt);
+            out.gate(h, hinf, T(1) / (ah + bh));
+            out.gate(j, hinf, T(1) / (aj + bj));
             const T dinf = T(1) / (T(1) + cexp((T(-5) - Vv) / T(7.5)));
             const T ad = T(1.4) / (T(1) + cexp((T(-35) - Vv) / T(13))) + T(0.25);
             const T bd = T(1.4) / (T(1) + cexp((Vv + T(5)) / T(5)));
             const T gd = T(1) / (T(1) + cexp((T(50) - Vv) / T(20)));
-            y[d] = rl(y[d], dinf, ad * bd + gd, dt);
+            out.gate(d, dinf, ad * bd + gd);
             const T finf = T(1) / (T(1) + cexp((Vv + T(20)) / T(7)));
             const T tf = T(1125) * cexp(-sq(Vv + T(27)) / T(240)) + T(80) +
                          T(165) / (T(1) + cexp((T(25) - Vv) / T(10)));
-            y[f] = rl(y[f], finf, tf, dt);
+            out.gate(f, finf, tf);
             T sinf, ts;
             if (tissue == 1) {
                 sinf = T(1) / (T(1) + cexp((Vv + T(20)) / T(5)));
@@ -171,30 +171,49 @@ struct TNNP {
                 sinf = T(1.1) / (T(1) + cexp((Vv + T(28)) / T(6)));
                 ts = T(1000) * cexp(-sq(Vv + T(67)) / T(1000)) + T(8);
             }
-            y[s] = rl(y[s], sinf, ts, dt);
+            out.gate(s, sinf, ts);
             const T rinf = T(1) / (T(1) + cexp((T(20) - Vv) / T(6)));
             const T tr = T(9.5) * cexp(-sq(Vv + T(40)) / T(1800)) + T(0.8);
-            y[r] = rl(y[r], rinf, tr, dt);
+            out.gate(r, rinf, tr);
             const T xr1inf = T(1) / (T(1) + cexp((T(-26) - Vv) / T(7)));
             const T txr1 = (T(450) / (T(1) + cexp((T(-45) - Vv) / T(10)))) *
                            (T(6) / (T(1) + cexp((Vv + T(30)) / T(11.5))));
-            y[Xr1] = rl(y[Xr1], xr1inf, txr1, dt);
+            out.gate(Xr1, xr1inf, txr1);
             const T xr2inf = T(1) / (T(1) + cexp((Vv + T(88)) / T(24)));
             const T txr2 = (T(3) / (T(1) + cexp((T(-60) - Vv) / T(20)))) *
                            (T(1.12) / (T(1) + cexp((Vv - T(60)) / T(20))));
-            y[Xr2] = rl(y[Xr2], xr2inf, txr2, dt);
+            out.gate(Xr2, xr2inf, txr2);
             const T xsinf = T(1) / (T(1) + cexp((T(-5) - Vv) / T(14)));
             const T txs = (T(1100) / csqrt(T(1) + cexp((T(-10) - Vv) / T(6)))) *
                           (T(1) / (T(1) + cexp((Vv - T(60)) / T(20))));
-            y[Xs] = rl(y[Xs], xsinf, txs, dt);
+            out.gate(Xs, xsinf, txs);
         }
-        y[fCa] = rl(y[fCa], fca_t, p[taufCa], dt);
-        y[g] = rl(y[g], g_t, p[taug], dt);
-        y[Nai] += dt * dNai;
-        y[Ki] += dt * dKi;
-        y[Cai] += dt * dCai;
-        y[CaSR] += dt * dCasr;
-        if (solve_v) y[V] = Vv + dt * (-Iion - istim);
+        out.gate(fCa, fca_t, p[taufCa]);
+        out.gate(g, g_t, p[taug]);
+        out.rate(Nai, dNai);
+        out.rate(Ki, dKi);
+        out.rate(Cai, dCai);
+        out.rate(CaSR, dCasr);
+        if (solve_v) out.rate(V, -Iion - istim);
+    }
+
+    // one RL/FE step of dt in place
+    template <class T>
+    CG_HD static void step(T* y, T dt, const T* p, int tissue, const T* gs, bool solve_v,
+                            T istim, const VTerms<T>& q)
+    {
+        RlStep<T> out{y, dt};
+        rates(y, out, p, tissue, gs, solve_v, istim, q);
+    }
+
+    // dy/dt at y (dV/dt = 0 unless V is integrated with the other states)
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int tissue, const T* gs, bool solve_v,
+                             T istim, const VTerms<T>& q)
+    {
+        Deriv<T> out{y, d};
+        d[V] = T(0);
+        rates(y, out, p, tissue, gs, solve_v, istim, q);
     }
 };
 

@@ -5,6 +5,9 @@
 //                does with its final computeVariables call (BuenoOrovio.C:134-152,
 //                TNNP.C:146-174, ORd.C:134-150, AlievPanfilov.C:134-152).
 //   cell_step:   single-cell step with V integrated inside (singleCellSolver.C:186-192).
+//   vt / deriv:  terms that depend on V only, and dy/dt given them (the derivatives()
+//                an ODE solver calls); with V held by the PDE the V terms of a step are
+//                computed once and reused by every stage.
 #pragma once
 #include "model_bo.hpp"
 #include "model_tnnp.hpp"
@@ -23,43 +26,59 @@ template <class M> struct Ops;
 
 template <> struct Ops<BuenoOrovio> {
     template <class T>
-    CG_DEV static T tissue_step(T u, T* y, T dt, const T* p, int, const T*)
+    CG_HD static T tissue_step(T u, T* y, T dt, const T* p, int, const T*)
     {
         y[0] = u;
         BuenoOrovio::gates(y, dt, p);
         return BuenoOrovio::jion(u, y, p);
     }
-    template <class T> CG_DEV static void cell_step(T* y, T dt, T istim, const T* p, int, const T*)
+    template <class T> CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int, const T*)
     {
         BuenoOrovio::cell_step(y, dt, istim, p);
     }
-    template <class T> CG_DEV static T iion(const T* y, const T* p, int, const T*)
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T*)
     {
         return BuenoOrovio::jion(y[0], y, p);
+    }
+    struct None {};
+    template <class T> CG_HD static None vt(T, const T*) { return None{}; }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int, const T*, bool solve_v, T istim,
+                             None)
+    {
+        BuenoOrovio::deriv(y, d, p, solve_v, istim);
     }
 };
 
 template <> struct Ops<AlievPanfilov> {
     template <class T>
-    CG_DEV static T tissue_step(T u, T* y, T dt, const T* p, int, const T*)
+    CG_HD static T tissue_step(T u, T* y, T dt, const T* p, int, const T*)
     {
         y[0] = u;
         AlievPanfilov::gates(y, dt, p);
         return AlievPanfilov::jion(u, y, p);
     }
-    template <class T> CG_DEV static void cell_step(T* y, T dt, T istim, const T* p, int, const T*)
+    template <class T> CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int, const T*)
     {
         AlievPanfilov::cell_step(y, dt, istim, p);
     }
-    template <class T> CG_DEV static T iion(const T* y, const T* p, int, const T*)
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T*)
     {
         return AlievPanfilov::jion(y[0], y, p);
+    }
+    struct None {};
+    template <class T> CG_HD static None vt(T, const T*) { return None{}; }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int, const T*, bool solve_v, T istim,
+                             None)
+    {
+        AlievPanfilov::deriv(y, d, p, solve_v, istim);
     }
 };
 
 template <> struct Ops<TNNP> {
     template <class T>
-    CG_DEV static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
     {
         y[0] = v;
         const auto q = TNNP::vterms(v, p);
@@ -67,21 +86,28 @@ template <> struct Ops<TNNP> {
         return TNNP::currents(v, y, p, q, gs).total();
     }
     template <class T>
-    CG_DEV static void cell_step(T* y, T dt, T istim, const T* p, int tissue, const T* gs)
+    CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int tissue, const T* gs)
     {
         const auto q = TNNP::vterms(y[0], p);
         TNNP::step(y, dt, p, tissue, gs, true, istim, q);
     }
-    template <class T> CG_DEV static T iion(const T* y, const T* p, int, const T* gs)
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T* gs)
     {
         const auto q = TNNP::vterms(y[0], p);
         return TNNP::currents(y[0], y, p, q, gs).total();
+    }
+    template <class T> CG_HD static TNNP::VTerms<T> vt(T v, const T* p) { return TNNP::vterms(v, p); }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int tissue, const T* gs, bool solve_v,
+                             T istim, const TNNP::VTerms<T>& q)
+    {
+        TNNP::deriv(y, d, p, tissue, gs, solve_v, istim, q);
     }
 };
 
 template <> struct Ops<ORd> {
     template <class T>
-    CG_DEV static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
     {
         y[0] = v;
         const auto q = ORd::vterms(v);
@@ -89,15 +115,22 @@ template <> struct Ops<ORd> {
         return ORd::currents(v, y, p, q, gs).total();
     }
     template <class T>
-    CG_DEV static void cell_step(T* y, T dt, T istim, const T* p, int tissue, const T* gs)
+    CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int tissue, const T* gs)
     {
         const auto q = ORd::vterms(y[0]);
         ORd::step(y, dt, p, tissue, gs, true, istim, q);
     }
-    template <class T> CG_DEV static T iion(const T* y, const T* p, int, const T* gs)
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T* gs)
     {
         const auto q = ORd::vterms(y[0]);
         return ORd::currents(y[0], y, p, q, gs).total();
+    }
+    template <class T> CG_HD static ORd::VTerms<T> vt(T v, const T*) { return ORd::vterms(v); }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int tissue, const T* gs, bool solve_v,
+                             T istim, const ORd::VTerms<T>& q)
+    {
+        ORd::deriv(y, d, p, tissue, gs, solve_v, istim, q);
     }
 };
 

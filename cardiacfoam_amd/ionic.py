@@ -86,6 +86,12 @@ class IonicModel:
         prec = precision or dict_.get("precision", "fp64")
         m = IonicModel(key, dict_["tissue"], nIntegrationPoints, solveVmWithinODESolver, prec)
         m.initialDeltaT = initialDeltaT
+        solver = dict_.get("solver", "RushLarsen")
+        # ODESolver::New keys (ionicModel.H:233-242); OpenFOAM's defaults when absent.
+        # The kept per-cell step starts at initialDeltaT (ionicModel.C:38).
+        m.set_integrator(solver, float(dict_.get("absTol", 1e-15)),
+                         float(dict_.get("relTol", 1e-4)), int(dict_.get("maxSteps", 10000)),
+                         float(initialDeltaT))
         stim = dict_.get("singleCellStimulus")
         if stim:
             m.set_stimulus_protocol(stim)
@@ -109,6 +115,20 @@ class IonicModel:
         if len(Vm) != self.n or len(Im) != self.n:
             raise ValueError("field size %d/%d != nIntegrationPoints %d" % (len(Vm), len(Im), self.n))
         check(lib().cg_ionic_step(self.h, float(stepStartTime), float(deltaT), pv, pi, fv))
+
+    ODE_SOLVERS = {"RushLarsen": _capi.ODE_RUSH_LARSEN, "RKF45": _capi.ODE_RKF45}
+
+    def set_integrator(self, solver="RKF45", absTol=1e-15, relTol=1e-4, maxSteps=10000,
+                       initial_step=0.0):
+        """Per-cell ODE integrator: "RKF45" (OpenFOAM's adaptive RKF45, the reference's
+        `solver RKF45`) or "RushLarsen".  initial_step (model time units) seeds the kept
+        per-cell trial steps; 0 seeds them with the first step's deltaT in seconds."""
+        if solver not in self.ODE_SOLVERS:
+            raise FatalIOError("Unknown ODESolver type %s\nValid ODESolver types on the device: %s"
+                               % (solver, sorted(self.ODE_SOLVERS)))
+        check(lib().cg_ionic_set_integrator(self.h, self.ODE_SOLVERS[solver], float(absTol),
+                                            float(relTol), int(maxSteps), float(initial_step)))
+        self.solver = solver
 
     def run_cells(self, t0, dt, n_steps):
         check(lib().cg_ionic_run_cells(self.h, float(t0), float(dt), int(n_steps)))

@@ -35,7 +35,8 @@ enum {
     CG_ERR_NONFINITE = 6,  /* NaN/Inf detected in a checked output */
     CG_ERR_NOT_CONVERGED = 7, /* linear solver hit max iterations */
     CG_ERR_STIMULUS = 8,   /* malformed stimulus (stimulusIO.C:123-177) */
-    CG_ERR_STATE = 9       /* call not valid in the handle's configuration */
+    CG_ERR_STATE = 9,      /* call not valid in the handle's configuration */
+    CG_ERR_ODE = 10        /* ODE solver failure: step-size underflow or maxSteps exceeded */
 };
 
 /* model ids: the ionicModel runtime-selection keys */
@@ -47,6 +48,8 @@ enum { CG_TISSUE_EPI = 1, CG_TISSUE_MCELL = 2, CG_TISSUE_ENDO = 3, CG_TISSUE_MYO
 enum { CG_FP64 = 0, CG_MIXED = 1 };
 /* flags */
 enum { CG_DEVICE_PTRS = 1 };
+/* per-cell ODE integrators of the ionic step */
+enum { CG_ODE_RUSH_LARSEN = 0, CG_ODE_RKF45 = 1 };
 
 const char* cg_last_error(void);
 /* library version string */
@@ -94,15 +97,34 @@ typedef struct {
 } cg_stimulus;
 int cg_ionic_set_stimulus(cg_ionic* h, const cg_stimulus* s);
 
+/* Selects the per-cell ODE integrator; replaces the `solver`, `absTol`, `relTol`,
+ * `maxSteps` keys that ionicModel::odeSolver() hands to ODESolver::New
+ * (ionicModel.H:233-248) and the per-cell step list step_ (ionicModel.H:244-248,
+ * ionicModel.C:38).
+ *   CG_ODE_RUSH_LARSEN: gates exponentially with frozen (xinf, tau), the rest forward
+ *     Euler, one step per deltaT (batchedKernelExecution.H:563-653); tolerances unused.
+ *   CG_ODE_RKF45: OpenFOAM's adaptive RKF45 over [tStart, tEnd] per cell with the error
+ *     norm max |err| / (abs_tol + rel_tol max(|y0|, |y|)); each cell keeps its last trial
+ *     step between calls and starts from min(step, deltaT) (BuenoOrovio.C:124-132).
+ *     initial_step (model time units) seeds the kept steps; <= 0 seeds them with the
+ *     first call's deltaT in seconds, as the reference's initialDeltaT does.  OpenFOAM's
+ *     defaults when the keys are absent: abs_tol 1e-15, rel_tol 1e-4, max_steps 10000.
+ * A failing cell (step-size underflow, more than max_steps sub-steps) makes the next
+ * synchronising call return CG_ERR_ODE.  Batched protocols (cg_batch_run) use
+ * Rush-Larsen only. */
+int cg_ionic_set_integrator(cg_ionic* h, int kind, double abs_tol, double rel_tol,
+                            int64_t max_steps, double initial_step);
+
 /* Replaces ionicModel::solveODE(stepStartTime, deltaT, Vm, Im) (ionicModel.H:250-256)
  * for a tissue handle: Vm [n] in V, Im [n] out in V/s.  Pointers are host memory unless
  * flags has CG_DEVICE_PTRS.  Synchronous. */
 int cg_ionic_step(cg_ionic* h, double t0_s, double dt_s, const double* Vm, double* Im,
                   int flags);
 
-/* Single-cell handles: n_steps Rush-Larsen steps of dt_s from t0_s with Vm inside the
- * ODE (SingleCellSolver::evolve, singleCellSolver.C:179-229), all cells in one launch
- * with the states kept in registers.  Asynchronous; cg_ionic_get_states synchronises. */
+/* Single-cell handles: n_steps steps of dt_s from t0_s with Vm inside the ODE
+ * (SingleCellSolver::evolve, singleCellSolver.C:179-229), all cells in one launch with
+ * the states kept in registers; RKF45 evaluates the stimulus at every stage time.
+ * Asynchronous; cg_ionic_get_states synchronises. */
 int cg_ionic_run_cells(cg_ionic* h, double t0_s, double dt_s, int n_steps);
 
 /* ---------------------------------------------------- batched single cells
