@@ -457,6 +457,82 @@ def run_adjoint_c4(cf, torch, shape=(64, 64, 32), h=2e-4, steps=5000, n_probes=2
             "full_horizon_one_direction": {"adjoint": float(g @ d0), "difference_quotients": spread}}
 
 
+def run_reference_semantics(cf, wname, w, sigma, cpu_steps, with_cpu, ord_steps=20):
+    """The reference's default integrator on the device.
+    (1) NiedererEtAl2012 regression case as the reference runs it: 100x15x35 hex of
+        0.2 mm, BuenoOrovio epi, explicit, backward ddt, dt from maxCo 0.1, RKF45 with
+        OpenFOAM's default tolerances (absTol 1e-15, relTol 1e-4), 40 ms; probe activation
+        times checked against NiedererEtAl2012.reference; the CPU FP64 oracle runs the same
+        case with the same integrator for a bounded number of steps.
+    (2) The 2.1 M-cell ORd implicit workload with RKF45 (absTol 1e-6, relTol 1e-4) in place
+        of Rush-Larsen, FP64 and mixed, ord_steps steps from rest after a 2-step start."""
+    from tests.golden import cases
+    c = cases.NIEDERER
+    nx, ny, nz = c["shape"]
+    b = c["box"]
+    box = {"stimulusLocationMin": b["min"], "stimulusLocationMax": b["max"],
+           "stimulusDuration": b["duration"], "stimulusIntensity": b["intensity"],
+           "stimulusStartTime": 0.0}
+    coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": c["chi"],
+              "cm": c["cm"], "solutionAlgorithm": "explicit", "conductivity": c["sigma9"],
+              "externalStimulus": box, "solver": "RKF45", "maxSteps": 1000000000}
+    mesh = cf.HexMesh(nx, ny, nz, c["h"])
+    warm = cf.Monodomain(mesh, coeffs, max_co=c["max_co"], ddt="backward")
+    warm.step(5)
+    warm.sync()
+    dom = cf.Monodomain(mesh, coeffs, max_co=c["max_co"], ddt="backward")
+    steps = 0
+    while steps * dom.dt < c["end_time"] - 0.5 * dom.dt:
+        steps += 1
+    t0 = time.perf_counter()
+    dom.step(steps)
+    dom.sync()
+    el = time.perf_counter() - t0
+    ta = dom.activationTime.reshape(nz, ny, nx)
+    errs, ok = [], True
+    for p in c["probes"]:
+        i, j, k = p["cell"]
+        e = float(abs(ta[k, j, i] - p["expected"]))
+        errs.append(e)
+        ok = ok and e <= p.get("tol_here", p["tol"])
+    n = nx * ny * nz
+    out = {"niederer_rkf45": {"cells": n, "steps": steps, "dt_s": dom.dt, "gpu_s": el,
+                              "cell_steps_per_s": n * steps / el, "probes_within_golden": ok,
+                              "probe_abs_err_max_s": max(errs)}}
+    if with_cpu:
+        import oracle
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        T = oracle.OracleTissue(nx, ny, nz, c["h"], np.asarray(c["sigma9"]), oracle.BO,
+                                oracle.EPI, dom.dt, c["chi"], c["cm"], integrator=1, ddt=1,
+                                abs_tol=1e-15, rel_tol=1e-4, nthreads=cores)
+        bx = oracle.make_boxes(b["min"], b["max"], [0.0], [b["duration"]], [b["intensity"]])
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            T.step(bx)
+        cel = time.perf_counter() - t0
+        out["niederer_rkf45"]["cpu"] = {
+            "value": n * cpu_steps / cel, "unit": "cell-steps/s", "cores": cores,
+            "kind": "port", "sample": "CPU FP64 oracle, RKF45, first %d of %d steps (includes "
+                                      "the stimulated upstroke), %.1f s" % (cpu_steps, steps, cel)}
+    ord_res = {}
+    cw = coeffs_for(w, sigma)
+    cw.update({"solver": "RKF45", "absTol": 1e-6, "relTol": 1e-4, "maxSteps": 100000})
+    m2 = cf.HexMesh(*w["shape"], w["h"])
+    for prec in ("fp64", "mixed"):
+        d = cf.Monodomain(m2, cw, dt=w["dt"], precision=prec, tol=w["tol"])
+        d.step(2)
+        d.sync()
+        t0 = time.perf_counter()
+        d.step(ord_steps)
+        d.sync()
+        e2 = time.perf_counter() - t0
+        ord_res[prec] = {"ms_per_step": e2 / ord_steps * 1e3,
+                         "cell_steps_per_s": int(np.prod(w["shape"])) * ord_steps / e2}
+        del d
+    out[wname + "_rkf45"] = dict(steps=ord_steps, **ord_res)
+    return out
+
+
 def cpu_baseline(w, sigma, sample_steps):
     """The CPU FP64 oracle (OpenMP over all host cores) on the same workload: the full
     mesh for `sample_steps` steps after a 2-step start, timed as cell-steps/s."""
@@ -581,6 +657,8 @@ def main():
     c1 = run_batch_c1(cf) if not args.no_batch else None
     c2 = run_batch_c2(cf) if not args.no_batch else None
     c4 = run_adjoint_c4(cf, torch) if not args.no_batch else None
+    ref_sem = run_reference_semantics(cf, args.workload, w, sigma, 100, not args.no_cpu) \
+        if w["model"] == "ORd" and not args.no_batch else None
 
     cpu = None
     if not args.no_cpu:
@@ -614,6 +692,7 @@ def main():
         "batch_c1": c1,
         "batch_c2": c2,
         "adjoint_c4": c4,
+        "rkf45": ref_sem,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
