@@ -15,6 +15,7 @@ ERRORS = {1: "ARG", 2: "MODEL", 3: "TISSUE", 4: "HIP", 5: "NOMEM", 6: "NONFINITE
 FP64, MIXED = 0, 1
 DEVICE_PTRS = 1
 ODE_RUSH_LARSEN, ODE_RKF45 = 0, 1
+NORM_L2, NORM_OPENFOAM = 0, 1
 EXPLICIT, IMPLICIT = 0, 1
 DDT_EULER, DDT_BACKWARD = 0, 1
 FIELD_VM, FIELD_IION, FIELD_ACTIVATION, FIELD_VM_OLD = 0, 1, 2, 3
@@ -62,6 +63,9 @@ def lib():
     L.cg_ionic_set_constant.argtypes = [vp, C.c_int, C.c_double]
     L.cg_ionic_set_conductance_scale.argtypes = [vp, vp]
     L.cg_ionic_set_stimulus.argtypes = [vp, C.POINTER(Stimulus)]
+    L.cg_tissue_set_solver_controls.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                                C.c_int, C.c_double]
+    L.cg_tissue_solver_report.argtypes = [vp, C.POINTER(C.c_int), dp, dp]
     L.cg_ionic_set_integrator.argtypes = [vp, C.c_int, C.c_double, C.c_double, ip64,
                                           C.c_double]
     L.cg_ionic_step.argtypes = [vp, C.c_double, C.c_double, vp, vp, C.c_int]

@@ -1131,6 +1131,40 @@ int cg_tissue_solver_stats(const cg_tissue* t, int64_t* iters, double* res)
     return CG_OK;
 }
 
+int cg_tissue_set_solver_controls(cg_tissue* t, int norm, double tolerance, double rel_tol,
+                                  int max_iter, int n_outer_correctors,
+                                  double residual_control)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
+    if (norm != CG_NORM_L2 && norm != CG_NORM_OPENFOAM)
+        return fail(CG_ERR_ARG, "unknown residual norm %d", norm);
+    if (!(tolerance >= 0.0) || !(rel_tol >= 0.0) || max_iter < 1 || !(residual_control >= 0.0))
+        return fail(CG_ERR_ARG, "bad solver controls");
+    if (norm == CG_NORM_L2 && !(tolerance > 0.0))
+        return fail(CG_ERR_ARG, "the relative L2 norm needs a positive tolerance");
+    HIPCHK(hipStreamSynchronize(t->ion->stream));
+    cg::SolveControls c;
+    c.norm = norm;
+    c.rel_tol = rel_tol;
+    c.n_outer = n_outer_correctors;
+    c.residual_control = residual_control;
+    const int rc = cg::implicit_set_controls(t->implicit, c);
+    if (rc == 5) return fail(CG_ERR_NOMEM, "allocation for the outer correctors failed");
+    if (rc) return fail(CG_ERR_ARG, "bad solver controls");
+    t->cfg.tol = tolerance;
+    t->cfg.max_iter = max_iter;
+    return CG_OK;
+}
+
+int cg_tissue_solver_report(const cg_tissue* t, int* solves, double* init_res, double* final_res)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
+    cg::implicit_report(t->implicit, solves, init_res, final_res);
+    return CG_OK;
+}
+
 // ------------------------------------------------------------------ adjoint
 struct cg_adjoint {
     cg_tissue* t;

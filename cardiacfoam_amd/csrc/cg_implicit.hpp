@@ -27,6 +27,19 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
                   int max_iter, Profiler* prof, bool ionic_queued, const NextIonic* next,
                   bool extrapolate, std::string* err);
 void implicit_stats(const ImplicitSolver* S, int64_t* iters, double* res);
+// fvSolution controls beyond tolerance and maxIter: norm 0 = relative L2, 1 = OpenFOAM's
+// normalised residual (with rel_tol); PIMPLE nOuterCorrectors (n_outer <= 1: one solve)
+// and the initial-residual tolerance residual_control of the outer loop
+struct SolveControls {
+    int norm = 0;
+    double rel_tol = 0.0;
+    int n_outer = 1;
+    double residual_control = 0.0;
+};
+int implicit_set_controls(ImplicitSolver* S, const SolveControls& c);
+// last step: linear solves (outer correctors), initial residual of the last corrector and
+// final residual of the last solve, both in the controls' norm
+void implicit_report(const ImplicitSolver* S, int* solves, double* init_res, double* final_res);
 // y = A x, A = coefft chiCm/dt I + orthogonal Laplacian matrix; prec 0: double, 1: float
 // (device pointers).  Returns 0, 9 when the fp32 operator was not built, 4 on HIP error.
 int implicit_apply(ImplicitSolver* S, int prec, double coefft, const void* x, void* y,

@@ -46,7 +46,10 @@ enum { P_G = 0,      // (r,u), parity sets 0 and 1
        P_D = 4,      // (w,u)
        P_RR0 = 5,    // (r,r) of the fp64 residual
        P_BB = 6,     // (b,b)
-       P_NSLOT = 7 };
+       P_L1 = 7,     // sum |r| (OpenFOAM norm), parity sets 7 and 8
+       P_NF = 9,     // normalisation factor terms of the solve's first residual
+       P_XS = 10,    // sum of the initial guess (its mean is OpenFOAM's xRef)
+       P_NSLOT = 11 };
 
 enum { S_RR, S_BB, S_R0, S_NSCAL };
 struct DevState {
@@ -58,6 +61,9 @@ struct DevState {
     int iters;                // iterations of the current solve
     int pad;
     long long iters_sum;      // iterations of all solves, added by each residual pass
+    double nf;                // OpenFOAM normalisation factor of the current solve
+    double l1_0, l1;          // sum |r| at the start and at the end of the current solve
+    double init_res;          // initial residual of the outer corrector (in its norm)
 };
 
 template <int NT, int NV> CG_DEV void block_sum(double (&v)[NV])
@@ -211,23 +217,25 @@ __global__ __launch_bounds__(BS) void k_invdiag(HexGeom g, HexOp<T> A, T* __rest
 //   r = b - A x;  rt = TP(r), ut = M^-1 rt.
 //   partials: (rt,ut) -> G set 0, (rt,rt) -> RR set 0, (r,r) -> RR0, (b,b) -> BB.
 //   Block 0 adds the iterations of the previous solve to the running sum.
-template <class TP, class LD, bool CORR>
+//   OF: also sum |r| and the normalisation terms |A x - mass xref| + |b - mass xref|
+//   (lduMatrix::solver::normFactor; the row sums of A are the mass coefficient).
+template <class TP, class LD, bool CORR, bool OF>
 CG_DEV void residual_range(const HexGeom& g, const HexOp<double>& A, const TP* __restrict__ invd_t,
                            const LD& X, const double* __restrict__ xin, double* __restrict__ xout,
-                           double* __restrict__ b, const double* __restrict__ od,
+                           const double* bsrc, double* bout, const double* __restrict__ od,
                            unsigned od_mask, TP* __restrict__ rt, TP* __restrict__ ut,
-                           double (&v)[4])
+                           double xref, double (&v)[6])
 {
+    const double mref = A.mass * xref;
     RANGE_LOOP(g)
     {
         const IJK q = ijk_of((uint32_t)c, g);
-        double bc = b[c];
-        if (CORR) {
-            bc += hex_tensor_correction(xin, od, n, c, q.i, q.j, q.k, g, od_mask);
-            b[c] = bc;
-        }
+        double bc = bsrc[c];
+        if (CORR) bc += hex_tensor_correction(xin, od, n, c, q.i, q.j, q.k, g, od_mask);
+        if (bout) bout[c] = bc;
         if (xout) xout[c] = X(c);
-        const double r = bc - apply_row(A, X, c, q, g);
+        const double ax = apply_row(A, X, c, q, g);
+        const double r = bc - ax;
         const TP rr = TP(r);
         const TP u = invd_t[c] * rr;
         rt[c] = rr;
@@ -236,48 +244,69 @@ CG_DEV void residual_range(const HexGeom& g, const HexOp<double>& A, const TP* _
         v[1] += double(rr) * double(rr);
         v[2] += r * r;
         v[3] += bc * bc;
+        if (OF) {
+            v[4] += fabs(r);
+            v[5] += fabs(ax - mref) + fabs(bc - mref);
+        }
     }
 }
 
-template <class TP>
+// ---- sum of the initial guess x (OpenFOAM's xRef is its mean)
+template <class LD>
+__global__ __launch_bounds__(RBS) void k_mean(HexGeom g, LD X, double* __restrict__ part)
+{
+    double v[1] = {0.0};
+    RANGE_LOOP(g) v[0] += X(c);
+    block_sum<RBS, 1>(v);
+    if (threadIdx.x == 0) part[P_XS * RG + bid] = v[0];
+}
+
+//   bsrc: right-hand side read; bout: where b plus the correction goes (null: nowhere,
+//   may equal bsrc).  OF: partial sum |r| -> L1 set 0 and, with of_nf (first residual of a
+//   solve), the normalisation terms -> NF with xref = mean from the k_mean partials.
+template <class TP, bool OF>
 __global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
                                                   const TP* __restrict__ invd_t,
                                                   const double* __restrict__ xin,
                                                   const double* __restrict__ xprev,
                                                   const float* __restrict__ e,
                                                   double* __restrict__ xout,
-                                                  double* __restrict__ b,
+                                                  const double* bsrc, double* bout,
                                                   const double* __restrict__ od, unsigned od_mask,
                                                   TP* __restrict__ rt, TP* __restrict__ ut,
                                                   double* __restrict__ part, DevState* S,
-                                                  long long gen)
+                                                  long long gen, int of_nf)
 {
     const bool add_e = e != nullptr && S->outer_gen != gen - 1;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double xref = 0.0;
+    if (OF && of_nf) {
+        double xs[1];
+        sum_slots<1>(part, {P_XS}, xs);
+        xref = xs[0] / double((int64_t)g.nx * g.ny * g.nz);
+    }
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const PlainLoad<double> X0{xin};
     const ExtrapLoad XE{xin, xprev};
-    if (od && xprev)
-        residual_range<TP, ExtrapLoad, true>(g, A, invd_t, XE, xin, xout, b, od, od_mask, rt,
-                                             ut, v);
-    else if (od)
-        residual_range<TP, PlainLoad<double>, true>(g, A, invd_t, X0, xin, xout, b, od, od_mask,
-                                                    rt, ut, v);
-    else if (xprev)
-        residual_range<TP, ExtrapLoad, false>(g, A, invd_t, XE, xin, xout, b, od, od_mask, rt,
-                                              ut, v);
-    else if (add_e)
-        residual_range<TP, CorrectedLoad, false>(g, A, invd_t, CorrectedLoad{xin, e}, xin, xout,
-                                                 b, od, od_mask, rt, ut, v);
-    else
-        residual_range<TP, PlainLoad<double>, false>(g, A, invd_t, X0, xin, xout, b, od,
-                                                     od_mask, rt, ut, v);
-    block_sum<RBS, 4>(v);
+#define CG_RESIDUAL(LD, XL, CORR)                                                          \
+    residual_range<TP, LD, CORR, OF>(g, A, invd_t, XL, xin, xout, bsrc, bout, od, od_mask, \
+                                     rt, ut, xref, v)
+    if (od && xprev) CG_RESIDUAL(ExtrapLoad, XE, true);
+    else if (od) CG_RESIDUAL(PlainLoad<double>, X0, true);
+    else if (xprev) CG_RESIDUAL(ExtrapLoad, XE, false);
+    else if (add_e) CG_RESIDUAL(CorrectedLoad, (CorrectedLoad{xin, e}), false);
+    else CG_RESIDUAL(PlainLoad<double>, X0, false);
+#undef CG_RESIDUAL
+    block_sum<RBS, 6>(v);
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         part[(P_G + 0) * RG + bid] = v[0];
         part[(P_RR + 0) * RG + bid] = v[1];
         part[P_RR0 * RG + bid] = v[2];
         part[P_BB * RG + bid] = v[3];
+        if (OF) {
+            part[(P_L1 + 0) * RG + bid] = v[4];
+            if (of_nf) part[P_NF * RG + bid] = v[5];
+        }
         if (blockIdx.x == 0) {
             S->iters_sum += S->iters;
             S->iters = 0;
@@ -290,26 +319,57 @@ __global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
 // or (b,b), or when (r,r) <= abs2 (b,b) (an inner solve that already meets the step
 // tolerance); at k == 0 with outer_tol2 > 0 also when the fp64 residual already meets
 // (r0,r0) <= outer_tol2 (b,b) (the refinement is finished).
-template <class T, bool QUAD>
+//
+// NORM 0: the tests above in relative L2.
+// NORM 1 (fp64 CG in OpenFOAM's norm): converged when sum|r| < of_tol nf, or
+//   of_rel > 1e-20 and sum|r| < of_rel sum|r_0| (SolverPerformance::checkConvergence).
+// NORM 2 (mixed, OpenFOAM's norm): the k == 0 outer test is sum|r| < of_tol nf on the fp64
+//   residual; the inner test stays relative L2.
+// record_init (k == 0 of an outer corrector's first solve): S->init_res = the initial
+// residual, sum|r|/nf (NORM 1, 2) or ||r||/||b|| (NORM 0).
+template <class T, bool QUAD, int NORM>
 __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __restrict__ u,
                                               T* __restrict__ w, double* __restrict__ part,
                                               DevState* S, long long gen, int k, double tol2,
                                               int relative_to_r0, double abs2,
-                                              double outer_tol2, int max_iter)
+                                              double outer_tol2, int max_iter, double of_tol,
+                                              double of_rel, int record_init)
 {
     if (S->done_gen == gen) return;
-    double v3[3];
-    sum_slots<3>(part, {P_RR + (k & 1), P_RR0, P_BB}, v3);
-    const double rr = v3[0], rr0 = v3[1], bb = v3[2];
-    const bool outer = k == 0 && outer_tol2 > 0.0 && rr0 <= outer_tol2 * bb;
-    const bool conv = rr <= tol2 * (relative_to_r0 ? rr0 : bb) || rr <= abs2 * bb;
+    double rr, rr0, bb, l1 = 0.0, nf = 1.0;
+    if (NORM == 1 || (NORM == 2 && k == 0)) {
+        double v5[5];
+        sum_slots<5>(part, {P_RR + (k & 1), P_RR0, P_BB, P_L1 + (k & 1), P_NF}, v5);
+        rr = v5[0]; rr0 = v5[1]; bb = v5[2]; l1 = v5[3]; nf = v5[4] + 1e-20;
+    } else {
+        double v3[3];
+        sum_slots<3>(part, {P_RR + (k & 1), P_RR0, P_BB}, v3);
+        rr = v3[0]; rr0 = v3[1]; bb = v3[2];
+    }
+    bool outer, conv;
+    if (NORM == 1) {
+        const double l10 = k == 0 ? l1 : S->l1_0;
+        outer = false;
+        conv = l1 < of_tol * nf || (of_rel > 1e-20 && l1 < of_rel * l10);
+    } else {
+        outer = k == 0 && outer_tol2 > 0.0 &&
+                (NORM == 2 ? l1 < of_tol * nf : rr0 <= outer_tol2 * bb);
+        conv = rr <= tol2 * (relative_to_r0 ? rr0 : bb) || rr <= abs2 * bb;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (k == 0) {
             S->s[S_R0] = rr0;
             S->s[S_BB] = bb;
+            if (NORM != 0) {
+                S->nf = nf;
+                S->l1_0 = l1;
+            }
+            if (record_init)
+                S->init_res = NORM != 0 ? l1 / nf : sqrt(rr0 / (bb > 0.0 ? bb : 1.0));
         }
         if (outer || conv || k >= max_iter) {
             S->s[S_RR] = rr;
+            S->l1 = l1;
             S->iters = k;
             if (outer) S->outer_gen = gen;
             if (outer || (conv && !relative_to_r0)) S->conv_gen = gen;
@@ -347,7 +407,8 @@ __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __
 //   alpha = gamma_k / (delta_k - beta gamma_k / alpha_{k-1})   (gamma/delta at k = 0);
 //   p = u + beta p, s = w + beta s, x += alpha p (x = alpha p at k = 0 when x_from_zero),
 //   r -= alpha s, u = M^-1 r; partials (r,u), (r,r) of iteration k+1.
-template <class T, class TX, bool QUAD>
+//   L1: also the partial sum |r| of iteration k+1 (OpenFOAM norm, fp64 path).
+template <class T, class TX, bool QUAD, bool L1>
 __global__ __launch_bounds__(RBS) void k_cg_u(HexGeom g, const T* __restrict__ invd,
                                               TX* __restrict__ x, T* __restrict__ r,
                                               T* __restrict__ u, T* __restrict__ p,
@@ -375,7 +436,7 @@ __global__ __launch_bounds__(RBS) void k_cg_u(HexGeom g, const T* __restrict__ i
     const T a = T(alpha), bt = T(beta);
     const bool first = k == 0;
     const bool xz = first && x_from_zero;
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     if constexpr (QUAD) {
         const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
         RANGE_LOOP_Q(g)
@@ -394,6 +455,9 @@ __global__ __launch_bounds__(RBS) void k_cg_u(HexGeom g, const T* __restrict__ i
             st4(u + c0, un);
             v[0] += dot4(rn, un);
             v[1] += dot4(rn, rn);
+            if (L1)
+                v[2] += (fabs(double(rn.x)) + fabs(double(rn.y))) +
+                        (fabs(double(rn.z)) + fabs(double(rn.w)));
         }
     } else {
         RANGE_LOOP(g)
@@ -409,13 +473,21 @@ __global__ __launch_bounds__(RBS) void k_cg_u(HexGeom g, const T* __restrict__ i
             u[c] = un;
             v[0] += double(rn) * double(un);
             v[1] += double(rn) * double(rn);
+            if (L1) v[2] += fabs(double(rn));
         }
     }
-    block_sum<RBS, 2>(v);
+    if constexpr (L1) block_sum<RBS, 3>(v);
+    else {
+        double v2[2] = {v[0], v[1]};
+        block_sum<RBS, 2>(v2);
+        v[0] = v2[0];
+        v[1] = v2[1];
+    }
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) {
         part[(P_G + ((k + 1) & 1)) * RG + bid] = v[0];
         part[(P_RR + ((k + 1) & 1)) * RG + bid] = v[1];
+        if (L1) part[(P_L1 + ((k + 1) & 1)) * RG + bid] = v[2];
     }
 }
 
@@ -528,7 +600,28 @@ struct ImplicitSolver {
     double last_res = 0.0;
     int last_inner = 16;
     long long iters_sum_seen = 0;  // DevState::iters_sum at the previous read
+    SolveControls ctl;
+    double* bc = nullptr;    // b plus the correction of the current outer corrector (mixed)
+    int last_solves = 0;
+    double last_init = 0.0, last_final = 0.0;
 };
+
+int implicit_set_controls(ImplicitSolver* S, const SolveControls& c)
+{
+    if (c.norm != 0 && c.norm != 1) return 1;
+    S->ctl = c;
+    if (c.n_outer > 1 && S->prec == 1 && S->od && !S->bc &&
+        hipMalloc((void**)&S->bc, 8 * S->n) != hipSuccess)
+        return 5;
+    return 0;
+}
+
+void implicit_report(const ImplicitSolver* S, int* solves, double* init_res, double* final_res)
+{
+    if (solves) *solves = S->last_solves;
+    if (init_res) *init_res = S->last_init;
+    if (final_res) *final_res = S->last_final;
+}
 
 template <class T> static HexOp<T> op_of(const ImplicitSolver* S, double coefft);
 template <> HexOp<double> op_of<double>(const ImplicitSolver* S, double c)
@@ -613,7 +706,7 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
 void implicit_destroy(ImplicitSolver* S)
 {
     if (!S) return;
-    void* ptrs[] = {S->fx,     S->fy,     S->fz,     S->invd,     S->b,        S->xalt,
+    void* ptrs[] = {S->bc,     S->fx,     S->fy,     S->fz,     S->invd,     S->b,        S->xalt,
                     S->partials, S->S,    S->od,     S->fx32,     S->fy32,     S->fz32,
                     S->invd32, S->v32.r,  S->v32.u,  S->v32.p,    S->v32.s,    S->v32.w,
                     S->e32,    S->v64.r,  S->v64.u,  S->v64.p,    S->v64.s,    S->v64.w};
@@ -674,18 +767,28 @@ template <class T> struct CgRun {
     Profiler* prof;
     std::vector<CgLaunch>* log;
 
+    // norm of the convergence tests: 0 relative L2; 1 OpenFOAM's on the fp64 CG; 2
+    // OpenFOAM's on the outer test of the refinement (mixed)
+    int norm_kind() const { return S->ctl.norm == 0 ? 0 : (sizeof(T) == 8 ? 1 : 2); }
+
     void s_kernel(long long gen, int k, double tol2, int rel, double abs2, double outer_tol2,
-                  int max_iter)
+                  int max_iter, double of_tol = 0.0, int record_init = 0)
     {
         const int tk = prof ? prof->begin(K_SPMV) : -1;
-        if (S->quad)
-            hipLaunchKernelGGL((k_cg_s<T, true>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,
-                               (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel,
-                               abs2, outer_tol2, max_iter);
-        else
-            hipLaunchKernelGGL((k_cg_s<T, false>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,
-                               (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel,
-                               abs2, outer_tol2, max_iter);
+        const double of_rel = S->ctl.rel_tol;
+#define CG_S(Q, N)                                                                         \
+    hipLaunchKernelGGL((k_cg_s<T, Q, N>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,      \
+                       (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel, abs2,     \
+                       outer_tol2, max_iter, of_tol, of_rel, record_init)
+        switch (norm_kind() + (S->quad ? 3 : 0)) {
+        case 0: CG_S(false, 0); break;
+        case 1: CG_S(false, 1); break;
+        case 2: CG_S(false, 2); break;
+        case 3: CG_S(true, 0); break;
+        case 4: CG_S(true, 1); break;
+        default: CG_S(true, 2); break;
+        }
+#undef CG_S
         if (prof) {
             prof->end(tk);
             log->push_back({(size_t)tk, gen, k, true});
@@ -694,26 +797,47 @@ template <class T> struct CgRun {
     template <class TX> void u_kernel(long long gen, int k, TX* x, int x_from_zero)
     {
         const int tk = prof ? prof->begin(K_UPDATE) : -1;
-        if (S->quad)
-            hipLaunchKernelGGL((k_cg_u<T, TX, true>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
-                               A.invd, x, v.r, v.u, v.p, v.s, (const T*)v.w, S->partials, S->S,
-                               gen, k, x_from_zero);
-        else
-            hipLaunchKernelGGL((k_cg_u<T, TX, false>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
-                               A.invd, x, v.r, v.u, v.p, v.s, (const T*)v.w, S->partials, S->S,
-                               gen, k, x_from_zero);
+        const bool l1 = norm_kind() == 1;
+#define CG_U(Q, L)                                                                         \
+    hipLaunchKernelGGL((k_cg_u<T, TX, Q, L>), dim3(RG), dim3(RBS), 0, S->stream, S->g,     \
+                       A.invd, x, v.r, v.u, v.p, v.s, (const T*)v.w, S->partials, S->S,    \
+                       gen, k, x_from_zero)
+        if (S->quad) {
+            if (l1) CG_U(true, true);
+            else CG_U(true, false);
+        } else {
+            if (l1) CG_U(false, true);
+            else CG_U(false, false);
+        }
+#undef CG_U
         if (prof) {
             prof->end(tk);
             log->push_back({(size_t)tk, gen, k, false});
         }
     }
+    // the sum of the initial guess (xin, or 2 xin - xprev) for OpenFOAM's xRef
+    void mean(const double* xin, const double* xprev)
+    {
+        if (xprev)
+            hipLaunchKernelGGL((k_mean<ExtrapLoad>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
+                               ExtrapLoad{xin, xprev}, S->partials);
+        else
+            hipLaunchKernelGGL((k_mean<PlainLoad<double>>), dim3(RG), dim3(RBS), 0, S->stream,
+                               S->g, PlainLoad<double>{xin}, S->partials);
+    }
     void residual(int kid, long long gen, const double* xin, const double* xprev,
-                  const float* e, double* xout, const double* od, const HexOp<double>& A64)
+                  const float* e, double* xout, const double* od, const HexOp<double>& A64,
+                  const double* bsrc, double* bout, int of_nf)
     {
         const int tk = prof ? prof->begin(kid) : -1;
-        hipLaunchKernelGGL((k_residual<T>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A64,
-                           (const T*)A.invd, xin, xprev, e, xout, S->b, od, S->od_mask, v.r,
-                           v.u, S->partials, S->S, gen);
+        if (S->ctl.norm == 1)
+            hipLaunchKernelGGL((k_residual<T, true>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
+                               A64, (const T*)A.invd, xin, xprev, e, xout, bsrc, bout, od,
+                               S->od_mask, v.r, v.u, S->partials, S->S, gen, of_nf);
+        else
+            hipLaunchKernelGGL((k_residual<T, false>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
+                               A64, (const T*)A.invd, xin, xprev, e, xout, bsrc, bout, od,
+                               S->od_mask, v.r, v.u, S->partials, S->S, gen, 0);
         if (prof) prof->end(tk);
     }
 };
@@ -760,25 +884,30 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
     std::vector<CgLaunch> log;
     int used_total = 0;
     const long long* conv_gen = &S->S->conv_gen;
+    const bool of = S->ctl.norm == 1;
+    const int n_outer = S->ctl.n_outer > 1 ? S->ctl.n_outer : 1;
 
     // Before each read-back: the step's activation (x into V^{n+1}) and the next step's
-    // ionic kernel, both gated on "solve g met the tolerance", then the 64-byte state.  The
-    // host waits for the read-back only; the queued ionic kernel of the next step keeps the
-    // device busy meanwhile.  Gated launches of a pass that did not converge do nothing and
-    // are dropped from the timings.
-    auto finish_and_read = [&](long long g, const double* x, size_t* mark) -> bool {
+    // ionic kernel, both gated on "solve g met the tolerance", then the state read-back.
+    // The host waits for the read-back only; the queued ionic kernel of the next step keeps
+    // the device busy meanwhile.  Gated launches of a pass that did not converge do nothing
+    // and are dropped from the timings.  Outer correctors before the last one read back
+    // without the gated launches (the step is not finished).
+    auto finish_and_read = [&](long long g, const double* x, size_t* mark, bool last) -> bool {
         *mark = prof ? prof->mark() : 0;
-        int t = prof ? prof->begin(K_ACTIVATION) : -1;
-        hipLaunchKernelGGL(k_activation, dim3(grid), dim3(BS), 0, S->stream, S->g, Vn, x, Vnew,
-                           flag, tact, st.t_old, st.dt, conv_gen, g);
-        if (prof) prof->end(t);
+        if (last) {
+            int t = prof ? prof->begin(K_ACTIVATION) : -1;
+            hipLaunchKernelGGL(k_activation, dim3(grid), dim3(BS), 0, S->stream, S->g, Vn, x,
+                               Vnew, flag, tact, st.t_old, st.dt, conv_gen, g);
+            if (prof) prof->end(t);
+        }
         hipMemcpyAsync(S->hS, S->S, sizeof(DevState), hipMemcpyDeviceToHost, S->stream);
         hipEventRecord(S->read_done, S->stream);
-        if (next) {
+        if (last && next) {
             ExplicitStep sn = next->st;
             sn.gate = conv_gen;
             sn.gate_val = g;
-            t = prof ? prof->begin(K_IONIC) : -1;
+            int t = prof ? prof->begin(K_IONIC) : -1;
             launch_ionic_rhs(model, prec, tissue, S->g, states, Vnew, Vn, nullptr, Iion, S->b, sn,
                              P, S->stream);
             if (prof) prof->end(t);
@@ -787,90 +916,140 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
         return hipEventSynchronize(S->read_done) == hipSuccess;
     };
 
+    // PIMPLE outer correctors (monodomainSolver.C:139-150): each corrector re-evaluates the
+    // explicit tensor correction from the latest V and solves from it; after the first, a
+    // previous initial residual below residualControl makes the corrector the last one
+    double prev_init = 0.0;
+    int solves = 0;
     if (prec == 0) {
-        // one fp64 CG on the system, x0 = V^n copied into V^{n+1} by the first residual
+        // fp64 CG per corrector; x0 = V^n (copied into V^{n+1} by the first residual)
         CgRun<double> cg{S, A, S->v64, prof, &log};
-        const long long gen = ++S->gen;
-        cg.residual(K_RESIDUAL0, gen, Vn, nullptr, nullptr, Vnew, S->od, A);
-        cg.s_kernel(gen, 0, tol * tol, 0, 0.0, 0.0, max_iter);
-        int k = 0;
-        int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
-        for (;;) {
-            for (int q = 0; q < chunk; ++q, ++k) {
-                cg.u_kernel(gen, k, Vnew, 0);
-                cg.s_kernel(gen, k + 1, tol * tol, 0, 0.0, 0.0, max_iter);
-            }
-            size_t gated;
-            if (!finish_and_read(gen, Vnew, &gated)) { *err = "CG launch or read-back failed"; return 4; }
-            const DevState& h = *S->hS;
-            if (h.done_gen == gen) {
-                drop_idle(prof, log, gen, h.iters);
-                const bool ok = h.conv_gen == gen;
-                if (prof) {
-                    if (!ok) prof->invalidate_range(gated, prof->mark());
-                    prof->resolve(false);
+        for (int outer = 1;; ++outer) {
+            const bool last = n_outer == 1 || outer >= n_outer ||
+                              (outer > 1 && prev_init < S->ctl.residual_control);
+            const long long gen = ++S->gen;
+            const double* xin = outer == 1 ? Vn : Vnew;
+            if (of) cg.mean(xin, nullptr);
+            cg.residual(K_RESIDUAL0, gen, xin, nullptr, nullptr, outer == 1 ? Vnew : nullptr,
+                        S->od, A, S->b, nullptr, 1);
+            cg.s_kernel(gen, 0, tol * tol, 0, 0.0, 0.0, max_iter, tol, 1);
+            int k = 0;
+            int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
+            for (;;) {
+                for (int q = 0; q < chunk; ++q, ++k) {
+                    cg.u_kernel(gen, k, Vnew, 0);
+                    cg.s_kernel(gen, k + 1, tol * tol, 0, 0.0, 0.0, max_iter, tol, 0);
                 }
-                used_total = h.iters;
-                S->last_inner = h.iters;
-                const double bb = h.s[S_BB] > 0 ? h.s[S_BB] : 1.0;
-                S->last_res = std::sqrt(h.s[S_RR] / bb);
-                if (!ok) {
-                    *err = "PCG did not converge within max_iter";
-                    return 7;
+                size_t gated;
+                if (!finish_and_read(gen, Vnew, &gated, last)) {
+                    *err = "CG launch or read-back failed";
+                    return 4;
                 }
-                break;
+                const DevState& h = *S->hS;
+                if (h.done_gen == gen) {
+                    drop_idle(prof, log, gen, h.iters);
+                    const bool ok = h.conv_gen == gen;
+                    if (prof) {
+                        if (!ok && last) prof->invalidate_range(gated, prof->mark());
+                        prof->resolve(false);
+                    }
+                    used_total += h.iters;
+                    S->last_inner = h.iters > 0 ? h.iters : S->last_inner;
+                    const double bb = h.s[S_BB] > 0 ? h.s[S_BB] : 1.0;
+                    S->last_res = of ? h.l1 / h.nf : std::sqrt(h.s[S_RR] / bb);
+                    prev_init = h.init_res;
+                    if (!ok) {
+                        *err = "PCG did not converge within max_iter";
+                        return 7;
+                    }
+                    break;
+                }
+                if (prof) prof->invalidate_range(gated, prof->mark());
+                chunk = 4;  // not finished: continue the same solve
             }
-            if (prof) prof->invalidate_range(gated, prof->mark());
-            chunk = 4;  // not finished: continue the same solve
+            log.clear();
+            ++solves;
+            if (last) break;
         }
     } else {
-        // mixed: refinement passes; x lives in V^n (pass 0), then alternates Vnew / xalt
+        // mixed: refinement passes per corrector; x starts in V^n or, extrapolated, in xalt,
+        // then alternates Vnew / xalt.  An inner solve stops at 1e-4 relative or (relative
+        // L2 only) as soon as its residual meets half the step tolerance.
         CgRun<float> cg{S, op_of<float>(S, st.c), S->v32, prof, &log};
-        long long gen = ++S->gen;
-        // x0 = 2 V^n - V^{n-1} (V^n on the first step); an inner solve stops at 1e-4
-        // relative or as soon as its residual meets half the step tolerance
-        const double* xprev = extrapolate ? Vm1 : nullptr;
-        const double abs2 = 0.25 * tol * tol;
-        // x0 goes to xalt, so the first refinement lands in V^{n+1} itself
-        cg.residual(K_RESIDUAL0, gen, Vn, xprev, nullptr, xprev ? S->xalt : nullptr, S->od, A);
-        cg.s_kernel(gen, 0, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter);
-        const double* cur = xprev ? S->xalt : Vn;
-        for (int pass = 0;; ++pass) {
-            const int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
-            for (int k = 0; k < chunk; ++k) {
-                cg.u_kernel(gen, k, S->e32, 1);
-                cg.s_kernel(gen, k + 1, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter);
+        const double abs2 = of ? 0.0 : 0.25 * tol * tol;
+        // b plus the correction lives in b itself unless later correctors need b again
+        double* bcur = S->od ? (n_outer > 1 ? S->bc : S->b) : nullptr;
+        const double* bref = bcur ? bcur : S->b;
+        const double* cur = Vn;
+        for (int outer = 1;; ++outer) {
+            const bool last = n_outer == 1 || outer >= n_outer ||
+                              (outer > 1 && prev_init < S->ctl.residual_control);
+            long long gen = ++S->gen;
+            if (outer == 1) {
+                // x0 = 2 V^n - V^{n-1} (V^n on the first step), written to xalt so the first
+                // refinement lands in V^{n+1} itself
+                const double* xprev = extrapolate ? Vm1 : nullptr;
+                if (of) cg.mean(Vn, xprev);
+                cg.residual(K_RESIDUAL0, gen, Vn, xprev, nullptr, xprev ? S->xalt : nullptr,
+                            S->od, A, S->b, bcur, 1);
+                cur = xprev ? S->xalt : Vn;
+            } else {
+                if (of) cg.mean(cur, nullptr);
+                cg.residual(K_RESIDUAL0, gen, cur, nullptr, nullptr, nullptr, S->od, A, S->b,
+                            bcur, 1);
             }
-            // x' = x + e into the other buffer, fp64 residual, start of the next solve
-            double* nxt = cur == Vnew ? S->xalt : Vnew;
-            const long long ngen = ++S->gen;
-            cg.residual(K_RESIDUAL, ngen, cur, nullptr, S->e32, nxt, nullptr, A);
-            cg.s_kernel(ngen, 0, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter);
-            cur = nxt;
-            size_t gated;
-            if (!finish_and_read(ngen, cur, &gated)) { *err = "CG launch or read-back failed"; return 4; }
-            const DevState& h = *S->hS;
-            // iterations of solve gen, folded into the running sum by the residual pass
-            const int run = (int)(h.iters_sum - S->iters_sum_seen);
-            S->iters_sum_seen = h.iters_sum;
-            drop_idle(prof, log, gen, run);
-            const bool finished = h.outer_gen == ngen;
-            if (finished) drop_idle(prof, log, ngen, 0);
-            else if (prof) prof->invalidate_range(gated, prof->mark());
-            if (prof) prof->resolve(false);
-            log.clear();
-            used_total += run;
-            if (run > 0) S->last_inner = run;
-            const double bb = h.s[S_BB] > 0 ? h.s[S_BB] : 1.0;
-            S->last_res = std::sqrt(h.s[S_R0] / bb);
-            if (finished) break;
-            if (used_total >= max_iter || pass >= 50) {
-                *err = "mixed-precision refinement did not converge";
-                return 7;
+            cg.s_kernel(gen, 0, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter, tol, 1);
+            bool init_read = false;
+            for (int pass = 0;; ++pass) {
+                const int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
+                for (int k = 0; k < chunk; ++k) {
+                    cg.u_kernel(gen, k, S->e32, 1);
+                    cg.s_kernel(gen, k + 1, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter);
+                }
+                // x' = x + e into the other buffer, fp64 residual, start of the next solve
+                double* nxt = cur == Vnew ? S->xalt : Vnew;
+                const long long ngen = ++S->gen;
+                cg.residual(K_RESIDUAL, ngen, cur, nullptr, S->e32, nxt, nullptr, A, bref,
+                            nullptr, 0);
+                cg.s_kernel(ngen, 0, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter, tol, 0);
+                cur = nxt;
+                size_t gated;
+                if (!finish_and_read(ngen, cur, &gated, last)) {
+                    *err = "CG launch or read-back failed";
+                    return 4;
+                }
+                const DevState& h = *S->hS;
+                if (!init_read) {  // recorded by the corrector's first solve
+                    prev_init = h.init_res;
+                    init_read = true;
+                }
+                // iterations of solve gen, folded into the running sum by the residual pass
+                const int run = (int)(h.iters_sum - S->iters_sum_seen);
+                S->iters_sum_seen = h.iters_sum;
+                drop_idle(prof, log, gen, run);
+                const bool finished = h.outer_gen == ngen;
+                if (finished) drop_idle(prof, log, ngen, 0);
+                else if (prof && last) prof->invalidate_range(gated, prof->mark());
+                if (prof) prof->resolve(false);
+                log.clear();
+                used_total += run;
+                if (run > 0) S->last_inner = run;
+                const double bb = h.s[S_BB] > 0 ? h.s[S_BB] : 1.0;
+                S->last_res = of ? h.l1_0 / h.nf : std::sqrt(h.s[S_R0] / bb);
+                if (finished) break;
+                if (used_total >= max_iter * n_outer || pass >= 50) {
+                    *err = "mixed-precision refinement did not converge";
+                    return 7;
+                }
+                gen = ngen;
             }
-            gen = ngen;
+            ++solves;
+            if (last) break;
         }
     }
+    S->last_solves = solves;
+    S->last_init = prev_init;
+    S->last_final = S->last_res;
     S->total_iters += used_total;
     e = hipGetLastError();
     if (e != hipSuccess) { *err = hipGetErrorString(e); return 4; }

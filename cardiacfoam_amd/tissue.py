@@ -209,3 +209,41 @@ class Monodomain:
         r = C.c_double()
         check(lib().cg_tissue_solver_stats(self.t, C.byref(it), C.byref(r)))
         return it.value, r.value
+
+    def set_solver_controls(self, norm="OpenFOAM", tolerance=1e-11, relTol=0.0, maxIter=1000,
+                            nOuterCorrectors=1, residualControl=0.0):
+        """fvSolution controls of the Vm solve: norm "OpenFOAM" (lduMatrix normalised
+        residual) or "L2" (relative); PIMPLE nOuterCorrectors and the residualControl
+        tolerance of the outer loop."""
+        k = {"L2": _capi.NORM_L2, "OpenFOAM": _capi.NORM_OPENFOAM}[norm]
+        check(lib().cg_tissue_set_solver_controls(self.t, k, float(tolerance), float(relTol),
+                                                  int(maxIter), int(nOuterCorrectors),
+                                                  float(residualControl)))
+
+    def set_fv_solution(self, fv_solution, field="Vm"):
+        """Reads the Vm solver entry and the PIMPLE dictionary of a parsed fvSolution
+        (solvers keys may be regular expressions such as "Vm|VmFinal"), as fvMatrix::solve
+        and pimpleControl look them up; the residual norm is OpenFOAM's."""
+        import re
+        entry = None
+        for key, d in fv_solution.get("solvers", {}).items():
+            if re.fullmatch(key.strip('"'), field):
+                entry = d
+                break
+        if entry is None:
+            raise FatalIOError("keyword %s is undefined in solvers" % field)
+        pimple = fv_solution.get("PIMPLE", {})
+        rc = 0.0
+        for key, d in pimple.get("residualControl", {}).items():
+            if re.fullmatch(key.strip('"'), field):
+                rc = float(d.get("tolerance", 0.0))
+        self.set_solver_controls("OpenFOAM", float(entry.get("tolerance", 1e-6)),
+                                 float(entry.get("relTol", 0.0)), int(entry.get("maxIter", 1000)),
+                                 int(pimple.get("nOuterCorrectors", 1)), rc)
+
+    def solver_report(self):
+        """last step: (linear solves, initial residual of the last corrector, final residual)"""
+        n = C.c_int()
+        r0, r1 = C.c_double(), C.c_double()
+        check(lib().cg_tissue_solver_report(self.t, C.byref(n), C.byref(r0), C.byref(r1)))
+        return n.value, r0.value, r1.value

@@ -233,6 +233,28 @@ int cg_tissue_time(const cg_tissue* t, double* time_s, int64_t* steps);
 /* implicit path: total inner iterations and last relative residual */
 int cg_tissue_solver_stats(const cg_tissue* t, int64_t* iterations, double* last_residual);
 
+/* Implicit path: the fvSolution controls of the reference's Vm solve
+ * (tutorials/regressionTests/NiedererEtAl2012/system/fvSolution, solvers{Vm} and PIMPLE):
+ *   norm CG_NORM_L2: ||b - A x||_2 <= tolerance ||b||_2 (rel_tol unused);
+ *   norm CG_NORM_OPENFOAM: OpenFOAM's lduMatrix residual sum|b - A x| / normFactor,
+ *     normFactor = sum |A x0 - sumA xRef| + |b - sumA xRef| (xRef = mean of the initial
+ *     guess x0, sumA = row sums), converged below tolerance or below rel_tol x initial;
+ *   n_outer_correctors, residual_control: the PIMPLE loop around solveDiffusionImplicit
+ *     (monodomainSolver.C:139-150).  Every corrector re-evaluates the explicit tensor
+ *     correction from the latest Vm and solves again; after the first, an initial residual
+ *     of the previous corrector below residual_control makes the current one the last.
+ *     n_outer_correctors <= 1: one solve per step.
+ * The preconditioner is Jacobi in both norms (the reference's DIC is a sequential sweep).
+ * Replaces the solver-dictionary lookups of fvMatrix::solve and pimpleControl. */
+enum { CG_NORM_L2 = 0, CG_NORM_OPENFOAM = 1 };
+int cg_tissue_set_solver_controls(cg_tissue* t, int norm, double tolerance, double rel_tol,
+                                  int max_iter, int n_outer_correctors,
+                                  double residual_control);
+/* last step of the implicit path: linear solves (outer correctors), initial residual of the
+ * last corrector and final residual of the last solve, both in the controls' norm */
+int cg_tissue_solver_report(const cg_tissue* t, int* solves, double* initial_residual,
+                            double* final_residual);
+
 /* Per-kernel HIP-event timing on the tissue's stream (the equivalent of the clockTime
  * phases staggeredElectrophysicsAdvanceScheme.C:54-74 stores in
  * electrophysicsAdvanceTimings).  Enabling resets the totals. */

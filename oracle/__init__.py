@@ -48,7 +48,9 @@ class TissueCfg(C.Structure):
                 ("n_sub", C.c_int), ("abs_tol", C.c_double), ("rel_tol", C.c_double),
                 ("ddt", C.c_int), ("implicit", C.c_int), ("pcg_tol", C.c_double),
                 ("pcg_max_iter", C.c_int), ("chi", C.c_double), ("cm", C.c_double),
-                ("dt", C.c_double), ("nthreads", C.c_int)]
+                ("dt", C.c_double), ("nthreads", C.c_int), ("norm", C.c_int),
+                ("pcg_rel_tol", C.c_double), ("precond", C.c_int), ("n_outer", C.c_int),
+                ("residual_control", C.c_double)]
 
 
 class Tissue(C.Structure):
@@ -59,7 +61,8 @@ class Tissue(C.Structure):
                 ("V00", C.POINTER(C.c_double)), ("Iion", C.POINTER(C.c_double)),
                 ("src", C.POINTER(C.c_double)), ("tact", C.POINTER(C.c_double)),
                 ("flag", C.POINTER(C.c_uint8)), ("work", C.POINTER(C.c_double)),
-                ("t", C.c_double), ("step_index", C.c_long), ("pcg_iters_total", C.c_long)]
+                ("t", C.c_double), ("step_index", C.c_long), ("pcg_iters_total", C.c_long),
+                ("outer_total", C.c_long), ("last_init_res", C.c_double)]
 
 
 class Beat(C.Structure):
@@ -109,6 +112,8 @@ def lib():
         L.orc_assemble_implicit.argtypes = [C.POINTER(Mesh), _dp, C.c_double, C.c_double,
                                             C.c_double, _dp, _dp]
         L.orc_ldu_amul.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, _dp]
+        L.orc_pcg_of.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                 C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_pcg.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, _dp, C.c_double, C.c_int,
                               C.POINTER(C.c_double)]
         L.orc_box_stimulus.argtypes = [C.POINTER(Mesh), C.POINTER(Boxes), C.c_double, _dp]
@@ -260,6 +265,17 @@ class OracleMesh:
                            x, tol, max_iter, C.byref(res))
         return x, it, res.value
 
+    def pcg_of(self, diag, upper, b, x0, tol=1e-11, rel_tol=0.0, max_iter=1000, precond=1):
+        """OpenFOAM PCG (normalised residual; precond 0 diagonal, 1 DIC): returns
+        (x, iterations, initial residual, final residual)"""
+        x = np.array(x0, dtype=np.float64)
+        r0, r1 = C.c_double(), C.c_double()
+        it = lib().orc_pcg_of(C.byref(self.m), np.ascontiguousarray(diag, dtype=np.float64),
+                              np.ascontiguousarray(upper, dtype=np.float64),
+                              np.ascontiguousarray(b, dtype=np.float64), x, tol, rel_tol,
+                              max_iter, precond, C.byref(r0), C.byref(r1))
+        return x, it, r0.value, r1.value
+
     def explicit_dt(self, sigma9, chiCm, maxCo):
         return lib().orc_explicit_dt(C.byref(self.m), np.ascontiguousarray(sigma9, dtype=np.float64),
                                      chiCm, maxCo)
@@ -278,11 +294,16 @@ class OracleTissue:
 
     def __init__(self, nx, ny, nz, h, sigma9, model, tissue, dt, chi, cm, integrator=0,
                  n_sub=1, ddt=0, implicit=0, pcg_tol=1e-10, pcg_max_iter=20000,
-                 abs_tol=1e-6, rel_tol=1e-4, nthreads=8):
+                 abs_tol=1e-6, rel_tol=1e-4, nthreads=8, norm=0, pcg_rel_tol=0.0, precond=0,
+                 n_outer=1, residual_control=0.0):
+        """norm 0: ||r||2 <= pcg_tol ||b||2 with Jacobi PCG; norm 1: OpenFOAM's normalised
+        residual (pcg_tol, pcg_rel_tol, precond 0 diagonal / 1 DIC).  n_outer and
+        residual_control: PIMPLE nOuterCorrectors and the outer initial-residual tolerance."""
         hx, hy, hz = (h, h, h) if np.isscalar(h) else h
         self.T = Tissue()
         cfg = TissueCfg(model, tissue, integrator, n_sub, abs_tol, rel_tol, ddt, implicit,
-                        pcg_tol, pcg_max_iter, chi, cm, dt, nthreads)
+                        pcg_tol, pcg_max_iter, chi, cm, dt, nthreads, norm, pcg_rel_tol,
+                        precond, n_outer, residual_control)
         sig = np.ascontiguousarray(sigma9, dtype=np.float64)
         if sig.size == 9:
             sig = np.tile(sig.reshape(9), nx * ny * nz)
@@ -312,6 +333,11 @@ class OracleTissue:
     @property
     def V(self):
         return self._arr(self.T.V, self.n)
+
+    @property
+    def solver_counts(self):
+        """(PCG iterations, linear solves) over all steps, initial residual of the last solve"""
+        return self.T.pcg_iters_total, self.T.outer_total, self.T.last_init_res
 
     @property
     def Iion(self):

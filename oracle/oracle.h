@@ -169,6 +169,12 @@ void orc_activation(long n, const double* V, const double* Vold, uint8_t* flag,
 /* explicit dt = maxCo * min_f dx^2 / (n.sigma_f.n / chiCm), myocardiumDomain.C:532-550 */
 double orc_explicit_dt(const orc_mesh_t* m, const double* sigma, double chiCm, double maxCo);
 
+/* OpenFOAM PCG on the LDU system (diag, upper in face order); returns iterations,
+ * initial and final normalised residuals */
+int orc_pcg_of(const orc_mesh_t* m, const double* diag, const double* upper, const double* b,
+               double* x, double tol, double rel_tol, int max_iter, int precond,
+               double* init_res, double* final_res);
+
 typedef struct {
     int model, tissue;
     int integrator;          /* 0 = Rush-Larsen (n_sub substeps), 1 = RKF45 */
@@ -180,6 +186,15 @@ typedef struct {
     double chi, cm;
     double dt;               /* s */
     int nthreads;
+    /* implicit solver controls (fvSolution): norm 0 = ||r||2 <= pcg_tol ||b||2 (Jacobi
+     * PCG), 1 = OpenFOAM's normalised residual with pcg_rel_tol and precond (0 diagonal,
+     * 1 DIC); PIMPLE n_outer (nOuterCorrectors, <= 1: one solve) and residual_control
+     * (initial-residual tolerance of the outer loop) */
+    int norm;
+    double pcg_rel_tol;
+    int precond;
+    int n_outer;
+    double residual_control;
 } orc_tissue_cfg_t;
 
 typedef struct {
@@ -196,6 +211,8 @@ typedef struct {
     double t;                /* accumulated time (s) */
     long step_index;
     long pcg_iters_total;
+    long outer_total;        /* linear solves (outer correctors) over all steps */
+    double last_init_res;    /* initial residual of the last solve */
 } orc_tissue_t;
 
 int orc_tissue_create(orc_tissue_t* T, int nx, int ny, int nz, double hx, double hy,

@@ -259,6 +259,116 @@ int orc_pcg(const orc_mesh_t* m, const double* diag, const double* upper, const 
     return it;
 }
 
+/* ---------------------------------------------- OpenFOAM lduMatrix PCG semantics
+ * Restated from OpenFOAM (ESI v2312..v2512; third party, not in the reference tree),
+ * the solver the reference's fvSolution selects for Vm (PCG, tolerance, relTol, maxIter;
+ * tutorials/regressionTests/NiedererEtAl2012/system/fvSolution):
+ *  - normalisation factor of lduMatrix::solver::normFactor:
+ *      sum_i |(A psi)_i - sumA_i xRef| + |b_i - sumA_i xRef| + 1e-20,
+ *    xRef = mean(psi), sumA = row sums of A;
+ *  - residual = sum_i |b - A psi|_i / normFactor (initial and final);
+ *  - converged when residual < tolerance, or relTol > 1e-20 and residual < relTol x initial
+ *    (SolverPerformance::checkConvergence), tested before the first iteration;
+ *  - PCG::solve iteration: w = M^-1 r, (w,r), p = w (+ beta p), w = A p, (w,p), singular
+ *    when |(w,p)|/normFactor < 1e-300, psi += alpha p, r -= alpha w;
+ *  - preconditioner 0 = diagonal (Jacobi), 1 = DIC (incomplete Cholesky of the lduMatrix
+ *    in face order: rD[u] -= upper^2 / rD[l], then the forward and backward face sweeps). */
+static double of_norm_factor(const orc_mesh_t* m, const double* diag, const double* upper,
+                             const double* psi, const double* b, const double* Apsi,
+                             double* sumA)
+{
+    const long n = m->n_cells;
+    double xs = 0.0;
+    for (long i = 0; i < n; ++i) xs += psi[i];
+    const double xRef = xs / (double)n;
+    for (long i = 0; i < n; ++i) sumA[i] = diag[i];
+    for (long f = 0; f < m->n_faces; ++f) {
+        sumA[m->owner[f]] += upper[f];
+        sumA[m->neighbour[f]] += upper[f];
+    }
+    double nf = 0.0;
+    for (long i = 0; i < n; ++i)
+        nf += fabs(Apsi[i] - sumA[i] * xRef) + fabs(b[i] - sumA[i] * xRef);
+    return nf + 1e-20;
+}
+
+static void dic_rd(const orc_mesh_t* m, const double* diag, const double* upper, double* rD)
+{
+    const long n = m->n_cells;
+    for (long i = 0; i < n; ++i) rD[i] = diag[i];
+    for (long f = 0; f < m->n_faces; ++f)
+        rD[m->neighbour[f]] -= upper[f] * upper[f] / rD[m->owner[f]];
+    for (long i = 0; i < n; ++i) rD[i] = 1.0 / rD[i];
+}
+
+static void precondition(const orc_mesh_t* m, int kind, const double* diag,
+                         const double* upper, const double* rD, const double* r, double* w)
+{
+    const long n = m->n_cells;
+    if (kind == 0) {
+        for (long i = 0; i < n; ++i) w[i] = r[i] / diag[i];
+        return;
+    }
+    for (long i = 0; i < n; ++i) w[i] = rD[i] * r[i];
+    for (long f = 0; f < m->n_faces; ++f)
+        w[m->neighbour[f]] -= rD[m->neighbour[f]] * upper[f] * w[m->owner[f]];
+    for (long f = m->n_faces - 1; f >= 0; --f)
+        w[m->owner[f]] -= rD[m->owner[f]] * upper[f] * w[m->neighbour[f]];
+}
+
+int orc_pcg_of(const orc_mesh_t* m, const double* diag, const double* upper, const double* b,
+               double* x, double tol, double rel_tol, int max_iter, int precond,
+               double* init_res, double* final_res)
+{
+    const long n = m->n_cells;
+    double* wA = malloc(sizeof(double) * n);
+    double* rA = malloc(sizeof(double) * n);
+    double* pA = malloc(sizeof(double) * n);
+    double* rD = malloc(sizeof(double) * n);
+    orc_ldu_amul(m, diag, upper, x, wA);
+    for (long i = 0; i < n; ++i) rA[i] = b[i] - wA[i];
+    const double nf = of_norm_factor(m, diag, upper, x, b, wA, pA);
+    double l1 = 0.0;
+    for (long i = 0; i < n; ++i) l1 += fabs(rA[i]);
+    const double init = l1 / nf;
+    double fin = init;
+#define OF_CONVERGED(r) ((r) < tol || (rel_tol > 1e-20 && (r) < rel_tol * init))
+    int it = 0;
+    if (!OF_CONVERGED(fin)) {
+        if (precond == 1) dic_rd(m, diag, upper, rD);
+        double wArA = 1e300, wArAold;
+        do {
+            wArAold = wArA;
+            precondition(m, precond, diag, upper, rD, rA, wA);
+            wArA = 0.0;
+            for (long i = 0; i < n; ++i) wArA += wA[i] * rA[i];
+            if (it == 0) {
+                for (long i = 0; i < n; ++i) pA[i] = wA[i];
+            } else {
+                const double beta = wArA / wArAold;
+                for (long i = 0; i < n; ++i) pA[i] = wA[i] + beta * pA[i];
+            }
+            orc_ldu_amul(m, diag, upper, pA, wA);
+            double wApA = 0.0;
+            for (long i = 0; i < n; ++i) wApA += wA[i] * pA[i];
+            if (fabs(wApA) / nf < 1e-300) break;
+            const double alpha = wArA / wApA;
+            l1 = 0.0;
+            for (long i = 0; i < n; ++i) {
+                x[i] += alpha * pA[i];
+                rA[i] -= alpha * wA[i];
+                l1 += fabs(rA[i]);
+            }
+            fin = l1 / nf;
+        } while (++it < max_iter && !OF_CONVERGED(fin));
+    }
+#undef OF_CONVERGED
+    if (init_res) *init_res = init;
+    if (final_res) *final_res = fin;
+    free(wA); free(rA); free(pA); free(rD);
+    return it;
+}
+
 void orc_box_stimulus(const orc_mesh_t* m, const orc_boxes_t* b, double t0, double* src)
 {
     const int nx = m->nx, ny = m->ny;
@@ -369,17 +479,49 @@ int orc_tissue_step(orc_tissue_t* T, const orc_boxes_t* boxes)
             T->V[c] = (coefft0 * T->V0[c] - coefft00 * T->V00[c] +
                        dt / chiCm * (lap[c] - chiCm * T->Iion[c] + T->src[c])) / coefft;
     } else {
+        /* solveDiffusionImplicit inside `while (pimple.loop())` (monodomainSolver.C:139-150):
+         * every outer corrector re-assembles fvm::laplacian, whose explicit tensor
+         * correction comes from the latest V, and solves from the latest V.  The loop
+         * restates OpenFOAM's pimpleLoop: after the first corrector, the initial residual
+         * of the previous solve below residualControl makes the current corrector the
+         * final one; nOuterCorrectors caps the count (n_outer = 1: one solve). */
         double* diag = T->work + n;
-        double* rhs = T->work + 2 * n;
+        double* rhs0 = T->work + 2 * n;
+        double* rhs = T->work + 3 * n;
         double* upper = malloc(sizeof(double) * T->mesh.n_faces);
         orc_assemble_implicit(&T->mesh, T->sigma, chiCm, dt, coefft, diag, upper);
-        orc_correction_div(&T->mesh, T->sigma, T->V0, lap, k->nthreads);
         for (long c = 0; c < n; ++c)
-            rhs[c] = chiCm / dt * (coefft0 * T->V0[c] - coefft00 * T->V00[c]) + lap[c] -
-                     chiCm * T->Iion[c] + T->src[c];
-        double res;
-        int it = orc_pcg(&T->mesh, diag, upper, rhs, T->V, k->pcg_tol, k->pcg_max_iter, &res);
-        T->pcg_iters_total += it;
+            rhs0[c] = chiCm / dt * (coefft0 * T->V0[c] - coefft00 * T->V00[c]) -
+                      chiCm * T->Iion[c] + T->src[c];
+        const int n_outer = k->n_outer > 0 ? k->n_outer : 1;
+        double prev_init = 0.0;
+        for (int outer = 1;; ++outer) {
+            const int final_corr = n_outer == 1 || outer >= n_outer ||
+                                   (outer > 1 && prev_init < k->residual_control);
+            orc_correction_div(&T->mesh, T->sigma, T->V, lap, k->nthreads);
+            for (long c = 0; c < n; ++c) rhs[c] = rhs0[c] + lap[c];
+            double init, res;
+            int it;
+            if (k->norm == 1) {
+                it = orc_pcg_of(&T->mesh, diag, upper, rhs, T->V, k->pcg_tol, k->pcg_rel_tol,
+                                k->pcg_max_iter, k->precond, &init, &res);
+            } else {
+                /* relative L2 initial residual, then the plain PCG */
+                orc_ldu_amul(&T->mesh, diag, upper, T->V, lap);
+                double rr = 0.0, bb = 0.0;
+                for (long c = 0; c < n; ++c) {
+                    rr += (rhs[c] - lap[c]) * (rhs[c] - lap[c]);
+                    bb += rhs[c] * rhs[c];
+                }
+                init = sqrt(rr / (bb > 0.0 ? bb : 1.0));
+                it = orc_pcg(&T->mesh, diag, upper, rhs, T->V, k->pcg_tol, k->pcg_max_iter, &res);
+            }
+            T->pcg_iters_total += it;
+            T->outer_total++;
+            T->last_init_res = init;
+            prev_init = init;
+            if (final_corr) break;
+        }
         free(upper);
     }
     orc_activation(n, T->V, T->V0, T->flag, T->tact, T->t - dt, dt);

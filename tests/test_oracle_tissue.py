@@ -16,6 +16,8 @@ import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from tests.gpu_cases import rotating_fibre_sigma
+
 
 def _fd7(V, shape, h, s):
     nx, ny, nz = shape
@@ -134,3 +136,87 @@ def test_activation_interpolation(orc):
     assert tact[1] == 0.0 and flag[1] == 1          # not crossed
     assert tact[2] == 0.0 and flag[2] == 1          # 1e-16 is not > SMALL (1e-15)
     assert tact[3] == 0.0                           # already activated
+
+
+def _ldu_csr(m, diag, upper):
+    n = m.n
+    return sp.coo_matrix((np.concatenate([diag, upper, upper]),
+                          (np.concatenate([np.arange(n), m.owner, m.neighbour]),
+                           np.concatenate([np.arange(n), m.neighbour, m.owner]))),
+                         shape=(n, n)).tocsr()
+
+
+def test_dic_is_exact_cholesky_in_one_dimension(orc):
+    """On a chain of cells the lduMatrix is tridiagonal, incomplete Cholesky with zero
+    fill is the exact factorisation, and DIC-preconditioned CG converges in one
+    iteration; the diagonal preconditioner needs many."""
+    m = orc.OracleMesh(40, 1, 1, 2e-4, 2e-4, 2e-4)
+    sig = np.tile(np.array([0.17, 0, 0, 0, 0.019, 0, 0, 0, 0.019]), m.n)
+    diag, upper = m.assemble_implicit(sig, 1400.0, 2e-5)
+    b = np.random.default_rng(2).normal(size=m.n)
+    x, it, r0, r1 = m.pcg_of(diag, upper, b, np.zeros(m.n), tol=1e-14, precond=1)
+    assert it == 1 and r1 < 1e-14
+    np.testing.assert_allclose(x, spla.spsolve(_ldu_csr(m, diag, upper).tocsc(), b), rtol=1e-12)
+    _, it_j, _, _ = m.pcg_of(diag, upper, b, np.zeros(m.n), tol=1e-14, precond=0)
+    assert it_j > 5
+
+
+def test_openfoam_normalised_residual(orc):
+    """The normalised residual is invariant to scaling the system, zero at the exact
+    solution, independent of the preconditioner at the start, and DIC-PCG meets it."""
+    shape = (6, 5, 4)
+    m = orc.OracleMesh(*shape, 2e-4, 2e-4, 2e-4)
+    rng = np.random.default_rng(4)
+    sig = np.stack([_tensor(t) for t in rng.uniform(-1, 1, m.n)])
+    diag, upper = m.assemble_implicit(sig, 1400.0, 2e-5)
+    b = rng.normal(size=m.n) * 1e3 - 80.0
+    x0 = rng.normal(size=m.n) * 0.01 - 0.08
+    _, _, r0, _ = m.pcg_of(diag, upper, b, x0, tol=1.0, precond=1)
+    _, _, r0s, _ = m.pcg_of(7.5 * diag, 7.5 * upper, 7.5 * b, x0, tol=1.0, precond=0)
+    assert abs(r0 - r0s) <= 1e-13 * r0
+    xs = spla.spsolve(_ldu_csr(m, diag, upper).tocsc(), b)
+    _, it, rz, _ = m.pcg_of(diag, upper, b, xs, tol=1e-11, precond=1)
+    assert it == 0 and rz < 1e-13
+    for pc in (0, 1):
+        x, it, ri, rf = m.pcg_of(diag, upper, b, x0, tol=1e-14, precond=pc)
+        assert ri == r0 and rf < 1e-14 and it > 0
+        assert np.max(np.abs(x - xs)) <= 1e-9 * np.max(np.abs(xs))
+    # relTol: stops at relTol x initial
+    _, it_r, ri, rf = m.pcg_of(diag, upper, b, x0, tol=0.0, rel_tol=1e-3, precond=1)
+    assert rf < 1e-3 * ri and it_r > 0
+
+
+def test_pimple_outer_correctors_reach_implicit_correction(orc):
+    """With outer correctors iterated to convergence, one implicit step solves
+    (A - C) V = rhs0 where C is the tensor-correction operator (linear in V, assembled here
+    column by column from the oracle's correction_div): the explicit correction becomes
+    implicit.  One corrector leaves a difference of the size of C (V - V^n)."""
+    shape, h = (6, 5, 4), 2e-4
+    nx, ny, nz = shape
+    sig = rotating_fibre_sigma(*shape)
+    dt, chi, cm = 2e-5, 1.4e5, 0.01
+    box = orc.make_boxes([0, 0, 0], [6e-4, 6e-4, 6e-4], [0.0], [2e-3], [5e4])
+    runs = {}
+    for n_outer in (1, 400):
+        T = orc.OracleTissue(nx, ny, nz, h, sig, orc.BO, orc.EPI, dt, chi, cm, integrator=0,
+                             implicit=1, norm=1, pcg_tol=1e-15, pcg_max_iter=1000, precond=1,
+                             n_outer=n_outer, residual_control=1e-13)
+        T.step(box)
+        runs[n_outer] = (T.V, T.Iion, T.solver_counts)
+    m = orc.OracleMesh(nx, ny, nz, h, h, h)
+    n = m.n
+    diag, upper = m.assemble_implicit(sig, chi * cm, dt)
+    A = _ldu_csr(m, diag, upper).toarray()
+    Cm = np.column_stack([m.correction_div(sig, np.eye(n)[j]) for j in range(n)])
+    src = np.zeros(n)
+    lib = orc.lib()
+    lib.orc_box_stimulus(ctypes.byref(m.m), ctypes.byref(box), 0.0, src)
+    V0 = np.full(n, -0.084)
+    rhs0 = chi * cm / dt * V0 - chi * cm * runs[400][1] + src
+    V_exact = np.linalg.solve(A - Cm, rhs0)
+    V1, Vk = runs[1][0], runs[400][0]
+    e_k = np.max(np.abs(Vk - V_exact)) / np.max(np.abs(V_exact))
+    e_1 = np.max(np.abs(V1 - V_exact)) / np.max(np.abs(V_exact))
+    assert e_k < 1e-12, e_k
+    assert e_1 > 1e3 * e_k
+    assert runs[400][2][1] > 2 and runs[1][2][1] == 1
