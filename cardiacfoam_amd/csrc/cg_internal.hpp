@@ -88,7 +88,7 @@ struct ExplicitStep {
     /* ionic step + right-hand side of the implicit monodomain system */                   \
     hipError_t launch_ionic_rhs_##SFX(int model, int tissue, const HexGeom& g,             \
                                       void* states, const double* Vn, const double* Vm1,   \
-                                      const double* corr, double* Iion, double* b,         \
+                                      double* x0, double* Iion, double* b,                 \
                                       const ExplicitStep& st, const ModelParams& P,        \
                                       hipStream_t s);
 
@@ -169,11 +169,11 @@ inline hipError_t launch_explicit_hex(int model, int prec, int tissue, const Hex
 
 inline hipError_t launch_ionic_rhs(int model, int prec, int tissue, const HexGeom& g,
                                    void* states, const double* Vn, const double* Vm1,
-                                   const double* corr, double* Iion, double* b,
+                                   double* x0, double* Iion, double* b,
                                    const ExplicitStep& st, const ModelParams& P, hipStream_t s)
 {
     return (prec ? launch_ionic_rhs_fp32 : launch_ionic_rhs_fp64)(
-        model, tissue, g, states, Vn, Vm1, corr, Iion, b, st, P, s);
+        model, tissue, g, states, Vn, Vm1, x0, Iion, b, st, P, s);
 }
 
 } // namespace cg

@@ -197,15 +197,17 @@ __global__ __launch_bounds__(MBS) void k_explicit_hex(
 }
 
 // ------------------------------------------ ionic step + implicit right-hand side
-// rhs = chiCm/dt (c0 V^n - c00 V^{n-1}) + corr - chiCm Iion + source, where corr is the
-// explicit tensor correction of the Laplacian (null when sigma is diagonal).  With a gate
-// (s.gate) the launch may be queued before the previous step's solve is known to have
-// converged: it then does nothing unless the solver recorded the expected generation.
+// rhs = chiCm/dt (c0 V^n - c00 V^{n-1}) - chiCm Iion + source (the explicit tensor
+// correction is added by the solver's first residual); x0 (when not null) receives the
+// solver's extrapolated initial guess 2 V^n - V^{n-1}, riding on this kernel's pass over
+// V^n.  With a gate (s.gate) the launch may be queued before the previous step's solve is
+// known to have converged: it then does nothing unless the solver recorded the expected
+// generation.
 template <class M, class T, int INTEG>
 __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st,
                                                    const double* __restrict__ Vn,
                                                    const double* __restrict__ Vm1,
-                                                   const double* __restrict__ corr,
+                                                   double* __restrict__ x0,
                                                    double* __restrict__ Iion,
                                                    double* __restrict__ b, ExplicitStep s,
                                                    KParams<T> P, int tissue)
@@ -228,8 +230,8 @@ __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st
     const double src = box_source(cc.i, cc.j, cc.k, g, s.boxes);
     double rhs = s.chiCm / s.dt * (s.c0 * vn - s.c00 * (s.c00 != 0.0 ? Vm1[c] : 0.0)) +
                  src - s.chiCm * iion;
-    if (corr) rhs += corr[c];
     b[c] = rhs;
+    if (x0) x0[c] = 2.0 * vn - Vm1[c];
 }
 
 // ---------------------------------------------------------------- launchers
@@ -295,17 +297,17 @@ template <class T> struct ModelLaunch {
     }
     template <class M>
     static hipError_t ionic_rhs(int tissue, const HexGeom& g, void* states, const double* Vn,
-                                const double* Vm1, const double* corr, double* Iion,
+                                const double* Vm1, double* x0, double* Iion,
                                 double* b, const ExplicitStep& st, const ModelParams& P,
                                 hipStream_t s)
     {
         const int64_t n = (int64_t)g.nx * g.ny * g.nz;
         if (st.ode.rkf)
             hipLaunchKernelGGL((k_ionic_rhs<M, T, 1>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,
-                               (T*)states, Vn, Vm1, corr, Iion, b, st, kparams<T>(P), tissue);
+                               (T*)states, Vn, Vm1, x0, Iion, b, st, kparams<T>(P), tissue);
         else
             hipLaunchKernelGGL((k_ionic_rhs<M, T, 0>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,
-                               (T*)states, Vn, Vm1, corr, Iion, b, st, kparams<T>(P), tissue);
+                               (T*)states, Vn, Vm1, x0, Iion, b, st, kparams<T>(P), tissue);
         return hipGetLastError();
     }
 };
@@ -350,11 +352,11 @@ template <class T> struct ModelLaunch {
     }                                                                                      \
     hipError_t launch_ionic_rhs_##SFX(int model, int tissue, const HexGeom& g,             \
                                       void* states, const double* Vn, const double* Vm1,   \
-                                      const double* corr, double* Iion, double* b,         \
+                                      double* x0, double* Iion, double* b,                 \
                                       const ExplicitStep& st, const ModelParams& P,        \
                                       hipStream_t s)                                       \
     {                                                                                      \
-        CG_MODEL_SWITCH(T, ionic_rhs, tissue, g, states, Vn, Vm1, corr, Iion, b, st, P, s) \
+        CG_MODEL_SWITCH(T, ionic_rhs, tissue, g, states, Vn, Vm1, x0, Iion, b, st, P, s)   \
     }
 
 } // namespace cg

@@ -123,45 +123,93 @@ CG_DEV T apply_row(const HexOp<T>& A, const LD& x, int64_t c, const IJK& q, cons
     return A.mass * xc + fl;
 }
 
-template <class T> struct PlainLoad {
-    const T* __restrict__ x;
-    CG_DEV T operator()(int64_t c) const { return x[c]; }
-};
-struct ExtrapLoad {  // linear extrapolation 2 x - x_prev (initial guess of a step)
-    const double* __restrict__ x;
-    const double* __restrict__ xp;
-    CG_DEV double operator()(int64_t c) const { return 2.0 * x[c] - xp[c]; }
-};
-struct CorrectedLoad {  // fp64 x plus the fp32 correction e
-    const double* __restrict__ x;
-    const float* __restrict__ e;
-    CG_DEV double operator()(int64_t c) const { return x[c] + double(e[c]); }
-};
-
 // ---- four consecutive rows (c0..c0+3, one x-row segment; nx % 4 == 0) of A x with
 // 16/32-byte vector loads: 7 vector loads of x and face planes plus two x scalars per quad
 template <class T> using V4 = T __attribute__((ext_vector_type(4)));
 template <class T> CG_DEV V4<T> ld4(const T* p) { return *reinterpret_cast<const V4<T>*>(p); }
 
-template <class T>
-CG_DEV V4<T> apply_quad(const HexOp<T>& A, const T* __restrict__ x, int64_t c0, const IJK& q,
-                        const HexGeom& g)
+// x loaders: one value (operator()) or four consecutive ones from a 32-byte aligned c (q4)
+template <class T> struct PlainLoad {
+    const T* __restrict__ x;
+    CG_DEV T operator()(int64_t c) const { return x[c]; }
+    CG_DEV V4<T> q4(int64_t c) const { return ld4(x + c); }
+};
+struct ExtrapLoad {  // linear extrapolation 2 x - x_prev (initial guess of a step)
+    const double* __restrict__ x;
+    const double* __restrict__ xp;
+    CG_DEV double operator()(int64_t c) const { return 2.0 * x[c] - xp[c]; }
+    CG_DEV V4<double> q4(int64_t c) const { return 2.0 * ld4(x + c) - ld4(xp + c); }
+};
+struct CorrectedLoad {  // fp64 x plus the fp32 correction e
+    const double* __restrict__ x;
+    const float* __restrict__ e;
+    CG_DEV double operator()(int64_t c) const { return x[c] + double(e[c]); }
+    CG_DEV V4<double> q4(int64_t c) const
+    {
+        return ld4(x + c) + __builtin_convertvector(ld4(e + c), V4<double>);
+    }
+};
+
+template <class T, class LD>
+CG_DEV V4<T> apply_quad_ld(const HexOp<T>& A, const LD& x, int64_t c0, const IJK& q,
+                           const HexGeom& g)
 {
     const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
     const int64_t ym = q.j > 0 ? c0 - sy : c0, yp = q.j < g.ny - 1 ? c0 + sy : c0;
     const int64_t zm = q.k > 0 ? c0 - sz : c0, zp = q.k < g.nz - 1 ? c0 + sz : c0;
     const int64_t xl = q.i > 0 ? c0 - 1 : c0, xr = q.i + 4 < g.nx ? c0 + 4 : c0 + 3;
-    const V4<T> xc = ld4(x + c0);
-    const V4<T> xym = ld4(x + ym), xyp = ld4(x + yp), xzm = ld4(x + zm), xzp = ld4(x + zp);
+    const V4<T> xc = x.q4(c0);
+    const V4<T> xym = x.q4(ym), xyp = x.q4(yp), xzm = x.q4(zm), xzp = x.q4(zp);
     const V4<T> fx = ld4(A.fx + c0), fyp = ld4(A.fy + c0), fym = ld4(A.fy + ym);
     const V4<T> fzp = ld4(A.fz + c0), fzm = ld4(A.fz + zm);
-    const T x_l = x[xl], x_r = x[xr], fx_l = A.fx[xl];
+    const T x_l = x(xl), x_r = x(xr), fx_l = A.fx[xl];
     // x-direction neighbours of the four rows; fx of the +x wall face is 0
     const V4<T> xxm = {x_l, xc.x, xc.y, xc.z};
     const V4<T> xxp = {xc.y, xc.z, xc.w, x_r};
     const V4<T> fxm = {fx_l, fx.x, fx.y, fx.z};
     return A.mass * xc + (fx * (xc - xxp) + fxm * (xc - xxm)) + (fyp * (xc - xyp) + fym * (xc - xym)) +
            (fzp * (xc - xzp) + fzm * (xc - xzm));
+}
+
+template <class T>
+CG_DEV V4<T> apply_quad(const HexOp<T>& A, const T* __restrict__ x, int64_t c0, const IJK& q,
+                        const HexGeom& g)
+{
+    return apply_quad_ld(A, PlainLoad<T>{x}, c0, q, g);
+}
+
+// The explicit tensor correction of hex_tensor_correction for four consecutive cells when
+// only the xy conductivity is non-zero (fibres in the xy plane): the rows j-1, j, j+1 of V
+// and of sigma_xy as 32-byte loads plus their end values, the y-gradients of the cells and
+// their x-neighbours and the x-gradients of the cells and their y-neighbours taken from
+// those rows.  Same walls (clamped neighbours) and the same expression order per cell.
+CG_DEV V4<double> quad_correction_xy(const double* __restrict__ V,
+                                     const double* __restrict__ oxy, int64_t c0, const IJK& q,
+                                     const HexGeom& g)
+{
+    const int64_t sy = g.nx;
+    const int64_t rm = q.j > 0 ? c0 - sy : c0, rp = q.j < g.ny - 1 ? c0 + sy : c0;
+    const int64_t xl = q.i > 0 ? -1 : 0, xr = q.i + 4 < g.nx ? 4 : 3;
+    const V4<double> v0 = ld4(V + c0), vm = ld4(V + rm), vp = ld4(V + rp);
+    const double v0l = V[c0 + xl], v0r = V[c0 + xr], vml = V[rm + xl], vmr = V[rm + xr];
+    const double vpl = V[rp + xl], vpr = V[rp + xr];
+    const V4<double> o0 = ld4(oxy + c0), om = ld4(oxy + rm), op = ld4(oxy + rp);
+    const double o0l = oxy[c0 + xl], o0r = oxy[c0 + xr];
+    const double rhx = 0.5 / g.hx, rhy = 0.5 / g.hy;
+    // faces -x/+x (d = x, e = y): y-gradients at the cells and their x-neighbours
+    const V4<double> gy = (vp - vm) * rhy;
+    const double gyl = (vpl - vml) * rhy, gyr = (vpr - vmr) * rhy;
+    const V4<double> gyxm = {gyl, gy.x, gy.y, gy.z}, gyxp = {gy.y, gy.z, gy.w, gyr};
+    const V4<double> oxm = {o0l, o0.x, o0.y, o0.z}, oxp = {o0.y, o0.z, o0.w, o0r};
+    const V4<double> qx = (o0 + oxp) * (gy + gyxp) - (o0 + oxm) * (gy + gyxm);
+    // faces -y/+y (d = y, e = x): x-gradients at the cells and their y-neighbours
+    auto ddx = [&](const V4<double>& r, double l, double rr) {
+        const V4<double> up = {r.y, r.z, r.w, rr}, dn = {l, r.x, r.y, r.z};
+        return (up - dn) * rhx;
+    };
+    const V4<double> gx = ddx(v0, v0l, v0r), gxm = ddx(vm, vml, vmr), gxp = ddx(vp, vpl, vpr);
+    const V4<double> qy = (o0 + op) * (gx + gxp) - (o0 + om) * (gx + gxm);
+    return 0.25 * qx * (2.0 * rhx) + 0.25 * qy * (2.0 * rhy);
 }
 
 // one cell per lane (elementwise kernels)
@@ -251,6 +299,43 @@ CG_DEV void residual_range(const HexGeom& g, const HexOp<double>& A, const TP* _
     }
 }
 
+// the same on quads of cells (nx % 4 == 0), the correction limited to sigma_xy (CORRXY)
+template <class TP, class LD, bool CORRXY, bool OF>
+CG_DEV void residual_range_q(const HexGeom& g, const HexOp<double>& A,
+                             const TP* __restrict__ invd_t, const LD& X,
+                             const double* __restrict__ xin, double* __restrict__ xout,
+                             const double* bsrc, double* bout, const double* __restrict__ oxy,
+                             TP* __restrict__ rt, TP* __restrict__ ut, double xref,
+                             double (&v)[6], uint32_t bid)
+{
+    const double mref = A.mass * xref;
+    RANGE_LOOP_Q(g)
+    {
+        const int64_t c0 = qi * 4;
+        const IJK q = ijk_of((uint32_t)c0, g);
+        V4<double> bc = ld4(bsrc + c0);
+        if (CORRXY) bc += quad_correction_xy(xin, oxy, c0, q, g);
+        if (bout) st4(bout + c0, bc);
+        if (xout) st4(xout + c0, X.q4(c0));
+        const V4<double> ax = apply_quad_ld(A, X, c0, q, g);
+        const V4<double> r = bc - ax;
+        const V4<TP> rr = __builtin_convertvector(r, V4<TP>);
+        const V4<TP> u = ld4(invd_t + c0) * rr;
+        st4(rt + c0, rr);
+        st4(ut + c0, u);
+        v[0] += dot4(rr, u);
+        v[1] += dot4(rr, rr);
+        v[2] += dot4(r, r);
+        v[3] += dot4(bc, bc);
+        if (OF) {
+            v[4] += (fabs(r.x) + fabs(r.y)) + (fabs(r.z) + fabs(r.w));
+            const V4<double> t1 = ax - mref, t2 = bc - mref;
+            v[5] += ((fabs(t1.x) + fabs(t2.x)) + (fabs(t1.y) + fabs(t2.y))) +
+                    ((fabs(t1.z) + fabs(t2.z)) + (fabs(t1.w) + fabs(t2.w)));
+        }
+    }
+}
+
 // ---- sum of the initial guess x (OpenFOAM's xRef is its mean)
 template <class LD>
 __global__ __launch_bounds__(RBS) void k_mean(HexGeom g, LD X, double* __restrict__ part)
@@ -264,10 +349,12 @@ __global__ __launch_bounds__(RBS) void k_mean(HexGeom g, LD X, double* __restric
 //   bsrc: right-hand side read; bout: where b plus the correction goes (null: nowhere,
 //   may equal bsrc).  OF: partial sum |r| -> L1 set 0 and, with of_nf (first residual of a
 //   solve), the normalisation terms -> NF with xref = mean from the k_mean partials.
-template <class TP, bool OF>
+// QUAD (nx % 4 == 0, correction absent or sigma_xy only): four cells per lane.
+template <class TP, bool OF, bool QUAD>
 __global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
                                                   const TP* __restrict__ invd_t,
                                                   const double* __restrict__ xin,
+                                                  const double* __restrict__ vcor,
                                                   const double* __restrict__ xprev,
                                                   const float* __restrict__ e,
                                                   double* __restrict__ xout,
@@ -285,19 +372,32 @@ __global__ __launch_bounds__(RBS) void k_residual(HexGeom g, HexOp<double> A,
         xref = xs[0] / double((int64_t)g.nx * g.ny * g.nz);
     }
     double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double* vc = vcor ? vcor : xin;  // field of the tensor correction
     const PlainLoad<double> X0{xin};
     const ExtrapLoad XE{xin, xprev};
-#define CG_RESIDUAL(LD, XL, CORR)                                                          \
-    residual_range<TP, LD, CORR, OF>(g, A, invd_t, XL, xin, xout, bsrc, bout, od, od_mask, \
-                                     rt, ut, xref, v)
-    if (od && xprev) CG_RESIDUAL(ExtrapLoad, XE, true);
-    else if (od) CG_RESIDUAL(PlainLoad<double>, X0, true);
-    else if (xprev) CG_RESIDUAL(ExtrapLoad, XE, false);
-    else if (add_e) CG_RESIDUAL(CorrectedLoad, (CorrectedLoad{xin, e}), false);
-    else CG_RESIDUAL(PlainLoad<double>, X0, false);
-#undef CG_RESIDUAL
-    block_sum<RBS, 6>(v);
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    if constexpr (QUAD) {
+#define CG_RESIDUAL(LD, XL, CORR)                                                          \
+    residual_range_q<TP, LD, CORR, OF>(g, A, invd_t, XL, vc, xout, bsrc, bout, od, rt, ut,   \
+                                       xref, v, bid)
+        if (od && xprev) CG_RESIDUAL(ExtrapLoad, XE, true);
+        else if (od) CG_RESIDUAL(PlainLoad<double>, X0, true);
+        else if (xprev) CG_RESIDUAL(ExtrapLoad, XE, false);
+        else if (add_e) CG_RESIDUAL(CorrectedLoad, (CorrectedLoad{xin, e}), false);
+        else CG_RESIDUAL(PlainLoad<double>, X0, false);
+#undef CG_RESIDUAL
+    } else {
+#define CG_RESIDUAL(LD, XL, CORR)                                                          \
+    residual_range<TP, LD, CORR, OF>(g, A, invd_t, XL, vc, xout, bsrc, bout, od, od_mask,  \
+                                     rt, ut, xref, v)
+        if (od && xprev) CG_RESIDUAL(ExtrapLoad, XE, true);
+        else if (od) CG_RESIDUAL(PlainLoad<double>, X0, true);
+        else if (xprev) CG_RESIDUAL(ExtrapLoad, XE, false);
+        else if (add_e) CG_RESIDUAL(CorrectedLoad, (CorrectedLoad{xin, e}), false);
+        else CG_RESIDUAL(PlainLoad<double>, X0, false);
+#undef CG_RESIDUAL
+    }
+    block_sum<RBS, 6>(v);
     if (threadIdx.x == 0) {
         part[(P_G + 0) * RG + bid] = v[0];
         part[(P_RR + 0) * RG + bid] = v[1];
@@ -827,17 +927,24 @@ template <class T> struct CgRun {
     }
     void residual(int kid, long long gen, const double* xin, const double* xprev,
                   const float* e, double* xout, const double* od, const HexOp<double>& A64,
-                  const double* bsrc, double* bout, int of_nf)
+                  const double* bsrc, double* bout, int of_nf, const double* vcor = nullptr)
     {
         const int tk = prof ? prof->begin(kid) : -1;
-        if (S->ctl.norm == 1)
-            hipLaunchKernelGGL((k_residual<T, true>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
-                               A64, (const T*)A.invd, xin, xprev, e, xout, bsrc, bout, od,
-                               S->od_mask, v.r, v.u, S->partials, S->S, gen, of_nf);
-        else
-            hipLaunchKernelGGL((k_residual<T, false>), dim3(RG), dim3(RBS), 0, S->stream, S->g,
-                               A64, (const T*)A.invd, xin, xprev, e, xout, bsrc, bout, od,
-                               S->od_mask, v.r, v.u, S->partials, S->S, gen, 0);
+        // quads need nx % 4 == 0 and no correction beyond sigma_xy
+        const bool quad = S->quad && (!od || S->od_mask == 1u);
+#define CG_R(OF, Q, NF)                                                                    \
+    hipLaunchKernelGGL((k_residual<T, OF, Q>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A64, \
+                       (const T*)A.invd, xin, vcor, xprev, e, xout, bsrc, bout, od, S->od_mask, \
+                       v.r,                                                                \
+                       v.u, S->partials, S->S, gen, NF)
+        if (S->ctl.norm == 1) {
+            if (quad) CG_R(true, true, of_nf);
+            else CG_R(true, false, of_nf);
+        } else {
+            if (quad) CG_R(false, true, 0);
+            else CG_R(false, false, 0);
+        }
+#undef CG_R
         if (prof) prof->end(tk);
     }
 };
@@ -862,10 +969,16 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
     const unsigned grid = grid_of(S->n);
     int tk;
     hipError_t e;
+    // With one corrector per step the ionic kernel (mixed path, extrapolating) also writes
+    // the initial guess 2 V^n - V^{n-1} into xalt, so the first residual reads one x.
+    // Outer correctors keep the extrapolation in the residual.
+    const bool fuse = S->ctl.n_outer <= 1;
+    const double* od_res = S->od;
+    const bool x0_fused = fuse && prec == 1 && extrapolate;
     if (!ionic_queued) {
         tk = prof ? prof->begin(K_IONIC) : -1;
-        e = launch_ionic_rhs(model, prec, tissue, S->g, states, Vn, Vm1, nullptr, Iion, S->b, st,
-                             P, S->stream);
+        e = launch_ionic_rhs(model, prec, tissue, S->g, states, Vn, Vm1,
+                             x0_fused ? S->xalt : nullptr, Iion, S->b, st, P, S->stream);
         if (prof) prof->end(tk);
         if (e != hipSuccess) { *err = std::string("ionic/rhs launch: ") + hipGetErrorString(e); return 4; }
     }
@@ -904,12 +1017,14 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
         hipMemcpyAsync(S->hS, S->S, sizeof(DevState), hipMemcpyDeviceToHost, S->stream);
         hipEventRecord(S->read_done, S->stream);
         if (last && next) {
+            // the next step (which extrapolates) gets the same fusion
             ExplicitStep sn = next->st;
             sn.gate = conv_gen;
             sn.gate_val = g;
             int t = prof ? prof->begin(K_IONIC) : -1;
-            launch_ionic_rhs(model, prec, tissue, S->g, states, Vnew, Vn, nullptr, Iion, S->b, sn,
-                             P, S->stream);
+            launch_ionic_rhs(model, prec, tissue, S->g, states, Vnew, Vn,
+                             fuse && prec == 1 ? S->xalt : nullptr, Iion, S->b, sn, P,
+                             S->stream);
             if (prof) prof->end(t);
         }
         if (hipGetLastError() != hipSuccess) return false;
@@ -931,7 +1046,7 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
             const double* xin = outer == 1 ? Vn : Vnew;
             if (of) cg.mean(xin, nullptr);
             cg.residual(K_RESIDUAL0, gen, xin, nullptr, nullptr, outer == 1 ? Vnew : nullptr,
-                        S->od, A, S->b, nullptr, 1);
+                        outer == 1 ? od_res : S->od, A, S->b, nullptr, 1);
             cg.s_kernel(gen, 0, tol * tol, 0, 0.0, 0.0, max_iter, tol, 1);
             int k = 0;
             int chunk = S->last_inner + 1 > 2 ? S->last_inner + 1 : 2;
@@ -978,20 +1093,26 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
         CgRun<float> cg{S, op_of<float>(S, st.c), S->v32, prof, &log};
         const double abs2 = of ? 0.0 : 0.25 * tol * tol;
         // b plus the correction lives in b itself unless later correctors need b again
-        double* bcur = S->od ? (n_outer > 1 ? S->bc : S->b) : nullptr;
+        double* bcur = od_res ? (n_outer > 1 ? S->bc : S->b) : nullptr;
         const double* bref = bcur ? bcur : S->b;
         const double* cur = Vn;
         for (int outer = 1;; ++outer) {
             const bool last = n_outer == 1 || outer >= n_outer ||
                               (outer > 1 && prev_init < S->ctl.residual_control);
             long long gen = ++S->gen;
-            if (outer == 1) {
-                // x0 = 2 V^n - V^{n-1} (V^n on the first step), written to xalt so the first
+            if (outer == 1 && x0_fused) {
+                // x0 = 2 V^n - V^{n-1} already in xalt (ionic kernel), so the first
                 // refinement lands in V^{n+1} itself
+                if (of) cg.mean(S->xalt, nullptr);
+                cg.residual(K_RESIDUAL0, gen, S->xalt, nullptr, nullptr, nullptr, od_res, A,
+                            S->b, bcur, 1, Vn);
+                cur = S->xalt;
+            } else if (outer == 1) {
+                // x0 = 2 V^n - V^{n-1} (V^n on the first step), written to xalt
                 const double* xprev = extrapolate ? Vm1 : nullptr;
                 if (of) cg.mean(Vn, xprev);
                 cg.residual(K_RESIDUAL0, gen, Vn, xprev, nullptr, xprev ? S->xalt : nullptr,
-                            S->od, A, S->b, bcur, 1);
+                            od_res, A, S->b, bcur, 1);
                 cur = xprev ? S->xalt : Vn;
             } else {
                 if (of) cg.mean(cur, nullptr);
