@@ -40,20 +40,30 @@ WORKLOADS = {
 NSTATES = {"ORd": 41, "BuenoOrovio": 4, "TNNP": 17, "AlievPanfilov": 2}
 
 
-def algo_bytes_per_cell(kernel, prec, model, algorithm):
+def algo_bytes_per_cell(kernel, prec, model, algorithm, od_planes=1):
+    """od_planes: non-zero off-diagonal conductivity planes (1 for fibres in the xy plane);
+    each costs 8 B read by the first residual, which then also writes the corrected b."""
     w = 8 if prec == "fp64" else 4
     s = NSTATES[model] - 1
     if kernel == "ionic":
-        if algorithm == "implicit":  # V^n read, states r/w, Iion + rhs written
-            return 8 + 2 * s * w + 8 + 8
+        if algorithm == "implicit":
+            # V^n read, states r/w, Iion + rhs written; mixed: V^{n-1} read and the
+            # extrapolated initial guess written as well
+            return 8 + 2 * s * w + 8 + 8 + (16 if prec == "mixed" else 0)
         return 8 + 2 * s * w + 8 + 8 + 1  # V^n, states, Iion, V^{n+1}, flag
     if kernel == "spmv_dot":         # u, 3 face coefficients read; w written
         return 5 * w
     if kernel == "cg_update":        # u, p, w, s, r, 1/diag, x read; p, s, x, r, u written
         return 12 * w
-    if kernel == "residual_first":   # V^n, b, 3 fp64 coefficients, 1/diag, (b, V^{n+1}) ; r, u
-        corr = 24 + 8                # off-diagonal sigma read, corrected b written
-        return 8 + 8 + 24 + w + 2 * w + corr + (8 if prec == "fp64" else 0)
+    if kernel == "residual_first":
+        # b, 3 fp64 coefficients, 1/diag, x0 read; r, u written; plus
+        #   fp64: x0 = V^n copied into V^{n+1};
+        #   mixed: x0 (written by the ionic kernel) and V^n for the correction;
+        #   correction: od planes read, corrected b written (mixed; fp64 keeps it in r)
+        base = 8 + 24 + w + 8 + 2 * w
+        if prec == "fp64":
+            return base + 8 + 8 * od_planes
+        return base + 8 * od_planes + (8 + 8 if od_planes else 0)
     if kernel == "residual":         # x, e, b, 3 fp64 coefficients, 1/diag; x', r, u written
         return 8 + 4 + 8 + 24 + 4 + 8 + 2 * 4
     if kernel == "activation":       # solution, flag read (V^n on crossing only)
@@ -633,7 +643,7 @@ def main():
     kernels_out = {}
     for prec in ("fp64", "mixed"):
         for k, (ms, cnt) in res[prec]["kernels"].items():
-            bpc = algo_bytes_per_cell(k, prec, w["model"], w["algorithm"])
+            bpc = algo_bytes_per_cell(k, prec, w["model"], w["algorithm"], int(w["rotating"]))
             avg = ms / cnt if cnt else 0.0
             gbs = bpc * n / (avg * 1e-3) / 1e9 if (bpc and avg > 0) else None
             kernels_out["%s/%s" % (prec, k)] = {"avg_us": round(avg * 1e3, 2), "launches": cnt,
@@ -642,7 +652,7 @@ def main():
                                                 "share": round(ms / sum(v[0] for v in res[prec]["kernels"].values()), 3)}
     if dom_k:
         ms, cnt = kt[dom_k]
-        bpc = algo_bytes_per_cell(dom_k, "mixed", w["model"], w["algorithm"])
+        bpc = algo_bytes_per_cell(dom_k, "mixed", w["model"], w["algorithm"], int(w["rotating"]))
         avg_s = ms / cnt * 1e-3
         ach = bpc * n / avg_s / 1e9
         sym = kernel_symbol(dom_k, "mixed", w["model"], w["algorithm"])

@@ -37,7 +37,7 @@
 namespace cg {
 
 constexpr int BS = 256;    // elementwise kernels
-constexpr int RBS = 512;   // reducing kernels
+constexpr int RBS = 256;   // reducing kernels (measured best with RG = 1024 on 2.1 M cells)
 constexpr int RG = 1024;   // blocks of every reducing kernel
 
 // partial-sum slots (RG doubles each)
