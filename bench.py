@@ -170,6 +170,59 @@ def measure_spmv(torch, dom, n, prec, launches=200):
             "frac": round(gbs / HBM_PEAK_GBS, 4), "rows": n}
 
 
+def hex_ldu(shape, h, sigma, chiCm, dt):
+    """The workload's implicit operator as an lduMatrix (blockMesh cell order, faces in
+    upper-triangular order): per unit volume, diag = chiCm/dt + sum a_f, upper = -a_f,
+    a_f = sigma_nn,f / h^2 with sigma_nn,f the face average."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    c = np.arange(n).reshape(nz, ny, nx)
+    sig = np.asarray(sigma, dtype=np.float64)
+    sig = np.tile(sig.reshape(9), n).reshape(n, 9) if sig.size == 9 else sig.reshape(n, 9)
+    lo, up, a = [], [], []
+    for d, (sl_lo, sl_up) in enumerate([((slice(None), slice(None), slice(0, -1)),
+                                          (slice(None), slice(None), slice(1, None))),
+                                         ((slice(None), slice(0, -1), slice(None)),
+                                          (slice(None), slice(1, None), slice(None))),
+                                         ((slice(0, -1), slice(None), slice(None)),
+                                          (slice(1, None), slice(None), slice(None)))]):
+        l_, u_ = c[sl_lo].ravel(), c[sl_up].ravel()
+        lo.append(l_)
+        up.append(u_)
+        a.append(0.5 * (sig[l_, 4 * d] + sig[u_, 4 * d]) / (h * h))
+    lo, up, a = np.concatenate(lo), np.concatenate(up), np.concatenate(a)
+    order = np.lexsort((up, lo))
+    lo, up, a = lo[order].astype(np.int32), up[order].astype(np.int32), a[order]
+    diag = np.full(n, chiCm / dt)
+    np.add.at(diag, lo, a)
+    np.add.at(diag, up, a)
+    return n, lo, up, diag, -a
+
+
+def measure_ell_spmv(cf, torch, w, sigma, launches=100):
+    """The same operator through the general-mesh path: lduAddressing -> ELL rows, y = A x
+    in FP64, kernel time from HIP events.  Algorithmic bytes per row: the CSR count of one
+    FP64 product, (12 nnz + 4 (N+1) + 16 N) / N (values, column indices, row pointers, x
+    and y), and the ELL layout actually read (diag, x_i, K slots of 4 + 8 B, y)."""
+    n, lo, up, diag, u = hex_ldu(w["shape"], w["h"], sigma, 1.4e5 * 0.01, w["dt"])
+    L = cf.LduMatrix(lo, up, n)
+    L.set_coeffs(diag, u)
+    x = torch.randn(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(10):
+        L.amul_kernel_ms(x, y)
+    ms = sorted(L.amul_kernel_ms(x, y) for _ in range(launches))
+    avg = float(np.mean(ms[: int(0.9 * launches)])) * 1e-3  # the slowest tenth dropped
+    nnz = n + 2 * lo.size
+    csr_bpr = (12 * nnz + 4 * (n + 1) + 16 * n) / n
+    ell_bpr = 8 + 8 + L.width * 12 + 8
+    gbs = csr_bpr * n / avg / 1e9
+    return {"avg_us": round(avg * 1e6, 2), "rows": n, "nnz": nnz, "ell_width": L.width,
+            "bytes_per_row_csr": round(csr_bpr, 2), "bytes_per_row_ell": ell_bpr,
+            "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "GBps_ell_bytes": round(ell_bpr * n / avg / 1e9, 1)}
+
+
 def run_explicit_c0(cf, torch, steps=20000, warmup=200):
     """Plain-method workload: BuenoOrovio epi on 128x48x24 (h 0.15 mm), explicit Euler
     diffusion with Rush-Larsen gates, dt 0.005 ms, 100 ms; FP64 and mixed from rest."""
@@ -663,6 +716,8 @@ def main():
 
     spmv = {p: measure_spmv(torch, res[p]["dom"], n, p) for p in ("fp64", "mixed")} \
         if w["algorithm"] == "implicit" else None
+    if spmv is not None:
+        spmv["ell_fp64"] = measure_ell_spmv(cf, torch, w, sigma)
     c0 = run_explicit_c0(cf, torch) if args.workload != "bo_explicit_147k" else None
     c1 = run_batch_c1(cf) if not args.no_batch else None
     c2 = run_batch_c2(cf) if not args.no_batch else None

@@ -66,6 +66,15 @@ def lib():
     L.cg_tissue_set_solver_controls.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int,
                                                 C.c_int, C.c_double]
     L.cg_tissue_solver_report.argtypes = [vp, C.POINTER(C.c_int), dp, dp]
+    ip32 = C.POINTER(C.c_int32)
+    L.cg_ldu_create.argtypes = [ip64, ip64, ip32, ip32, C.POINTER(vp)]
+    L.cg_ldu_destroy.argtypes = [vp]
+    L.cg_ldu_width.argtypes = [vp]
+    L.cg_ldu_set_coeffs.argtypes = [vp, vp, vp, C.c_int]
+    L.cg_ldu_amul.argtypes = [vp, vp, vp, C.c_int]
+    L.cg_ldu_amul_timed.argtypes = [vp, vp, vp, C.POINTER(C.c_float)]
+    L.cg_ldu_solve.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                               C.POINTER(C.c_int), dp, dp]
     L.cg_ionic_set_integrator.argtypes = [vp, C.c_int, C.c_double, C.c_double, ip64,
                                           C.c_double]
     L.cg_ionic_step.argtypes = [vp, C.c_double, C.c_double, vp, vp, C.c_int]

@@ -13,6 +13,7 @@
 #include "cg_internal.hpp"
 #include "cg_implicit.hpp"
 #include "cg_adjoint.hpp"
+#include "cg_ldu.hpp"
 
 namespace {
 
@@ -1162,6 +1163,84 @@ int cg_tissue_solver_report(const cg_tissue* t, int* solves, double* init_res, d
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
     if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
     cg::implicit_report(t->implicit, solves, init_res, final_res);
+    return CG_OK;
+}
+
+// --------------------------------------------------------------- lduMatrix
+struct cg_ldu {
+    cg::LduSolver* L = nullptr;
+    bool coeffs = false;
+};
+
+int cg_ldu_create(int64_t n_cells, int64_t n_faces, const int32_t* lower, const int32_t* upper,
+                  cg_ldu** out)
+{
+    if (!out || (n_faces > 0 && (!lower || !upper))) return fail(CG_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (n_cells < 1 || n_faces < 0 || n_cells >= ((int64_t)1 << 31))
+        return fail(CG_ERR_ARG, "bad lduAddressing sizes");
+    cg_ldu* h = new cg_ldu();
+    const int rc = cg::ldu_create(&h->L, n_cells, n_faces, lower, upper, &g_err);
+    if (rc) {
+        delete h;
+        return rc == 5 ? CG_ERR_NOMEM : CG_ERR_ARG;
+    }
+    *out = h;
+    return CG_OK;
+}
+
+int cg_ldu_destroy(cg_ldu* h)
+{
+    if (!h) return CG_OK;
+    cg::ldu_destroy(h->L);
+    delete h;
+    return CG_OK;
+}
+
+int cg_ldu_width(const cg_ldu* h) { return h ? cg::ldu_width(h->L) : -1; }
+
+int cg_ldu_set_coeffs(cg_ldu* h, const double* diag, const double* upper, int flags)
+{
+    if (!h || !diag) return fail(CG_ERR_ARG, "NULL argument");
+    if (cg::ldu_set_coeffs(h->L, diag, upper, flags & CG_DEVICE_PTRS))
+        return fail(CG_ERR_HIP, "coefficient upload failed");
+    h->coeffs = true;
+    return CG_OK;
+}
+
+int cg_ldu_amul(cg_ldu* h, const double* x, double* y, int flags)
+{
+    if (!h || !x || !y) return fail(CG_ERR_ARG, "NULL argument");
+    if (!h->coeffs) return fail(CG_ERR_STATE, "coefficients not set");
+    if (cg::ldu_amul(h->L, x, y, flags & CG_DEVICE_PTRS, nullptr, nullptr))
+        return fail(CG_ERR_HIP, "lduMatrix Amul failed");
+    return CG_OK;
+}
+
+int cg_ldu_amul_timed(cg_ldu* h, const double* x, double* y, float* ms)
+{
+    if (!h || !x || !y || !ms) return fail(CG_ERR_ARG, "NULL argument");
+    if (!h->coeffs) return fail(CG_ERR_STATE, "coefficients not set");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int rc = cg::ldu_amul(h->L, x, y, true, e0, e1);
+    if (rc == 0) hipEventElapsedTime(ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc ? fail(CG_ERR_HIP, "lduMatrix Amul failed") : CG_OK;
+}
+
+int cg_ldu_solve(cg_ldu* h, double* psi, const double* source, double tol, double rel_tol,
+                 int max_iter, int flags, int* iters, double* r0, double* r1)
+{
+    if (!h || !psi || !source) return fail(CG_ERR_ARG, "NULL argument");
+    if (!h->coeffs) return fail(CG_ERR_STATE, "coefficients not set");
+    if (!(tol >= 0.0) || !(rel_tol >= 0.0) || max_iter < 0)
+        return fail(CG_ERR_ARG, "bad solver controls");
+    if (cg::ldu_solve(h->L, psi, source, flags & CG_DEVICE_PTRS, tol, rel_tol, max_iter, iters,
+                      r0, r1))
+        return fail(CG_ERR_HIP, "lduMatrix PCG failed");
     return CG_OK;
 }
 

@@ -33,12 +33,11 @@
 #include <vector>
 #include "cg_implicit.hpp"
 #include "cg_hexops.hpp"
+#include "cg_reduce.hpp"
 
 namespace cg {
 
 constexpr int BS = 256;    // elementwise kernels
-constexpr int RBS = 256;   // reducing kernels (measured best with RG = 1024 on 2.1 M cells)
-constexpr int RG = 1024;   // blocks of every reducing kernel
 
 // partial-sum slots (RG doubles each)
 enum { P_G = 0,      // (r,u), parity sets 0 and 1
@@ -65,41 +64,6 @@ struct DevState {
     double l1_0, l1;          // sum |r| at the start and at the end of the current solve
     double init_res;          // initial residual of the outer corrector (in its norm)
 };
-
-template <int NT, int NV> CG_DEV void block_sum(double (&v)[NV])
-{
-    constexpr int NW = NT / 64;
-    __shared__ double sm[NV][NW];
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) sm[k][w] = v[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double t = 0.0;
-        for (int q = 0; q < NW; ++q) t += sm[k][q];
-        v[k] = t;
-    }
-    __syncthreads();
-}
-
-// fixed-order sums of the RG partials of NV slots: the same values in every block
-template <int NV>
-CG_DEV void sum_slots(const double* __restrict__ part, const int (&slot)[NV], double (&v)[NV])
-{
-    static_assert(RG % RBS == 0, "partials must split evenly over the block");
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        v[k] = 0.0;
-#pragma unroll
-        for (int q = 0; q < RG / RBS; ++q) v[k] += part[slot[k] * RG + q * RBS + threadIdx.x];
-    }
-    block_sum<RBS, NV>(v);
-}
 
 template <class T> struct HexOp {
     const T* fx;

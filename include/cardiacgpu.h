@@ -264,6 +264,34 @@ int cg_tissue_set_profiling(cg_tissue* t, int on);
 int cg_tissue_kernel_times(cg_tissue* t, int max, char* names, double* total_ms,
                            int64_t* launches, int* n_out);
 
+/* ------------------------------------------- lduMatrix solve on any mesh
+ * The Vm equation's lduMatrix of an arbitrary (non-hex) mesh: diag[n_cells] and one
+ * coefficient per face (upper == lower, symmetric), addressed by the mesh's lduAddressing
+ * (lowerAddr/upperAddr).  Replaces fvMatrix::solve -> lduMatrix::solve with PCG for Vm
+ * (monodomainSolver.C:129-135; tolerance/relTol/maxIter of the fvSolution entry), for the
+ * meshes the hex-block path does not cover.  The addressing is rebuilt once into ELL rows
+ * on the device; the preconditioner is Jacobi (the reference's DIC is a sequential sweep). */
+typedef struct cg_ldu cg_ldu;
+int cg_ldu_create(int64_t n_cells, int64_t n_faces, const int32_t* lower_addr,
+                  const int32_t* upper_addr, cg_ldu** out);
+int cg_ldu_destroy(cg_ldu* L);
+/* slots per ELL row (the largest number of neighbours of a cell) */
+int cg_ldu_width(const cg_ldu* L);
+/* lduMatrix::diag() and upper() (flags: CG_DEVICE_PTRS) */
+int cg_ldu_set_coeffs(cg_ldu* L, const double* diag, const double* upper, int flags);
+/* y = A x (lduMatrix::Amul) */
+int cg_ldu_amul(cg_ldu* L, const double* x, double* y, int flags);
+/* PCG on A psi = source from the given psi, stopping as lduMatrix::solver does: residual =
+ * sum|source - A psi| / normFactor, normFactor = sum |A psi0 - sumA xRef| + |source -
+ * sumA xRef| + 1e-20 (xRef = mean of psi0, sumA = row sums); converged below tolerance or
+ * below rel_tol x the initial residual, tested before the first iteration; at most
+ * max_iter iterations.  Reports the iterations and the initial and final residuals. */
+int cg_ldu_solve(cg_ldu* L, double* psi, const double* source, double tolerance,
+                 double rel_tol, int max_iter, int flags, int* iterations,
+                 double* initial_residual, double* final_residual);
+/* milliseconds of the last cg_ldu_amul kernel alone (device pointers), for rooflines */
+int cg_ldu_amul_timed(cg_ldu* L, const double* x, double* y, float* kernel_ms);
+
 /* ------------------------------------------------------- standalone operators */
 /* Explicit FV Laplacian div(sigma grad V) per unit volume (A/m^3) of the "Gauss linear
  * corrected" scheme with the mesh's conductivity (fvc::laplacian, monodomainSolver.C:111);

@@ -281,6 +281,26 @@ class OracleMesh:
                                      chiCm, maxCo)
 
 
+def ldu_pcg_of(n_cells, lower_addr, upper_addr, diag, upper, b, x0, tol=1e-11, rel_tol=0.0,
+               max_iter=1000, precond=0):
+    """OpenFOAM PCG (orc_pcg_of) on an arbitrary lduAddressing: returns (x, iterations,
+    initial residual, final residual)"""
+    lo = np.ascontiguousarray(lower_addr, dtype=np.int32)
+    up = np.ascontiguousarray(upper_addr, dtype=np.int32)
+    m = Mesh()
+    m.n_cells = int(n_cells)
+    m.n_faces = int(lo.size)
+    m.owner = lo.ctypes.data_as(C.POINTER(C.c_int32))
+    m.neighbour = up.ctypes.data_as(C.POINTER(C.c_int32))
+    x = np.array(x0, dtype=np.float64)
+    r0, r1 = C.c_double(), C.c_double()
+    it = lib().orc_pcg_of(C.byref(m), np.ascontiguousarray(diag, dtype=np.float64),
+                          np.ascontiguousarray(upper, dtype=np.float64),
+                          np.ascontiguousarray(b, dtype=np.float64), x, tol, rel_tol, max_iter,
+                          precond, C.byref(r0), C.byref(r1))
+    return x, it, r0.value, r1.value
+
+
 def make_boxes(box_min, box_max, t_start, duration, intensity):
     arrs = [np.ascontiguousarray(np.ravel(a), dtype=np.float64)
             for a in (box_min, box_max, t_start, duration, intensity)]

@@ -220,3 +220,23 @@ def test_pimple_outer_correctors_reach_implicit_correction(orc):
     assert e_k < 1e-12, e_k
     assert e_1 > 1e3 * e_k
     assert runs[400][2][1] > 2 and runs[1][2][1] == 1
+
+
+@pytest.mark.parametrize("kind", ["triangles", "renumbered_hex"])
+def test_openfoam_pcg_on_unstructured_addressing(orc, kind):
+    """The oracle's lduMatrix PCG on addressing that is not a hex block (a Delaunay
+    triangulation; a randomly renumbered hex block) against scipy's direct solve, with
+    the diagonal and the DIC preconditioner."""
+    from tests import ldu_cases as lc
+    if kind == "triangles":
+        n, lo, up = lc.triangle_mesh(400, seed=1)
+    else:
+        n, lo, up, _ = lc.renumbered_hex(8, 6, 5, seed=2)
+    diag, u = lc.diffusion_coeffs(n, lo, up, seed=3)
+    b = np.random.default_rng(4).normal(size=n) * 10.0 + 3.0
+    xs = spla.spsolve(lc.csr(n, lo, up, diag, u).tocsc(), b)
+    for pc in (0, 1):
+        x, it, r0, r1 = orc.ldu_pcg_of(n, lo, up, diag, u, b, np.zeros(n), tol=1e-14,
+                                       precond=pc)
+        assert r1 < 1e-14 and it > 0
+        assert np.max(np.abs(x - xs)) <= 1e-10 * np.max(np.abs(xs))
