@@ -62,6 +62,12 @@ struct ExplicitStep {
     const long long* gate;
     long long gate_val;
     OdeArgs ode;
+    // implicit path: the previous step's activation-time update (myocardiumDomain.C:
+    // 345-374), done by this step's ionic kernel, which reads V^n and V^{n-1} anyway;
+    // act_flag null: none
+    uint8_t* act_flag;
+    double* act_tact;
+    double act_t_old, act_dt;
 };
 
 // Model kernels, one set per arithmetic type: _fp64 (IEEE double, model_fp64.hip) and

@@ -200,9 +200,9 @@ __global__ __launch_bounds__(MBS) void k_explicit_hex(
 // rhs = chiCm/dt (c0 V^n - c00 V^{n-1}) - chiCm Iion + source (the explicit tensor
 // correction is added by the solver's first residual); x0 (when not null) receives the
 // solver's extrapolated initial guess 2 V^n - V^{n-1}, riding on this kernel's pass over
-// V^n.  With a gate (s.gate) the launch may be queued before the previous step's solve is
-// known to have converged: it then does nothing unless the solver recorded the expected
-// generation.
+// V^n; with s.act_flag it also completes the previous step's activation times.  With a
+// gate (s.gate) the launch may be queued before the previous step's solve is known to have
+// converged: it then does nothing unless the solver recorded the expected generation.
 template <class M, class T, int INTEG>
 __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st,
                                                    const double* __restrict__ Vn,
@@ -222,16 +222,24 @@ __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st
     T y[M::NS];
     load_states(y, st, n, c, 1);
     const double vn = Vn[c];
+    // loads of the epilogue issued before the model's arithmetic, so their latency hides
+    // behind it
+    const double vo = (x0 || s.c00 != 0.0 || s.act_flag) ? Vm1[c] : 0.0;
+    const uint8_t fl = s.act_flag ? s.act_flag[c] : 0;
     const T v = T((vn * 1000.0 + M::VOFF) / M::VSCALE);
     const T I = tissue_advance<M, T, INTEG>(v, y, T(s.dt_model), p, tissue, one, s.ode, c);
     store_states(y, st, n, c, 1);
     const double iion = double(I) * M::ISCALE;
     Iion[c] = iion;
     const double src = box_source(cc.i, cc.j, cc.k, g, s.boxes);
-    double rhs = s.chiCm / s.dt * (s.c0 * vn - s.c00 * (s.c00 != 0.0 ? Vm1[c] : 0.0)) +
-                 src - s.chiCm * iion;
+    double rhs = s.chiCm / s.dt * (s.c0 * vn - s.c00 * vo) + src - s.chiCm * iion;
     b[c] = rhs;
-    if (x0) x0[c] = 2.0 * vn - Vm1[c];
+    if (x0) x0[c] = 2.0 * vn - vo;
+    // the previous step's activation: V^n is its new potential, V^{n-1} its old one
+    if (fl && vn > 1e-15) {
+        s.act_flag[c] = 0;
+        s.act_tact[c] = s.act_t_old + (0.0 - vo) / (vn - vo) * s.act_dt;
+    }
 }
 
 // ---------------------------------------------------------------- launchers

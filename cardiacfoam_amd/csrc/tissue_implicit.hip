@@ -972,7 +972,10 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
     // without the gated launches (the step is not finished).
     auto finish_and_read = [&](long long g, const double* x, size_t* mark, bool last) -> bool {
         *mark = prof ? prof->mark() : 0;
-        if (last) {
+        // with the solution already in V^{n+1}, the next step's ionic kernel does the
+        // activation update (nothing to copy); otherwise k_activation copies and updates
+        const bool act_in_next = last && next && x == Vnew;
+        if (last && !act_in_next) {
             int t = prof ? prof->begin(K_ACTIVATION) : -1;
             hipLaunchKernelGGL(k_activation, dim3(grid), dim3(BS), 0, S->stream, S->g, Vn, x,
                                Vnew, flag, tact, st.t_old, st.dt, conv_gen, g);
@@ -985,6 +988,12 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
             ExplicitStep sn = next->st;
             sn.gate = conv_gen;
             sn.gate_val = g;
+            if (act_in_next) {
+                sn.act_flag = flag;
+                sn.act_tact = tact;
+                sn.act_t_old = st.t_old;
+                sn.act_dt = st.dt;
+            }
             int t = prof ? prof->begin(K_IONIC) : -1;
             launch_ionic_rhs(model, prec, tissue, S->g, states, Vnew, Vn,
                              fuse && prec == 1 ? S->xalt : nullptr, Iion, S->b, sn, P,
