@@ -47,9 +47,10 @@ def algo_bytes_per_cell(kernel, prec, model, algorithm, od_planes=1):
     s = NSTATES[model] - 1
     if kernel == "ionic":
         if algorithm == "implicit":
-            # V^n read, states r/w, Iion + rhs written; mixed: V^{n-1} read and the
-            # extrapolated initial guess written as well
-            return 8 + 2 * s * w + 8 + 8 + (16 if prec == "mixed" else 0)
+            # V^n read, states r/w, Iion + rhs written; V^{n-1} and the activation flag
+            # read for the previous step's activation update; mixed: the extrapolated
+            # initial guess written as well
+            return 8 + 2 * s * w + 8 + 8 + 8 + 1 + (8 if prec == "mixed" else 0)
         return 8 + 2 * s * w + 8 + 8 + 1  # V^n, states, Iion, V^{n+1}, flag
     if kernel == "spmv_dot":         # u, 3 face coefficients read; w written
         return 5 * w
@@ -97,9 +98,11 @@ def kernel_symbol(kernel, prec, model, algorithm):
     t = "float" if prec == "mixed" else "double"
     if kernel == "ionic":
         if algorithm == "implicit":
-            return "cg::k_ionic_rhs<cg::%s, %s>" % (model, t)
+            return "cg::k_ionic_rhs<cg::%s, %s, 0>" % (model, t)  # Rush-Larsen
         return None
-    return {"spmv_dot": "cg::k_cg_s<%s>" % t, "cg_update": "cg::k_cg_u<%s, %s>" % (t, t),
+    # the workload's kernels: quads (nx % 4 == 0), relative L2 norm
+    return {"spmv_dot": "cg::k_cg_s<%s, true, 0>" % t,
+            "cg_update": "cg::k_cg_u<%s, %s, true, false>" % (t, t),
             "activation": "cg::k_activation"}.get(kernel)
 
 
