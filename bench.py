@@ -173,6 +173,25 @@ def measure_spmv(torch, dom, n, prec, launches=200):
             "frac": round(gbs / HBM_PEAK_GBS, 4), "rows": n}
 
 
+def copy_bandwidth(torch, nbytes=1 << 30, reps=20):
+    """What a plain device-to-device copy of 1 GiB achieves (read + write bytes / time),
+    the practical streaming ceiling beside the 8 TB/s peak"""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / reps * 1e-3
+    del a, b
+    return round(2 * nbytes / t / 1e9, 1)
+
+
 def hex_ldu(shape, h, sigma, chiCm, dt):
     """The workload's implicit operator as an lduMatrix (blockMesh cell order, faces in
     upper-triangular order): per unit volume, diag = chiCm/dt + sum a_f, upper = -a_f,
@@ -715,7 +734,8 @@ def main():
         roof = {"bound": "hbm", "kernel": sym or dom_k, "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(args.workload, "mixed", sym),
-                "bytes_per_launch": bpc * n, "avg_launch_us": round(avg_s * 1e6, 2)}
+                "bytes_per_launch": bpc * n, "avg_launch_us": round(avg_s * 1e6, 2),
+                "achievable_copy_GBps": copy_bandwidth(torch)}
 
     spmv = {p: measure_spmv(torch, res[p]["dom"], n, p) for p in ("fp64", "mixed")} \
         if w["algorithm"] == "implicit" else None
