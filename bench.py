@@ -646,10 +646,23 @@ def cpu_baseline(w, sigma, sample_steps):
                     break
     except OSError:
         pass
+    # one thread on the same mesh, a few steps: the scalar restatement's own speed
+    del T
+    T1 = oracle.OracleTissue(nx, ny, nz, w["h"], sigma, model, tissue, w["dt"], 1.4e5, 0.01,
+                             integrator=0, ddt=0, implicit=int(w["algorithm"] == "implicit"),
+                             pcg_tol=w["tol"] or 1e-8, nthreads=1)
+    T1.step(bx)
+    one_steps = 2
+    t1 = time.perf_counter()
+    for _ in range(one_steps):
+        T1.step(bx)
+    el1 = time.perf_counter() - t1
     return {"value": n * sample_steps / el, "unit": "cell-steps/s", "cores": cores,
             "kind": "port",
             "sample": "CPU FP64 oracle (C, OpenMP, %d threads, %s): full %dx%dx%d mesh, %d steps "
-                      "after 1 warm-up step, %.1f s" % (cores, cpu, nx, ny, nz, sample_steps, el)}
+                      "after 1 warm-up step, %.1f s" % (cores, cpu, nx, ny, nz, sample_steps, el),
+            "one_thread": {"value": n * one_steps / el1, "steps": one_steps,
+                           "seconds": round(el1, 2)}}
 
 
 def main():

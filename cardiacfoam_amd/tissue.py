@@ -67,6 +67,33 @@ class HexMesh:
         return dt.value
 
 
+def fv_solution_controls(fv_solution, field="Vm"):
+    """Solver controls of `field` from a parsed fvSolution, looked up as fvMatrix::solve and
+    pimpleControl do: the first solvers entry whose key (possibly a regular expression such
+    as "Vm|VmFinal") matches the field (tolerance, relTol, maxIter with OpenFOAM's default
+    1000), and PIMPLE nOuterCorrectors (default 1) and residualControl tolerance (default
+    0).  The residual norm is OpenFOAM's."""
+    import re
+    entry = None
+    for key, d in fv_solution.get("solvers", {}).items():
+        if re.fullmatch(key.strip('"'), field):
+            entry = d
+            break
+    if entry is None:
+        raise FatalIOError("keyword %s is undefined in solvers" % field)
+    if "tolerance" not in entry:
+        raise FatalIOError("keyword tolerance is undefined in solvers/%s" % field)
+    pimple = fv_solution.get("PIMPLE", {})
+    rc = 0.0
+    for key, d in pimple.get("residualControl", {}).items():
+        if re.fullmatch(key.strip('"'), field):
+            rc = float(d.get("tolerance", 0.0))
+            break
+    return {"norm": "OpenFOAM", "tolerance": float(entry["tolerance"]),
+            "relTol": float(entry.get("relTol", 0.0)), "maxIter": int(entry.get("maxIter", 1000)),
+            "nOuterCorrectors": int(pimple.get("nOuterCorrectors", 1)), "residualControl": rc}
+
+
 class Monodomain:
     """Device monodomain tissue: MyocardiumDomain state + MonodomainSolver operator."""
 
@@ -221,25 +248,8 @@ class Monodomain:
                                                   float(residualControl)))
 
     def set_fv_solution(self, fv_solution, field="Vm"):
-        """Reads the Vm solver entry and the PIMPLE dictionary of a parsed fvSolution
-        (solvers keys may be regular expressions such as "Vm|VmFinal"), as fvMatrix::solve
-        and pimpleControl look them up; the residual norm is OpenFOAM's."""
-        import re
-        entry = None
-        for key, d in fv_solution.get("solvers", {}).items():
-            if re.fullmatch(key.strip('"'), field):
-                entry = d
-                break
-        if entry is None:
-            raise FatalIOError("keyword %s is undefined in solvers" % field)
-        pimple = fv_solution.get("PIMPLE", {})
-        rc = 0.0
-        for key, d in pimple.get("residualControl", {}).items():
-            if re.fullmatch(key.strip('"'), field):
-                rc = float(d.get("tolerance", 0.0))
-        self.set_solver_controls("OpenFOAM", float(entry.get("tolerance", 1e-6)),
-                                 float(entry.get("relTol", 0.0)), int(entry.get("maxIter", 1000)),
-                                 int(pimple.get("nOuterCorrectors", 1)), rc)
+        """Applies fv_solution_controls(fv_solution, field)."""
+        self.set_solver_controls(**fv_solution_controls(fv_solution, field))
 
     def solver_report(self):
         """last step: (linear solves, initial residual of the last corrector, final residual)"""

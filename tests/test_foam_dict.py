@@ -42,3 +42,36 @@ def test_parse_niederer():
     full = symm_to_full(c["conductivity"])
     assert full[0] == 0.1334177215 and full[4] == full[8] == 0.01760617761
     assert full[1] == full[3] == 0
+
+
+FV_SOLUTION = """
+solvers
+{
+    "Vm|VmFinal|u|uFinal" { solver PCG; preconditioner DIC; tolerance 1e-11; relTol 0.0; }
+    "psi|psiFinal" { solver PCG; preconditioner DIC; tolerance 1e-11; relTol 0.1; }
+}
+PIMPLE
+{
+    nOuterCorrectors 1000;
+    residualControl { "Vm|psi" { relTol 0.0; tolerance 1e-6; } }
+}
+"""
+
+
+def test_fv_solution_controls_of_vm():
+    """The Vm entry is found through the regular-expression key, maxIter defaults to
+    OpenFOAM's 1000, PIMPLE's corrector count and the Vm residual control are read; a
+    field without an entry is an error, as fvSolution::solverDict makes it."""
+    import pytest
+    from cardiacfoam_amd.tissue import fv_solution_controls
+    from cardiacfoam_amd.ionic import FatalIOError
+    c = fv_solution_controls(parse(FV_SOLUTION))
+    assert c == {"norm": "OpenFOAM", "tolerance": 1e-11, "relTol": 0.0, "maxIter": 1000,
+                 "nOuterCorrectors": 1000, "residualControl": 1e-6}
+    c = fv_solution_controls(parse(FV_SOLUTION), field="psi")
+    assert c["relTol"] == 0.1 and c["residualControl"] == 1e-6
+    with pytest.raises(FatalIOError):
+        fv_solution_controls(parse(FV_SOLUTION), field="T")
+    single = fv_solution_controls(parse('solvers { Vm { tolerance 1e-8; maxIter 50; } }'))
+    assert single["maxIter"] == 50 and single["nOuterCorrectors"] == 1
+    assert single["residualControl"] == 0.0
