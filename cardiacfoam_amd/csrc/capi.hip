@@ -37,11 +37,13 @@ int fail(int code, const char* fmt, ...)
             return fail(CG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));           \
     } while (0)
 
-const char* const kModelKeys[4] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov"};
+const char* const kModelKeys[5] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
+                                   "monodomainFDAManufactured"};
 
 // state names, order of the *_STATES_NAMES tables / Names.H enums
 const char* const kBONames[4] = {"u", "v", "w", "s"};
 const char* const kAPNames[2] = {"u", "recovery_r"};
+const char* const kFDANames[4] = {"V", "u1", "u2", "u3"};
 const char* const kTNNPNames[17] = {"V", "K_i", "Na_i", "Ca_i", "Xr1", "Xr2", "Xs", "m", "h",
                                     "j", "d", "f", "fCa", "s", "r", "Ca_SR", "g"};
 const char* const kORdNames[41] = {
@@ -74,6 +76,9 @@ bool model_info(int model, ModelInfo* mi)
         *mi = {cg::AlievPanfilov::NS, cg::AlievPanfilov::NP, cg::AlievPanfilov::VSCALE,
                cg::AlievPanfilov::VOFF, cg::AlievPanfilov::ISCALE, cg::AlievPanfilov::TSCALE};
         return true;
+    case cg::MODEL_FDA:
+        *mi = {cg::FdaManufactured::NS, cg::FdaManufactured::NP, 1.0, 0.0, 1.0, 1.0};
+        return true;
     default: return false;
     }
 }
@@ -85,6 +90,7 @@ bool model_host_init(int model, int tissue, double* p, double* y)
     case cg::MODEL_TNNP: return cg::TNNP::host_init(tissue, p, y);
     case cg::MODEL_ORD: return cg::ORd::host_init(tissue, p, y);
     case cg::MODEL_AP: return cg::AlievPanfilov::host_init(tissue, p, y);
+    case cg::MODEL_FDA: return cg::FdaManufactured::host_init(tissue, p, y);
     default: return false;
     }
 }
@@ -93,7 +99,23 @@ bool model_host_init(int model, int tissue, double* p, double* y)
 // (BuenoOrovio.C:107-108, AlievPanfilov.C:107-108)
 double model_time(int model, double t_s)
 {
+    if (model == cg::MODEL_FDA) return t_s;  // monodomainFDAManufactured.C:107
     return model == cg::MODEL_AP ? t_s * 1000.0 / 12.9 : t_s * 1000.0;
+}
+
+// deltaT in model time units (BuenoOrovio.C:127, AlievPanfilov.C:127, ...)
+double model_dt(int model, double dt_s)
+{
+    if (model == cg::MODEL_FDA) return dt_s;
+    return model == cg::MODEL_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+}
+
+// tStart and tEnd of one step as each solveODE forms them: scaled (t0 + dt) for the
+// physiological models, tStart + deltaT for the manufactured one
+void model_step_times(int model, double t0_s, double dt_s, double* t0, double* t1)
+{
+    *t0 = model_time(model, t0_s);
+    *t1 = model == cg::MODEL_FDA ? *t0 + dt_s : model_time(model, t0_s + dt_s);
 }
 
 } // namespace
@@ -181,9 +203,8 @@ int ode_args(cg_ionic* h, double t0_s, double dt_s, cg::OdeArgs* o)
     o->rkf = h->ode_kind == CG_ODE_RKF45;
     if (!o->rkf) return CG_OK;
     const bool ap = h->model == cg::MODEL_AP;
-    o->t0 = ap ? t0_s * 1000.0 / 12.9 : t0_s * 1000.0;
-    o->t1 = ap ? (t0_s + dt_s) * 1000.0 / 12.9 : (t0_s + dt_s) * 1000.0;
-    o->dt_clamp = ap ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+    model_step_times(h->model, t0_s, dt_s, &o->t0, &o->t1);
+    o->dt_clamp = model_dt(h->model, dt_s);
     o->abs_tol = h->abs_tol;
     o->rel_tol = h->rel_tol;
     o->max_steps = h->max_steps;
@@ -227,7 +248,7 @@ const char* cg_version(void) { return "cardiacgpu 0.1 gfx950"; }
 int cg_model_id(const char* key)
 {
     if (!key) return -1;
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 5; ++i)
         if (!strcmp(key, kModelKeys[i])) return i;
     return -1;
 }
@@ -239,6 +260,10 @@ int cg_tissue_flag(const char* t)
     if (!strcmp(t, "mCells")) return 2;
     if (!strcmp(t, "endocardialCells")) return 3;
     if (!strcmp(t, "myocyte")) return 4;
+    // the manufactured model's `dimension` word
+    if (!strcmp(t, "1D")) return 11;
+    if (!strcmp(t, "2D")) return 12;
+    if (!strcmp(t, "3D")) return 13;
     return -1;
 }
 
@@ -254,7 +279,7 @@ int cg_model_state_name(int model, int i, char* buf, size_t len)
     if (!model_info(model, &mi)) return fail(CG_ERR_MODEL, "unknown model id %d", model);
     if (i < 0 || i >= mi.ns || !buf || len == 0) return fail(CG_ERR_ARG, "bad state index %d", i);
     const char* nm = model == 0 ? kBONames[i] : model == 1 ? kTNNPNames[i]
-                   : model == 2 ? kORdNames[i] : kAPNames[i];
+                   : model == 2 ? kORdNames[i] : model == 3 ? kAPNames[i] : kFDANames[i];
     snprintf(buf, len, "%s", nm);
     return CG_OK;
 }
@@ -269,6 +294,8 @@ int cg_ionic_create(int model, int tissue, int64_t n, int solve_vm, int precisio
     if (n <= 0) return fail(CG_ERR_ARG, "n_cells must be positive");
     if (precision != CG_FP64 && precision != CG_MIXED)
         return fail(CG_ERR_ARG, "unknown precision %d", precision);
+    if (model == cg::MODEL_FDA && solve_vm)
+        return fail(CG_ERR_STATE, "monodomainFDAManufactured is a tissue model (dV/dt is 0)");
     double p[cg::MAX_PARAMS] = {0}, y0[64] = {0};
     if (!model_host_init(model, tissue, p, y0))
         return fail(CG_ERR_TISSUE, "Unsupported tissue flag %d for ionicModel %s", tissue,
@@ -413,7 +440,7 @@ int cg_ionic_step(cg_ionic* h, double t0, double dt, const double* Vm, double* I
         dv = h->d_vm;
         di = h->d_im;
     }
-    const double dtm = h->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
+    const double dtm = model_dt(h->model, dt);
     HIPCHK(cg::launch_ionic_tissue(h->model, h->prec, h->tissue, h->d_states, h->n, dv, di, dtm,
                                    h->P, h->d_gscale, ode, h->stream));
     if (!(flags & CG_DEVICE_PTRS))
@@ -427,7 +454,7 @@ int cg_ionic_run_cells(cg_ionic* h, double t0, double dt, int n_steps)
     if (!h) return fail(CG_ERR_ARG, "NULL handle");
     if (!h->solve_vm) return fail(CG_ERR_STATE, "cg_ionic_run_cells needs a single-cell handle");
     if (n_steps < 0 || !(dt > 0.0)) return fail(CG_ERR_ARG, "bad step arguments");
-    const double dtm = h->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
+    const double dtm = model_dt(h->model, dt);
     cg::OdeArgs ode;
     if (int rc = ode_args(h, t0, dt, &ode)) return rc;
     HIPCHK(cg::launch_ionic_cells(h->model, h->prec, h->tissue, h->d_states, h->n,
@@ -558,6 +585,8 @@ int batch_check(cg_ionic* const* hs, int nh, double dt_s)
         if (!hs[i]) return fail(CG_ERR_ARG, "NULL handle %d", i);
         if (!hs[i]->solve_vm) return fail(CG_ERR_STATE, "handle %d is a tissue handle", i);
         if (!hs[i]->lens_set) return fail(CG_ERR_STATE, "handle %d: run lengths not set", i);
+        if (hs[i]->model == cg::MODEL_FDA)
+            return fail(CG_ERR_STATE, "handle %d: the manufactured model has no pacing protocol", i);
         if (hs[i]->ode_kind != CG_ODE_RUSH_LARSEN)
             return fail(CG_ERR_STATE, "handle %d: batched protocols integrate with Rush-Larsen; "
                                       "use cg_ionic_run_cells for RKF45", i);
@@ -579,7 +608,7 @@ int cg_batch_run(cg_ionic* const* hs, int nh, double dt_s, int chunk)
         cg_ionic* h = hs[i];
         HIPCHK(hipStreamSynchronize(h->stream));
         if (h->max_beats <= 0) continue;
-        const double dtm = h->model == cg::MODEL_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+        const double dtm = model_dt(h->model, dt_s);
         const int need = (int)std::ceil(h->beat_base_s * model_time(h->model, 1.0) / dtm) + 4;
         if (need != h->ring_len) {
             if (h->d_ring) hipFree(h->d_ring);
@@ -612,7 +641,7 @@ int cg_batch_run(cg_ionic* const* hs, int nh, double dt_s, int chunk)
     for (int i = 1; i < nh && err == hipSuccess; ++i) err = hipStreamWaitEvent(hs[i]->stream, fork, 0);
     for (int i = 0; i < nh && err == hipSuccess; ++i) {
         cg_ionic* h = hs[i];
-        const double dtm = h->model == cg::MODEL_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+        const double dtm = model_dt(h->model, dt_s);
         for (int64_t st = 0; st < h->max_len && err == hipSuccess; st += chunk)
             err = batch_launch(h, dtm, st, chunk, h->stream);
     }
@@ -642,7 +671,7 @@ int cg_batch_autotune(cg_ionic* const* hs, int nh, double dt_s, int trial_steps,
     std::string rep;
     for (int i = 0; i < nh; ++i) {
         cg_ionic* h = hs[i];
-        const double dtm = h->model == cg::MODEL_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+        const double dtm = model_dt(h->model, dt_s);
         const size_t bytes = h->elem * (size_t)h->mi.ns * h->n;
         void* save = nullptr;
         if (hipMalloc(&save, bytes) != hipSuccess) return fail(CG_ERR_NOMEM, "autotune scratch");
@@ -995,7 +1024,7 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
     cg_ionic* ion = t->ion;
     cg_mesh* m = t->mesh;
     const double dt = t->cfg.dt, chiCm = t->cfg.chi * t->cfg.cm;
-    const double dtm = ion->model == cg::MODEL_AP ? dt * 1000.0 / 12.9 : dt * 1000.0;
+    const double dtm = model_dt(ion->model, dt);
     const cg::HexCoefs coefs = mesh_coefs(m);
     // constants of the step that starts at accumulated time t_before after steps_before
     // steps: t += dt; t0 = t - dt; prepareTimeStep(t0) (myocardiumDomain.C:487-495)
@@ -1013,10 +1042,7 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
         st.od = m->d_od;
         st.od_mask = m->od_mask;
         st.ode = ode0;
-        if (ode0.rkf) {
-            st.ode.t0 = model_time(ion->model, t0);
-            st.ode.t1 = model_time(ion->model, t0 + dt);
-        }
+        if (ode0.rkf) model_step_times(ion->model, t0, dt, &st.ode.t0, &st.ode.t1);
         return st;
     };
     bool ionic_queued = false;  // this step's ionic launch was queued by the previous one

@@ -14,7 +14,7 @@ namespace cg {
 struct BuenoOrovio {
     static constexpr int NS = 4;   // u v w s
     static constexpr int NP = 28;  // constants, order of BuenoOrovio_2008Names.H
-    static constexpr double VSCALE = 85.7, VOFF = 84.0, ISCALE = 85.7;
+    static constexpr double VSCALE = 85.7, VOFF = 84.0, ISCALE = 85.7, VMUL = 1000.0;
     static constexpr double TSCALE = 1000.0;  // model ms per second
     enum { uo, uu, thv, thw, thvm, tho, tv1m, tv2m, tvp, tw1m, tw2m, kwm, uwm, twp, tfi,
            to1, to2, tso1, tso2, kso, uso, ts1, ts2, ks, us, tsi, twinf, winfstar };
@@ -103,7 +103,7 @@ struct BuenoOrovio {
 struct AlievPanfilov {
     static constexpr int NS = 2;
     static constexpr int NP = 8;
-    static constexpr double VSCALE = 100.0, VOFF = 80.0, ISCALE = 100.0;
+    static constexpr double VSCALE = 100.0, VOFF = 80.0, ISCALE = 100.0, VMUL = 1000.0;
     static constexpr double TSCALE = 1000.0 / 12.9;
     enum { Vp, Vr, Vth, eps0, kk, mu1, mu2, aa };
 

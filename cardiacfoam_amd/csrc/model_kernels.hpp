@@ -87,7 +87,7 @@ __global__ __launch_bounds__(MBS) void k_ionic_tissue(T* __restrict__ st, int64_
     }
     T y[M::NS];
     load_states(y, st, n, i, 1);
-    const T v = T((Vm[i] * 1000.0 + M::VOFF) / M::VSCALE);
+    const T v = T((Vm[i] * M::VMUL + M::VOFF) / M::VSCALE);
     const T I = tissue_advance<M, T, INTEG>(v, y, dt, p, tissue, gs, ode, i);
     store_states(y, st, n, i, 1);
     Im[i] = double(I) * M::ISCALE;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(MBS) void k_explicit_hex(
     T y[M::NS];
     load_states(y, st, n, c, 1);
     const double vn = V[c];
-    const T v = T((vn * 1000.0 + M::VOFF) / M::VSCALE);
+    const T v = T((vn * M::VMUL + M::VOFF) / M::VSCALE);
     const T I = tissue_advance<M, T, INTEG>(v, y, T(s.dt_model), p, tissue, one, s.ode, c);
     store_states(y, st, n, c, 1);
     const double iion = double(I) * M::ISCALE;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st
     // behind it
     const double vo = (x0 || s.c00 != 0.0 || s.act_flag) ? Vm1[c] : 0.0;
     const uint8_t fl = s.act_flag ? s.act_flag[c] : 0;
-    const T v = T((vn * 1000.0 + M::VOFF) / M::VSCALE);
+    const T v = T((vn * M::VMUL + M::VOFF) / M::VSCALE);
     const T I = tissue_advance<M, T, INTEG>(v, y, T(s.dt_model), p, tissue, one, s.ode, c);
     store_states(y, st, n, c, 1);
     const double iion = double(I) * M::ISCALE;
@@ -327,6 +327,7 @@ template <class T> struct ModelLaunch {
     case MODEL_TNNP: return ModelLaunch<T>::template FN<TNNP>(__VA_ARGS__);                \
     case MODEL_ORD: return ModelLaunch<T>::template FN<ORd>(__VA_ARGS__);                  \
     case MODEL_AP: return ModelLaunch<T>::template FN<AlievPanfilov>(__VA_ARGS__);         \
+    case MODEL_FDA: return ModelLaunch<T>::template FN<FdaManufactured>(__VA_ARGS__);      \
     default: return hipErrorInvalidValue;                                                  \
     }
 

@@ -18,7 +18,7 @@ namespace cg {
 struct ORd {
     static constexpr int NS = 41;
     static constexpr int NP = 15;
-    static constexpr double VSCALE = 1.0, VOFF = 0.0, ISCALE = 1.0, TSCALE = 1000.0;
+    static constexpr double VSCALE = 1.0, VOFF = 0.0, ISCALE = 1.0, TSCALE = 1000.0, VMUL = 1000.0;
     enum { V, CaMKt, Nai, Nass, Ki, Kss, Cass, Cansr, Cajsr, Cai, m, hf, hs, j, hsp, jp, mL,
            hL, hLp, a, iF, iS, ap, iFp, iSp, d, ff, fs, fcaf, fcas, jca, ffp, fcafp, nca,
            xrf, xrs, xs1, xs2, xk1, Jrelnp, Jrelp };

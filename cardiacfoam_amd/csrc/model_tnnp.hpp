@@ -12,7 +12,7 @@ namespace cg {
 struct TNNP {
     static constexpr int NS = 17;
     static constexpr int NP = 42;
-    static constexpr double VSCALE = 1.0, VOFF = 0.0, ISCALE = 1.0, TSCALE = 1000.0;
+    static constexpr double VSCALE = 1.0, VOFF = 0.0, ISCALE = 1.0, TSCALE = 1000.0, VMUL = 1000.0;
     enum { V, Ki, Nai, Cai, Xr1, Xr2, Xs, m, h, j, d, f, fCa, s, r, CaSR, g };
     enum { R, Tk, F, Cm, Vc, Pkna, Ko, Nao, Cao, gK1, gKr, gKs, gNa, gbNa, gCaL, gbCa, gto,
            PNaK, Kmk, KmNa, KNaCa, Ksat, alpha, gamma, KmCa, KmNai, gpCa, KpCa, gpK, taug,

@@ -12,10 +12,11 @@
 #include "model_bo.hpp"
 #include "model_tnnp.hpp"
 #include "model_ord.hpp"
+#include "model_fda.hpp"
 
 namespace cg {
 
-enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3 };
+enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3, MODEL_FDA = 4 };
 
 constexpr int MAX_PARAMS = 48;
 struct ModelParams {
@@ -131,6 +132,37 @@ template <> struct Ops<ORd> {
                              T istim, const ORd::VTerms<T>& q)
     {
         ORd::deriv(y, d, p, tissue, gs, solve_v, istim, q);
+    }
+};
+
+template <> struct Ops<FdaManufactured> {
+    template <class T>
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T*)
+    {
+        y[0] = v;
+        RlStep<T> out{y, dt};
+        FdaManufactured::rates(y, out, p);
+        return FdaManufactured::iion(y, p);
+    }
+    // dV/dt is 0 in this model (monodomainFDAManufactured_2014.H:68): a single cell only
+    // moves u1, u2
+    template <class T> CG_HD static void cell_step(T* y, T dt, T, const T* p, int, const T*)
+    {
+        RlStep<T> out{y, dt};
+        FdaManufactured::rates(y, out, p);
+    }
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T*)
+    {
+        return FdaManufactured::iion(y, p);
+    }
+    struct None {};
+    template <class T> CG_HD static None vt(T, const T*) { return None{}; }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int, const T*, bool, T, None)
+    {
+        Deriv<T> out{y, d};
+        d[0] = T(0);
+        FdaManufactured::rates(y, out, p);
     }
 };
 

@@ -14,13 +14,17 @@ import numpy as np
 from . import _capi
 from ._capi import check, lib
 
-MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov"]
+MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov", "monodomainFDAManufactured"]
 SUPPORTED_TISSUES = {
     "BuenoOrovio": ["endocardialCells", "mCells", "epicardialCells"],
     "TNNP": ["endocardialCells", "mCells", "epicardialCells"],
     "ORd": ["myocyte"],
     "AlievPanfilov": ["myocyte"],
+    # the manufactured model's supportedDimensions() (monodomainFDAManufactured.C:85-88)
+    "monodomainFDAManufactured": ["1D", "2D", "3D"],
 }
+# models selected by `dimension` instead of `tissue`
+DIMENSION_MODELS = ("monodomainFDAManufactured",)
 
 
 class FatalIOError(KeyError):
@@ -77,14 +81,17 @@ class IonicModel:
         ``precision`` (fp64 | mixed), an addition of this backend."""
         if "ionicModel" not in dict_:
             raise FatalIOError("keyword ionicModel is undefined")
-        if "tissue" not in dict_:
-            raise FatalIOError("Normal model requires 'tissue' entry")
         key = dict_["ionicModel"]
         if key not in MODEL_KEYS:
             raise FatalIOError("Unknown ionicModel type %s\nValid ionicModel types: %s"
                                % (key, MODEL_KEYS))
+        word_key = "dimension" if key in DIMENSION_MODELS else "tissue"
+        if word_key not in dict_:
+            raise FatalIOError("Normal model requires 'tissue' entry" if word_key == "tissue"
+                               else "Manufactured model requires 'dimension' entry")
         prec = precision or dict_.get("precision", "fp64")
-        m = IonicModel(key, dict_["tissue"], nIntegrationPoints, solveVmWithinODESolver, prec)
+        m = IonicModel(key, str(dict_[word_key]).strip('"'), nIntegrationPoints,
+                       solveVmWithinODESolver, prec)
         m.initialDeltaT = initialDeltaT
         solver = dict_.get("solver", "RushLarsen")
         # ODESolver::New keys (ionicModel.H:233-242); OpenFOAM's defaults when absent.

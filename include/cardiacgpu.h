@@ -40,9 +40,12 @@ enum {
 };
 
 /* model ids: the ionicModel runtime-selection keys */
-enum { CG_MODEL_BUENO_OROVIO = 0, CG_MODEL_TNNP = 1, CG_MODEL_ORD = 2, CG_MODEL_ALIEV_PANFILOV = 3 };
-/* tissue flags of ionicSelector::tissueFlag (ionicSelector.C:60-66) */
-enum { CG_TISSUE_EPI = 1, CG_TISSUE_MCELL = 2, CG_TISSUE_ENDO = 3, CG_TISSUE_MYOCYTE = 4 };
+enum { CG_MODEL_BUENO_OROVIO = 0, CG_MODEL_TNNP = 1, CG_MODEL_ORD = 2, CG_MODEL_ALIEV_PANFILOV = 3,
+       CG_MODEL_FDA_MANUFACTURED = 4 /* monodomainFDAManufactured (verification model) */ };
+/* tissue flags of ionicSelector::tissueFlag (ionicSelector.C:60-66); the manufactured
+ * model takes its `dimension` word instead ("1D", "2D", "3D") */
+enum { CG_TISSUE_EPI = 1, CG_TISSUE_MCELL = 2, CG_TISSUE_ENDO = 3, CG_TISSUE_MYOCYTE = 4,
+       CG_DIMENSION_1D = 11, CG_DIMENSION_2D = 12, CG_DIMENSION_3D = 13 };
 /* arithmetic: CG_FP64 = plain FP64 path; CG_MIXED = FP32 states/arithmetic, FP64 time,
  * reductions, residuals and activation times */
 enum { CG_FP64 = 0, CG_MIXED = 1 };
@@ -58,8 +61,9 @@ const char* cg_version(void);
 /* ------------------------------------------------------------- ionic model */
 typedef struct cg_ionic cg_ionic;
 
-/* Model lookup by ionicModel key ("BuenoOrovio", "TNNP", "ORd", "AlievPanfilov") and
- * tissue word ("epicardialCells", "mCells", "endocardialCells", "myocyte").
+/* Model lookup by ionicModel key ("BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
+ * "monodomainFDAManufactured") and tissue word ("epicardialCells", "mCells",
+ * "endocardialCells", "myocyte"; "1D", "2D", "3D" for the manufactured model).
  * Replaces ionicModel::New's table lookup (ionicModel.C:76-94) and
  * ionicSelector::tissueFlag (ionicSelector.C:60-66).  Return -1 when unknown. */
 int cg_model_id(const char* ionic_model_key);

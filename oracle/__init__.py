@@ -14,10 +14,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
 _lib = None
 
-BO, TNNP, ORD, AP = 0, 1, 2, 3
+BO, TNNP, ORD, AP, FDA = 0, 1, 2, 3, 4
 EPI, MCELL, ENDO, MYOCYTE = 1, 2, 3, 4
+DIM1, DIM2, DIM3 = 11, 12, 13          # dimension flags of the manufactured model
 MAX_STATES, MAX_CONSTS = 41, 160
-NSTATES = {BO: 4, TNNP: 17, ORD: 41, AP: 2}
+NSTATES = {BO: 4, TNNP: 17, ORD: 41, AP: 2, FDA: 4}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -353,6 +354,16 @@ class OracleTissue:
     @property
     def V(self):
         return self._arr(self.T.V, self.n)
+
+    def set_V(self, V, old=True):
+        """Vm (and, with old, its old levels) from an array of n values"""
+        for ptr in ((self.T.V, self.T.V0, self.T.V00) if old else (self.T.V,)):
+            np.ctypeslib.as_array(ptr, shape=(self.n,))[:] = V
+
+    def set_states(self, S):
+        """all states, [nstates, n]"""
+        np.ctypeslib.as_array(self.T.states, shape=(self.nstates * self.n,))[:] = \
+            np.asarray(S, dtype=np.float64).ravel()
 
     @property
     def solver_counts(self):

@@ -640,6 +640,31 @@ static void ap_eval(double t, const double* c, const double* y, int solve_vm,
     *iion = Iion; *istim = Ist;
 }
 
+/* ========================================= monodomainFDAManufactured (FDA)
+ * monodomainFDAManufactured_2014.H:40-80 (constants per dimension, rates, Iion) and
+ * monodomainFDAManufactured.C:100-147 (time in s, V = Vm, Im = Iion / Cm).  The states
+ * start from the manufactured fields of manufacturedFDAReference.H, set by the caller.
+ * tissue carries the dimension: ORC_DIM1/2/3. */
+static void fda_init(int dim, double* c, double* y)
+{
+    c[0] = 2.0;                                               /* Cm */
+    c[1] = dim == ORC_DIM1 ? -1.1 : dim == ORC_DIM2 ? -5.9 : -8.6;  /* Beta */
+    c[2] = 3.0;                                               /* Chi */
+    y[0] = y[1] = y[2] = y[3] = 0.0;
+}
+
+static void fda_eval(const double* c, const double* y, double* r, int* g, double* iion)
+{
+    const double V = y[0], u1 = y[1], u2 = y[2], u3 = y[3];
+    const double a = u1 + u3 - V;
+    r[0] = 0.0;
+    r[1] = pow(a, 2.0) * pow(u2, 2.0) + 0.5 * a * pow(u2, 2.0) * (V - u3);
+    r[2] = -a * pow(u2, 3.0);
+    r[3] = 0.0;
+    g[0] = g[1] = g[2] = g[3] = 0;
+    *iion = -0.5 * c[0] * a * pow(u2, 2.0) * (V - u3) + (c[1] * (V - u3)) / c[2];
+}
+
 /* ============================================================== dispatch */
 int orc_model_nstates(int model)
 {
@@ -648,6 +673,7 @@ int orc_model_nstates(int model)
     case ORC_TNNP: return TN_NS;
     case ORC_ORD: return OR_NS;
     case ORC_AP: return 2;
+    case ORC_FDA: return 4;
     default: return -1;
     }
 }
@@ -668,6 +694,9 @@ int orc_model_init(int model, int tissue, double* c, double* y)
     case ORC_AP:
         if (tissue != ORC_MYOCYTE) return -1;
         ap_init(c, y); return 0;
+    case ORC_FDA:
+        if (tissue < ORC_DIM1 || tissue > ORC_DIM3) return -1;
+        fda_init(tissue, c, y); return 0;
     default: return -1;
     }
 }
@@ -700,6 +729,7 @@ void orc_model_eval(int model, int tissue, double t, const double* c, const doub
     case ORC_TNNP: tnnp_eval(tissue, t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_ORD: ord_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_AP: ap_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_FDA: fda_eval(c, y, r, g, iion); *istim = 0.0; break;
     default: break;
     }
 }

@@ -152,6 +152,7 @@ static void model_scaling(int model, double* vscale, double* voff, double* tscal
     switch (model) {
     case ORC_BO: *vscale = 85.7; *voff = 84.0; *tscale = 1000.0; *iscale = 85.7; break;
     case ORC_AP: *vscale = 100.0; *voff = 80.0; *tscale = 1000.0 / 12.9; *iscale = 100.0; break;
+    case ORC_FDA: *vscale = 1.0; *voff = 0.0; *tscale = 1.0; *iscale = 1.0; break;
     default: *vscale = 1.0; *voff = 0.0; *tscale = 1000.0; *iscale = 1.0; break;
     }
 }
@@ -164,10 +165,14 @@ void orc_ionic_tissue_step(int model, int tissue, const double* consts, long c_s
     const int ns = orc_model_nstates(model);
     double vscale, voff, tscale, iscale;
     model_scaling(model, &vscale, &voff, &tscale, &iscale);
-    /* AlievPanfilov.C:107-108 scales both ends; the others scale start and end */
-    const double tStart = model == ORC_AP ? t0_s * 1000.0 / 12.9 : t0_s * 1000.0;
-    const double tEnd = model == ORC_AP ? (t0_s + dt_s) * 1000.0 / 12.9 : (t0_s + dt_s) * 1000.0;
-    const double dtModel = model == ORC_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+    /* AlievPanfilov.C:107-108 scales both ends; the others scale start and end; the
+     * manufactured model runs in seconds, tEnd = tStart + deltaT
+     * (monodomainFDAManufactured.C:107-108) */
+    const int fda = model == ORC_FDA;
+    const double tStart = fda ? t0_s : model == ORC_AP ? t0_s * 1000.0 / 12.9 : t0_s * 1000.0;
+    const double tEnd = fda ? tStart + dt_s
+                            : model == ORC_AP ? (t0_s + dt_s) * 1000.0 / 12.9 : (t0_s + dt_s) * 1000.0;
+    const double dtModel = fda ? dt_s : model == ORC_AP ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
     if (n_sub < 1) n_sub = 1;
     (void)tscale;
 
@@ -176,7 +181,7 @@ void orc_ionic_tissue_step(int model, int tissue, const double* consts, long c_s
         double y[ORC_MAX_STATES];
         const double* c = consts + (c_stride ? cell * c_stride : 0);
         for (int i = 0; i < ns; ++i) y[i] = states[(long)i * n + cell];
-        y[0] = vscale == 1.0 ? Vm[cell] * 1000.0 : (Vm[cell] * 1000.0 + voff) / vscale;
+        y[0] = fda ? Vm[cell] : vscale == 1.0 ? Vm[cell] * 1000.0 : (Vm[cell] * 1000.0 + voff) / vscale;
         if (integrator == 0) {
             const double h = dtModel / n_sub;
             for (int s = 0; s < n_sub; ++s)
@@ -190,7 +195,7 @@ void orc_ionic_tissue_step(int model, int tissue, const double* consts, long c_s
         double r[ORC_MAX_STATES], xi[ORC_MAX_STATES], ta[ORC_MAX_STATES], iion, ist;
         int g[ORC_MAX_STATES];
         orc_model_eval(model, tissue, tEnd, c, y, 0, NULL, r, g, xi, ta, &iion, &ist);
-        Im[cell] = iion * iscale;
+        Im[cell] = fda ? iion / c[0] : iion * iscale;  /* FDA: Iion / Cm */
         for (int i = 0; i < ns; ++i) states[(long)i * n + cell] = y[i];
     }
 }
