@@ -79,6 +79,10 @@ def lib():
                                           C.c_double]
     L.cg_ionic_step.argtypes = [vp, C.c_double, C.c_double, vp, vp, C.c_int]
     L.cg_ionic_run_cells.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+    L.cg_ionic_run_cells_trace.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int64, vp]
+    L.cg_format_scalars.argtypes = [vp, C.c_int64, C.c_int, C.c_int, C.c_int64, vp, C.c_int64,
+                                    C.POINTER(C.c_int64)]
+    L.cg_parse_scalars.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.cg_ionic_get_states.argtypes = [vp, C.c_int, C.c_int, vp]
     L.cg_ionic_set_states.argtypes = [vp, C.c_int, C.c_int, vp]
     L.cg_mesh_create_hex.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,

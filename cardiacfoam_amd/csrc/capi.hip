@@ -14,6 +14,7 @@
 #include "cg_implicit.hpp"
 #include "cg_adjoint.hpp"
 #include "cg_ldu.hpp"
+#include "foam_format.hpp"
 
 namespace {
 
@@ -156,6 +157,9 @@ struct cg_ionic {
     bool step_set = false;            // d_step holds the kept per-cell trial steps
     double* d_step = nullptr;         // [n]
     int* d_fail = nullptr;            // set by a cell whose integration failed
+    // single-cell trace buffer, [steps][states] in double
+    double* d_trace = nullptr;
+    size_t trace_cap = 0;
 };
 
 struct cg_mesh {
@@ -294,8 +298,6 @@ int cg_ionic_create(int model, int tissue, int64_t n, int solve_vm, int precisio
     if (n <= 0) return fail(CG_ERR_ARG, "n_cells must be positive");
     if (precision != CG_FP64 && precision != CG_MIXED)
         return fail(CG_ERR_ARG, "unknown precision %d", precision);
-    if (model == cg::MODEL_FDA && solve_vm)
-        return fail(CG_ERR_STATE, "monodomainFDAManufactured is a tissue model (dV/dt is 0)");
     double p[cg::MAX_PARAMS] = {0}, y0[64] = {0};
     if (!model_host_init(model, tissue, p, y0))
         return fail(CG_ERR_TISSUE, "Unsupported tissue flag %d for ionicModel %s", tissue,
@@ -341,7 +343,7 @@ int cg_ionic_destroy(cg_ionic* h)
     if (h->d_vm) hipFree(h->d_vm);
     if (h->d_im) hipFree(h->d_im);
     void* batch[] = {h->d_prot, h->d_len, h->d_order, h->d_det, h->d_ring, h->d_beats,
-                     h->d_nbeats, h->d_step, h->d_fail};
+                     h->d_nbeats, h->d_step, h->d_fail, h->d_trace};
     for (void* q : batch)
         if (q) hipFree(q);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -461,6 +463,35 @@ int cg_ionic_run_cells(cg_ionic* h, double t0, double dt, int n_steps)
                                   model_time(h->model, t0), dtm, n_steps, h->P, h->d_gscale,
                                   h->stim, ode, h->stream));
     return CG_OK;
+}
+
+int cg_ionic_run_cells_trace(cg_ionic* h, double t0, double dt, int n_steps, int64_t cell,
+                             double* trace)
+{
+    if (!h || !trace) return fail(CG_ERR_ARG, "NULL argument");
+    if (!h->solve_vm) return fail(CG_ERR_STATE, "cg_ionic_run_cells needs a single-cell handle");
+    if (n_steps < 0 || !(dt > 0.0)) return fail(CG_ERR_ARG, "bad step arguments");
+    if (cell < 0 || cell >= h->n) return fail(CG_ERR_ARG, "traced cell %lld out of range",
+                                              (long long)cell);
+    if (n_steps == 0) return CG_OK;
+    const size_t bytes = sizeof(double) * (size_t)n_steps * 2 * (size_t)h->mi.ns;
+    if (bytes > h->trace_cap) {
+        if (h->d_trace) hipFree(h->d_trace);
+        h->d_trace = nullptr;
+        h->trace_cap = 0;
+        if (hipMalloc(&h->d_trace, bytes) != hipSuccess)
+            return fail(CG_ERR_NOMEM, "trace allocation of %zu bytes failed", bytes);
+        h->trace_cap = bytes;
+    }
+    const double dtm = model_dt(h->model, dt);
+    cg::OdeArgs ode;
+    if (int rc = ode_args(h, t0, dt, &ode)) return rc;
+    HIPCHK(cg::launch_ionic_cells(h->model, h->prec, h->tissue, h->d_states, h->n,
+                                  model_time(h->model, t0), dtm, n_steps, h->P, h->d_gscale,
+                                  h->stim, ode, h->stream, h->d_trace, cell));
+    HIPCHK(hipMemcpyAsync(trace, h->d_trace, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return check_ode(h);
 }
 
 // ----------------------------------------------------------- batched protocols
@@ -1370,6 +1401,31 @@ int cg_tissue_apply_operator(cg_tissue* t, double ddt_coeff, int precision, cons
     const int rc = cg::implicit_apply(t->implicit, precision, ddt_coeff, x_dev, y_dev, prof);
     if (rc == 9) return fail(CG_ERR_STATE, "fp32 operator exists on a mixed-precision tissue only");
     if (rc != 0) return fail(CG_ERR_HIP, "operator launch failed");
+    return CG_OK;
+}
+
+// ---------------------------------------------------------------- ASCII formats
+int cg_format_scalars(const double* values, int64_t n, int precision, int fixed, int64_t row,
+                      char* buf, int64_t cap, int64_t* len)
+{
+    if (!len || (n > 0 && (!values || !buf))) return fail(CG_ERR_ARG, "NULL argument");
+    if (n < 0 || cap < 0 || row < 1) return fail(CG_ERR_ARG, "bad size");
+    if (precision < 0 || precision > 30) return fail(CG_ERR_ARG, "precision %d out of range",
+                                                     precision);
+    *len = 0;
+    if (!cg::format_scalars(values, n, precision, fixed != 0, row, buf, cap, len))
+        return fail(CG_ERR_ARG, "output buffer of %lld bytes is too small", (long long)cap);
+    return CG_OK;
+}
+
+int cg_parse_scalars(const char* text, int64_t len, double* out, int64_t n, int64_t* consumed)
+{
+    if (!consumed || (n > 0 && (!text || !out))) return fail(CG_ERR_ARG, "NULL argument");
+    if (n < 0 || len < 0) return fail(CG_ERR_ARG, "negative size");
+    *consumed = 0;
+    const int rc = cg::parse_scalars(text, len, out, n, consumed);
+    if (rc == 1) return fail(CG_ERR_ARG, "malformed number in list");
+    if (rc == 2) return fail(CG_ERR_ARG, "list holds fewer than %lld numbers", (long long)n);
     return CG_OK;
 }
 

@@ -78,12 +78,13 @@ struct ExplicitStep {
                                          const double* Vm, double* Im, double dt_model,    \
                                          const ModelParams& P, const void* gscale,         \
                                          const OdeArgs& ode, hipStream_t s);               \
-    /* n independent single cells (V inside the ODE), n_steps steps from model time t0 */  \
+    /* n independent single cells (V inside the ODE), n_steps steps from model time t0;  \
+     * with trace, cell tcell stores its states after every step, [n_steps][NS] */         \
     hipError_t launch_ionic_cells_##SFX(int model, int tissue, void* states, int64_t n,    \
                                         double t0, double dt, int n_steps,                 \
                                         const ModelParams& P, const void* gscale,          \
                                         const Stimulus& stim, const OdeArgs& ode,          \
-                                        hipStream_t s);                                    \
+                                        double* trace, int64_t tcell, hipStream_t s);      \
     /* fused explicit monodomain step on the hex block: ionic + diffusion + activation */  \
     hipError_t launch_explicit_hex_##SFX(int model, int tissue, const HexGeom& g,          \
                                          const HexCoefs& a, void* states, const double* V, \
@@ -156,11 +157,13 @@ inline hipError_t launch_ionic_tissue(int model, int prec, int tissue, void* sta
 inline hipError_t launch_ionic_cells(int model, int prec, int tissue, void* states, int64_t n,
                                      double t0_model, double dt_model, int n_steps,
                                      const ModelParams& P, const void* gscale,
-                                     const Stimulus& stim, const OdeArgs& ode, hipStream_t s)
+                                     const Stimulus& stim, const OdeArgs& ode, hipStream_t s,
+                                     double* trace = nullptr, int64_t tcell = -1)
 {
     if (n <= 0 || n_steps <= 0) return hipSuccess;
     return (prec ? launch_ionic_cells_fp32 : launch_ionic_cells_fp64)(
-        model, tissue, states, n, t0_model, dt_model, n_steps, P, gscale, stim, ode, s);
+        model, tissue, states, n, t0_model, dt_model, n_steps, P, gscale, stim, ode, trace,
+        tcell, s);
 }
 
 inline hipError_t launch_explicit_hex(int model, int prec, int tissue, const HexGeom& g,

@@ -99,10 +99,13 @@ template <class M, class T, int INTEG>
 __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t n, double t0,
                                                      T dt, int n_steps, KParams<T> P,
                                                      const T* __restrict__ gsc, int tissue,
-                                                     Stimulus stim, OdeArgs ode)
+                                                     Stimulus stim, OdeArgs ode,
+                                                     double* __restrict__ trace, int64_t tcell)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // the traced cell stores its states and rates after every step
+    const bool rec = trace != nullptr && i == tcell;
     const T* p = P.p;  // model constants read from the kernel arguments
     T gs[4] = {T(1), T(1), T(1), T(1)};
     if (gsc) {
@@ -111,10 +114,24 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
     }
     T y[M::NS];
     load_states(y, st, n, i, 0);
+    // trace row of the traced cell after a step ending at model time te: the states,
+    // then their rates at te (the STATES and RATES columns of the single-cell trace)
+    auto record = [&](int s, double te) __attribute__((always_inline)) {
+        T d[M::NS];
+        Ops<M>::deriv(y, d, p, tissue, gs, true, T(stimulus_current(te, stim)),
+                      Ops<M>::vt(y[0], p));
+        double* row = trace + (int64_t)s * 2 * M::NS;
+#pragma unroll
+        for (int k = 0; k < M::NS; ++k) {
+            row[k] = double(y[k]);
+            row[M::NS + k] = double(d[k]);
+        }
+    };
     if constexpr (INTEG == 0) {
         for (int s = 0; s < n_steps; ++s) {
             const double t = t0 + s * double(dt);
             Ops<M>::cell_step(y, dt, T(stimulus_current(t, stim)), p, tissue, gs);
+            if (rec) record(s, t0 + (s + 1) * double(dt));
         }
     } else {
         auto f = [&](double t, const T* yy, T* d) __attribute__((always_inline)) {
@@ -128,11 +145,16 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
             const double tn = tacc + ode.dt_s;
             const double ts = tn - ode.dt_s;
             tacc = tn;
-            const double x0 = ode.ap ? ts * 1000.0 / 12.9 : ts * 1000.0;
-            const double x1 = ode.ap ? (ts + ode.dt_s) * 1000.0 / 12.9 : (ts + ode.dt_s) * 1000.0;
+            // model time: ms (s for the manufactured model, tEnd = tStart + deltaT)
+            constexpr bool secs = M::TSCALE == 1.0;
+            const double x0 = secs ? ts : ode.ap ? ts * 1000.0 / 12.9 : ts * 1000.0;
+            const double x1 = secs      ? ts + ode.dt_s
+                              : ode.ap ? (ts + ode.dt_s) * 1000.0 / 12.9
+                                       : (ts + ode.dt_s) * 1000.0;
             if (!(step < ode.dt_clamp)) step = ode.dt_clamp;
             ok = rkf45_solve<M::NS>(f, y, x0, x1, step, T(ode.abs_tol), T(ode.rel_tol),
                                     ode.max_steps);
+            if (rec) record(s, x1);
         }
         ode.step[i] = step;
         if (!ok) *ode.fail = 1;
@@ -269,17 +291,18 @@ template <class T> struct ModelLaunch {
     template <class M>
     static hipError_t cells(int tissue, void* states, int64_t n, double t0, double dt,
                             int n_steps, const ModelParams& P, const void* gscale,
-                            const Stimulus& stim, const OdeArgs& ode, hipStream_t s)
+                            const Stimulus& stim, const OdeArgs& ode, double* trace,
+                            int64_t tcell, hipStream_t s)
     {
         const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
         if (ode.rkf)
             hipLaunchKernelGGL((k_ionic_cells<M, T, 1>), dim3(grid), dim3(MBS), 0, s,
                                (T*)states, n, t0, T(dt), n_steps, kparams<T>(P),
-                               (const T*)gscale, tissue, stim, ode);
+                               (const T*)gscale, tissue, stim, ode, trace, tcell);
         else
             hipLaunchKernelGGL((k_ionic_cells<M, T, 0>), dim3(grid), dim3(MBS), 0, s,
                                (T*)states, n, t0, T(dt), n_steps, kparams<T>(P),
-                               (const T*)gscale, tissue, stim, ode);
+                               (const T*)gscale, tissue, stim, ode, trace, tcell);
         return hipGetLastError();
     }
     template <class M>
@@ -344,10 +367,10 @@ template <class T> struct ModelLaunch {
                                         double t0, double dt, int n_steps,                 \
                                         const ModelParams& P, const void* gscale,          \
                                         const Stimulus& stim, const OdeArgs& ode,          \
-                                        hipStream_t s)                                     \
+                                        double* trace, int64_t tcell, hipStream_t s)       \
     {                                                                                      \
         CG_MODEL_SWITCH(T, cells, tissue, states, n, t0, dt, n_steps, P, gscale, stim,     \
-                        ode, s)                                                            \
+                        ode, trace, tcell, s)                                              \
     }                                                                                      \
     hipError_t launch_explicit_hex_##SFX(int model, int tissue, const HexGeom& g,          \
                                          const HexCoefs& a, void* states, const double* V, \

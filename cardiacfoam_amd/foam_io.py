@@ -1,0 +1,390 @@
+"""On-disk formats of the reference's tissue and single-cell runs, written and read on
+the host so that existing post-processing and the probe-based regression checks consume
+them unchanged.
+
+- OpenFOAM ASCII field files: the volScalarFields MyocardiumDomain::write emits (Vm,
+  externalStimulusCurrent, ionicCurrent, activationTime; myocardiumDomain.C:196-251,
+  581-634), the same fields read back as initial conditions (READ_IF_PRESENT), and the
+  volTensorField `conductivity` of monodomainSolver.C:48-100.
+- Probe files of OpenFOAM's `probes` function object (as configured by
+  tutorials/regressionTests/NiedererEtAl2012/system/Niedererpoints): one file per field
+  under postProcessing/<name>/<startTime>/, '#' header lines, then a row per write time.
+- The single-cell trace of SingleCellSolver (singleCellSolver.C:98-120, 218-221):
+  postProcessing/<ionicModel>_<tissueName>_<protocolSuffix>.txt, a "time ..." header and
+  fixed-point rows with 7 decimals (ionicModelIO.C:27-85, 244-318; stimulusIO.C:392-423).
+
+Numbers are formatted and parsed by the native library (cg_format_scalars,
+cg_parse_scalars); this module lays out the files around them.
+"""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+
+from ._capi import check, lib
+from .foam_dict import _Parser, _tokens, parse as parse_dictionary
+from .ionic import FatalIOError
+
+# dimension sets of the fields the reference writes (myocardiumDomain.C:207-249,
+# monodomainSolver.C:66-70): [kg m s K mol A cd]
+DIMENSIONS = {
+    "Vm": (1, 2, -3, 0, 0, -1, 0),                       # dimVoltage
+    "externalStimulusCurrent": (0, -3, 0, 0, 0, 1, 0),   # dimCurrent/dimVolume
+    "ionicCurrent": (1, 2, -4, 0, 0, -1, 0),             # dimVoltage/dimTime
+    "activationTime": (0, 0, 1, 0, 0, 0, 0),             # dimTime
+    "conductivity": (-1, -3, 3, 0, 0, 2, 0),             # S/m
+}
+# components per cell of each field class
+_NCOMP = {"scalar": 1, "vector": 3, "symmTensor": 6, "tensor": 9, "sphericalTensor": 1}
+_CLASS_TYPE = {"volScalarField": "scalar", "volVectorField": "vector",
+               "volSymmTensorField": "symmTensor", "volTensorField": "tensor"}
+SMALL = 1e-15  # OpenFOAM's SMALL for double precision
+
+
+# --------------------------------------------------------------------- numbers
+def format_numbers(values, precision=6, fixed=False, row=1):
+    """bytes of the values as std::ostream writes them (general: %g with `precision`
+    significant digits; fixed: `precision` decimals), `row` values per line."""
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    n = v.size
+    if n == 0:
+        return b""
+    if fixed:
+        finite = v[np.isfinite(v)]
+        big = float(np.max(np.abs(finite))) if finite.size else 0.0
+        digits = 1 if big < 1.0 else int(math.log10(big)) + 2
+        per = 2 + digits + 1 + precision + 1
+    else:
+        per = 8 + max(precision, 1) + 8
+    cap = n * per + 64
+    buf = C.create_string_buffer(cap)
+    used = C.c_int64()
+    check(lib().cg_format_scalars(v.ctypes.data_as(C.c_void_p), n, int(precision), int(fixed),
+                                  int(row), buf, cap, C.byref(used)))
+    return buf.raw[:used.value]
+
+
+def parse_numbers(data, n, offset=0):
+    """n numbers from data[offset:] (whitespace and '(' ')' separate them) -> (array,
+    offset just past the last one)."""
+    out = np.empty(n, dtype=np.float64)
+    if n == 0:
+        return out, offset
+    view = (C.c_char * (len(data) - offset)).from_buffer_copy(data, offset) \
+        if not isinstance(data, bytearray) else (C.c_char * (len(data) - offset)).from_buffer(data, offset)
+    used = C.c_int64()
+    check(lib().cg_parse_scalars(view, len(data) - offset, out.ctypes.data_as(C.c_void_p), n,
+                                 C.byref(used)))
+    return out, offset + used.value
+
+
+def time_name(t, precision=6):
+    """Time::timeName with timeFormat general: the %g form with timePrecision digits."""
+    return "%.*g" % (precision, t)
+
+
+def _g(x, precision=6):
+    return "%.*g" % (precision, x)
+
+
+# ----------------------------------------------------------------- field files
+_BANNER = b"""/*--------------------------------*- C++ -*----------------------------------*\\
+  written by cardiacfoam_amd (MI355X monodomain step) in OpenFOAM's ASCII format
+\\*---------------------------------------------------------------------------*/
+"""
+
+
+def _header(cls, location, obj):
+    return ("FoamFile\n{\n    version     2.0;\n    format      ascii;\n"
+            "    class       %s;\n    location    \"%s\";\n    object      %s;\n}\n"
+            "// * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * //"
+            "\n\n" % (cls, location, obj)).encode()
+
+
+def _dims(d):
+    return ("[" + " ".join("%d" % int(x) for x in d) + "]").encode()
+
+
+def write_field(path, values, name=None, cls=None, dimensions=None, location="0",
+                patches=None, precision=6):
+    """Writes an OpenFOAM ASCII vol field file.  values: [n] (scalar) or [n, 3|6|9];
+    identical values are written `uniform`, as Field::writeEntry does.  patches: name ->
+    type of each boundary patch (default one zeroGradient patch `walls`, the hex block's
+    single patch in the tissue tutorials)."""
+    v = np.asarray(values, dtype=np.float64)
+    name = name or os.path.basename(path)
+    ncomp = 1 if v.ndim == 1 else v.shape[1]
+    vtype = {1: "scalar", 3: "vector", 6: "symmTensor", 9: "tensor"}.get(ncomp)
+    if vtype is None:
+        raise ValueError("values must be [n] or [n, 3|6|9]")
+    cls = cls or {"scalar": "volScalarField", "vector": "volVectorField",
+                  "symmTensor": "volSymmTensorField", "tensor": "volTensorField"}[vtype]
+    dims = dimensions if dimensions is not None else DIMENSIONS.get(name, (0,) * 7)
+    patches = patches if patches is not None else {"walls": "zeroGradient"}
+    parts = [_BANNER, _header(cls, location, name), b"dimensions      ", _dims(dims), b";\n\n"]
+    n = v.shape[0]
+    flat = v.reshape(n, ncomp)
+    if n > 0 and np.all(flat == flat[0]):
+        one = format_numbers(flat[0], precision, row=ncomp).strip()
+        parts += [b"internalField   uniform ", one if ncomp == 1 else b"(" + one + b")", b";\n\n"]
+    else:
+        parts += [b"internalField   nonuniform List<", vtype.encode(), b"> \n", b"%d\n(\n" % n]
+        body = format_numbers(flat, precision, row=ncomp)
+        if ncomp > 1:
+            # every line is one (x y z ...) entry
+            body = b"(" + body[:-1].replace(b"\n", b")\n(") + b")\n"
+        parts += [body, b")\n;\n\n"]
+    parts.append(b"boundaryField\n{\n")
+    for p, t in patches.items():
+        parts.append(b"    %s\n    {\n        type            %s;\n    }\n" % (p.encode(), t.encode()))
+    parts.append(b"}\n\n\n// ************************************************************************* //\n")
+    d = os.path.dirname(path)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    with open(path, "wb") as f:
+        f.write(b"".join(parts))
+    return path
+
+
+class FoamField:
+    """A field file read back: class, object name, dimensions, values ([n] or [n, k], or
+    the single `uniform` value with n None) and the boundaryField dictionary."""
+
+    def __init__(self, cls, name, dimensions, values, uniform, boundary):
+        self.cls, self.name, self.dimensions = cls, name, dimensions
+        self.values, self.uniform, self.boundary = values, uniform, boundary
+
+    def expand(self, n):
+        """values for n cells (a uniform value repeated)."""
+        if not self.uniform:
+            if len(self.values) != n:
+                raise FatalIOError("field %s has %d values, mesh has %d cells"
+                                   % (self.name, len(self.values), n))
+            return self.values
+        v = np.asarray(self.values, dtype=np.float64)
+        return np.tile(v, (n, 1)) if v.ndim else np.full(n, float(v))
+
+
+_INTERNAL = re.compile(rb"\binternalField\s+")
+_NONUNIFORM = re.compile(rb"nonuniform\s+List<(\w+)>\s*(\d+)\s*\(")
+_DIMS = re.compile(r"\bdimensions\s*\[([^\]]*)\]")
+
+
+def read_field(path):
+    """Parses an OpenFOAM ASCII vol field file (the subset OpenFOAM writes for the cell
+    fields above).  Binary files are rejected."""
+    with open(path, "rb") as f:
+        data = f.read()
+    m = _INTERNAL.search(data)
+    if not m:
+        raise FatalIOError("%s: no internalField entry" % path)
+    head_text = data[:m.start()].decode("latin-1")
+    ff = _Parser(_tokens(head_text)).parse_dict().get("FoamFile", {})
+    if str(ff.get("format", "ascii")) != "ascii":
+        raise FatalIOError("%s: only ascii field files are read" % path)
+    cls = str(ff.get("class", ""))
+    vtype = _CLASS_TYPE.get(cls)
+    if vtype is None:
+        raise FatalIOError("%s: unsupported field class %s" % (path, cls))
+    ncomp = _NCOMP[vtype]
+    dm = _DIMS.search(head_text)
+    dims = [int(float(x)) for x in dm.group(1).split()] if dm else None
+    pos = m.end()
+    if data.startswith(b"uniform", pos):
+        end = data.index(b";", pos)
+        vals, _ = parse_numbers(data[pos + 7:end], ncomp)
+        values = vals if ncomp > 1 else vals[0]
+        uniform = True
+        pos = end + 1
+    else:
+        mm = _NONUNIFORM.match(data, pos)
+        if not mm:
+            raise FatalIOError("%s: malformed internalField" % path)
+        if mm.group(1).decode() != vtype:
+            raise FatalIOError("%s: List<%s> in a %s" % (path, mm.group(1).decode(), cls))
+        n = int(mm.group(2))
+        vals, pos = parse_numbers(data, n * ncomp, mm.end())
+        close = data.index(b";", pos)
+        tail = data[pos:close]
+        # the last entry's ')' (vector, tensor) and the list's ')', nothing else
+        if tail.count(b")") != (2 if ncomp > 1 else 1) or tail.strip(b" \t\r\n)"):
+            raise FatalIOError("%s: internalField list is not closed after %d entries"
+                               % (path, n))
+        values = vals.reshape(n, ncomp) if ncomp > 1 else vals
+        uniform = False
+        pos = close + 1
+    rest = parse_dictionary(data[pos:].decode("latin-1"))
+    return FoamField(cls, str(ff.get("object", os.path.basename(path))), dims, values,
+                     uniform, rest.get("boundaryField", {}))
+
+
+# ---------------------------------------------------------------------- probes
+class Probes:
+    """OpenFOAM's `probes` function object on the hex block (blockMesh order
+    c = i + nx (j + ny k), spacing h): each location is sampled in the cell that contains
+    it, a point on a shared face going to the cell above it and a point on the far wall to
+    the last cell.  Files: <out_dir>/<name>/<start_time>/<field>."""
+
+    def __init__(self, shape, h, locations, fields, out_dir, name="probes", start_time="0",
+                 precision=6):
+        self.shape = tuple(int(s) for s in shape)
+        self.h = float(h)
+        self.locations = np.asarray(locations, dtype=np.float64).reshape(-1, 3)
+        self.fields = list(fields)
+        self.precision = precision
+        self.dir = os.path.join(out_dir, name, str(start_time))
+        self.cells = self.cell_indices()
+        self.paths = {}
+        os.makedirs(self.dir, exist_ok=True)
+        for f in self.fields:
+            p = os.path.join(self.dir, f)
+            with open(p, "w") as fh:
+                fh.write(self._header())
+            self.paths[f] = p
+
+    def cell_indices(self):
+        nx, ny, nz = self.shape
+        ext = np.array([nx, ny, nz]) * self.h
+        tol = 1e-9 * ext
+        if np.any(self.locations < -tol) or np.any(self.locations > ext + tol):
+            raise FatalIOError("probe location outside the mesh bounding box")
+        ijk = np.floor(self.locations / self.h).astype(np.int64)
+        ijk = np.clip(ijk, 0, np.array([nx, ny, nz]) - 1)
+        return ijk[:, 0] + nx * (ijk[:, 1] + ny * ijk[:, 2])
+
+    def _header(self):
+        w = self.precision + 7
+        lines = ["# Probe %d (%s)\n" % (i, " ".join(_g(x, self.precision) for x in p))
+                 for i, p in enumerate(self.locations)]
+        lines.append("# %*s" % (w - 2, "Probe") +
+                     "".join(" %*d" % (w, i) for i in range(len(self.locations))) + "\n")
+        lines.append("# %*s\n" % (w - 2, "Time"))
+        return "".join(lines)
+
+    def write(self, t, fields):
+        """One row per field file: time, then the field in each probe's cell.  fields:
+        name -> [n_cells] array (or a callable returning one)."""
+        w = self.precision + 7
+        for f in self.fields:
+            v = fields[f]() if callable(fields[f]) else fields[f]
+            vals = np.asarray(v)[self.cells]
+            row = "%*s" % (w, _g(t, self.precision)) + "".join(
+                " %*s" % (w, _g(x, self.precision)) for x in vals) + "\n"
+            with open(self.paths[f], "a") as fh:
+                fh.write(row)
+
+
+# ------------------------------------------------------------ single-cell trace
+def protocol_suffix(coeffs):
+    """stimulusIO::protocolSuffix (stimulusIO.C:392-423): noStim, S1_<period> or
+    S1_<period>_S2_<period>, periods written as Foam::name does (%g)."""
+    d = coeffs.get("singleCellStimulus") if coeffs else None
+    if not d:
+        return "noStim"
+    s1 = float(d.get("stim_period_S1", 0.0))
+    n1 = int(d.get("nstim1", 1 if s1 > SMALL else 0))
+    n2 = int(d.get("nstim2", 0))
+    if s1 <= SMALL or n1 <= 0:
+        return "noStim"
+    out = "S1_" + _g(s1)
+    if n2 > 0:
+        out += "_S2_" + _g(float(d.get("stim_period_S2", 0.0)))
+    return out
+
+
+def _canonical(name):
+    """ionicVariableCompatibility.C canonicalName: lower case, no '_', '-' or ' ', and an
+    `av` prefix dropped."""
+    out = "".join(c for c in name.lower() if c not in "_- ")
+    return out[2:] if len(out) > 2 and out.startswith("av") else out
+
+
+# Vm of the trace from the model's first state (the models' ioVmTransform:
+# BuenoOrovio.H:78-81, AlievPanfilov.H:78-81; TNNP and ORd store mV)
+VM_TRANSFORM = {"BuenoOrovio": (85.7, -84.0), "AlievPanfilov": (100.0, -80.0)}
+
+
+def trace_plan(model_key, state_names, export=None):
+    """Columns of the trace as (header name, source, index), source "vm", "state" or
+    "rate".  With an export list, ionicVariableCompatibility::resolveVariable
+    (ionicVariableCompatibility.C:213-279) maps each name; without one, the full plan of
+    ionicModelIO.C:170-225 minus its ALGEBRAIC block.  Algebraic variables are not
+    exported by this backend: a name that is neither Vm, a state nor RATES_<state> is an
+    error here (the reference writes algebraic names and ignores unknown ones)."""
+    ns = len(state_names)
+    if not export:
+        cols = [("Vm", "vm", 0)]
+        cols += [(state_names[s], "state", s) for s in range(1, ns)]
+        cols += [("RATES_" + state_names[s], "rate", s) for s in range(ns)]
+        return cols
+    cols = []
+    for name in export:
+        name = str(name)
+        if name == "Vm" or _canonical(name) == "vm":
+            cols.append((name, "vm", 0))
+            continue
+        s = next((i for i, sn in enumerate(state_names)
+                  if name == sn or _canonical(name) == _canonical(sn)), None)
+        if s is not None:
+            cols.append((name, "state", s))
+            continue
+        if len(name) > 6 and name[:6].upper() == "RATES_":
+            rs = name[6:]
+            s = next((i for i, sn in enumerate(state_names)
+                      if rs == sn or _canonical(rs) == _canonical(sn)), None)
+            if s is not None:
+                cols.append((name, "rate", s))
+                continue
+        raise FatalIOError("export variable '%s' is not Vm, a state or RATES_<state> of %s "
+                           "(algebraic variables are not exported by this backend)"
+                           % (name, model_key))
+    return cols
+
+
+class SingleCellTrace:
+    """Trace file of SingleCellSolver: postProcessing/<model>_<tissue>_<suffix>.txt in
+    case_dir, header "time <names>", then one fixed-point row (7 decimals) per step."""
+
+    def __init__(self, case_dir, model_key, tissue_name, coeffs, state_names):
+        export = None
+        out = (coeffs or {}).get("outputVariables", {})
+        if isinstance(out, dict) and isinstance(out.get("ionic"), dict):
+            export = out["ionic"].get("export")
+        self.plan = trace_plan(model_key, state_names, export)
+        self.vm = VM_TRANSFORM.get(model_key, (1.0, 0.0))
+        d = os.path.join(case_dir, "postProcessing")
+        os.makedirs(d, exist_ok=True)
+        self.path = os.path.join(d, "%s_%s_%s.txt" % (model_key, tissue_name,
+                                                      protocol_suffix(coeffs)))
+        with open(self.path, "w") as f:
+            f.write("time" + "".join(" " + c[0] for c in self.plan) + "\n")
+
+    def columns(self, states, rates):
+        """[steps, ncols] of the plan from the device trace's states and rates."""
+        out = np.empty((states.shape[0], len(self.plan)))
+        a, b = self.vm
+        for k, (_, src, i) in enumerate(self.plan):
+            if src == "vm":
+                out[:, k] = states[:, 0] * a + b
+            elif src == "state":
+                out[:, k] = states[:, i]
+            else:
+                out[:, k] = rates[:, i]
+        return out
+
+    def append(self, t, states, rates):
+        rows = np.column_stack([np.asarray(t, dtype=np.float64), self.columns(states, rates)])
+        with open(self.path, "ab") as f:
+            f.write(format_numbers(rows, 7, fixed=True, row=rows.shape[1]))
+
+
+def read_trace(path):
+    """(names, [rows, ncols] array) of a trace file (header line, then numbers)."""
+    with open(path, "rb") as f:
+        names = f.readline().decode().split()
+        data = f.read()
+    n = len(data.split())
+    vals, _ = parse_numbers(data, n)
+    return names, vals.reshape(-1, len(names))

@@ -140,6 +140,14 @@ class IonicModel:
     def run_cells(self, t0, dt, n_steps):
         check(lib().cg_ionic_run_cells(self.h, float(t0), float(dt), int(n_steps)))
 
+    def run_cells_trace(self, t0, dt, n_steps, cell=0):
+        """run_cells that returns cell `cell` after every step: (states, rates), each
+        [n_steps, nstates] in model units (RATES at the step's end time)."""
+        tr = np.empty((int(n_steps), 2, self.nstates))
+        check(lib().cg_ionic_run_cells_trace(self.h, float(t0), float(dt), int(n_steps),
+                                             int(cell), tr.ctypes.data_as(C.c_void_p)))
+        return tr[:, 0, :], tr[:, 1, :]
+
     def set_stimulus_protocol(self, d):
         """singleCellStimulus sub-dictionary (stimulusIO.C:142-194)."""
         keys = ("stim_start", "stim_period_S1", "stim_duration", "stim_amplitude")

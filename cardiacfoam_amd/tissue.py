@@ -7,6 +7,7 @@ electroProperties keys as MyocardiumDomain / MonodomainSolver
 (myocardiumDomain.C:180-277, monodomainSolver.C:48-100, stimulusIO.C:305-390).
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -61,6 +62,19 @@ class HexMesh:
             check(lib().cg_mesh_set_conductivity(self.m, _dp(s), 0))
         self._sigma = s
 
+    def read_conductivity(self, path):
+        """Per-cell conductivity from a volTensorField file (the `conductivity` field that
+        monodomainSolver.C:48-100 reads when it is present); True if the file exists."""
+        from . import foam_io
+        if not os.path.exists(path):
+            return False
+        f = foam_io.read_field(path)
+        if f.cls != "volTensorField":
+            raise FatalIOError("%s: conductivity must be a volTensorField" % path)
+        v = f.expand(self.n)
+        self.set_conductivity(v if not f.uniform else v[0])
+        return True
+
     def explicit_dt(self, chi, cm, max_co):
         dt = C.c_double()
         check(lib().cg_mesh_explicit_dt(self.m, chi, cm, max_co, C.byref(dt)))
@@ -98,13 +112,18 @@ class Monodomain:
     """Device monodomain tissue: MyocardiumDomain state + MonodomainSolver operator."""
 
     def __init__(self, mesh, coeffs, dt=None, max_co=None, precision=None, tol=1e-8,
-                 max_iter=2000, ddt=None, vm0=-0.084):
+                 max_iter=2000, ddt=None, vm0=-0.084, case_dir=None, start_time="0"):
+        """case_dir: an OpenFOAM case whose <start_time>/ directory may hold the fields the
+        reference reads if present: conductivity (monodomainSolver.C:54-92, taking
+        precedence over the dictionary's value), Vm and activationTime."""
         self.mesh = mesh
         self.coeffs = coeffs
-        if "conductivity" not in coeffs:
-            raise FatalIOError("keyword conductivity is undefined")
-        cond = coeffs["conductivity"]
-        mesh.set_conductivity(cond if not isinstance(cond, np.ndarray) else cond)
+        cpath = os.path.join(case_dir, start_time, "conductivity") if case_dir else None
+        if not (cpath and mesh.read_conductivity(cpath)):
+            if "conductivity" not in coeffs:
+                raise FatalIOError("keyword conductivity is undefined")
+            cond = coeffs["conductivity"]
+            mesh.set_conductivity(cond if not isinstance(cond, np.ndarray) else cond)
         self.chi = float(coeffs["chi"])
         self.cm = float(coeffs["cm"])
         alg = coeffs.get("solutionAlgorithm", "implicit")
@@ -132,6 +151,8 @@ class Monodomain:
         stim = coeffs.get("externalStimulus")
         if stim:
             self.set_external_stimulus(stim)
+        if case_dir:
+            self.read_fields(case_dir, start_time)
 
     def __del__(self):
         if getattr(self, "t", None):
@@ -250,6 +271,56 @@ class Monodomain:
     def set_fv_solution(self, fv_solution, field="Vm"):
         """Applies fv_solution_controls(fv_solution, field)."""
         self.set_solver_controls(**fv_solution_controls(fv_solution, field))
+
+    # ------------------------------------------------------------ on-disk fields
+    def external_stimulus_current(self):
+        """sourceField_ of the last step (myocardiumDomain.C:307-342), A/m^3: the summed
+        intensities of the boxes active at that step's start time t0 (inclusive ends)
+        whose closed extent holds the cell centre; zero before the first step."""
+        t, steps = self.time
+        out = np.zeros(self.mesh.n)
+        boxes = getattr(self, "_boxes", None)
+        if steps == 0 or boxes is None:
+            return out
+        t0 = (t + 0.0) - self.dt
+        lo, hi, ts, dur, inten = boxes
+        c = self.mesh.cell_centres()
+        for b in range(len(ts)):
+            if t0 < ts[b] or t0 > ts[b] + dur[b]:
+                continue
+            inside = np.all((c >= lo[3 * b:3 * b + 3]) & (c <= hi[3 * b:3 * b + 3]), axis=1)
+            out[inside] += inten[b]
+        return out
+
+    def write_fields(self, case_dir, precision=6, time_name=None):
+        """MyocardiumDomain::write (myocardiumDomain.C:621-634): Vm,
+        externalStimulusCurrent, ionicCurrent and activationTime as OpenFOAM ASCII
+        volScalarFields in <case_dir>/<timeName>/.  Returns that directory."""
+        from . import foam_io
+        tn = time_name or foam_io.time_name(self.time[0])
+        d = os.path.join(case_dir, tn)
+        for name, v in (("Vm", self.Vm), ("externalStimulusCurrent",
+                                          self.external_stimulus_current()),
+                        ("ionicCurrent", self.Iion), ("activationTime", self.activationTime)):
+            foam_io.write_field(os.path.join(d, name), v, location=tn, precision=precision)
+        return d
+
+    def read_fields(self, case_dir, time_name="0"):
+        """The READ_IF_PRESENT fields of MyocardiumDomain (myocardiumDomain.C:196-251) from
+        <case_dir>/<timeName>/: Vm (also the old level, so a first backward step starts
+        from it) and activationTime.  Returns the names read."""
+        from . import foam_io
+        got = []
+        for name, field in (("Vm", _capi.FIELD_VM), ("activationTime", _capi.FIELD_ACTIVATION)):
+            path = os.path.join(case_dir, time_name, name)
+            if not os.path.exists(path):
+                continue
+            v = foam_io.read_field(path).expand(self.mesh.n)
+            self.set_field(field, v)
+            if field == _capi.FIELD_VM:
+                self.set_field(_capi.FIELD_VM_OLD, v)
+            got.append(name)
+        return got
 
     def solver_report(self):
         """last step: (linear solves, initial residual of the last corrector, final residual)"""

@@ -130,6 +130,13 @@ int cg_ionic_step(cg_ionic* h, double t0_s, double dt_s, const double* Vm, doubl
  * the states kept in registers; RKF45 evaluates the stimulus at every stage time.
  * Asynchronous; cg_ionic_get_states synchronises. */
 int cg_ionic_run_cells(cg_ionic* h, double t0_s, double dt_s, int n_steps);
+/* cg_ionic_run_cells that also records one cell after every step, the rows
+ * SingleCellSolver::evolve writes to its trace file (singleCellSolver.C:218-221,
+ * ionicModelIO.C:244-318): trace [n_steps][2][n_states] host doubles, the states and then
+ * their rates (RATES, with the stimulus at the step's end time), in model units and state
+ * order.  Synchronous; reports an RKF45 failure like cg_ionic_get_states. */
+int cg_ionic_run_cells_trace(cg_ionic* h, double t0_s, double dt_s, int n_steps,
+                             int64_t cell, double* trace);
 
 /* ---------------------------------------------------- batched single cells
  * Thousands to millions of independent single cells (solve_vm = 1 handles), each with its
@@ -338,6 +345,24 @@ int cg_adjoint_backward(cg_adjoint* a, const double* dprobes_dev, double* dkappa
  * when profiling is on. */
 int cg_tissue_apply_operator(cg_tissue* t, double ddt_coeff, int precision, const void* x_dev,
                              void* y_dev);
+
+/* ------------------------------------------------------ on-disk ASCII formats
+ * Numbers of OpenFOAM's ASCII field files (the internalField List<scalar> that
+ * volScalarField::write emits for Vm, activationTime, ionicCurrent and
+ * externalStimulusCurrent, myocardiumDomain.C:581-634, and the volTensorField
+ * conductivity that monodomainSolver.C:48-100 reads) and of the single-cell trace
+ * (ionicModelIO.C:27-85, fixed with 7 decimals per singleCellSolver.C:117-118). */
+/* values in the std::ostream form, general with `precision` significant digits (printf
+ * %g) or, when fixed != 0, fixed with `precision` decimals; `row` values to a line,
+ * separated by one space, each line ended by a newline (row 1: one value per line, the
+ * List<scalar> body).  buf has cap bytes; *len gets the bytes written.  CG_ERR_ARG if cap
+ * is too small.  Host memory; several threads format slices of a large list. */
+int cg_format_scalars(const double* values, int64_t n, int precision, int fixed, int64_t row,
+                      char* buf, int64_t cap, int64_t* len);
+/* n numbers from text[0, len): whitespace and the brackets '(' ')' separate them, so a
+ * List<scalar>, List<vector> or List<tensor> body parses in one call.  *consumed gets the
+ * offset just past the n-th number.  CG_ERR_ARG on a malformed number or too few. */
+int cg_parse_scalars(const char* text, int64_t len, double* out, int64_t n, int64_t* consumed);
 
 #ifdef __cplusplus
 }

@@ -209,12 +209,17 @@ int orc_single_cell_run(int model, int tissue, const double* c, double* y,
     double t = 0.0;
     double step = dt_s;
     const int ap = model == ORC_AP;
+    /* the manufactured model runs in seconds with tEnd = tStart + deltaT
+     * (monodomainFDAManufactured.C:107-108); its dV/dt is 0, so a single cell moves u1, u2
+     * at frozen V (tutorials/manufacturedSolutions/singleCellMonodomain) */
+    const int fda = model == ORC_FDA;
     for (long k = 0; k < n_steps; ++k) {
         t = t + dt_s;
         const double t0 = t - dt_s;
-        const double tStart = ap ? t0 * 1000.0 / 12.9 : t0 * 1000.0;
-        const double tEnd = ap ? (t0 + dt_s) * 1000.0 / 12.9 : (t0 + dt_s) * 1000.0;
-        const double dtModel = ap ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
+        const double tStart = fda ? t0 : ap ? t0 * 1000.0 / 12.9 : t0 * 1000.0;
+        const double tEnd = fda ? t0 + dt_s
+                                : ap ? (t0 + dt_s) * 1000.0 / 12.9 : (t0 + dt_s) * 1000.0;
+        const double dtModel = fda ? dt_s : ap ? dt_s * 1000.0 / 12.9 : dt_s * 1000.0;
         if (integrator == 0) {
             orc_rl_step(model, tissue, tStart, dtModel, c, y, 1, st);
         } else {

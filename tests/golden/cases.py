@@ -11,6 +11,17 @@ NIEDERER:    tutorials/regressionTests/NiedererEtAl2012/constant/electroProperti
 """
 
 SINGLE_CELL = dict(
+    # singleCellSolverCoeffs of the case (ionicModel, tissue, integrator, stimulus, the
+    # outputVariables export list) and the trace file singleCell.reference:2-6 names
+    coeffs={"ionicModel": "BuenoOrovio", "tissue": "endocardialCells",
+            "solutionAlgorithm": "explicit", "solver": "RKF45", "initialODEStep": 1e-5,
+            "maxSteps": 1000000000, "absTol": 1e-6, "relTol": 1e-4,
+            "singleCellStimulus": {"stim_start": 20, "stim_duration": 1,
+                                   "stim_amplitude": 0.4, "stim_period_S1": 1000,
+                                   "nstim1": 3, "stim_period_S2": 0, "nstim2": 0},
+            "writeAfterTime": 0, "outputVariables": {"ionic": {"export": ["Vm", "s"],
+                                                              "debug": ["Vm"]}}},
+    trace_file="BuenoOrovio_endocardialCells_S1_1000.txt",
     stimulus=dict(start=20.0, period_s1=1000.0, duration=1.0, amplitude=0.4, nstim1=3,
                   period_s2=0.0, nstim2=0),
     dt=5e-5, n_steps=20000, abs_tol=1e-6, rel_tol=1e-4,
@@ -40,4 +51,17 @@ NIEDERER = dict(
         dict(name="p7", point=(0.019999, 0.003, 0.0), cell=(99, 14, 0), expected=0.0, tol=1e-6),
     ],
     passing_log={"p0": 0.0012048, "p2": 0.0327112, "p4": 0.0105765, "p8": 0.0218436},
+    # NiedererEtAl2012.reference:9-14 as (file, time, awk column, expected, tolerance);
+    # probes of system/Niedererpoints in their listed order, column 2 = probe 0
+    probe_file="Niedererpoints/0/activationTime",
+    probe_locations=[(0.0, 0.0, 0.007), (0.0, 0.0, 0.0), (0.019999, 0.0, 0.007),
+                     (0.019999, 0.0, 0.0), (0.0, 0.003, 0.007), (0.0, 0.003, 0.0),
+                     (0.019999, 0.003, 0.007), (0.019999, 0.003, 0.0), (0.01, 0.0015, 0.0035)],
+    probe_rows=[("Niedererpoints/0/activationTime", 0.04, 2, 0.00120503, 1e-5),
+                ("Niedererpoints/0/activationTime", 0.04, 4, 0.03271, 1e-4),
+                ("Niedererpoints/0/activationTime", 0.04, 6, 0.0105757, 1e-4),
+                ("Niedererpoints/0/activationTime", 0.04, 10, 0.0218433, 1e-4),
+                ("Niedererpoints/0/activationTime", 0.04, 5, 0.0, 1e-6),
+                ("Niedererpoints/0/activationTime", 0.04, 9, 0.0, 1e-6)],
+    write_interval=5e-3,
 )

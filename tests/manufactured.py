@@ -27,3 +27,9 @@ def fields(x, y, z, t):
     u1 = (1.0 + t) * G + np.sqrt(1.0 + t) * F
     u2 = 1.0 / (1.0 + t) * G ** -0.5
     return V, u1, u2, np.zeros_like(V)
+
+
+def single_cell(G, t):
+    """u1, u2 of a cell where F = 0 (V = 0): the single-cell tutorial's one 1D cell at
+    x = 0.5, G = 1 + x."""
+    return (1.0 + t) * G, G ** -0.5 / (1.0 + t)

@@ -1,0 +1,143 @@
+"""The reference's regression cases end to end through the on-disk formats: the device
+run writes the files the reference writes, and the comparison of the reference's own
+regression scripts (restated in tests/regression_check.py) reads them against the
+goldens.
+
+- singleCell: SingleCellSolver with RKF45 writes
+  postProcessing/BuenoOrovio_endocardialCells_S1_1000.txt; singleCell.reference:2-6 must
+  pass, and every row's Vm must lie within 1e-6 mV of the oracle's single-cell run.
+- NiedererEtAl2012: the tissue run writes postProcessing/Niedererpoints/0/activationTime
+  at the write times of system/controlDict (runTime, 5e-3 s) and the fields of the last
+  time; NiedererEtAl2012.reference:9-14 passes except probe 4, the documented 0.177 ms
+  deviation of this restatement, which stays within the bound recorded in the golden set.
+- Fields written by one tissue and read by another reproduce the run."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from tests import regression_check as rc
+from tests.golden import cases
+from tests.gpu_cases import rotating_fibre_sigma
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_single_cell_regression_files(gpu, orc, tmp_path):
+    _say("single-cell regression files")
+    from cardiacfoam_amd.single_cell import SingleCellSolver
+    from cardiacfoam_amd import foam_io
+    s = cases.SINGLE_CELL
+    end = 0.5
+    solver = SingleCellSolver(s["coeffs"], s["dt"], str(tmp_path), precision="fp64")
+    path = solver.run(end, chunk=3000)
+    assert os.path.basename(path) == s["trace_file"]
+    rows = [(s["trace_file"], t, e, tol) for t, e, tol in s["reference"]]
+    passed, failed = rc.check_single_cell(str(tmp_path / "postProcessing"), rows)
+    for r in passed + failed:
+        _say("  %s t=%.7f actual=%s expected=%.7f tol=%g" % (r[0], r[1], r[3], r[4], r[5]))
+    assert len(passed) == 5 and not failed
+    names, data = foam_io.read_trace(path)
+    assert names == ["time", "Vm", "s"]
+    n = int(round(end / s["dt"]))
+    assert data.shape == (n, 3)
+    st = s["stimulus"]
+    t_or, v_or, _ = orc.single_cell_run(orc.BO, orc.ENDO, orc.make_stim(**st), s["dt"], n,
+                                        integrator=1, abs_tol=s["abs_tol"],
+                                        rel_tol=s["rel_tol"])
+    np.testing.assert_allclose(data[:, 0], t_or, atol=5e-8, rtol=0)
+    err = np.max(np.abs(data[:, 1] - v_or))
+    _say("  trace Vm vs oracle: max %.3e mV over %d rows" % (err, n))
+    assert err <= 1e-6
+
+
+def _write_times(t, dt, interval, last):
+    """Time::writeTime for writeControl runTime: the write-interval index of t + dt/2
+    moved on since the last write."""
+    idx = int((t + 0.5 * dt) / interval)
+    return idx > last, idx
+
+
+def test_niederer_probe_and_field_files(gpu, orc, tmp_path):
+    _say("niederer probe files")
+    from cardiacfoam_amd import foam_io
+    c = cases.NIEDERER
+    nx, ny, nz = c["shape"]
+    mesh = gpu.HexMesh(nx, ny, nz, c["h"])
+    b = c["box"]
+    box = {"stimulusLocationMin": b["min"], "stimulusLocationMax": b["max"],
+           "stimulusDuration": b["duration"], "stimulusIntensity": b["intensity"],
+           "stimulusStartTime": 0.0}
+    coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": c["chi"],
+              "cm": c["cm"], "solutionAlgorithm": "explicit", "conductivity": c["sigma9"],
+              "externalStimulus": box, "solver": "RKF45", "maxSteps": 1000000000}
+    dom = gpu.Monodomain(mesh, coeffs, max_co=c["max_co"], ddt="backward")
+    post = str(tmp_path / "postProcessing")
+    probes = foam_io.Probes(c["shape"], c["h"], c["probe_locations"], ["activationTime"], post,
+                            name="Niedererpoints")
+    t, last, writes = 0.0, 0, 0
+    while t < c["end_time"] - 0.5 * dom.dt:
+        dom.step(1)
+        t = dom.time[0]
+        due, idx = _write_times(t, dom.dt, c["write_interval"], last)
+        if due or t >= c["end_time"] - 0.5 * dom.dt:
+            probes.write(t, {"activationTime": dom.activationTime})
+            last = idx
+            writes += 1
+    d = dom.write_fields(str(tmp_path))
+    _say("  %d probe rows, last time %.7g, fields in %s" % (writes, t, os.path.basename(d)))
+    assert writes == 8
+    passed, failed = rc.check_probes(post, c["probe_rows"])
+    for r in passed + failed:
+        _say("  col %d t=%g actual=%s expected=%g tol=%g %s" %
+             (r[2], r[1], r[3], r[4], r[5], "PASS" if r in passed else "FAIL"))
+    assert len(passed) == 5 and [r[2] for r in failed] == [6]
+    p4 = next(p for p in c["probes"] if p["name"] == "p4")
+    assert abs(failed[0][3] - failed[0][4]) <= p4["tol_here"]
+    # the written fields read back to the device values at 6 significant digits
+    for name, ref in (("activationTime", dom.activationTime), ("Vm", dom.Vm),
+                      ("ionicCurrent", dom.Iion)):
+        f = foam_io.read_field(os.path.join(d, name))
+        want = np.array([float("%.6g" % x) for x in ref])
+        np.testing.assert_array_equal(f.expand(mesh.n), want)
+    src = foam_io.read_field(os.path.join(d, "externalStimulusCurrent"))
+    assert src.uniform and src.values == 0.0          # the box is off at t = 0.04
+
+
+def test_fields_and_conductivity_from_case(gpu, orc, tmp_path):
+    """A case directory whose 0/ holds Vm and a per-cell conductivity tensor written at 17
+    digits: the tissue built from it steps exactly like one given the same arrays."""
+    _say("fields from case")
+    from cardiacfoam_amd import foam_io
+    shape, h, dt = (16, 12, 8), 2e-4, 2e-5
+    sig = np.asarray(rotating_fibre_sigma(*shape)).reshape(-1, 9)
+    n = int(np.prod(shape))
+    vm0 = -0.084 + 1e-3 * np.random.default_rng(5).random(n)
+    foam_io.write_field(str(tmp_path / "0" / "conductivity"), sig, precision=17)
+    foam_io.write_field(str(tmp_path / "0" / "Vm"), vm0, precision=17)
+    coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": 1.4e5,
+              "cm": 0.01, "solutionAlgorithm": "implicit",
+              "externalStimulus": {"stimulusLocationMin": [0, 0, 0],
+                                   "stimulusLocationMax": [6e-4, 6e-4, 6e-4],
+                                   "stimulusDuration": 2e-3, "stimulusIntensity": 5e4,
+                                   "stimulusStartTime": 0.0}}
+    a = gpu.Monodomain(gpu.HexMesh(*shape, h), coeffs, dt=dt, tol=1e-12,
+                       case_dir=str(tmp_path))
+    b = gpu.Monodomain(gpu.HexMesh(*shape, h), dict(coeffs, conductivity=sig), dt=dt, tol=1e-12)
+    b.set_field(gpu._capi.FIELD_VM, vm0)
+    b.set_field(gpu._capi.FIELD_VM_OLD, vm0)
+    np.testing.assert_array_equal(a.Vm, vm0)
+    a.step(100)
+    b.step(100)
+    np.testing.assert_array_equal(a.Vm, b.Vm)
+    np.testing.assert_array_equal(a.activationTime, b.activationTime)
+    src = a.external_stimulus_current()
+    c = a.mesh.cell_centres()
+    inside = np.all(c <= 6e-4, axis=1)
+    assert np.all(src[inside] == 5e4) and np.all(src[~inside] == 0.0)

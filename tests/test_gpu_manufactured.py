@@ -92,3 +92,52 @@ def test_manufactured_mixed_precision(gpu, orc):
     d = np.max(np.abs(d32.Vm - d64.Vm))
     _say("  error fp64 %.3e mixed %.3e, mixed - fp64 %.3e" % (e64, e32, d))
     assert d < 0.05 * e64
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_manufactured_single_cells_closed_form(gpu, orc, precision):
+    """tutorials/manufacturedSolutions/singleCellMonodomain through cg_ionic_run_cells
+    (RKF45 at the tutorial's tolerances, dt 5e-5 s to 0.2 s): cells with V = 0 and
+    G = 1 + x over x in (0, 1) stay on u1 = (1 + t) G, u2 = G^-1/2 / (1 + t), V = 0; FP64
+    equals the oracle's single-cell RKF45; Rush-Larsen (forward Euler) converges at first
+    order.  Mixed (FP32 states) stays within the rounding drift of 4000 FP32 updates."""
+    _say("manufactured single cells", precision)
+    n = 1000
+    G = 1.0 + (np.arange(n) + 0.5) / n
+    y0 = np.stack([np.zeros(n), G, G ** -0.5, np.zeros(n)])
+    dt, t_end = 5e-5, 0.2
+    steps = int(round(t_end / dt))
+    u1e, u2e = mf.single_cell(G, t_end)
+    stim = {"stim_start": 0.0, "stim_duration": 0.0, "stim_amplitude": 0.0,
+            "stim_period_S1": 1000.0, "nstim1": 0, "stim_period_S2": 0.0, "nstim2": 0}
+    coeffs = {"ionicModel": "monodomainFDAManufactured", "dimension": "1D", "solver": "RKF45",
+              "maxSteps": 1000000000, "absTol": 1e-6, "relTol": 1e-4,
+              "singleCellStimulus": stim}
+    # initialDeltaT is runTime.deltaT, as singleCellSolver.C:48-57 passes it
+    ion = gpu.IonicModel.New(coeffs, n, dt, True, precision=precision)
+    ion.set_states(y0)
+    ion.run_cells(0.0, dt, steps)
+    y = ion.states()
+    e1 = np.max(np.abs(y[1] - u1e) / u1e)
+    e2 = np.max(np.abs(y[2] - u2e) / u2e)
+    _say("  RKF45 u1 rel %.3e, u2 rel %.3e" % (e1, e2))
+    assert np.all(y[0] == 0.0) and np.all(y[3] == 0.0)
+    # FP32 states: 4000 rounded increments bound the drift by steps * 2^-23
+    bound = 1e-10 if precision == "fp64" else steps * 2.0 ** -23
+    assert e1 <= bound and e2 <= bound
+    if precision == "mixed":
+        return
+    s = orc.make_stim(start=0.0, period_s1=1000.0, duration=0.0, amplitude=0.0, nstim1=0)
+    for c in (0, 499, 999):
+        _, _, yo = orc.single_cell_run(orc.FDA, orc.DIM1, s, dt, steps, integrator=1,
+                                       abs_tol=1e-6, rel_tol=1e-4, y0=y0[:, c])
+        np.testing.assert_allclose(y[:, c], yo, rtol=1e-12, atol=0)
+    errs = []
+    for k in (1, 2):
+        ion = gpu.IonicModel.New(dict(coeffs, solver="RushLarsen"), n, dt, True)
+        ion.set_states(y0)
+        ion.run_cells(0.0, 8 * dt / k, steps // 8 * k)
+        errs.append(np.max(np.abs(ion.states()[2] - u2e)))
+    order = np.log2(errs[0] / errs[1])
+    _say("  Rush-Larsen u2 errors %.3e %.3e, order %.3f" % (errs[0], errs[1], order))
+    assert 0.9 < order < 1.1

@@ -266,3 +266,31 @@ def test_manufactured_solution_converges_second_order(orc):
         errs.append(np.max(np.abs(T.V - Ve)))
     order = np.log2(errs[0] / errs[1])
     assert errs[1] < 2e-3 and 1.7 < order < 2.3, (errs, order)
+
+
+def test_manufactured_single_cell_closed_form(orc):
+    """tutorials/manufacturedSolutions/singleCellMonodomain: one 1D cell at x = 0.5, where
+    F = cos(pi x) = 0, so V = 0 and the model (dV/dt = 0) leaves u1 = (1 + t) G and
+    u2 = G^-1/2 / (1 + t) with G = 1 + x.  RKF45 at the tutorial's tolerances stays on the
+    closed form; Rush-Larsen (forward Euler for both states) converges at first order."""
+    from tests import manufactured as mf
+    dt, t_end = 5e-5, 0.2                     # the tutorial's controlDict
+    steps = int(round(t_end / dt))
+    stim = orc.make_stim(start=0.0, period_s1=1000.0, duration=0.0, amplitude=0.0, nstim1=0)
+    errs = {}
+    for G in (1.5, 1.05, 1.95):
+        y0 = [0.0, G, G ** -0.5, 0.0]
+        u1e, u2e = mf.single_cell(G, t_end)
+        _, v, y = orc.single_cell_run(orc.FDA, orc.DIM1, stim, dt, steps, integrator=1,
+                                      abs_tol=1e-6, rel_tol=1e-4, y0=y0)
+        assert y[0] == 0.0 and np.all(v == 0.0) and y[3] == 0.0
+        assert abs(y[1] - u1e) <= 1e-4 * u1e and abs(y[2] - u2e) <= 1e-4 * u2e, (y, u1e, u2e)
+        _, _, y = orc.single_cell_run(orc.FDA, orc.DIM1, stim, dt, steps, integrator=1,
+                                      abs_tol=1e-12, rel_tol=1e-12, y0=y0)
+        assert abs(y[1] - u1e) <= 1e-10 * u1e and abs(y[2] - u2e) <= 1e-10 * u2e
+        for k in (1, 2):
+            _, _, y = orc.single_cell_run(orc.FDA, orc.DIM1, stim, dt * 8 / k,
+                                          steps // 8 * k, integrator=0, y0=y0)
+            errs[G, k] = abs(y[2] - u2e)
+        order = np.log2(errs[G, 1] / errs[G, 2])
+        assert 0.9 < order < 1.1, (G, errs, order)
