@@ -618,6 +618,49 @@ def run_reference_semantics(cf, wname, w, sigma, cpu_steps, with_cpu, ord_steps=
     return out
 
 
+def run_field_io(fields, sample=200_000):
+    """Writing and reading the tissue's fields as OpenFOAM ASCII files (the on-disk form of
+    MyocardiumDomain::write): native formatting and parsing, timed on the full field, with
+    a one-thread Python %g formatter on a sample beside it.  Round trip checked on a
+    sample against %.6g."""
+    import tempfile
+    import shutil
+    from cardiacfoam_amd import foam_io
+    d = tempfile.mkdtemp(prefix="cf_io_")
+    try:
+        out = {}
+        total_bytes, t_write, t_read = 0, 0.0, 0.0
+        rng = np.random.default_rng(7)
+        for name, v in fields.items():
+            t0 = time.perf_counter()
+            path = foam_io.write_field(os.path.join(d, "0.02", name), v, location="0.02")
+            t_write += time.perf_counter() - t0
+            total_bytes += os.path.getsize(path)
+            t0 = time.perf_counter()
+            f = foam_io.read_field(path)
+            t_read += time.perf_counter() - t0
+            got = f.expand(v.size)
+            idx = rng.integers(0, v.size, 10000)
+            want = np.array([float("%.6g" % x) for x in v[idx]])
+            if not np.array_equal(got[idx], want):
+                raise RuntimeError("field round trip mismatch in %s" % name)
+        n_vals = sum(v.size for v in fields.values())
+        vs = next(iter(fields.values()))[:sample]
+        t0 = time.perf_counter()
+        txt = "\n".join("%.6g" % x for x in vs.tolist())
+        t_py = time.perf_counter() - t0
+        out = {"fields": list(fields), "values": n_vals, "bytes": total_bytes,
+               "write_s": round(t_write, 4), "read_s": round(t_read, 4),
+               "write_MBps": round(total_bytes / t_write / 1e6, 1),
+               "read_MBps": round(total_bytes / t_read / 1e6, 1),
+               "write_values_per_s": n_vals / t_write, "read_values_per_s": n_vals / t_read,
+               "python_format_values_per_s": len(vs) / t_py,
+               "python_sample": len(vs), "python_bytes": len(txt)}
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def cpu_baseline(w, sigma, sample_steps):
     """The CPU FP64 oracle (OpenMP over all host cores) on the same workload: the full
     mesh for `sample_steps` steps after a 2-step start, timed as cell-steps/s."""
@@ -761,6 +804,10 @@ def main():
     ref_sem = run_reference_semantics(cf, args.workload, w, sigma, 100, not args.no_cpu) \
         if w["model"] == "ORd" and not args.no_batch else None
 
+    io = run_field_io({"Vm": V64, "activationTime": a64})
+    print("[bench] field files: write %.3f s, read %.3f s for %d values" %
+          (io["write_s"], io["read_s"], io["values"]), file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -794,6 +841,7 @@ def main():
         "batch_c2": c2,
         "adjoint_c4": c4,
         "rkf45": ref_sem,
+        "field_io": io,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

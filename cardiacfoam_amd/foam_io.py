@@ -44,13 +44,12 @@ SMALL = 1e-15  # OpenFOAM's SMALL for double precision
 
 
 # --------------------------------------------------------------------- numbers
-def format_numbers(values, precision=6, fixed=False, row=1):
-    """bytes of the values as std::ostream writes them (general: %g with `precision`
-    significant digits; fixed: `precision` decimals), `row` values per line."""
+def _format_view(values, precision=6, fixed=False, row=1):
+    """The formatted text as a uint8 array view (no copy to bytes)."""
     v = np.ascontiguousarray(values, dtype=np.float64).ravel()
     n = v.size
     if n == 0:
-        return b""
+        return np.zeros(0, dtype=np.uint8)
     if fixed:
         finite = v[np.isfinite(v)]
         big = float(np.max(np.abs(finite))) if finite.size else 0.0
@@ -59,11 +58,17 @@ def format_numbers(values, precision=6, fixed=False, row=1):
     else:
         per = 8 + max(precision, 1) + 8
     cap = n * per + 64
-    buf = C.create_string_buffer(cap)
+    buf = np.empty(cap, dtype=np.uint8)
     used = C.c_int64()
     check(lib().cg_format_scalars(v.ctypes.data_as(C.c_void_p), n, int(precision), int(fixed),
-                                  int(row), buf, cap, C.byref(used)))
-    return buf.raw[:used.value]
+                                  int(row), buf.ctypes.data_as(C.c_void_p), cap, C.byref(used)))
+    return buf[:used.value]
+
+
+def format_numbers(values, precision=6, fixed=False, row=1):
+    """bytes of the values as std::ostream writes them (general: %g with `precision`
+    significant digits; fixed: `precision` decimals), `row` values per line."""
+    return _format_view(values, precision, fixed, row).tobytes()
 
 
 def parse_numbers(data, n, offset=0):
@@ -72,11 +77,12 @@ def parse_numbers(data, n, offset=0):
     out = np.empty(n, dtype=np.float64)
     if n == 0:
         return out, offset
-    view = (C.c_char * (len(data) - offset)).from_buffer_copy(data, offset) \
-        if not isinstance(data, bytearray) else (C.c_char * (len(data) - offset)).from_buffer(data, offset)
+    raw = np.frombuffer(data, dtype=np.uint8)       # no copy of bytes / bytearray
+    if offset > raw.size:
+        raise ValueError("offset beyond the data")
     used = C.c_int64()
-    check(lib().cg_parse_scalars(view, len(data) - offset, out.ctypes.data_as(C.c_void_p), n,
-                                 C.byref(used)))
+    check(lib().cg_parse_scalars(C.c_void_p(raw.ctypes.data + offset), raw.size - offset,
+                                 out.ctypes.data_as(C.c_void_p), n, C.byref(used)))
     return out, offset + used.value
 
 
@@ -131,10 +137,12 @@ def write_field(path, values, name=None, cls=None, dimensions=None, location="0"
         parts += [b"internalField   uniform ", one if ncomp == 1 else b"(" + one + b")", b";\n\n"]
     else:
         parts += [b"internalField   nonuniform List<", vtype.encode(), b"> \n", b"%d\n(\n" % n]
-        body = format_numbers(flat, precision, row=ncomp)
         if ncomp > 1:
             # every line is one (x y z ...) entry
+            body = format_numbers(flat, precision, row=ncomp)
             body = b"(" + body[:-1].replace(b"\n", b")\n(") + b")\n"
+        else:
+            body = _format_view(flat, precision)   # written from the buffer, no copy
         parts += [body, b")\n;\n\n"]
     parts.append(b"boundaryField\n{\n")
     for p, t in patches.items():
@@ -144,7 +152,8 @@ def write_field(path, values, name=None, cls=None, dimensions=None, location="0"
     if d:
         os.makedirs(d, exist_ok=True)
     with open(path, "wb") as f:
-        f.write(b"".join(parts))
+        for part in parts:
+            f.write(part)
     return path
 
 
