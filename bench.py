@@ -173,6 +173,36 @@ def measure_spmv(torch, dom, n, prec, launches=200):
             "frac": round(gbs / HBM_PEAK_GBS, 4), "rows": n}
 
 
+def measure_pseudo_ecg(dom, n, per_cell_sigma, calls=50):
+    """Pseudo-ECG of the tissue's Vm (cg_tissue_pseudo_ecg) at the 5 electrodes of the
+    monodomainPseudoECG tutorial (scaled to the block) and at 10: kernel time per call from
+    HIP events on the library stream.  Algorithmic bytes per cell and pass of up to 16
+    electrodes: Vm 8 B (+ 48 B of per-cell symmetric conductivity)."""
+    ext = np.array([dom.mesh.nx, dom.mesh.ny, dom.mesh.nz]) * dom.mesh.h
+    unit = np.array([(-0.5, 0.5, 0.5), (1.5, 0.5, 0.5), (1.2, 0.23, 0.61), (1.35, 0.74, 0.28),
+                     (1.55, 0.41, 0.83)])
+    out = {}
+    for ne in (5, 10):
+        E = np.concatenate([unit, unit[:, ::-1] * 1.1])[:ne] * ext
+        for _ in range(3):
+            dom.pseudo_ecg(E)
+        dom.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            dom.pseudo_ecg(E)
+        wall = (time.perf_counter() - t0) / calls
+        ms, cnt = dom.kernel_times()["pseudo_ecg"]
+        dom.set_profiling(False)
+        avg = ms / cnt * 1e-3
+        passes = (ne + 15) // 16
+        bpc = (8 + (48 if per_cell_sigma else 0)) * passes
+        out["electrodes_%d" % ne] = {
+            "kernel_us": round(avg * 1e6, 2), "call_us": round(wall * 1e6, 2),
+            "bytes_per_cell": bpc, "GBps": round(bpc * n / avg / 1e9, 1),
+            "cell_electrodes_per_s": n * ne / avg}
+    return out
+
+
 def copy_bandwidth(torch, nbytes=1 << 30, reps=20):
     """What a plain device-to-device copy of 1 GiB achieves (read + write bytes / time),
     the practical streaming ceiling beside the 8 TB/s peak"""
@@ -797,6 +827,7 @@ def main():
         if w["algorithm"] == "implicit" else None
     if spmv is not None:
         spmv["ell_fp64"] = measure_ell_spmv(cf, torch, w, sigma)
+    ecg = measure_pseudo_ecg(res["fp64"]["dom"], n, bool(w["rotating"]))
     c0 = run_explicit_c0(cf, torch) if args.workload != "bo_explicit_147k" else None
     c1 = run_batch_c1(cf) if not args.no_batch else None
     c2 = run_batch_c2(cf) if not args.no_batch else None
@@ -842,6 +873,7 @@ def main():
         "adjoint_c4": c4,
         "rkf45": ref_sem,
         "field_io": io,
+        "pseudo_ecg": ecg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -15,6 +15,7 @@
 #include "cg_adjoint.hpp"
 #include "cg_ldu.hpp"
 #include "foam_format.hpp"
+#include "cg_ecg.hpp"
 
 namespace {
 
@@ -174,6 +175,8 @@ struct cg_mesh {
     double* d_fy = nullptr;
     double* d_fz = nullptr;
     std::vector<double> h_sigma9;  // host copy for dt and implicit assembly
+    double* d_sigma9 = nullptr;    // per-cell tensors on the device, made on demand:
+    bool sigma_sym = false;        // [6][n] symmetric (xx xy xz yy yz zz), else [9][n]
 };
 
 struct cg_tissue {
@@ -192,6 +195,8 @@ struct cg_tissue {
     int64_t steps = 0;
     cg::ImplicitSolver* implicit = nullptr;
     cg::Profiler prof;
+    // pseudo-ECG buffers: electrode positions, per-block partial sums, results
+    double* d_ecg = nullptr;   // [ECG_MAX_ELECTRODES * (3 + ECG_BLOCKS + 1)]
 };
 
 namespace {
@@ -856,6 +861,7 @@ int cg_mesh_destroy(cg_mesh* m)
 {
     if (!m) return CG_OK;
     hipFree(m->d_od); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
+    hipFree(m->d_sigma9);
     delete m;
     return CG_OK;
 }
@@ -868,7 +874,8 @@ int cg_mesh_set_conductivity(cg_mesh* m, const double* s, int uniform)
     const cg::HexGeom& g = m->g;
     const int64_t n = m->n;
     hipFree(m->d_od); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
-    m->d_od = m->d_fx = m->d_fy = m->d_fz = nullptr;
+    hipFree(m->d_sigma9);
+    m->d_od = m->d_fx = m->d_fy = m->d_fz = m->d_sigma9 = nullptr;
     m->h_sigma9.assign(9 * (size_t)n, 0.0);
     for (int64_t c = 0; c < n; ++c)
         for (int q = 0; q < 9; ++q) m->h_sigma9[9 * c + q] = uniform ? s[q] : s[9 * c + q];
@@ -1014,6 +1021,7 @@ int cg_tissue_destroy(cg_tissue* t)
     if (t->ion && t->ion->stream) hipStreamSynchronize(t->ion->stream);
     for (int q = 0; q < 3; ++q) hipFree(t->d_V[q]);
     hipFree(t->d_Iion); hipFree(t->d_flag); hipFree(t->d_tact);
+    if (t->d_ecg) hipFree(t->d_ecg);
     if (t->implicit) cg::implicit_destroy(t->implicit);
     delete t;
     return CG_OK;
@@ -1135,6 +1143,61 @@ int cg_tissue_get_field(cg_tissue* t, int which, double* host)
     HIPCHK(hipStreamSynchronize(t->ion->stream));
     if (int rc = check_ode(t->ion)) return rc;
     HIPCHK(hipMemcpy(host, d, sizeof(double) * t->mesh->n, hipMemcpyDeviceToHost));
+    return CG_OK;
+}
+
+int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions,
+                         double* values)
+{
+    if (!t || (n_electrodes > 0 && (!positions || !values)))
+        return fail(CG_ERR_ARG, "NULL argument");
+    if (n_electrodes < 0 || n_electrodes > cg::ECG_MAX_ELECTRODES)
+        return fail(CG_ERR_ARG, "between 0 and %d electrodes", cg::ECG_MAX_ELECTRODES);
+    if (n_electrodes == 0) return CG_OK;
+    cg_mesh* m = t->mesh;
+    hipStream_t s = t->ion->stream;
+    for (int i = 0; i < 3 * n_electrodes; ++i)
+        if (!std::isfinite(positions[i])) return fail(CG_ERR_ARG, "electrode position not finite");
+    if (!m->uniform && !m->d_sigma9) {
+        const std::vector<double>& s9 = m->h_sigma9;
+        bool sym = true;
+        for (int64_t c = 0; c < m->n && sym; ++c) {
+            const double* q = &s9[9 * c];
+            sym = q[1] == q[3] && q[2] == q[6] && q[5] == q[7];
+        }
+        // component-major ([6][n] or [9][n]) so that lanes read consecutive cells
+        static const int k6[6] = {0, 1, 2, 4, 5, 8};
+        const int nc = sym ? 6 : 9;
+        std::vector<double> soa((size_t)nc * m->n);
+        for (int q = 0; q < nc; ++q)
+            for (int64_t c = 0; c < m->n; ++c)
+                soa[(size_t)q * m->n + c] = s9[9 * c + (sym ? k6[q] : q)];
+        const size_t bytes = sizeof(double) * soa.size();
+        if (hipMalloc(&m->d_sigma9, bytes) != hipSuccess)
+            return fail(CG_ERR_NOMEM, "conductivity allocation failed");
+        HIPCHK(hipMemcpy(m->d_sigma9, soa.data(), bytes, hipMemcpyHostToDevice));
+        m->sigma_sym = sym;
+    }
+    const size_t per = 3 + cg::ECG_BLOCKS + 1;
+    if (!t->d_ecg && hipMalloc(&t->d_ecg, sizeof(double) * per * cg::ECG_MAX_ELECTRODES) !=
+                         hipSuccess)
+        return fail(CG_ERR_NOMEM, "ECG buffer allocation failed");
+    double* pos = t->d_ecg;
+    double* out = pos + 3 * cg::ECG_MAX_ELECTRODES;
+    double* partial = out + cg::ECG_MAX_ELECTRODES;
+    HIPCHK(hipStreamSynchronize(s));
+    if (int rc = check_ode(t->ion)) return rc;
+    HIPCHK(hipMemcpyAsync(pos, positions, sizeof(double) * 3 * n_electrodes,
+                          hipMemcpyHostToDevice, s));
+    cg::Profiler* prof = t->prof.enabled ? &t->prof : nullptr;
+    const int tk = prof ? prof->begin(cg::K_ECG) : -1;
+    HIPCHK(cg::launch_pseudo_ecg(m->g, t->d_V[t->cur], m->uniform ? nullptr : m->d_sigma9,
+                                 m->sigma_sym ? 1 : 0, m->uniform ? m->sig : nullptr,
+                                 n_electrodes, pos, partial, out, s));
+    if (prof) prof->end(tk);
+    HIPCHK(hipMemcpyAsync(values, out, sizeof(double) * n_electrodes, hipMemcpyDeviceToHost,
+                          s));
+    HIPCHK(hipStreamSynchronize(s));
     return CG_OK;
 }
 

@@ -1,0 +1,23 @@
+// Pseudo-ECG on the hex block (ecg.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cg_internal.hpp"
+
+namespace cg {
+
+constexpr int ECG_BLOCKS = 1024;       // most partial sums per electrode
+constexpr int ECG_MAX_ELECTRODES = 256;
+
+// blocks of the first kernel for n cells (<= ECG_BLOCKS): partial holds [E][that]
+int ecg_partial_blocks(int64_t n);
+
+// out_dev[e] = pseudo-ECG at electrode e from V (device, blockMesh order) and the
+// conductivity: sig per cell (device; [6][n] xx xy xz yy yz zz when sym, else [9][n]) or,
+// when null, the uniform s9 (host [9]).  pos_dev [E][3] device; partial [E][ECG_BLOCKS]
+// device scratch.  Asynchronous on s.
+hipError_t launch_pseudo_ecg(const HexGeom& g, const double* V, const double* sig, int sym,
+                             const double* s9, int n_el, const double* pos_dev,
+                             double* partial, double* out_dev, hipStream_t s);
+
+} // namespace cg

@@ -195,7 +195,7 @@ namespace cg {
 // Per-kernel HIP-event timing on the handle's stream (the trace equivalent of the
 // clockTime phases of staggeredElectrophysicsAdvanceScheme.C:54-74).
 enum KernelId { K_IONIC = 0, K_SPMV, K_UPDATE, K_RESIDUAL0, K_RESIDUAL, K_ACTIVATION,
-                K_INVDIAG, K_APPLY, K_COUNT };
+                K_INVDIAG, K_APPLY, K_ECG, K_COUNT };
 const char* kernel_name(int id);
 
 struct Profiler {

@@ -614,7 +614,8 @@ __global__ __launch_bounds__(BS) void k_laplacian(HexGeom g, HexCoefs a,
 const char* kernel_name(int id)
 {
     static const char* names[K_COUNT] = {"ionic",    "spmv_dot",   "cg_update", "residual_first",
-                                         "residual", "activation", "invdiag",    "apply"};
+                                         "residual", "activation", "invdiag",    "apply",
+                                         "pseudo_ecg"};
     return id >= 0 && id < K_COUNT ? names[id] : "?";
 }
 

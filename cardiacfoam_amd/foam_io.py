@@ -285,6 +285,35 @@ class Probes:
                 fh.write(row)
 
 
+# ------------------------------------------------------------------ ECG file
+class PseudoEcgFile:
+    """postProcessing/pseudoECG.dat of ECGDomain (ecgDomain.C:145-147, 180-188;
+    ecgModelIO.C:30-68): "# time  <electrode names>", then a row per output time, every
+    number in scientific form with 8 decimals, separated by two spaces."""
+
+    def __init__(self, case_dir, names, file_name="pseudoECG.dat"):
+        d = os.path.join(case_dir, "postProcessing")
+        os.makedirs(d, exist_ok=True)
+        self.path = os.path.join(d, file_name)
+        self.names = list(names)
+        with open(self.path, "w") as f:
+            f.write("# time" + "".join("  " + n for n in self.names) + "\n")
+
+    def write(self, t, values):
+        if len(values) != len(self.names):
+            raise ValueError("one value per electrode")
+        with open(self.path, "a") as f:
+            f.write("%.8e" % t + "".join("  %.8e" % v for v in values) + "\n")
+
+
+def electrodes(ecg_dict):
+    """(names, positions [E][3]) of an ecgDomains entry's electrodePositions sub-dictionary
+    in its listed order (ecgDomain.C:53-81)."""
+    ep = ecg_dict.get("electrodePositions", {})
+    names = list(ep)
+    return names, np.array([[float(x) for x in ep[k]] for k in names]).reshape(-1, 3)
+
+
 # ------------------------------------------------------------ single-cell trace
 def protocol_suffix(coeffs):
     """stimulusIO::protocolSuffix (stimulusIO.C:392-423): noStim, S1_<period> or

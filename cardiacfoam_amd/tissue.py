@@ -272,6 +272,14 @@ class Monodomain:
         """Applies fv_solution_controls(fv_solution, field)."""
         self.set_solver_controls(**fv_solution_controls(fv_solution, field))
 
+    def pseudo_ecg(self, positions):
+        """PseudoECGSolver::compute (pseudoECGSolver.C:36-83) of the current Vm at the
+        electrode positions [E][3] (m): the dipole sum per electrode, [E]."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(pos))
+        check(lib().cg_tissue_pseudo_ecg(self.t, len(pos), _dp(pos), _dp(out)))
+        return out
+
     # ------------------------------------------------------------ on-disk fields
     def external_stimulus_current(self):
         """sourceField_ of the last step (myocardiumDomain.C:307-342), A/m^3: the summed

@@ -346,6 +346,16 @@ int cg_adjoint_backward(cg_adjoint* a, const double* dprobes_dev, double* dkappa
 int cg_tissue_apply_operator(cg_tissue* t, double ddt_coeff, int precision, const void* x_dev,
                              void* y_dev);
 
+/* Pseudo-ECG of the current Vm (PseudoECGSolver::compute, pseudoECGSolver.C:36-83, called
+ * by ECGDomain::evolve, ecgDomain.C:167-190): for each electrode P,
+ *   phi(P) = sum_c ((sigma_c . grad Vm_c) vol_c) . (C_c - P) / |C_c - P|^3
+ * over cells with |C_c - P| > VSMALL, grad Vm by Gauss linear with zeroGradient walls.
+ * positions [n][3] in m (host), values [n] out (host; the sum itself, no 1/(4 pi sigma)
+ * factor, as the reference reports it);
+ * 1 <= n <= 256.  Synchronous. */
+int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions,
+                         double* values);
+
 /* ------------------------------------------------------ on-disk ASCII formats
  * Numbers of OpenFOAM's ASCII field files (the internalField List<scalar> that
  * volScalarField::write emits for Vm, activationTime, ionicCurrent and

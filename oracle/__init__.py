@@ -119,6 +119,8 @@ def lib():
                               C.POINTER(C.c_double)]
         L.orc_box_stimulus.argtypes = [C.POINTER(Mesh), C.POINTER(Boxes), C.c_double, _dp]
         L.orc_activation.argtypes = [C.c_long, _dp, _dp, _u8, _dp, C.c_double, C.c_double]
+        L.orc_pseudo_ecg.argtypes = [C.POINTER(Mesh), _dp, _dp, C.c_int, _dp, _dp, _dp,
+                                     C.c_int]
         L.orc_explicit_dt.argtypes = [C.POINTER(Mesh), _dp, C.c_double, C.c_double]
         L.orc_explicit_dt.restype = C.c_double
         L.orc_tissue_create.argtypes = [C.POINTER(Tissue), C.c_int, C.c_int, C.c_int, C.c_double,
@@ -276,6 +278,18 @@ class OracleMesh:
                               np.ascontiguousarray(b, dtype=np.float64), x, tol, rel_tol,
                               max_iter, precond, C.byref(r0), C.byref(r1))
         return x, it, r0.value, r1.value
+
+    def pseudo_ecg(self, sigma9, V, positions, nthreads=8):
+        """pseudo-ECG at the electrodes (orc_pseudo_ecg): (values, sums of |terms|)"""
+        n = self.m.n_cells
+        s = np.asarray(sigma9, dtype=np.float64)
+        s = np.ascontiguousarray(np.tile(s.ravel(), n) if s.size == 9 else s.ravel())
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(pos))
+        ab = np.zeros(len(pos))
+        lib().orc_pseudo_ecg(C.byref(self.m), s, np.ascontiguousarray(V, dtype=np.float64),
+                             len(pos), pos, out, ab, nthreads)
+        return out, ab
 
     def explicit_dt(self, sigma9, chiCm, maxCo):
         return lib().orc_explicit_dt(C.byref(self.m), np.ascontiguousarray(sigma9, dtype=np.float64),

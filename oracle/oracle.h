@@ -123,6 +123,11 @@ typedef struct {
     int8_t* face_dir;    /* 0,1,2 = x,y,z normal */
 } orc_mesh_t;
 
+/* Pseudo-ECG of V at ne electrodes pos[3 ne] (pseudoECGSolver.C:36-83), sigma [9n]
+ * per cell: out[ne], absout[ne] (sum of |terms|, may be NULL). */
+void orc_pseudo_ecg(const orc_mesh_t* m, const double* sigma, const double* V, int ne,
+                    const double* pos, double* out, double* absout, int nthreads);
+
 /* Builds the OpenFOAM blockMesh ordering of a uniform nx*ny*nz hex block:
  * cell id = i + nx*(j + ny*k); internal faces in owner order, each owner's
  * faces sorted by neighbour (upper-triangular order). */

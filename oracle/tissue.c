@@ -528,3 +528,41 @@ int orc_tissue_step(orc_tissue_t* T, const orc_boxes_t* boxes)
     T->step_index++;
     return 0;
 }
+
+/* Pseudo-ECG (PseudoECGSolver::compute, pseudoECGSolver.C:36-83): for each electrode P,
+ *   phi = sum_c ((sigma_c . grad V_c) vol_c) . r / |r|^3,  r = C_c - P,  |r| > VSMALL,
+ * grad V the Gauss-linear gradient with zeroGradient walls.  out[e] the sum, absout[e]
+ * the sum of the magnitudes of its terms (the scale of its rounding error). */
+void orc_pseudo_ecg(const orc_mesh_t* m, const double* sigma, const double* V, int ne,
+                    const double* pos, double* out, double* absout, int nthreads)
+{
+    const long n = m->n_cells;
+    const int nx = m->nx, ny = m->ny;
+    const double vol = m->hx * m->hy * m->hz;
+    double* grad = malloc(sizeof(double) * 3 * n);
+    gauss_grad(m, V, grad, nthreads);
+    for (int e = 0; e < ne; ++e) {
+        double acc = 0.0, aacc = 0.0;
+        const double P0 = pos[3 * e], P1 = pos[3 * e + 1], P2 = pos[3 * e + 2];
+#pragma omp parallel for schedule(static) reduction(+ : acc, aacc) \
+    num_threads(nthreads > 0 ? nthreads : 1)
+        for (long c = 0; c < n; ++c) {
+            const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
+            const double* s = sigma + 9 * c;
+            const double* g = grad + 3 * c;
+            double d[3];
+            for (int a = 0; a < 3; ++a) d[a] = (s[3 * a] * g[0] + s[3 * a + 1] * g[1] + s[3 * a + 2] * g[2]) * vol;
+            const double r0 = (i + 0.5) * m->hx - P0, r1 = (j + 0.5) * m->hy - P1,
+                         r2 = (k + 0.5) * m->hz - P2;
+            const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+            if (r > 1e-300) {
+                const double term = (d[0] * r0 + d[1] * r1 + d[2] * r2) / (r * r * r);
+                acc += term;
+                aacc += fabs(term);
+            }
+        }
+        out[e] = acc;
+        if (absout) absout[e] = aacc;
+    }
+    free(grad);
+}

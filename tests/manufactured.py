@@ -33,3 +33,28 @@ def single_cell(G, t):
     """u1, u2 of a cell where F = 0 (V = 0): the single-cell tutorial's one 1D cell at
     x = 0.5, G = 1 + x."""
     return (1.0 + t) * G, G ** -0.5 / (1.0 + t)
+
+
+# electrodes of tutorials/manufacturedSolutions/monodomainPseudoECG (ecgDomains.ECG)
+ELECTRODES = ((-0.5, 0.5, 0.5), (1.5, 0.5, 0.5), (1.2, 0.23, 0.61), (1.35, 0.74, 0.28),
+              (1.55, 0.41, 0.83))
+
+
+def pseudo_ecg_reference(P, t=0.0, order=96):
+    """The continuous pseudo-ECG of the manufactured V on the unit cube,
+    int (sigma grad V) . (x - P) / |x - P|^3 dx, by Gauss-Legendre quadrature of the given
+    order per direction (the reference verifier's referenceQuadratureOrder 96)."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    x = 0.5 * (x + 1.0)
+    w = 0.5 * w
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    W = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    c = np.sqrt(1.0 + t)
+    cx, cy, cz = np.cos(np.pi * X), np.cos(2 * np.pi * Y), np.cos(3 * np.pi * Z)
+    gx = -np.pi * np.sin(np.pi * X) * cy * cz * c
+    gy = -2 * np.pi * cx * np.sin(2 * np.pi * Y) * cz * c
+    gz = -3 * np.pi * cx * cy * np.sin(3 * np.pi * Z) * c
+    rx, ry, rz = X - P[0], Y - P[1], Z - P[2]
+    r = np.sqrt(rx * rx + ry * ry + rz * rz)
+    return float(np.sum(W * (SIGMA[0] * gx * rx + SIGMA[1] * gy * ry + SIGMA[2] * gz * rz)
+                        / r ** 3))

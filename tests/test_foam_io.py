@@ -239,3 +239,18 @@ def test_time_name(fio):
     assert [fio.time_name(t) for t in (0.0, 0.005, 0.04, 1e-5, 0.1 + 0.2, 12.5)] == \
         ["0", "0.005", "0.04", "1e-05", "0.3", "12.5"]
     assert math.isclose(float(fio.time_name(0.0400000001)), 0.04)
+
+
+def test_pseudo_ecg_file_and_electrodes(fio, tmp_path):
+    from cardiacfoam_amd import parse_dictionary
+    d = parse_dictionary("""
+        ECG { ecgSolver pseudoECG;
+              electrodePositions { E1 (-0.5 0.5 0.5); E2 (1.5 0.5 0.5); E3 (1.2 0.23 0.61); } }
+    """)
+    names, pos = fio.electrodes(d["ECG"])
+    assert names == ["E1", "E2", "E3"] and pos.shape == (3, 3) and pos[2, 1] == 0.23
+    f = fio.PseudoEcgFile(str(tmp_path), names)
+    f.write(0.01, [1.5e-3, -2.25e-5, 0.0])
+    lines = open(f.path).read().splitlines()
+    assert lines[0] == "# time  E1  E2  E3"
+    assert lines[1] == "1.00000000e-02  1.50000000e-03  -2.25000000e-05  0.00000000e+00"

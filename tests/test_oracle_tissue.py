@@ -294,3 +294,28 @@ def test_manufactured_single_cell_closed_form(orc):
             errs[G, k] = abs(y[2] - u2e)
         order = np.log2(errs[G, 1] / errs[G, 2])
         assert 0.9 < order < 1.1, (G, errs, order)
+
+
+def test_pseudo_ecg_converges_to_the_continuous_integral(orc):
+    """The oracle's pseudo-ECG (pseudoECGSolver.C:36-83) of the manufactured V at the
+    monodomainPseudoECG tutorial's electrodes: second-order convergence to the
+    Gauss-Legendre quadrature (order 96) of the continuous dipole integral; the two
+    electrodes on the line y = z = 1/2 see zero by the symmetry of cos(2 pi y) cos(3 pi z);
+    the sum is linear in V and scales with sqrt(1 + t)."""
+    from tests import manufactured as mf
+    E = np.array(mf.ELECTRODES)
+    ref = np.array([mf.pseudo_ecg_reference(P) for P in E])
+    assert np.max(np.abs(ref[:2])) < 1e-15 and np.min(np.abs(ref[2:])) > 1e-3
+    sig = np.array([mf.SIGMA[0], 0, 0, 0, mf.SIGMA[1], 0, 0, 0, mf.SIGMA[2]])
+    errs = []
+    for n in (12, 24):
+        x, y, z, h = mf.centres(n)
+        V = mf.fields(x, y, z, 0.0)[0]
+        M = orc.OracleMesh(n, n, n, h, h, h)
+        v, ab = M.pseudo_ecg(sig, V, E)
+        assert np.max(np.abs(v[:2])) < 1e-13 * ab.max()
+        errs.append(np.max(np.abs(v - ref)))
+        v3, _ = M.pseudo_ecg(sig, np.sqrt(1.3) * V + 2.0, E)   # a constant adds nothing
+        np.testing.assert_allclose(v3, np.sqrt(1.3) * v, rtol=1e-12, atol=1e-15)
+    order = np.log2(errs[0] / errs[1])
+    assert 1.85 < order < 2.15 and errs[1] < 1e-4, (errs, order)
