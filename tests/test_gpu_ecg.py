@@ -36,7 +36,8 @@ def test_pseudo_ecg_equals_oracle(gpu, orc, rotating):
     dom.set_field(gpu._capi.FIELD_VM, V)
     ext = np.array(shape) * h
     rng = np.random.default_rng(5)
-    E = np.concatenate([rng.uniform(-1, 2, size=(12, 3)) * ext,           # around the block
+    # 20 electrodes: two kernel passes of 16 and 4
+    E = np.concatenate([rng.uniform(-1, 2, size=(16, 3)) * ext,           # around the block
                         [[0.5 * h, 0.5 * h, 0.5 * h]],                    # on a cell centre
                         rng.uniform(0.1, 0.9, size=(3, 3)) * ext])        # inside
     got = dom.pseudo_ecg(E)

@@ -141,3 +141,35 @@ def test_fields_and_conductivity_from_case(gpu, orc, tmp_path):
     c = a.mesh.cell_centres()
     inside = np.all(c <= 6e-4, axis=1)
     assert np.all(src[inside] == 5e4) and np.all(src[~inside] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["TNNP_epi", "BO_epi"])
+def test_trace_rows_states_and_rates(gpu, orc, name):
+    """cg_ionic_run_cells_trace (Rush-Larsen): the traced cell's last row equals its states
+    after the run, and every recorded rate row equals the oracle's right-hand side of the
+    recorded states at the step's end time (stimulus included), the RATES columns of the
+    trace."""
+    from tests.gpu_cases import MODELS
+    _say("trace rows", name)
+    m, tis, key, word, amp = MODELS[name]
+    n, cell, dt, steps = 64, 17, 1e-5, 1500
+    ion = gpu.IonicModel(key, word, n, solve_vm=True)
+    stim = {"stim_start": 1.0025, "stim_period_S1": 500.0, "stim_duration": 1.0,
+            "stim_amplitude": amp, "nstim1": 1}
+    ion.set_stimulus_protocol(stim)
+    states, rates = ion.run_cells_trace(0.0, dt, steps, cell)
+    assert states.shape == (steps, ion.nstates) and rates.shape == states.shape
+    np.testing.assert_array_equal(states[-1], ion.states()[:, cell])
+    c, _ = orc.model_init(m, tis)
+    st = orc.make_stim(start=1.0025, period_s1=500.0, duration=1.0, amplitude=amp, nstim1=1)
+    scale = 1000.0 / 12.9 if m == 3 else 1000.0
+    worst = 0.0
+    for s in range(0, steps, 37):
+        te = (s + 1) * (dt * scale)
+        if min(abs(te - 1.0025), abs(te - 2.0025)) < 1e-6:
+            continue                                   # stimulus edge: either side is right
+        ref = orc.model_eval(m, tis, te, c, states[s], solve_vm=True, stim=st)["rates"]
+        err = np.max(np.abs(rates[s] - ref) / (np.abs(ref) + 1e-9 * np.max(np.abs(ref))))
+        worst = max(worst, err)
+    _say("  worst rate rel err %.3e" % worst)
+    assert worst < 1e-9
