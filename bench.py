@@ -750,6 +750,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched single-cell legs")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both",
                     help="run one precision only (profiling runs; no error gate)")
+    ap.add_argument("--ecg-only", action="store_true",
+                    help="only the pseudo-ECG leg on the workload's tissue after 110 FP64 steps")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 10 times (HBM counter scale)")
     args = ap.parse_args()
@@ -777,6 +779,11 @@ def main():
     sigma = make_sigma(w["shape"], w["rotating"])
     n = int(np.prod(w["shape"]))
     res = {}
+    if args.ecg_only:
+        r = run_precision(cf, torch, w, sigma, "fp64", 10, 100, 0)
+        print(json.dumps({"workload": args.workload, "cells": n,
+                          "pseudo_ecg": measure_pseudo_ecg(r["dom"], n, bool(w["rotating"]))}))
+        return
     if args.only != "both":
         r = run_precision(cf, torch, w, sigma, args.only, args.steps, args.warmup,
                           args.prof_steps)
