@@ -9,11 +9,11 @@ reference's ionicModel / MyocardiumDomain interface.
 from . import _capi
 from .foam_dict import parse as parse_dictionary, read as read_dictionary
 from .ionic import IonicModel, FatalIOError, batch_run, batch_autotune
-from .tissue import HexMesh, Monodomain
+from .tissue import HexMesh, PolyMesh, Monodomain
 from .adjoint import TissueAdjoint, probe_traces
 from .ldu import LduMatrix
 
-__all__ = ["LduMatrix", "IonicModel", "FatalIOError", "batch_run", "batch_autotune", "HexMesh", "Monodomain", "TissueAdjoint", "probe_traces", "parse_dictionary",
+__all__ = ["LduMatrix", "IonicModel", "FatalIOError", "batch_run", "batch_autotune", "HexMesh", "PolyMesh", "Monodomain", "TissueAdjoint", "probe_traces", "parse_dictionary",
            "read_dictionary", "library_path", "version"]
 
 

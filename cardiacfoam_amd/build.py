@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcardiacgpu.so")
 SOURCES = ["model_fp64.hip", "model_fp32.hip", "tissue_implicit.hip", "ldu_solver.hip",
-           "adjoint.hip", "ecg.hip", "capi.hip"]
+           "adjoint.hip", "ecg.hip", "general_mesh.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -16,6 +16,7 @@
 #include "cg_ldu.hpp"
 #include "foam_format.hpp"
 #include "cg_ecg.hpp"
+#include "cg_general.hpp"
 
 namespace {
 
@@ -175,6 +176,7 @@ struct cg_mesh {
     double* d_fy = nullptr;
     double* d_fz = nullptr;
     std::vector<double> h_sigma9;  // host copy for dt and implicit assembly
+    cg::GeneralMesh* gm = nullptr; // set for a general (polyhedral) mesh; g is unused then
     double* d_sigma9 = nullptr;    // per-cell tensors on the device, made on demand:
     bool sigma_sym = false;        // [6][n] symmetric (xx xy xz yy yz zz), else [9][n]
 };
@@ -197,6 +199,16 @@ struct cg_tissue {
     cg::Profiler prof;
     // pseudo-ECG buffers: electrode positions, per-block partial sums, results
     double* d_ecg = nullptr;   // [ECG_MAX_ELECTRODES * (3 + ECG_BLOCKS + 1)]
+    // general mesh, implicit: the lduMatrix solver, right-hand sides, controls, report
+    cg::LduSolver* gldu = nullptr;
+    double* d_b0 = nullptr;
+    double* d_b = nullptr;
+    double g_c = 0.0;                 // ddt coefficient the solver's matrix holds (0: none)
+    double g_tol = 0.0, g_rel = 0.0, g_rc = 0.0;
+    int g_max_iter = 1000, g_outer = 1;
+    int g_solves = 0;
+    double g_r0 = 0.0, g_r1 = 0.0;
+    int64_t g_iters = 0;
 };
 
 namespace {
@@ -834,6 +846,28 @@ int cg_mesh_create_hex(int nx, int ny, int nz, double hx, double hy, double hz, 
     return CG_OK;
 }
 
+int cg_mesh_create_general(int64_t n_cells, int64_t n_faces, const int32_t* owner,
+                           const int32_t* neighbour, const double* Sf, const double* Cf,
+                           const double* C, const double* V, int64_t n_boundary_faces,
+                           const int32_t* boundary_owner, const double* boundary_Sf,
+                           cg_mesh** out)
+{
+    if (!out) return fail(CG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!C || !V || (n_faces > 0 && (!owner || !neighbour || !Sf || !Cf)) ||
+        (n_boundary_faces > 0 && (!boundary_owner || !boundary_Sf)))
+        return fail(CG_ERR_ARG, "NULL mesh array");
+    cg::GeneralMesh* G = nullptr;
+    const int rc = cg::general_create(&G, n_cells, n_faces, owner, neighbour, Sf, Cf, C, V,
+                                      n_boundary_faces, boundary_owner, boundary_Sf, &g_err);
+    if (rc) return rc == 5 ? CG_ERR_NOMEM : CG_ERR_ARG;
+    cg_mesh* m = new cg_mesh();
+    m->gm = G;
+    m->n = n_cells;
+    *out = m;
+    return CG_OK;
+}
+
 int cg_mesh_match_ldu(int nx, int ny, int nz, int64_t nf, const int32_t* owner,
                       const int32_t* neighbour)
 {
@@ -862,6 +896,7 @@ int cg_mesh_destroy(cg_mesh* m)
     if (!m) return CG_OK;
     hipFree(m->d_od); hipFree(m->d_fx); hipFree(m->d_fy); hipFree(m->d_fz);
     hipFree(m->d_sigma9);
+    cg::general_destroy(m->gm);
     delete m;
     return CG_OK;
 }
@@ -886,6 +921,11 @@ int cg_mesh_set_conductivity(cg_mesh* m, const double* s, int uniform)
                 if (a != b && m->h_sigma9[9 * c + 3 * a + b] != 0.0) offdiag = true;
     m->has_corr = offdiag;
     m->uniform = uniform != 0;
+    if (m->gm) {
+        if (uniform) memcpy(m->sig, s, sizeof m->sig);
+        if (cg::general_set_sigma(m->gm, m->h_sigma9.data(), &g_err)) return CG_ERR_HIP;
+        return CG_OK;
+    }
     if (uniform) {
         memcpy(m->sig, s, sizeof m->sig);
     } else {
@@ -928,6 +968,11 @@ int cg_mesh_explicit_dt(const cg_mesh* m, double chi, double cm, double maxco, d
 {
     if (!m || !dt) return fail(CG_ERR_ARG, "NULL argument");
     if (m->h_sigma9.empty()) return fail(CG_ERR_STATE, "conductivity not set");
+    if (m->gm) {
+        if (cg::general_faces(m->gm) == 0) return fail(CG_ERR_STATE, "no internal faces");
+        *dt = cg::general_explicit_dt(m->gm, chi * cm, maxco);
+        return CG_OK;
+    }
     const cg::HexGeom& g = m->g;
     const double chiCm = chi * cm;
     double best = INFINITY;
@@ -965,6 +1010,11 @@ int cg_mesh_laplacian(cg_mesh* m, const double* V, double* out)
 {
     if (!m || !V || !out) return fail(CG_ERR_ARG, "NULL argument");
     if (m->h_sigma9.empty()) return fail(CG_ERR_STATE, "conductivity not set");
+    if (m->gm) {
+        HIPCHK(cg::general_laplacian(m->gm, V, out, nullptr));
+        HIPCHK(hipDeviceSynchronize());
+        return CG_OK;
+    }
     HIPCHK(cg::launch_hex_laplacian(m->g, mesh_coefs(m), V, m->d_od, m->od_mask, out, nullptr));
     HIPCHK(hipDeviceSynchronize());
     return CG_OK;
@@ -1003,7 +1053,18 @@ int cg_tissue_create(cg_mesh* m, cg_ionic* ion, const cg_tissue_config* cfg, cg_
     HIPCHK(hipMemset(t->d_Iion, 0, sizeof(double) * n));
     HIPCHK(hipMemset(t->d_flag, 1, n));
     HIPCHK(hipMemset(t->d_tact, 0, sizeof(double) * n));
-    if (cfg->algorithm == CG_IMPLICIT) {
+    if (cfg->algorithm == CG_IMPLICIT && m->gm) {
+        const int rc = cg::ldu_create(&t->gldu, n, cg::general_faces(m->gm),
+                                      cg::general_owner(m->gm), cg::general_neighbour(m->gm),
+                                      &g_err);
+        if (rc || hipMalloc(&t->d_b0, sizeof(double) * n) != hipSuccess ||
+            hipMalloc(&t->d_b, sizeof(double) * n) != hipSuccess) {
+            cg_tissue_destroy(t);
+            return rc == 1 ? CG_ERR_ARG : CG_ERR_NOMEM;
+        }
+        t->g_tol = cfg->tol;
+        t->g_max_iter = cfg->max_iter;
+    } else if (cfg->algorithm == CG_IMPLICIT) {
         int rc = cg::implicit_create(&t->implicit, m->g, m->h_sigma9.data(), cfg->chi * cfg->cm,
                                      cfg->dt, ion->prec, ion->stream, &g_err);
         if (rc != 0) {
@@ -1022,6 +1083,9 @@ int cg_tissue_destroy(cg_tissue* t)
     for (int q = 0; q < 3; ++q) hipFree(t->d_V[q]);
     hipFree(t->d_Iion); hipFree(t->d_flag); hipFree(t->d_tact);
     if (t->d_ecg) hipFree(t->d_ecg);
+    if (t->gldu) cg::ldu_destroy(t->gldu);
+    if (t->d_b0) hipFree(t->d_b0);
+    if (t->d_b) hipFree(t->d_b);
     if (t->implicit) cg::implicit_destroy(t->implicit);
     delete t;
     return CG_OK;
@@ -1057,9 +1121,90 @@ static cg::ActiveBoxes active_boxes(const cg_tissue* t, double t0)
     return a;
 }
 
+// The step on a general mesh (myocardiumDomain.C:487-529 with monodomainSolver.C:103-150):
+// the ionic kernel of cg_ionic_step on the device Vm, then the explicit update, or the
+// fvMatrix solved by the lduMatrix PCG with OpenFOAM's norm inside the PIMPLE corrector
+// loop (the tensor and non-orthogonal correction re-evaluated from the latest Vm).
+static int general_tissue_step(cg_tissue* t, int n_steps)
+{
+    cg_ionic* ion = t->ion;
+    cg_mesh* m = t->mesh;
+    hipStream_t s = ion->stream;
+    const double dt = t->cfg.dt, chiCm = t->cfg.chi * t->cfg.cm;
+    const double dtm = model_dt(ion->model, dt);
+    cg::OdeArgs ode0;
+    if (int rc = ode_args(ion, 0.0, dt, &ode0)) return rc;
+    std::vector<double> diag, upper;
+    for (int q = 0; q < n_steps; ++q) {
+        const double tn = t->t + dt;
+        const double t0 = tn - dt;
+        const bool bdf2 = t->cfg.ddt == CG_DDT_BACKWARD && t->steps >= 1;
+        cg::GeneralStep st{};
+        st.dt = dt; st.chiCm = chiCm; st.t_old = t0;
+        st.c = bdf2 ? 1.5 : 1.0; st.c0 = bdf2 ? 2.0 : 1.0; st.c00 = bdf2 ? 0.5 : 0.0;
+        st.boxes = active_boxes(t, t0);
+        cg::OdeArgs ode = ode0;
+        if (ode.rkf) model_step_times(ion->model, t0, dt, &ode.t0, &ode.t1);
+        double* Vn = t->d_V[t->cur];
+        double* Vm1 = t->d_V[(t->cur + 2) % 3];
+        double* Vnew = t->d_V[(t->cur + 1) % 3];
+        cg::Profiler* prof = t->prof.enabled ? &t->prof : nullptr;
+        int tk = prof ? prof->begin(cg::K_IONIC) : -1;
+        HIPCHK(cg::launch_ionic_tissue(ion->model, ion->prec, ion->tissue, ion->d_states, ion->n,
+                                       Vn, t->d_Iion, dtm, ion->P, ion->d_gscale, ode, s));
+        if (prof) prof->end(tk);
+        if (t->cfg.algorithm == CG_EXPLICIT) {
+            tk = prof ? prof->begin(cg::K_APPLY) : -1;
+            HIPCHK(cg::general_explicit(m->gm, Vn, Vm1, Vnew, t->d_Iion, t->d_flag, t->d_tact,
+                                        st, s));
+            if (prof) prof->end(tk);
+        } else {
+            if (t->g_c != st.c) {
+                diag.resize(m->n);
+                upper.resize(cg::general_faces(m->gm));
+                cg::general_matrix(m->gm, chiCm, st.c, dt, diag.data(), upper.data());
+                if (cg::ldu_set_coeffs(t->gldu, diag.data(), upper.data(), false))
+                    return fail(CG_ERR_HIP, "matrix upload failed");
+                t->g_c = st.c;
+            }
+            HIPCHK(cg::general_rhs0(m->gm, Vn, Vm1, t->d_Iion, t->d_b0, st, s));
+            HIPCHK(hipMemcpyAsync(Vnew, Vn, sizeof(double) * m->n, hipMemcpyDeviceToDevice, s));
+            // pimpleLoop: after the first corrector, an initial residual of the previous
+            // solve below residualControl makes the current corrector the last
+            const int n_outer = t->g_outer > 1 ? t->g_outer : 1;
+            double prev = 0.0;
+            int solves = 0;
+            for (int outer = 1;; ++outer) {
+                const bool last = n_outer == 1 || outer >= n_outer ||
+                                  (outer > 1 && prev < t->g_rc);
+                HIPCHK(cg::general_rhs_corr(m->gm, Vnew, t->d_b0, t->d_b, s));
+                HIPCHK(hipStreamSynchronize(s));
+                int it = 0;
+                double r0 = 0.0, r1 = 0.0;
+                if (cg::ldu_solve(t->gldu, Vnew, t->d_b, true, t->g_tol, t->g_rel,
+                                  t->g_max_iter, &it, &r0, &r1))
+                    return fail(CG_ERR_HIP, "lduMatrix solve failed");
+                t->g_iters += it;
+                t->g_r0 = r0;
+                t->g_r1 = r1;
+                prev = r0;
+                ++solves;
+                if (last) break;
+            }
+            t->g_solves = solves;
+            HIPCHK(cg::general_activation(m->gm, Vn, Vnew, t->d_flag, t->d_tact, t0, dt, s));
+        }
+        t->cur = (t->cur + 1) % 3;
+        t->t = tn;
+        t->steps++;
+    }
+    return CG_OK;
+}
+
 int cg_tissue_step(cg_tissue* t, int n_steps)
 {
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (t->mesh->gm) return general_tissue_step(t, n_steps);
     cg_ionic* ion = t->ion;
     cg_mesh* m = t->mesh;
     const double dt = t->cfg.dt, chiCm = t->cfg.chi * t->cfg.cm;
@@ -1155,6 +1300,7 @@ int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions
         return fail(CG_ERR_ARG, "between 0 and %d electrodes", cg::ECG_MAX_ELECTRODES);
     if (n_electrodes == 0) return CG_OK;
     cg_mesh* m = t->mesh;
+    if (m->gm) return fail(CG_ERR_STATE, "the pseudo-ECG entry covers the hex block");
     hipStream_t s = t->ion->stream;
     for (int i = 0; i < 3 * n_electrodes; ++i)
         if (!std::isfinite(positions[i])) return fail(CG_ERR_ARG, "electrode position not finite");
@@ -1247,6 +1393,11 @@ int cg_tissue_kernel_times(cg_tissue* t, int max, char* names, double* total_ms,
 int cg_tissue_solver_stats(const cg_tissue* t, int64_t* iters, double* res)
 {
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (t->gldu) {
+        if (iters) *iters = t->g_iters;
+        if (res) *res = t->g_r1;
+        return CG_OK;
+    }
     if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
     cg::implicit_stats(t->implicit, iters, res);
     return CG_OK;
@@ -1257,11 +1408,21 @@ int cg_tissue_set_solver_controls(cg_tissue* t, int norm, double tolerance, doub
                                   double residual_control)
 {
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
-    if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
+    if (!t->implicit && !t->gldu) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
     if (norm != CG_NORM_L2 && norm != CG_NORM_OPENFOAM)
         return fail(CG_ERR_ARG, "unknown residual norm %d", norm);
     if (!(tolerance >= 0.0) || !(rel_tol >= 0.0) || max_iter < 1 || !(residual_control >= 0.0))
         return fail(CG_ERR_ARG, "bad solver controls");
+    if (t->gldu) {
+        if (norm != CG_NORM_OPENFOAM)
+            return fail(CG_ERR_ARG, "a general mesh solves in OpenFOAM's residual norm only");
+        t->g_tol = tolerance;
+        t->g_rel = rel_tol;
+        t->g_max_iter = max_iter;
+        t->g_outer = n_outer_correctors > 1 ? n_outer_correctors : 1;
+        t->g_rc = residual_control;
+        return CG_OK;
+    }
     if (norm == CG_NORM_L2 && !(tolerance > 0.0))
         return fail(CG_ERR_ARG, "the relative L2 norm needs a positive tolerance");
     HIPCHK(hipStreamSynchronize(t->ion->stream));
@@ -1281,6 +1442,12 @@ int cg_tissue_set_solver_controls(cg_tissue* t, int norm, double tolerance, doub
 int cg_tissue_solver_report(const cg_tissue* t, int* solves, double* init_res, double* final_res)
 {
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (t->gldu) {
+        if (solves) *solves = t->g_solves;
+        if (init_res) *init_res = t->g_r0;
+        if (final_res) *final_res = t->g_r1;
+        return CG_OK;
+    }
     if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no linear solver");
     cg::implicit_report(t->implicit, solves, init_res, final_res);
     return CG_OK;
@@ -1379,6 +1546,7 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
     *out = nullptr;
     cg_ionic* ion = t->ion;
     cg_mesh* m = t->mesh;
+    if (m->gm) return fail(CG_ERR_STATE, "the adjoint covers the hex block");
     if (t->cfg.algorithm != CG_EXPLICIT || t->cfg.ddt != CG_DDT_EULER)
         return fail(CG_ERR_STATE, "the adjoint covers the explicit Euler tissue step");
     if (ion->model != cg::MODEL_BO || ion->prec != CG_FP64)
@@ -1456,6 +1624,7 @@ int cg_tissue_apply_operator(cg_tissue* t, double ddt_coeff, int precision, cons
                              void* y_dev)
 {
     if (!t || !x_dev || !y_dev) return fail(CG_ERR_ARG, "NULL argument");
+    if (t->mesh->gm) return fail(CG_ERR_STATE, "the operator entry covers the hex block");
     if (!t->implicit) return fail(CG_ERR_STATE, "explicit tissue has no implicit operator");
     if (precision != CG_FP64 && precision != CG_MIXED)
         return fail(CG_ERR_ARG, "unknown precision %d", precision);

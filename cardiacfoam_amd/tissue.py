@@ -21,17 +21,8 @@ def _dp(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-class HexMesh:
-    """Uniform nx*ny*nz hex block, blockMesh cell order c = i + nx*(j + ny*k)."""
-
-    def __init__(self, nx, ny, nz, h):
-        hx, hy, hz = (h, h, h) if np.isscalar(h) else h
-        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
-        self.h = (float(hx), float(hy), float(hz))
-        self.n = self.nx * self.ny * self.nz
-        m = C.c_void_p()
-        check(lib().cg_mesh_create_hex(self.nx, self.ny, self.nz, *self.h, C.byref(m)))
-        self.m = m
+class _Mesh:
+    """The device mesh handle and its conductivity (shared by HexMesh and PolyMesh)."""
 
     def __del__(self):
         if getattr(self, "m", None):
@@ -40,14 +31,6 @@ class HexMesh:
             except Exception:
                 pass
             self.m = None
-
-    def cell_centres(self):
-        i = np.arange(self.nx)
-        j = np.arange(self.ny)
-        k = np.arange(self.nz)
-        K, J, I = np.meshgrid(k, j, i, indexing="ij")
-        return np.stack([(I.ravel() + 0.5) * self.h[0], (J.ravel() + 0.5) * self.h[1],
-                         (K.ravel() + 0.5) * self.h[2]], axis=1)
 
     def set_conductivity(self, sigma):
         """sigma: 9 values (uniform tensor), 6 (symmTensor) or [n, 9]/[n, 3, 3] per cell."""
@@ -79,6 +62,54 @@ class HexMesh:
         dt = C.c_double()
         check(lib().cg_mesh_explicit_dt(self.m, chi, cm, max_co, C.byref(dt)))
         return dt.value
+
+
+class HexMesh(_Mesh):
+    """Uniform nx*ny*nz hex block, blockMesh cell order c = i + nx*(j + ny*k)."""
+
+    def __init__(self, nx, ny, nz, h):
+        hx, hy, hz = (h, h, h) if np.isscalar(h) else h
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.h = (float(hx), float(hy), float(hz))
+        self.n = self.nx * self.ny * self.nz
+        m = C.c_void_p()
+        check(lib().cg_mesh_create_hex(self.nx, self.ny, self.nz, *self.h, C.byref(m)))
+        self.m = m
+
+    def cell_centres(self):
+        i = np.arange(self.nx)
+        j = np.arange(self.ny)
+        k = np.arange(self.nz)
+        K, J, I = np.meshgrid(k, j, i, indexing="ij")
+        return np.stack([(I.ravel() + 0.5) * self.h[0], (J.ravel() + 0.5) * self.h[1],
+                         (K.ravel() + 0.5) * self.h[2]], axis=1)
+
+
+class PolyMesh(_Mesh):
+    """Any polyhedral mesh as the fvMesh provides it (cg_mesh_create_general): internal
+    faces in lduAddressing order (owner < neighbour) with area vectors Sf [F, 3] pointing
+    from owner to neighbour and centres Cf [F, 3], cell centres C [n, 3] and volumes V [n],
+    boundary faces by owner cell and outward area vector (zeroGradient for Vm)."""
+
+    def __init__(self, owner, neighbour, Sf, Cf, C_, V, boundary_owner, boundary_Sf):
+        own = np.ascontiguousarray(owner, dtype=np.int32)
+        nbr = np.ascontiguousarray(neighbour, dtype=np.int32)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (Sf, Cf, C_, V)]
+        bown = np.ascontiguousarray(boundary_owner, dtype=np.int32)
+        bsf = np.ascontiguousarray(boundary_Sf, dtype=np.float64)
+        self.n = int(arrs[3].size)
+        self.C = arrs[2].reshape(self.n, 3).copy()
+        self.V = arrs[3].copy()
+        m = C.c_void_p()
+        check(lib().cg_mesh_create_general(self.n, own.size, own.ctypes.data_as(C.c_void_p),
+                                           nbr.ctypes.data_as(C.c_void_p), _dp(arrs[0]),
+                                           _dp(arrs[1]), _dp(arrs[2]), _dp(arrs[3]), bown.size,
+                                           bown.ctypes.data_as(C.c_void_p), _dp(bsf),
+                                           C.byref(m)))
+        self.m = m
+
+    def cell_centres(self):
+        return self.C
 
 
 def fv_solution_controls(fv_solution, field="Vm"):

@@ -185,6 +185,21 @@ typedef struct cg_mesh cg_mesh;
  * for single-block hex meshes. */
 int cg_mesh_create_hex(int nx, int ny, int nz, double hx, double hy, double hz,
                        cg_mesh** out);
+/* Any polyhedral mesh, as the fvMesh hands it over (the cg_mesh_create of the survey's
+ * boundary; myocardiumSolver.H:34-106, monodomainSolver.C:103-150): n_faces internal faces
+ * in lduAddressing order (owner < neighbour, mesh.owner()/mesh.neighbour()), their area
+ * vectors Sf [n_faces][3] (pointing from owner to neighbour) and centres Cf [n_faces][3],
+ * cell centres C [n_cells][3] and volumes V [n_cells], and the boundary faces (all
+ * zeroGradient for Vm, as in every tissue case) by owner cell and outward Sf.  Host arrays,
+ * copied.  Tissues on this mesh run fvc::laplacian / fvm::laplacian with a tensor
+ * conductivity and Gauss linear corrected (non-orthogonal correction included) and solve
+ * the implicit system with the lduMatrix PCG of cg_ldu_solve in OpenFOAM's residual norm;
+ * cg_tissue_pseudo_ecg, cg_tissue_apply_operator and the adjoint cover the hex block only. */
+int cg_mesh_create_general(int64_t n_cells, int64_t n_faces, const int32_t* owner,
+                           const int32_t* neighbour, const double* Sf, const double* Cf,
+                           const double* C, const double* V, int64_t n_boundary_faces,
+                           const int32_t* boundary_owner, const double* boundary_Sf,
+                           cg_mesh** out);
 /* Checks that an lduAddressing (owner/neighbour, upper-triangular order) is the
  * blockMesh ordering of an nx*ny*nz block; 0 if it is. */
 int cg_mesh_match_ldu(int nx, int ny, int nz, int64_t n_faces, const int32_t* owner,
@@ -305,8 +320,9 @@ int cg_ldu_amul_timed(cg_ldu* L, const double* x, double* y, float* kernel_ms);
 
 /* ------------------------------------------------------- standalone operators */
 /* Explicit FV Laplacian div(sigma grad V) per unit volume (A/m^3) of the "Gauss linear
- * corrected" scheme with the mesh's conductivity (fvc::laplacian, monodomainSolver.C:111);
- * V and out are device pointers of n doubles. */
+ * corrected" scheme with the mesh's conductivity (fvc::laplacian, monodomainSolver.C:111),
+ * on the hex block or a general mesh (non-orthogonal correction included there);
+ * V and out are device pointers of n doubles.  Synchronous. */
 int cg_mesh_laplacian(cg_mesh* m, const double* V_dev, double* out_dev);
 
 /* ------------------------------------------------------------------ adjoint

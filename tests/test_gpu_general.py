@@ -1,0 +1,159 @@
+"""The tissue step on a general polyhedral mesh (cg_mesh_create_general): the mesh as the
+fvMesh hands it over, fvc::laplacian / fvm::laplacian of a tensor conductivity with Gauss
+linear corrected (non-orthogonal correction included), the implicit system solved by the
+lduMatrix PCG in OpenFOAM's norm inside the PIMPLE loop.  Checked against the hex path on
+an undeformed block, against the oracle's face-by-face restatement (oracle/general.py) on
+warped blocks, and by second-order convergence to the manufactured solution on a warped
+unit cube (volume-weighted L2 error; dropping the non-orthogonal correction stops the
+convergence, which the oracle's test shows)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from tests import manufactured as mf
+from tests import poly_mesh as pm
+from tests.gpu_cases import rotating_fibre_sigma
+
+pytestmark = pytest.mark.gpu
+
+BOX = {"stimulusLocationMin": [0.0, 0.0, 0.0], "stimulusLocationMax": [6e-4, 6e-4, 6e-4],
+       "stimulusDuration": 2e-3, "stimulusIntensity": 5e4, "stimulusStartTime": 0.0}
+BOXES = [((0.0, 0.0, 0.0), (6e-4, 6e-4, 6e-4), 0.0, 2e-3, 5e4)]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _coeffs(alg, sig):
+    return {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": 1.4e5,
+            "cm": 0.01, "solutionAlgorithm": alg, "conductivity": sig, "externalStimulus": BOX}
+
+
+def _poly(gpu, m):
+    return gpu.PolyMesh(*pm.mesh_args(m))
+
+
+def test_general_laplacian_vs_oracle(gpu, orc):
+    import torch
+    from oracle import general as og
+    _say("general laplacian")
+    shape, L = (10, 9, 8), (2e-3, 1.8e-3, 1.6e-3)
+    m = pm.block(*shape, lengths=L, warp=pm.smooth_warp(0.05, L))
+    sig = rotating_fibre_sigma(*shape)
+    mesh = _poly(gpu, m)
+    mesh.set_conductivity(sig)
+    V = np.random.default_rng(1).random(m["n"]) * 0.1 - 0.08
+    out = torch.empty(m["n"], dtype=torch.float64, device="cuda")
+    Vd = torch.tensor(V, device="cuda")
+    gpu._capi.check(gpu._capi.lib().cg_mesh_laplacian(mesh.m, ctypes.c_void_p(Vd.data_ptr()),
+                                                     ctypes.c_void_p(out.data_ptr())))
+    ref = og.laplacian(m, sig, V)
+    err = np.max(np.abs(out.cpu().numpy() - ref)) / np.max(np.abs(ref))
+    _say("  rel err %.3e" % err)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("alg", ["explicit", "implicit"])
+def test_general_block_equals_hex(gpu, orc, alg):
+    """An undeformed block through the general path equals the hex path (rotating fibres,
+    box stimulus, backward ddt)."""
+    _say("general block vs hex", alg)
+    shape, h, dt = (16, 12, 10), 2e-4, 1e-5
+    n_steps = 300 if alg == "explicit" else 150
+    L = tuple(s * h for s in shape)
+    sig = rotating_fibre_sigma(*shape)
+    gen = gpu.Monodomain(_poly(gpu, pm.block(*shape, lengths=L)), _coeffs(alg, sig), dt=dt,
+                         ddt="backward", tol=1e-13, max_iter=1000)
+    hexd = gpu.Monodomain(gpu.HexMesh(*shape, h), _coeffs(alg, sig), dt=dt, ddt="backward",
+                          tol=1e-13, max_iter=1000)
+    if alg == "implicit":
+        for d in (gen, hexd):
+            d.set_solver_controls("OpenFOAM", 1e-13, 0.0, 1000, 1, 0.0)
+    gen.step(n_steps)
+    hexd.step(n_steps)
+    err = np.max(np.abs(gen.Vm - hexd.Vm)) / np.max(np.abs(hexd.Vm))
+    ta, tb = gen.activationTime, hexd.activationTime
+    _say("  Vm rel %.3e, activated %d / %d" % (err, (ta > 0).sum(), (tb > 0).sum()))
+    assert err <= (1e-12 if alg == "explicit" else 1e-9)
+    assert (tb > 0).sum() > 0 and np.array_equal(ta > 0, tb > 0)
+    assert np.max(np.abs(ta - tb)) <= 1e-10
+
+
+@pytest.mark.parametrize("alg,n_outer", [("explicit", 1), ("implicit", 1), ("implicit", 1000)])
+def test_general_warped_vs_oracle(gpu, orc, alg, n_outer):
+    """A warped block (faces tilted up to ~15 degrees) with rotating fibres: the device
+    against the oracle's face-by-face restatement, the same solves per step."""
+    from oracle import general as og
+    _say("general warped vs oracle", alg, n_outer)
+    shape, dt = (12, 10, 8), 1e-5
+    L = (2.4e-3, 2e-3, 1.6e-3)
+    n_steps = 200 if alg == "explicit" else 260
+    m = pm.block(*shape, lengths=L, warp=pm.smooth_warp(0.05, L))
+    sig = rotating_fibre_sigma(*shape)
+    dom = gpu.Monodomain(_poly(gpu, m), _coeffs(alg, sig), dt=dt, ddt="backward", tol=1e-13,
+                         max_iter=1000)
+    rc = 1e-6 if n_outer > 1 else 0.0
+    if alg == "implicit":
+        dom.set_solver_controls("OpenFOAM", 1e-13, 0.0, 1000, n_outer, rc)
+    solves = []
+    for _ in range(n_steps):
+        dom.step(1)
+        if alg == "implicit":
+            solves.append(dom.solver_report()[0])
+    ref = og.run_tissue(orc, m, sig, 0, 1, dt, 1.4e5, 0.01, n_steps, implicit=alg == "implicit",
+                        backward=True, boxes=BOXES, tol=1e-13, n_outer=n_outer,
+                        residual_control=rc)
+    err = np.max(np.abs(dom.Vm - ref["V"])) / np.max(np.abs(ref["V"]))
+    ta = dom.activationTime
+    _say("  Vm rel %.3e, activated %d, solves %s / %s" %
+         (err, (ref["tact"] > 0).sum(), sorted(set(solves)), sorted(set(ref["solves"]))))
+    assert (ref["tact"] > 0).sum() > 0
+    assert err <= 1e-9
+    assert np.array_equal(ta > 0, ref["tact"] > 0)
+    assert np.max(np.abs(ta - ref["tact"])) <= 1e-10
+    if alg == "implicit":
+        assert solves == ref["solves"]
+        if n_outer > 1:
+            assert max(solves) > 2
+
+
+@pytest.mark.parametrize("alg", ["explicit", "implicit"])
+def test_general_manufactured_convergence(gpu, orc, alg):
+    """monodomainFDAManufactured on a warped unit cube (eps 0.03): the volume-weighted L2
+    error against the closed form at t = 0.05 s falls at second order (dt ~ h^2)."""
+    _say("general manufactured", alg)
+    t_end = 0.05
+    errs = []
+    for N in (8, 16, 24):
+        m = pm.block(N, N, N, warp=pm.smooth_warp(0.03))
+        x, y, z = m["C"].T
+        h = 1.0 / N
+        dt = (0.2 if alg == "implicit" else 0.05) * h * h
+        steps = int(round(t_end / dt))
+        dt = t_end / steps
+        coeffs = {"ionicModel": "monodomainFDAManufactured", "dimension": "3D", "chi": mf.CHI,
+                  "cm": mf.CM, "solutionAlgorithm": alg,
+                  "conductivity": [mf.SIGMA[0], 0, 0, 0, mf.SIGMA[1], 0, 0, 0, mf.SIGMA[2]],
+                  "solver": "RKF45", "absTol": 1e-12, "relTol": 1e-10, "maxSteps": 1000000}
+        dom = gpu.Monodomain(_poly(gpu, m), coeffs, dt=dt, ddt="backward", tol=1e-13,
+                             max_iter=2000)
+        if alg == "implicit":
+            dom.set_solver_controls("OpenFOAM", 1e-13, 0.0, 2000, 100, 1e-10)
+        V0, u1, u2, u3 = mf.fields(x, y, z, 0.0)
+        dom.set_field(gpu._capi.FIELD_VM, V0)
+        dom.set_field(gpu._capi.FIELD_VM_OLD, V0)
+        dom.ionic.set_states(np.stack([V0, u1, u2, u3]))
+        dom.step(steps)
+        Ve = mf.fields(x, y, z, t_end)[0]
+        e2 = np.sqrt(np.sum((dom.Vm - Ve) ** 2 * m["V"]))
+        errs.append(e2)
+        _say("  N %d, %d steps, L2 %.3e, Linf %.3e" % (N, steps, e2,
+                                                       np.max(np.abs(dom.Vm - Ve))))
+    orders = [np.log(errs[i] / errs[i + 1]) / np.log((16, 24)[i] / (8, 16)[i])
+              for i in range(2)]
+    _say("  orders", orders)
+    assert min(orders) > 1.7 and errs[-1] < 3e-4
