@@ -1,122 +1,3 @@
-"""Polyhedral test meshes with the geometry OpenFOAM's primitiveMesh computes: a hex block
-of nx*ny*nz cells whose points are moved by a map (so faces tilt and warp and the mesh is
-non-orthogonal), faces in OpenFOAM's order (internal faces by owner, then neighbour;
-boundary faces after), face centres and area vectors from the triangle fan around the
-face's point average, cell centres and volumes from the pyramids on each face around the
-average of the cell's face centres (primitiveMeshFaceCentresAndAreas.C,
-primitiveMeshCellCentresAndVols.C)."""
-import numpy as np
-
-
-def _face_geometry(P):
-    """P [F, k, 3] face points in order -> (Cf [F, 3], Sf [F, 3])"""
-    est = P.mean(axis=1)
-    sumN = np.zeros_like(est)
-    sumA = np.zeros(len(P))
-    sumAc = np.zeros_like(est)
-    k = P.shape[1]
-    for i in range(k):
-        a, b = P[:, i], P[:, (i + 1) % k]
-        c = a + b + est
-        nv = np.cross(b - a, est - a)
-        ar = np.linalg.norm(nv, axis=1)
-        sumN += nv
-        sumA += ar
-        sumAc += ar[:, None] * c
-    return sumAc / (3.0 * sumA[:, None]), 0.5 * sumN
-
-
-def block(nx, ny, nz, lengths=(1.0, 1.0, 1.0), warp=None):
-    """dict(owner, neighbour, Sf, Cf, C, V, b_owner, b_Sf, n) of the (warped) block.
-    warp(x, y, z) -> (x', y', z') moves the points of the regular lattice."""
-    Lx, Ly, Lz = lengths
-    xs, ys, zs = (np.linspace(0, Lx, nx + 1), np.linspace(0, Ly, ny + 1),
-                  np.linspace(0, Lz, nz + 1))
-    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
-    if warp is not None:
-        X, Y, Z = warp(X, Y, Z)
-    pts = np.stack([X, Y, Z], axis=-1)          # [nz+1, ny+1, nx+1, 3]
-
-    def p(i, j, k):
-        return pts[k, j, i]
-
-    n = nx * ny * nz
-    cell = lambda i, j, k: i + nx * (j + ny * k)
-    owner, nbr, fpts = [], [], []
-    for k in range(nz):
-        for j in range(ny):
-            for i in range(nx):
-                c = cell(i, j, k)
-                if i < nx - 1:   # +x face, normal from c to c+1
-                    owner.append(c); nbr.append(c + 1)
-                    fpts.append([p(i + 1, j, k), p(i + 1, j + 1, k), p(i + 1, j + 1, k + 1),
-                                 p(i + 1, j, k + 1)])
-                if j < ny - 1:   # +y face
-                    owner.append(c); nbr.append(c + nx)
-                    fpts.append([p(i, j + 1, k), p(i, j + 1, k + 1), p(i + 1, j + 1, k + 1),
-                                 p(i + 1, j + 1, k)])
-                if k < nz - 1:   # +z face
-                    owner.append(c); nbr.append(c + nx * ny)
-                    fpts.append([p(i, j, k + 1), p(i + 1, j, k + 1), p(i + 1, j + 1, k + 1),
-                                 p(i, j + 1, k + 1)])
-    bown, bpts = [], []
-    for k in range(nz):
-        for j in range(ny):
-            bown.append(cell(0, j, k))        # x = 0 wall, outward -x
-            bpts.append([p(0, j, k), p(0, j, k + 1), p(0, j + 1, k + 1), p(0, j + 1, k)])
-            bown.append(cell(nx - 1, j, k))   # x = Lx wall, outward +x
-            bpts.append([p(nx, j, k), p(nx, j + 1, k), p(nx, j + 1, k + 1), p(nx, j, k + 1)])
-    for k in range(nz):
-        for i in range(nx):
-            bown.append(cell(i, 0, k))
-            bpts.append([p(i, 0, k), p(i + 1, 0, k), p(i + 1, 0, k + 1), p(i, 0, k + 1)])
-            bown.append(cell(i, ny - 1, k))
-            bpts.append([p(i, ny, k), p(i, ny, k + 1), p(i + 1, ny, k + 1), p(i + 1, ny, k)])
-    for j in range(ny):
-        for i in range(nx):
-            bown.append(cell(i, j, 0))
-            bpts.append([p(i, j, 0), p(i, j + 1, 0), p(i + 1, j + 1, 0), p(i + 1, j, 0)])
-            bown.append(cell(i, j, nz - 1))
-            bpts.append([p(i, j, nz), p(i + 1, j, nz), p(i + 1, j + 1, nz), p(i, j + 1, nz)])
-    owner, nbr, bown = np.array(owner), np.array(nbr), np.array(bown)
-    Cf, Sf = _face_geometry(np.array(fpts))
-    bCf, bSf = _face_geometry(np.array(bpts))
-    # cells: pyramids on every face around the average of the cell's face centres
-    cnt = np.zeros(n)
-    est = np.zeros((n, 3))
-    for o, cf in ((owner, Cf), (nbr, Cf), (bown, bCf)):
-        np.add.at(est, o, cf)
-        np.add.at(cnt, o, 1.0)
-    est /= cnt[:, None]
-    vol3 = np.zeros(n)
-    ctr = np.zeros((n, 3))
-    for o, cf, sf, sgn in ((owner, Cf, Sf, 1.0), (nbr, Cf, Sf, -1.0), (bown, bCf, bSf, 1.0)):
-        pv = sgn * np.einsum("ij,ij->i", sf, cf - est[o])
-        pc = 0.75 * cf + 0.25 * est[o]
-        np.add.at(vol3, o, pv)
-        np.add.at(ctr, o, pv[:, None] * pc)
-    C = ctr / vol3[:, None]
-    V = vol3 / 3.0
-    return dict(n=n, owner=owner, neighbour=nbr, Sf=Sf, Cf=Cf, C=C, V=V, b_owner=bown,
-                b_Sf=bSf, b_Cf=bCf)
-
-
-def smooth_warp(eps, lengths=(1.0, 1.0, 1.0)):
-    """A displacement (eps times the box size) that vanishes on the box's walls, so the
-    domain stays the box while the interior faces tilt and the cells skew."""
-    Lx, Ly, Lz = lengths
-
-    def w(X, Y, Z):
-        s = np.sin
-        x, y, z = X / Lx, Y / Ly, Z / Lz
-        dx = eps * s(np.pi * x) * s(2 * np.pi * y) * s(np.pi * z)
-        dy = eps * s(np.pi * y) * s(2 * np.pi * z) * s(np.pi * x)
-        dz = eps * s(np.pi * z) * s(2 * np.pi * x) * s(np.pi * y)
-        return X + dx * Lx, Y + dy * Ly, Z + dz * Lz
-    return w
-
-
-def mesh_args(m):
-    """the arguments of cardiacfoam_amd.PolyMesh"""
-    return (m["owner"], m["neighbour"], m["Sf"], m["Cf"], m["C"], m["V"], m["b_owner"],
-            m["b_Sf"])
+"""Polyhedral test meshes: the package's block generator (cardiacfoam_amd/meshes.py, with
+OpenFOAM's primitiveMesh geometry) and its smooth warp."""
+from cardiacfoam_amd.meshes import block, mesh_args, smooth_warp  # noqa: F401
