@@ -1300,7 +1300,6 @@ int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions
         return fail(CG_ERR_ARG, "between 0 and %d electrodes", cg::ECG_MAX_ELECTRODES);
     if (n_electrodes == 0) return CG_OK;
     cg_mesh* m = t->mesh;
-    if (m->gm) return fail(CG_ERR_STATE, "the pseudo-ECG entry covers the hex block");
     hipStream_t s = t->ion->stream;
     for (int i = 0; i < 3 * n_electrodes; ++i)
         if (!std::isfinite(positions[i])) return fail(CG_ERR_ARG, "electrode position not finite");
@@ -1337,9 +1336,14 @@ int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions
                           hipMemcpyHostToDevice, s));
     cg::Profiler* prof = t->prof.enabled ? &t->prof : nullptr;
     const int tk = prof ? prof->begin(cg::K_ECG) : -1;
-    HIPCHK(cg::launch_pseudo_ecg(m->g, t->d_V[t->cur], m->uniform ? nullptr : m->d_sigma9,
-                                 m->sigma_sym ? 1 : 0, m->uniform ? m->sig : nullptr,
-                                 n_electrodes, pos, partial, out, s));
+    if (m->gm)
+        HIPCHK(cg::general_pseudo_ecg(m->gm, t->d_V[t->cur], m->uniform ? nullptr : m->d_sigma9,
+                                      m->sigma_sym ? 1 : 0, m->uniform ? m->sig : nullptr,
+                                      n_electrodes, pos, cg::ECG_BLOCKS, partial, out, s));
+    else
+        HIPCHK(cg::launch_pseudo_ecg(m->g, t->d_V[t->cur], m->uniform ? nullptr : m->d_sigma9,
+                                     m->sigma_sym ? 1 : 0, m->uniform ? m->sig : nullptr,
+                                     n_electrodes, pos, partial, out, s));
     if (prof) prof->end(tk);
     HIPCHK(hipMemcpyAsync(values, out, sizeof(double) * n_electrodes, hipMemcpyDeviceToHost,
                           s));

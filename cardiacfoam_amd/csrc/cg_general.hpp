@@ -56,6 +56,12 @@ hipError_t general_rhs_corr(GeneralMesh* G, const double* psi, const double* b0,
 hipError_t general_activation(GeneralMesh* G, const double* Vn, const double* Vnew,
                               uint8_t* flag, double* tact, double t_old, double dt,
                               hipStream_t s);
+// pseudo-ECG (pseudoECGSolver.C:36-83) of V at n_el electrodes pos [E][3] (device): the
+// conductivity per cell sig ([6][n] when sym, else [9][n]; device) or the uniform s9 (host);
+// partial [E][nblocks] scratch, out [E] (device).  Asynchronous on s.
+hipError_t general_pseudo_ecg(GeneralMesh* G, const double* V, const double* sig, int sym,
+                              const double* s9, int n_el, const double* pos, int nblocks,
+                              double* partial, double* out, hipStream_t s);
 // div(sigma grad V) per unit volume into lap (device), for tests and adapters
 hipError_t general_laplacian(GeneralMesh* G, const double* V, double* lap, hipStream_t s);
 

@@ -38,6 +38,10 @@ namespace {
 constexpr int GBS = 256;
 constexpr int32_t EMPTY = INT32_MIN;  // unused slot (~f of a face f >= 0 is at least -2^31+1)
 
+struct EcgSigma {
+    double v[9];
+};
+
 struct GView {
     int64_t n;
     int K, KB;
@@ -209,6 +213,96 @@ __global__ __launch_bounds__(GBS) void k_gactivation(int64_t n, const double* __
         flag[c] = 0;
         tact[c] = t_old + (0.0 - vn) / (vnew - vn) * dt;
     }
+}
+
+// pseudo-ECG partial sums on a general mesh (the dipole of pseudoECGSolver.C:36-83 from the
+// Gauss gradient, the cell centres and volumes); one pass covers up to 16 electrodes
+constexpr int GE_EG = 16;
+__global__ __launch_bounds__(GBS) void k_gecg(GView g, const double* __restrict__ grad,
+                                              const double* __restrict__ sig, int sym,
+                                              EcgSigma s9, const double* __restrict__ pos,
+                                              int e0, int ne, double* __restrict__ partial)
+{
+    double px[GE_EG], py[GE_EG], pz[GE_EG], acc[GE_EG];
+#pragma unroll
+    for (int q = 0; q < GE_EG; ++q) {
+        const int e = e0 + (q < ne ? q : 0);
+        px[q] = pos[3 * e];
+        py[q] = pos[3 * e + 1];
+        pz[q] = pos[3 * e + 2];
+        acc[q] = 0.0;
+    }
+    const int64_t stride = (int64_t)gridDim.x * GBS;
+    for (int64_t c = (int64_t)blockIdx.x * GBS + threadIdx.x; c < g.n; c += stride) {
+        const double gx = grad[c], gy = grad[g.n + c], gz = grad[2 * g.n + c];
+        double s[9];
+        if (sig && sym) {
+            const double* p = sig + c;
+            s[0] = p[0]; s[1] = p[g.n]; s[2] = p[2 * g.n];
+            s[3] = s[1]; s[4] = p[3 * g.n]; s[5] = p[4 * g.n];
+            s[6] = s[2]; s[7] = s[5]; s[8] = p[5 * g.n];
+        } else if (sig) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) s[q] = sig[q * g.n + c];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) s[q] = s9.v[q];
+        }
+        const double vol = g.vol[c];
+        const double dx = (s[0] * gx + s[1] * gy + s[2] * gz) * vol;
+        const double dy = (s[3] * gx + s[4] * gy + s[5] * gz) * vol;
+        const double dz = (s[6] * gx + s[7] * gy + s[8] * gz) * vol;
+        const double cx = g.C[c], cy = g.C[g.n + c], cz = g.C[2 * g.n + c];
+#pragma unroll
+        for (int q = 0; q < GE_EG; ++q) {
+            if (q < ne) {
+                const double rx = cx - px[q], ry = cy - py[q], rz = cz - pz[q];
+                const double r2 = rx * rx + ry * ry + rz * rz;
+                // |r| > VSMALL as r2 > 0; 1/|r| from the reciprocal square root and one
+                // Newton step
+                if (r2 > 0.0) {
+                    double y = __builtin_amdgcn_rsq(r2);
+                    y = y * (1.5 - 0.5 * r2 * y * y);
+                    acc[q] += (dx * rx + dy * ry + dz * rz) * (y * y * y);
+                }
+            }
+        }
+    }
+    constexpr int NW = GBS / 64;
+    __shared__ double red[NW][GE_EG];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < GE_EG; ++q) {
+        if (q < ne) {
+            double v = acc[q];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) red[wave][q] = v;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ne) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) v += red[w][threadIdx.x];
+        partial[(int64_t)(e0 + threadIdx.x) * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(GBS) void k_gecg_sum(const double* __restrict__ partial,
+                                                  int nblocks, double* __restrict__ out)
+{
+    __shared__ double red[GBS];
+    const double* p = partial + (int64_t)blockIdx.x * nblocks;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += GBS) v += p[b];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = GBS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 } // namespace
@@ -485,6 +579,26 @@ hipError_t general_rhs_corr(GeneralMesh* G, const double* psi, const double* b0,
     hipLaunchKernelGGL(k_ggrad, dim3(grid_of(G->n)), dim3(GBS), 0, s, g, psi, G->grad);
     hipLaunchKernelGGL(k_grhs_corr, dim3(grid_of(G->n)), dim3(GBS), 0, s, g, psi,
                        (const double*)G->grad, b0, b);
+    return hipGetLastError();
+}
+
+hipError_t general_pseudo_ecg(GeneralMesh* G, const double* V, const double* sig, int sym,
+                              const double* s9, int n_el, const double* pos, int nblocks,
+                              double* partial, double* out, hipStream_t s)
+{
+    const GView g = G->view();
+    hipLaunchKernelGGL(k_ggrad, dim3(grid_of(G->n)), dim3(GBS), 0, s, g, V, G->grad);
+    EcgSigma u{};
+    for (int q = 0; q < 9; ++q) u.v[q] = s9 ? s9[q] : 0.0;
+    const int64_t want = (G->n + GBS - 1) / GBS;
+    const int nb = (int)(want < nblocks ? want : nblocks);
+    for (int e0 = 0; e0 < n_el; e0 += GE_EG) {
+        const int ne = n_el - e0 < GE_EG ? n_el - e0 : GE_EG;
+        hipLaunchKernelGGL(k_gecg, dim3(nb), dim3(GBS), 0, s, g, (const double*)G->grad, sig,
+                           sym, u, pos, e0, ne, partial);
+    }
+    hipLaunchKernelGGL(k_gecg_sum, dim3(n_el), dim3(GBS), 0, s, (const double*)partial, nb,
+                       out);
     return hipGetLastError();
 }
 

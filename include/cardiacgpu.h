@@ -194,7 +194,7 @@ int cg_mesh_create_hex(int nx, int ny, int nz, double hx, double hy, double hz,
  * copied.  Tissues on this mesh run fvc::laplacian / fvm::laplacian with a tensor
  * conductivity and Gauss linear corrected (non-orthogonal correction included) and solve
  * the implicit system with the lduMatrix PCG of cg_ldu_solve in OpenFOAM's residual norm;
- * cg_tissue_pseudo_ecg, cg_tissue_apply_operator and the adjoint cover the hex block only. */
+ * cg_tissue_apply_operator and the adjoint cover the hex block only. */
 int cg_mesh_create_general(int64_t n_cells, int64_t n_faces, const int32_t* owner,
                            const int32_t* neighbour, const double* Sf, const double* Cf,
                            const double* C, const double* V, int64_t n_boundary_faces,

@@ -157,3 +157,46 @@ def test_general_manufactured_convergence(gpu, orc, alg):
               for i in range(2)]
     _say("  orders", orders)
     assert min(orders) > 1.7 and errs[-1] < 3e-4
+
+
+def test_general_pseudo_ecg(gpu, orc):
+    """The pseudo-ECG on a general mesh: equal to the hex path on an undeformed block, and
+    to the dipole sum over the oracle's Gauss gradient, cell centres and volumes on a
+    warped block with rotating fibres (pseudoECGSolver.C:36-83)."""
+    from oracle import general as og
+    _say("general pseudo-ECG")
+    shape, h = (16, 12, 10), 2e-4
+    L = tuple(s * h for s in shape)
+    sig = rotating_fibre_sigma(*shape)
+    rng = np.random.default_rng(3)
+    # electrodes around and inside the block (none on a cell centre: there the hex path's
+    # centre is exact and the general path's is a rounding away from it)
+    E = rng.uniform(-1, 2, size=(19, 3)) * np.array(L)
+    V = -0.084 + 0.1 * rng.random(int(np.prod(shape)))
+    gen = gpu.Monodomain(_poly(gpu, pm.block(*shape, lengths=L)), _coeffs("explicit", sig),
+                         dt=1e-5)
+    hexd = gpu.Monodomain(gpu.HexMesh(*shape, h), _coeffs("explicit", sig), dt=1e-5)
+    for d in (gen, hexd):
+        d.set_field(gpu._capi.FIELD_VM, V)
+    a, b = gen.pseudo_ecg(E), hexd.pseudo_ecg(E)
+    scale = orc.OracleMesh(*shape, h, h, h).pseudo_ecg(sig, V, E)[1]
+    _say("  block vs hex %.3e" % np.max(np.abs(a - b) / scale))
+    assert np.all(np.abs(a - b) <= 1e-13 * scale)
+    m = pm.block(*shape, lengths=L, warp=pm.smooth_warp(0.05, L))
+    w = gpu.Monodomain(_poly(gpu, m), _coeffs("explicit", sig), dt=1e-5)
+    w.set_field(gpu._capi.FIELD_VM, V)
+    got = w.pseudo_ecg(E)
+    g = og.grad(m, V)
+    S = sig.reshape(-1, 3, 3)
+    dip = np.einsum("cij,cj->ci", S, g) * m["V"][:, None]
+    ref = np.zeros(len(E))
+    ab = np.zeros(len(E))
+    for e, P in enumerate(E):
+        r = m["C"] - P
+        rn = np.linalg.norm(r, axis=1)
+        ok = rn > 1e-300
+        term = np.einsum("ci,ci->c", dip[ok], r[ok]) / rn[ok] ** 3
+        ref[e], ab[e] = term.sum(), np.abs(term).sum()
+    err = np.abs(got - ref) / ab
+    _say("  warped vs oracle %.3e" % err.max())
+    assert np.all(err <= 1e-13)
