@@ -201,6 +201,8 @@ struct cg_tissue {
     double* d_ecg = nullptr;   // [ECG_MAX_ELECTRODES * (3 + ECG_BLOCKS + 1)]
     // general mesh, implicit: the lduMatrix solver, right-hand sides, controls, report
     cg::LduSolver* gldu = nullptr;
+    cg::GeneralMesh* own_gm = nullptr;   // a hex tissue's block as a general mesh (diffusion entry)
+    double* d_src = nullptr;             // caller's source field (diffusion entry)
     double* d_b0 = nullptr;
     double* d_b = nullptr;
     double g_c = 0.0;                 // ddt coefficient the solver's matrix holds (0: none)
@@ -1053,6 +1055,8 @@ int cg_tissue_create(cg_mesh* m, cg_ionic* ion, const cg_tissue_config* cfg, cg_
     HIPCHK(hipMemset(t->d_Iion, 0, sizeof(double) * n));
     HIPCHK(hipMemset(t->d_flag, 1, n));
     HIPCHK(hipMemset(t->d_tact, 0, sizeof(double) * n));
+    t->g_tol = cfg->tol;
+    t->g_max_iter = cfg->max_iter;
     if (cfg->algorithm == CG_IMPLICIT && m->gm) {
         const int rc = cg::ldu_create(&t->gldu, n, cg::general_faces(m->gm),
                                       cg::general_owner(m->gm), cg::general_neighbour(m->gm),
@@ -1062,8 +1066,6 @@ int cg_tissue_create(cg_mesh* m, cg_ionic* ion, const cg_tissue_config* cfg, cg_
             cg_tissue_destroy(t);
             return rc == 1 ? CG_ERR_ARG : CG_ERR_NOMEM;
         }
-        t->g_tol = cfg->tol;
-        t->g_max_iter = cfg->max_iter;
     } else if (cfg->algorithm == CG_IMPLICIT) {
         int rc = cg::implicit_create(&t->implicit, m->g, m->h_sigma9.data(), cfg->chi * cfg->cm,
                                      cfg->dt, ion->prec, ion->stream, &g_err);
@@ -1084,6 +1086,8 @@ int cg_tissue_destroy(cg_tissue* t)
     hipFree(t->d_Iion); hipFree(t->d_flag); hipFree(t->d_tact);
     if (t->d_ecg) hipFree(t->d_ecg);
     if (t->gldu) cg::ldu_destroy(t->gldu);
+    cg::general_destroy(t->own_gm);
+    if (t->d_src) hipFree(t->d_src);
     if (t->d_b0) hipFree(t->d_b0);
     if (t->d_b) hipFree(t->d_b);
     if (t->implicit) cg::implicit_destroy(t->implicit);
@@ -1125,7 +1129,8 @@ static cg::ActiveBoxes active_boxes(const cg_tissue* t, double t0)
 // the ionic kernel of cg_ionic_step on the device Vm, then the explicit update, or the
 // fvMatrix solved by the lduMatrix PCG with OpenFOAM's norm inside the PIMPLE corrector
 // loop (the tensor and non-orthogonal correction re-evaluated from the latest Vm).
-static int general_tissue_step(cg_tissue* t, int n_steps)
+static int general_advance(cg_tissue* t, cg::GeneralMesh* G, int n_steps, bool ionic,
+                           const double* src_dev)
 {
     cg_ionic* ion = t->ion;
     cg_mesh* m = t->mesh;
@@ -1143,31 +1148,36 @@ static int general_tissue_step(cg_tissue* t, int n_steps)
         st.dt = dt; st.chiCm = chiCm; st.t_old = t0;
         st.c = bdf2 ? 1.5 : 1.0; st.c0 = bdf2 ? 2.0 : 1.0; st.c00 = bdf2 ? 0.5 : 0.0;
         st.boxes = active_boxes(t, t0);
+        st.src = src_dev;
         cg::OdeArgs ode = ode0;
         if (ode.rkf) model_step_times(ion->model, t0, dt, &ode.t0, &ode.t1);
         double* Vn = t->d_V[t->cur];
         double* Vm1 = t->d_V[(t->cur + 2) % 3];
         double* Vnew = t->d_V[(t->cur + 1) % 3];
         cg::Profiler* prof = t->prof.enabled ? &t->prof : nullptr;
-        int tk = prof ? prof->begin(cg::K_IONIC) : -1;
-        HIPCHK(cg::launch_ionic_tissue(ion->model, ion->prec, ion->tissue, ion->d_states, ion->n,
-                                       Vn, t->d_Iion, dtm, ion->P, ion->d_gscale, ode, s));
-        if (prof) prof->end(tk);
+        int tk = -1;
+        if (ionic) {
+            tk = prof ? prof->begin(cg::K_IONIC) : -1;
+            HIPCHK(cg::launch_ionic_tissue(ion->model, ion->prec, ion->tissue, ion->d_states,
+                                           ion->n, Vn, t->d_Iion, dtm, ion->P, ion->d_gscale,
+                                           ode, s));
+            if (prof) prof->end(tk);
+        }
         if (t->cfg.algorithm == CG_EXPLICIT) {
             tk = prof ? prof->begin(cg::K_APPLY) : -1;
-            HIPCHK(cg::general_explicit(m->gm, Vn, Vm1, Vnew, t->d_Iion, t->d_flag, t->d_tact,
+            HIPCHK(cg::general_explicit(G, Vn, Vm1, Vnew, t->d_Iion, t->d_flag, t->d_tact,
                                         st, s));
             if (prof) prof->end(tk);
         } else {
             if (t->g_c != st.c) {
                 diag.resize(m->n);
-                upper.resize(cg::general_faces(m->gm));
-                cg::general_matrix(m->gm, chiCm, st.c, dt, diag.data(), upper.data());
+                upper.resize(cg::general_faces(G));
+                cg::general_matrix(G, chiCm, st.c, dt, diag.data(), upper.data());
                 if (cg::ldu_set_coeffs(t->gldu, diag.data(), upper.data(), false))
                     return fail(CG_ERR_HIP, "matrix upload failed");
                 t->g_c = st.c;
             }
-            HIPCHK(cg::general_rhs0(m->gm, Vn, Vm1, t->d_Iion, t->d_b0, st, s));
+            HIPCHK(cg::general_rhs0(G, Vn, Vm1, t->d_Iion, t->d_b0, st, s));
             HIPCHK(hipMemcpyAsync(Vnew, Vn, sizeof(double) * m->n, hipMemcpyDeviceToDevice, s));
             // pimpleLoop: after the first corrector, an initial residual of the previous
             // solve below residualControl makes the current corrector the last
@@ -1177,7 +1187,7 @@ static int general_tissue_step(cg_tissue* t, int n_steps)
             for (int outer = 1;; ++outer) {
                 const bool last = n_outer == 1 || outer >= n_outer ||
                                   (outer > 1 && prev < t->g_rc);
-                HIPCHK(cg::general_rhs_corr(m->gm, Vnew, t->d_b0, t->d_b, s));
+                HIPCHK(cg::general_rhs_corr(G, Vnew, t->d_b0, t->d_b, s));
                 HIPCHK(hipStreamSynchronize(s));
                 int it = 0;
                 double r0 = 0.0, r1 = 0.0;
@@ -1192,7 +1202,7 @@ static int general_tissue_step(cg_tissue* t, int n_steps)
                 if (last) break;
             }
             t->g_solves = solves;
-            HIPCHK(cg::general_activation(m->gm, Vn, Vnew, t->d_flag, t->d_tact, t0, dt, s));
+            HIPCHK(cg::general_activation(G, Vn, Vnew, t->d_flag, t->d_tact, t0, dt, s));
         }
         t->cur = (t->cur + 1) % 3;
         t->t = tn;
@@ -1201,10 +1211,44 @@ static int general_tissue_step(cg_tissue* t, int n_steps)
     return CG_OK;
 }
 
+int cg_tissue_diffusion_step(cg_tissue* t, const double* Iion, const double* source, int flags)
+{
+    if (!t || !Iion) return fail(CG_ERR_ARG, "NULL argument");
+    cg_mesh* m = t->mesh;
+    hipStream_t s = t->ion->stream;
+    const size_t bytes = sizeof(double) * m->n;
+    cg::GeneralMesh* G = m->gm;
+    if (!G) {
+        if (!t->own_gm) {
+            const int rc = cg::general_from_hex(&t->own_gm, m->g, &g_err);
+            if (rc) return rc == 5 ? CG_ERR_NOMEM : CG_ERR_ARG;
+            if (cg::general_set_sigma(t->own_gm, m->h_sigma9.data(), &g_err)) return CG_ERR_HIP;
+        }
+        G = t->own_gm;
+    }
+    if (t->cfg.algorithm == CG_IMPLICIT && !t->gldu) {
+        const int rc = cg::ldu_create(&t->gldu, m->n, cg::general_faces(G), cg::general_owner(G),
+                                      cg::general_neighbour(G), &g_err);
+        if (rc) return rc == 1 ? CG_ERR_ARG : CG_ERR_NOMEM;
+        if (hipMalloc(&t->d_b0, bytes) != hipSuccess || hipMalloc(&t->d_b, bytes) != hipSuccess)
+            return fail(CG_ERR_NOMEM, "right-hand side allocation failed");
+    }
+    if (source && !t->d_src && hipMalloc(&t->d_src, bytes) != hipSuccess)
+        return fail(CG_ERR_NOMEM, "source allocation failed");
+    const hipMemcpyKind k = (flags & CG_DEVICE_PTRS) ? hipMemcpyDeviceToDevice
+                                                    : hipMemcpyHostToDevice;
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpyAsync(t->d_Iion, Iion, bytes, k, s));
+    if (source) HIPCHK(hipMemcpyAsync(t->d_src, source, bytes, k, s));
+    if (int rc = general_advance(t, G, 1, false, source ? t->d_src : nullptr)) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    return CG_OK;
+}
+
 int cg_tissue_step(cg_tissue* t, int n_steps)
 {
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
-    if (t->mesh->gm) return general_tissue_step(t, n_steps);
+    if (t->mesh->gm) return general_advance(t, t->mesh->gm, n_steps, true, nullptr);
     cg_ionic* ion = t->ion;
     cg_mesh* m = t->mesh;
     const double dt = t->cfg.dt, chiCm = t->cfg.chi * t->cfg.cm;
@@ -1417,16 +1461,16 @@ int cg_tissue_set_solver_controls(cg_tissue* t, int norm, double tolerance, doub
         return fail(CG_ERR_ARG, "unknown residual norm %d", norm);
     if (!(tolerance >= 0.0) || !(rel_tol >= 0.0) || max_iter < 1 || !(residual_control >= 0.0))
         return fail(CG_ERR_ARG, "bad solver controls");
-    if (t->gldu) {
-        if (norm != CG_NORM_OPENFOAM)
-            return fail(CG_ERR_ARG, "a general mesh solves in OpenFOAM's residual norm only");
+    if (t->mesh->gm && norm != CG_NORM_OPENFOAM)
+        return fail(CG_ERR_ARG, "a general mesh solves in OpenFOAM's residual norm only");
+    if (norm == CG_NORM_OPENFOAM) {  // also the controls of cg_tissue_diffusion_step
         t->g_tol = tolerance;
         t->g_rel = rel_tol;
         t->g_max_iter = max_iter;
         t->g_outer = n_outer_correctors > 1 ? n_outer_correctors : 1;
         t->g_rc = residual_control;
-        return CG_OK;
     }
+    if (t->mesh->gm) return CG_OK;
     if (norm == CG_NORM_L2 && !(tolerance > 0.0))
         return fail(CG_ERR_ARG, "the relative L2 norm needs a positive tolerance");
     HIPCHK(hipStreamSynchronize(t->ion->stream));

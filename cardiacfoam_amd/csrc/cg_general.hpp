@@ -33,7 +33,12 @@ int64_t general_faces(const GeneralMesh* G);
 struct GeneralStep {
     double dt, chiCm, c, c0, c00, t_old;
     ActiveBoxes boxes;
+    const double* src;  // source field [n] (device, A/m^3) in place of the boxes, or null
 };
+
+// the general-mesh description of a uniform hex block (blockMesh order, walls as boundary
+// faces), for the diffusion-only entry on a hex tissue
+int general_from_hex(GeneralMesh** out, const HexGeom& g, std::string* err);
 
 // explicit step: Vnew = (c0 Vn - c00 Vm1 + dt/chiCm (lap(Vn) - chiCm Iion + src)) / c and the
 // activation update.  Asynchronous on s.

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(GBS) void k_gexplicit(GView g, const double* __rest
     if (c >= g.n) return;
     const double vn = V[c];
     const double lap = flux_sum(g, V, grad, c, vn, true) / g.vol[c];
-    const double src = general_box(g, c, st.boxes);
+    const double src = st.src ? st.src[c] : general_box(g, c, st.boxes);
     double rhs = st.c0 * vn + st.dt / st.chiCm * (lap - st.chiCm * Iion[c] + src);
     if (st.c00 != 0.0) rhs -= st.c00 * Vold[c];
     const double vnew = rhs / st.c;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(GBS) void k_grhs0(GView g, const double* __restrict
     const int64_t c = (int64_t)blockIdx.x * GBS + threadIdx.x;
     if (c >= g.n) return;
     const double vo = st.c00 != 0.0 ? Vm1[c] : 0.0;
-    const double src = general_box(g, c, st.boxes);
+    const double src = st.src ? st.src[c] : general_box(g, c, st.boxes);
     b0[c] = g.vol[c] * (st.chiCm / st.dt * (st.c0 * Vn[c] - st.c00 * vo) -
                         st.chiCm * Iion[c] + src);
 }
@@ -456,6 +456,45 @@ int general_create(GeneralMesh** out, int64_t n, int64_t nf, const int32_t* owne
     }
     *out = G;
     return 0;
+}
+
+int general_from_hex(GeneralMesh** out, const HexGeom& g, std::string* err)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int64_t n = (int64_t)nx * ny * nz;
+    const double h[3] = {g.hx, g.hy, g.hz};
+    const double area[3] = {g.hy * g.hz, g.hx * g.hz, g.hx * g.hy};
+    std::vector<int32_t> own, nbr, bown;
+    std::vector<double> Sf, Cf, C(3 * n), V(n, g.hx * g.hy * g.hz), bSf;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int32_t c = i + nx * (j + ny * k);
+                const double cc[3] = {(i + 0.5) * g.hx, (j + 0.5) * g.hy, (k + 0.5) * g.hz};
+                for (int q = 0; q < 3; ++q) C[3 * c + q] = cc[q];
+                const int idx[3] = {i, j, k}, nn[3] = {nx, ny, nz};
+                const int32_t st[3] = {1, nx, nx * ny};
+                for (int d = 0; d < 3; ++d) {
+                    if (idx[d] < nn[d] - 1) {   // internal face towards +d
+                        own.push_back(c);
+                        nbr.push_back(c + st[d]);
+                        for (int q = 0; q < 3; ++q) {
+                            Sf.push_back(q == d ? area[d] : 0.0);
+                            Cf.push_back(q == d ? cc[q] + 0.5 * h[d] : cc[q]);
+                        }
+                    }
+                    for (int side = 0; side < 2; ++side) {  // walls
+                        const bool wall = side == 0 ? idx[d] == 0 : idx[d] == nn[d] - 1;
+                        if (!wall) continue;
+                        bown.push_back(c);
+                        for (int q = 0; q < 3; ++q)
+                            bSf.push_back(q == d ? (side == 0 ? -area[d] : area[d]) : 0.0);
+                    }
+                }
+            }
+    return general_create(out, n, (int64_t)own.size(), own.data(), nbr.data(), Sf.data(),
+                          Cf.data(), C.data(), V.data(), (int64_t)bown.size(), bown.data(),
+                          bSf.data(), err);
 }
 
 void general_destroy(GeneralMesh* G)

@@ -303,6 +303,18 @@ class Monodomain:
         """Applies fv_solution_controls(fv_solution, field)."""
         self.set_solver_controls(**fv_solution_controls(fv_solution, field))
 
+    def diffusion_step(self, Iion, source=None):
+        """solveDiffusionExplicit / solveDiffusionImplicit (monodomainSolver.C:103-150) with
+        the caller's ionic current Iion [n] (V/s) and optional source [n] (A/m^3; default:
+        the stimulus boxes): one step without the tissue's ionic model
+        (cg_tissue_diffusion_step)."""
+        i = np.ascontiguousarray(Iion, dtype=np.float64)
+        src = None if source is None else np.ascontiguousarray(source, dtype=np.float64)
+        if i.size != self.mesh.n or (src is not None and src.size != self.mesh.n):
+            raise ValueError("fields of %d cells expected" % self.mesh.n)
+        check(lib().cg_tissue_diffusion_step(self.t, _dp(i), None if src is None else _dp(src),
+                                             0))
+
     def pseudo_ecg(self, positions):
         """PseudoECGSolver::compute (pseudoECGSolver.C:36-83) of the current Vm at the
         electrode positions [E][3] (m): the dipole sum per electrode, [E]."""

@@ -362,6 +362,19 @@ int cg_adjoint_backward(cg_adjoint* a, const double* dprobes_dev, double* dkappa
 int cg_tissue_apply_operator(cg_tissue* t, double ddt_coeff, int precision, const void* x_dev,
                              void* y_dev);
 
+/* The diffusion part of one step, MonodomainSolver::solveDiffusionExplicit /
+ * solveDiffusionImplicit (monodomainSolver.C:103-150), with the ionic current supplied by
+ * the caller: for an adapter that keeps its ionic model elsewhere.  Iion [n] in V/s,
+ * source [n] in A/m^3 (NULL: the tissue's stimulus boxes at the step's start time), host
+ * arrays unless flags has CG_DEVICE_PTRS.  Advances Vm, its old levels, the activation
+ * times and the time by one step without calling the tissue's ionic handle.  Runs on the
+ * general-mesh operators (on a hex tissue, the block described as a general mesh, built
+ * on the first call); implicit solves use the lduMatrix PCG in OpenFOAM's norm with the
+ * tolerances and PIMPLE controls of cg_tissue_set_solver_controls (CG_NORM_OPENFOAM) or the
+ * tissue's tolerance.  Synchronous. */
+int cg_tissue_diffusion_step(cg_tissue* t, const double* Iion, const double* source,
+                             int flags);
+
 /* Pseudo-ECG of the current Vm (PseudoECGSolver::compute, pseudoECGSolver.C:36-83, called
  * by ECGDomain::evolve, ecgDomain.C:167-190): for each electrode P,
  *   phi(P) = sum_c ((sigma_c . grad Vm_c) vol_c) . (C_c - P) / |C_c - P|^3

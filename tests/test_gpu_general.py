@@ -200,3 +200,39 @@ def test_general_pseudo_ecg(gpu, orc):
     err = np.abs(got - ref) / ab
     _say("  warped vs oracle %.3e" % err.max())
     assert np.all(err <= 1e-13)
+
+
+@pytest.mark.parametrize("kind,alg", [("hex", "explicit"), ("hex", "implicit"),
+                                      ("warped", "explicit"), ("warped", "implicit")])
+def test_diffusion_step_with_caller_ionic(gpu, orc, kind, alg):
+    """cg_tissue_diffusion_step: the caller runs the ionic model (solveODE on the host) and
+    hands Iion over; the result equals the fused step (cg_tissue_step) of a twin tissue."""
+    _say("diffusion entry", kind, alg)
+    shape, h, dt = (16, 12, 10), 2e-4, 1e-5
+    n_steps = 200 if alg == "explicit" else 220
+    L = tuple(s * h for s in shape)
+    sig = rotating_fibre_sigma(*shape)
+
+    def mesh():
+        if kind == "hex":
+            return gpu.HexMesh(*shape, h)
+        return _poly(gpu, pm.block(*shape, lengths=L, warp=pm.smooth_warp(0.05, L)))
+
+    a = gpu.Monodomain(mesh(), _coeffs(alg, sig), dt=dt, ddt="backward", tol=1e-13)
+    b = gpu.Monodomain(mesh(), _coeffs(alg, sig), dt=dt, ddt="backward", tol=1e-13)
+    for d in (a, b):
+        if alg == "implicit":
+            d.set_solver_controls("OpenFOAM", 1e-13, 0.0, 1000, 1, 0.0)
+    n = b.mesh.n
+    Im = np.zeros(n)
+    for _ in range(n_steps):
+        t0 = b.time[0]
+        b.ionic.solveODE(t0, dt, b.Vm, Im)
+        b.diffusion_step(Im)
+    a.step(n_steps)
+    err = np.max(np.abs(a.Vm - b.Vm)) / np.max(np.abs(a.Vm))
+    ta, tb = a.activationTime, b.activationTime
+    _say("  Vm rel %.3e, activated %d / %d" % (err, (ta > 0).sum(), (tb > 0).sum()))
+    assert (ta > 0).sum() > 0
+    assert err <= (1e-12 if alg == "explicit" else 1e-9)
+    assert np.array_equal(ta > 0, tb > 0) and np.max(np.abs(ta - tb)) <= 1e-10
