@@ -1178,16 +1178,21 @@ static int general_advance(cg_tissue* t, cg::GeneralMesh* G, int n_steps, bool i
                 t->g_c = st.c;
             }
             HIPCHK(cg::general_rhs0(G, Vn, Vm1, t->d_Iion, t->d_b0, st, s));
-            HIPCHK(hipMemcpyAsync(Vnew, Vn, sizeof(double) * m->n, hipMemcpyDeviceToDevice, s));
             // pimpleLoop: after the first corrector, an initial residual of the previous
             // solve below residualControl makes the current corrector the last
             const int n_outer = t->g_outer > 1 ? t->g_outer : 1;
+            // the solve starts from Vn, as the reference's; with a single corrector (no
+            // residual-controlled loop that its initial residual would steer) from the
+            // extrapolation 2 Vn - Vn-1, which changes the iterations, not the solution
+            HIPCHK(cg::general_guess(G, Vn, Vm1, Vnew, n_outer == 1 && t->steps >= 1, s));
             double prev = 0.0;
             int solves = 0;
             for (int outer = 1;; ++outer) {
                 const bool last = n_outer == 1 || outer >= n_outer ||
                                   (outer > 1 && prev < t->g_rc);
-                HIPCHK(cg::general_rhs_corr(G, Vnew, t->d_b0, t->d_b, s));
+                // the explicit correction of the first corrector is the one of Vn (the
+                // field the equation is assembled from), later ones of the latest solution
+                HIPCHK(cg::general_rhs_corr(G, outer == 1 ? Vn : Vnew, t->d_b0, t->d_b, s));
                 HIPCHK(hipStreamSynchronize(s));
                 int it = 0;
                 double r0 = 0.0, r1 = 0.0;

@@ -57,6 +57,9 @@ hipError_t general_rhs0(GeneralMesh* G, const double* Vn, const double* Vm1,
 // face fluxes); asynchronous on s
 hipError_t general_rhs_corr(GeneralMesh* G, const double* psi, const double* b0, double* b,
                             hipStream_t s);
+// initial guess of the solve: Vn, or 2 Vn - Vm1 (extrapolate); asynchronous on s
+hipError_t general_guess(GeneralMesh* G, const double* Vn, const double* Vm1, double* psi,
+                         bool extrapolate, hipStream_t s);
 // activation update after an implicit solve (myocardiumDomain.C:345-374)
 hipError_t general_activation(GeneralMesh* G, const double* Vn, const double* Vnew,
                               uint8_t* flag, double* tact, double t_old, double dt,

@@ -199,6 +199,14 @@ __global__ __launch_bounds__(GBS) void k_grhs_corr(GView g, const double* __rest
     b[c] = b0[c] + flux_sum(g, psi, grad, c, psi[c], false);
 }
 
+__global__ __launch_bounds__(GBS) void k_gguess(int64_t n, const double* __restrict__ Vn,
+                                                const double* __restrict__ Vm1,
+                                                double* __restrict__ psi)
+{
+    const int64_t c = (int64_t)blockIdx.x * GBS + threadIdx.x;
+    if (c < n) psi[c] = 2.0 * Vn[c] - Vm1[c];
+}
+
 __global__ __launch_bounds__(GBS) void k_gactivation(int64_t n, const double* __restrict__ Vn,
                                                      const double* __restrict__ Vnew,
                                                      uint8_t* __restrict__ flag,
@@ -638,6 +646,15 @@ hipError_t general_pseudo_ecg(GeneralMesh* G, const double* V, const double* sig
     }
     hipLaunchKernelGGL(k_gecg_sum, dim3(n_el), dim3(GBS), 0, s, (const double*)partial, nb,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t general_guess(GeneralMesh* G, const double* Vn, const double* Vm1, double* psi,
+                         bool extrapolate, hipStream_t s)
+{
+    if (!extrapolate)
+        return hipMemcpyAsync(psi, Vn, sizeof(double) * G->n, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL(k_gguess, dim3(grid_of(G->n)), dim3(GBS), 0, s, G->n, Vn, Vm1, psi);
     return hipGetLastError();
 }
 
