@@ -267,6 +267,7 @@ template <class T> struct BatchLaunch {
         case MODEL_TNNP: return BatchLaunch<T>::variant<TNNP>(variant, tissue, states, P, gscale, a, s); \
         case MODEL_ORD: return BatchLaunch<T>::variant<ORd>(variant, tissue, states, P, gscale, a, s); \
         case MODEL_AP: return BatchLaunch<T>::variant<AlievPanfilov>(variant, tissue, states, P, gscale, a, s); \
+        case MODEL_CRN: return BatchLaunch<T>::variant<Courtemanche>(variant, tissue, states, P, gscale, a, s); \
         default: return hipErrorInvalidValue;                                              \
         }                                                                                  \
     }

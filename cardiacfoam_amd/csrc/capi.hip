@@ -40,8 +40,9 @@ int fail(int code, const char* fmt, ...)
             return fail(CG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));           \
     } while (0)
 
-const char* const kModelKeys[5] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
-                                   "monodomainFDAManufactured"};
+constexpr int kNumModels = 6;
+const char* const kModelKeys[kNumModels] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
+                                            "monodomainFDAManufactured", "Courtemanche"};
 
 // state names, order of the *_STATES_NAMES tables / Names.H enums
 const char* const kBONames[4] = {"u", "v", "w", "s"};
@@ -56,6 +57,10 @@ const char* const kORdNames[41] = {
     "ICaL_ff", "ICaL_fs", "ICaL_fcaf", "ICaL_fcas", "ICaL_jca", "ICaL_ffp", "ICaL_fcafp",
     "ICaL_nca", "IKr_xrf", "IKr_xrs", "IKs_xs1", "IKs_xs2", "IK1_xk1", "ryr_Jrelnp",
     "ryr_Jrelp"};
+const char* const kCRNNames[21] = {
+    "membrane_V", "sodium_Nai", "potassium_Ki", "calcium_Cai", "calcium_CaUp", "calcium_CaRel",
+    "ina_m", "ina_h", "ina_j", "ito_oa", "ito_oi", "ikur_ua", "ikur_ui", "ikr_xr", "iks_xs",
+    "ical_d", "ical_f", "ical_fCa", "cajsr_u", "cajsr_v", "cajsr_w"};
 
 struct ModelInfo {
     int ns, np;
@@ -82,6 +87,9 @@ bool model_info(int model, ModelInfo* mi)
     case cg::MODEL_FDA:
         *mi = {cg::FdaManufactured::NS, cg::FdaManufactured::NP, 1.0, 0.0, 1.0, 1.0};
         return true;
+    case cg::MODEL_CRN:
+        *mi = {cg::Courtemanche::NS, cg::Courtemanche::NP, 1.0, 0.0, 1.0, 1000.0};
+        return true;
     default: return false;
     }
 }
@@ -94,6 +102,7 @@ bool model_host_init(int model, int tissue, double* p, double* y)
     case cg::MODEL_ORD: return cg::ORd::host_init(tissue, p, y);
     case cg::MODEL_AP: return cg::AlievPanfilov::host_init(tissue, p, y);
     case cg::MODEL_FDA: return cg::FdaManufactured::host_init(tissue, p, y);
+    case cg::MODEL_CRN: return cg::Courtemanche::host_init(tissue, p, y);
     default: return false;
     }
 }
@@ -271,7 +280,7 @@ const char* cg_version(void) { return "cardiacgpu 0.1 gfx950"; }
 int cg_model_id(const char* key)
 {
     if (!key) return -1;
-    for (int i = 0; i < 5; ++i)
+    for (int i = 0; i < kNumModels; ++i)
         if (!strcmp(key, kModelKeys[i])) return i;
     return -1;
 }
@@ -302,7 +311,8 @@ int cg_model_state_name(int model, int i, char* buf, size_t len)
     if (!model_info(model, &mi)) return fail(CG_ERR_MODEL, "unknown model id %d", model);
     if (i < 0 || i >= mi.ns || !buf || len == 0) return fail(CG_ERR_ARG, "bad state index %d", i);
     const char* nm = model == 0 ? kBONames[i] : model == 1 ? kTNNPNames[i]
-                   : model == 2 ? kORdNames[i] : model == 3 ? kAPNames[i] : kFDANames[i];
+                   : model == 2 ? kORdNames[i] : model == 3 ? kAPNames[i]
+                   : model == 4 ? kFDANames[i] : kCRNNames[i];
     snprintf(buf, len, "%s", nm);
     return CG_OK;
 }
@@ -383,8 +393,8 @@ int cg_ionic_set_constant(cg_ionic* h, int k, double v)
 int cg_ionic_set_conductance_scale(cg_ionic* h, const double* s)
 {
     if (!h) return fail(CG_ERR_ARG, "NULL handle");
-    if (h->model != cg::MODEL_TNNP && h->model != cg::MODEL_ORD && s)
-        return fail(CG_ERR_STATE, "conductance scaling is defined for TNNP and ORd only");
+    if (h->model != cg::MODEL_TNNP && h->model != cg::MODEL_ORD && h->model != cg::MODEL_CRN && s)
+        return fail(CG_ERR_STATE, "conductance scaling is defined for TNNP, ORd and Courtemanche only");
     HIPCHK(hipStreamSynchronize(h->stream));
     if (!s) {
         if (h->d_gscale) hipFree(h->d_gscale);

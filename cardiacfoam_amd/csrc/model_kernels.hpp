@@ -343,7 +343,7 @@ template <class T> struct ModelLaunch {
     }
 };
 
-// switch over the four models for one arithmetic type
+// switch over the models for one arithmetic type
 #define CG_MODEL_SWITCH(T, FN, ...)                                                        \
     switch (model) {                                                                       \
     case MODEL_BO: return ModelLaunch<T>::template FN<BuenoOrovio>(__VA_ARGS__);           \
@@ -351,6 +351,7 @@ template <class T> struct ModelLaunch {
     case MODEL_ORD: return ModelLaunch<T>::template FN<ORd>(__VA_ARGS__);                  \
     case MODEL_AP: return ModelLaunch<T>::template FN<AlievPanfilov>(__VA_ARGS__);         \
     case MODEL_FDA: return ModelLaunch<T>::template FN<FdaManufactured>(__VA_ARGS__);      \
+    case MODEL_CRN: return ModelLaunch<T>::template FN<Courtemanche>(__VA_ARGS__);         \
     default: return hipErrorInvalidValue;                                                  \
     }
 

@@ -1,4 +1,4 @@
-// Uniform device interface over the four ionic models of the tissue path.
+// Uniform device interface over the ionic models of the tissue path.
 //   tissue_step: V held at the PDE value for the whole step (solveVmWithinODESolver
 //                false); advances the other states by dt and returns the total ionic
 //                current at the step end (model units), as the reference's solveODE
@@ -13,10 +13,12 @@
 #include "model_tnnp.hpp"
 #include "model_ord.hpp"
 #include "model_fda.hpp"
+#include "model_crn.hpp"
 
 namespace cg {
 
-enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3, MODEL_FDA = 4 };
+enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3, MODEL_FDA = 4,
+               MODEL_CRN = 5 };
 
 constexpr int MAX_PARAMS = 48;
 struct ModelParams {
@@ -163,6 +165,39 @@ template <> struct Ops<FdaManufactured> {
         Deriv<T> out{y, d};
         d[0] = T(0);
         FdaManufactured::rates(y, out, p);
+    }
+};
+
+template <> struct Ops<Courtemanche> {
+    template <class T>
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = Courtemanche::vterms(v, p);
+        Courtemanche::step(y, dt, p, gs, false, T(0), q);
+        return Courtemanche::currents(v, y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = Courtemanche::vterms(y[0], p);
+        Courtemanche::step(y, dt, p, gs, true, istim, q);
+    }
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T* gs)
+    {
+        const auto q = Courtemanche::vterms(y[0], p);
+        return Courtemanche::currents(y[0], y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static Courtemanche::VTerms<T> vt(T v, const T* p)
+    {
+        return Courtemanche::vterms(v, p);
+    }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int, const T* gs, bool solve_v,
+                             T istim, const Courtemanche::VTerms<T>& q)
+    {
+        Courtemanche::deriv(y, d, p, gs, solve_v, istim, q);
     }
 };
 

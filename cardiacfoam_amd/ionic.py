@@ -14,12 +14,14 @@ import numpy as np
 from . import _capi
 from ._capi import check, lib
 
-MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov", "monodomainFDAManufactured"]
+MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov", "monodomainFDAManufactured",
+              "Courtemanche"]
 SUPPORTED_TISSUES = {
     "BuenoOrovio": ["endocardialCells", "mCells", "epicardialCells"],
     "TNNP": ["endocardialCells", "mCells", "epicardialCells"],
     "ORd": ["myocyte"],
     "AlievPanfilov": ["myocyte"],
+    "Courtemanche": ["myocyte"],  # Courtemanche.C:92-95
     # the manufactured model's supportedDimensions() (monodomainFDAManufactured.C:85-88)
     "monodomainFDAManufactured": ["1D", "2D", "3D"],
 }

@@ -665,6 +665,203 @@ static void fda_eval(const double* c, const double* y, double* r, int* g, double
     *iion = -0.5 * c[0] * a * pow(u2, 2.0) * (V - u3) + (c[1] * (V - u3)) / c[2];
 }
 
+/* ============================================================== Courtemanche
+ * Courtemanche-Ramirez-Nattel 1998 human atrial model, 21 states: constants and initial
+ * state of Courtemanche_1998.H:337-455, right-hand side of :458-658, state order of
+ * Courtemanche_1998Names.H.  The reference accepts tissue "myocyte" only
+ * (Courtemanche.C:92-95).  The K_i balance reads ALGEBRAIC[Istim] before the call
+ * assigns it (Courtemanche_1998.H:644 vs :652) from the zeroed scratch of derivatives()
+ * (Courtemanche.C:161), so it omits the stimulus, as for ORd.
+ * Parity pins (tests/test_oracle_models.py): charge conservation of the concentration
+ * balances against Cm dV/dt, the published action potential at 1 Hz, and the
+ * removable singularities of the rate expressions checked against their limits. */
+enum { CR_V, CR_Nai, CR_Ki, CR_Cai, CR_CaUp, CR_CaRel, CR_m, CR_h, CR_j, CR_oa, CR_oi,
+       CR_ua, CR_ui, CR_xr, CR_xs, CR_d, CR_f, CR_fCa, CR_u, CR_v, CR_w, CR_NS };
+enum { CC_CMDN_max, CC_CSQN_max, CC_Km_CMDN, CC_Km_CSQN, CC_Km_TRPN, CC_TRPN_max,
+       CC_Ca_up_max, CC_I_up_max, CC_K_up, CC_Cao, CC_Ko, CC_Nao, CC_Cm, CC_V_cell, CC_V_i,
+       CC_V_rel, CC_V_up, CC_ECaL, CC_gCaL, CC_fCa_tau, CC_IpCa_max, CC_tau_tr, CC_F,
+       CC_R, CC_T, CC_RTF, CC_FRT, CC_KQ10, CC_INaCa_max, CC_KmCa, CC_KmNa, CC_gam,
+       CC_ksat, CC_INaK_max, CC_KmKo, CC_KmNai, CC_sigma, CC_K_rel, CC_c1, CC_c2,
+       CC_u_tau, CC_gbCa, CC_gbNa, CC_gK1, CC_gKr, CC_gKs, CC_gKur_base, CC_gNa, CC_gto,
+       CC_NC };
+
+static void crn_init(double* c, double* y)
+{
+    c[CC_CMDN_max] = 0.05; c[CC_CSQN_max] = 10.0; c[CC_Km_CMDN] = 0.00238;
+    c[CC_Km_CSQN] = 0.8; c[CC_Km_TRPN] = 0.0005; c[CC_TRPN_max] = 0.07;
+    c[CC_Ca_up_max] = 15.0; c[CC_I_up_max] = 0.005; c[CC_K_up] = 0.00092;
+    c[CC_Cao] = 1.8; c[CC_Ko] = 5.4; c[CC_Nao] = 140.0;
+    c[CC_Cm] = 100.0; c[CC_V_cell] = 20100.0;
+    c[CC_V_i] = 0.68 * c[CC_V_cell]; c[CC_V_rel] = 0.0048 * c[CC_V_cell];
+    c[CC_V_up] = 0.0552 * c[CC_V_cell];
+    c[CC_ECaL] = 65.0; c[CC_gCaL] = 0.12375; c[CC_fCa_tau] = 2.0; c[CC_IpCa_max] = 0.275;
+    c[CC_tau_tr] = 180.0;
+    c[CC_F] = 96.4867; c[CC_R] = 8.3143; c[CC_T] = 310.0;
+    c[CC_RTF] = c[CC_R] * c[CC_T] / c[CC_F]; c[CC_FRT] = 1.0 / c[CC_RTF];
+    c[CC_KQ10] = 3.0;
+    c[CC_INaCa_max] = 1600.0; c[CC_KmCa] = 1.38; c[CC_KmNa] = 87.5; c[CC_gam] = 0.35;
+    c[CC_ksat] = 0.1;
+    c[CC_INaK_max] = 0.59933874; c[CC_KmKo] = 1.5; c[CC_KmNai] = 10.0;
+    c[CC_sigma] = (exp(c[CC_Nao] / 67.3) - 1.0) / 7.0;
+    c[CC_K_rel] = 30.0; c[CC_c1] = 3.4175e-13; c[CC_c2] = 1.367e-15; c[CC_u_tau] = 8.0;
+    c[CC_gbCa] = 0.001131; c[CC_gbNa] = 0.0006744375; c[CC_gK1] = 0.09;
+    c[CC_gKr] = 0.029411765; c[CC_gKs] = 0.12941176; c[CC_gKur_base] = 0.005;
+    c[CC_gNa] = 7.8; c[CC_gto] = 0.1652;
+    static const double y0[CR_NS] = {
+        -81.9463303822041098, 13.8169746305367962, 136.355229902154434,
+        1.23092247890489894e-04, 1.54668119199095355, 1.07650740580354909,
+        2.56385228666526068e-03, 0.970298907063270155, 0.981123905023234988,
+        2.91755626557170314e-02, 0.999342865333055497, 4.58838038240151104e-03,
+        0.991468962753066063, 8.33819909884048389e-04, 1.86683180787284714e-02,
+        1.24231529593716656e-04, 0.951907788168154578, 0.739682838459564729,
+        -1.97647749727073971e-40, 1.0, 0.999233799248152699 };
+    memcpy(y, y0, sizeof y0);
+}
+
+/* x / (1 - exp(-x/s)) with the generated code's guard |x| < eps -> s */
+static double crn_xe(double x, double s, double eps)
+{
+    return fabs(x) < eps ? s : x / (1.0 - exp(-x / s));
+}
+
+static void crn_eval(double t, const double* c, const double* y, int solve_vm,
+                     const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                     double* iion, double* istim)
+{
+    const double V = y[CR_V], Cai = y[CR_Cai], Nai = y[CR_Nai], Ki = y[CR_Ki];
+    const double CaUp = y[CR_CaUp], CaRel = y[CR_CaRel];
+    const double FRT = c[CC_FRT], RTF = c[CC_RTF];
+    for (int i = 0; i < CR_NS; ++i) g[i] = 0;
+#define CR_GATE(idx, inf, tau) do { r[idx] = ((inf) - y[idx]) / (tau); g[idx] = 1; \
+                                    xi[idx] = (inf); ta[idx] = (tau); } while (0)
+    /* reversal potentials */
+    const double ECa = 0.5 * RTF * log(c[CC_Cao] / Cai);
+    const double EK = RTF * log(c[CC_Ko] / Ki);
+    const double ENa = RTF * log(c[CC_Nao] / Nai);
+
+    /* INa: m, h, j (alpha/beta form; h and j alphas vanish above -40 mV) */
+    const double am = V == -47.13 ? 3.2 : 0.32 * (V + 47.13) / (1.0 - exp(-0.1 * (V + 47.13)));
+    const double bm = 0.08 * exp(-V / 11.0);
+    CR_GATE(CR_m, am / (am + bm), 1.0 / (am + bm));
+    const int lo = V < -40.0;
+    const double ah = lo ? 0.135 * exp((V + 80.0) / -6.8) : 0.0;
+    const double bh = lo ? 3.56 * exp(0.079 * V) + 310000.0 * exp(0.35 * V)
+                         : 1.0 / (0.13 * (1.0 + exp((V + 10.66) / -11.1)));
+    CR_GATE(CR_h, ah / (ah + bh), 1.0 / (ah + bh));
+    const double aj = lo ? (-127140.0 * exp(0.2444 * V) - 3.474e-05 * exp(-0.04391 * V)) *
+                               (V + 37.78) / (1.0 + exp(0.311 * (V + 79.23)))
+                         : 0.0;
+    const double bj = lo ? 0.1212 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)))
+                         : 0.3 * exp(-2.535e-07 * V) / (1.0 + exp(-0.1 * (V + 32.0)));
+    CR_GATE(CR_j, aj / (aj + bj), 1.0 / (aj + bj));
+    const double INa = c[CC_gNa] * pow(y[CR_m], 3.0) * y[CR_h] * y[CR_j] * (V - ENa);
+
+    /* Ito: oa, oi (temperature factor KQ10 on both time constants) */
+    {
+        const double a = 0.65 / (exp((V + 10.0) / -8.5) + exp((V - 30.0) / -59.0));
+        const double b = 0.65 / (2.5 + exp((V + 82.0) / 17.0));
+        CR_GATE(CR_oa, 1.0 / (1.0 + exp((V + 20.47) / -17.54)), 1.0 / (a + b) / c[CC_KQ10]);
+        const double a2 = 1.0 / (18.53 + exp((V + 113.7) / 10.95));
+        const double b2 = 1.0 / (35.56 + exp((V + 1.26) / -7.44));
+        CR_GATE(CR_oi, 1.0 / (1.0 + exp((V + 43.1) / 5.3)), 1.0 / (a2 + b2) / c[CC_KQ10]);
+    }
+    const double Ito = c[CC_gto] * pow(y[CR_oa], 3.0) * y[CR_oi] * (V - EK);
+
+    /* IKur: ua, ui; the conductance rises sigmoidally with V */
+    {
+        const double a = 0.65 / (exp((V + 10.0) / -8.5) + exp((V - 30.0) / -59.0));
+        const double b = 0.65 / (2.5 + exp((V + 82.0) / 17.0));
+        CR_GATE(CR_ua, 1.0 / (1.0 + exp((V + 30.3) / -9.6)), 1.0 / (a + b) / c[CC_KQ10]);
+        const double a2 = 1.0 / (21.0 + exp((V - 185.0) / -28.0));
+        const double b2 = 1.0 / exp((V - 158.0) / -16.0);
+        CR_GATE(CR_ui, 1.0 / (1.0 + exp((V - 99.45) / 27.48)), 1.0 / (a2 + b2) / c[CC_KQ10]);
+    }
+    const double gKur = c[CC_gKur_base] * (1.0 + 10.0 / (1.0 + exp((V - 15.0) / -13.0)));
+    const double IKur = gKur * pow(y[CR_ua], 3.0) * y[CR_ui] * (V - EK);
+
+    /* IKr: xr, with inward rectification */
+    {
+        const double a = 0.0003 * crn_xe(V + 14.1, 5.0, 1e-6);
+        const double xb = V - 3.3328;
+        const double b = 7.3898e-05 * (fabs(xb) < 1e-07 ? 5.1237 : xb / (exp(xb / 5.1237) - 1.0));
+        CR_GATE(CR_xr, 1.0 / (1.0 + exp((V + 14.1) / -6.5)), 1.0 / (a + b));
+    }
+    const double IKr = c[CC_gKr] * y[CR_xr] * (V - EK) / (1.0 + exp((V + 15.0) / 22.4));
+
+    /* IKs: xs, squared */
+    {
+        const double a = 4e-05 * crn_xe(V - 19.9, 17.0, 1e-6);
+        const double xb = V - 19.9;
+        const double b = 3.5e-05 * (fabs(xb) < 1e-06 ? 9.0 : xb / (exp(xb / 9.0) - 1.0));
+        CR_GATE(CR_xs, 1.0 / sqrt(1.0 + exp((V - 19.9) / -12.7)), 0.5 / (a + b));
+    }
+    const double IKs = c[CC_gKs] * pow(y[CR_xs], 2.0) * (V - EK);
+
+    /* ICaL: d, f, fCa; fixed reversal potential ECaL */
+    {
+        const double xd = V + 10.0;
+        const double td = fabs(xd) < 1e-06 ? 1.0 / (6.24 * 2.0 * 0.035)
+                          : (1.0 - exp(xd / -6.24)) / (0.035 * xd * (1.0 + exp(xd / -6.24)));
+        CR_GATE(CR_d, 1.0 / (1.0 + exp(xd / -8.0)), td);
+        CR_GATE(CR_f, 1.0 / (1.0 + exp((V + 28.0) / 6.9)),
+                9.0 / (0.0197 * exp(-pow(0.0337, 2.0) * pow(V + 10.0, 2.0)) + 0.02));
+        CR_GATE(CR_fCa, 1.0 / (1.0 + Cai / 0.00035), c[CC_fCa_tau]);
+    }
+    const double ICaL = c[CC_gCaL] * y[CR_d] * y[CR_f] * y[CR_fCa] * (V - c[CC_ECaL]);
+
+    /* pumps, exchanger and background currents */
+    const double eg = exp(c[CC_gam] * V * FRT), eg1 = exp((c[CC_gam] - 1.0) * V * FRT);
+    const double INaCa = c[CC_INaCa_max] *
+                         (eg * pow(Nai, 3.0) * c[CC_Cao] - eg1 * pow(c[CC_Nao], 3.0) * Cai) /
+                         ((pow(c[CC_KmNa], 3.0) + pow(c[CC_Nao], 3.0)) * (c[CC_KmCa] + c[CC_Cao]) *
+                          (1.0 + c[CC_ksat] * eg1));
+    const double fNaK = 1.0 / (1.0 + 0.1245 * exp(-0.1 * V * FRT) +
+                               0.0365 * c[CC_sigma] * exp(-V * FRT));
+    const double INaK = c[CC_INaK_max] * fNaK * c[CC_Ko] / (c[CC_Ko] + c[CC_KmKo]) /
+                        (1.0 + pow(c[CC_KmNai] / Nai, 1.5));
+    const double IpCa = c[CC_IpCa_max] * Cai / (0.0005 + Cai);
+    const double IbCa = c[CC_gbCa] * (V - ECa);
+    const double IbNa = c[CC_gbNa] * (V - ENa);
+    const double IK1 = c[CC_gK1] * (V - EK) / (1.0 + exp(0.07 * (V + 80.0)));
+
+    /* SR release (u, v, w) driven by the flux signal Fn */
+    const double Irel = c[CC_K_rel] * pow(y[CR_u], 2.0) * y[CR_v] * y[CR_w] * (CaRel - Cai);
+    {
+        const double xw = V - 7.9;
+        const double tw = 6.0 * (fabs(xw) < 1e-06 ? 2.0 / 13.0
+                                 : (1.0 - exp(-xw / 5.0)) / ((1.0 + 0.3 * exp(-xw / 5.0)) * xw));
+        CR_GATE(CR_w, 1.0 - 1.0 / (1.0 + exp(-(V - 40.0) / 17.0)), tw);
+        const double Fn = 1e-12 * c[CC_V_rel] * Irel -
+                          5e-13 / c[CC_F] * (0.5 * ICaL - 0.2 * INaCa) * c[CC_Cm];
+        const double eu = exp(-(Fn - c[CC_c1]) / c[CC_c2]);
+        CR_GATE(CR_u, 1.0 / (1.0 + eu), c[CC_u_tau]);
+        CR_GATE(CR_v, 1.0 - 1.0 / (1.0 + exp(-(Fn - 0.2 * c[CC_c1]) / c[CC_c2])),
+                1.91 + 2.09 / (1.0 + eu));
+    }
+#undef CR_GATE
+
+    /* calcium handling */
+    const double Iup = c[CC_I_up_max] / (1.0 + c[CC_K_up] / Cai);
+    const double Ileak = c[CC_I_up_max] * CaUp / c[CC_Ca_up_max];
+    const double Itr = (CaUp - CaRel) / c[CC_tau_tr];
+    r[CR_CaRel] = (Itr - Irel) /
+                  (1.0 + c[CC_CSQN_max] * c[CC_Km_CSQN] / pow(CaRel + c[CC_Km_CSQN], 2.0));
+    r[CR_CaUp] = Iup - (Ileak + Itr * c[CC_V_rel] / c[CC_V_up]);
+    const double B1 = (2.0 * INaCa - (IpCa + ICaL + IbCa)) * c[CC_Cm] / (2.0 * c[CC_V_i] * c[CC_F]) +
+                      (c[CC_V_up] * (Ileak - Iup) + Irel * c[CC_V_rel]) / c[CC_V_i];
+    const double B2 = 1.0 + c[CC_TRPN_max] * c[CC_Km_TRPN] / pow(Cai + c[CC_Km_TRPN], 2.0) +
+                      c[CC_CMDN_max] * c[CC_Km_CMDN] / pow(Cai + c[CC_Km_CMDN], 2.0);
+    r[CR_Cai] = B1 / B2;
+
+    const double Iion = INa + IK1 + Ito + IKur + IKr + IKs + ICaL + IpCa + INaK + INaCa + IbNa + IbCa;
+    const double cvf = c[CC_Cm] / (c[CC_V_i] * c[CC_F]);
+    r[CR_Ki] = (2.0 * INaK - (IK1 + Ito + IKur + IKr + IKs)) * cvf;
+    r[CR_Nai] = (-3.0 * INaK - (3.0 * INaCa + IbNa + INa)) * cvf;
+    const double Ist = orc_stimulus(t, st);
+    r[CR_V] = solve_vm ? -Iion - Ist : 0.0;
+    *iion = Iion; *istim = Ist;
+}
+
 /* ============================================================== dispatch */
 int orc_model_nstates(int model)
 {
@@ -674,6 +871,7 @@ int orc_model_nstates(int model)
     case ORC_ORD: return OR_NS;
     case ORC_AP: return 2;
     case ORC_FDA: return 4;
+    case ORC_CRN: return CR_NS;
     default: return -1;
     }
 }
@@ -697,6 +895,9 @@ int orc_model_init(int model, int tissue, double* c, double* y)
     case ORC_FDA:
         if (tissue < ORC_DIM1 || tissue > ORC_DIM3) return -1;
         fda_init(tissue, c, y); return 0;
+    case ORC_CRN:
+        if (tissue != ORC_MYOCYTE) return -1;
+        crn_init(c, y); return 0;
     default: return -1;
     }
 }
@@ -713,6 +914,11 @@ int orc_model_const_index(int model, const char* name)
         if (!strcmp(name, "GKr")) return OC_GKr;
         if (!strcmp(name, "PCa")) return OC_PCa;
         if (!strcmp(name, "GKs")) return OC_GKs;
+    } else if (model == ORC_CRN) {
+        if (!strcmp(name, "gNa")) return CC_gNa;
+        if (!strcmp(name, "gKr")) return CC_gKr;
+        if (!strcmp(name, "gCaL")) return CC_gCaL;
+        if (!strcmp(name, "gKs")) return CC_gKs;
     } else if (model == ORC_BO) {
         if (!strcmp(name, "tauFi")) return BO_tfi;
         if (!strcmp(name, "tauSi")) return BO_tsi;
@@ -730,6 +936,7 @@ void orc_model_eval(int model, int tissue, double t, const double* c, const doub
     case ORC_ORD: ord_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_AP: ap_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_FDA: fda_eval(c, y, r, g, iion); *istim = 0.0; break;
+    case ORC_CRN: crn_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     default: break;
     }
 }

@@ -9,6 +9,7 @@ MODELS = {  # name -> (oracle id, tissue flag, library key, tissue word, stimulu
     "TNNP_endo": (1, 3, "TNNP", "endocardialCells", 52.0),
     "ORd": (2, 4, "ORd", "myocyte", 80.0),
     "AP": (3, 4, "AlievPanfilov", "myocyte", 0.5),
+    "CRN": (5, 4, "Courtemanche", "myocyte", 60.0),
 }
 
 

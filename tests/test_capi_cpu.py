@@ -17,11 +17,13 @@ def test_library_exports_header_symbols():
 def test_model_and_tissue_lookup():
     L = _capi.lib()
     assert [L.cg_model_id(k) for k in (b"BuenoOrovio", b"TNNP", b"ORd", b"AlievPanfilov")] == [0, 1, 2, 3]
-    assert L.cg_model_id(b"Courtemanche") == -1
+    assert L.cg_model_id(b"Courtemanche") == 5
+    assert L.cg_model_id(b"ToRORd_dynCl") == -1
     assert [L.cg_tissue_flag(t) for t in (b"epicardialCells", b"mCells", b"endocardialCells",
                                           b"myocyte")] == [1, 2, 3, 4]
     assert L.cg_tissue_flag(b"purkinje") == -1
-    assert [L.cg_model_nstates(m) for m in range(4)] == [4, 17, 41, 2]
+    assert [L.cg_model_nstates(m) for m in range(6)] == [4, 17, 41, 2, 4, 21]
+    assert L.cg_model_nstates(6) == -1
     buf = C.create_string_buffer(32)
     names = []
     for i in range(4):
@@ -30,6 +32,11 @@ def test_model_and_tissue_lookup():
     assert names == [b"u", b"v", b"w", b"s"]
     _capi.check(L.cg_model_state_name(2, 40, buf, 32))
     assert buf.value == b"ryr_Jrelp"
+    # Courtemanche_1998.H:24-46 state names, first and last
+    _capi.check(L.cg_model_state_name(5, 0, buf, 32))
+    assert buf.value == b"membrane_V"
+    _capi.check(L.cg_model_state_name(5, 20, buf, 32))
+    assert buf.value == b"cajsr_w"
     with pytest.raises(_capi.CgError):
         _capi.check(L.cg_model_state_name(1, 17, buf, 32))
 
@@ -51,8 +58,10 @@ def test_mesh_ldu_matching(orc):
 def test_ionic_new_errors_without_device():
     import cardiacfoam_amd as cf
     with pytest.raises(cf.FatalIOError):
-        cf.IonicModel.New({"ionicModel": "Courtemanche", "tissue": "myocyte"}, 10, 1e-5)
+        cf.IonicModel.New({"ionicModel": "ToRORd_dynCl", "tissue": "myocyte"}, 10, 1e-5)
     with pytest.raises(cf.FatalIOError):
         cf.IonicModel.New({"ionicModel": "ORd", "tissue": "epicardialCells"}, 10, 1e-5)
+    with pytest.raises(cf.FatalIOError):
+        cf.IonicModel.New({"ionicModel": "Courtemanche", "tissue": "endocardialCells"}, 10, 1e-5)
     with pytest.raises(cf.FatalIOError):
         cf.IonicModel.New({"ionicModel": "TNNP"}, 10, 1e-5)

@@ -6,6 +6,9 @@ published models fix and a transcription slip would break:
     exactly the charge of the summed ionic current, F sum z V dc_total = -Cm Iion;
   - published action-potential shape at 1 Hz (resting potential, peak, APD90);
   - Rush-Larsen exactness for a gate with constant (xinf, tau).
+Courtemanche: the same charge balance, the published action potential from the
+published initial state, and the limits its generated code substitutes at the removable
+singularities of the rate expressions.
 AlievPanfilov: the cubic's roots 0, a, 1 and the recovery nullcline.
 Stimulus: stimulusIO.C:238-286 cases, including the S2 placement at the end of S1.
 """
@@ -137,13 +140,17 @@ def test_aliev_panfilov_cubic(orc):
 
 
 def test_tissue_support_matches_reference(orc):
-    # BuenoOrovio.C:92-96, TNNP.C:90-94, ORd.C:93-96, AlievPanfilov.C:92-95
+    # BuenoOrovio.C:92-96, TNNP.C:90-94, ORd.C:93-96, AlievPanfilov.C:92-95,
+    # Courtemanche.C:92-95
     for tissue in (1, 2, 3):
         orc.model_init(orc.BO, tissue)
         orc.model_init(orc.TNNP, tissue)
         with pytest.raises(ValueError):
             orc.model_init(orc.ORD, tissue)
+        with pytest.raises(ValueError):
+            orc.model_init(orc.CRN, tissue)
     orc.model_init(orc.ORD, 4)
+    orc.model_init(orc.CRN, 4)
     orc.model_init(orc.AP, 4)
     with pytest.raises(ValueError):
         orc.model_init(orc.BO, 4)
@@ -174,3 +181,63 @@ def test_stimulus_s2_placement(orc):
     # defaults: nstim1 = 1 when a positive S1 period is given (stimulusIO.C:179-183)
     assert orc.make_stim(period_s1=10.0).nstim1 == 1
     assert orc.make_stim().nstim1 == 0
+
+
+def test_crn_charge_conservation(orc):
+    # Courtemanche_1998.H:341-431 geometry and buffer constants
+    F, Cm, Vcell = 96.4867, 100.0, 20100.0
+    Vi, Vup, Vrel = 0.68 * Vcell, 0.0552 * Vcell, 0.0048 * Vcell
+    cmdn, kcmdn, trpn, ktrpn, csqn, kcsqn = 0.05, 0.00238, 0.07, 0.0005, 10.0, 0.8
+    c, states = _states_along_ap(orc, orc.CRN, orc.MYOCYTE, 40.0)
+    st = orc.make_stim(start=0.0, period_s1=1000.0, duration=2.0, amplitude=40.0, nstim1=1)
+    Nai, Ki, Cai, CaUp, CaRel = 1, 2, 3, 4, 5
+    for y in states:
+        for t, stim in ((500.0, None), (1.0, st)):
+            r = orc.model_eval(orc.CRN, orc.MYOCYTE, t, c, y, solve_vm=True, stim=stim)
+            d = r["rates"]
+            dCai_t = d[Cai] * (1 + trpn * ktrpn / (y[Cai] + ktrpn) ** 2 + cmdn * kcmdn / (y[Cai] + kcmdn) ** 2)
+            dCaRel_t = d[CaRel] * (1 + csqn * kcsqn / (y[CaRel] + kcsqn) ** 2)
+            lhs = F * (Vi * (d[Nai] + d[Ki] + 2 * dCai_t) + 2 * Vup * d[CaUp] + 2 * Vrel * dCaRel_t)
+            # the K_i balance omits the stimulus (Courtemanche_1998.H:644 reads Istim before :652)
+            rhs = -Cm * r["iion"]
+            assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-9 * Cm)
+            assert d[0] == pytest.approx(-(r["iion"] + r["istim"]), rel=1e-14)
+
+
+def test_crn_published_action_potential(orc):
+    """From the initial state of Courtemanche et al. (Am J Physiol 1998, Table 4) and a
+    2 ms, 2 nA (20 pA/pF) stimulus at 1 Hz the paper's action potential comes back:
+    resting potential -81 mV, overshoot +25 mV, APD90 about 300 ms."""
+    y0 = [-81.18, 11.17, 139.0, 1.013e-4, 1.488, 1.488, 2.908e-3, 0.9649, 0.9775, 3.043e-2,
+          0.9992, 4.966e-3, 0.9986, 3.296e-5, 1.869e-2, 1.367e-4, 0.9996, 0.7755, 0.0, 1.0,
+          0.9992]
+    st = orc.make_stim(start=10.0, period_s1=1000.0, duration=2.0, amplitude=20.0, nstim1=2)
+    t, v, _ = orc.single_cell_run(orc.CRN, orc.MYOCYTE, st, 1e-5, 200000, integrator=0, y0=y0)
+    beats = orc.detect_beats(t, v)
+    assert len(beats) == 2
+    for b in beats:
+        assert -82.0 < b["v_base"] < -80.0
+        assert 22.0 < b["v_peak"] < 28.0
+        assert 290.0 < (b["t_repol90"] - b["t_up"]) * 1e3 < 315.0
+    # the reference's own initial state (higher Na_i) shortens APD90; RL and RKF45 agree
+    t1, v1, _ = orc.single_cell_run(orc.CRN, orc.MYOCYTE, st, 1e-5, 200000, integrator=0)
+    t2, v2, _ = orc.single_cell_run(orc.CRN, orc.MYOCYTE, st, 1e-5, 200000, integrator=1)
+    b1, b2 = orc.detect_beats(t1, v1)[-1], orc.detect_beats(t2, v2)[-1]
+    assert 200.0 < (b1["t_repol90"] - b1["t_up"]) * 1e3 < 280.0
+    assert abs((b2["t_repol90"] - b2["t_up"]) - (b1["t_repol90"] - b1["t_up"])) < 1e-3
+
+
+@pytest.mark.parametrize("vs,gate", [(-47.13, 6), (-10.0, 15), (7.9, 20), (-14.1, 13),
+                                     (3.3328, 13), (19.9, 14)])
+def test_crn_removable_singularities(orc, vs, gate):
+    """At each removable singularity of a rate expression the generated code substitutes
+    a limit; the gate's rate there must join its values at V +- 1e-4 mV."""
+    c, y = orc.model_init(orc.CRN, orc.MYOCYTE)
+    rates = []
+    for dv in (-1e-4, 0.0, 1e-4):
+        yy = np.array(y)
+        yy[0] = vs + dv
+        yy[gate] = 0.5
+        rates.append(orc.model_eval(orc.CRN, orc.MYOCYTE, 0.0, c, yy, solve_vm=False)["rates"][gate])
+    mid = 0.5 * (rates[0] + rates[2])
+    assert rates[1] == pytest.approx(mid, rel=1e-5, abs=1e-12)
