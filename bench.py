@@ -37,7 +37,7 @@ WORKLOADS = {
 }
 
 # algorithmic bytes per cell (row) of one launch, by kernel and precision
-NSTATES = {"ORd": 41, "BuenoOrovio": 4, "TNNP": 17, "AlievPanfilov": 2}
+NSTATES = {"ORd": 41, "BuenoOrovio": 4, "TNNP": 17, "AlievPanfilov": 2, "Courtemanche": 21}
 
 
 def algo_bytes_per_cell(kernel, prec, model, algorithm, od_planes=1):
@@ -602,6 +602,41 @@ def run_adjoint_c4(cf, torch, shape=(64, 64, 32), h=2e-4, steps=5000, n_probes=2
             "full_horizon_one_direction": {"adjoint": float(g @ d0), "difference_quotients": spread}}
 
 
+def run_model_leg(cf, torch, w, sigma, model, steps=100, warmup=100, prof_steps=20):
+    """The workload's implicit tissue step with another ionic model (same mesh, fibres,
+    stimulus and solver): FP64 and mixed throughput, the mixed error gate, and the ionic
+    kernel's HBM rate at its algorithmic bytes."""
+    wm = dict(w, model=model, tissue="myocyte")
+    n = int(np.prod(wm["shape"]))
+    out = {"model": model, "steps": steps}
+    res = {}
+    for prec in ("fp64", "mixed"):
+        r = run_precision(cf, torch, wm, sigma, prec, steps, warmup, prof_steps)
+        res[prec] = r
+        ms, cnt = r["kernels"].get("ionic", (0.0, 0))
+        bpc = algo_bytes_per_cell("ionic", prec, model, wm["algorithm"])
+        avg = ms / cnt * 1e-3 if cnt else None
+        out[prec] = {"cell_steps_per_s": n * steps / r["elapsed"],
+                     "ms_per_step": r["elapsed"] / steps * 1e3,
+                     "cg_iterations_per_step": r["iters"] / steps,
+                     "ionic_us": round(avg * 1e6, 2) if avg else None,
+                     "ionic_bytes_per_cell": bpc,
+                     "ionic_GBps": round(bpc * n / avg / 1e9, 1) if avg else None}
+        del r["dom"]
+    a64, a32 = res["fp64"]["act"], res["mixed"]["act"]
+    both = (a64 > 0) & (a32 > 0)
+    out["mixed_over_fp64"] = out["mixed"]["cell_steps_per_s"] / out["fp64"]["cell_steps_per_s"]
+    out["mixed_vm_linf_mV"] = float(np.max(np.abs(res["fp64"]["V"] - res["mixed"]["V"])) * 1e3)
+    out["mixed_activation_linf_ms"] = float(np.max(np.abs(a64[both] - a32[both])) * 1e3) \
+        if both.any() else 0.0
+    out["activated_cells"] = int(np.count_nonzero(a64 > 0))
+    out["mixed_activation_set_mismatch"] = int(np.count_nonzero((a64 > 0) != (a32 > 0)))
+    print("[bench] %s leg: fp64 %.3f ms, mixed %.3f ms per step" %
+          (model, out["fp64"]["ms_per_step"], out["mixed"]["ms_per_step"]),
+          file=sys.stderr, flush=True)
+    return out
+
+
 def run_reference_semantics(cf, wname, w, sigma, cpu_steps, with_cpu, ord_steps=20):
     """The reference's default integrator on the device.
     (1) NiedererEtAl2012 regression case as the reference runs it: 100x15x35 hex of
@@ -872,6 +907,8 @@ def main():
     c4 = run_adjoint_c4(cf, torch) if not args.no_batch else None
     ref_sem = run_reference_semantics(cf, args.workload, w, sigma, 100, not args.no_cpu) \
         if w["model"] == "ORd" and not args.no_batch else None
+    crn = run_model_leg(cf, torch, w, sigma, "Courtemanche") \
+        if w["algorithm"] == "implicit" and not args.no_batch else None
 
     io = run_field_io({"Vm": V64, "activationTime": a64})
     print("[bench] field files: write %.3f s, read %.3f s for %d values" %
@@ -910,6 +947,7 @@ def main():
         "batch_c2": c2,
         "adjoint_c4": c4,
         "rkf45": ref_sem,
+        "courtemanche_2M": crn,
         "field_io": io,
         "pseudo_ecg": ecg,
         "general_mesh": general,
