@@ -40,9 +40,10 @@ int fail(int code, const char* fmt, ...)
             return fail(CG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));           \
     } while (0)
 
-constexpr int kNumModels = 6;
+constexpr int kNumModels = 7;
 const char* const kModelKeys[kNumModels] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
-                                            "monodomainFDAManufactured", "Courtemanche"};
+                                            "monodomainFDAManufactured", "Courtemanche",
+                                            "PerisYague_2022"};
 
 // state names, order of the *_STATES_NAMES tables / Names.H enums
 const char* const kBONames[4] = {"u", "v", "w", "s"};
@@ -61,6 +62,10 @@ const char* const kCRNNames[21] = {
     "membrane_V", "sodium_Nai", "potassium_Ki", "calcium_Cai", "calcium_CaUp", "calcium_CaRel",
     "ina_m", "ina_h", "ina_j", "ito_oa", "ito_oi", "ikur_ua", "ikur_ui", "ikr_xr", "iks_xs",
     "ical_d", "ical_f", "ical_fCa", "cajsr_u", "cajsr_v", "cajsr_w"};
+const char* const kPYSNames[22] = {
+    "membrane_V", "sodium_Nai", "potassium_Ki", "chloride_Cli", "calcium_Cai", "calcium_CaUp",
+    "calcium_CaRel", "ina_m", "ina_h", "ina_j", "ikur_ua", "ikur_uif", "ikur_uis", "ikr_xr",
+    "iks_xs", "ical_d", "ical_f", "ical_fCa", "iclca_qCa", "ryr_u", "ryr_v", "ryr_w"};
 
 struct ModelInfo {
     int ns, np;
@@ -90,6 +95,9 @@ bool model_info(int model, ModelInfo* mi)
     case cg::MODEL_CRN:
         *mi = {cg::Courtemanche::NS, cg::Courtemanche::NP, 1.0, 0.0, 1.0, 1000.0};
         return true;
+    case cg::MODEL_PYS:
+        *mi = {cg::PerisYague::NS, cg::PerisYague::NP, 1.0, 0.0, 1.0, 1000.0};
+        return true;
     default: return false;
     }
 }
@@ -103,6 +111,7 @@ bool model_host_init(int model, int tissue, double* p, double* y)
     case cg::MODEL_AP: return cg::AlievPanfilov::host_init(tissue, p, y);
     case cg::MODEL_FDA: return cg::FdaManufactured::host_init(tissue, p, y);
     case cg::MODEL_CRN: return cg::Courtemanche::host_init(tissue, p, y);
+    case cg::MODEL_PYS: return cg::PerisYague::host_init(tissue, p, y);
     default: return false;
     }
 }
@@ -312,7 +321,7 @@ int cg_model_state_name(int model, int i, char* buf, size_t len)
     if (i < 0 || i >= mi.ns || !buf || len == 0) return fail(CG_ERR_ARG, "bad state index %d", i);
     const char* nm = model == 0 ? kBONames[i] : model == 1 ? kTNNPNames[i]
                    : model == 2 ? kORdNames[i] : model == 3 ? kAPNames[i]
-                   : model == 4 ? kFDANames[i] : kCRNNames[i];
+                   : model == 4 ? kFDANames[i] : model == 5 ? kCRNNames[i] : kPYSNames[i];
     snprintf(buf, len, "%s", nm);
     return CG_OK;
 }
@@ -393,8 +402,10 @@ int cg_ionic_set_constant(cg_ionic* h, int k, double v)
 int cg_ionic_set_conductance_scale(cg_ionic* h, const double* s)
 {
     if (!h) return fail(CG_ERR_ARG, "NULL handle");
-    if (h->model != cg::MODEL_TNNP && h->model != cg::MODEL_ORD && h->model != cg::MODEL_CRN && s)
-        return fail(CG_ERR_STATE, "conductance scaling is defined for TNNP, ORd and Courtemanche only");
+    if (h->model != cg::MODEL_TNNP && h->model != cg::MODEL_ORD && h->model != cg::MODEL_CRN &&
+        h->model != cg::MODEL_PYS && s)
+        return fail(CG_ERR_STATE,
+                    "conductance scaling is defined for TNNP, ORd, Courtemanche and PerisYague_2022 only");
     HIPCHK(hipStreamSynchronize(h->stream));
     if (!s) {
         if (h->d_gscale) hipFree(h->d_gscale);

@@ -352,6 +352,7 @@ template <class T> struct ModelLaunch {
     case MODEL_AP: return ModelLaunch<T>::template FN<AlievPanfilov>(__VA_ARGS__);         \
     case MODEL_FDA: return ModelLaunch<T>::template FN<FdaManufactured>(__VA_ARGS__);      \
     case MODEL_CRN: return ModelLaunch<T>::template FN<Courtemanche>(__VA_ARGS__);         \
+    case MODEL_PYS: return ModelLaunch<T>::template FN<PerisYague>(__VA_ARGS__);           \
     default: return hipErrorInvalidValue;                                                  \
     }
 

@@ -14,11 +14,12 @@
 #include "model_ord.hpp"
 #include "model_fda.hpp"
 #include "model_crn.hpp"
+#include "model_pys.hpp"
 
 namespace cg {
 
 enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3, MODEL_FDA = 4,
-               MODEL_CRN = 5 };
+               MODEL_CRN = 5, MODEL_PYS = 6 };
 
 constexpr int MAX_PARAMS = 48;
 struct ModelParams {
@@ -198,6 +199,39 @@ template <> struct Ops<Courtemanche> {
                              T istim, const Courtemanche::VTerms<T>& q)
     {
         Courtemanche::deriv(y, d, p, gs, solve_v, istim, q);
+    }
+};
+
+template <> struct Ops<PerisYague> {
+    template <class T>
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = PerisYague::vterms(v, p);
+        PerisYague::step(y, dt, p, gs, false, T(0), q);
+        return PerisYague::currents(v, y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = PerisYague::vterms(y[0], p);
+        PerisYague::step(y, dt, p, gs, true, istim, q);
+    }
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T* gs)
+    {
+        const auto q = PerisYague::vterms(y[0], p);
+        return PerisYague::currents(y[0], y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static PerisYague::VTerms<T> vt(T v, const T* p)
+    {
+        return PerisYague::vterms(v, p);
+    }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int, const T* gs, bool solve_v,
+                             T istim, const PerisYague::VTerms<T>& q)
+    {
+        PerisYague::deriv(y, d, p, gs, solve_v, istim, q);
     }
 };
 

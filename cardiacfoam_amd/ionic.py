@@ -15,13 +15,14 @@ from . import _capi
 from ._capi import check, lib
 
 MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov", "monodomainFDAManufactured",
-              "Courtemanche"]
+              "Courtemanche", "PerisYague_2022"]
 SUPPORTED_TISSUES = {
     "BuenoOrovio": ["endocardialCells", "mCells", "epicardialCells"],
     "TNNP": ["endocardialCells", "mCells", "epicardialCells"],
     "ORd": ["myocyte"],
     "AlievPanfilov": ["myocyte"],
     "Courtemanche": ["myocyte"],  # Courtemanche.C:92-95
+    "PerisYague_2022": ["myocyte"],  # PerisYague.C:92-95
     # the manufactured model's supportedDimensions() (monodomainFDAManufactured.C:85-88)
     "monodomainFDAManufactured": ["1D", "2D", "3D"],
 }

@@ -862,6 +862,169 @@ static void crn_eval(double t, const double* c, const double* y, int solve_vm,
     *iion = Iion; *istim = Ist;
 }
 
+/* ========================================================= PerisYague_2022
+ * Peris-Yague et al. 2022 human atrial model with chloride, 22 states: constants and
+ * initial state of PerisYague_2022.H:211-294, right-hand side of :296-445, state order
+ * of its STATES_NAMES table (:24-47).  Selected by the key "PerisYague_2022"
+ * (PerisYague.H:124); tissue "myocyte" only (PerisYague.C:92-95).  Differences from
+ * Courtemanche that the restatement keeps as the reference has them: the h-gate beta
+ * above -40 mV is 0.13/(1 + exp(-(V + 10.66)/11.1)) (:333); the d-gate time constant's
+ * guard at V = -5 mV returns 1.0, not the expression's limit (:368); the IKr and IKs
+ * alpha/beta quotients have no guard (0/0 at V = -14.1, 3.3328, 18.80816 mV exactly);
+ * IbK (gbK = 0) enters neither Iion nor the K_i balance; the K_i balance has no stimulus.
+ * Parity pins (tests/test_oracle_models.py): charge conservation including the chloride
+ * balance, continuity at the guarded singularities of m and w, and the rate limits of the
+ * unguarded quotients approached from both sides. */
+enum { PY_V, PY_Nai, PY_Ki, PY_Cli, PY_Cai, PY_CaUp, PY_CaRel, PY_m, PY_h, PY_j, PY_ua,
+       PY_uif, PY_uis, PY_xr, PY_xs, PY_d, PY_f, PY_fCa, PY_qCa, PY_u, PY_v, PY_w, PY_NS };
+enum { PC_R, PC_T, PC_F, PC_Cm, PC_V_i, PC_V_up, PC_V_rel, PC_Ko, PC_Nao, PC_Cao, PC_Clo,
+       PC_gNa, PC_gK1, PC_gClCa, PC_gKur_amp, PC_gKr, PC_gKs, PC_gCaL, PC_gbCa, PC_gbNa,
+       PC_gbK, PC_INaK_max, PC_INaCa_max, PC_IpCa_max, PC_I_up_max, PC_KQ10, PC_gam,
+       PC_KmNai, PC_KmKo, PC_KmNa, PC_KmCa, PC_ksat, PC_krel, PC_K_up, PC_Ca_up_max,
+       PC_CMDN_max, PC_TRPN_max, PC_CSQN_max, PC_Km_CMDN, PC_Km_TRPN, PC_Km_CSQN,
+       PC_tau_tr, PC_c1, PC_c2, PC_u_tau, PC_fCa_tau, PC_RTF, PC_FRT, PC_sigma, PC_NC };
+
+static void pys_init(double* c, double* y)
+{
+    c[PC_R] = 8.3143; c[PC_T] = 310.0; c[PC_F] = 96.4867; c[PC_Cm] = 100.0;
+    c[PC_V_i] = 13668.0; c[PC_V_up] = 1109.52; c[PC_V_rel] = 96.48;
+    c[PC_Ko] = 5.4; c[PC_Nao] = 140.0; c[PC_Cao] = 1.8; c[PC_Clo] = 132.0;
+    c[PC_gNa] = 13.99; c[PC_gK1] = 0.08218; c[PC_gClCa] = 0.15731; c[PC_gKur_amp] = 0.45539;
+    c[PC_gKr] = 0.01730; c[PC_gKs] = 0.0594; c[PC_gCaL] = 0.06574; c[PC_gbCa] = 0.00113;
+    c[PC_gbNa] = 0.000674; c[PC_gbK] = 0.0;
+    c[PC_INaK_max] = 0.94935; c[PC_INaCa_max] = 2304.0; c[PC_IpCa_max] = 0.275;
+    c[PC_I_up_max] = 0.005; c[PC_KQ10] = 3.0; c[PC_gam] = 0.35; c[PC_KmNai] = 10.0;
+    c[PC_KmKo] = 1.5; c[PC_KmNa] = 87.5; c[PC_KmCa] = 1.38; c[PC_ksat] = 0.1;
+    c[PC_krel] = 30.0; c[PC_K_up] = 0.00092; c[PC_Ca_up_max] = 15.0;
+    c[PC_CMDN_max] = 0.05; c[PC_TRPN_max] = 0.07; c[PC_CSQN_max] = 10.0;
+    c[PC_Km_CMDN] = 0.00238; c[PC_Km_TRPN] = 0.0005; c[PC_Km_CSQN] = 0.8;
+    c[PC_tau_tr] = 180.0; c[PC_c1] = 1.367e-13; c[PC_c2] = 13.67e-16; c[PC_u_tau] = 8.0;
+    c[PC_fCa_tau] = 2.0;
+    c[PC_RTF] = c[PC_R] * c[PC_T] / c[PC_F]; c[PC_FRT] = 1.0 / c[PC_RTF];
+    c[PC_sigma] = (exp(c[PC_Nao] / 67.3) - 1.0) / 7.0;
+    static const double y0[PY_NS] = {
+        -76.0, 11.17, 139.0, 30.0, 0.0001013, 1.488, 1.488, 2.91e-3, 0.965, 0.978, 4.96e-3,
+        1.0, 1.0, 3.29e-5, 1.87e-2, 1.37e-4, 0.999, 0.775, 0.0, 0.0235, 1.0, 0.999 };
+    memcpy(y, y0, sizeof y0);
+}
+
+static void pys_eval(double t, const double* c, const double* y, int solve_vm,
+                     const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                     double* iion, double* istim)
+{
+    const double V = y[PY_V], Cai = y[PY_Cai], Nai = y[PY_Nai], Ki = y[PY_Ki];
+    const double Cli = y[PY_Cli], CaRel = y[PY_CaRel], CaUp = y[PY_CaUp];
+    const double RTF = c[PC_RTF], FRT = c[PC_FRT];
+    for (int i = 0; i < PY_NS; ++i) g[i] = 0;
+#define PY_GATE(idx, inf, tau) do { r[idx] = ((inf) - y[idx]) / (tau); g[idx] = 1; \
+                                    xi[idx] = (inf); ta[idx] = (tau); } while (0)
+    const double ENa = RTF * log(c[PC_Nao] / Nai);
+    const double EK = RTF * log(c[PC_Ko] / Ki);
+    const double ECa = 0.5 * RTF * log(c[PC_Cao] / Cai);
+    const double ECl = -RTF * log(c[PC_Clo] / Cli);
+
+    /* INa (time constants scaled by 1.7, 2, 2) */
+    const double am = fabs(V + 47.13) < 1e-6 ? 3.2 : 0.32 * (V + 47.13) / (1.0 - exp(-0.1 * (V + 47.13)));
+    const double bm = 0.08 * exp(-V / 11.0);
+    PY_GATE(PY_m, am / (am + bm), 1.7 / (am + bm));
+    const int lo = V < -40.0;
+    const double ah = lo ? 0.135 * exp(-(V + 80.0) / 6.8) : 0.0;
+    const double bh = lo ? 3.56 * exp(0.079 * V) + 3.1e5 * exp(0.35 * V)
+                         : 0.13 / (1.0 + exp(-(V + 10.66) / 11.1));
+    PY_GATE(PY_h, ah / (ah + bh), 2.0 / (ah + bh));
+    const double aj = lo ? (-127140.0 * exp(0.2444 * V) - 3.474e-5 * exp(-0.04391 * V)) *
+                               (V + 37.78) / (1.0 + exp(0.311 * (V + 79.23)))
+                         : 0.0;
+    const double bj = lo ? 0.1212 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)))
+                         : 0.3 * exp(-2.535e-7 * V) / (1.0 + exp(-0.1 * (V + 32.0)));
+    PY_GATE(PY_j, aj / (aj + bj), 2.0 / (aj + bj));
+    const double INa = c[PC_gNa] * pow(y[PY_m], 3.0) * y[PY_h] * y[PY_j] * (V - ENa);
+
+    const double IK1 = c[PC_gK1] * (V - EK - 5.0) / (1.0 + exp(0.063 * (V + 70.0)));
+
+    /* IKr: xr (steady state from alpha/beta) */
+    const double axr = 0.0003 * (V + 14.1) / (1.0 - exp(-(V + 14.1) / 5.0));
+    const double bxr = 7.3898e-5 * (V - 3.3328) / (exp((V - 3.3328) / 5.1237) - 1.0);
+    PY_GATE(PY_xr, axr / (axr + bxr), 1.0 / (axr + bxr));
+    const double IKr = c[PC_gKr] * y[PY_xr] * (V - EK) / (1.0 + exp((V + 15.0) / 22.4));
+
+    /* IKs: xs, rates around 18.80816 mV, steady state around 19.9 mV */
+    const double axs = 4e-5 * (V - 18.80816) / (1.0 - exp(-(V - 18.80816) / 17.0));
+    const double bxs = 3.5e-5 * (V - 18.80816) / (exp((V - 18.80816) / 9.0) - 1.0);
+    PY_GATE(PY_xs, 1.0 / sqrt(1.0 + exp(-(V - 19.9) / 12.7)), 0.5 / (axs + bxs));
+    const double IKs = c[PC_gKs] * pow(y[PY_xs], 2.0) * (V - EK);
+
+    /* IKur: ua, fast and slow inactivation */
+    const double gKur = c[PC_gKur_amp] * (0.005 + 0.05 / (1.0 + exp(-(V - 15.0) / 13.0)));
+    const double aua = 0.65 / (exp(-(V + 10.0) / 8.5) + exp(-(V - 30.0) / 59.0));
+    const double bua = 0.65 / (2.5 + exp((V + 82.0) / 17.0));
+    PY_GATE(PY_ua, 1.0 / (1.0 + exp(-(V + 30.3) / 9.6)), 1.0 / (c[PC_KQ10] * (aua + bua)));
+    const double uinf = 1.0 / (1.0 + exp((V + 17.5) / 5.85));
+    PY_GATE(PY_uif, uinf, 400.0 + 1068.0 * exp(-pow(V / 50.0, 2.0)));
+    PY_GATE(PY_uis, uinf, 2000.0 + 60000.0 * exp(-pow((V + 39.3) / 30.0, 2.0)));
+    const double IKur = gKur * pow(y[PY_ua], 3.0) * (0.25 * y[PY_uif] + 0.75 * y[PY_uis]) * (V - EK);
+
+    /* ICaL */
+    const double td = fabs(V + 5.0) < 1e-6 ? 1.0
+                      : (1.0 - exp(-(V + 5.0) / 6.24)) /
+                            (0.035 * (V + 5.0) * (1.0 + exp(-(V + 5.0) / 6.24)));
+    PY_GATE(PY_d, 1.0 / (1.0 + exp(-(V + 5.0) / 8.0)), td);
+    PY_GATE(PY_f, 1.0 / (1.0 + exp((V + 28.0) / 6.9)),
+            9.0 / (0.0197 * exp(-0.03372 * pow(V + 5.0, 2.0)) + 0.02));
+    PY_GATE(PY_fCa, 1.0 / (1.0 + Cai / 0.00035), c[PC_fCa_tau]);
+    const double ICaL = c[PC_gCaL] * y[PY_d] * y[PY_f] * y[PY_fCa] * (V - 65.0);
+
+    /* pumps, exchanger, background */
+    const double fNaK = 1.0 / (1.0 + 0.1245 * exp(-0.1 * V * FRT) + 0.0365 * c[PC_sigma] * exp(-V * FRT));
+    const double INaK = c[PC_INaK_max] * fNaK * c[PC_Ko] / (c[PC_Ko] + c[PC_KmKo]) /
+                        (1.0 + pow(c[PC_KmNai] / Nai, 1.5));
+    const double eg1 = exp((c[PC_gam] - 1.0) * V * FRT);
+    const double INaCa = c[PC_INaCa_max] *
+                         (exp(c[PC_gam] * V * FRT) * pow(Nai, 3.0) * c[PC_Cao] - eg1 * pow(c[PC_Nao], 3.0) * Cai) /
+                         ((pow(c[PC_KmNa], 3.0) + pow(c[PC_Nao], 3.0)) * (c[PC_KmCa] + c[PC_Cao]) *
+                          (1.0 + c[PC_ksat] * eg1));
+    const double IbNa = c[PC_gbNa] * (V - ENa);
+    const double IbCa = c[PC_gbCa] * (V - ECa);
+    const double IpCa = c[PC_IpCa_max] * Cai / (0.0005 + Cai);
+
+    /* SR release and the Ca-activated chloride current, both driven by Fn */
+    const double Irel = c[PC_krel] * pow(y[PY_u], 2.0) * y[PY_v] * y[PY_w] * (CaRel - Cai);
+    const double Itr = (CaUp - CaRel) / c[PC_tau_tr];
+    const double Iup = c[PC_I_up_max] / (1.0 + c[PC_K_up] / Cai);
+    const double Ileak = c[PC_I_up_max] * CaUp / c[PC_Ca_up_max];
+    const double Fn = 1e-12 * c[PC_V_rel] * Irel - 5e-13 / c[PC_F] * (0.5 * ICaL - 0.2 * INaCa) * c[PC_Cm];
+    PY_GATE(PY_qCa, 1.0 - pow(1.0 / (1.0 + Fn / 1.1e-10), 3.0), 2.0);
+    const double IClCa = c[PC_gClCa] * y[PY_qCa] * (V - ECl);
+    {
+        const double xw = V - 7.9;
+        const double tw = 6.0 * (fabs(xw) < 1e-6 ? 2.0 / 13.0
+                                 : (1.0 - exp(-xw / 5.0)) / ((1.0 + 0.3 * exp(-xw / 5.0)) * xw));
+        PY_GATE(PY_w, 1.0 - 1.0 / (1.0 + exp(-(V - 40.0) / 17.0)), tw);
+        const double eu = exp(-(Fn - c[PC_c1]) / c[PC_c2]);
+        PY_GATE(PY_u, 1.0 / (1.0 + eu), c[PC_u_tau]);
+        PY_GATE(PY_v, 1.0 - 1.0 / (1.0 + exp(-(Fn - 0.2 * c[PC_c1]) / c[PC_c2])),
+                1.91 + 2.09 / (1.0 + eu));
+    }
+#undef PY_GATE
+
+    const double B1 = (2.0 * INaCa - (IpCa + ICaL + IbCa)) * c[PC_Cm] / (2.0 * c[PC_V_i] * c[PC_F]) +
+                      (c[PC_V_up] * (Ileak - Iup) + Irel * c[PC_V_rel]) / c[PC_V_i];
+    const double B2 = 1.0 + c[PC_TRPN_max] * c[PC_Km_TRPN] / pow(Cai + c[PC_Km_TRPN], 2.0) +
+                      c[PC_CMDN_max] * c[PC_Km_CMDN] / pow(Cai + c[PC_Km_CMDN], 2.0);
+    r[PY_Cai] = B1 / B2;
+    r[PY_CaRel] = (Itr - Irel) / (1.0 + c[PC_CSQN_max] * c[PC_Km_CSQN] / pow(CaRel + c[PC_Km_CSQN], 2.0));
+    r[PY_CaUp] = Iup - (Ileak + Itr * c[PC_V_rel] / c[PC_V_up]);
+    const double cvf = c[PC_Cm] / (c[PC_V_i] * c[PC_F]);
+    r[PY_Nai] = (-3.0 * INaK - (3.0 * INaCa + IbNa + INa)) * cvf;
+    r[PY_Ki] = (2.0 * INaK - (IK1 + IKur + IKr + IKs)) * cvf;
+    r[PY_Cli] = IClCa * cvf;
+
+    const double Iion = INa + IK1 + IKur + IKr + IKs + ICaL + IpCa + INaK + INaCa + IbNa + IbCa + IClCa;
+    const double Ist = orc_stimulus(t, st);
+    r[PY_V] = solve_vm ? -Iion - Ist : 0.0;
+    *iion = Iion; *istim = Ist;
+}
+
 /* ============================================================== dispatch */
 int orc_model_nstates(int model)
 {
@@ -872,6 +1035,7 @@ int orc_model_nstates(int model)
     case ORC_AP: return 2;
     case ORC_FDA: return 4;
     case ORC_CRN: return CR_NS;
+    case ORC_PYS: return PY_NS;
     default: return -1;
     }
 }
@@ -898,6 +1062,9 @@ int orc_model_init(int model, int tissue, double* c, double* y)
     case ORC_CRN:
         if (tissue != ORC_MYOCYTE) return -1;
         crn_init(c, y); return 0;
+    case ORC_PYS:
+        if (tissue != ORC_MYOCYTE) return -1;
+        pys_init(c, y); return 0;
     default: return -1;
     }
 }
@@ -919,6 +1086,11 @@ int orc_model_const_index(int model, const char* name)
         if (!strcmp(name, "gKr")) return CC_gKr;
         if (!strcmp(name, "gCaL")) return CC_gCaL;
         if (!strcmp(name, "gKs")) return CC_gKs;
+    } else if (model == ORC_PYS) {
+        if (!strcmp(name, "gNa")) return PC_gNa;
+        if (!strcmp(name, "gKr")) return PC_gKr;
+        if (!strcmp(name, "gCaL")) return PC_gCaL;
+        if (!strcmp(name, "gKs")) return PC_gKs;
     } else if (model == ORC_BO) {
         if (!strcmp(name, "tauFi")) return BO_tfi;
         if (!strcmp(name, "tauSi")) return BO_tsi;
@@ -937,6 +1109,7 @@ void orc_model_eval(int model, int tissue, double t, const double* c, const doub
     case ORC_AP: ap_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_FDA: fda_eval(c, y, r, g, iion); *istim = 0.0; break;
     case ORC_CRN: crn_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_PYS: pys_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     default: break;
     }
 }

@@ -10,6 +10,7 @@ MODELS = {  # name -> (oracle id, tissue flag, library key, tissue word, stimulu
     "ORd": (2, 4, "ORd", "myocyte", 80.0),
     "AP": (3, 4, "AlievPanfilov", "myocyte", 0.5),
     "CRN": (5, 4, "Courtemanche", "myocyte", 60.0),
+    "PYS": (6, 4, "PerisYague_2022", "myocyte", 40.0),
 }
 
 

@@ -9,6 +9,10 @@ published models fix and a transcription slip would break:
 Courtemanche: the same charge balance, the published action potential from the
 published initial state, and the limits its generated code substitutes at the removable
 singularities of the rate expressions.
+PerisYague_2022: the charge balance with chloride, its guard values and its unguarded
+quotients at their singular voltages.  Its gate kinetics have no published trace to pin
+against (the reference's h-gate beta differs from Courtemanche's): parity unpinned there
+beyond the device-vs-oracle agreement.
 AlievPanfilov: the cubic's roots 0, a, 1 and the recovery nullcline.
 Stimulus: stimulusIO.C:238-286 cases, including the S2 placement at the end of S1.
 """
@@ -141,7 +145,7 @@ def test_aliev_panfilov_cubic(orc):
 
 def test_tissue_support_matches_reference(orc):
     # BuenoOrovio.C:92-96, TNNP.C:90-94, ORd.C:93-96, AlievPanfilov.C:92-95,
-    # Courtemanche.C:92-95
+    # Courtemanche.C:92-95, PerisYague.C:92-95
     for tissue in (1, 2, 3):
         orc.model_init(orc.BO, tissue)
         orc.model_init(orc.TNNP, tissue)
@@ -149,8 +153,11 @@ def test_tissue_support_matches_reference(orc):
             orc.model_init(orc.ORD, tissue)
         with pytest.raises(ValueError):
             orc.model_init(orc.CRN, tissue)
+        with pytest.raises(ValueError):
+            orc.model_init(orc.PYS, tissue)
     orc.model_init(orc.ORD, 4)
     orc.model_init(orc.CRN, 4)
+    orc.model_init(orc.PYS, 4)
     orc.model_init(orc.AP, 4)
     with pytest.raises(ValueError):
         orc.model_init(orc.BO, 4)
@@ -241,3 +248,60 @@ def test_crn_removable_singularities(orc, vs, gate):
         rates.append(orc.model_eval(orc.CRN, orc.MYOCYTE, 0.0, c, yy, solve_vm=False)["rates"][gate])
     mid = 0.5 * (rates[0] + rates[2])
     assert rates[1] == pytest.approx(mid, rel=1e-5, abs=1e-12)
+
+
+def test_pys_charge_conservation(orc):
+    # PerisYague_2022.H:214-263 volumes and buffer constants; chloride carries charge -1
+    F, Cm, Vi, Vup, Vrel = 96.4867, 100.0, 13668.0, 1109.52, 96.48
+    cmdn, kcmdn, trpn, ktrpn, csqn, kcsqn = 0.05, 0.00238, 0.07, 0.0005, 10.0, 0.8
+    c, states = _states_along_ap(orc, orc.PYS, orc.MYOCYTE, 40.0)
+    st = orc.make_stim(start=0.0, period_s1=1000.0, duration=2.0, amplitude=40.0, nstim1=1)
+    Nai, Ki, Cli, Cai, CaUp, CaRel = 1, 2, 3, 4, 5, 6
+    for y in states:
+        for t, stim in ((500.0, None), (1.0, st)):
+            r = orc.model_eval(orc.PYS, orc.MYOCYTE, t, c, y, solve_vm=True, stim=stim)
+            d = r["rates"]
+            dCai_t = d[Cai] * (1 + trpn * ktrpn / (y[Cai] + ktrpn) ** 2 + cmdn * kcmdn / (y[Cai] + kcmdn) ** 2)
+            dCaRel_t = d[CaRel] * (1 + csqn * kcsqn / (y[CaRel] + kcsqn) ** 2)
+            lhs = F * (Vi * (d[Nai] + d[Ki] - d[Cli] + 2 * dCai_t) + 2 * Vup * d[CaUp] + 2 * Vrel * dCaRel_t)
+            rhs = -Cm * r["iion"]
+            assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-9 * Cm)
+            assert d[0] == pytest.approx(-(r["iion"] + r["istim"]), rel=1e-14)
+
+
+@pytest.mark.parametrize("vs,gate,guarded", [(-47.13, 7, True), (7.9, 21, True),
+                                             (-14.1, 13, False), (3.3328, 13, False),
+                                             (18.80816, 14, False)])
+def test_pys_singularities(orc, vs, gate, guarded):
+    """Guarded singularities (m, w) return the expression's limit; the unguarded IKr/IKs
+    quotients are 0/0 exactly at the singular voltage (PerisYague_2022.H:348-357) and join
+    their neighbours on both sides."""
+    c, y = orc.model_init(orc.PYS, orc.MYOCYTE)
+
+    def rate(v):
+        yy = np.array(y)
+        yy[0], yy[gate] = v, 0.5
+        return orc.model_eval(orc.PYS, orc.MYOCYTE, 0.0, c, yy, solve_vm=False)["rates"][gate]
+
+    lo, hi = rate(vs - 1e-4), rate(vs + 1e-4)
+    if guarded:
+        assert rate(vs) == pytest.approx(0.5 * (lo + hi), rel=1e-5, abs=1e-12)
+    else:
+        assert np.isnan(rate(vs))
+        # linear extrapolations to vs from either side agree: the quotient has a finite,
+        # continuous limit there
+        left, right = 2 * lo - rate(vs - 2e-4), 2 * hi - rate(vs + 2e-4)
+        assert left == pytest.approx(right, rel=1e-6, abs=1e-12)
+
+
+def test_pys_d_gate_guard_value(orc):
+    """The d-gate time constant's guard at V = -5 mV returns 1.0 (PerisYague_2022.H:368),
+    while the expression tends to 1/(6.24 * 2 * 0.035) on both sides."""
+    c, y = orc.model_init(orc.PYS, orc.MYOCYTE)
+    dinf = 1.0 / (1.0 + np.exp(0.0))
+    for v, tau in ((-5.0, 1.0), (-5.0 + 1e-5, 1.0 / (6.24 * 2 * 0.035)),
+                   (-5.0 - 1e-5, 1.0 / (6.24 * 2 * 0.035))):
+        yy = np.array(y)
+        yy[0], yy[15] = v, 0.25
+        r = orc.model_eval(orc.PYS, orc.MYOCYTE, 0.0, c, yy, solve_vm=False)["rates"][15]
+        assert r == pytest.approx((dinf - 0.25) / tau, rel=1e-4)
