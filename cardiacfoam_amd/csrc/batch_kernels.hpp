@@ -269,6 +269,7 @@ template <class T> struct BatchLaunch {
         case MODEL_AP: return BatchLaunch<T>::variant<AlievPanfilov>(variant, tissue, states, P, gscale, a, s); \
         case MODEL_CRN: return BatchLaunch<T>::variant<Courtemanche>(variant, tissue, states, P, gscale, a, s); \
         case MODEL_PYS: return BatchLaunch<T>::variant<PerisYague>(variant, tissue, states, P, gscale, a, s); \
+        case MODEL_STW: return BatchLaunch<T>::variant<Stewart>(variant, tissue, states, P, gscale, a, s); \
         default: return hipErrorInvalidValue;                                              \
         }                                                                                  \
     }

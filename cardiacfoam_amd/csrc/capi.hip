@@ -40,10 +40,10 @@ int fail(int code, const char* fmt, ...)
             return fail(CG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));           \
     } while (0)
 
-constexpr int kNumModels = 7;
+constexpr int kNumModels = 8;
 const char* const kModelKeys[kNumModels] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
                                             "monodomainFDAManufactured", "Courtemanche",
-                                            "PerisYague_2022"};
+                                            "PerisYague_2022", "Stewart"};
 
 // state names, order of the *_STATES_NAMES tables / Names.H enums
 const char* const kBONames[4] = {"u", "v", "w", "s"};
@@ -66,6 +66,16 @@ const char* const kPYSNames[22] = {
     "membrane_V", "sodium_Nai", "potassium_Ki", "chloride_Cli", "calcium_Cai", "calcium_CaUp",
     "calcium_CaRel", "ina_m", "ina_h", "ina_j", "ikur_ua", "ikur_uif", "ikur_uis", "ikr_xr",
     "iks_xs", "ical_d", "ical_f", "ical_fCa", "iclca_qCa", "ryr_u", "ryr_v", "ryr_w"};
+const char* const kSTWNames[20] = {
+    "membrane_V", "Ihyperpolarization_activated_current_y_gate_y",
+    "Irapid_time_dependent_potassium_current_Xr1_gate_Xr1",
+    "Irapid_time_dependent_potassium_current_Xr2_gate_Xr2",
+    "Islow_time_dependent_potassium_current_Xs_gate_Xs", "Ifast_sodium_current_m_gate_m",
+    "Ifast_sodium_current_h_gate_h", "Ifast_sodium_current_j_gate_j",
+    "IL_type_Ca_current_d_gate_d", "IL_type_Ca_current_f_gate_f",
+    "IL_type_Ca_current_f2_gate_f2", "IL_type_Ca_current_fCass_gate_fCass",
+    "Itransient_outward_current_s_gate_s", "Itransient_outward_current_r_gate_r", "Ca_i",
+    "Ca_SR", "Ca_ss", "R_prime", "Na_i", "K_i"};
 
 struct ModelInfo {
     int ns, np;
@@ -98,6 +108,9 @@ bool model_info(int model, ModelInfo* mi)
     case cg::MODEL_PYS:
         *mi = {cg::PerisYague::NS, cg::PerisYague::NP, 1.0, 0.0, 1.0, 1000.0};
         return true;
+    case cg::MODEL_STW:
+        *mi = {cg::Stewart::NS, cg::Stewart::NP, 1.0, 0.0, 1.0, 1000.0};
+        return true;
     default: return false;
     }
 }
@@ -112,6 +125,7 @@ bool model_host_init(int model, int tissue, double* p, double* y)
     case cg::MODEL_FDA: return cg::FdaManufactured::host_init(tissue, p, y);
     case cg::MODEL_CRN: return cg::Courtemanche::host_init(tissue, p, y);
     case cg::MODEL_PYS: return cg::PerisYague::host_init(tissue, p, y);
+    case cg::MODEL_STW: return cg::Stewart::host_init(tissue, p, y);
     default: return false;
     }
 }
@@ -321,7 +335,8 @@ int cg_model_state_name(int model, int i, char* buf, size_t len)
     if (i < 0 || i >= mi.ns || !buf || len == 0) return fail(CG_ERR_ARG, "bad state index %d", i);
     const char* nm = model == 0 ? kBONames[i] : model == 1 ? kTNNPNames[i]
                    : model == 2 ? kORdNames[i] : model == 3 ? kAPNames[i]
-                   : model == 4 ? kFDANames[i] : model == 5 ? kCRNNames[i] : kPYSNames[i];
+                   : model == 4 ? kFDANames[i] : model == 5 ? kCRNNames[i]
+                   : model == 6 ? kPYSNames[i] : kSTWNames[i];
     snprintf(buf, len, "%s", nm);
     return CG_OK;
 }
@@ -402,10 +417,9 @@ int cg_ionic_set_constant(cg_ionic* h, int k, double v)
 int cg_ionic_set_conductance_scale(cg_ionic* h, const double* s)
 {
     if (!h) return fail(CG_ERR_ARG, "NULL handle");
-    if (h->model != cg::MODEL_TNNP && h->model != cg::MODEL_ORD && h->model != cg::MODEL_CRN &&
-        h->model != cg::MODEL_PYS && s)
-        return fail(CG_ERR_STATE,
-                    "conductance scaling is defined for TNNP, ORd, Courtemanche and PerisYague_2022 only");
+    if ((h->model == cg::MODEL_BO || h->model == cg::MODEL_AP || h->model == cg::MODEL_FDA) && s)
+        return fail(CG_ERR_STATE, "conductance scaling is not defined for ionicModel %s",
+                    kModelKeys[h->model]);
     HIPCHK(hipStreamSynchronize(h->stream));
     if (!s) {
         if (h->d_gscale) hipFree(h->d_gscale);

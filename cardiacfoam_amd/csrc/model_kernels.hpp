@@ -353,6 +353,7 @@ template <class T> struct ModelLaunch {
     case MODEL_FDA: return ModelLaunch<T>::template FN<FdaManufactured>(__VA_ARGS__);      \
     case MODEL_CRN: return ModelLaunch<T>::template FN<Courtemanche>(__VA_ARGS__);         \
     case MODEL_PYS: return ModelLaunch<T>::template FN<PerisYague>(__VA_ARGS__);           \
+    case MODEL_STW: return ModelLaunch<T>::template FN<Stewart>(__VA_ARGS__);              \
     default: return hipErrorInvalidValue;                                                  \
     }
 

@@ -15,11 +15,12 @@
 #include "model_fda.hpp"
 #include "model_crn.hpp"
 #include "model_pys.hpp"
+#include "model_stw.hpp"
 
 namespace cg {
 
 enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3, MODEL_FDA = 4,
-               MODEL_CRN = 5, MODEL_PYS = 6 };
+               MODEL_CRN = 5, MODEL_PYS = 6, MODEL_STW = 7 };
 
 constexpr int MAX_PARAMS = 48;
 struct ModelParams {
@@ -232,6 +233,39 @@ template <> struct Ops<PerisYague> {
                              T istim, const PerisYague::VTerms<T>& q)
     {
         PerisYague::deriv(y, d, p, gs, solve_v, istim, q);
+    }
+};
+
+template <> struct Ops<Stewart> {
+    template <class T>
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = Stewart::vterms(v, p);
+        Stewart::step(y, dt, p, gs, false, T(0), q);
+        return Stewart::currents(v, y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = Stewart::vterms(y[0], p);
+        Stewart::step(y, dt, p, gs, true, istim, q);
+    }
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T* gs)
+    {
+        const auto q = Stewart::vterms(y[0], p);
+        return Stewart::currents(y[0], y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static Stewart::VTerms<T> vt(T v, const T* p)
+    {
+        return Stewart::vterms(v, p);
+    }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int, const T* gs, bool solve_v,
+                             T istim, const Stewart::VTerms<T>& q)
+    {
+        Stewart::deriv(y, d, p, gs, solve_v, istim, q);
     }
 };
 

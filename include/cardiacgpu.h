@@ -42,7 +42,7 @@ enum {
 /* model ids: the ionicModel runtime-selection keys */
 enum { CG_MODEL_BUENO_OROVIO = 0, CG_MODEL_TNNP = 1, CG_MODEL_ORD = 2, CG_MODEL_ALIEV_PANFILOV = 3,
        CG_MODEL_FDA_MANUFACTURED = 4, /* monodomainFDAManufactured (verification model) */
-       CG_MODEL_COURTEMANCHE = 5, CG_MODEL_PERIS_YAGUE_2022 = 6 };
+       CG_MODEL_COURTEMANCHE = 5, CG_MODEL_PERIS_YAGUE_2022 = 6, CG_MODEL_STEWART = 7 };
 /* tissue flags of ionicSelector::tissueFlag (ionicSelector.C:60-66); the manufactured
  * model takes its `dimension` word instead ("1D", "2D", "3D") */
 enum { CG_TISSUE_EPI = 1, CG_TISSUE_MCELL = 2, CG_TISSUE_ENDO = 3, CG_TISSUE_MYOCYTE = 4,
@@ -63,7 +63,7 @@ const char* cg_version(void);
 typedef struct cg_ionic cg_ionic;
 
 /* Model lookup by ionicModel key ("BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
- * "monodomainFDAManufactured", "Courtemanche", "PerisYague_2022") and tissue word ("epicardialCells", "mCells",
+ * "monodomainFDAManufactured", "Courtemanche", "PerisYague_2022", "Stewart") and tissue word ("epicardialCells", "mCells",
  * "endocardialCells", "myocyte"; "1D", "2D", "3D" for the manufactured model).
  * Replaces ionicModel::New's table lookup (ionicModel.C:76-94) and
  * ionicSelector::tissueFlag (ionicSelector.C:60-66).  Return -1 when unknown. */
@@ -76,7 +76,8 @@ int cg_model_state_name(int model, int i, char* buf, size_t len);
 
 /* Replaces <Model>::<Model>(dict, nIntegrationPoints, initialDeltaT, solveVm)
  * (ionicModel.H:184-196; BuenoOrovio.C:45-81, TNNP.C:44-78, ORd.C:45-80,
- * AlievPanfilov.C:45-80, Courtemanche.C:43-79, PerisYague.C:44-80): allocates SoA states for n_cells integration points
+ * AlievPanfilov.C:45-80, Courtemanche.C:44-80, PerisYague.C:44-80,
+ * Stewart.C:44-81): allocates SoA states for n_cells integration points
  * initialised from the model's initConsts.  solve_vm = 1 is the single-cell mode
  * (singleCellSolver.C:48-57). */
 int cg_ionic_create(int model, int tissue, int64_t n_cells, int solve_vm, int precision,
@@ -91,8 +92,8 @@ int cg_ionic_nstates(const cg_ionic* h);
  * CONSTANTS_ after initConsts. */
 int cg_ionic_set_constant(cg_ionic* h, int k, double value);
 /* Per-cell multipliers of four conductances, SoA [4][n_cells] (NULL clears):
- * TNNP g_Na g_Kr g_CaL g_Ks; ORd GNa GKr PCa GKs; Courtemanche and PerisYague_2022
- * gNa gKr gCaL gKs.
+ * TNNP g_Na g_Kr g_CaL g_Ks; ORd GNa GKr PCa GKs; Courtemanche, PerisYague_2022 and
+ * Stewart gNa gKr gCaL gKs.
  * No reference counterpart
  * (constants are per model instance there, TNNP.H:61). */
 int cg_ionic_set_conductance_scale(cg_ionic* h, const double* scale4xn);

@@ -1025,6 +1025,171 @@ static void pys_eval(double t, const double* c, const double* y, int solve_vm,
     *iion = Iion; *istim = Ist;
 }
 
+/* ================================================================= Stewart
+ * Stewart et al. 2009 human Purkinje-fibre model, 20 states: constants and initial state
+ * of Stewart_2009.H:262-377, right-hand side of :380-530, state order of its
+ * STATES_NAMES table.  Tissue "myocyte" only (Stewart.C:93-96).  In tissue mode the
+ * reference leaves RATES[V] unwritten (:524-528); here dV/dt = 0 there, as for the other
+ * models.  R_prime (dR/dt = k4 - (k2 Ca_ss + k4) R) is integrated as a linear gate.
+ * Parity pins (tests/test_oracle_models.py): charge conservation over the cytosolic,
+ * subspace and SR compartments; the GHK quotient of ICaL continuous through V = 15 mV
+ * from both sides; Rush-Larsen and RKF45 agreeing on the automatic rhythm from the
+ * reference's initial state.  Gate kinetics: parity unpinned beyond device-vs-oracle
+ * agreement (no published trace for the reference's constants). */
+enum { SW_V, SW_y, SW_Xr1, SW_Xr2, SW_Xs, SW_m, SW_h, SW_j, SW_d, SW_f, SW_f2, SW_fCass,
+       SW_s, SW_r, SW_Cai, SW_CaSR, SW_Cass, SW_Rp, SW_Nai, SW_Ki, SW_NS };
+enum { SC_KpCa, SC_gpCa, SC_gCaL, SC_gbca, SC_Bufc, SC_Bufsr, SC_Bufss, SC_Cao, SC_EC,
+       SC_Kbufc, SC_Kbufsr, SC_Kbufss, SC_Kup, SC_Vleak, SC_Vrel, SC_Vsr, SC_Vss, SC_Vxfer,
+       SC_Vmaxup, SC_k1p, SC_k2p, SC_k3, SC_k4, SC_maxsr, SC_minsr, SC_gNa, SC_gfK, SC_gfNa,
+       SC_gK1, SC_Cm, SC_F, SC_R, SC_T, SC_Vc, SC_Ko, SC_gpK, SC_gKr, SC_Pkna, SC_gKs,
+       SC_gbna, SC_KNaCa, SC_Ksat, SC_KmCa, SC_KmNai, SC_alpha, SC_gam, SC_Nao, SC_KmNa,
+       SC_Kmk, SC_PNaK, SC_gsus, SC_gto, SC_NC };
+
+static void stw_init(double* c, double* y)
+{
+    c[SC_KpCa] = 0.0005; c[SC_gpCa] = 0.1238; c[SC_gCaL] = 3.98e-05; c[SC_gbca] = 0.000592;
+    c[SC_Bufc] = 0.2; c[SC_Bufsr] = 10.0; c[SC_Bufss] = 0.4; c[SC_Cao] = 2.0; c[SC_EC] = 1.5;
+    c[SC_Kbufc] = 0.001; c[SC_Kbufsr] = 0.3; c[SC_Kbufss] = 0.00025; c[SC_Kup] = 0.00025;
+    c[SC_Vleak] = 0.00036; c[SC_Vrel] = 0.102; c[SC_Vsr] = 0.001094; c[SC_Vss] = 5.468e-05;
+    c[SC_Vxfer] = 0.0038; c[SC_Vmaxup] = 0.006375; c[SC_k1p] = 0.15; c[SC_k2p] = 0.045;
+    c[SC_k3] = 0.06; c[SC_k4] = 0.005; c[SC_maxsr] = 2.5; c[SC_minsr] = 1.0;
+    c[SC_gNa] = 130.5744; c[SC_gfK] = 0.0234346; c[SC_gfNa] = 0.0145654; c[SC_gK1] = 0.065;
+    c[SC_Cm] = 0.185; c[SC_F] = 96485.3415; c[SC_R] = 8314.472; c[SC_T] = 310.0;
+    c[SC_Vc] = 0.016404; c[SC_Ko] = 5.4; c[SC_gpK] = 0.0146; c[SC_gKr] = 0.0918;
+    c[SC_Pkna] = 0.03; c[SC_gKs] = 0.2352; c[SC_gbna] = 0.00029; c[SC_KNaCa] = 1000.0;
+    c[SC_Ksat] = 0.1; c[SC_KmCa] = 1.38; c[SC_KmNai] = 87.5; c[SC_alpha] = 2.5;
+    c[SC_gam] = 0.35; c[SC_Nao] = 140.0; c[SC_KmNa] = 40.0; c[SC_Kmk] = 1.0;
+    c[SC_PNaK] = 2.724; c[SC_gsus] = 0.0227; c[SC_gto] = 0.08184;
+    static const double y0[SW_NS] = {
+        -69.1370441635924067, 4.57562667986601973e-02, 5.50281999719088016e-03,
+        0.313213286437995009, 9.53708522974788982e-03, 4.17391656294996971e-02,
+        0.190678733735144990, 0.238219836154029002, 2.87906256206415015e-04,
+        0.989328560287986991, 0.995474890442184956, 0.999955429598213041,
+        0.963861017995010005, 1.03618091196912006e-03, 1.01878186157052002e-04,
+        3.10836886659417022, 4.46818714055411022e-04, 0.991580051907845039,
+        8.80420286531673035, 136.781894160227012 };
+    memcpy(y, y0, sizeof y0);
+}
+
+static void stw_eval(double t, const double* c, const double* y, int solve_vm,
+                     const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                     double* iion, double* istim)
+{
+    const double V = y[SW_V];
+    const double RTF = c[SC_R] * c[SC_T] / c[SC_F], FRT = c[SC_F] / (c[SC_R] * c[SC_T]);
+    for (int i = 0; i < SW_NS; ++i) g[i] = 0;
+#define SW_GATE(idx, inf, tau) do { r[idx] = ((inf) - y[idx]) / (tau); g[idx] = 1; \
+                                    xi[idx] = (inf); ta[idx] = (tau); } while (0)
+    /* ICaL gates d, f, f2, fCass */
+    {
+        const double ad = 1.4 / (1.0 + exp((-35.0 - V) / 13.0)) + 0.25;
+        const double bd = 1.4 / (1.0 + exp((V + 5.0) / 5.0));
+        const double gd = 1.0 / (1.0 + exp((50.0 - V) / 20.0));
+        SW_GATE(SW_d, 1.0 / (1.0 + exp((-8.0 - V) / 7.5)), ad * bd + gd);
+        SW_GATE(SW_f2, 0.67 / (1.0 + exp((V + 35.0) / 7.0)) + 0.33,
+                562.0 * exp(-pow(V + 27.0, 2.0) / 240.0) + 31.0 / (1.0 + exp((25.0 - V) / 10.0)) +
+                    80.0 / (1.0 + exp((V + 30.0) / 10.0)));
+        const double cq = 1.0 + pow(y[SW_Cass] / 0.05, 2.0);
+        SW_GATE(SW_fCass, 0.6 / cq + 0.4, 80.0 / cq + 2.0);
+        SW_GATE(SW_f, 1.0 / (1.0 + exp((V + 20.0) / 7.0)),
+                1102.5 * exp(-pow(V + 27.0, 2.0) / 225.0) + 200.0 / (1.0 + exp((13.0 - V) / 10.0)) +
+                    180.0 / (1.0 + exp((V + 30.0) / 10.0)) + 20.0);
+    }
+    /* INa gates m, h, j (TNNP form) */
+    {
+        const int lo = V < -40.0;
+        const double ah = lo ? 0.057 * exp(-(V + 80.0) / 6.8) : 0.0;
+        const double bh = lo ? 2.7 * exp(0.079 * V) + 310000.0 * exp(0.3485 * V)
+                             : 0.77 / (0.13 * (1.0 + exp((V + 10.66) / -11.1)));
+        const double hinf = 1.0 / pow(1.0 + exp((V + 71.55) / 7.43), 2.0);
+        SW_GATE(SW_h, hinf, 1.0 / (ah + bh));
+        const double aj = lo ? (-25428.0 * exp(0.2444 * V) - 6.948e-06 * exp(-0.04391 * V)) *
+                                   (V + 37.78) / (1.0 + exp(0.311 * (V + 79.23)))
+                             : 0.0;
+        const double bj = lo ? 0.02424 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)))
+                             : 0.6 * exp(0.057 * V) / (1.0 + exp(-0.1 * (V + 32.0)));
+        SW_GATE(SW_j, hinf, 1.0 / (aj + bj));
+        const double am = 1.0 / (1.0 + exp((-60.0 - V) / 5.0));
+        const double bm = 0.1 / (1.0 + exp((V + 35.0) / 5.0)) + 0.1 / (1.0 + exp((V - 50.0) / 200.0));
+        SW_GATE(SW_m, 1.0 / pow(1.0 + exp((-56.86 - V) / 9.03), 2.0), am * bm);
+    }
+    /* If gate y, IKr Xr1 Xr2, IKs Xs, Ito r s */
+    SW_GATE(SW_y, 1.0 / (1.0 + exp((V + 80.6) / 6.8)),
+            4000.0 / (exp(-2.9 - 0.04 * V) + exp(3.6 + 0.11 * V)));
+    SW_GATE(SW_Xr1, 1.0 / (1.0 + exp((-26.0 - V) / 7.0)),
+            450.0 / (1.0 + exp((-45.0 - V) / 10.0)) * (6.0 / (1.0 + exp((V + 30.0) / 11.5))));
+    SW_GATE(SW_Xr2, 1.0 / (1.0 + exp((V + 88.0) / 24.0)),
+            3.0 / (1.0 + exp((-60.0 - V) / 20.0)) * (1.12 / (1.0 + exp((V - 60.0) / 20.0))));
+    SW_GATE(SW_Xs, 1.0 / (1.0 + exp((-5.0 - V) / 14.0)),
+            1400.0 / sqrt(1.0 + exp((5.0 - V) / 6.0)) * (1.0 / (1.0 + exp((V - 35.0) / 15.0))) + 80.0);
+    SW_GATE(SW_r, 1.0 / (1.0 + exp((20.0 - V) / 13.0)),
+            10.45 * exp(-pow(V + 40.0, 2.0) / 1800.0) + 7.3);
+    SW_GATE(SW_s, 1.0 / (1.0 + exp((V + 27.0) / 13.0)),
+            85.0 * exp(-pow(V + 25.0, 2.0) / 320.0) + 5.0 / (1.0 + exp((V - 40.0) / 5.0)) + 42.0);
+
+    /* calcium handling: subspace release through the RyR state R' */
+    const double Cai = y[SW_Cai], CaSR = y[SW_CaSR], Cass = y[SW_Cass];
+    const double bufc = 1.0 / (1.0 + c[SC_Bufc] * c[SC_Kbufc] / pow(Cai + c[SC_Kbufc], 2.0));
+    const double bufsr = 1.0 / (1.0 + c[SC_Bufsr] * c[SC_Kbufsr] / pow(CaSR + c[SC_Kbufsr], 2.0));
+    const double bufss = 1.0 / (1.0 + c[SC_Bufss] * c[SC_Kbufss] / pow(Cass + c[SC_Kbufss], 2.0));
+    const double Ileak = c[SC_Vleak] * (CaSR - Cai);
+    const double Iup = c[SC_Vmaxup] / (1.0 + pow(c[SC_Kup], 2.0) / pow(Cai, 2.0));
+    const double Ixfer = c[SC_Vxfer] * (Cass - Cai);
+    const double kcasr = c[SC_maxsr] - (c[SC_maxsr] - c[SC_minsr]) / (1.0 + pow(c[SC_EC] / CaSR, 2.0));
+    const double k1 = c[SC_k1p] / kcasr, k2 = c[SC_k2p] * kcasr;
+    const double O = k1 * pow(Cass, 2.0) * y[SW_Rp] / (c[SC_k3] + k1 * pow(Cass, 2.0));
+    /* dR'/dt = k4 - (k2 Ca_ss + k4) R' */
+    {
+        const double b = k2 * Cass + c[SC_k4];
+        SW_GATE(SW_Rp, c[SC_k4] / b, 1.0 / b);
+    }
+    const double Irel = c[SC_Vrel] * O * (CaSR - Cass);
+    r[SW_CaSR] = bufsr * (Iup - (Irel + Ileak));
+
+    /* currents */
+    const double EK = RTF * log(c[SC_Ko] / y[SW_Ki]);
+    const double ENa = RTF * log(c[SC_Nao] / y[SW_Nai]);
+    const double ECa = 0.5 * RTF * log(c[SC_Cao] / Cai);
+    const double EKs = RTF * log((c[SC_Ko] + c[SC_Pkna] * c[SC_Nao]) / (y[SW_Ki] + c[SC_Pkna] * y[SW_Nai]));
+    const double INaK = c[SC_PNaK] * c[SC_Ko] / (c[SC_Ko] + c[SC_Kmk]) * y[SW_Nai] /
+                        (y[SW_Nai] + c[SC_KmNa]) /
+                        (1.0 + 0.1245 * exp(-0.1 * V * FRT) + 0.0353 * exp(-V * FRT));
+    const double Isus = c[SC_gsus] * (1.0 / (1.0 + exp((5.0 - V) / 17.0))) * (V - EK);
+    const double Ito = c[SC_gto] * y[SW_r] * y[SW_s] * (V - EK);
+    const double e2 = exp(2.0 * (V - 15.0) * FRT);
+    const double ICaL = c[SC_gCaL] * y[SW_d] * y[SW_f] * y[SW_f2] * y[SW_fCass] * 4.0 * (V - 15.0) *
+                        pow(c[SC_F], 2.0) / (c[SC_R] * c[SC_T]) * (0.25 * Cass * e2 - c[SC_Cao]) / (e2 - 1.0);
+    const double IbCa = c[SC_gbca] * (V - ECa);
+    const double IfK = y[SW_y] * c[SC_gfK] * (V - EK);
+    const double IfNa = y[SW_y] * c[SC_gfNa] * (V - ENa);
+    const double IK1 = c[SC_gK1] * (1.0 / (1.0 + exp(0.1 * (V + 75.44)))) * (V - 8.0 - EK);
+    const double IpK = c[SC_gpK] * (V - EK) / (1.0 + exp((25.0 - V) / 5.98));
+    const double IKr = c[SC_gKr] * sqrt(c[SC_Ko] / 5.4) * y[SW_Xr1] * y[SW_Xr2] * (V - EK);
+    const double IKs = c[SC_gKs] * pow(y[SW_Xs], 2.0) * (V - EKs);
+    const double IpCa = c[SC_gpCa] * Cai / (Cai + c[SC_KpCa]);
+    const double INaCa = c[SC_KNaCa] *
+                         (exp(c[SC_gam] * V * FRT) * pow(y[SW_Nai], 3.0) * c[SC_Cao] -
+                          exp((c[SC_gam] - 1.0) * V * FRT) * pow(c[SC_Nao], 3.0) * Cai * c[SC_alpha]) /
+                         ((pow(c[SC_KmNai], 3.0) + pow(c[SC_Nao], 3.0)) * (c[SC_KmCa] + c[SC_Cao]) *
+                          (1.0 + c[SC_Ksat] * exp((c[SC_gam] - 1.0) * V * FRT)));
+    const double INa = c[SC_gNa] * pow(y[SW_m], 3.0) * y[SW_h] * y[SW_j] * (V - ENa);
+    const double IbNa = c[SC_gbna] * (V - ENa);
+#undef SW_GATE
+
+    r[SW_Cai] = bufc * ((Ileak - Iup) * c[SC_Vsr] / c[SC_Vc] + Ixfer -
+                        (IbCa + IpCa - 2.0 * INaCa) * c[SC_Cm] / (2.0 * c[SC_Vc] * c[SC_F]));
+    r[SW_Cass] = bufss * (-ICaL * c[SC_Cm] / (2.0 * c[SC_Vss] * c[SC_F]) +
+                          Irel * c[SC_Vsr] / c[SC_Vss] - Ixfer * c[SC_Vc] / c[SC_Vss]);
+    const double cvf = c[SC_Cm] / (c[SC_Vc] * c[SC_F]);
+    r[SW_Ki] = -(IK1 + Ito + IfK + Isus + IKr + IKs + IpK - 2.0 * INaK) * cvf;
+    r[SW_Nai] = -(INa + IbNa + IfNa + 3.0 * INaK + 3.0 * INaCa) * cvf;
+    const double Iion = IK1 + Ito + Isus + IKr + IKs + ICaL + INaK + INa + IbNa + INaCa + IbCa +
+                        IpK + IpCa + IfNa + IfK;
+    const double Ist = orc_stimulus(t, st);
+    r[SW_V] = solve_vm ? -Iion - Ist : 0.0;
+    *iion = Iion; *istim = Ist;
+}
+
 /* ============================================================== dispatch */
 int orc_model_nstates(int model)
 {
@@ -1036,6 +1201,7 @@ int orc_model_nstates(int model)
     case ORC_FDA: return 4;
     case ORC_CRN: return CR_NS;
     case ORC_PYS: return PY_NS;
+    case ORC_STW: return SW_NS;
     default: return -1;
     }
 }
@@ -1065,6 +1231,9 @@ int orc_model_init(int model, int tissue, double* c, double* y)
     case ORC_PYS:
         if (tissue != ORC_MYOCYTE) return -1;
         pys_init(c, y); return 0;
+    case ORC_STW:
+        if (tissue != ORC_MYOCYTE) return -1;
+        stw_init(c, y); return 0;
     default: return -1;
     }
 }
@@ -1091,6 +1260,11 @@ int orc_model_const_index(int model, const char* name)
         if (!strcmp(name, "gKr")) return PC_gKr;
         if (!strcmp(name, "gCaL")) return PC_gCaL;
         if (!strcmp(name, "gKs")) return PC_gKs;
+    } else if (model == ORC_STW) {
+        if (!strcmp(name, "g_Na")) return SC_gNa;
+        if (!strcmp(name, "g_Kr")) return SC_gKr;
+        if (!strcmp(name, "g_CaL")) return SC_gCaL;
+        if (!strcmp(name, "g_Ks")) return SC_gKs;
     } else if (model == ORC_BO) {
         if (!strcmp(name, "tauFi")) return BO_tfi;
         if (!strcmp(name, "tauSi")) return BO_tsi;
@@ -1110,6 +1284,7 @@ void orc_model_eval(int model, int tissue, double t, const double* c, const doub
     case ORC_FDA: fda_eval(c, y, r, g, iion); *istim = 0.0; break;
     case ORC_CRN: crn_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_PYS: pys_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_STW: stw_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     default: break;
     }
 }

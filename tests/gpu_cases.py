@@ -11,6 +11,7 @@ MODELS = {  # name -> (oracle id, tissue flag, library key, tissue word, stimulu
     "AP": (3, 4, "AlievPanfilov", "myocyte", 0.5),
     "CRN": (5, 4, "Courtemanche", "myocyte", 60.0),
     "PYS": (6, 4, "PerisYague_2022", "myocyte", 40.0),
+    "STW": (7, 4, "Stewart", "myocyte", 40.0),
 }
 
 

@@ -19,12 +19,13 @@ def test_model_and_tissue_lookup():
     assert [L.cg_model_id(k) for k in (b"BuenoOrovio", b"TNNP", b"ORd", b"AlievPanfilov")] == [0, 1, 2, 3]
     assert L.cg_model_id(b"Courtemanche") == 5
     assert L.cg_model_id(b"PerisYague_2022") == 6
+    assert L.cg_model_id(b"Stewart") == 7
     assert L.cg_model_id(b"ToRORd_dynCl") == -1
     assert [L.cg_tissue_flag(t) for t in (b"epicardialCells", b"mCells", b"endocardialCells",
                                           b"myocyte")] == [1, 2, 3, 4]
     assert L.cg_tissue_flag(b"purkinje") == -1
-    assert [L.cg_model_nstates(m) for m in range(7)] == [4, 17, 41, 2, 4, 21, 22]
-    assert L.cg_model_nstates(7) == -1
+    assert [L.cg_model_nstates(m) for m in range(8)] == [4, 17, 41, 2, 4, 21, 22, 20]
+    assert L.cg_model_nstates(8) == -1
     buf = C.create_string_buffer(32)
     names = []
     for i in range(4):

@@ -45,7 +45,8 @@ def _oracle_cell(orc, m, tis, p, n_steps, scale=None):
     c, y = orc.model_init(m, tis)
     if scale is not None:
         names = {1: ["g_Na", "g_Kr", "g_CaL", "g_Ks"], 2: ["GNa", "GKr", "PCa", "GKs"],
-                 5: ["gNa", "gKr", "gCaL", "gKs"], 6: ["gNa", "gKr", "gCaL", "gKs"]}[m]
+                 5: ["gNa", "gKr", "gCaL", "gKs"], 6: ["gNa", "gKr", "gCaL", "gKs"],
+                 7: ["g_Na", "g_Kr", "g_CaL", "g_Ks"]}[m]
         for k, nm in enumerate(names):
             c[orc.const_index(m, nm)] *= scale[k]
     st = orc.make_stim(start=p["stim_start"], period_s1=p["stim_period_S1"],
@@ -83,7 +84,8 @@ def _setup(gpu, name, n, precision, seed, s2=False, scales=True):
 
 
 @pytest.mark.parametrize("name,n,s2", [("TNNP_epi", 6, False), ("ORd", 3, True),
-                                       ("CRN", 3, True), ("PYS", 3, False)])
+                                       ("CRN", 3, True), ("PYS", 3, False),
+                                       ("STW", 3, False)])
 def test_batch_beats_fp64_vs_oracle(gpu, orc, name, n, s2):
     """FP64 batch: every beat record and the final states equal the oracle's trace and its
     offline beat detection on the same protocol."""

@@ -145,7 +145,7 @@ def test_aliev_panfilov_cubic(orc):
 
 def test_tissue_support_matches_reference(orc):
     # BuenoOrovio.C:92-96, TNNP.C:90-94, ORd.C:93-96, AlievPanfilov.C:92-95,
-    # Courtemanche.C:92-95, PerisYague.C:92-95
+    # Courtemanche.C:92-95, PerisYague.C:92-95, Stewart.C:93-96
     for tissue in (1, 2, 3):
         orc.model_init(orc.BO, tissue)
         orc.model_init(orc.TNNP, tissue)
@@ -155,9 +155,12 @@ def test_tissue_support_matches_reference(orc):
             orc.model_init(orc.CRN, tissue)
         with pytest.raises(ValueError):
             orc.model_init(orc.PYS, tissue)
+        with pytest.raises(ValueError):
+            orc.model_init(orc.STW, tissue)
     orc.model_init(orc.ORD, 4)
     orc.model_init(orc.CRN, 4)
     orc.model_init(orc.PYS, 4)
+    orc.model_init(orc.STW, 4)
     orc.model_init(orc.AP, 4)
     with pytest.raises(ValueError):
         orc.model_init(orc.BO, 4)
@@ -305,3 +308,64 @@ def test_pys_d_gate_guard_value(orc):
         yy[0], yy[15] = v, 0.25
         r = orc.model_eval(orc.PYS, orc.MYOCYTE, 0.0, c, yy, solve_vm=False)["rates"][15]
         assert r == pytest.approx((dinf - 0.25) / tau, rel=1e-4)
+
+
+def test_stw_charge_conservation(orc):
+    # Stewart_2009.H:264-358 volumes (uL), capacitance (uF) and buffer constants
+    F, Cm, Vc, Vsr, Vss = 96485.3415, 0.185, 0.016404, 0.001094, 5.468e-05
+    Bufc, Kbufc, Bufsr, Kbufsr, Bufss, Kbufss = 0.2, 0.001, 10.0, 0.3, 0.4, 0.00025
+    c, states = _states_along_ap(orc, orc.STW, orc.MYOCYTE, 40.0)
+    st = orc.make_stim(start=0.0, period_s1=1000.0, duration=2.0, amplitude=40.0, nstim1=1)
+    Cai, CaSR, Cass, Nai, Ki = 14, 15, 16, 18, 19
+    for y in states:
+        for t, stim in ((500.0, None), (1.0, st)):
+            r = orc.model_eval(orc.STW, orc.MYOCYTE, t, c, y, solve_vm=True, stim=stim)
+            d = r["rates"]
+            dCai_t = d[Cai] * (1 + Bufc * Kbufc / (y[Cai] + Kbufc) ** 2)
+            dCaSR_t = d[CaSR] * (1 + Bufsr * Kbufsr / (y[CaSR] + Kbufsr) ** 2)
+            dCass_t = d[Cass] * (1 + Bufss * Kbufss / (y[Cass] + Kbufss) ** 2)
+            lhs = F * (Vc * (d[Nai] + d[Ki] + 2 * dCai_t) + 2 * Vsr * dCaSR_t + 2 * Vss * dCass_t)
+            rhs = -Cm * r["iion"]
+            assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-9 * Cm)
+            assert d[0] == pytest.approx(-(r["iion"] + r["istim"]), rel=1e-14)
+
+
+def test_stw_limit_cycle_returns_to_reference_state(orc):
+    """The Purkinje cell fires on its own.  The reference's initial state (given to 17
+    digits, Stewart_2009.H:358-377) is a point of that limit cycle: one cycle of the
+    unstimulated cell brings all 20 states back to it."""
+    import ctypes as C
+    c, y0 = orc.model_init(orc.STW, orc.MYOCYTE)
+    st = orc.make_stim(start=0.0, period_s1=1000.0, duration=0.0, amplitude=0.0, nstim1=0)
+    yy = np.zeros(41)
+    yy[:20] = y0
+    step = C.c_double(1e-3)
+    t, dt, cross = 0.0, 0.05, []
+    while len(cross) < 2 and t < 3000.0:
+        prev = yy.copy()
+        assert orc.lib().orc_rkf45_solve(orc.STW, orc.MYOCYTE, c, yy, 1, C.byref(st), t, t + dt,
+                                         C.byref(step), 1e-12, 1e-10, 100000) >= 0
+        t += dt
+        # the slow diastolic rise through the initial potential
+        if t > 200.0 and prev[0] < y0[0] <= yy[0] and yy[0] - prev[0] < 0.1 * dt:
+            a = (y0[0] - prev[0]) / (yy[0] - prev[0])
+            cross.append((t - dt + a * dt, prev[:20] + a * (yy[:20] - prev[:20])))
+    assert len(cross) == 2
+    period = cross[1][0] - cross[0][0]
+    assert 1200.0 < period < 1250.0
+    np.testing.assert_allclose(cross[0][1], y0, rtol=1e-6, atol=1e-12)
+
+
+def test_stw_ical_ghk_limit(orc):
+    """ICaL's GHK quotient (V - 15)/(exp(2 (V - 15) F/RT) - 1) is 0/0 at V = 15 mV exactly
+    (Stewart_2009.H:467) and continuous through it."""
+    c, y = orc.model_init(orc.STW, orc.MYOCYTE)
+
+    def iion(v):
+        yy = np.array(y)
+        yy[0], yy[8] = v, 0.5  # d gate open
+        return orc.model_eval(orc.STW, orc.MYOCYTE, 0.0, c, yy, solve_vm=False)["iion"]
+
+    assert np.isnan(iion(15.0))
+    left, right = 2 * iion(15.0 - 1e-4) - iion(15.0 - 2e-4), 2 * iion(15.0 + 1e-4) - iion(15.0 + 2e-4)
+    assert left == pytest.approx(right, rel=1e-7)
