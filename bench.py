@@ -37,7 +37,10 @@ WORKLOADS = {
 }
 
 # algorithmic bytes per cell (row) of one launch, by kernel and precision
-NSTATES = {"ORd": 41, "BuenoOrovio": 4, "TNNP": 17, "AlievPanfilov": 2, "Courtemanche": 21}
+NSTATES = {"ORd": 41, "BuenoOrovio": 4, "TNNP": 17, "AlievPanfilov": 2, "Courtemanche": 21,
+           "PerisYague_2022": 22, "Stewart": 20}
+# the ionic models run on the workload's mesh beside its own model
+OTHER_MODELS = ("Courtemanche", "PerisYague_2022", "Stewart")
 
 
 def algo_bytes_per_cell(kernel, prec, model, algorithm, od_planes=1):
@@ -817,6 +820,8 @@ def main():
                     help="run one precision only (profiling runs; no error gate)")
     ap.add_argument("--ecg-only", action="store_true",
                     help="only the pseudo-ECG leg on the workload's tissue after 110 FP64 steps")
+    ap.add_argument("--models-only", action="store_true",
+                    help="only the other ionic models on the workload's mesh (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 10 times (HBM counter scale)")
     args = ap.parse_args()
@@ -848,6 +853,12 @@ def main():
         r = run_precision(cf, torch, w, sigma, "fp64", 10, 100, 0)
         print(json.dumps({"workload": args.workload, "cells": n,
                           "pseudo_ecg": measure_pseudo_ecg(r["dom"], n, bool(w["rotating"]))}))
+        return
+    if args.models_only:
+        print(json.dumps({"workload": args.workload, "cells": n,
+                          "models_2M": {m: run_model_leg(cf, torch, w, sigma, m, steps=args.steps,
+                                                         warmup=args.warmup, prof_steps=0)
+                                        for m in OTHER_MODELS}}))
         return
     if args.only != "both":
         r = run_precision(cf, torch, w, sigma, args.only, args.steps, args.warmup,
@@ -907,7 +918,7 @@ def main():
     c4 = run_adjoint_c4(cf, torch) if not args.no_batch else None
     ref_sem = run_reference_semantics(cf, args.workload, w, sigma, 100, not args.no_cpu) \
         if w["model"] == "ORd" and not args.no_batch else None
-    crn = run_model_leg(cf, torch, w, sigma, "Courtemanche") \
+    models = {m: run_model_leg(cf, torch, w, sigma, m) for m in OTHER_MODELS} \
         if w["algorithm"] == "implicit" and not args.no_batch else None
 
     io = run_field_io({"Vm": V64, "activationTime": a64})
@@ -947,7 +958,7 @@ def main():
         "batch_c2": c2,
         "adjoint_c4": c4,
         "rkf45": ref_sem,
-        "courtemanche_2M": crn,
+        "models_2M": models,
         "field_io": io,
         "pseudo_ecg": ecg,
         "general_mesh": general,
