@@ -340,7 +340,7 @@ def _canonical(name):
 
 
 # Vm of the trace from the model's first state (the models' ioVmTransform:
-# BuenoOrovio.H:78-81, AlievPanfilov.H:78-81; TNNP and ORd store mV)
+# BuenoOrovio.H:78-81, AlievPanfilov.H:78-81; the other models store mV)
 VM_TRANSFORM = {"BuenoOrovio": (85.7, -84.0), "AlievPanfilov": (100.0, -80.0)}
 
 

@@ -143,7 +143,7 @@ def test_fields_and_conductivity_from_case(gpu, orc, tmp_path):
     assert np.all(src[inside] == 5e4) and np.all(src[~inside] == 0.0)
 
 
-@pytest.mark.parametrize("name", ["TNNP_epi", "BO_epi"])
+@pytest.mark.parametrize("name", ["TNNP_epi", "BO_epi", "CRN", "STW"])
 def test_trace_rows_states_and_rates(gpu, orc, name):
     """cg_ionic_run_cells_trace (Rush-Larsen): the traced cell's last row equals its states
     after the run, and every recorded rate row equals the oracle's right-hand side of the
