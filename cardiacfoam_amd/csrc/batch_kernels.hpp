@@ -215,8 +215,8 @@ void k_cells_protocol(T* __restrict__ st, KParams<T> P, const T* __restrict__ gs
 #pragma unroll
         for (int k = 0; k < 4; ++k) gs[k] = gsc[k * n + cell];
     }
-    T y[M::NS];
-    load_states(y, st, n, cell, 0);
+    T y[M::NS], sw[M::NS];
+    load_model_states<M>(y, sw, st, n, cell, 0);
     const Stimulus stim = a.prot ? a.prot[cell] : a.stim;
     Detector<T> det{a, lane, cell, (T*)a.ring};
     if (a.det) det.load();
@@ -224,7 +224,7 @@ void k_cells_protocol(T* __restrict__ st, KParams<T> P, const T* __restrict__ gs
         Ops<M>::cell_step(y, T(a.dt), T(stimulus_current((double)s * a.dt, stim)), p, tissue, gs);
         if (a.det) det.push(s, (double)y[0] * a.vscale - a.voff);
     }
-    store_states(y, st, n, cell, 0);
+    store_model_full<M>(y, st, n, cell, 0);
     if (a.det) {
         if (s1 == len) det.flush(len);
         det.store();

@@ -130,6 +130,22 @@ bool model_host_init(int model, int tissue, double* p, double* y)
     }
 }
 
+// reference of state q in the FP32 offset storage (StateRef in models.hpp; 0 = plain),
+// rounded to FP32 as the kernels use it
+double state_ref(int model, int q)
+{
+    double r = 0.0;
+    switch (model) {
+    case cg::MODEL_TNNP: r = cg::StateRef<cg::TNNP>::at(q); break;
+    case cg::MODEL_ORD: r = cg::StateRef<cg::ORd>::at(q); break;
+    case cg::MODEL_CRN: r = cg::StateRef<cg::Courtemanche>::at(q); break;
+    case cg::MODEL_PYS: r = cg::StateRef<cg::PerisYague>::at(q); break;
+    case cg::MODEL_STW: r = cg::StateRef<cg::Stewart>::at(q); break;
+    default: break;
+    }
+    return (double)(float)r;
+}
+
 // model time of a PDE time in seconds, as each wrapper's solveODE scales it
 // (BuenoOrovio.C:107-108, AlievPanfilov.C:107-108)
 double model_time(int model, double t_s)
@@ -377,8 +393,10 @@ int cg_ionic_create(int model, int tissue, int64_t n, int solve_vm, int precisio
             double* q = (double*)plane.data();
             for (int64_t i = 0; i < n; ++i) q[i] = y0[k];
         } else {
+            // FP32 storage: offsets from the state's reference
             float* q = (float*)plane.data();
-            for (int64_t i = 0; i < n; ++i) q[i] = (float)y0[k];
+            const float w = (float)(y0[k] - state_ref(model, k));
+            for (int64_t i = 0; i < n; ++i) q[i] = w;
         }
         hipMemcpy((char*)h->d_states + h->elem * (size_t)k * n, plane.data(), plane.size(),
                   hipMemcpyHostToDevice);
@@ -845,12 +863,21 @@ static int states_io(cg_ionic* h, int first, int nv, double* host, const double*
         if (host) HIPCHK(hipMemcpy(host, dsrc, sizeof(double) * cnt, hipMemcpyDeviceToHost));
         else HIPCHK(hipMemcpy(dsrc, hin, sizeof(double) * cnt, hipMemcpyHostToDevice));
     } else {
+        // FP32 storage holds offsets from each state's reference
         std::vector<float> f(cnt);
         if (host) {
             HIPCHK(hipMemcpy(f.data(), dsrc, sizeof(float) * cnt, hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < cnt; ++i) host[i] = f[i];
+            for (int v = 0; v < nv; ++v) {
+                const double ref = state_ref(h->model, first + v);
+                for (int64_t i = 0; i < h->n; ++i)
+                    host[v * h->n + i] = ref + (double)f[v * h->n + i];
+            }
         } else {
-            for (size_t i = 0; i < cnt; ++i) f[i] = (float)hin[i];
+            for (int v = 0; v < nv; ++v) {
+                const double ref = state_ref(h->model, first + v);
+                for (int64_t i = 0; i < h->n; ++i)
+                    f[v * h->n + i] = (float)(hin[v * h->n + i] - ref);
+            }
             HIPCHK(hipMemcpy(dsrc, f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice));
         }
     }

@@ -47,8 +47,22 @@ template <class T> CG_HD T sq(T x) { return x * x; }
 template <class T> CG_HD T cube(T x) { return x * x * x; }
 
 // Rush-Larsen update of one gate over dt: xinf + (x - xinf) exp(-dt/tau)
-// (batchedKernelExecution.H:590-596).
-template <class T> CG_HD T rl(T x, T xinf, T tau, T dt) { return xinf + (x - xinf) * cexp(-dt / tau); }
+// (batchedKernelExecution.H:590-596).  FP32 writes it as the increment
+// x + (xinf - x) (1 - exp(-dt/tau)), the form the offset storage of the mixed path adds to
+// its stored words (RlStep below); 1 - exp(-e) is exact in FP32 once exp(-e) is rounded.
+// That rounding (absolute 6e-8 near 1) still biases the increments of slow gates
+// (e = dt/tau ~ 1e-5) by up to 0.6 %; a series for small e removes it (measured on the
+// 2.1M-cell ORd run over 200 ms: Vm L-inf against FP64 0.20 -> 0.06 mV) but costs 21 % of
+// the ionic kernel's time, and both stay well inside the 0.5 mV bound, so the plain
+// exponential is kept.
+CG_HD float rl_frac(float e) { return 1.0f - cexp(-e); }
+template <class T> CG_HD T rl(T x, T xinf, T tau, T dt)
+{
+    if constexpr (is_fp32<T>())
+        return x + (xinf - x) * rl_frac(dt / tau);
+    else
+        return xinf + (x - xinf) * cexp(-dt / tau);
+}
 
 // Update policies for the models' right-hand sides.  A model writes each state once,
 // as out.gate(i, xinf, tau) (dx/dt = (xinf - x)/tau), out.lin(i, a, b) (dx/dt = a - b x)
@@ -57,12 +71,34 @@ template <class T> CG_HD T rl(T x, T xinf, T tau, T dt) { return xinf + (x - xin
 //           xinf, tau; the rest forward Euler).
 //   Deriv:  the time derivative, the RATES array an ODE solver asks for
 //           (TNNP.C:181-203, ORd.C:157-180, BuenoOrovio.C / AlievPanfilov.C derivatives).
+//   RlStep with s: the FP32 tissue step on offset storage -- y (the full values) is only
+//           read, and each state's increment is added to its stored offset s[i] instead
+//           (see StateRef in models.hpp).
 template <class T> struct RlStep {
     T* y;
     T dt;
-    CG_HD void gate(int i, T xinf, T tau) { y[i] = rl(y[i], xinf, tau, dt); }
-    CG_HD void lin(int i, T a, T b) { y[i] = rl(y[i], a / b, T(1) / b, dt); }
-    CG_HD void rate(int i, T r) { y[i] += dt * r; }
+    T* s = nullptr;
+    CG_HD void gate(int i, T xinf, T tau)
+    {
+        if constexpr (is_fp32<T>()) {
+            if (s) {
+                s[i] += (xinf - y[i]) * rl_frac(dt / tau);
+                return;
+            }
+        }
+        y[i] = rl(y[i], xinf, tau, dt);
+    }
+    CG_HD void lin(int i, T a, T b) { gate(i, a / b, T(1) / b); }
+    CG_HD void rate(int i, T r)
+    {
+        if constexpr (is_fp32<T>()) {
+            if (s) {
+                s[i] += dt * r;
+                return;
+            }
+        }
+        y[i] += dt * r;
+    }
 };
 template <class T> struct Deriv {
     const T* y;

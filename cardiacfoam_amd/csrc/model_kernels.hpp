@@ -42,13 +42,41 @@ CG_DEV void store_states(const T (&y)[NS], T* __restrict__ st, int64_t n, int64_
     for (int q = first; q < NS; ++q, sp += n) *sp = y[q];
 }
 
-// Ionic step of cell c in tissue mode with V held at v over the step; returns the total
-// ionic current at the step end.  INTEG 0: Rush-Larsen; 1: RKF45 over [o.t0, o.t1] from
-// the cell's kept trial step (BuenoOrovio.C:124-132 and its TNNP/ORd/AP twins), the V
-// terms computed once for all stages.
+// States of model M in the handle's storage (offsets from StateRef on FP32 handles):
+// offsets_stored says whether this type stores offsets; load_model_states gives the full
+// values y and the stored words s, store_model_full writes full values y back in the
+// storage's form.
+template <class M, class T> constexpr bool offsets_stored()
+{
+    return is_fp32<T>() && StateRef<M>::any;
+}
+template <class M, class T>
+CG_DEV void load_model_states(T (&y)[M::NS], T (&s)[M::NS], const T* __restrict__ st, int64_t n,
+                              int64_t c, int first)
+{
+    load_states(s, st, n, c, first);
+#pragma unroll
+    for (int q = first; q < M::NS; ++q)
+        y[q] = offsets_stored<M, T>() ? T(StateRef<M>::at(q)) + s[q] : s[q];
+}
+template <class M, class T>
+CG_DEV void store_model_full(const T (&y)[M::NS], T* __restrict__ st, int64_t n, int64_t c,
+                             int first)
+{
+    T w[M::NS];
+#pragma unroll
+    for (int q = first; q < M::NS; ++q)
+        w[q] = offsets_stored<M, T>() ? y[q] - T(StateRef<M>::at(q)) : y[q];
+    store_states(w, st, n, c, first);
+}
+
+// Ionic step of cell c in tissue mode with V held at v over the step, on full values in
+// place; returns the total ionic current at the step end.  INTEG 0: Rush-Larsen; 1: RKF45
+// over [o.t0, o.t1] from the cell's kept trial step (BuenoOrovio.C:124-132 and its
+// TNNP/ORd/AP twins), the V terms computed once for all stages.
 template <class M, class T, int INTEG>
-CG_DEV T tissue_advance(T v, T (&y)[M::NS], T dt, const T* p, int tissue, const T* gs,
-                        const OdeArgs& o, int64_t c)
+CG_DEV T tissue_advance_full(T v, T (&y)[M::NS], T dt, const T* p, int tissue, const T* gs,
+                             const OdeArgs& o, int64_t c)
 {
     if constexpr (INTEG == 0) {
         return Ops<M>::tissue_step(v, y, dt, p, tissue, gs);
@@ -65,6 +93,24 @@ CG_DEV T tissue_advance(T v, T (&y)[M::NS], T dt, const T* p, int tissue, const 
         o.step[c] = step;
         if (!ok) *o.fail = 1;
         return Ops<M>::iion(y, p, tissue, gs);
+    }
+}
+
+// The tissue step of one cell on the handle's storage: y enters with the full values and
+// s with the stored words; on return s holds the words to store (the offsets on FP32
+// handles of models with StateRef, else the full values).
+template <class M, class T, int INTEG>
+CG_DEV T tissue_advance(T v, T (&y)[M::NS], T (&s)[M::NS], T dt, const T* p, int tissue,
+                        const T* gs, const OdeArgs& o, int64_t c)
+{
+    if constexpr (INTEG == 0 && offsets_stored<M, T>()) {
+        return Ops<M>::tissue_step_off(v, y, s, dt, p, tissue, gs);
+    } else {
+        const T I = tissue_advance_full<M, T, INTEG>(v, y, dt, p, tissue, gs, o, c);
+#pragma unroll
+        for (int q = 1; q < M::NS; ++q)
+            s[q] = offsets_stored<M, T>() ? y[q] - T(StateRef<M>::at(q)) : y[q];
+        return I;
     }
 }
 
@@ -85,11 +131,11 @@ __global__ __launch_bounds__(MBS) void k_ionic_tissue(T* __restrict__ st, int64_
 #pragma unroll
         for (int k = 0; k < 4; ++k) gs[k] = gsc[k * n + i];
     }
-    T y[M::NS];
-    load_states(y, st, n, i, 1);
+    T y[M::NS], sw[M::NS];
+    load_model_states<M>(y, sw, st, n, i, 1);
     const T v = T((Vm[i] * M::VMUL + M::VOFF) / M::VSCALE);
-    const T I = tissue_advance<M, T, INTEG>(v, y, dt, p, tissue, gs, ode, i);
-    store_states(y, st, n, i, 1);
+    const T I = tissue_advance<M, T, INTEG>(v, y, sw, dt, p, tissue, gs, ode, i);
+    store_states(sw, st, n, i, 1);
     Im[i] = double(I) * M::ISCALE;
 }
 
@@ -112,8 +158,8 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
 #pragma unroll
         for (int k = 0; k < 4; ++k) gs[k] = gsc[k * n + i];
     }
-    T y[M::NS];
-    load_states(y, st, n, i, 0);
+    T y[M::NS], sw[M::NS];
+    load_model_states<M>(y, sw, st, n, i, 0);
     // trace row of the traced cell after a step ending at model time te: the states,
     // then their rates at te (the STATES and RATES columns of the single-cell trace)
     auto record = [&](int s, double te) __attribute__((always_inline)) {
@@ -159,7 +205,7 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
         ode.step[i] = step;
         if (!ok) *ode.fail = 1;
     }
-    store_states(y, st, n, i, 0);
+    store_model_full<M>(y, st, n, i, 0);
 }
 
 // ------------------------------------------------------- fused explicit step
@@ -196,12 +242,12 @@ __global__ __launch_bounds__(MBS) void k_explicit_hex(
     const IJK ix = ijk_of((uint32_t)c, g);
     const T* p = P.p;  // model constants read from the kernel arguments
     const T one[4] = {T(1), T(1), T(1), T(1)};
-    T y[M::NS];
-    load_states(y, st, n, c, 1);
+    T y[M::NS], sw[M::NS];
+    load_model_states<M>(y, sw, st, n, c, 1);
     const double vn = V[c];
     const T v = T((vn * M::VMUL + M::VOFF) / M::VSCALE);
-    const T I = tissue_advance<M, T, INTEG>(v, y, T(s.dt_model), p, tissue, one, s.ode, c);
-    store_states(y, st, n, c, 1);
+    const T I = tissue_advance<M, T, INTEG>(v, y, sw, T(s.dt_model), p, tissue, one, s.ode, c);
+    store_states(sw, st, n, c, 1);
     const double iion = double(I) * M::ISCALE;
     if (Iion) Iion[c] = iion;
 
@@ -241,16 +287,16 @@ __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st
     const IJK cc = ijk_of((uint32_t)c, g);
     const T* p = P.p;  // model constants read from the kernel arguments
     const T one[4] = {T(1), T(1), T(1), T(1)};
-    T y[M::NS];
-    load_states(y, st, n, c, 1);
+    T y[M::NS], sw[M::NS];
+    load_model_states<M>(y, sw, st, n, c, 1);
     const double vn = Vn[c];
     // loads of the epilogue issued before the model's arithmetic, so their latency hides
     // behind it
     const double vo = (x0 || s.c00 != 0.0 || s.act_flag) ? Vm1[c] : 0.0;
     const uint8_t fl = s.act_flag ? s.act_flag[c] : 0;
     const T v = T((vn * M::VMUL + M::VOFF) / M::VSCALE);
-    const T I = tissue_advance<M, T, INTEG>(v, y, T(s.dt_model), p, tissue, one, s.ode, c);
-    store_states(y, st, n, c, 1);
+    const T I = tissue_advance<M, T, INTEG>(v, y, sw, T(s.dt_model), p, tissue, one, s.ode, c);
+    store_states(sw, st, n, c, 1);
     const double iion = double(I) * M::ISCALE;
     Iion[c] = iion;
     const double src = box_source(cc.i, cc.j, cc.k, g, s.boxes);

@@ -29,6 +29,75 @@ struct ModelParams {
 
 template <class M> struct Ops;
 
+// Offset storage of the mixed path.  A state that stays near a fixed magnitude (Na_i,
+// K_i, Cl_i, SR calcium) moves by far less than half an FP32 ulp per step while the cell
+// rests (Na_i ~ 7 mM, ulp 4.8e-7, drifts by ~1e-8 per 0.02 ms step): stored as a plain
+// FP32 value it never moves, and the resting tissue ahead of a wave drifts away from the
+// FP64 path's, which shows as a conduction-velocity bias of ~1e-4 over 200 ms.  FP32
+// handles therefore store such a state as s = x - ref with a per-model constant ref, the
+// state's initial value: |s| starts at 0 and stays small, so its ulp resolves the
+// increments.  at(q) is the ref of state q (used as its FP32 rounding on host and device
+// alike), 0 for plain storage.  FP64 handles always store plain values.
+template <class M> struct StateRef {
+    static constexpr bool any = false;
+    CG_HD static constexpr double at(int) { return 0.0; }
+};
+template <> struct StateRef<TNNP> {
+    static constexpr bool any = true;
+    CG_HD static constexpr double at(int q)
+    {
+        return q == TNNP::Ki ? 138.3 : q == TNNP::Nai ? 11.6 : q == TNNP::CaSR ? 0.2 : 0.0;
+    }
+};
+template <> struct StateRef<ORd> {
+    static constexpr bool any = true;
+    CG_HD static constexpr double at(int q)
+    {
+        return (q == ORd::Nai || q == ORd::Nass) ? 7.0
+             : (q == ORd::Ki || q == ORd::Kss) ? 145.0
+             : (q == ORd::Cansr || q == ORd::Cajsr) ? 1.2 : 0.0;
+    }
+};
+template <> struct StateRef<Courtemanche> {
+    static constexpr bool any = true;
+    CG_HD static constexpr double at(int q)
+    {
+        return q == Courtemanche::Nai ? 13.8169746 : q == Courtemanche::Ki ? 136.355230
+             : q == Courtemanche::CaUp ? 1.5466812 : q == Courtemanche::CaRel ? 1.0765074 : 0.0;
+    }
+};
+template <> struct StateRef<PerisYague> {
+    static constexpr bool any = true;
+    CG_HD static constexpr double at(int q)
+    {
+        return q == PerisYague::Nai ? 11.17 : q == PerisYague::Ki ? 139.0
+             : q == PerisYague::Cli ? 30.0
+             : (q == PerisYague::CaUp || q == PerisYague::CaRel) ? 1.488 : 0.0;
+    }
+};
+template <> struct StateRef<Stewart> {
+    static constexpr bool any = true;
+    CG_HD static constexpr double at(int q)
+    {
+        return q == Stewart::Nai ? 8.8042029 : q == Stewart::Ki ? 136.7818942
+             : q == Stewart::CaSR ? 3.1083689 : 0.0;
+    }
+};
+
+// The Rush-Larsen tissue step on offset storage: y holds the full values (y[0] = v is
+// set here), s the stored offsets; the increments of the step land in s, y is refreshed
+// from them, and the total ionic current at the step end is returned.  RATES is the
+// model's rates() call with the RlStep policy given.
+template <class M, class T, class RATES, class IION>
+CG_HD T tissue_step_offsets(T* y, T* s, T dt, RATES rates, IION iion)
+{
+    RlStep<T> out{y, dt, s};
+    rates(out);
+#pragma unroll
+    for (int q = 1; q < M::NS; ++q) y[q] = T(StateRef<M>::at(q)) + s[q];
+    return iion();
+}
+
 template <> struct Ops<BuenoOrovio> {
     template <class T>
     CG_HD static T tissue_step(T u, T* y, T dt, const T* p, int, const T*)
@@ -83,6 +152,16 @@ template <> struct Ops<AlievPanfilov> {
 
 template <> struct Ops<TNNP> {
     template <class T>
+    CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int tissue, const T* gs)
+    {
+        y[0] = v;
+        const auto q = TNNP::vterms(v, p);
+        return tissue_step_offsets<TNNP>(
+            y, s, dt,
+            [&](RlStep<T>& out) { TNNP::rates(y, out, p, tissue, gs, false, T(0), q); },
+            [&]() { return TNNP::currents(v, y, p, q, gs).total(); });
+    }
+    template <class T>
     CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
     {
         y[0] = v;
@@ -111,6 +190,16 @@ template <> struct Ops<TNNP> {
 };
 
 template <> struct Ops<ORd> {
+    template <class T>
+    CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int tissue, const T* gs)
+    {
+        y[0] = v;
+        const auto q = ORd::vterms(v);
+        return tissue_step_offsets<ORd>(
+            y, s, dt,
+            [&](RlStep<T>& out) { ORd::rates(y, out, p, tissue, gs, false, T(0), q); },
+            [&]() { return ORd::currents(v, y, p, q, gs).total(); });
+    }
     template <class T>
     CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
     {
@@ -172,6 +261,15 @@ template <> struct Ops<FdaManufactured> {
 
 template <> struct Ops<Courtemanche> {
     template <class T>
+    CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = Courtemanche::vterms(v, p);
+        return tissue_step_offsets<Courtemanche>(
+            y, s, dt, [&](RlStep<T>& out) { Courtemanche::rates(y, out, p, gs, false, T(0), q); },
+            [&]() { return Courtemanche::currents(v, y, p, q, gs).total(); });
+    }
+    template <class T>
     CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T* gs)
     {
         y[0] = v;
@@ -205,6 +303,15 @@ template <> struct Ops<Courtemanche> {
 
 template <> struct Ops<PerisYague> {
     template <class T>
+    CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = PerisYague::vterms(v, p);
+        return tissue_step_offsets<PerisYague>(
+            y, s, dt, [&](RlStep<T>& out) { PerisYague::rates(y, out, p, gs, false, T(0), q); },
+            [&]() { return PerisYague::currents(v, y, p, q, gs).total(); });
+    }
+    template <class T>
     CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T* gs)
     {
         y[0] = v;
@@ -237,6 +344,15 @@ template <> struct Ops<PerisYague> {
 };
 
 template <> struct Ops<Stewart> {
+    template <class T>
+    CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = Stewart::vterms(v, p);
+        return tissue_step_offsets<Stewart>(
+            y, s, dt, [&](RlStep<T>& out) { Stewart::rates(y, out, p, gs, false, T(0), q); },
+            [&]() { return Stewart::currents(v, y, p, q, gs).total(); });
+    }
     template <class T>
     CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T* gs)
     {

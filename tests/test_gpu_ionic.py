@@ -72,3 +72,43 @@ def test_single_cell_run_parity(gpu, orc, name):
         orc.lib().orc_rl_step(m, tis, k * dtm, dtm, c, yy, 1, st)
     ref = yy[:len(y)]
     np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["ORd", "TNNP_epi", "CRN"])
+def test_mixed_resting_concentration_drift(gpu, orc, name):
+    """Resting cells (V held at the initial potential) for 2000 steps of 0.02 ms: Na_i and
+    K_i move by ~1e-8 per step, far below half an FP32 ulp of their values.  The mixed
+    path stores them as offsets from a reference and must follow the FP64 path's drift
+    (a plain FP32 store leaves them where they started)."""
+    _say("mixed resting drift", name)
+    m, tis, key, word, _ = MODELS[name]
+    n, steps, dt = 256, 2000, 2e-5
+    _, y0 = orc.model_init(m, tis)
+    V = np.full(n, y0[0] * 1e-3)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ion = gpu.IonicModel(key, word, n, solve_vm=False, precision=prec)
+        Im = np.zeros(n)
+        for k in range(steps):
+            ion.solveODE(k * dt, dt, V, Im)
+        res[prec] = ion.states()
+    idx = {"ORd": (2, 4), "TNNP_epi": (2, 1), "CRN": (1, 2)}[name]   # Na_i, K_i
+    for q in idx:
+        change = res["fp64"][q] - y0[q]
+        err = res["mixed"][q] - res["fp64"][q]
+        _say("  state %d: fp64 change %.3e, mixed - fp64 %.3e" % (q, change[0], np.max(np.abs(err))))
+        assert np.max(np.abs(change)) > 1e-6            # the drift is real
+        assert np.max(np.abs(err)) <= 0.02 * np.max(np.abs(change)) + 1e-7
+
+
+def test_mixed_state_round_trip(gpu, orc):
+    """set_states/get_states on an FP32 handle go through the offset storage: values come
+    back to FP32 precision of the offset, far finer than FP32 of the value itself."""
+    _say("mixed state round trip")
+    c, states, _ = physiological_states(orc, "ORd", 128, seed=2)
+    ion = gpu.IonicModel("ORd", "myocyte", 128, precision="mixed")
+    ion.set_states(states)
+    got = ion.states()
+    # K_i ~ 145 mM stored as an offset from 145: error ~ulp(offset), not ulp(145) = 1.5e-5
+    assert np.max(np.abs(got[4] - states[4])) < 2e-6
+    np.testing.assert_allclose(got[1:], states[1:], rtol=2e-7, atol=1e-12)
