@@ -221,10 +221,11 @@ void k_cells_protocol(T* __restrict__ st, KParams<T> P, const T* __restrict__ gs
     Detector<T> det{a, lane, cell, (T*)a.ring};
     if (a.det) det.load();
     for (int64_t s = a.s0; s < s1; ++s) {
-        Ops<M>::cell_step(y, T(a.dt), T(stimulus_current((double)s * a.dt, stim)), p, tissue, gs);
+        cell_step_stored<M>(y, sw, T(a.dt), T(stimulus_current((double)s * a.dt, stim)), p,
+                            tissue, gs);
         if (a.det) det.push(s, (double)y[0] * a.vscale - a.voff);
     }
-    store_model_full<M>(y, st, n, cell, 0);
+    store_states(sw, st, n, cell, 0);
     if (a.det) {
         if (s1 == len) det.flush(len);
         det.store();

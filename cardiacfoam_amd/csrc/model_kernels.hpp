@@ -114,6 +114,22 @@ CG_DEV T tissue_advance(T v, T (&y)[M::NS], T (&s)[M::NS], T dt, const T* p, int
     }
 }
 
+// One single-cell step (V inside the ODE) on the handle's storage: y the full values,
+// s the stored words, both kept current (the offsets on FP32 handles of models with
+// StateRef, where the increments land; else s mirrors y).
+template <class M, class T>
+CG_DEV void cell_step_stored(T (&y)[M::NS], T (&s)[M::NS], T dt, T istim, const T* p,
+                             int tissue, const T* gs)
+{
+    if constexpr (offsets_stored<M, T>()) {
+        Ops<M>::cell_step_off(y, s, dt, istim, p, tissue, gs);
+    } else {
+        Ops<M>::cell_step(y, dt, istim, p, tissue, gs);
+#pragma unroll
+        for (int q = 0; q < M::NS; ++q) s[q] = y[q];
+    }
+}
+
 // ---------------------------------------------------------------- ionic only
 template <class M, class T, int INTEG>
 __global__ __launch_bounds__(MBS) void k_ionic_tissue(T* __restrict__ st, int64_t n,
@@ -176,9 +192,11 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
     if constexpr (INTEG == 0) {
         for (int s = 0; s < n_steps; ++s) {
             const double t = t0 + s * double(dt);
-            Ops<M>::cell_step(y, dt, T(stimulus_current(t, stim)), p, tissue, gs);
+            cell_step_stored<M>(y, sw, dt, T(stimulus_current(t, stim)), p, tissue, gs);
             if (rec) record(s, t0 + (s + 1) * double(dt));
         }
+        store_states(sw, st, n, i, 0);
+        return;
     } else {
         auto f = [&](double t, const T* yy, T* d) __attribute__((always_inline)) {
             const auto q = Ops<M>::vt(yy[0], p);

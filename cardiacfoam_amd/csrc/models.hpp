@@ -88,6 +88,16 @@ template <> struct StateRef<Stewart> {
 // set here), s the stored offsets; the increments of the step land in s, y is refreshed
 // from them, and the total ionic current at the step end is returned.  RATES is the
 // model's rates() call with the RlStep policy given.
+// The single-cell step (V integrated inside) on offset storage, the same way.
+template <class M, class T, class RATES>
+CG_HD void cell_step_offsets(T* y, T* s, T dt, RATES rates)
+{
+    RlStep<T> out{y, dt, s};
+    rates(out);
+#pragma unroll
+    for (int q = 0; q < M::NS; ++q) y[q] = T(StateRef<M>::at(q)) + s[q];
+}
+
 template <class M, class T, class RATES, class IION>
 CG_HD T tissue_step_offsets(T* y, T* s, T dt, RATES rates, IION iion)
 {
@@ -152,6 +162,14 @@ template <> struct Ops<AlievPanfilov> {
 
 template <> struct Ops<TNNP> {
     template <class T>
+    CG_HD static void cell_step_off(T* y, T* s, T dt, T istim, const T* p, int tissue,
+                                    const T* gs)
+    {
+        const auto q = TNNP::vterms(y[0], p);
+        cell_step_offsets<TNNP>(
+            y, s, dt, [&](RlStep<T>& out) { TNNP::rates(y, out, p, tissue, gs, true, istim, q); });
+    }
+    template <class T>
     CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int tissue, const T* gs)
     {
         y[0] = v;
@@ -190,6 +208,14 @@ template <> struct Ops<TNNP> {
 };
 
 template <> struct Ops<ORd> {
+    template <class T>
+    CG_HD static void cell_step_off(T* y, T* s, T dt, T istim, const T* p, int tissue,
+                                    const T* gs)
+    {
+        const auto q = ORd::vterms(y[0]);
+        cell_step_offsets<ORd>(
+            y, s, dt, [&](RlStep<T>& out) { ORd::rates(y, out, p, tissue, gs, true, istim, q); });
+    }
     template <class T>
     CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int tissue, const T* gs)
     {
@@ -261,6 +287,13 @@ template <> struct Ops<FdaManufactured> {
 
 template <> struct Ops<Courtemanche> {
     template <class T>
+    CG_HD static void cell_step_off(T* y, T* s, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = Courtemanche::vterms(y[0], p);
+        cell_step_offsets<Courtemanche>(
+            y, s, dt, [&](RlStep<T>& out) { Courtemanche::rates(y, out, p, gs, true, istim, q); });
+    }
+    template <class T>
     CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int, const T* gs)
     {
         y[0] = v;
@@ -303,6 +336,13 @@ template <> struct Ops<Courtemanche> {
 
 template <> struct Ops<PerisYague> {
     template <class T>
+    CG_HD static void cell_step_off(T* y, T* s, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = PerisYague::vterms(y[0], p);
+        cell_step_offsets<PerisYague>(
+            y, s, dt, [&](RlStep<T>& out) { PerisYague::rates(y, out, p, gs, true, istim, q); });
+    }
+    template <class T>
     CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int, const T* gs)
     {
         y[0] = v;
@@ -344,6 +384,13 @@ template <> struct Ops<PerisYague> {
 };
 
 template <> struct Ops<Stewart> {
+    template <class T>
+    CG_HD static void cell_step_off(T* y, T* s, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = Stewart::vterms(y[0], p);
+        cell_step_offsets<Stewart>(
+            y, s, dt, [&](RlStep<T>& out) { Stewart::rates(y, out, p, gs, true, istim, q); });
+    }
     template <class T>
     CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int, const T* gs)
     {
