@@ -179,7 +179,9 @@ int cg_batch_autotune(cg_ionic* const* handles, int n_handles, double dt_s, int 
  * trace ends first); unused records are NaN. */
 int cg_ionic_get_beats(cg_ionic* h, int32_t* counts, double* records);
 
-/* States as host SoA doubles [n_vars][n_cells], starting at state first_var.
+/* States as host SoA doubles [n_vars][n_cells], starting at state first_var: the full
+ * values (CG_MIXED handles keep their slow concentrations on the device as FP32 offsets
+ * from the model's initial values; both calls convert).
  * Replaces the io*Ptr hooks / exportStates (ionicModel.H:115-168, 584-638). */
 int cg_ionic_get_states(cg_ionic* h, int first_var, int n_vars, double* host_soa);
 int cg_ionic_set_states(cg_ionic* h, int first_var, int n_vars, const double* host_soa);
