@@ -44,8 +44,7 @@ CG_DEV void store_states(const T (&y)[NS], T* __restrict__ st, int64_t n, int64_
 
 // States of model M in the handle's storage (offsets from StateRef on FP32 handles):
 // offsets_stored says whether this type stores offsets; load_model_states gives the full
-// values y and the stored words s, store_model_full writes full values y back in the
-// storage's form.
+// values y and the stored words s (the kernels keep both current and store s).
 template <class M, class T> constexpr bool offsets_stored()
 {
     return is_fp32<T>() && StateRef<M>::any;
@@ -58,16 +57,6 @@ CG_DEV void load_model_states(T (&y)[M::NS], T (&s)[M::NS], const T* __restrict_
 #pragma unroll
     for (int q = first; q < M::NS; ++q)
         y[q] = offsets_stored<M, T>() ? T(StateRef<M>::at(q)) + s[q] : s[q];
-}
-template <class M, class T>
-CG_DEV void store_model_full(const T (&y)[M::NS], T* __restrict__ st, int64_t n, int64_t c,
-                             int first)
-{
-    T w[M::NS];
-#pragma unroll
-    for (int q = first; q < M::NS; ++q)
-        w[q] = offsets_stored<M, T>() ? y[q] - T(StateRef<M>::at(q)) : y[q];
-    store_states(w, st, n, c, first);
 }
 
 // Ionic step of cell c in tissue mode with V held at v over the step, on full values in
@@ -105,6 +94,29 @@ CG_DEV T tissue_advance(T v, T (&y)[M::NS], T (&s)[M::NS], T dt, const T* p, int
 {
     if constexpr (INTEG == 0 && offsets_stored<M, T>()) {
         return Ops<M>::tissue_step_off(v, y, s, dt, p, tissue, gs);
+    } else if constexpr (offsets_stored<M, T>()) {
+        // RKF45 on the stored offsets, the rates at ref + s, the norm on the values
+        T ref[M::NS];
+#pragma unroll
+        for (int q = 0; q < M::NS; ++q) ref[q] = T(StateRef<M>::at(q));
+        y[0] = v;
+        s[0] = v;
+        const auto qv = Ops<M>::vt(v, p);
+        auto f = [&](double, const T* ss, T* d) __attribute__((always_inline)) {
+            T yy[M::NS];
+#pragma unroll
+            for (int q = 0; q < M::NS; ++q) yy[q] = ref[q] + ss[q];
+            Ops<M>::deriv(yy, d, p, tissue, gs, false, T(0), qv);
+        };
+        double step = o.step[c];
+        if (!(step < o.dt_clamp)) step = o.dt_clamp;
+        const bool ok = rkf45_solve<M::NS>(f, s, o.t0, o.t1, step, T(o.abs_tol), T(o.rel_tol),
+                                           o.max_steps, ref);
+        o.step[c] = step;
+        if (!ok) *o.fail = 1;
+#pragma unroll
+        for (int q = 1; q < M::NS; ++q) y[q] = ref[q] + s[q];
+        return Ops<M>::iion(y, p, tissue, gs);
     } else {
         const T I = tissue_advance_full<M, T, INTEG>(v, y, dt, p, tissue, gs, o, c);
 #pragma unroll
@@ -198,7 +210,14 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
         store_states(sw, st, n, i, 0);
         return;
     } else {
-        auto f = [&](double t, const T* yy, T* d) __attribute__((always_inline)) {
+        // offsets_stored: integrate the stored offsets sw, rates at ref + sw, norm on values
+        T ref[M::NS];
+#pragma unroll
+        for (int q = 0; q < M::NS; ++q) ref[q] = T(StateRef<M>::at(q));
+        auto f = [&](double t, const T* ss, T* d) __attribute__((always_inline)) {
+            T yy[M::NS];
+#pragma unroll
+            for (int q = 0; q < M::NS; ++q) yy[q] = offsets_stored<M, T>() ? ref[q] + ss[q] : ss[q];
             const auto q = Ops<M>::vt(yy[0], p);
             Ops<M>::deriv(yy, d, p, tissue, gs, true, T(stimulus_current(t, stim)), q);
         };
@@ -216,14 +235,17 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
                               : ode.ap ? (ts + ode.dt_s) * 1000.0 / 12.9
                                        : (ts + ode.dt_s) * 1000.0;
             if (!(step < ode.dt_clamp)) step = ode.dt_clamp;
-            ok = rkf45_solve<M::NS>(f, y, x0, x1, step, T(ode.abs_tol), T(ode.rel_tol),
-                                    ode.max_steps);
+            ok = rkf45_solve<M::NS>(f, sw, x0, x1, step, T(ode.abs_tol), T(ode.rel_tol),
+                                    ode.max_steps, offsets_stored<M, T>() ? ref : nullptr);
+#pragma unroll
+            for (int q = 0; q < M::NS; ++q)
+                y[q] = offsets_stored<M, T>() ? ref[q] + sw[q] : sw[q];
             if (rec) record(s, x1);
         }
         ode.step[i] = step;
         if (!ok) *ode.fail = 1;
     }
-    store_model_full<M>(y, st, n, i, 0);
+    store_states(sw, st, n, i, 0);
 }
 
 // ------------------------------------------------------- fused explicit step

@@ -19,10 +19,11 @@
 
 namespace cg {
 
-// f(t, y, dydt): right-hand side at model time t
+// f(t, y, dydt): right-hand side at model time t.  ref (optional): y holds offsets from
+// ref (the FP32 offset storage); the error norm is taken on the values ref + y.
 template <int NS, class T, class F>
 CG_HD double rkf45_trial(F& f, double x0, const T (&y0)[NS], const T (&d0)[NS], double dxd,
-                          T (&y)[NS], T at, T rt)
+                          T (&y)[NS], T at, T rt, const T* ref = nullptr)
 {
     const T dx = T(dxd);
     T yt[NS], k2[NS], k3[NS], k4[NS], k5[NS], k6[NS];
@@ -60,7 +61,8 @@ CG_HD double rkf45_trial(F& f, double x0, const T (&y0)[NS], const T (&d0)[NS], 
                              T(b6) * k6[i]);
         const T err = dx * (T(e1) * d0[i] + T(e3) * k3[i] + T(e4) * k4[i] + T(e5) * k5[i] +
                             T(e6) * k6[i]);
-        const T tol = at + rt * (fabs(y0[i]) > fabs(y[i]) ? fabs(y0[i]) : fabs(y[i]));
+        const T v0 = ref ? ref[i] + y0[i] : y0[i], v1 = ref ? ref[i] + y[i] : y[i];
+        const T tol = at + rt * (fabs(v0) > fabs(v1) ? fabs(v0) : fabs(v1));
         const T q = fabs(err) / tol;
         maxErr = q > maxErr ? q : maxErr;
     }
@@ -69,13 +71,14 @@ CG_HD double rkf45_trial(F& f, double x0, const T (&y0)[NS], const T (&d0)[NS], 
 
 // one accepted step from x; dxTry in: trial size, out: next trial size
 template <int NS, class T, class F>
-CG_HD bool rkf45_adaptive(F& f, double& x, T (&y)[NS], double& dxTry, T at, T rt)
+CG_HD bool rkf45_adaptive(F& f, double& x, T (&y)[NS], double& dxTry, T at, T rt,
+                          const T* ref = nullptr)
 {
     T d0[NS], yt[NS];
     f(x, y, d0);
     double dx = dxTry, err;
     for (;;) {
-        err = rkf45_trial<NS>(f, x, y, d0, dx, yt, at, rt);
+        err = rkf45_trial<NS>(f, x, y, d0, dx, yt, at, rt, ref);
         if (!(err > 1.0)) break;
         const double scale = 0.9 * pow(err, -0.25);
         dx *= scale > 0.2 ? scale : 0.2;
@@ -96,7 +99,7 @@ CG_HD bool rkf45_adaptive(F& f, double& x, T (&y)[NS], double& dxTry, T at, T rt
 // integrate y over [xStart, xEnd]; false on step-size underflow or max_steps
 template <int NS, class T, class F>
 CG_HD bool rkf45_solve(F& f, T (&y)[NS], double xStart, double xEnd, double& dxTry, T at,
-                        T rt, long long max_steps)
+                        T rt, long long max_steps, const T* ref = nullptr)
 {
     double x = xStart;
     bool last = false;
@@ -106,7 +109,7 @@ CG_HD bool rkf45_solve(F& f, T (&y)[NS], double xStart, double xEnd, double& dxT
             last = true;
             dxTry = xEnd - x;
         }
-        if (!rkf45_adaptive<NS>(f, x, y, dxTry, at, rt)) return false;
+        if (!rkf45_adaptive<NS>(f, x, y, dxTry, at, rt, ref)) return false;
         if ((x - xEnd) * (xEnd - xStart) >= 0) {
             if (k > 0 && last) dxTry = dxTry0;
             return true;

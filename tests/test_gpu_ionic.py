@@ -74,13 +74,14 @@ def test_single_cell_run_parity(gpu, orc, name):
     np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-10)
 
 
-@pytest.mark.parametrize("name", ["ORd", "TNNP_epi", "CRN"])
-def test_mixed_resting_concentration_drift(gpu, orc, name):
+@pytest.mark.parametrize("name,integ", [("ORd", "RL"), ("TNNP_epi", "RL"), ("CRN", "RL"),
+                                        ("ORd", "RKF45")])
+def test_mixed_resting_concentration_drift(gpu, orc, name, integ):
     """Resting cells (V held at the initial potential) for 2000 steps of 0.02 ms: Na_i and
     K_i move by ~1e-8 per step, far below half an FP32 ulp of their values.  The mixed
     path stores them as offsets from a reference and must follow the FP64 path's drift
     (a plain FP32 store leaves them where they started)."""
-    _say("mixed resting drift", name)
+    _say("mixed resting drift", name, integ)
     m, tis, key, word, _ = MODELS[name]
     n, steps, dt = 256, 2000, 2e-5
     _, y0 = orc.model_init(m, tis)
@@ -88,6 +89,8 @@ def test_mixed_resting_concentration_drift(gpu, orc, name):
     res = {}
     for prec in ("fp64", "mixed"):
         ion = gpu.IonicModel(key, word, n, solve_vm=False, precision=prec)
+        if integ == "RKF45":
+            ion.set_integrator("RKF45", 1e-6, 1e-4, 100000)
         Im = np.zeros(n)
         for k in range(steps):
             ion.solveODE(k * dt, dt, V, Im)
