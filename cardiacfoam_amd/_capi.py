@@ -96,6 +96,7 @@ def lib():
     L.cg_mesh_ncells.argtypes = [vp]
     L.cg_mesh_ncells.restype = C.c_int64
     L.cg_mesh_set_conductivity.argtypes = [vp, vp, C.c_int]
+    L.cg_mesh_set_cell_centres.argtypes = [vp, vp]
     L.cg_mesh_explicit_dt.argtypes = [vp, C.c_double, C.c_double, C.c_double, dp]
     L.cg_mesh_laplacian.argtypes = [vp, vp, vp]
     L.cg_tissue_create.argtypes = [vp, vp, C.POINTER(TissueConfig), C.POINTER(vp)]

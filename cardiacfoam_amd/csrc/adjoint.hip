@@ -45,6 +45,8 @@ enum { SL_V, SL_v, SL_w, SL_s, SL_Ev, SL_Ew, SL_Es, SL_thw, SL_ths, SL_thso, SL_
 struct BoxTable {
     int n;
     double lo[MAX_BOXES][3], hi[MAX_BOXES][3], t0[MAX_BOXES], dur[MAX_BOXES], I[MAX_BOXES];
+    const uint8_t* mask;  // bit b of mask[c]: box b contains cell c's centre (the mesh's
+                          // own centres); null: the (i+1/2)h lattice
 };
 
 struct AdjGeom {
@@ -63,6 +65,10 @@ CG_DEV double box_src(int i, int j, int k, const HexGeom& g, const BoxTable& b, 
     const double p0 = (i + 0.5) * g.hx, p1 = (j + 0.5) * g.hy, p2 = (k + 0.5) * g.hz;
     for (int q = 0; q < b.n; ++q) {
         if (t0 < b.t0[q] || t0 > b.t0[q] + b.dur[q]) continue;
+        if (b.mask) {
+            if ((b.mask[i + (int64_t)g.nx * (j + (int64_t)g.ny * k)] >> q) & 1u) src += b.I[q];
+            continue;
+        }
         if (p0 >= b.lo[q][0] && p0 <= b.hi[q][0] && p1 >= b.lo[q][1] && p1 <= b.hi[q][1] &&
             p2 >= b.lo[q][2] && p2 <= b.hi[q][2])
             src += b.I[q];
@@ -338,6 +344,7 @@ struct Adjoint {
     int n_probes;
     double* d_prm = nullptr;    // model constants [NP]
     BoxTable* d_boxes = nullptr;
+    uint8_t* d_mask = nullptr;  // copy of the tissue's box-membership mask, or null
     int* d_slot = nullptr;      // [n] probe slot or -1
     double* d_V0 = nullptr;     // initial state
     double* d_y0 = nullptr;     // [3][n]
@@ -359,7 +366,7 @@ void adjoint_destroy(Adjoint* A)
     if (!A) return;
     void* p[] = {A->d_prm, A->d_boxes, A->d_slot, A->d_V0, A->d_y0,  A->fx,    A->fy,
                  A->fz,    A->ck,      A->seg,    A->lV[0], A->lV[1], A->ly[0], A->ly[1],
-                 A->gtheta};
+                 A->gtheta,  A->d_mask};
     for (void* q : p)
         if (q) hipFree(q);
     delete A;
@@ -402,6 +409,7 @@ int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, d
     auto M = [&](void** p, size_t b) { ok &= hipMalloc(p, b) == hipSuccess; };
     M((void**)&A->d_prm, sizeof(double) * MAX_PARAMS);
     M((void**)&A->d_boxes, sizeof(BoxTable));
+    if (boxes.mask) M((void**)&A->d_mask, n);
     M((void**)&A->d_slot, sizeof(int) * n);
     M((void**)&A->d_V0, 8 * n);
     M((void**)&A->d_y0, 8 * 3 * n);
@@ -421,6 +429,8 @@ int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, d
         bt.dur[b] = boxes.dur[b];
         bt.I[b] = boxes.I[b];
     }
+    if (boxes.mask) hipMemcpy(A->d_mask, boxes.mask, n, hipMemcpyDeviceToDevice);
+    bt.mask = A->d_mask;
     hipMemcpy(A->d_boxes, &bt, sizeof bt, hipMemcpyHostToDevice);
     hipMemcpy(A->d_slot, slot.data(), sizeof(int) * n, hipMemcpyHostToDevice);
     hipMemcpy(A->d_V0, V0, 8 * n, hipMemcpyDeviceToDevice);

@@ -227,6 +227,8 @@ struct cg_mesh {
     cg::GeneralMesh* gm = nullptr; // set for a general (polyhedral) mesh; g is unused then
     double* d_sigma9 = nullptr;    // per-cell tensors on the device, made on demand:
     bool sigma_sym = false;        // [6][n] symmetric (xx xy xz yy yz zz), else [9][n]
+    std::vector<double> h_centres; // hex block: the caller's cell centres [3n], or empty
+    uint64_t centres_ver = 0;      // bumped by every cg_mesh_set_cell_centres
 };
 
 struct cg_tissue {
@@ -241,6 +243,11 @@ struct cg_tissue {
     int n_boxes = 0;
     double box_lo[cg::MAX_BOXES][3], box_hi[cg::MAX_BOXES][3];
     double box_t0[cg::MAX_BOXES], box_dur[cg::MAX_BOXES], box_I[cg::MAX_BOXES];
+    // box membership of each cell from the mesh's own centres (hex block with centres):
+    // bit b of d_box_mask[c] set when box b contains the centre; rebuilt when the boxes
+    // or the centres change
+    uint8_t* d_box_mask = nullptr;
+    uint64_t boxes_ver = 1, mask_boxes_ver = 0, mask_centres_ver = 0;
     double t = 0.0;
     int64_t steps = 0;
     cg::ImplicitSolver* implicit = nullptr;
@@ -965,6 +972,17 @@ int cg_mesh_destroy(cg_mesh* m)
     return CG_OK;
 }
 
+int cg_mesh_set_cell_centres(cg_mesh* m, const double* C)
+{
+    if (!m || !C) return fail(CG_ERR_ARG, "NULL argument");
+    if (m->gm) return fail(CG_ERR_STATE, "a general mesh carries its centres from cg_mesh_create_general");
+    for (int64_t q = 0; q < 3 * m->n; ++q)
+        if (!std::isfinite(C[q])) return fail(CG_ERR_ARG, "cell centre %lld is not finite", (long long)(q / 3));
+    m->h_centres.assign(C, C + 3 * m->n);
+    ++m->centres_ver;
+    return CG_OK;
+}
+
 int64_t cg_mesh_ncells(const cg_mesh* m) { return m ? m->n : -1; }
 
 int cg_mesh_set_conductivity(cg_mesh* m, const double* s, int uniform)
@@ -1146,6 +1164,7 @@ int cg_tissue_destroy(cg_tissue* t)
     if (t->ion && t->ion->stream) hipStreamSynchronize(t->ion->stream);
     for (int q = 0; q < 3; ++q) hipFree(t->d_V[q]);
     hipFree(t->d_Iion); hipFree(t->d_flag); hipFree(t->d_tact);
+    if (t->d_box_mask) hipFree(t->d_box_mask);
     if (t->d_ecg) hipFree(t->d_ecg);
     if (t->gldu) cg::ldu_destroy(t->gldu);
     cg::general_destroy(t->own_gm);
@@ -1171,17 +1190,53 @@ int cg_tissue_set_stimulus_boxes(cg_tissue* t, int n, const double* lo, const do
         t->box_t0[b] = t0[b]; t->box_dur[b] = dur[b]; t->box_I[b] = I[b];
     }
     t->n_boxes = n;
+    ++t->boxes_ver;
+    return CG_OK;
+}
+
+// The membership mask of the boxes over the mesh's own centres (hex block given them):
+// myocardiumDomain.C:307-342 tests mesh().C() with the inclusive boundBox::contains.
+static int ensure_box_mask(cg_tissue* t)
+{
+    const cg_mesh* m = t->mesh;
+    if (m->gm || m->h_centres.empty()) return CG_OK;
+    if (t->d_box_mask && t->mask_boxes_ver == t->boxes_ver && t->mask_centres_ver == m->centres_ver)
+        return CG_OK;
+    const int64_t n = m->n;
+    std::vector<uint8_t> mask((size_t)n, 0);
+    const double* C = m->h_centres.data();
+    for (int64_t c = 0; c < n; ++c) {
+        unsigned bits = 0;
+        for (int b = 0; b < t->n_boxes; ++b) {
+            bool in = true;
+            for (int d = 0; d < 3; ++d)
+                in = in && C[3 * c + d] >= t->box_lo[b][d] && C[3 * c + d] <= t->box_hi[b][d];
+            if (in) bits |= 1u << b;
+        }
+        mask[(size_t)c] = (uint8_t)bits;
+    }
+    if (!t->d_box_mask && hipMalloc(&t->d_box_mask, (size_t)n) != hipSuccess) {
+        t->d_box_mask = nullptr;
+        return fail(CG_ERR_NOMEM, "box mask allocation failed");
+    }
+    HIPCHK(hipStreamSynchronize(t->ion->stream));
+    HIPCHK(hipMemcpy(t->d_box_mask, mask.data(), (size_t)n, hipMemcpyHostToDevice));
+    t->mask_boxes_ver = t->boxes_ver;
+    t->mask_centres_ver = m->centres_ver;
     return CG_OK;
 }
 
 static cg::ActiveBoxes active_boxes(const cg_tissue* t, double t0)
 {
     cg::ActiveBoxes a{};
+    const bool use_mask = !t->mesh->gm && !t->mesh->h_centres.empty();
+    a.mask = use_mask ? t->d_box_mask : nullptr;
     for (int b = 0; b < t->n_boxes; ++b) {
         const double tS = t->box_t0[b], tE = tS + t->box_dur[b];
         if (t0 < tS || t0 > tE) continue;
         for (int d = 0; d < 3; ++d) { a.lo[a.n][d] = t->box_lo[b][d]; a.hi[a.n][d] = t->box_hi[b][d]; }
         a.intensity[a.n] = t->box_I[b];
+        a.bit[a.n] = (uint8_t)b;
         ++a.n;
     }
     return a;
@@ -1281,6 +1336,7 @@ static int general_advance(cg_tissue* t, cg::GeneralMesh* G, int n_steps, bool i
 int cg_tissue_diffusion_step(cg_tissue* t, const double* Iion, const double* source, int flags)
 {
     if (!t || !Iion) return fail(CG_ERR_ARG, "NULL argument");
+    if (int rc = ensure_box_mask(t)) return rc;
     cg_mesh* m = t->mesh;
     hipStream_t s = t->ion->stream;
     const size_t bytes = sizeof(double) * m->n;
@@ -1316,6 +1372,7 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
 {
     if (!t) return fail(CG_ERR_ARG, "NULL handle");
     if (t->mesh->gm) return general_advance(t, t->mesh->gm, n_steps, true, nullptr);
+    if (int rc = ensure_box_mask(t)) return rc;
     cg_ionic* ion = t->ion;
     cg_mesh* m = t->mesh;
     const double dt = t->cfg.dt, chiCm = t->cfg.chi * t->cfg.cm;
@@ -1668,6 +1725,7 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
         return fail(CG_ERR_STATE, "the adjoint covers BuenoOrovio in FP64");
     if (t->ion->ode_kind != CG_ODE_RUSH_LARSEN)
         return fail(CG_ERR_STATE, "the adjoint differentiates the Rush-Larsen ionic step");
+    if (int rc = ensure_box_mask(t)) return rc;
     if (!m->uniform) return fail(CG_ERR_STATE, "the adjoint scales a uniform conductivity");
     if (m->sig[1] != 0 || m->sig[2] != 0 || m->sig[5] != 0 || m->sig[3] != 0 ||
         m->sig[6] != 0 || m->sig[7] != 0)
@@ -1677,6 +1735,7 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
     const double diag[3] = {m->sig[0], m->sig[4], m->sig[8]};
     cg::ActiveBoxesAll bx{};
     bx.n = t->n_boxes;
+    bx.mask = m->h_centres.empty() ? nullptr : t->d_box_mask;
     for (int b = 0; b < t->n_boxes; ++b) {
         for (int d = 0; d < 3; ++d) { bx.lo[b][d] = t->box_lo[b][d]; bx.hi[b][d] = t->box_hi[b][d]; }
         bx.t0[b] = t->box_t0[b];

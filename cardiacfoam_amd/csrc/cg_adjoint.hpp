@@ -12,6 +12,7 @@ struct Adjoint;
 struct ActiveBoxesAll {
     int n;
     double lo[MAX_BOXES][3], hi[MAX_BOXES][3], t0[MAX_BOXES], dur[MAX_BOXES], I[MAX_BOXES];
+    const uint8_t* mask;  // device box-membership mask from the mesh's centres, or null
 };
 
 // Snapshot of the initial state (V0 [n], ionic states [4][n] of BuenoOrovio, FP64) and the

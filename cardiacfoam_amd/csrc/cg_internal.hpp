@@ -10,11 +10,16 @@ constexpr int MAX_BOXES = 8;
 
 // Box stimulus active at one step (myocardiumDomain.C:307-342): cells whose centre lies
 // inside [lo, hi] (inclusive, boundBox::contains) receive the summed intensity (A/m^3).
+// On a hex block the centres are the (i+1/2)h lattice unless the caller gave the mesh's
+// own centres: then mask[c] holds bit b for every box b that contains cell c's centre
+// (computed once on the host) and bit[q] names the box of active slot q.
 struct ActiveBoxes {
     int n;
     double lo[MAX_BOXES][3];
     double hi[MAX_BOXES][3];
     double intensity[MAX_BOXES];
+    const uint8_t* mask;
+    uint8_t bit[MAX_BOXES];
 };
 
 struct HexGeom {

@@ -132,9 +132,18 @@ __device__ __forceinline__ double flux_sum(const GView& g, const double* __restr
     return sum;
 }
 
+// box stimulus on the mesh's centres; a hex block described as a general mesh uses the
+// membership mask made from the caller's centres when it has one
 __device__ __forceinline__ double general_box(const GView& g, int64_t c, const ActiveBoxes& b)
 {
     double src = 0.0;
+    if (b.mask) {
+        if (b.n == 0) return 0.0;
+        const unsigned m = b.mask[c];
+        for (int q = 0; q < b.n; ++q)
+            if ((m >> b.bit[q]) & 1u) src += b.intensity[q];
+        return src;
+    }
     const double p0 = g.C[c], p1 = g.C[g.n + c], p2 = g.C[2 * g.n + c];
     for (int q = 0; q < b.n; ++q) {
         if (p0 >= b.lo[q][0] && p0 <= b.hi[q][0] && p1 >= b.lo[q][1] && p1 <= b.hi[q][1] &&

@@ -249,11 +249,19 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
 }
 
 // ------------------------------------------------------- fused explicit step
-// Box stimulus of myocardiumDomain.C:307-342: summed intensities of the boxes whose
-// closed extent contains the cell centre.
-CG_DEV double box_source(int i, int j, int k, const HexGeom& g, const ActiveBoxes& b)
+// Box stimulus of myocardiumDomain.C:307-342: summed intensities, in box order, of the
+// boxes whose closed extent contains the cell centre.  The membership mask (when the
+// mesh's centres were given) is read only while some box is active.
+CG_DEV double box_source(int i, int j, int k, int64_t c, const HexGeom& g, const ActiveBoxes& b)
 {
     double src = 0.0;
+    if (b.mask) {
+        if (b.n == 0) return 0.0;
+        const unsigned m = b.mask[c];
+        for (int q = 0; q < b.n; ++q)
+            if ((m >> b.bit[q]) & 1u) src += b.intensity[q];
+        return src;
+    }
     const double p0 = (i + 0.5) * g.hx, p1 = (j + 0.5) * g.hy, p2 = (k + 0.5) * g.hz;
     for (int q = 0; q < b.n; ++q) {
         if (p0 >= b.lo[q][0] && p0 <= b.hi[q][0] && p1 >= b.lo[q][1] && p1 <= b.hi[q][1] &&
@@ -293,7 +301,7 @@ __global__ __launch_bounds__(MBS) void k_explicit_hex(
 
     double lap = hex_laplacian<UNIFORM>(V, c, ix, g, a);
     if (s.od) lap += hex_tensor_correction(V, s.od, n, c, ix.i, ix.j, ix.k, g, s.od_mask);
-    const double src = box_source(ix.i, ix.j, ix.k, g, s.boxes);
+    const double src = box_source(ix.i, ix.j, ix.k, c, g, s.boxes);
     double rhs = s.c0 * vn + s.dt / s.chiCm * (lap - s.chiCm * iion + src);
     if (s.c00 != 0.0) rhs -= s.c00 * Vold[c];
     const double vnew = rhs / s.c;
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st
     store_states(sw, st, n, c, 1);
     const double iion = double(I) * M::ISCALE;
     Iion[c] = iion;
-    const double src = box_source(cc.i, cc.j, cc.k, g, s.boxes);
+    const double src = box_source(cc.i, cc.j, cc.k, c, g, s.boxes);
     double rhs = s.chiCm / s.dt * (s.c0 * vn - s.c00 * vo) + src - s.chiCm * iion;
     b[c] = rhs;
     if (x0) x0[c] = 2.0 * vn - vo;

@@ -3,9 +3,11 @@
 Handles ``key value;``, ``key [dims] value;``, lists ``(a b c)``, nested lists,
 sub-dictionaries ``name { ... }``, ``//`` and ``/* */`` comments and the ``FoamFile``
 header.  Numbers become float/int, everything else stays a string.  This covers the
-keys read on the tissue path (SURVEY of electroProperties keys: conductivity, chi, cm,
-ionicModel, tissue, solutionAlgorithm, solver, absTol, relTol, maxSteps,
-externalStimulus{...}, singleCellStimulus{...}).
+keys read on the tissue path (electroProperties: conductivity, chi, cm, ionicModel,
+tissue, solutionAlgorithm, solver, absTol, relTol, maxSteps, externalStimulus{...},
+singleCellStimulus{...}; fvSchemes, fvSolution, controlDict) and blockMeshDict's
+vertices, blocks and boundary lists (a list entry ``name { ... }`` becomes the name
+followed by a dict).
 """
 import re
 
@@ -59,6 +61,10 @@ class _Parser:
         while self.peek() != ")":
             if self.peek() == "(":
                 out.append(self.parse_list())
+            elif self.peek() == "{":   # dictionary entry of a list (blockMeshDict patches)
+                self.take()
+                out.append(self.parse_dict("}"))
+                self.take()
             else:
                 out.append(_atom(self.take()))
         self.take()

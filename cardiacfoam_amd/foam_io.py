@@ -232,9 +232,11 @@ def read_field(path):
 # ---------------------------------------------------------------------- probes
 class Probes:
     """OpenFOAM's `probes` function object on the hex block (blockMesh order
-    c = i + nx (j + ny k), spacing h): each location is sampled in the cell that contains
-    it, a point on a shared face going to the cell above it and a point on the far wall to
-    the last cell.  Files: <out_dir>/<name>/<start_time>/<field>."""
+    c = i + nx (j + ny k), spacing h): each location is sampled in the lowest-numbered cell
+    whose closed extent contains it (polyMesh::findCell's first match), so a point on a
+    face shared by two cells goes to the lower one.  The cell faces are the block's points
+    as blockMesh writes them (10 significant digits).  Files:
+    <out_dir>/<name>/<start_time>/<field>."""
 
     def __init__(self, shape, h, locations, fields, out_dir, name="probes", start_time="0",
                  precision=6):
@@ -259,9 +261,13 @@ class Probes:
         tol = 1e-9 * ext
         if np.any(self.locations < -tol) or np.any(self.locations > ext + tol):
             raise FatalIOError("probe location outside the mesh bounding box")
-        ijk = np.floor(self.locations / self.h).astype(np.int64)
-        ijk = np.clip(ijk, 0, np.array([nx, ny, nz]) - 1)
-        return ijk[:, 0] + nx * (ijk[:, 1] + ny * ijk[:, 2])
+        from .meshes import written_points
+        ijk = []
+        for d, m in enumerate((nx, ny, nz)):
+            nodes = written_points(np.arange(m + 1) * self.h)
+            q = np.searchsorted(nodes, self.locations[:, d], side="left") - 1
+            ijk.append(np.clip(q, 0, m - 1))
+        return ijk[0] + nx * (ijk[1] + ny * ijk[2])
 
     def _header(self):
         w = self.precision + 7

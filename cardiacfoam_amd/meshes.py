@@ -1,28 +1,225 @@
-"""Polyhedral meshes in the form cg_mesh_create_general takes, with the geometry OpenFOAM's
-primitiveMesh computes (primitiveMeshFaceCentresAndAreas.C: triangle fan around the face's
-point average; primitiveMeshCellCentresAndVols.C: pyramids on the faces around the average
-of the cell's face centres).  `block` builds a hex block whose lattice points may be moved
-by a map, faces in OpenFOAM's order (internal faces by owner, then neighbour; boundary
-faces after), so a block run through the general path can be compared with the hex path,
-and a warped block exercises the non-orthogonal correction."""
+"""Mesh geometry as OpenFOAM computes it, for the hex block and for polyhedral meshes.
+
+- `block_mesh_points`: the points blockMesh generates for one straight-edged hex block
+  with uniform grading (lineDivide divisions i/N, the 12-edge weighted interpolation of
+  block::createPoints, the `scale` factor), as written to constant/polyMesh/points with
+  10 significant digits (blockMesh raises the write precision to at least 10) and read
+  back.
+- `hex_block_topology`: the faces of that block in polyMesh order
+  (polyMeshFromShapeMesh.C setTopology): internal faces cell by cell, each cell's faces
+  towards higher-numbered neighbours sorted by neighbour, with the owner's hex-model
+  point order; then the boundary faces patch by patch in the order the blockMeshDict
+  lists the block faces, each with the inside cell's face point order.
+- `foam_geometry`: primitiveMesh face centres/areas (triangle fan around the point
+  average, primitiveMeshFaceCentresAndAreas.C) and cell centres/volumes (pyramids on the
+  faces around the average of the face centres, primitiveMeshCellCentresAndVols.C), with
+  every sum taken in OpenFOAM's face order, so the centres carry the same rounding.  The
+  box stimulus test of myocardiumDomain.C:307-342 is inclusive, and stimulus boxes whose
+  edges sit on cell-centre planes select cells by the last bit of these centres.
+- `block`: a hex block (lattice points optionally moved by a map) in the form
+  cg_mesh_create_general takes.
+"""
 import numpy as np
+
+# block faces of the hex cell model (OpenFOAM's cellModels, hex): x-min, x-max, y-min,
+# y-max, z-min, z-max as local vertex lists of the hex (0 4 7 3) (1 2 6 5) (0 1 5 4)
+# (3 7 6 2) (0 3 2 1) (4 5 6 7); local vertex v of cell (i, j, k) sits at
+# (i + di, j + dj, k + dk) with these offsets
+_HEX_OFF = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                     (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+HEX_FACES = ((0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (3, 7, 6, 2), (0, 3, 2, 1), (4, 5, 6, 7))
+SIDES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
+
+
+def written_points(v, digits=10):
+    """values written as ASCII with `digits` significant digits and parsed back"""
+    v = np.asarray(v, dtype=np.float64)
+    u, inv = np.unique(v, return_inverse=True)
+    r = np.array([float("%.*g" % (digits, x)) for x in u])
+    return r[inv].reshape(v.shape)
+
+
+def block_mesh_points(lo, hi, n, scale=1.0, digits=10):
+    """[nz+1, ny+1, nx+1, 3] points of the box block with vertices lo..hi (blockMeshDict
+    units), n = (nx, ny, nz) cells, simpleGrading (1 1 1), multiplied by `scale` and
+    rounded through `digits`-digit text (None keeps blockMesh's doubles)."""
+    nx, ny, nz = n
+    x0, y0, z0 = (float(a) for a in lo)
+    x1, y1, z1 = (float(a) for a in hi)
+    v = np.array([(x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0),
+                  (x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)])
+    lam = [np.arange(m + 1) / m for m in (nx, ny, nz)]   # lineDivide: i/N
+    K, J, I = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1),
+                          indexing="ij")
+    wi, wj, wk = lam[0][I][..., None], lam[1][J][..., None], lam[2][K][..., None]
+    # straight edges: 0-3 along x, 4-7 along y, 8-11 along z (blockDescriptor numbering)
+    ex = [v[0] + (v[1] - v[0]) * wi, v[3] + (v[2] - v[3]) * wi,
+          v[7] + (v[6] - v[7]) * wi, v[4] + (v[5] - v[4]) * wi]
+    ey = [v[0] + (v[3] - v[0]) * wj, v[1] + (v[2] - v[1]) * wj,
+          v[5] + (v[6] - v[5]) * wj, v[4] + (v[7] - v[4]) * wj]
+    ez = [v[0] + (v[4] - v[0]) * wk, v[1] + (v[5] - v[1]) * wk,
+          v[2] + (v[6] - v[2]) * wk, v[3] + (v[7] - v[3]) * wk]
+    # uniform grading: every edge of a direction has the same divisions
+    a, b, c = wi, wj, wk
+    wx = [(1 - a) * (1 - b) * (1 - c) + a * (1 - b) * (1 - c),
+          (1 - a) * b * (1 - c) + a * b * (1 - c),
+          (1 - a) * b * c + a * b * c,
+          (1 - a) * (1 - b) * c + a * (1 - b) * c]
+    wy = [(1 - b) * (1 - a) * (1 - c) + b * (1 - a) * (1 - c),
+          (1 - b) * a * (1 - c) + b * a * (1 - c),
+          (1 - b) * a * c + b * a * c,
+          (1 - b) * (1 - a) * c + b * (1 - a) * c]
+    wz = [(1 - c) * (1 - a) * (1 - b) + c * (1 - a) * (1 - b),
+          (1 - c) * a * (1 - b) + c * a * (1 - b),
+          (1 - c) * a * b + c * a * b,
+          (1 - c) * (1 - a) * b + c * (1 - a) * b]
+    for w in (wx, wy, wz):
+        s = w[0] + w[1] + w[2] + w[3]
+        for q in range(4):
+            w[q] = w[q] / s
+    p = wx[0] * ex[0] + wx[1] * ex[1] + wx[2] * ex[2] + wx[3] * ex[3]
+    for q in range(4):
+        p = p + wy[q] * ey[q]
+    for q in range(4):
+        p = p + wz[q] * ez[q]
+    p = p / 3
+    # the block's own vertices are copied, not interpolated
+    for vi, (di, dj, dk) in enumerate(_HEX_OFF):
+        p[dk * nz, dj * ny, di * nx] = v[vi]
+    p = p * scale
+    return written_points(p, digits) if digits else p
+
+
+def hex_block_topology(nx, ny, nz, patches=(SIDES,)):
+    """Faces of an nx*ny*nz block in polyMesh order.  patches: per boundary patch, the
+    block sides it lists, in blockMeshDict order.  Returns (faces [F, 4] point labels
+    i + (nx+1)(j + (ny+1) k), owner [F], neighbour [Fi], patch_start list)."""
+    n = nx * ny * nz
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    c = np.arange(n)
+
+    def lab(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    def quad(side, i, j, k):
+        return np.stack([lab(i + _HEX_OFF[v][0], j + _HEX_OFF[v][1], k + _HEX_OFF[v][2])
+                         for v in HEX_FACES[side]], axis=1)
+
+    # internal faces: per owner, towards +x (c+1), +y (c+nx), +z (c+nx*ny), i.e. sorted by
+    # neighbour; the owner's faces 1, 3, 5 of the hex model
+    own, nbr, fq = [], [], []
+    for side, ok, step in ((1, I < nx - 1, 1), (3, J < ny - 1, nx), (5, K < nz - 1, nx * ny)):
+        s = np.nonzero(ok)[0]
+        own.append(c[s])
+        nbr.append(c[s] + step)
+        fq.append(quad(side, I[s], J[s], K[s]))
+    own, nbr, fq = np.concatenate(own), np.concatenate(nbr), np.concatenate(fq)
+    order = np.lexsort((nbr, own))
+    own, nbr, fq = own[order], nbr[order], fq[order]
+    bown, bq, starts = [], [], []
+    nb = len(own)
+    for sides in patches:
+        starts.append(nb)
+        for name in sides:
+            side = SIDES.index(name)
+            d, end = divmod(side, 2)
+            sel = np.nonzero((I, J, K)[d] == (0 if end == 0 else (nx, ny, nz)[d] - 1))[0]
+            bown.append(c[sel])
+            bq.append(quad(side, I[sel], J[sel], K[sel]))
+            nb += len(sel)
+    faces = np.concatenate([fq] + bq)
+    owner = np.concatenate([own] + bown)
+    return faces, owner.astype(np.int64), nbr.astype(np.int64), starts
+
+
+def _cross(a, b):
+    """OpenFOAM's Vector ^ on component tuples"""
+    return (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+
+
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
 
 
 def face_geometry(P):
-    """P [F, k, 3] face points in order -> (Cf [F, 3], Sf [F, 3])"""
-    est = P.mean(axis=1)
-    sumN = np.zeros_like(est)
-    sumA = np.zeros(len(P))
-    sumAc = np.zeros_like(est)
+    """P [F, k, 3] face points in order -> (Cf [F, 3], Sf [F, 3]) as
+    primitiveMeshFaceCentresAndAreas.C: point average, then the area-weighted triangle
+    fan (centre (1/3) sum(a c) / sum(a), area 0.5 sum(n))."""
     k = P.shape[1]
+    pts = [tuple(np.ascontiguousarray(P[:, i, d]) for d in range(3)) for i in range(k)]
+    est = list(pts[0])
+    for i in range(1, k):
+        est = [est[d] + pts[i][d] for d in range(3)]
+    est = [e / k for e in est]
+    sumN = [0.0, 0.0, 0.0]
+    sumA = 0.0
+    sumAc = [0.0, 0.0, 0.0]
     for i in range(k):
-        a, b = P[:, i], P[:, (i + 1) % k]
-        nv = np.cross(b - a, est - a)
-        ar = np.linalg.norm(nv, axis=1)
-        sumN += nv
-        sumA += ar
-        sumAc += ar[:, None] * (a + b + est)
-    return sumAc / (3.0 * sumA[:, None]), 0.5 * sumN
+        a, b = pts[i], pts[(i + 1) % k]
+        c = [(a[d] + b[d]) + est[d] for d in range(3)]
+        nv = _cross([b[d] - a[d] for d in range(3)], [est[d] - a[d] for d in range(3)])
+        ar = np.sqrt(_dot(nv, nv))
+        sumN = [sumN[d] + nv[d] for d in range(3)]
+        sumA = sumA + ar
+        sumAc = [sumAc[d] + ar * c[d] for d in range(3)]
+    Cf = np.stack([((1.0 / 3.0) * sumAc[d]) / sumA for d in range(3)], axis=1)
+    Sf = np.stack([0.5 * sumN[d] for d in range(3)], axis=1)
+    return Cf, Sf
+
+
+def _ranked(n, owner, neighbour):
+    """per-cell sequence of (face, sign) in primitiveMesh's loop order: the faces the cell
+    owns in face order, then the internal faces it neighbours in face order.  Returns a
+    list over rank r of (cells, faces, is_neighbour) with each cell at most once."""
+    nf = len(owner)
+    cells = np.concatenate([owner, neighbour])
+    o = np.argsort(cells, kind="stable")   # keeps owner-before-neighbour and face order
+    face = np.where(o < nf, o, o - nf)
+    role = o >= nf
+    counts = np.bincount(cells, minlength=n)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    out = []
+    for r in range(int(counts.max()) if n else 0):
+        sel = np.nonzero(counts > r)[0]
+        idx = starts[sel] + r
+        out.append((sel, face[idx], role[idx]))
+    return out
+
+
+def foam_geometry(points, faces, owner, neighbour, n_cells):
+    """primitiveMesh geometry of a mesh given as points [P, 3], faces [F, k] (internal
+    first), owner [F], neighbour [Fi]: (Cf, Sf, C, V)."""
+    Cf, Sf = face_geometry(points[faces])
+    cf = [np.ascontiguousarray(Cf[:, d]) for d in range(3)]
+    sf = [np.ascontiguousarray(Sf[:, d]) for d in range(3)]
+    seq = _ranked(n_cells, owner, neighbour)
+    est = [np.zeros(n_cells) for _ in range(3)]
+    cnt = np.zeros(n_cells)
+    for cells, fcs, _ in seq:
+        for d in range(3):
+            est[d][cells] = est[d][cells] + cf[d][fcs]
+        cnt[cells] += 1
+    est = [e / cnt for e in est]
+    ctr = [np.zeros(n_cells) for _ in range(3)]
+    vol = np.zeros(n_cells)
+    for cells, fcs, nb in seq:
+        fc = [cf[d][fcs] for d in range(3)]
+        ce = [est[d][cells] for d in range(3)]
+        dvec = [np.where(nb, ce[d] - fc[d], fc[d] - ce[d]) for d in range(3)]
+        pv = _dot([sf[d][fcs] for d in range(3)], dvec)
+        for d in range(3):
+            ctr[d][cells] = ctr[d][cells] + pv * (0.75 * fc[d] + 0.25 * ce[d])
+        vol[cells] = vol[cells] + pv
+    C = np.stack([ctr[d] / vol for d in range(3)], axis=1)
+    return Cf, Sf, C, vol * (1.0 / 3.0)
+
+
+def block_cell_centres(lo, hi, n, scale=1.0, patches=(SIDES,), digits=10):
+    """cell centres [n, 3] of a blockMesh box block as OpenFOAM's mesh().C() holds them"""
+    pts = block_mesh_points(lo, hi, n, scale, digits).reshape(-1, 3)
+    faces, owner, nbr, _ = hex_block_topology(*n, patches=patches)
+    return foam_geometry(pts, faces, owner, nbr, n[0] * n[1] * n[2])[2]
 
 
 def block(nx, ny, nz, lengths=(1.0, 1.0, 1.0), warp=None):
@@ -34,77 +231,14 @@ def block(nx, ny, nz, lengths=(1.0, 1.0, 1.0), warp=None):
     Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
     if warp is not None:
         X, Y, Z = warp(X, Y, Z)
-    pts = np.stack([X, Y, Z], axis=-1)          # [nz+1, ny+1, nx+1, 3]
+    pts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3)
     n = nx * ny * nz
-    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    I, J, K = I.ravel(), J.ravel(), K.ravel()
-    c = I + nx * (J + ny * K)
-
-    def P(i, j, k):
-        return pts[k, j, i]
-
-    # internal faces of each cell towards +x, +y, +z, in the order owner, then neighbour
-    faces = []
-    for d, (ok, nb, quad) in enumerate((
-            (I < nx - 1, c + 1, lambda i, j, k: [P(i + 1, j, k), P(i + 1, j + 1, k),
-                                                  P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1)]),
-            (J < ny - 1, c + nx, lambda i, j, k: [P(i, j + 1, k), P(i, j + 1, k + 1),
-                                                   P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k)]),
-            (K < nz - 1, c + nx * ny, lambda i, j, k: [P(i, j, k + 1), P(i + 1, j, k + 1),
-                                                        P(i + 1, j + 1, k + 1),
-                                                        P(i, j + 1, k + 1)]))):
-        sel = np.nonzero(ok)[0]
-        faces.append((c[sel], nb[sel], d, np.stack(quad(I[sel], J[sel], K[sel]), axis=1)))
-    owner = np.concatenate([f[0] for f in faces])
-    nbr = np.concatenate([f[1] for f in faces])
-    dirs = np.concatenate([np.full(len(f[0]), f[2]) for f in faces])
-    quads = np.concatenate([f[3] for f in faces])
-    order = np.lexsort((nbr, owner))            # by owner, then neighbour
-    owner, nbr, quads = owner[order], nbr[order], quads[order]
-    del dirs
-    # boundary faces, outward
-    b = []
-    jj, kk = np.meshgrid(np.arange(ny), np.arange(nz), indexing="xy")
-    jj, kk = jj.ravel(), kk.ravel()
-    b.append((0 + nx * (jj + ny * kk), np.stack([P(0, jj, kk), P(0, jj, kk + 1),
-                                                 P(0, jj + 1, kk + 1), P(0, jj + 1, kk)], 1)))
-    b.append((nx - 1 + nx * (jj + ny * kk), np.stack([P(nx, jj, kk), P(nx, jj + 1, kk),
-                                                      P(nx, jj + 1, kk + 1), P(nx, jj, kk + 1)], 1)))
-    ii, kk = np.meshgrid(np.arange(nx), np.arange(nz), indexing="xy")
-    ii, kk = ii.ravel(), kk.ravel()
-    b.append((ii + nx * (0 + ny * kk), np.stack([P(ii, 0, kk), P(ii + 1, 0, kk),
-                                                 P(ii + 1, 0, kk + 1), P(ii, 0, kk + 1)], 1)))
-    b.append((ii + nx * (ny - 1 + ny * kk), np.stack([P(ii, ny, kk), P(ii, ny, kk + 1),
-                                                      P(ii + 1, ny, kk + 1), P(ii + 1, ny, kk)], 1)))
-    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
-    ii, jj = ii.ravel(), jj.ravel()
-    b.append((ii + nx * jj, np.stack([P(ii, jj, 0), P(ii, jj + 1, 0), P(ii + 1, jj + 1, 0),
-                                      P(ii + 1, jj, 0)], 1)))
-    b.append((ii + nx * (jj + ny * (nz - 1)), np.stack([P(ii, jj, nz), P(ii + 1, jj, nz),
-                                                        P(ii + 1, jj + 1, nz),
-                                                        P(ii, jj + 1, nz)], 1)))
-    bown = np.concatenate([x[0] for x in b])
-    bquads = np.concatenate([x[1] for x in b])
-    Cf, Sf = face_geometry(quads)
-    bCf, bSf = face_geometry(bquads)
-    cnt = np.zeros(n)
-    est = np.zeros((n, 3))
-    for o, cf in ((owner, Cf), (nbr, Cf), (bown, bCf)):
-        for q in range(3):
-            est[:, q] += np.bincount(o, weights=cf[:, q], minlength=n)
-        cnt += np.bincount(o, minlength=n)
-    est /= cnt[:, None]
-    vol3 = np.zeros(n)
-    ctr = np.zeros((n, 3))
-    for o, cf, sf, sgn in ((owner, Cf, Sf, 1.0), (nbr, Cf, Sf, -1.0), (bown, bCf, bSf, 1.0)):
-        pv = sgn * np.einsum("ij,ij->i", sf, cf - est[o])
-        pc = 0.75 * cf + 0.25 * est[o]
-        vol3 += np.bincount(o, weights=pv, minlength=n)
-        for q in range(3):
-            ctr[:, q] += np.bincount(o, weights=pv * pc[:, q], minlength=n)
-    return dict(n=n, owner=owner.astype(np.int32), neighbour=nbr.astype(np.int32), Sf=Sf,
-                Cf=Cf, C=ctr / vol3[:, None], V=vol3 / 3.0, b_owner=bown.astype(np.int32),
-                b_Sf=bSf, b_Cf=bCf)
+    faces, owner, nbr, _ = hex_block_topology(nx, ny, nz)
+    Cf, Sf, C, V = foam_geometry(pts, faces, owner, nbr, n)
+    ni = len(nbr)
+    return dict(n=n, owner=owner[:ni].astype(np.int32), neighbour=nbr.astype(np.int32),
+                Sf=Sf[:ni], Cf=Cf[:ni], C=C, V=V, b_owner=owner[ni:].astype(np.int32),
+                b_Sf=Sf[ni:], b_Cf=Cf[ni:])
 
 
 def smooth_warp(eps, lengths=(1.0, 1.0, 1.0)):

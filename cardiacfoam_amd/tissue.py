@@ -67,16 +67,77 @@ class _Mesh:
 class HexMesh(_Mesh):
     """Uniform nx*ny*nz hex block, blockMesh cell order c = i + nx*(j + ny*k)."""
 
-    def __init__(self, nx, ny, nz, h):
+    def __init__(self, nx, ny, nz, h, centres=None):
+        """centres: optional [n, 3] cell centres as the fvMesh holds them (mesh().C()); the
+        box stimulus then tests them instead of the (i + 1/2) h lattice."""
         hx, hy, hz = (h, h, h) if np.isscalar(h) else h
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.h = (float(hx), float(hy), float(hz))
         self.n = self.nx * self.ny * self.nz
+        self._centres = None
         m = C.c_void_p()
         check(lib().cg_mesh_create_hex(self.nx, self.ny, self.nz, *self.h, C.byref(m)))
         self.m = m
+        if centres is not None:
+            self.set_cell_centres(centres)
+
+    @classmethod
+    def from_block_mesh_dict(cls, path):
+        """The mesh blockMesh writes for system/blockMeshDict (path to the file or to the
+        case): one straight-edged, axis-aligned hex block with simpleGrading (1 1 1), the
+        form of every tissue tutorial.  Cell centres are those OpenFOAM computes
+        (`meshes.block_cell_centres`), boundary faces in the dictionary's patch order."""
+        import os.path as op
+        from . import foam_dict, meshes
+        if op.isdir(path):
+            path = op.join(path, "system", "blockMeshDict")
+        d = foam_dict.read(path)
+        scale = float(d.get("scale", d.get("convertToMeters", 1.0)))
+        v = np.asarray(d["vertices"], dtype=np.float64)
+        blocks = d["blocks"]
+        if len(blocks) != 5 or blocks[0] != "hex" or blocks[3] != "simpleGrading":
+            raise FatalIOError("%s: the hex path takes one hex block with simpleGrading" % path)
+        labels, n, grading = blocks[1], [int(a) for a in blocks[2]], blocks[4]
+        if any(float(g) != 1.0 for g in np.ravel(grading)) or d.get("edges"):
+            raise FatalIOError("%s: the hex path takes uniform grading and straight edges"
+                               % path)
+        bv = v[np.asarray(labels)]
+        lo, hi = bv[0], bv[6]
+        want = np.array([[lo[0], lo[1], lo[2]], [hi[0], lo[1], lo[2]], [hi[0], hi[1], lo[2]],
+                         [lo[0], hi[1], lo[2]], [lo[0], lo[1], hi[2]], [hi[0], lo[1], hi[2]],
+                         [hi[0], hi[1], hi[2]], [lo[0], hi[1], hi[2]]])
+        if not np.array_equal(bv, want) or np.any(hi <= lo):
+            raise FatalIOError("%s: the block is not an axis-aligned box in hex vertex order"
+                               % path)
+        # boundary patches: each block face listed, matched to its side of the hex model
+        side_of = {frozenset(labels[q] for q in f): meshes.SIDES[s]
+                   for s, f in enumerate(meshes.HEX_FACES)}
+        patches = []
+        bd = d.get("boundary", [])
+        for q in range(0, len(bd), 2):
+            faces = bd[q + 1].get("faces", [])
+            patches.append(tuple(side_of[frozenset(int(a) for a in f)] for f in faces))
+        listed = [s for p in patches for s in p]
+        if sorted(listed) != sorted(meshes.SIDES):
+            # sides not listed go to blockMesh's defaultFaces patch, after the others
+            patches.append(tuple(s for s in meshes.SIDES if s not in listed))
+        if np.any(lo != 0.0):
+            raise FatalIOError("%s: the hex path takes a block with its corner at the origin"
+                               % path)
+        h = tuple((hi[q] - lo[q]) * scale / n[q] for q in range(3))
+        centres = meshes.block_cell_centres(lo, hi, n, scale, patches=tuple(patches))
+        return cls(n[0], n[1], n[2], h, centres=centres)
+
+    def set_cell_centres(self, centres):
+        """The fvMesh's cell centres [n, 3] (m) for the box stimulus test
+        (cg_mesh_set_cell_centres)."""
+        c = np.ascontiguousarray(centres, dtype=np.float64).reshape(self.n, 3)
+        check(lib().cg_mesh_set_cell_centres(self.m, _dp(c)))
+        self._centres = c
 
     def cell_centres(self):
+        if self._centres is not None:
+            return self._centres
         i = np.arange(self.nx)
         j = np.arange(self.ny)
         k = np.arange(self.nz)

@@ -219,6 +219,13 @@ int64_t cg_mesh_ncells(const cg_mesh* m);
 /* Conductivity, row-major 3x3 per cell [9*n] (uniform == 1: one tensor for all cells).
  * Replaces MonodomainSolver::initialiseConductivity (monodomainSolver.C:48-100). */
 int cg_mesh_set_conductivity(cg_mesh* m, const double* sigma9, int uniform);
+/* Cell centres of a hex block as the fvMesh holds them (mesh().C().primitiveField(), m),
+ * [3*n] interleaved x y z.  The box stimulus (myocardiumDomain.C:307-342) then tests these
+ * centres with the reference's inclusive boundBox::contains instead of the (i+1/2)h
+ * lattice; a box edge on a cell-centre plane selects cells by the centres' last bit.  The
+ * membership of each cell is computed once per change of boxes or centres (1 byte per
+ * cell on the device, read only while a box is active).  Hex blocks only. */
+int cg_mesh_set_cell_centres(cg_mesh* m, const double* centres);
 /* maxCo * min_f dx^2/(n.sigma_f.n/(chi Cm)) (myocardiumDomain.C:532-550) */
 int cg_mesh_explicit_dt(const cg_mesh* m, double chi, double cm, double max_co, double* dt);
 

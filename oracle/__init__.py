@@ -41,7 +41,8 @@ class Mesh(C.Structure):
 class Boxes(C.Structure):
     _fields_ = [("n_boxes", C.c_int), ("box_min", C.POINTER(C.c_double)),
                 ("box_max", C.POINTER(C.c_double)), ("t_start", C.POINTER(C.c_double)),
-                ("duration", C.POINTER(C.c_double)), ("intensity", C.POINTER(C.c_double))]
+                ("duration", C.POINTER(C.c_double)), ("intensity", C.POINTER(C.c_double)),
+                ("centres", C.POINTER(C.c_double))]
 
 
 class TissueCfg(C.Structure):
@@ -316,10 +317,18 @@ def ldu_pcg_of(n_cells, lower_addr, upper_addr, diag, upper, b, x0, tol=1e-11, r
     return x, it, r0.value, r1.value
 
 
-def make_boxes(box_min, box_max, t_start, duration, intensity):
+def make_boxes(box_min, box_max, t_start, duration, intensity, centres=None):
+    """centres: optional [n, 3] cell centres (m) for the inclusion test; None uses the
+    block's (i + 1/2) h lattice"""
     arrs = [np.ascontiguousarray(np.ravel(a), dtype=np.float64)
             for a in (box_min, box_max, t_start, duration, intensity)]
-    b = Boxes(len(arrs[2]), *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+    ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
+    if centres is not None:
+        arrs.append(np.ascontiguousarray(np.reshape(centres, -1), dtype=np.float64))
+        ptrs.append(arrs[-1].ctypes.data_as(C.POINTER(C.c_double)))
+    else:
+        ptrs.append(C.POINTER(C.c_double)())
+    b = Boxes(len(arrs[2]), *ptrs)
     b._keep = arrs
     return b
 

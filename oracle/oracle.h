@@ -167,9 +167,12 @@ typedef struct {
     const double* t_start;   /* s */
     const double* duration;  /* s */
     const double* intensity; /* A/m^3 */
+    const double* centres;   /* [3*n_cells] cell centres (m) as the mesh's C(), or null for
+                                the block's (i+1/2)h lattice */
 } orc_boxes_t;
 
-/* source[n] at t0 (myocardiumDomain.C:307-342), cell centres from the mesh */
+/* source[n] at t0 (myocardiumDomain.C:307-342): the boxes whose closed extent contains the
+ * cell centre (boundBox::contains is inclusive) */
 void orc_box_stimulus(const orc_mesh_t* m, const orc_boxes_t* b, double t0, double* src);
 /* activation (myocardiumDomain.C:345-374); flag 1 = still to be computed */
 void orc_activation(long n, const double* V, const double* Vold, uint8_t* flag,

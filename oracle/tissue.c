@@ -380,6 +380,8 @@ void orc_box_stimulus(const orc_mesh_t* m, const orc_boxes_t* b, double t0, doub
         for (long c = 0; c < m->n_cells; ++c) {
             int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
             double p[3] = {(i + 0.5) * m->hx, (j + 0.5) * m->hy, (k + 0.5) * m->hz};
+            if (b->centres)
+                for (int d = 0; d < 3; ++d) p[d] = b->centres[3 * c + d];
             int in = 1;
             for (int d = 0; d < 3; ++d)
                 if (p[d] < b->box_min[3 * s + d] || p[d] > b->box_max[3 * s + d]) in = 0;

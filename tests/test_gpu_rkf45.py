@@ -176,29 +176,18 @@ def test_tissue_rkf45_vs_oracle(gpu, orc, name, alg, ddt):
 
 def test_niederer_rkf45_on_device(gpu, orc):
     """NiedererEtAl2012: the regression case as the reference runs it (RKF45 with
-    OpenFOAM's default tolerances, backward ddt, dt from maxCo 0.1) against the golden
-    probe activation times and tolerances."""
+    OpenFOAM's default tolerances, backward ddt, dt from maxCo 0.1, the box stimulus on
+    the mesh's own centres) against the golden probes and the passing run."""
     _say("niederer rkf45 gpu")
+    from tests.gpu_cases import check_niederer_probes, niederer_domain
     c = cases.NIEDERER
     nx, ny, nz = c["shape"]
-    mesh = gpu.HexMesh(nx, ny, nz, c["h"])
-    b = c["box"]
-    box = {"stimulusLocationMin": b["min"], "stimulusLocationMax": b["max"],
-           "stimulusDuration": b["duration"], "stimulusIntensity": b["intensity"],
-           "stimulusStartTime": 0.0}
-    coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": c["chi"],
-              "cm": c["cm"], "solutionAlgorithm": "explicit", "conductivity": c["sigma9"],
-              "externalStimulus": box, "solver": "RKF45", "maxSteps": 1000000000}
-    dom = gpu.Monodomain(mesh, coeffs, max_co=c["max_co"], ddt="backward")
+    dom = niederer_domain(gpu, solver="RKF45", maxSteps=1000000000)
     steps = 0
     while steps * dom.dt < c["end_time"] - 0.5 * dom.dt:
         steps += 1
     dom.step(steps)
-    ta = dom.activationTime.reshape(nz, ny, nx)
-    for p in c["probes"]:
-        i, j, k = p["cell"]
-        _say("  probe %s: %.7e expected %.7e" % (p["cell"], ta[k, j, i], p["expected"]))
-        assert abs(ta[k, j, i] - p["expected"]) <= p.get("tol_here", p["tol"]), p
+    check_niederer_probes(dom.activationTime.reshape(nz, ny, nx), _say)
 
 
 def test_rkf45_max_steps_error(gpu, orc):
