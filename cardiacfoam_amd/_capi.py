@@ -121,8 +121,8 @@ def lib():
     L.cg_ionic_get_beats.argtypes = [vp, vp, vp]
     L.cg_adjoint_create.argtypes = [vp, C.c_int, vp, C.c_int, C.POINTER(vp)]
     L.cg_adjoint_destroy.argtypes = [vp]
-    L.cg_adjoint_forward.argtypes = [vp, vp, vp, C.c_int, vp]
-    L.cg_adjoint_backward.argtypes = [vp, vp, vp, vp]
+    L.cg_adjoint_forward.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_int64)]
+    L.cg_adjoint_backward.argtypes = [vp, C.c_int64, C.c_int, vp, vp, vp]
     _lib = L
     return L
 

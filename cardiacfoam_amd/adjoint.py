@@ -7,7 +7,10 @@ Euler ddt, BuenoOrovio, FP64, uniform diagonal conductivity) as the start state.
 cells, [n_steps, n_probes], as a differentiable function of the per-cell conductivity
 scale ``kappa`` (CUDA float64 [n_cells]) and ``theta`` = (tau_fi, tau_si) (float64 [2]).
 Any torch loss of the traces can then be back-propagated: the backward pass re-runs
-checkpointed segments and sweeps them in reverse on the device.
+checkpointed segments and sweeps them in reverse on the device.  The device keeps the
+checkpoints of one forward run per handle: a backward pass for an earlier run (a second
+probe_traces call on the same handle before backward) raises instead of returning the
+later run's gradient.
 """
 import ctypes as C
 
@@ -38,21 +41,39 @@ class TissueAdjoint:
                 pass
             self.a = None
 
+    generation = 0      # number of the latest forward run on the device (0: none)
+
     def forward(self, kappa, theta, n_steps, out):
+        """runs the forward map; returns its generation number"""
         import torch
         torch.cuda.current_stream().synchronize()   # kappa written by torch's stream
         th = None
         if theta is not None:
             th_host = np.ascontiguousarray(theta.detach().cpu().numpy(), dtype=np.float64)
             th = th_host.ctypes.data_as(C.c_void_p)
+        gen = C.c_int64()
+        self.generation = -1                        # a failed forward leaves no valid run
         check(lib().cg_adjoint_forward(self.a, C.c_void_p(kappa.data_ptr()), th, int(n_steps),
-                                       C.c_void_p(out.data_ptr())))
+                                       C.c_void_p(out.data_ptr()), C.byref(gen)))
+        self.generation = gen.value
+        return gen.value
 
-    def backward(self, dprobes, dkappa):
+    def check_backward(self, generation):
+        """raises unless `generation` is the latest forward run of this handle"""
+        if generation != self.generation:
+            raise RuntimeError(
+                "TissueAdjoint: backward of forward run %d, but run %d replaced its "
+                "checkpoints on the device; call backward before the next probe_traces on "
+                "the same handle (or use one TissueAdjoint per pending graph)"
+                % (generation, self.generation))
+
+    def backward(self, dprobes, dkappa, generation, n_steps):
         import torch
+        self.check_backward(generation)
         torch.cuda.current_stream().synchronize()
         dth = np.zeros(2)
-        check(lib().cg_adjoint_backward(self.a, C.c_void_p(dprobes.data_ptr()),
+        check(lib().cg_adjoint_backward(self.a, int(generation), int(n_steps),
+                                        C.c_void_p(dprobes.data_ptr()),
                                         C.c_void_p(dkappa.data_ptr()),
                                         dth.ctypes.data_as(C.c_void_p)))
         return dth
@@ -68,7 +89,8 @@ def _function():
                 raise ValueError("kappa must be a CUDA float64 tensor of n_cells values")
             kappa = kappa.contiguous()
             out = torch.empty(n_steps, adj.n_probes, dtype=torch.float64, device=kappa.device)
-            adj.forward(kappa, theta, n_steps, out)
+            ctx.generation = adj.forward(kappa, theta, n_steps, out)
+            ctx.n_steps = n_steps
             ctx.adj = adj
             ctx.theta_dev = None if theta is None else theta.device
             return out
@@ -76,9 +98,10 @@ def _function():
         @staticmethod
         def backward(ctx, grad_out):
             adj = ctx.adj
+            adj.check_backward(ctx.generation)
             g = grad_out.contiguous().to(torch.float64)
             dkappa = torch.empty(adj.n, dtype=torch.float64, device=g.device)
-            dth = adj.backward(g, dkappa)
+            dth = adj.backward(g, dkappa, ctx.generation, ctx.n_steps)
             dtheta = None
             if ctx.theta_dev is not None:
                 dtheta = torch.tensor(dth, dtype=torch.float64, device=ctx.theta_dev)

@@ -1709,6 +1709,8 @@ struct cg_adjoint {
     cg::Adjoint* A;
     double t0;  // tissue time at creation
     int n_probes;
+    int64_t generation = 0;  // number of the latest forward run (0: none yet)
+    int last_steps = 0;      // its step count
 };
 
 int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
@@ -1763,10 +1765,13 @@ int cg_adjoint_destroy(cg_adjoint* a)
 }
 
 int cg_adjoint_forward(cg_adjoint* a, const double* kappa_dev, const double* theta, int n_steps,
-                       double* probes_dev)
+                       double* probes_dev, int64_t* generation)
 {
     if (!a || !kappa_dev || !probes_dev) return fail(CG_ERR_ARG, "NULL argument");
     if (n_steps < 1) return fail(CG_ERR_ARG, "n_steps must be positive");
+    // whatever happens below, the previous run's checkpoints are no longer valid
+    ++a->generation;
+    a->last_steps = 0;
     // start time of each step, with the tissue's time accumulation
     std::vector<double> t0((size_t)n_steps);
     double tt = a->t0;
@@ -1781,13 +1786,22 @@ int cg_adjoint_forward(cg_adjoint* a, const double* kappa_dev, const double* the
     const int rc = cg::adjoint_forward(a->A, kappa_dev, theta, idx, theta ? 2 : 0, n_steps, t0,
                                        probes_dev, budget, &err);
     if (rc) return fail(rc, "%s", err.c_str());
+    a->last_steps = n_steps;
+    if (generation) *generation = a->generation;
     return CG_OK;
 }
 
-int cg_adjoint_backward(cg_adjoint* a, const double* dprobes_dev, double* dkappa_dev,
-                        double* dtheta)
+int cg_adjoint_backward(cg_adjoint* a, int64_t generation, int n_steps,
+                        const double* dprobes_dev, double* dkappa_dev, double* dtheta)
 {
     if (!a || !dprobes_dev || !dkappa_dev || !dtheta) return fail(CG_ERR_ARG, "NULL argument");
+    if (a->last_steps == 0 || generation != a->generation)
+        return fail(CG_ERR_STATE, "backward of forward run %lld, but the latest completed run "
+                    "is %lld: its checkpoints were replaced", (long long)generation,
+                    a->last_steps ? (long long)a->generation : 0LL);
+    if (n_steps != a->last_steps)
+        return fail(CG_ERR_STATE, "backward over %d steps of a forward run of %d steps", n_steps,
+                    a->last_steps);
     std::string err;
     const int rc = cg::adjoint_backward(a->A, dprobes_dev, dkappa_dev, dtheta, &err);
     if (rc) return fail(rc, "%s", err.c_str());

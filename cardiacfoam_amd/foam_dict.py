@@ -11,8 +11,9 @@ followed by a dict).
 """
 import re
 
-_TOKEN = re.compile(r'\s*(?:(//[^\n]*)|(/\*.*?\*/)|("(?:[^"\\]|\\.)*")|([{}();\[\]])|([^\s{}();\[\]"]+))',
-                    re.S)
+_TOKEN = re.compile(r'\s*(?:(//[^\n]*)|(/\*.*?\*/)|("(?:[^"\\]|\\.)*")|'
+                    r'([A-Za-z_][\w.:]*\([^()\s;{}]*\))|'   # a term name: ddt(Vm), grad(Vm)
+                    r'([{}();\[\]])|([^\s{}();\[\]"]+))', re.S)
 
 
 def _tokens(text):
@@ -25,7 +26,7 @@ def _tokens(text):
         pos = m.end()
         if m.group(1) or m.group(2):
             continue
-        tok = m.group(3) or m.group(4) or m.group(5)
+        tok = m.group(3) or m.group(4) or m.group(5) or m.group(6)
         if tok:
             out.append(tok)
     return out

@@ -200,15 +200,87 @@ def fv_solution_controls(fv_solution, field="Vm"):
             "nOuterCorrectors": int(pimple.get("nOuterCorrectors", 1)), "residualControl": rc}
 
 
+def _scheme_entry(table, name, kind):
+    """fvSchemes lookup as fvSchemes::ddtScheme / gradScheme / laplacianScheme do: the
+    term's own entry (a key or a quoted regular expression), else `default` unless that
+    is `none`."""
+    import re
+    table = table or {}
+    for key, v in table.items():
+        if key == name or (key.startswith('"') and re.fullmatch(key.strip('"'), name)):
+            return v
+    d = table.get("default")
+    if d is None or d == "none":
+        raise FatalIOError("keyword %s is undefined in dictionary %sSchemes" % (name, kind))
+    return d
+
+
+def _words(v):
+    return [str(a) for a in (v if isinstance(v, list) else [v])]
+
+
+def fv_schemes_selection(fv_schemes, general_mesh=False):
+    """The schemes system/fvSchemes selects for the monodomain equation
+    chi Cm fvm::ddt(Vm) == fvc/fvm::laplacian(conductivity, Vm) - chi Cm Iion + source
+    (monodomainSolver.C:103-150): {"ddt": "Euler" | "backward", "grad": ..., "laplacian":
+    ...}.  Raises FatalIOError for what the device path does not implement, rather than
+    substituting another scheme:
+    - ddt(Vm): Euler or backward (BDF2 with an Euler first step);
+    - grad(Vm) (the tensor correction's gradient): Gauss linear, or leastSquares on the
+      hex block, where with zeroGradient walls it equals Gauss linear (both give the
+      central difference, the wall cells the one-sided half difference);
+    - laplacian(conductivity,Vm): Gauss linear corrected (uncorrected and orthogonal are
+      the same operator on the orthogonal hex block);
+    - interpolate(conductivity): linear."""
+    ddt = " ".join(_words(_scheme_entry(fv_schemes.get("ddtSchemes"), "ddt(Vm)", "ddt")))
+    if ddt not in ("Euler", "backward"):
+        raise FatalIOError("ddt scheme '%s' for Vm is not implemented by the device path "
+                           "(Euler, backward)" % ddt)
+    grad = " ".join(_words(_scheme_entry(fv_schemes.get("gradSchemes"), "grad(Vm)", "grad")))
+    ok_grad = ["Gauss linear"] + ([] if general_mesh else ["leastSquares"])
+    if grad not in ok_grad:
+        raise FatalIOError("grad scheme '%s' for Vm is not implemented by the device path on "
+                           "this mesh (%s)" % (grad, ", ".join(ok_grad)))
+    lap = " ".join(_words(_scheme_entry(fv_schemes.get("laplacianSchemes"),
+                                        "laplacian(conductivity,Vm)", "laplacian")))
+    ok_lap = ["Gauss linear corrected"] + ([] if general_mesh else
+                                           ["Gauss linear uncorrected",
+                                            "Gauss linear orthogonal"])
+    if lap not in ok_lap:
+        raise FatalIOError("laplacian scheme '%s' is not implemented by the device path on "
+                           "this mesh (%s)" % (lap, ", ".join(ok_lap)))
+    interp = " ".join(_words(_scheme_entry(fv_schemes.get("interpolationSchemes"),
+                                           "interpolate(conductivity)", "interpolation")))
+    if interp != "linear":
+        raise FatalIOError("interpolation scheme '%s' is not implemented (linear)" % interp)
+    return {"ddt": ddt, "grad": grad, "laplacian": lap, "interpolation": interp}
+
+
 class Monodomain:
     """Device monodomain tissue: MyocardiumDomain state + MonodomainSolver operator."""
 
     def __init__(self, mesh, coeffs, dt=None, max_co=None, precision=None, tol=1e-8,
-                 max_iter=2000, ddt=None, vm0=-0.084, case_dir=None, start_time="0"):
-        """case_dir: an OpenFOAM case whose <start_time>/ directory may hold the fields the
-        reference reads if present: conductivity (monodomainSolver.C:54-92, taking
-        precedence over the dictionary's value), Vm and activationTime."""
+                 max_iter=2000, ddt=None, vm0=-0.084, case_dir=None, start_time="0",
+                 fv_schemes=None):
+        """case_dir: an OpenFOAM case.  Its system/fvSchemes selects the schemes
+        (fv_schemes_selection; a parsed dictionary may be passed as fv_schemes instead),
+        and its <start_time>/ directory may hold the fields the reference reads if
+        present: conductivity (monodomainSolver.C:54-92, taking precedence over the
+        dictionary's value), Vm and activationTime.  Without either, ddt ("Euler" or
+        "backward") is the caller's choice, Euler when not given."""
         self.mesh = mesh
+        if fv_schemes is None and case_dir:
+            from . import foam_dict
+            path = os.path.join(case_dir, "system", "fvSchemes")
+            if not os.path.exists(path):
+                raise FatalIOError("cannot find file %s" % path)
+            fv_schemes = foam_dict.read(path)
+        self.schemes = (fv_schemes_selection(fv_schemes, isinstance(mesh, PolyMesh))
+                        if fv_schemes is not None else None)
+        if ddt is None:
+            ddt = self.schemes["ddt"] if self.schemes else "Euler"
+        if ddt not in ("Euler", "backward"):
+            raise FatalIOError("ddt scheme '%s' is not implemented (Euler, backward)" % ddt)
         self.coeffs = coeffs
         cpath = os.path.join(case_dir, start_time, "conductivity") if case_dir else None
         if not (cpath and mesh.read_conductivity(cpath)):
@@ -232,10 +304,10 @@ class Monodomain:
         if coeffs.get("singleCellStimulus"):
             raise FatalIOError("Detected active ionic-model stimulus protocol while using a "
                                "reaction-diffusion myocardium domain.")
-        ddt_name = ddt or coeffs.get("ddtScheme", "Euler")
+        self.ddt = ddt
         cfg = _capi.TissueConfig(
             _capi.EXPLICIT if alg == "explicit" else _capi.IMPLICIT,
-            _capi.DDT_BACKWARD if ddt_name == "backward" else _capi.DDT_EULER,
+            _capi.DDT_BACKWARD if ddt == "backward" else _capi.DDT_EULER,
             self.chi, self.cm, self.dt, vm0, tol, max_iter)
         t = C.c_void_p()
         check(lib().cg_tissue_create(mesh.m, self.ionic.h, C.byref(cfg), C.byref(t)))

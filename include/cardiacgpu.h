@@ -359,14 +359,19 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
 int cg_adjoint_destroy(cg_adjoint* a);
 /* From the snapshot: kappa_dev [n_cells] (device), theta = {tau_fi, tau_si} (host, NULL
  * keeps the model's values), Vm (V) after each step at the probes into probes_dev
- * [n_steps][n_probes] (device).  Synchronous.  The tissue handle itself is not advanced. */
+ * [n_steps][n_probes] (device).  Synchronous.  The tissue handle itself is not advanced.
+ * *generation (may be NULL) receives this run's number: every forward run of the handle
+ * gets a new one, and only the latest can be differentiated. */
 int cg_adjoint_forward(cg_adjoint* a, const double* kappa_dev, const double* theta, int n_steps,
-                       double* probes_dev);
-/* dLoss/dprobes (device [n_steps][n_probes], same layout) of the last forward run ->
- * dLoss/dkappa (device [n_cells]) and dLoss/d{tau_fi, tau_si} (host [2]).  Re-runs each
- * checkpoint segment, then sweeps it in reverse.  Synchronous. */
-int cg_adjoint_backward(cg_adjoint* a, const double* dprobes_dev, double* dkappa_dev,
-                        double* dtheta);
+                       double* probes_dev, int64_t* generation);
+/* dLoss/dprobes (device [n_steps][n_probes], same layout) of the forward run `generation`
+ * with n_steps steps -> dLoss/dkappa (device [n_cells]) and dLoss/d{tau_fi, tau_si} (host
+ * [2]).  Re-runs each checkpoint segment, then sweeps it in reverse.  Synchronous.  Returns
+ * CG_ERR_STATE, touching nothing, when that run is not the handle's latest forward run
+ * (its checkpoints, face coefficients and constants have been replaced) or n_steps
+ * differs from it. */
+int cg_adjoint_backward(cg_adjoint* a, int64_t generation, int n_steps,
+                        const double* dprobes_dev, double* dkappa_dev, double* dtheta);
 
 /* y = A x on the device with the implicit tissue's matrix (the fvm::laplacian + fvm::ddt
  * assembly of monodomainSolver.C:129-135, orthogonal part):

@@ -39,6 +39,33 @@ def grad(m, V):
     return g / m["V"][:, None]
 
 
+def ls_grad(m, V):
+    """OpenFOAM's leastSquares gradient (leastSquaresVectors, leastSquaresGrad) with
+    zeroGradient walls, [n, 3]: dd_c = sum over the cell's faces of d d^T / |d|^2 with
+    d = C_N - C_P on internal faces and the wall-normal part of Cf - C_P on walls;
+    grad_P += |d|^-2 dd_P^-1 d (V_N - V_P), grad_N += |d|^-2 dd_N^-1 d (V_N - V_P); a
+    zeroGradient wall adds dd only (its face value is V_P)."""
+    o, nb, C = m["owner"], m["neighbour"], m["C"]
+    n = m["n"]
+    d = C[nb] - C[o]
+    w = 1.0 / np.einsum("ij,ij->i", d, d)
+    dd = np.zeros((n, 3, 3))
+    wdd = w[:, None, None] * d[:, :, None] * d[:, None, :]
+    np.add.at(dd, o, wdd)
+    np.add.at(dd, nb, wdd)
+    bo = m["b_owner"]
+    bn = m["b_Sf"] / np.linalg.norm(m["b_Sf"], axis=1)[:, None]
+    bd = bn * np.einsum("ij,ij->i", bn, m["b_Cf"] - C[bo])[:, None]
+    np.add.at(dd, bo, bd[:, :, None] * bd[:, None, :] /
+              np.einsum("ij,ij->i", bd, bd)[:, None, None])
+    inv = np.linalg.inv(dd)
+    dV = V[nb] - V[o]
+    g = np.zeros((n, 3))
+    np.add.at(g, o, w[:, None] * np.einsum("fij,fj->fi", inv[o], d) * dV[:, None])
+    np.add.at(g, nb, w[:, None] * np.einsum("fij,fj->fi", inv[nb], d) * dV[:, None])
+    return g
+
+
 def _sigma_cells(m, sigma9):
     s = np.asarray(sigma9, dtype=np.float64)
     return np.tile(s.reshape(1, 9), (m["n"], 1)) if s.size == 9 else s.reshape(m["n"], 9)

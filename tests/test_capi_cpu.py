@@ -67,3 +67,17 @@ def test_ionic_new_errors_without_device():
         cf.IonicModel.New({"ionicModel": "Courtemanche", "tissue": "endocardialCells"}, 10, 1e-5)
     with pytest.raises(cf.FatalIOError):
         cf.IonicModel.New({"ionicModel": "TNNP"}, 10, 1e-5)
+
+
+def test_adjoint_stale_backward_check():
+    """The host side of the adjoint refuses a backward for any forward run but the
+    latest (no device needed)."""
+    from cardiacfoam_amd.adjoint import TissueAdjoint
+    adj = TissueAdjoint.__new__(TissueAdjoint)
+    adj.generation = 3
+    adj.check_backward(3)
+    with pytest.raises(RuntimeError, match="replaced"):
+        adj.check_backward(2)
+    adj.generation = -1                       # a forward run that failed
+    with pytest.raises(RuntimeError):
+        adj.check_backward(3)

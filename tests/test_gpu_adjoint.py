@@ -126,3 +126,34 @@ def test_adjoint_store_all_equals_checkpointing(gpu, orc):
         res.append((kappa.grad.cpu().numpy(), theta.grad.numpy()))
     np.testing.assert_allclose(res[0][0], res[1][0], rtol=1e-13, atol=1e-300)
     np.testing.assert_allclose(res[0][1], res[1][1], rtol=1e-13)
+
+
+def test_adjoint_backward_of_a_replaced_forward_raises(gpu, orc):
+    """Two forward runs on one handle before backward: the first run's backward raises
+    (its checkpoints were replaced) instead of returning the second run's gradient; the
+    C-ABI refuses a stale generation or a different step count; the latest run still
+    differentiates."""
+    import torch
+    from cardiacfoam_amd import _capi
+    _say("adjoint stale backward")
+    shape, h = (10, 8, 6), 2e-4
+    mesh, dom = _dom(gpu, shape, h)
+    cells = np.arange(0, mesh.n, 37)
+    adj = gpu.TissueAdjoint(dom, cells, checkpoint_every=16)
+    k1 = torch.ones(mesh.n, dtype=torch.float64, device="cuda", requires_grad=True)
+    k2 = torch.ones(mesh.n, dtype=torch.float64, device="cuda", requires_grad=True)
+    t1 = gpu.probe_traces(adj, k1, None, 40)
+    t2 = gpu.probe_traces(adj, k2, None, 25)
+    with pytest.raises(RuntimeError, match="replaced"):
+        (t1 ** 2).sum().backward()
+    g = torch.zeros(25, cells.size, dtype=torch.float64, device="cuda")
+    dk = torch.empty(mesh.n, dtype=torch.float64, device="cuda")
+    dth = np.zeros(2)
+    L = _capi.lib()
+    import ctypes as C
+    for gen, n in ((adj.generation - 1, 25), (adj.generation, 40)):
+        rc = L.cg_adjoint_backward(adj.a, gen, n, C.c_void_p(g.data_ptr()),
+                                   C.c_void_p(dk.data_ptr()), dth.ctypes.data_as(C.c_void_p))
+        assert rc == 9, (gen, n, rc)          # CG_ERR_STATE
+    (t2 ** 2).sum().backward()
+    assert torch.isfinite(k2.grad).all()

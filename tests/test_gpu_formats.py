@@ -103,7 +103,8 @@ def test_niederer_probe_and_field_files(gpu, orc, tmp_path):
 
 def test_fields_and_conductivity_from_case(gpu, orc, tmp_path):
     """A case directory whose 0/ holds Vm and a per-cell conductivity tensor written at 17
-    digits: the tissue built from it steps exactly like one given the same arrays."""
+    digits (and whose system/fvSchemes selects backward): the tissue built from it steps
+    exactly like one given the same arrays and ddt."""
     _say("fields from case")
     from cardiacfoam_amd import foam_io
     shape, h, dt = (16, 12, 8), 2e-4, 2e-5
@@ -112,6 +113,11 @@ def test_fields_and_conductivity_from_case(gpu, orc, tmp_path):
     vm0 = -0.084 + 1e-3 * np.random.default_rng(5).random(n)
     foam_io.write_field(str(tmp_path / "0" / "conductivity"), sig, precision=17)
     foam_io.write_field(str(tmp_path / "0" / "Vm"), vm0, precision=17)
+    (tmp_path / "system").mkdir()
+    (tmp_path / "system" / "fvSchemes").write_text(
+        "ddtSchemes { default backward; }\ngradSchemes { default leastSquares; }\n"
+        "laplacianSchemes { default Gauss linear corrected; }\n"
+        "interpolationSchemes { default linear; }\n")
     coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": 1.4e5,
               "cm": 0.01, "solutionAlgorithm": "implicit",
               "externalStimulus": {"stimulusLocationMin": [0, 0, 0],
@@ -120,7 +126,9 @@ def test_fields_and_conductivity_from_case(gpu, orc, tmp_path):
                                    "stimulusStartTime": 0.0}}
     a = gpu.Monodomain(gpu.HexMesh(*shape, h), coeffs, dt=dt, tol=1e-12,
                        case_dir=str(tmp_path))
-    b = gpu.Monodomain(gpu.HexMesh(*shape, h), dict(coeffs, conductivity=sig), dt=dt, tol=1e-12)
+    assert a.ddt == "backward"
+    b = gpu.Monodomain(gpu.HexMesh(*shape, h), dict(coeffs, conductivity=sig), dt=dt, tol=1e-12,
+                       ddt="backward")
     b.set_field(gpu._capi.FIELD_VM, vm0)
     b.set_field(gpu._capi.FIELD_VM_OLD, vm0)
     np.testing.assert_array_equal(a.Vm, vm0)
