@@ -80,6 +80,7 @@ def lib():
     L.cg_ionic_step.argtypes = [vp, C.c_double, C.c_double, vp, vp, C.c_int]
     L.cg_ionic_run_cells.argtypes = [vp, C.c_double, C.c_double, C.c_int]
     L.cg_tissue_pseudo_ecg.argtypes = [vp, C.c_int, vp, vp]
+    L.cg_tissue_pseudo_ecg_noalloc.argtypes = [vp, C.c_int, vp, vp]
     L.cg_tissue_diffusion_step.argtypes = [vp, vp, vp, C.c_int]
     L.cg_mesh_create_general.argtypes = [C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp,
                                          C.c_int64, vp, vp, vp]

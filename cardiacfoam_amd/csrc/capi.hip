@@ -1459,45 +1459,52 @@ int cg_tissue_get_field(cg_tissue* t, int which, double* host)
     return CG_OK;
 }
 
-int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions,
-                         double* values)
+// Per-cell conductivity of a non-uniform mesh on the device, component-major ([6][n]
+// symmetric or [9][n]) so that lanes read consecutive cells; made on the first ECG call.
+static int ensure_sigma_dev(cg_mesh* m)
+{
+    if (m->uniform || m->d_sigma9) return CG_OK;
+    const std::vector<double>& s9 = m->h_sigma9;
+    bool sym = true;
+    for (int64_t c = 0; c < m->n && sym; ++c) {
+        const double* q = &s9[9 * c];
+        sym = q[1] == q[3] && q[2] == q[6] && q[5] == q[7];
+    }
+    static const int k6[6] = {0, 1, 2, 4, 5, 8};
+    const int nc = sym ? 6 : 9;
+    std::vector<double> soa((size_t)nc * m->n);
+    for (int q = 0; q < nc; ++q)
+        for (int64_t c = 0; c < m->n; ++c)
+            soa[(size_t)q * m->n + c] = s9[9 * c + (sym ? k6[q] : q)];
+    const size_t bytes = sizeof(double) * soa.size();
+    if (hipMalloc(&m->d_sigma9, bytes) != hipSuccess) {
+        m->d_sigma9 = nullptr;
+        return fail(CG_ERR_NOMEM, "conductivity allocation failed");
+    }
+    HIPCHK(hipMemcpy(m->d_sigma9, soa.data(), bytes, hipMemcpyHostToDevice));
+    m->sigma_sym = sym;
+    return CG_OK;
+}
+
+static int check_ecg_args(const cg_tissue* t, int n_electrodes, const double* positions,
+                          const double* values, int max_electrodes)
 {
     if (!t || (n_electrodes > 0 && (!positions || !values)))
         return fail(CG_ERR_ARG, "NULL argument");
-    if (n_electrodes < 0 || n_electrodes > cg::ECG_MAX_ELECTRODES)
-        return fail(CG_ERR_ARG, "between 0 and %d electrodes", cg::ECG_MAX_ELECTRODES);
-    if (n_electrodes == 0) return CG_OK;
-    cg_mesh* m = t->mesh;
-    hipStream_t s = t->ion->stream;
+    if (n_electrodes < 0 || n_electrodes > max_electrodes)
+        return fail(CG_ERR_ARG, "between 0 and %d electrodes", max_electrodes);
     for (int i = 0; i < 3 * n_electrodes; ++i)
         if (!std::isfinite(positions[i])) return fail(CG_ERR_ARG, "electrode position not finite");
-    if (!m->uniform && !m->d_sigma9) {
-        const std::vector<double>& s9 = m->h_sigma9;
-        bool sym = true;
-        for (int64_t c = 0; c < m->n && sym; ++c) {
-            const double* q = &s9[9 * c];
-            sym = q[1] == q[3] && q[2] == q[6] && q[5] == q[7];
-        }
-        // component-major ([6][n] or [9][n]) so that lanes read consecutive cells
-        static const int k6[6] = {0, 1, 2, 4, 5, 8};
-        const int nc = sym ? 6 : 9;
-        std::vector<double> soa((size_t)nc * m->n);
-        for (int q = 0; q < nc; ++q)
-            for (int64_t c = 0; c < m->n; ++c)
-                soa[(size_t)q * m->n + c] = s9[9 * c + (sym ? k6[q] : q)];
-        const size_t bytes = sizeof(double) * soa.size();
-        if (hipMalloc(&m->d_sigma9, bytes) != hipSuccess)
-            return fail(CG_ERR_NOMEM, "conductivity allocation failed");
-        HIPCHK(hipMemcpy(m->d_sigma9, soa.data(), bytes, hipMemcpyHostToDevice));
-        m->sigma_sym = sym;
-    }
-    const size_t per = 3 + cg::ECG_BLOCKS + 1;
-    if (!t->d_ecg && hipMalloc(&t->d_ecg, sizeof(double) * per * cg::ECG_MAX_ELECTRODES) !=
-                         hipSuccess)
-        return fail(CG_ERR_NOMEM, "ECG buffer allocation failed");
-    double* pos = t->d_ecg;
-    double* out = pos + 3 * cg::ECG_MAX_ELECTRODES;
-    double* partial = out + cg::ECG_MAX_ELECTRODES;
+    return CG_OK;
+}
+
+// The ECG kernels on the tissue's current Vm with the given device buffers: positions
+// are copied into pos, the sums come back through out; synchronous.
+static int run_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions,
+                          double* values, double* pos, double* out, double* partial)
+{
+    cg_mesh* m = t->mesh;
+    hipStream_t s = t->ion->stream;
     HIPCHK(hipStreamSynchronize(s));
     if (int rc = check_ode(t->ion)) return rc;
     HIPCHK(hipMemcpyAsync(pos, positions, sizeof(double) * 3 * n_electrodes,
@@ -1517,6 +1524,44 @@ int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions
                           s));
     HIPCHK(hipStreamSynchronize(s));
     return CG_OK;
+}
+
+int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions,
+                         double* values)
+{
+    if (int rc = check_ecg_args(t, n_electrodes, positions, values, cg::ECG_MAX_ELECTRODES))
+        return rc;
+    if (n_electrodes == 0) return CG_OK;
+    if (int rc = ensure_sigma_dev(t->mesh)) return rc;
+    const size_t per = 3 + cg::ECG_BLOCKS + 1;
+    if (!t->d_ecg && hipMalloc(&t->d_ecg, sizeof(double) * per * cg::ECG_MAX_ELECTRODES) !=
+                         hipSuccess) {
+        t->d_ecg = nullptr;
+        return fail(CG_ERR_NOMEM, "ECG buffer allocation failed");
+    }
+    double* pos = t->d_ecg;
+    double* out = pos + 3 * cg::ECG_MAX_ELECTRODES;
+    double* partial = out + cg::ECG_MAX_ELECTRODES;
+    return run_pseudo_ecg(t, n_electrodes, positions, values, pos, out, partial);
+}
+
+int cg_tissue_pseudo_ecg_noalloc(cg_tissue* t, int n_electrodes, const double* positions,
+                                 double* values)
+{
+    if (int rc = check_ecg_args(t, n_electrodes, positions, values,
+                                cg::ECG_NOALLOC_ELECTRODES))
+        return rc;
+    if (n_electrodes == 0) return CG_OK;
+    if (!t->mesh->uniform && !t->mesh->d_sigma9)
+        return fail(CG_ERR_STATE, "the per-cell conductivity is not on the device yet: one "
+                    "cg_tissue_pseudo_ecg call (which uploads it) must come first");
+    double* scr = nullptr;
+    if (hipError_t e = cg::ecg_scratch(&scr))
+        return fail(CG_ERR_HIP, "scratch address lookup failed: %s", hipGetErrorString(e));
+    double* pos = scr;
+    double* out = pos + 3 * cg::ECG_NOALLOC_ELECTRODES;
+    double* partial = out + cg::ECG_NOALLOC_ELECTRODES;
+    return run_pseudo_ecg(t, n_electrodes, positions, values, pos, out, partial);
 }
 
 int cg_tissue_set_field(cg_tissue* t, int which, const double* host)

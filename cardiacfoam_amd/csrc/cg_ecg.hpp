@@ -9,6 +9,16 @@ namespace cg {
 constexpr int ECG_BLOCKS = 1024;       // most partial sums per electrode
 constexpr int ECG_MAX_ELECTRODES = 256;
 
+// Allocation-free pseudo-ECG (cg_tissue_pseudo_ecg_noalloc): at most this many electrodes,
+// one kernel pass.  Its scratch is a file-scope device array of ecg.hip, one per GPU:
+// [3 * ECG_NOALLOC_ELECTRODES] positions, [ECG_NOALLOC_ELECTRODES] results, then
+// [ECG_NOALLOC_ELECTRODES][ECG_BLOCKS] per-block partial sums.
+constexpr int ECG_NOALLOC_ELECTRODES = 16;
+constexpr int ECG_SCRATCH_DOUBLES = ECG_NOALLOC_ELECTRODES * (3 + 1 + ECG_BLOCKS);
+// device address of the current GPU's scratch (looked up once per device with
+// hipGetSymbolAddress); 0 or a hipError_t
+hipError_t ecg_scratch(double** out);
+
 // blocks of the first kernel for n cells (<= ECG_BLOCKS): partial holds [E][that]
 int ecg_partial_blocks(int64_t n);
 

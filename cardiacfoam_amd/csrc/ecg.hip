@@ -14,10 +14,36 @@
 // adds those in a fixed order, so the result does not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "cg_internal.hpp"
 #include "cg_ecg.hpp"
 
+// Scratch of the allocation-free entry: each GPU holds its own copy until the process
+// ends.  Host code never names it in a launch or copy: it uses the address that
+// hipGetSymbolAddress returns for the current device (the name in host code is only a
+// host-side stand-in).
+static __device__ __align__(256) double scratch[cg::ECG_SCRATCH_DOUBLES];
+
 namespace cg {
+
+hipError_t ecg_scratch(double** out)
+{
+    constexpr int MAX_DEV = 64;
+    static std::mutex mu;
+    static double* ptr[MAX_DEV] = {};
+    int dev = -1;
+    if (hipError_t e = hipGetDevice(&dev)) return e;
+    if (dev < 0 || dev >= MAX_DEV) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        if (hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) return e;
+        ptr[dev] = static_cast<double*>(p);
+    }
+    *out = ptr[dev];
+    return hipSuccess;
+}
 
 namespace {
 

@@ -405,6 +405,17 @@ int cg_tissue_diffusion_step(cg_tissue* t, const double* Iion, const double* sou
  * 1 <= n <= 256.  Synchronous. */
 int cg_tissue_pseudo_ecg(cg_tissue* t, int n_electrodes, const double* positions,
                          double* values);
+/* cg_tissue_pseudo_ecg without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot call hipMalloc (serving loops, harnesses that forbid it).
+ * Its scratch (electrode positions, per-block partial sums, results: 16 x 1028 doubles,
+ * 131,584 B) is a fixed device array of the library, one per GPU, held until the process
+ * ends; every call overwrites it entirely, so nothing carries over between calls.
+ * 1 <= n <= 16 electrodes (CG_ERR_ARG beyond).  A tissue with per-cell conductivity needs
+ * one cg_tissue_pseudo_ecg call first, which uploads that conductivity (CG_ERR_STATE
+ * otherwise).  Calls that overlap on one GPU share the scratch and race: serialise them.
+ * Results equal cg_tissue_pseudo_ecg's bit for bit.  Synchronous. */
+int cg_tissue_pseudo_ecg_noalloc(cg_tissue* t, int n_electrodes, const double* positions,
+                                 double* values);
 
 /* ------------------------------------------------------ on-disk ASCII formats
  * Numbers of OpenFOAM's ASCII field files (the internalField List<scalar> that
