@@ -14,7 +14,7 @@ ERRORS = {1: "ARG", 2: "MODEL", 3: "TISSUE", 4: "HIP", 5: "NOMEM", 6: "NONFINITE
           7: "NOT_CONVERGED", 8: "STIMULUS", 9: "STATE", 10: "ODE"}
 FP64, MIXED = 0, 1
 DEVICE_PTRS = 1
-ODE_RUSH_LARSEN, ODE_RKF45 = 0, 1
+ODE_RUSH_LARSEN, ODE_RKF45, ODE_EULER = 0, 1, 2
 NORM_L2, NORM_OPENFOAM = 0, 1
 EXPLICIT, IMPLICIT = 0, 1
 DDT_EULER, DDT_BACKWARD = 0, 1
@@ -120,6 +120,7 @@ def lib():
     L.cg_batch_autotune.argtypes = [C.POINTER(vp), C.c_int, C.c_double, C.c_int, C.c_char_p,
                                     C.c_size_t]
     L.cg_ionic_get_beats.argtypes = [vp, vp, vp]
+    L.cg_ionic_get_restitution.argtypes = [vp, vp, vp]
     L.cg_adjoint_create.argtypes = [vp, C.c_int, vp, C.c_int, C.POINTER(vp)]
     L.cg_adjoint_destroy.argtypes = [vp]
     L.cg_adjoint_forward.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_int64)]

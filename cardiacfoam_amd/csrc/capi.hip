@@ -278,8 +278,8 @@ namespace {
 int ode_args(cg_ionic* h, double t0_s, double dt_s, cg::OdeArgs* o)
 {
     *o = cg::OdeArgs{};
-    o->rkf = h->ode_kind == CG_ODE_RKF45;
-    if (!o->rkf) return CG_OK;
+    o->method = h->ode_kind;   // CG_ODE_RUSH_LARSEN 0, CG_ODE_RKF45 1, CG_ODE_EULER 2
+    if (!o->method) return CG_OK;
     const bool ap = h->model == cg::MODEL_AP;
     model_step_times(h->model, t0_s, dt_s, &o->t0, &o->t1);
     o->dt_clamp = model_dt(h->model, dt_s);
@@ -306,14 +306,15 @@ int ode_args(cg_ionic* h, double t0_s, double dt_s, cg::OdeArgs* o)
 // handle's synchronisation points
 int check_ode(cg_ionic* h)
 {
-    if (h->ode_kind != CG_ODE_RKF45 || !h->d_fail) return CG_OK;
+    if (h->ode_kind == CG_ODE_RUSH_LARSEN || !h->d_fail) return CG_OK;
     int f = 0;
     HIPCHK(hipMemcpyAsync(&f, h->d_fail, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (!f) return CG_OK;
     HIPCHK(hipMemsetAsync(h->d_fail, 0, sizeof(int), h->stream));
-    return fail(CG_ERR_ODE, "RKF45 integration failed in at least one cell (step-size "
-                            "underflow or more than maxSteps = %lld steps)", h->max_steps);
+    return fail(CG_ERR_ODE, "%s integration failed in at least one cell (step-size "
+                            "underflow or more than maxSteps = %lld steps)",
+                h->ode_kind == CG_ODE_EULER ? "Euler" : "RKF45", h->max_steps);
 }
 
 } // namespace
@@ -481,13 +482,15 @@ int cg_ionic_set_integrator(cg_ionic* h, int kind, double abs_tol, double rel_to
                             int64_t max_steps, double initial_step)
 {
     if (!h) return fail(CG_ERR_ARG, "NULL handle");
-    if (kind != CG_ODE_RUSH_LARSEN && kind != CG_ODE_RKF45)
+    if (kind != CG_ODE_RUSH_LARSEN && kind != CG_ODE_RKF45 && kind != CG_ODE_EULER)
         return fail(CG_ERR_ARG, "unknown ODE solver %d", kind);
-    if (kind == CG_ODE_RKF45 &&
+    const bool adaptive = kind != CG_ODE_RUSH_LARSEN;
+    if (adaptive &&
         (!(abs_tol >= 0.0) || !(rel_tol >= 0.0) || !(abs_tol + rel_tol > 0.0) || max_steps < 1))
-        return fail(CG_ERR_ARG, "RKF45 needs absTol, relTol >= 0 (not both 0) and maxSteps >= 1");
+        return fail(CG_ERR_ARG, "%s needs absTol, relTol >= 0 (not both 0) and maxSteps >= 1",
+                    kind == CG_ODE_EULER ? "Euler" : "RKF45");
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (kind == CG_ODE_RKF45 && !h->d_step) {
+    if (adaptive && !h->d_step) {
         if (hipMalloc(&h->d_step, sizeof(double) * h->n) != hipSuccess ||
             hipMalloc(&h->d_fail, sizeof(int)) != hipSuccess)
             return fail(CG_ERR_NOMEM, "allocation of the per-cell ODE steps failed");
@@ -855,6 +858,33 @@ int cg_ionic_get_beats(cg_ionic* h, int32_t* counts, double* records)
                 r[4] /= per_s;
             }
     }
+    return CG_OK;
+}
+
+int cg_ionic_get_restitution(cg_ionic* h, double* apd_di, uint8_t* flags)
+{
+    if (!h || !apd_di) return fail(CG_ERR_ARG, "NULL argument");
+    if (h->max_beats <= 0 || !h->d_nbeats) return fail(CG_ERR_STATE, "beat detection is off");
+    const size_t n = (size_t)h->n;
+    double* d_out = nullptr;
+    uint8_t* d_flags = nullptr;
+    if (hipMalloc(&d_out, sizeof(double) * 2 * n) != hipSuccess ||
+        (flags && hipMalloc(&d_flags, n * h->max_beats) != hipSuccess)) {
+        hipFree(d_out);
+        return fail(CG_ERR_NOMEM, "restitution output allocation failed");
+    }
+    const double inv_per_s = 1.0 / model_time(h->model, 1.0);
+    hipError_t e = cg::launch_restitution(h->d_beats, h->d_nbeats, h->n, h->max_beats, inv_per_s,
+                                          d_out, d_flags, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(apd_di, d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
+                           h->stream);
+    if (e == hipSuccess && flags)
+        e = hipMemcpyAsync(flags, d_flags, n * h->max_beats, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d_out);
+    hipFree(d_flags);
+    if (e != hipSuccess) return fail(CG_ERR_HIP, "restitution: %s", hipGetErrorString(e));
     return CG_OK;
 }
 
@@ -1267,7 +1297,7 @@ static int general_advance(cg_tissue* t, cg::GeneralMesh* G, int n_steps, bool i
         st.boxes = active_boxes(t, t0);
         st.src = src_dev;
         cg::OdeArgs ode = ode0;
-        if (ode.rkf) model_step_times(ion->model, t0, dt, &ode.t0, &ode.t1);
+        if (ode.method) model_step_times(ion->model, t0, dt, &ode.t0, &ode.t1);
         double* Vn = t->d_V[t->cur];
         double* Vm1 = t->d_V[(t->cur + 2) % 3];
         double* Vnew = t->d_V[(t->cur + 1) % 3];
@@ -1394,7 +1424,7 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
         st.od = m->d_od;
         st.od_mask = m->od_mask;
         st.ode = ode0;
-        if (ode0.rkf) model_step_times(ion->model, t0, dt, &st.ode.t0, &st.ode.t1);
+        if (ode0.method) model_step_times(ion->model, t0, dt, &st.ode.t0, &st.ode.t1);
         return st;
     };
     bool ionic_queued = false;  // this step's ionic launch was queued by the previous one

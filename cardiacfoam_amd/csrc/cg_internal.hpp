@@ -37,10 +37,10 @@ struct HexCoefs {
     const double* fz;
 };
 
-// Per-cell ODE integrator of the ionic step: Rush-Larsen (rkf = 0) or adaptive RKF45
-// (rkf = 1) with the reference's tolerances and per-cell trial step.
+// Per-cell ODE integrator of the ionic step: Rush-Larsen (method 0), or OpenFOAM's
+// adaptive RKF45 (1) or Euler (2) with the reference's tolerances and per-cell trial step.
 struct OdeArgs {
-    int rkf;
+    int method;
     double t0, t1;     // model time at the step start and end (tStart, tEnd)
     double dt_clamp;   // deltaT in model units; a cell's trial step is min(step, dt_clamp)
     double* step;      // [n] per-cell trial step in model time units (RKF45)
@@ -117,6 +117,12 @@ enum {
 enum { BF_TU, BF_TEND, BF_BASE, BF_PEAK, BF_JP, BF_V90, BF_TREP, BF_FLAGS };
 enum { BEAT_OPEN = 1, BEAT_HAS = 2, BEAT_FOUND = 4 };
 constexpr int BEAT_REC = 5;  // t_up, v_base, v_peak, v90, t_repol90 (NaN when not found)
+// per-beat flags of the restitution classification (cg_ionic_get_restitution)
+constexpr unsigned RB_EARLY = 1, RB_VALID_APD = 2, RB_VALID_DI = 4, RB_TRUNCATED = 8;
+// beats [n][max_beats][BEAT_REC] (model units), nbeats [n] -> out [n][2] (APD90, DI in s,
+// NaN when rejected), flags [n][max_beats] (or null); asynchronous on s
+hipError_t launch_restitution(const double* beats, const int* nbeats, int64_t n, int max_beats,
+                              double inv_per_s, double* out, uint8_t* flags, hipStream_t s);
 
 struct BatchArgs {
     int64_t n;             // lanes (= cells)

@@ -60,8 +60,8 @@ CG_DEV void load_model_states(T (&y)[M::NS], T (&s)[M::NS], const T* __restrict_
 }
 
 // Ionic step of cell c in tissue mode with V held at v over the step, on full values in
-// place; returns the total ionic current at the step end.  INTEG 0: Rush-Larsen; 1: RKF45
-// over [o.t0, o.t1] from the cell's kept trial step (BuenoOrovio.C:124-132 and its
+// place; returns the total ionic current at the step end.  INTEG 0: Rush-Larsen; 1: RKF45,
+// 2: OpenFOAM's adaptive Euler, over [o.t0, o.t1] from the cell's kept trial step (BuenoOrovio.C:124-132 and its
 // TNNP/ORd/AP twins), the V terms computed once for all stages.
 template <class M, class T, int INTEG>
 CG_DEV T tissue_advance_full(T v, T (&y)[M::NS], T dt, const T* p, int tissue, const T* gs,
@@ -77,7 +77,7 @@ CG_DEV T tissue_advance_full(T v, T (&y)[M::NS], T dt, const T* p, int tissue, c
         };
         double step = o.step[c];
         if (!(step < o.dt_clamp)) step = o.dt_clamp;
-        const bool ok = rkf45_solve<M::NS>(f, y, o.t0, o.t1, step, T(o.abs_tol), T(o.rel_tol),
+        const bool ok = ode_solve<INTEG, M::NS>(f, y, o.t0, o.t1, step, T(o.abs_tol), T(o.rel_tol),
                                            o.max_steps);
         o.step[c] = step;
         if (!ok) *o.fail = 1;
@@ -94,28 +94,39 @@ CG_DEV T tissue_advance(T v, T (&y)[M::NS], T (&s)[M::NS], T dt, const T* p, int
 {
     if constexpr (INTEG == 0 && offsets_stored<M, T>()) {
         return Ops<M>::tissue_step_off(v, y, s, dt, p, tissue, gs);
-    } else if constexpr (offsets_stored<M, T>()) {
-        // RKF45 on the stored offsets, the rates at ref + s, the norm on the values
-        T ref[M::NS];
+    } else if constexpr (INTEG != 0 && sizeof(T) == 4) {
+        // FP32 adaptive solvers integrate this step's increments d from 0 against the
+        // step-start values (rates at base + d, the norm on those values), then add d to
+        // each stored word once: a sub-step's increment below half an FP32 ulp of the state
+        // would otherwise be rounded away (many short Euler sub-steps leave slow states
+        // frozen), while d resolves it
+        T base[M::NS], d[M::NS];
 #pragma unroll
-        for (int q = 0; q < M::NS; ++q) ref[q] = T(StateRef<M>::at(q));
-        y[0] = v;
-        s[0] = v;
+        for (int q = 0; q < M::NS; ++q) {
+            base[q] = y[q];
+            d[q] = T(0);
+        }
+        base[0] = v;
         const auto qv = Ops<M>::vt(v, p);
-        auto f = [&](double, const T* ss, T* d) __attribute__((always_inline)) {
+        auto f = [&](double, const T* dd, T* r) __attribute__((always_inline)) {
             T yy[M::NS];
 #pragma unroll
-            for (int q = 0; q < M::NS; ++q) yy[q] = ref[q] + ss[q];
-            Ops<M>::deriv(yy, d, p, tissue, gs, false, T(0), qv);
+            for (int q = 0; q < M::NS; ++q) yy[q] = base[q] + dd[q];
+            Ops<M>::deriv(yy, r, p, tissue, gs, false, T(0), qv);
         };
         double step = o.step[c];
         if (!(step < o.dt_clamp)) step = o.dt_clamp;
-        const bool ok = rkf45_solve<M::NS>(f, s, o.t0, o.t1, step, T(o.abs_tol), T(o.rel_tol),
-                                           o.max_steps, ref);
+        const bool ok = ode_solve<INTEG, M::NS>(f, d, o.t0, o.t1, step, T(o.abs_tol),
+                                                T(o.rel_tol), o.max_steps, base);
         o.step[c] = step;
         if (!ok) *o.fail = 1;
+        y[0] = v;
+        s[0] = v;
 #pragma unroll
-        for (int q = 1; q < M::NS; ++q) y[q] = ref[q] + s[q];
+        for (int q = 1; q < M::NS; ++q) {
+            s[q] = s[q] + d[q];
+            y[q] = offsets_stored<M, T>() ? T(StateRef<M>::at(q)) + s[q] : s[q];
+        }
         return Ops<M>::iion(y, p, tissue, gs);
     } else {
         const T I = tissue_advance_full<M, T, INTEG>(v, y, dt, p, tissue, gs, o, c);
@@ -210,16 +221,18 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
         store_states(sw, st, n, i, 0);
         return;
     } else {
-        // offsets_stored: integrate the stored offsets sw, rates at ref + sw, norm on values
-        T ref[M::NS];
-#pragma unroll
-        for (int q = 0; q < M::NS; ++q) ref[q] = T(StateRef<M>::at(q));
-        auto f = [&](double t, const T* ss, T* d) __attribute__((always_inline)) {
+        // FP64: the solver integrates the full values y (the stored words are y).  FP32:
+        // each step integrates its increments d from 0 against the step-start values
+        // (rates at base + d, norm on those values) and adds d to the stored words once,
+        // so sub-step increments below half an ulp of a state still count.
+        constexpr bool f32 = sizeof(T) == 4;
+        T base[M::NS], d[M::NS];
+        auto f = [&](double t, const T* ss, T* r) __attribute__((always_inline)) {
             T yy[M::NS];
 #pragma unroll
-            for (int q = 0; q < M::NS; ++q) yy[q] = offsets_stored<M, T>() ? ref[q] + ss[q] : ss[q];
+            for (int q = 0; q < M::NS; ++q) yy[q] = f32 ? base[q] + ss[q] : ss[q];
             const auto q = Ops<M>::vt(yy[0], p);
-            Ops<M>::deriv(yy, d, p, tissue, gs, true, T(stimulus_current(t, stim)), q);
+            Ops<M>::deriv(yy, r, p, tissue, gs, true, T(stimulus_current(t, stim)), q);
         };
         double step = ode.step[i];
         bool ok = true;
@@ -235,11 +248,25 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
                               : ode.ap ? (ts + ode.dt_s) * 1000.0 / 12.9
                                        : (ts + ode.dt_s) * 1000.0;
             if (!(step < ode.dt_clamp)) step = ode.dt_clamp;
-            ok = rkf45_solve<M::NS>(f, sw, x0, x1, step, T(ode.abs_tol), T(ode.rel_tol),
-                                    ode.max_steps, offsets_stored<M, T>() ? ref : nullptr);
+            if constexpr (f32) {
 #pragma unroll
-            for (int q = 0; q < M::NS; ++q)
-                y[q] = offsets_stored<M, T>() ? ref[q] + sw[q] : sw[q];
+                for (int q = 0; q < M::NS; ++q) {
+                    base[q] = y[q];
+                    d[q] = T(0);
+                }
+                ok = ode_solve<INTEG, M::NS>(f, d, x0, x1, step, T(ode.abs_tol),
+                                             T(ode.rel_tol), ode.max_steps, base);
+#pragma unroll
+                for (int q = 0; q < M::NS; ++q) {
+                    sw[q] = sw[q] + d[q];
+                    y[q] = offsets_stored<M, T>() ? T(StateRef<M>::at(q)) + sw[q] : sw[q];
+                }
+            } else {
+                ok = ode_solve<INTEG, M::NS>(f, y, x0, x1, step, T(ode.abs_tol),
+                                             T(ode.rel_tol), ode.max_steps);
+#pragma unroll
+                for (int q = 0; q < M::NS; ++q) sw[q] = y[q];
+            }
             if (rec) record(s, x1);
         }
         ode.step[i] = step;
@@ -372,14 +399,13 @@ template <class T> struct ModelLaunch {
                              const OdeArgs& ode, hipStream_t s)
     {
         const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
-        if (ode.rkf)
-            hipLaunchKernelGGL((k_ionic_tissue<M, T, 1>), dim3(grid), dim3(MBS), 0, s,
-                               (T*)states, n, Vm, Im, T(dt_model), kparams<T>(P),
-                               (const T*)gscale, tissue, ode);
-        else
-            hipLaunchKernelGGL((k_ionic_tissue<M, T, 0>), dim3(grid), dim3(MBS), 0, s,
-                               (T*)states, n, Vm, Im, T(dt_model), kparams<T>(P),
-                               (const T*)gscale, tissue, ode);
+#define CG_IONIC_TISSUE(I)                                                                 \
+    hipLaunchKernelGGL((k_ionic_tissue<M, T, I>), dim3(grid), dim3(MBS), 0, s, (T*)states, \
+                       n, Vm, Im, T(dt_model), kparams<T>(P), (const T*)gscale, tissue, ode)
+        if (ode.method == 2) CG_IONIC_TISSUE(2);
+        else if (ode.method == 1) CG_IONIC_TISSUE(1);
+        else CG_IONIC_TISSUE(0);
+#undef CG_IONIC_TISSUE
         return hipGetLastError();
     }
     template <class M>
@@ -389,14 +415,14 @@ template <class T> struct ModelLaunch {
                             int64_t tcell, hipStream_t s)
     {
         const unsigned grid = (unsigned)((n + MBS - 1) / MBS);
-        if (ode.rkf)
-            hipLaunchKernelGGL((k_ionic_cells<M, T, 1>), dim3(grid), dim3(MBS), 0, s,
-                               (T*)states, n, t0, T(dt), n_steps, kparams<T>(P),
-                               (const T*)gscale, tissue, stim, ode, trace, tcell);
-        else
-            hipLaunchKernelGGL((k_ionic_cells<M, T, 0>), dim3(grid), dim3(MBS), 0, s,
-                               (T*)states, n, t0, T(dt), n_steps, kparams<T>(P),
-                               (const T*)gscale, tissue, stim, ode, trace, tcell);
+#define CG_IONIC_CELLS(I)                                                                  \
+    hipLaunchKernelGGL((k_ionic_cells<M, T, I>), dim3(grid), dim3(MBS), 0, s, (T*)states,  \
+                       n, t0, T(dt), n_steps, kparams<T>(P), (const T*)gscale, tissue, stim, \
+                       ode, trace, tcell)
+        if (ode.method == 2) CG_IONIC_CELLS(2);
+        else if (ode.method == 1) CG_IONIC_CELLS(1);
+        else CG_IONIC_CELLS(0);
+#undef CG_IONIC_CELLS
         return hipGetLastError();
     }
     template <class M>
@@ -410,7 +436,10 @@ template <class T> struct ModelLaunch {
 #define CG_EXPLICIT_HEX(U, I)                                                              \
     hipLaunchKernelGGL((k_explicit_hex<M, T, U, I>), dim3(grid), dim3(MBS), 0, s, g, a,    \
                        (T*)states, V, Vold, Vnew, Iion, flag, tact, st, kparams<T>(P), tissue)
-        if (st.ode.rkf) {
+        if (st.ode.method == 2) {
+            if (a.fx) CG_EXPLICIT_HEX(false, 2);
+            else CG_EXPLICIT_HEX(true, 2);
+        } else if (st.ode.method == 1) {
             if (a.fx) CG_EXPLICIT_HEX(false, 1);
             else CG_EXPLICIT_HEX(true, 1);
         } else {
@@ -427,12 +456,13 @@ template <class T> struct ModelLaunch {
                                 hipStream_t s)
     {
         const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-        if (st.ode.rkf)
-            hipLaunchKernelGGL((k_ionic_rhs<M, T, 1>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,
-                               (T*)states, Vn, Vm1, x0, Iion, b, st, kparams<T>(P), tissue);
-        else
-            hipLaunchKernelGGL((k_ionic_rhs<M, T, 0>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,
-                               (T*)states, Vn, Vm1, x0, Iion, b, st, kparams<T>(P), tissue);
+#define CG_IONIC_RHS(I)                                                                    \
+    hipLaunchKernelGGL((k_ionic_rhs<M, T, I>), dim3(grid_8(n, MBS)), dim3(MBS), 0, s, g,    \
+                       (T*)states, Vn, Vm1, x0, Iion, b, st, kparams<T>(P), tissue)
+        if (st.ode.method == 2) CG_IONIC_RHS(2);
+        else if (st.ode.method == 1) CG_IONIC_RHS(1);
+        else CG_IONIC_RHS(0);
+#undef CG_IONIC_RHS
         return hipGetLastError();
     }
 };

@@ -13,7 +13,11 @@
 //     err <= (10/0.9)^-5;
 //   - the step that would pass xEnd is truncated to land on it, and the trial size from
 //     before the truncation is handed back (the per-cell `step` the model keeps).
+// OpenFOAM's `Euler` ODESolver (`solver Euler;`) runs under the same controller with a
+// one-stage step: y = y0 + dx dy/dx(x0), and the whole increment dx dy/dx(x0) as the
+// error estimate, so each accepted step changes no state by more than its tolerance.
 // One lane integrates one cell; lanes of a wave run different numbers of sub-steps.
+// METHOD 1: RKF45; 2: Euler.
 #pragma once
 #include "cg_common.hpp"
 
@@ -69,16 +73,36 @@ CG_HD double rkf45_trial(F& f, double x0, const T (&y0)[NS], const T (&d0)[NS], 
     return double(maxErr);
 }
 
+// Euler::solve(x0, y0, dydx0, dx, y): err = dx dydx0, y = y0 + err
+template <int NS, class T>
+CG_HD double euler_trial(const T (&y0)[NS], const T (&d0)[NS], double dxd, T (&y)[NS], T at,
+                         T rt, const T* ref = nullptr)
+{
+    const T dx = T(dxd);
+    T maxErr = T(0);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const T err = dx * d0[i];
+        y[i] = y0[i] + err;
+        const T v0 = ref ? ref[i] + y0[i] : y0[i], v1 = ref ? ref[i] + y[i] : y[i];
+        const T tol = at + rt * (fabs(v0) > fabs(v1) ? fabs(v0) : fabs(v1));
+        const T q = fabs(err) / tol;
+        maxErr = q > maxErr ? q : maxErr;
+    }
+    return double(maxErr);
+}
+
 // one accepted step from x; dxTry in: trial size, out: next trial size
-template <int NS, class T, class F>
-CG_HD bool rkf45_adaptive(F& f, double& x, T (&y)[NS], double& dxTry, T at, T rt,
-                          const T* ref = nullptr)
+template <int METHOD, int NS, class T, class F>
+CG_HD bool adaptive_step(F& f, double& x, T (&y)[NS], double& dxTry, T at, T rt,
+                         const T* ref = nullptr)
 {
     T d0[NS], yt[NS];
     f(x, y, d0);
     double dx = dxTry, err;
     for (;;) {
-        err = rkf45_trial<NS>(f, x, y, d0, dx, yt, at, rt, ref);
+        if constexpr (METHOD == 2) err = euler_trial<NS>(y, d0, dx, yt, at, rt, ref);
+        else err = rkf45_trial<NS>(f, x, y, d0, dx, yt, at, rt, ref);
         if (!(err > 1.0)) break;
         const double scale = 0.9 * pow(err, -0.25);
         dx *= scale > 0.2 ? scale : 0.2;
@@ -96,11 +120,13 @@ CG_HD bool rkf45_adaptive(F& f, double& x, T (&y)[NS], double& dxTry, T at, T rt
     return true;
 }
 
-// integrate y over [xStart, xEnd]; false on step-size underflow or max_steps
-template <int NS, class T, class F>
-CG_HD bool rkf45_solve(F& f, T (&y)[NS], double xStart, double xEnd, double& dxTry, T at,
-                        T rt, long long max_steps, const T* ref = nullptr)
+// integrate y over [xStart, xEnd] (ODESolver::solve); false on step-size underflow or
+// max_steps
+template <int METHOD, int NS, class T, class F>
+CG_HD bool ode_solve(F& f, T (&y)[NS], double xStart, double xEnd, double& dxTry, T at,
+                     T rt, long long max_steps, const T* ref = nullptr)
 {
+    static_assert(METHOD == 1 || METHOD == 2, "adaptive method: 1 RKF45, 2 Euler");
     double x = xStart;
     bool last = false;
     for (long long k = 0; k < max_steps; ++k) {
@@ -109,7 +135,7 @@ CG_HD bool rkf45_solve(F& f, T (&y)[NS], double xStart, double xEnd, double& dxT
             last = true;
             dxTry = xEnd - x;
         }
-        if (!rkf45_adaptive<NS>(f, x, y, dxTry, at, rt, ref)) return false;
+        if (!adaptive_step<METHOD, NS>(f, x, y, dxTry, at, rt, ref)) return false;
         if ((x - xEnd) * (xEnd - xStart) >= 0) {
             if (k > 0 && last) dxTry = dxTry0;
             return true;

@@ -127,12 +127,13 @@ class IonicModel:
             raise ValueError("field size %d/%d != nIntegrationPoints %d" % (len(Vm), len(Im), self.n))
         check(lib().cg_ionic_step(self.h, float(stepStartTime), float(deltaT), pv, pi, fv))
 
-    ODE_SOLVERS = {"RushLarsen": _capi.ODE_RUSH_LARSEN, "RKF45": _capi.ODE_RKF45}
+    ODE_SOLVERS = {"RushLarsen": _capi.ODE_RUSH_LARSEN, "RKF45": _capi.ODE_RKF45,
+                   "Euler": _capi.ODE_EULER}
 
     def set_integrator(self, solver="RKF45", absTol=1e-15, relTol=1e-4, maxSteps=10000,
                        initial_step=0.0):
-        """Per-cell ODE integrator: "RKF45" (OpenFOAM's adaptive RKF45, the reference's
-        `solver RKF45`) or "RushLarsen".  initial_step (model time units) seeds the kept
+        """Per-cell ODE integrator: "RKF45" or "Euler" (OpenFOAM's adaptive ODESolvers of
+        the reference's `solver RKF45;` / `solver Euler;`), or "RushLarsen".  initial_step (model time units) seeds the kept
         per-cell trial steps; 0 seeds them with the first step's deltaT in seconds."""
         if solver not in self.ODE_SOLVERS:
             raise FatalIOError("Unknown ODESolver type %s\nValid ODESolver types on the device: %s"
@@ -236,6 +237,18 @@ class IonicModel:
         check(lib().cg_ionic_enable_beats(self.h, int(max_beats), float(dvdt_threshold),
                                           float(baseline_s), float(peak_window_s)))
         self.max_beats = int(max_beats)
+
+    def restitution(self):
+        """classify_early_strokes + compute_apd_di (postProcessing_restCurves.py:203-297) of
+        every cell's beats from the last batch run, on the device: (apd[n], di[n] in s, NaN
+        where the pair is rejected, flags[n, max_beats]: 1 early, 2 valid APD, 4 valid DI, 8
+        more beats than max_beats)."""
+        mb = getattr(self, "max_beats", 0)
+        out = np.zeros((self.n, 2))
+        flags = np.zeros((self.n, max(mb, 1)), dtype=np.uint8)
+        check(lib().cg_ionic_get_restitution(self.h, out.ctypes.data_as(C.c_void_p),
+                                             flags.ctypes.data_as(C.c_void_p)))
+        return out[:, 0], out[:, 1], flags[:, :mb]
 
     def beats(self):
         """(counts[n], records[n, max_beats, 5]) of the last batch run: t_up (s), Vm base,

@@ -53,7 +53,7 @@ enum { CG_FP64 = 0, CG_MIXED = 1 };
 /* flags */
 enum { CG_DEVICE_PTRS = 1 };
 /* per-cell ODE integrators of the ionic step */
-enum { CG_ODE_RUSH_LARSEN = 0, CG_ODE_RKF45 = 1 };
+enum { CG_ODE_RUSH_LARSEN = 0, CG_ODE_RKF45 = 1, CG_ODE_EULER = 2 };
 
 const char* cg_last_error(void);
 /* library version string */
@@ -120,6 +120,8 @@ int cg_ionic_set_stimulus(cg_ionic* h, const cg_stimulus* s);
  *     initial_step (model time units) seeds the kept steps; <= 0 seeds them with the
  *     first call's deltaT in seconds, as the reference's initialDeltaT does.  OpenFOAM's
  *     defaults when the keys are absent: abs_tol 1e-15, rel_tol 1e-4, max_steps 10000.
+ *   CG_ODE_EULER: OpenFOAM's `Euler` ODESolver under the same controller and kept steps:
+ *     y = y0 + dx f(x0, y0) with the increment itself as the error estimate.
  * A failing cell (step-size underflow, more than max_steps sub-steps) makes the next
  * synchronising call return CG_ERR_ODE.  Batched protocols (cg_batch_run) use
  * Rush-Larsen only. */
@@ -178,6 +180,17 @@ int cg_batch_autotune(cg_ionic* const* handles, int n_handles, double dt_s, int 
  * [n_cells][max_beats][5] = t_up (s), Vm base, peak, V90 (V), t_repol90 (s, NaN when the
  * trace ends first); unused records are NaN. */
 int cg_ionic_get_beats(cg_ionic* h, int32_t* counts, double* records);
+/* Early-stroke classification and APD90/DI of every cell's beats from the last batch run:
+ * classify_early_strokes and compute_apd_di of the restitution post-processing
+ * (tutorials/singleCellprotocols/restitutionCurves_s1s2Protocol/
+ * setupRestitutionCurves_s1s2Protocol/postProcessing_restCurves.py:203-297), on the device.
+ * A beat is early when the next upstroke comes before its 90 % repolarisation or it never
+ * repolarised; the last two beats are the S1-S2 pair.  apd_di [n][2] (host): APD90 of S2
+ * and DI = S2 upstroke - S1 90 % repolarisation in s, both NaN where the reference rejects
+ * the pair (returns None), and where a cell detected more beats than max_beats.
+ * flags [n][max_beats] (host, may be NULL): per beat 1 early, 2 valid APD, 4 valid DI, 8
+ * the cell's beats overflowed max_beats.  Synchronous. */
+int cg_ionic_get_restitution(cg_ionic* h, double* apd_di, uint8_t* flags);
 
 /* States as host SoA doubles [n_vars][n_cells], starting at state first_var: the full
  * values (CG_MIXED handles keep their slow concentrations on the device as FP32 offsets

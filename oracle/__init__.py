@@ -96,6 +96,8 @@ def lib():
                                       C.c_double, C.c_double, C.POINTER(C.c_double), C.c_double,
                                       C.c_double, C.c_long]
         L.orc_rkf45_solve.restype = C.c_long
+        L.orc_ode_solve.argtypes = [C.c_int] + L.orc_rkf45_solve.argtypes
+        L.orc_ode_solve.restype = C.c_long
         L.orc_ionic_tissue_step.argtypes = [C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long,
                                             C.c_double, C.c_double, _dp, _dp, C.c_int, C.c_int,
                                             _dp, C.c_double, C.c_double, C.c_int]
@@ -170,6 +172,25 @@ def model_eval(model, tissue, t, c, y, solve_vm=False, stim=None):
                          C.byref(iion), C.byref(ist))
     return dict(rates=r[:n], gate=g[:n].astype(bool), xinf=xi[:n], tau=ta[:n],
                 iion=iion.value, istim=ist.value)
+
+
+EULER = 2   # integrator / method value of OpenFOAM's adaptive Euler (1: RKF45, 0: RL)
+
+
+def ode_solve(method, model, tissue, y, x0, x1, dx_try, abs_tol, rel_tol, solve_vm=True,
+              stim=None, consts=None, max_steps=1000000000):
+    """ODESolver::solve(x0, x1, y, dxTry) on one cell with `method` (1 RKF45, 2 Euler):
+    (y at x1, dxTry out, accepted steps)"""
+    c, _ = model_init(model, tissue)
+    if consts is not None:
+        c = np.array(consts, dtype=np.float64)
+    yy = np.zeros(MAX_STATES)
+    yy[:len(y)] = y
+    st = stim if stim is not None else make_stim()
+    dx = C.c_double(dx_try)
+    n = lib().orc_ode_solve(method, model, tissue, c, yy, int(solve_vm), C.byref(st), x0, x1,
+                            C.byref(dx), abs_tol, rel_tol, max_steps)
+    return yy[:NSTATES[model]], dx.value, n
 
 
 def single_cell_run(model, tissue, stim, dt_s, n_steps, integrator=1, abs_tol=1e-6,
@@ -412,6 +433,47 @@ class OracleTissue:
     @property
     def t(self):
         return self.T.t
+
+
+def classify_beats(beats):
+    """classify_early_strokes (postProcessing_restCurves.py:203-263) on detect_beats'
+    output: a beat is early when the next upstroke comes before its 90 % repolarisation
+    or it never repolarised; early beats lose their APD and DI; a beat's DI is valid when
+    the beat before it was not early and it repolarised itself.  Adds the keys early,
+    valid_apd, valid_di in place; returns the indices of the early beats."""
+    for b in beats:
+        b["early"] = False
+        b["valid_apd"] = bool(b["repol_found"])
+        b["valid_di"] = False
+    early = []
+    for i in range(len(beats) - 1):
+        cur, nxt = beats[i], beats[i + 1]
+        if not cur["repol_found"] or nxt["t_up"] < cur["t_repol90"]:
+            cur["early"] = True
+            cur["valid_apd"] = False
+            cur["valid_di"] = False
+            early.append(i)
+    for i in range(1, len(beats)):
+        if not beats[i - 1]["early"] and beats[i]["repol_found"]:
+            beats[i]["valid_di"] = True
+    return early
+
+
+def apd_di(beats):
+    """compute_apd_di (postProcessing_restCurves.py:270-297) on classified beats: the last
+    two beats as S1, S2; (APD90 of S2, DI = S2 upstroke - S1 repolarisation), or
+    (None, None) when there are fewer than two beats, S1 is early, either APD or S2's DI
+    is not valid, or APD or DI is not positive."""
+    if len(beats) < 2:
+        return None, None
+    s1, s2 = beats[-2], beats[-1]
+    if s1["early"] or not (s1["valid_apd"] and s2["valid_apd"] and s2["valid_di"]):
+        return None, None
+    di = s2["t_up"] - s1["t_repol90"]
+    apd = s2["t_repol90"] - s2["t_up"]
+    if di <= 0 or apd <= 0:
+        return None, None
+    return apd, di
 
 
 def detect_beats(t, vm, max_beats=256, dvdt_threshold=5000.0, baseline_s=0.002, peak_s=2.0):

@@ -11,6 +11,9 @@
  *    (safeScale 0.9, alphaInc 0.2, alphaDec 0.25, minScale 0.2, maxScale 10) and
  *    error norm max_i |err_i| / (absTol + relTol*max(|y0_i|,|y_i|)).
  *    In tissue mode the models leave dV/dt unwritten; it is taken as 0 here.
+ *  - orc_ode_solve with method 2: OpenFOAM's `Euler` ODESolver (`solver Euler;`, the
+ *    restitution tutorial's choice) under the same adaptiveSolver controller: one stage,
+ *    y = y0 + dx dy/dx(x0), and the whole increment dx dy/dx(x0) as the error estimate.
  */
 #include <math.h>
 #include <string.h>
@@ -67,8 +70,8 @@ static double rkf45_step(const rhs_ctx* k, double x0, const double* y0, const do
                          double dx, double* y, double at, double rt)
 {
     const int n = k->n;
-    double yt[ORC_MAX_STATES], k2[ORC_MAX_STATES], k3[ORC_MAX_STATES], k4[ORC_MAX_STATES],
-           k5[ORC_MAX_STATES], k6[ORC_MAX_STATES];
+    double yt[ORC_MAX_STATES] = {0}, k2[ORC_MAX_STATES], k3[ORC_MAX_STATES],
+           k4[ORC_MAX_STATES], k5[ORC_MAX_STATES], k6[ORC_MAX_STATES];
     for (int i = 0; i < n; ++i) yt[i] = y0[i] + a21 * dx * d0[i];
     rhs(k, x0 + c2 * dx, yt, k2);
     for (int i = 0; i < n; ++i) yt[i] = y0[i] + dx * (a31 * d0[i] + a32 * k2[i]);
@@ -92,9 +95,28 @@ static double rkf45_step(const rhs_ctx* k, double x0, const double* y0, const do
     return maxErr;
 }
 
+/* Euler::solve(x0, y0, dydx0, dx, y): err = dx dydx0, y = y0 + err */
+static double euler_step(const rhs_ctx* k, double x0, const double* y0, const double* d0,
+                         double dx, double* y, double at, double rt)
+{
+    (void)x0;
+    double maxErr = 0.0;
+    for (int i = 0; i < k->n; ++i) {
+        double err = dx * d0[i];
+        y[i] = y0[i] + err;
+        double tol = at + rt * fmax(fabs(y0[i]), fabs(y[i]));
+        double q = fabs(err) / tol;
+        if (q > maxErr) maxErr = q;
+    }
+    return maxErr;
+}
+
+typedef double (*trial_fn)(const rhs_ctx*, double, const double*, const double*, double,
+                           double*, double, double);
+
 /* adaptiveSolver::solve(x, y, dxTry): retry until err <= 1, then grow */
-static int adaptive_step(const rhs_ctx* k, double* x, double* y, double* dxTry, double at,
-                         double rt)
+static int adaptive_step(const rhs_ctx* k, trial_fn trial, double* x, double* y,
+                         double* dxTry, double at, double rt)
 {
     const double safeScale = 0.9, alphaInc = 0.2, alphaDec = 0.25, minScale = 0.2,
                  maxScale = 10.0;
@@ -102,7 +124,7 @@ static int adaptive_step(const rhs_ctx* k, double* x, double* y, double* dxTry, 
     double dx = *dxTry, err;
     rhs(k, *x, y, d0);
     do {
-        err = rkf45_step(k, *x, y, d0, dx, yt, at, rt);
+        err = trial(k, *x, y, d0, dx, yt, at, rt);
         if (err > 1) {
             double scale = fmax(safeScale * pow(err, -alphaDec), minScale);
             dx *= scale;
@@ -120,11 +142,12 @@ static int adaptive_step(const rhs_ctx* k, double* x, double* y, double* dxTry, 
     return 0;
 }
 
-long orc_rkf45_solve(int model, int tissue, const double* c, double* y, int solve_vm,
-                     const orc_stim_t* st, double xStart, double xEnd, double* dxTryIO,
-                     double at, double rt, long maxSteps)
+long orc_ode_solve(int method, int model, int tissue, const double* c, double* y,
+                   int solve_vm, const orc_stim_t* st, double xStart, double xEnd,
+                   double* dxTryIO, double at, double rt, long maxSteps)
 {
     rhs_ctx k = {model, tissue, solve_vm, orc_model_nstates(model), c, st};
+    const trial_fn trial = method == 2 ? euler_step : rkf45_step;
     double x = xStart;
     double dxTry = *dxTryIO;
     int last = 0;
@@ -134,7 +157,7 @@ long orc_rkf45_solve(int model, int tissue, const double* c, double* y, int solv
             last = 1;
             dxTry = xEnd - x;
         }
-        if (adaptive_step(&k, &x, y, &dxTry, at, rt) != 0) return -1;
+        if (adaptive_step(&k, trial, &x, y, &dxTry, at, rt) != 0) return -1;
         if ((x - xEnd) * (xEnd - xStart) >= 0) {
             if (nStep > 0 && last) dxTry = dxTry0;
             *dxTryIO = dxTry;
@@ -142,6 +165,14 @@ long orc_rkf45_solve(int model, int tissue, const double* c, double* y, int solv
         }
     }
     return -1;
+}
+
+long orc_rkf45_solve(int model, int tissue, const double* c, double* y, int solve_vm,
+                     const orc_stim_t* st, double xStart, double xEnd, double* dxTryIO,
+                     double at, double rt, long maxSteps)
+{
+    return orc_ode_solve(1, model, tissue, c, y, solve_vm, st, xStart, xEnd, dxTryIO, at, rt,
+                         maxSteps);
 }
 
 /* ------------------------------------------------------ tissue solveODE */
@@ -189,7 +220,8 @@ void orc_ionic_tissue_step(int model, int tissue, const double* consts, long c_s
         } else {
             double st = step[cell];
             if (st > dtModel) st = dtModel;
-            orc_rkf45_solve(model, tissue, c, y, 0, NULL, tStart, tEnd, &st, at, rt, 1000000000L);
+            orc_ode_solve(integrator, model, tissue, c, y, 0, NULL, tStart, tEnd, &st, at, rt,
+                          1000000000L);
             step[cell] = st;
         }
         double r[ORC_MAX_STATES], xi[ORC_MAX_STATES], ta[ORC_MAX_STATES], iion, ist;
@@ -224,8 +256,8 @@ int orc_single_cell_run(int model, int tissue, const double* c, double* y,
             orc_rl_step(model, tissue, tStart, dtModel, c, y, 1, st);
         } else {
             if (step > dtModel) step = dtModel;
-            if (orc_rkf45_solve(model, tissue, c, y, 1, st, tStart, tEnd, &step, at, rt,
-                                1000000000L) < 0)
+            if (orc_ode_solve(integrator, model, tissue, c, y, 1, st, tStart, tEnd, &step, at,
+                              rt, 1000000000L) < 0)
                 return -1;
         }
         if (t_out) t_out[k] = t;

@@ -79,15 +79,20 @@ double orc_stimulus(double t, const orc_stim_t* s);
 void orc_rl_step(int model, int tissue, double t, double dt, const double* c, double* y,
                  int solve_vm, const orc_stim_t* stim);
 
-/* OpenFOAM ODESolver::solve(xStart, xEnd, y, dxTry) with RKF45 + adaptiveSolver.
- * Returns number of accepted steps, or -1 on maxSteps exceeded. */
+/* OpenFOAM ODESolver::solve(xStart, xEnd, y, dxTry) with adaptiveSolver and the stage
+ * scheme `method`: 1 RKF45, 2 Euler.  Returns the number of accepted steps, or -1 on
+ * maxSteps exceeded or step-size underflow. */
+long orc_ode_solve(int method, int model, int tissue, const double* c, double* y,
+                   int solve_vm, const orc_stim_t* stim, double x0, double x1, double* dx_try,
+                   double abs_tol, double rel_tol, long max_steps);
 long orc_rkf45_solve(int model, int tissue, const double* c, double* y, int solve_vm,
                      const orc_stim_t* stim, double x0, double x1, double* dx_try,
                      double abs_tol, double rel_tol, long max_steps);
 
 /* Tissue ionic step of the reference's solveODE (ionicModel.H:250-256):
  * states SoA [nstates][n]; Vm in V; Im out in V/s.  integrator 0 = Rush-Larsen
- * (n_sub substeps), 1 = RKF45 with per-cell step[] (ms, persistent).
+ * (n_sub substeps), 1 = RKF45, 2 = OpenFOAM's adaptive Euler, with per-cell step[] (model
+ * time units, persistent).
  * c_stride = 0: one constant set for all cells; else consts[cell*c_stride+k]. */
 void orc_ionic_tissue_step(int model, int tissue, const double* consts, long c_stride,
                            double* states, long n, double t0_s, double dt_s,
@@ -188,7 +193,7 @@ int orc_pcg_of(const orc_mesh_t* m, const double* diag, const double* upper, con
 
 typedef struct {
     int model, tissue;
-    int integrator;          /* 0 = Rush-Larsen (n_sub substeps), 1 = RKF45 */
+    int integrator;          /* 0 = Rush-Larsen (n_sub substeps), 1 = RKF45, 2 = Euler */
     int n_sub;
     double abs_tol, rel_tol; /* RKF45 */
     int ddt;                 /* 0 = Euler, 1 = backward (BDF2, Euler on step 1) */

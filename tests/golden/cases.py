@@ -31,6 +31,30 @@ SINGLE_CELL = dict(
                (0.5000000, -83.6520453, 5e-3)],
 )
 
+# tutorials/manufacturedSolutions/singleCellBidomain/postProcessing/
+# AlievPanfilov_myocyte_S1_1000.txt, the reference's trace of an AlievPanfilov myocyte
+# (rows kept in aliev_panfilov_trace.txt).  The case's electroProperties now select
+# another model; the trace's own name and rows fix the rest: the run's integrator
+# controls are the case's (RKF45, absTol 1e-6, relTol 1e-4, deltaT 5e-5 s, endTime 1),
+# the initial state relaxes from -75 mV (u = 0.05) to rest, and the upstroke starts at
+# 0.258 s = 20 model time units of 12.9 ms with dV/dt = 3014 mV/s, i.e. an S1 stimulus
+# of amplitude 0.4 from model time 20 (period 1000 from the file name, duration 1 as
+# the singleCell tutorials use).
+ALIEV_PANFILOV = dict(
+    coeffs={"ionicModel": "AlievPanfilov", "tissue": "myocyte", "solutionAlgorithm": "explicit",
+            "solver": "RKF45", "initialODEStep": 1e-5, "maxSteps": 1000000000,
+            "absTol": 1e-6, "relTol": 1e-4,
+            "singleCellStimulus": {"stim_start": 20, "stim_duration": 1, "stim_amplitude": 0.4,
+                                   "stim_period_S1": 1000, "nstim1": 1, "stim_period_S2": 0,
+                                   "nstim2": 0},
+            "writeAfterTime": 0, "outputVariables": {"ionic": {"export": ["Vm"]}}},
+    stimulus=dict(start=20.0, period_s1=1000.0, duration=1.0, amplitude=0.4, nstim1=1,
+                  period_s2=0.0, nstim2=0),
+    trace_file="AlievPanfilov_myocyte_S1_1000.txt",
+    fixture="aliev_panfilov_trace.txt",
+    dt=5e-5, n_steps=20000, abs_tol=1e-6, rel_tol=1e-4, upstroke=0.258,
+)
+
 NIEDERER = dict(
     shape=(100, 15, 35), h=0.2e-3,
     sigma9=[0.1334177215, 0, 0, 0, 0.01760617761, 0, 0, 0, 0.01760617761],

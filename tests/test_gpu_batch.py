@@ -185,3 +185,52 @@ def test_batch_mixed_crossing_zero_mv(gpu, orc):
     assert c64[0] == c32[0]
     ok = ~np.isnan(r64[0, :c64[0], 4])
     assert np.max(np.abs(r64[0, :c64[0], 4][ok] - r32[0, :c32[0], 4][ok])) < 1e-3
+
+
+def test_restitution_classification_vs_reference_fixture(gpu, orc):
+    """On-device beats, early-stroke classification and APD90/DI (cg_ionic_get_restitution)
+    of the S1-S2 cells of tests/golden/restitution_beats.json, whose expected values the
+    reference's own detect_beats / classify_early_strokes / compute_apd_di produced on the
+    oracle's Rush-Larsen traces: one batched run per model, every case a cell."""
+    import json
+    import os
+    from tests.golden import cases as gc
+    _say("restitution classification vs reference fixture")
+    with open(os.path.join(os.path.dirname(gc.__file__), "restitution_beats.json")) as f:
+        fx = [c for c in json.load(f)["cases"] if "model" in c]
+    keys = {0: ("BuenoOrovio", "epicardialCells"), 1: ("TNNP", "epicardialCells"),
+            2: ("ORd", "myocyte")}
+    for m in sorted({c["model"] for c in fx}):
+        cs = [c for c in fx if c["model"] == m]
+        ion = gpu.IonicModel(*keys[m], len(cs), solve_vm=True, precision="fp64")
+        ion.set_protocols([{"stim_start": c["stimulus"]["start"],
+                            "stim_period_S1": c["stimulus"]["period_s1"],
+                            "stim_duration": c["stimulus"]["duration"],
+                            "stim_amplitude": c["stimulus"]["amplitude"],
+                            "nstim1": c["stimulus"]["nstim1"],
+                            "stim_period_S2": c["stimulus"]["period_s2"],
+                            "nstim2": c["stimulus"]["nstim2"]} for c in cs])
+        ion.set_run_lengths(np.array([c["n_steps"] for c in cs], dtype=np.int64))
+        ion.enable_beats(max_beats=16)
+        gpu.batch_run([ion], cs[0]["dt_ms"] * 1e-3)
+        cnt, rec = ion.beats()
+        apd, di, flags = ion.restitution()
+        for i, c in enumerate(cs):
+            want = c["beats"]
+            assert cnt[i] == len(want), (c["name"], cnt[i], len(want))
+            for k, w in enumerate(want):
+                assert abs(rec[i, k, 0] - w["t_up"]) <= 1e-9, (c["name"], k)
+                if w["t_repol90"] is None:
+                    assert np.isnan(rec[i, k, 4])
+                else:
+                    assert abs(rec[i, k, 4] - w["t_repol90"]) <= 1e-9, (c["name"], k)
+                fw = (1 if w["early"] else 0) | (2 if w["valid_apd"] else 0) | \
+                    (4 if w["valid_di"] else 0)
+                assert flags[i, k] == fw, (c["name"], k, flags[i, k], fw)
+            for got, ref in ((apd[i], c["apd"]), (di[i], c["di"])):
+                if ref is None:
+                    assert np.isnan(got), (c["name"], got)
+                else:
+                    assert abs(got - ref) <= 1e-9, (c["name"], got, ref)
+            _say("  %s: %d beats, early %s, APD90 %s DI %s" % (c["name"], cnt[i], c["early"],
+                                                             apd[i], di[i]))

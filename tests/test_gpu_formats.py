@@ -57,6 +57,30 @@ def test_single_cell_regression_files(gpu, orc, tmp_path):
     assert err <= 1e-6
 
 
+def test_aliev_panfilov_reference_trace_on_device(gpu, orc, tmp_path):
+    """The device's single-cell run writes the reference's own AlievPanfilov trace
+    (tests/golden/aliev_panfilov_trace.txt, rows of AlievPanfilov_myocyte_S1_1000.txt):
+    the rest relaxation to the 7 printed decimals, and the action potential within the
+    RKF45 agreement bound of two correct implementations (10 relTol of the swing)."""
+    _say("aliev-panfilov reference trace on device")
+    from cardiacfoam_amd.single_cell import SingleCellSolver
+    from cardiacfoam_amd import foam_io
+    c = cases.ALIEV_PANFILOV
+    solver = SingleCellSolver(c["coeffs"], c["dt"], str(tmp_path), precision="fp64")
+    path = solver.run(c["n_steps"] * c["dt"], chunk=5000)
+    assert os.path.basename(path) == c["trace_file"]
+    names, data = foam_io.read_trace(path)
+    d = np.loadtxt(os.path.join(os.path.dirname(cases.__file__), c["fixture"]), skiprows=2)
+    idx = np.rint(d[:, 0] / c["dt"]).astype(int) - 1
+    np.testing.assert_allclose(data[idx, 0], d[:, 0], atol=5e-8, rtol=0)
+    err = np.abs(data[idx, 1] - d[:, 1])
+    pre = d[:, 0] < c["upstroke"]
+    _say("  rows %d: max |Vm - reference| %.3e mV before the stimulus, %.3e mV overall"
+         % (len(d), err[pre].max(), err.max()))
+    assert err[pre].max() <= 5e-8
+    assert err.max() <= 10 * c["rel_tol"] * 100.0
+
+
 def _write_times(t, dt, interval, last):
     """Time::writeTime for writeControl runTime: the write-interval index of t + dt/2
     moved on since the last write."""

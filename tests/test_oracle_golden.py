@@ -71,3 +71,25 @@ def test_niederer_golden_needs_the_mesh_centres(orc):
     p4 = next(p for p in cases.NIEDERER["probes"] if p["name"] == "p4")
     i, j, k = p4["cell"]
     assert ta[k, j, i] - p4["expected"] < -1.5e-4
+
+
+def _aliev_panfilov_fixture():
+    import os
+    c = cases.ALIEV_PANFILOV
+    d = np.loadtxt(os.path.join(os.path.dirname(cases.__file__), c["fixture"]), skiprows=2)
+    return d[:, 0], d[:, 1]
+
+
+def test_aliev_panfilov_reference_trace(orc):
+    """The oracle's RKF45 AlievPanfilov run reproduces the reference's own trace
+    (AlievPanfilov_myocyte_S1_1000.txt): every kept row, rest relaxation and action
+    potential, to the 7 printed decimals."""
+    c = cases.ALIEV_PANFILOV
+    t_ref, v_ref = _aliev_panfilov_fixture()
+    t, v, _ = orc.single_cell_run(orc.AP, orc.MYOCYTE, orc.make_stim(**c["stimulus"]), c["dt"],
+                                  c["n_steps"], integrator=1, abs_tol=c["abs_tol"],
+                                  rel_tol=c["rel_tol"])
+    idx = np.rint(t_ref / c["dt"]).astype(int) - 1
+    np.testing.assert_allclose(t[idx], t_ref, atol=5e-8, rtol=0)
+    assert np.array_equal(np.round(v[idx], 7), v_ref)
+    assert v_ref.max() > 19.0 and abs(v_ref[-1] + 80.0) < 1e-6   # one action potential

@@ -296,6 +296,33 @@ def test_manufactured_single_cell_closed_form(orc):
         assert 0.9 < order < 1.1, (G, errs, order)
 
 
+def test_adaptive_euler_closed_forms(orc):
+    """OpenFOAM's adaptive Euler (`solver Euler;`) on the manufactured single cell: the
+    error estimate is the whole increment, so on u' ~ u / (1 + t) every accepted step has
+    err = dx / (relTol (1 + t)) and the controller's fixed point 0.9 err^-1/5 = 1 gives
+    dx = 0.9^5 relTol (1 + t): ln(1 + T) / (0.9^5 relTol) accepted steps over [0, T]
+    (a wrong growth exponent or error estimate moves the count by >= 10 %); the error of a
+    first-order step taken at dx ~ relTol falls linearly with relTol."""
+    from tests import manufactured as mf
+    G, T = 1.5, 0.2
+    u1e, u2e = mf.single_cell(G, T)
+    errs = []
+    for rt in (1e-5, 1e-6):
+        y, _, n = orc.ode_solve(orc.EULER, orc.FDA, orc.DIM1, [0.0, G, G ** -0.5, 0.0], 0.0, T,
+                                1e-9, 1e-4 * rt, rt)
+        pred = np.log(1.0 + T) / (0.9 ** 5 * rt)
+        assert abs(n / pred - 1.0) < 2e-3, (rt, n, pred)
+        assert y[0] == 0.0 and y[3] == 0.0
+        errs.append((abs(y[1] - u1e), abs(y[2] - u2e)))
+    for q in (0, 1):
+        assert 9.5 < errs[0][q] / errs[1][q] < 10.5, errs
+    # RKF45 under the same controller: its fifth-order error estimate lets it take
+    # steps orders of magnitude longer for the same tolerance
+    y, _, n45 = orc.ode_solve(1, orc.FDA, orc.DIM1, [0.0, G, G ** -0.5, 0.0], 0.0, T, 1e-9,
+                              1e-10, 1e-6)
+    assert n45 < 1e-3 * n and abs(y[2] - u2e) < 1e-6 * u2e
+
+
 def test_pseudo_ecg_converges_to_the_continuous_integral(orc):
     """The oracle's pseudo-ECG (pseudoECGSolver.C:36-83) of the manufactured V at the
     monodomainPseudoECG tutorial's electrodes: second-order convergence to the
