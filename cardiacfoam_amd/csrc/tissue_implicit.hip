@@ -65,6 +65,9 @@ struct DevState {
     double init_res;          // initial residual of the outer corrector (in its norm)
 };
 
+static_assert(sizeof(DevState) % 8 == 0 && sizeof(DevState) / 8 <= 64,
+              "the state is published by the lanes of one wave, one 8-byte word each");
+
 template <class T> struct HexOp {
     const T* fx;
     const T* fy;
@@ -397,9 +400,19 @@ __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __
                                               DevState* S, long long gen, int k, double tol2,
                                               int relative_to_r0, double abs2,
                                               double outer_tol2, int max_iter, double of_tol,
-                                              double of_rel, int record_init)
+                                              double of_rel, int record_init, DevState* pub)
 {
-    if (S->done_gen == gen) return;
+    // pub (the host mirror, or null): block 0 copies the state there once it is final for
+    // this launch, one 8-byte word per lane
+    auto publish = [&]() {
+        if (pub && blockIdx.x == 0 && threadIdx.x < sizeof(DevState) / 8)
+            reinterpret_cast<volatile unsigned long long*>(pub)[threadIdx.x] =
+                reinterpret_cast<const volatile unsigned long long*>(S)[threadIdx.x];
+    };
+    if (S->done_gen == gen) {
+        publish();
+        return;
+    }
     double rr, rr0, bb, l1 = 0.0, nf = 1.0;
     if (NORM == 1 || (NORM == 2 && k == 0)) {
         double v5[5];
@@ -439,6 +452,10 @@ __global__ __launch_bounds__(RBS) void k_cg_s(HexGeom g, HexOp<T> A, const T* __
             if (outer || (conv && !relative_to_r0)) S->conv_gen = gen;
             S->done_gen = gen;
         }
+    }
+    if (pub && blockIdx.x == 0) {
+        __syncthreads();   // thread 0's updates of S before the copy by the block's lanes
+        publish();
     }
     if (outer || conv || k >= max_iter) return;
     double v[1] = {0.0};
@@ -656,8 +673,10 @@ struct ImplicitSolver {
     float* e32 = nullptr;
     double* partials = nullptr;
     DevState* S = nullptr;
-    DevState* hS = nullptr;  // pinned mirror
-    hipEvent_t read_done = nullptr;  // after the read-back of S
+    DevState* hS = nullptr;  // pinned, coherent host mirror of S
+    DevState* hS_dev = nullptr;      // the mirror's device address: the pass's last S-kernel
+                                     // writes S there, so no copy kernel runs
+    hipEvent_t read_done = nullptr;  // after the kernel that published S
     double mass_c = -1.0;    // coefft of the current invd
     bool quad = false;       // nx % 4 == 0: CG kernels on quads of cells (vector loads)
     long long gen = 0;       // generation of the last CG solve started
@@ -739,7 +758,9 @@ int implicit_create(ImplicitSolver** out, const HexGeom& g, const double* sig, d
         A((void**)&S->v64.r, 8 * n); A((void**)&S->v64.u, 8 * n); A((void**)&S->v64.p, 8 * n);
         A((void**)&S->v64.s, 8 * n); A((void**)&S->v64.w, 8 * n);
     }
-    ok &= hipHostMalloc((void**)&S->hS, sizeof(DevState), hipHostMallocDefault) == hipSuccess;
+    ok &= hipHostMalloc((void**)&S->hS, sizeof(DevState),
+                        hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess;
+    ok &= S->hS && hipHostGetDevicePointer((void**)&S->hS_dev, S->hS, 0) == hipSuccess;
     ok &= hipEventCreateWithFlags(&S->read_done, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         implicit_destroy(S);
@@ -836,15 +857,17 @@ template <class T> struct CgRun {
     // OpenFOAM's on the outer test of the refinement (mixed)
     int norm_kind() const { return S->ctl.norm == 0 ? 0 : (sizeof(T) == 8 ? 1 : 2); }
 
+    // publish: the launch that ends a pass before the host reads the state
     void s_kernel(long long gen, int k, double tol2, int rel, double abs2, double outer_tol2,
-                  int max_iter, double of_tol = 0.0, int record_init = 0)
+                  int max_iter, double of_tol = 0.0, int record_init = 0, bool publish = false)
     {
         const int tk = prof ? prof->begin(K_SPMV) : -1;
         const double of_rel = S->ctl.rel_tol;
+        DevState* pub = publish ? S->hS_dev : nullptr;
 #define CG_S(Q, N)                                                                         \
     hipLaunchKernelGGL((k_cg_s<T, Q, N>), dim3(RG), dim3(RBS), 0, S->stream, S->g, A,      \
                        (const T*)v.u, v.w, S->partials, S->S, gen, k, tol2, rel, abs2,     \
-                       outer_tol2, max_iter, of_tol, of_rel, record_init)
+                       outer_tol2, max_iter, of_tol, of_rel, record_init, pub)
         switch (norm_kind() + (S->quad ? 3 : 0)) {
         case 0: CG_S(false, 0); break;
         case 1: CG_S(false, 1); break;
@@ -965,12 +988,13 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
     const bool of = S->ctl.norm == 1;
     const int n_outer = S->ctl.n_outer > 1 ? S->ctl.n_outer : 1;
 
-    // Before each read-back: the step's activation (x into V^{n+1}) and the next step's
-    // ionic kernel, both gated on "solve g met the tolerance", then the state read-back.
-    // The host waits for the read-back only; the queued ionic kernel of the next step keeps
-    // the device busy meanwhile.  Gated launches of a pass that did not converge do nothing
-    // and are dropped from the timings.  Outer correctors before the last one read back
-    // without the gated launches (the step is not finished).
+    // Before each read: the step's activation (x into V^{n+1}) and the next step's ionic
+    // kernel, both gated on "solve g met the tolerance".  The pass's last S-kernel has
+    // already written the state into the host mirror; the host waits for the event behind
+    // it only, and the queued ionic kernel of the next step keeps the device busy
+    // meanwhile.  Gated launches of a pass that did not converge do nothing and are dropped
+    // from the timings.  Outer correctors before the last one read back without the gated
+    // launches (the step is not finished).
     auto finish_and_read = [&](long long g, const double* x, size_t* mark, bool last) -> bool {
         *mark = prof ? prof->mark() : 0;
         // with the solution already in V^{n+1}, the next step's ionic kernel does the
@@ -982,7 +1006,6 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
                                Vnew, flag, tact, st.t_old, st.dt, conv_gen, g);
             if (prof) prof->end(t);
         }
-        hipMemcpyAsync(S->hS, S->S, sizeof(DevState), hipMemcpyDeviceToHost, S->stream);
         hipEventRecord(S->read_done, S->stream);
         if (last && next) {
             // the next step (which extrapolates) gets the same fusion
@@ -1027,7 +1050,8 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
             for (;;) {
                 for (int q = 0; q < chunk; ++q, ++k) {
                     cg.u_kernel(gen, k, Vnew, 0);
-                    cg.s_kernel(gen, k + 1, tol * tol, 0, 0.0, 0.0, max_iter, tol, 0);
+                    cg.s_kernel(gen, k + 1, tol * tol, 0, 0.0, 0.0, max_iter, tol, 0,
+                                q == chunk - 1);
                 }
                 size_t gated;
                 if (!finish_and_read(gen, Vnew, &gated, last)) {
@@ -1054,6 +1078,10 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
                     break;
                 }
                 if (prof) prof->invalidate_range(gated, prof->mark());
+                if (k > max_iter + 64) {   // the device stops a solve at max_iter
+                    *err = "solver state read-back did not report the end of the solve";
+                    return 4;
+                }
                 chunk = 4;  // not finished: continue the same solve
             }
             log.clear();
@@ -1106,7 +1134,8 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
                 const long long ngen = ++S->gen;
                 cg.residual(K_RESIDUAL, ngen, cur, nullptr, S->e32, nxt, nullptr, A, bref,
                             nullptr, 0);
-                cg.s_kernel(ngen, 0, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter, tol, 0);
+                cg.s_kernel(ngen, 0, inner_tol * inner_tol, 1, abs2, tol * tol, max_iter, tol, 0,
+                            true);
                 cur = nxt;
                 size_t gated;
                 if (!finish_and_read(ngen, cur, &gated, last)) {
