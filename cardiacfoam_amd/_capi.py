@@ -19,6 +19,7 @@ NORM_L2, NORM_OPENFOAM = 0, 1
 EXPLICIT, IMPLICIT = 0, 1
 DDT_EULER, DDT_BACKWARD = 0, 1
 FIELD_VM, FIELD_IION, FIELD_ACTIVATION, FIELD_VM_OLD = 0, 1, 2, 3
+RATES, ALGEBRAIC = 0, 1
 
 
 
@@ -57,6 +58,7 @@ def lib():
     L.cg_tissue_flag.argtypes = [C.c_char_p]
     L.cg_model_nstates.argtypes = [C.c_int]
     L.cg_model_state_name.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_size_t]
+    L.cg_model_units.argtypes = [C.c_int, dp, dp, dp, dp]
     L.cg_ionic_create.argtypes = [C.c_int, C.c_int, ip64, C.c_int, C.c_int, C.POINTER(vp)]
     L.cg_ionic_destroy.argtypes = [vp]
     L.cg_ionic_nstates.argtypes = [vp]
@@ -120,6 +122,9 @@ def lib():
     L.cg_batch_autotune.argtypes = [C.POINTER(vp), C.c_int, C.c_double, C.c_int, C.c_char_p,
                                     C.c_size_t]
     L.cg_ionic_get_beats.argtypes = [vp, vp, vp]
+    L.cg_ionic_eval.argtypes = [vp, C.c_int, ip64, vp, vp, vp]
+    L.cg_model_nalgebraic.argtypes = [C.c_int]
+    L.cg_model_algebraic_name.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_size_t]
     L.cg_ionic_get_restitution.argtypes = [vp, vp, vp]
     L.cg_adjoint_create.argtypes = [vp, C.c_int, vp, C.c_int, C.POINTER(vp)]
     L.cg_adjoint_destroy.argtypes = [vp]

@@ -179,6 +179,7 @@ struct cg_ionic {
     size_t elem;
     void* d_states = nullptr;
     void* d_gscale = nullptr;
+    std::vector<double> h_gscale;     // host copy of the scales [4][n] (derived-variable calls)
     double* d_vm = nullptr;
     double* d_im = nullptr;
     hipStream_t stream = nullptr;
@@ -352,6 +353,17 @@ int cg_model_nstates(int model)
     return model_info(model, &mi) ? mi.ns : -1;
 }
 
+int cg_model_units(int model, double* vmul, double* voff, double* vscale, double* t_per_s)
+{
+    ModelInfo mi;
+    if (!model_info(model, &mi)) return fail(CG_ERR_MODEL, "unknown model id %d", model);
+    if (vmul) *vmul = model == cg::MODEL_FDA ? cg::FdaManufactured::VMUL : 1000.0;
+    if (voff) *voff = mi.voff;
+    if (vscale) *vscale = mi.vscale;
+    if (t_per_s) *t_per_s = mi.tscale;
+    return CG_OK;
+}
+
 int cg_model_state_name(int model, int i, char* buf, size_t len)
 {
     ModelInfo mi;
@@ -362,6 +374,29 @@ int cg_model_state_name(int model, int i, char* buf, size_t len)
                    : model == 4 ? kFDANames[i] : model == 5 ? kCRNNames[i]
                    : model == 6 ? kPYSNames[i] : kSTWNames[i];
     snprintf(buf, len, "%s", nm);
+    return CG_OK;
+}
+
+// ALGEBRAIC names in the reference's order (BuenoOrovio_2008.H:30-43,
+// AlievPanfilov_1996.H:28-32) of the models whose algebraic list the device evaluates
+static const char* const kBOAlgNames[] = {"Jfi", "Jso", "Jsi", "Jion", "Istim", "tauSo",
+                                          "tauO", "tauVMinus", "tauWMinus", "tauS", "vInfty",
+                                          "wInfty"};
+static const char* const kAPAlgNames[] = {"AV_eps", "Istim", "Iion_cm"};
+
+int cg_model_nalgebraic(int model)
+{
+    ModelInfo mi;
+    if (!model_info(model, &mi)) return -1;
+    return cg::model_nalgebraic(model);
+}
+
+int cg_model_algebraic_name(int model, int i, char* buf, size_t len)
+{
+    const int na = cg_model_nalgebraic(model);
+    if (na < 0) return fail(CG_ERR_MODEL, "unknown model id %d", model);
+    if (i < 0 || i >= na || !buf || len == 0) return fail(CG_ERR_ARG, "bad algebraic index %d", i);
+    snprintf(buf, len, "%s", model == cg::MODEL_BO ? kBOAlgNames[i] : kAPAlgNames[i]);
     return CG_OK;
 }
 
@@ -450,9 +485,11 @@ int cg_ionic_set_conductance_scale(cg_ionic* h, const double* s)
     if (!s) {
         if (h->d_gscale) hipFree(h->d_gscale);
         h->d_gscale = nullptr;
+        h->h_gscale.clear();
         return CG_OK;
     }
     const size_t cnt = 4 * (size_t)h->n;
+    h->h_gscale.assign(s, s + cnt);
     if (!h->d_gscale && hipMalloc(&h->d_gscale, h->elem * cnt) != hipSuccess)
         return fail(CG_ERR_NOMEM, "hipMalloc for conductance scales failed");
     if (h->prec == CG_FP64) {
@@ -858,6 +895,44 @@ int cg_ionic_get_beats(cg_ionic* h, int32_t* counts, double* records)
                 r[4] /= per_s;
             }
     }
+    return CG_OK;
+}
+
+int cg_ionic_eval(cg_ionic* h, int kind, int64_t n, const double* states,
+                  const double* t_model, double* out)
+{
+    if (!h || !states || !out || (h->solve_vm && !t_model)) return fail(CG_ERR_ARG, "NULL argument");
+    if (n < 1) return fail(CG_ERR_ARG, "no rows");
+    if (kind != CG_RATES && kind != CG_ALGEBRAIC) return fail(CG_ERR_ARG, "unknown kind %d", kind);
+    const int nv = kind == CG_RATES ? h->mi.ns : cg::model_nalgebraic(h->model);
+    if (nv <= 0)
+        return fail(CG_ERR_MODEL, "the algebraic variables of ionicModel %s are not evaluated "
+                    "on the device", kModelKeys[h->model]);
+    const int ns = h->mi.ns;
+    const bool scales = n == h->n && !h->h_gscale.empty();
+    double *d_st = nullptr, *d_t = nullptr, *d_g = nullptr, *d_out = nullptr;
+    bool ok = hipMalloc(&d_st, sizeof(double) * ns * n) == hipSuccess &&
+              hipMalloc(&d_t, sizeof(double) * n) == hipSuccess &&
+              hipMalloc(&d_out, sizeof(double) * nv * n) == hipSuccess &&
+              (!scales || hipMalloc(&d_g, sizeof(double) * 4 * n) == hipSuccess);
+    hipError_t e = ok ? hipSuccess : hipErrorOutOfMemory;
+    if (e == hipSuccess)
+        e = hipMemcpy(d_st, states, sizeof(double) * ns * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (t_model) e = hipMemcpy(d_t, t_model, sizeof(double) * n, hipMemcpyHostToDevice);
+        else e = hipMemset(d_t, 0, sizeof(double) * n);
+    }
+    if (e == hipSuccess && scales)
+        e = hipMemcpy(d_g, h->h_gscale.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = cg::launch_ionic_eval(h->model, h->tissue, n, d_st, d_t, kind, h->solve_vm, h->stim,
+                                  h->P, d_g, d_out, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d_out, sizeof(double) * nv * n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d_st); hipFree(d_t); hipFree(d_g); hipFree(d_out);
+    if (e == hipErrorOutOfMemory) return fail(CG_ERR_NOMEM, "eval buffers");
+    if (e != hipSuccess) return fail(CG_ERR_HIP, "eval: %s", hipGetErrorString(e));
     return CG_OK;
 }
 

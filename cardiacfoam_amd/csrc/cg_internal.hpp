@@ -117,6 +117,12 @@ enum {
 enum { BF_TU, BF_TEND, BF_BASE, BF_PEAK, BF_JP, BF_V90, BF_TREP, BF_FLAGS };
 enum { BEAT_OPEN = 1, BEAT_HAS = 2, BEAT_FOUND = 4 };
 constexpr int BEAT_REC = 5;  // t_up, v_base, v_peak, v90, t_repol90 (NaN when not found)
+// RATES (kind 0) / ALGEBRAIC (kind 1) of a model at n host-given rows, FP64 (model_fp64.hip);
+// model_nalgebraic: the length of the model's algebraic list on the device (0: none)
+int model_nalgebraic(int model);
+hipError_t launch_ionic_eval(int model, int tissue, int64_t n, const double* st, const double* t,
+                             int kind, int solve_v, const Stimulus& stim, const ModelParams& P,
+                             const double* gsc, double* out, hipStream_t s);
 // per-beat flags of the restitution classification (cg_ionic_get_restitution)
 constexpr unsigned RB_EARLY = 1, RB_VALID_APD = 2, RB_VALID_DI = 4, RB_TRUNCATED = 8;
 // beats [n][max_beats][BEAT_REC] (model units), nbeats [n] -> out [n][2] (APD90, DI in s,

@@ -74,6 +74,44 @@ hipError_t launch_restitution(const double* beats, const int* nbeats, int64_t n,
     return hipGetLastError();
 }
 
+template <class M>
+static hipError_t eval_of(int tissue, int64_t n, const double* st, const double* t, int kind,
+                          int solve_v, const Stimulus& stim, const ModelParams& P,
+                          const double* gsc, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ionic_eval<M>, dim3((unsigned)((n + MBS - 1) / MBS)), dim3(MBS), 0, s,
+                       tissue, n, st, t, kind, solve_v, stim, kparams<double>(P), gsc, out);
+    return hipGetLastError();
+}
+
+int model_nalgebraic(int model)
+{
+    switch (model) {
+    case MODEL_BO: return Algebraic<BuenoOrovio>::NA;
+    case MODEL_AP: return Algebraic<AlievPanfilov>::NA;
+    default: return 0;
+    }
+}
+
+hipError_t launch_ionic_eval(int model, int tissue, int64_t n, const double* st, const double* t,
+                             int kind, int solve_v, const Stimulus& stim, const ModelParams& P,
+                             const double* gsc, double* out, hipStream_t s)
+{
+#define CG_EVAL(M) eval_of<M>(tissue, n, st, t, kind, solve_v, stim, P, gsc, out, s)
+    switch (model) {
+    case MODEL_BO: return CG_EVAL(BuenoOrovio);
+    case MODEL_TNNP: return CG_EVAL(TNNP);
+    case MODEL_ORD: return CG_EVAL(ORd);
+    case MODEL_AP: return CG_EVAL(AlievPanfilov);
+    case MODEL_FDA: return CG_EVAL(FdaManufactured);
+    case MODEL_CRN: return CG_EVAL(Courtemanche);
+    case MODEL_PYS: return CG_EVAL(PerisYague);
+    case MODEL_STW: return CG_EVAL(Stewart);
+    default: return hipErrorInvalidValue;
+    }
+#undef CG_EVAL
+}
+
 const char* batch_variant_name(int v)
 {
     static const char* names[BATCH_VARIANTS] = {"256 threads", "64 threads",

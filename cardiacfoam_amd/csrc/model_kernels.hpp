@@ -385,6 +385,38 @@ __global__ __launch_bounds__(MBS) void k_ionic_rhs(HexGeom g, T* __restrict__ st
     }
 }
 
+// RATES (kind 0) or ALGEBRAIC (kind 1) variables of model M at n given rows: states
+// [NS][n] (V as state 0, model units), t [n] model time of each row (the single-cell
+// stimulus when solve_v), conductance scales [4][n] or null; out [NV][n].  FP64.
+template <class M>
+__global__ __launch_bounds__(MBS) void k_ionic_eval(int tissue, int64_t n,
+                                                    const double* __restrict__ st,
+                                                    const double* __restrict__ t, int kind,
+                                                    int solve_v, Stimulus stim,
+                                                    KParams<double> P,
+                                                    const double* __restrict__ gsc,
+                                                    double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* p = P.p;
+    double gs[4] = {1.0, 1.0, 1.0, 1.0};
+    if (gsc)
+        for (int k = 0; k < 4; ++k) gs[k] = gsc[k * n + i];
+    double y[M::NS];
+    for (int q = 0; q < M::NS; ++q) y[q] = st[q * n + i];
+    const double istim = solve_v ? stimulus_current(t[i], stim) : 0.0;
+    if (kind == 0) {
+        double d[M::NS];
+        Ops<M>::deriv(y, d, p, tissue, gs, solve_v != 0, istim, Ops<M>::vt(y[0], p));
+        for (int q = 0; q < M::NS; ++q) out[q * n + i] = d[q];
+    } else if constexpr (Algebraic<M>::NA > 0) {
+        double a[Algebraic<M>::NA];
+        Algebraic<M>::eval(y, p, istim, a);
+        for (int q = 0; q < Algebraic<M>::NA; ++q) out[q * n + i] = a[q];
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 inline unsigned grid_8(int64_t n, int bs)
 {

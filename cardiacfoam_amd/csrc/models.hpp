@@ -432,4 +432,47 @@ template <> struct Ops<Stewart> {
     }
 };
 
+// ALGEBRAIC variables of a model in the reference's order, as its computeVariables leaves
+// them at a step's end; NA = 0 where the model's algebraic list is not evaluated on the
+// device.  BuenoOrovio: BuenoOrovio_2008.H:30-43 (names), :166-225 (values);
+// AlievPanfilov: AlievPanfilov_1996.H:28-32, :94-120.  istim: the single-cell stimulus at
+// that time (0 in tissue, where the stimulus comes from the PDE).
+template <class M> struct Algebraic {
+    static constexpr int NA = 0;
+    template <class T> CG_HD static void eval(const T*, const T*, T, T*) {}
+};
+template <> struct Algebraic<BuenoOrovio> {
+    static constexpr int NA = 12;
+    template <class T> CG_HD static void eval(const T* y, const T* p, T istim, T* a)
+    {
+        using B = BuenoOrovio;
+        const T u = y[0];
+        const bool Hv = pos(u - p[B::thv]), Hw = pos(u - p[B::thw]), Ho = pos(u - p[B::tho]),
+                   Hvm = pos(u - p[B::thvm]);
+        a[5] = p[B::tso1] + (p[B::tso2] - p[B::tso1]) * (T(1) + ctanh(p[B::kso] * (u - p[B::uso]))) / T(2);
+        a[6] = Ho ? p[B::to2] : p[B::to1];
+        a[0] = Hv ? -y[1] * (u - p[B::thv]) * (p[B::uu] - u) / p[B::tfi] : T(0);
+        a[1] = Hw ? T(1) / a[5] : (u - p[B::uo]) / a[6];
+        a[2] = Hw ? -y[2] * y[3] / p[B::tsi] : T(0);
+        a[3] = a[0] + a[1] + a[2];
+        a[4] = istim;
+        a[7] = Hvm ? p[B::tv2m] : p[B::tv1m];
+        a[8] = p[B::tw1m] + (p[B::tw2m] - p[B::tw1m]) * (T(1) + ctanh(p[B::kwm] * (u - p[B::uwm]))) / T(2);
+        a[9] = Hw ? p[B::ts2] : p[B::ts1];
+        a[10] = pos(p[B::thvm] - u) ? T(1) : T(0);
+        a[11] = Ho ? p[B::winfstar] : T(1) - u / p[B::twinf];
+    }
+};
+template <> struct Algebraic<AlievPanfilov> {
+    static constexpr int NA = 3;
+    template <class T> CG_HD static void eval(const T* y, const T* p, T istim, T* a)
+    {
+        using A = AlievPanfilov;
+        const T u = y[0];
+        a[0] = p[A::eps0] + p[A::mu1] * y[1] / (p[A::mu2] + u);
+        a[1] = istim;
+        a[2] = A::jion(u, y, p);
+    }
+};
+
 } // namespace cg

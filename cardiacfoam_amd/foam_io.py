@@ -20,6 +20,7 @@ import ctypes as C
 import math
 import os
 import re
+import warnings
 
 import numpy as np
 
@@ -345,22 +346,25 @@ def _canonical(name):
     return out[2:] if len(out) > 2 and out.startswith("av") else out
 
 
-# Vm of the trace from the model's first state (the models' ioVmTransform:
-# BuenoOrovio.H:78-81, AlievPanfilov.H:78-81; the other models store mV)
-VM_TRANSFORM = {"BuenoOrovio": (85.7, -84.0), "AlievPanfilov": (100.0, -80.0)}
 
 
-def trace_plan(model_key, state_names, export=None):
-    """Columns of the trace as (header name, source, index), source "vm", "state" or
-    "rate".  With an export list, ionicVariableCompatibility::resolveVariable
-    (ionicVariableCompatibility.C:213-279) maps each name; without one, the full plan of
-    ionicModelIO.C:170-225 minus its ALGEBRAIC block.  Algebraic variables are not
-    exported by this backend: a name that is neither Vm, a state nor RATES_<state> is an
-    error here (the reference writes algebraic names and ignores unknown ones)."""
+def trace_plan(model_key, state_names, export=None, alg_names=()):
+    """Columns of the trace as (header name, source, index), source "vm", "state", "alg" or
+    "rate".  With an export list, each name resolves as
+    ionicVariableCompatibility::resolveVariable does (ionicVariableCompatibility.C:213-279):
+    Vm-like names, then states, then the algebraic list, then RATES_<state>, each by exact
+    or canonical spelling; without one, the full plan of ionicModelIO.C:170-225: Vm, the
+    states after V, the algebraic list, the rates.  alg_names: the model's algebraic list
+    where the device evaluates it (BuenoOrovio, AlievPanfilov), else empty.  A name that
+    resolves nowhere is dropped with a warning, as exportedFieldNamesRef does
+    (ionicModelIO.C:388-419), where the model's algebraic list is known; for the other
+    models it may name an algebraic variable the device does not evaluate, and is an
+    error."""
     ns = len(state_names)
     if not export:
         cols = [("Vm", "vm", 0)]
         cols += [(state_names[s], "state", s) for s in range(1, ns)]
+        cols += [(a, "alg", k) for k, a in enumerate(alg_names)]
         cols += [("RATES_" + state_names[s], "rate", s) for s in range(ns)]
         return cols
     cols = []
@@ -374,6 +378,11 @@ def trace_plan(model_key, state_names, export=None):
         if s is not None:
             cols.append((name, "state", s))
             continue
+        a = next((k for k, an in enumerate(alg_names)
+                  if name == an or _canonical(name) == _canonical(an)), None)
+        if a is not None:
+            cols.append((name, "alg", a))
+            continue
         if len(name) > 6 and name[:6].upper() == "RATES_":
             rs = name[6:]
             s = next((i for i, sn in enumerate(state_names)
@@ -381,8 +390,11 @@ def trace_plan(model_key, state_names, export=None):
             if s is not None:
                 cols.append((name, "rate", s))
                 continue
-        raise FatalIOError("export variable '%s' is not Vm, a state or RATES_<state> of %s "
-                           "(algebraic variables are not exported by this backend)"
+        if alg_names:
+            warnings.warn("Ignoring unknown ionic model variable '%s'." % name)
+            continue
+        raise FatalIOError("export variable '%s' is not Vm, a state or RATES_<state> of %s, "
+                           "whose algebraic list the device does not evaluate"
                            % (name, model_key))
     return cols
 
@@ -391,13 +403,18 @@ class SingleCellTrace:
     """Trace file of SingleCellSolver: postProcessing/<model>_<tissue>_<suffix>.txt in
     case_dir, header "time <names>", then one fixed-point row (7 decimals) per step."""
 
-    def __init__(self, case_dir, model_key, tissue_name, coeffs, state_names):
+    def __init__(self, case_dir, model_key, tissue_name, coeffs, state_names, alg_names=()):
         export = None
         out = (coeffs or {}).get("outputVariables", {})
         if isinstance(out, dict) and isinstance(out.get("ionic"), dict):
             export = out["ionic"].get("export")
-        self.plan = trace_plan(model_key, state_names, export)
-        self.vm = VM_TRANSFORM.get(model_key, (1.0, 0.0))
+        self.plan = trace_plan(model_key, state_names, export, alg_names)
+        self.needs_alg = any(src == "alg" for _, src, _ in self.plan)
+        # Vm of the trace from the model's first state, the models' ioVmTransform
+        # (BuenoOrovio.H:78-81, AlievPanfilov.H:78-81; the other models store mV)
+        from .ionic import model_units
+        _, voff, vscale, _ = model_units(model_key)
+        self.vm = (vscale, -voff)
         d = os.path.join(case_dir, "postProcessing")
         os.makedirs(d, exist_ok=True)
         self.path = os.path.join(d, "%s_%s_%s.txt" % (model_key, tissue_name,
@@ -405,8 +422,9 @@ class SingleCellTrace:
         with open(self.path, "w") as f:
             f.write("time" + "".join(" " + c[0] for c in self.plan) + "\n")
 
-    def columns(self, states, rates):
-        """[steps, ncols] of the plan from the device trace's states and rates."""
+    def columns(self, states, rates, alg=None):
+        """[steps, ncols] of the plan from the device trace's states and rates (and the
+        algebraic values [steps, nalg] when the plan has any)."""
         out = np.empty((states.shape[0], len(self.plan)))
         a, b = self.vm
         for k, (_, src, i) in enumerate(self.plan):
@@ -414,12 +432,15 @@ class SingleCellTrace:
                 out[:, k] = states[:, 0] * a + b
             elif src == "state":
                 out[:, k] = states[:, i]
+            elif src == "alg":
+                out[:, k] = alg[:, i]
             else:
                 out[:, k] = rates[:, i]
         return out
 
-    def append(self, t, states, rates):
-        rows = np.column_stack([np.asarray(t, dtype=np.float64), self.columns(states, rates)])
+    def append(self, t, states, rates, alg=None):
+        rows = np.column_stack([np.asarray(t, dtype=np.float64),
+                                self.columns(states, rates, alg)])
         with open(self.path, "ab") as f:
             f.write(format_numbers(rows, 7, fixed=True, row=rows.shape[1]))
 

@@ -268,6 +268,57 @@ class IonicModel:
             names.append(buf.value.decode())
         return names
 
+    def algebraic_names(self):
+        """the model's ALGEBRAIC list in the reference's order where the device evaluates it
+        (BuenoOrovio, AlievPanfilov), else []"""
+        buf = C.create_string_buffer(64)
+        out = []
+        for i in range(max(lib().cg_model_nalgebraic(self.model), 0)):
+            check(lib().cg_model_algebraic_name(self.model, i, buf, 64))
+            out.append(buf.value.decode())
+        return out
+
+    def units(self):
+        """(vmul, voff, vscale, t_per_s) of the model (cg_model_units): the V state fed
+        from Vm is (Vm vmul + voff)/vscale, the trace's Vm in mV is V vscale - voff, and
+        model time is t_s t_per_s"""
+        return model_units(self.model)
+
+    def model_time(self, t_s):
+        return np.asarray(t_s, dtype=np.float64) * self.units()[3]
+
+    def model_v(self, Vm):
+        """the V state solveODE feeds a tissue cell from Vm in V (BuenoOrovio.C:119-122)"""
+        vmul, voff, vscale, _ = self.units()
+        return (np.asarray(Vm, dtype=np.float64) * vmul + voff) / vscale
+
+    def eval(self, kind, states, t_model=None):
+        """RATES ("rates") or ALGEBRAIC ("algebraic") of the model at rows of states
+        [nstates, n] (model units, V as state 0) and model times t_model [n]
+        (cg_ionic_eval, on the device): [nvars, n]."""
+        k = {"rates": _capi.RATES, "algebraic": _capi.ALGEBRAIC}[kind]
+        st = np.ascontiguousarray(states, dtype=np.float64)
+        if st.ndim != 2 or st.shape[0] != self.nstates:
+            raise ValueError("states must be [nstates, n]")
+        n = st.shape[1]
+        t = None if t_model is None else np.ascontiguousarray(t_model, dtype=np.float64)
+        nv = self.nstates if k == _capi.RATES else max(lib().cg_model_nalgebraic(self.model), 0)
+        out = np.zeros((max(nv, 1), n))
+        check(lib().cg_ionic_eval(self.h, k, n, st.ctypes.data_as(C.c_void_p),
+                                  None if t is None else t.ctypes.data_as(C.c_void_p),
+                                  out.ctypes.data_as(C.c_void_p)))
+        return out[:nv]
+
+
+def model_units(model):
+    """cg_model_units of a model id or key: (vmul, voff, vscale, t_per_s)"""
+    L = lib()
+    if isinstance(model, str):
+        model = L.cg_model_id(model.encode())
+    out = [C.c_double() for _ in range(4)]
+    check(L.cg_model_units(int(model), *[C.byref(x) for x in out]))
+    return tuple(x.value for x in out)
+
 
 def _handles(models):
     arr = (C.c_void_p * len(models))()

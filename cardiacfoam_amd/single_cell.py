@@ -18,7 +18,8 @@ class SingleCellSolver:
         self.ionic = IonicModel.New(coeffs, 1, self.dt, True, precision=precision)
         self.write_after = float(coeffs.get("writeAfterTime", 0.0))
         self.trace = SingleCellTrace(case_dir, self.ionic.model_key, self.ionic.tissue_name,
-                                     coeffs, self.ionic.stateVariableNames())
+                                     coeffs, self.ionic.stateVariableNames(),
+                                     self.ionic.algebraic_names())
         self.t = 0.0
         self.steps = 0
 
@@ -38,7 +39,13 @@ class SingleCellSolver:
             ts = np.asarray(ts)
             keep = ts >= self.write_after
             if keep.any():
-                self.trace.append(ts[keep], states[keep], rates[keep])
+                alg = None
+                if self.trace.needs_alg:
+                    # computeVariables at each step's end (its stimulus included), on the
+                    # device from the recorded states
+                    alg = self.ionic.eval("algebraic", states[keep].T,
+                                          self.ionic.model_time(ts[keep])).T
+                self.trace.append(ts[keep], states[keep], rates[keep], alg)
             self.t = t
             self.steps += k
         return self.trace.path

@@ -476,10 +476,44 @@ class Monodomain:
             out[inside] += inten[b]
         return out
 
+    def exported_fields(self):
+        """The ionic fields of outputVariables.ionic.export as MyocardiumDomain::exportStates
+        fills them at an output time, after the step (electroModel.C:321-330,
+        ionicModelIO.C:434-500): [(name, values)] for the names that resolve
+        (exportedFieldNamesRef), each the raw model-unit value: Vm-like names and states
+        from the cells' states, whose V is the one the last step fed them (the model's V of
+        Vm at the step's start, BuenoOrovio.C:119-122); algebraic variables and RATES_ as
+        computeVariables leaves them at the step's end (BuenoOrovio.C:134-147), with no
+        stimulus and dV/dt = 0 in tissue mode."""
+        from . import foam_io
+        out = (self.coeffs or {}).get("outputVariables", {})
+        ionic = out.get("ionic") if isinstance(out, dict) else None
+        export = ionic.get("export") if isinstance(ionic, dict) else None
+        if not export:
+            return []
+        plan = foam_io.trace_plan(self.ionic.model_key, self.ionic.stateVariableNames(), export,
+                                  self.ionic.algebraic_names())
+        S = self.ionic.states()
+        if self.time[1] > 0:
+            S[0] = self.ionic.model_v(self.field(_capi.FIELD_VM_OLD))
+        srcs = {src for _, src, _ in plan}
+        R = self.ionic.eval("rates", S) if "rate" in srcs else None
+        A = self.ionic.eval("algebraic", S) if "alg" in srcs else None
+        got = []
+        for name, src, i in plan:
+            got.append((name, S[i] if src in ("vm", "state") else R[i] if src == "rate"
+                        else A[i]))
+        return got
+
     def write_fields(self, case_dir, precision=6, time_name=None):
-        """MyocardiumDomain::write (myocardiumDomain.C:621-634): Vm,
-        externalStimulusCurrent, ionicCurrent and activationTime as OpenFOAM ASCII
-        volScalarFields in <case_dir>/<timeName>/.  Returns that directory."""
+        """MyocardiumDomain::write (myocardiumDomain.C:617-638): Vm,
+        externalStimulusCurrent, ionicCurrent and activationTime, then the exported ionic
+        fields (exported_fields, dimensionless) as OpenFOAM ASCII volScalarFields in
+        <case_dir>/<timeName>/.  Returns that directory.  The write order is the
+        reference's, so an export list naming Vm (the NiedererEtAl2012 case's (Vm Jsi))
+        leaves the file Vm holding the raw V state in model units, as the reference's
+        second write of that name does (electroModel.C:216-224 writes the registered
+        fields first, MyocardiumDomain::write the exported ones after Vm_)."""
         from . import foam_io
         tn = time_name or foam_io.time_name(self.time[0])
         d = os.path.join(case_dir, tn)
@@ -487,6 +521,9 @@ class Monodomain:
                                           self.external_stimulus_current()),
                         ("ionicCurrent", self.Iion), ("activationTime", self.activationTime)):
             foam_io.write_field(os.path.join(d, name), v, location=tn, precision=precision)
+        for name, v in self.exported_fields():
+            foam_io.write_field(os.path.join(d, name), v, name=name, location=tn,
+                                dimensions=(0,) * 7, precision=precision)
         return d
 
     def read_fields(self, case_dir, time_name="0"):

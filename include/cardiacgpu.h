@@ -73,6 +73,14 @@ int cg_tissue_flag(const char* tissue_word);
 int cg_model_nstates(int model);
 /* name of state i (the *_STATES_NAMES tables, e.g. BuenoOrovio_2008.H:24-30) */
 int cg_model_state_name(int model, int i, char* buf, size_t len);
+/* The model's units (host only; any pointer may be NULL).  solveODE feeds a tissue cell
+ * the V state (Vm * vmul + voff) / vscale from Vm in V (BuenoOrovio.C:119-122: vmul 1000,
+ * voff 84, vscale 85.7; AlievPanfilov 1000, 80, 100; the mV models 1000, 0, 1; the
+ * manufactured model 1, 0, 1); ioVmTransform gives the trace's Vm in mV as
+ * V * vscale - voff (BuenoOrovio.H:78-81, AlievPanfilov.H:78-81); model time is PDE time
+ * in s times t_per_s (ms; AlievPanfilov 1000/12.9; s for the manufactured model,
+ * BuenoOrovio.C:107-108, AlievPanfilov.C:107-108, monodomainFDAManufactured.C:98). */
+int cg_model_units(int model, double* vmul, double* voff, double* vscale, double* t_per_s);
 
 /* Replaces <Model>::<Model>(dict, nIntegrationPoints, initialDeltaT, solveVm)
  * (ionicModel.H:184-196; BuenoOrovio.C:45-81, TNNP.C:44-78, ORd.C:45-80,
@@ -196,6 +204,25 @@ int cg_ionic_get_restitution(cg_ionic* h, double* apd_di, uint8_t* flags);
  * values (CG_MIXED handles keep their slow concentrations on the device as FP32 offsets
  * from the model's initial values; both calls convert).
  * Replaces the io*Ptr hooks / exportStates (ionicModel.H:115-168, 584-638). */
+/* Derived variables of the handle's model (the reference's RATES_ and ALGEBRAIC_ lists,
+ * ionicModel.H:115-168, as computeVariables leaves them at a step's end) evaluated on the
+ * device in FP64 at n rows given by the caller: states [nstates][n] (host, model units, V
+ * as state 0, e.g. the cells' states with their last Vm), t_model [n] the model time of
+ * each row (single-cell handles: the stimulus of the handle's protocol at that time; may
+ * be NULL for tissue handles, whose ODEs see no stimulus).  kind CG_RATES: dy/dt [nstates][n]
+ * (dV/dt = 0 in tissue mode, as the reference's solver leaves it); CG_ALGEBRAIC: the
+ * model's algebraic list [cg_model_nalgebraic][n] in the reference's order, evaluated for
+ * BuenoOrovio (12) and AlievPanfilov (3); other models return CG_ERR_MODEL.  With n equal
+ * to the handle's cells the per-cell conductance scales apply.  Synchronous. */
+enum { CG_RATES = 0, CG_ALGEBRAIC = 1 };
+int cg_ionic_eval(cg_ionic* h, int kind, int64_t n, const double* states,
+                  const double* t_model, double* out);
+/* Length of the model's algebraic list evaluated on the device (0: none; -1: unknown
+ * model) and its names in the reference's order (BuenoOrovio_2008.H:30-43,
+ * AlievPanfilov_1996.H:28-32). */
+int cg_model_nalgebraic(int model);
+int cg_model_algebraic_name(int model, int i, char* buf, size_t len);
+
 int cg_ionic_get_states(cg_ionic* h, int first_var, int n_vars, double* host_soa);
 int cg_ionic_set_states(cg_ionic* h, int first_var, int n_vars, const double* host_soa);
 

@@ -97,6 +97,8 @@ def lib():
                                       C.c_double, C.c_long]
         L.orc_rkf45_solve.restype = C.c_long
         L.orc_ode_solve.argtypes = [C.c_int] + L.orc_rkf45_solve.argtypes
+        L.orc_model_algebraic.argtypes = [C.c_int, C.c_double, _dp, _dp, C.POINTER(Stim), _dp]
+        L.orc_model_algebraic.restype = C.c_int
         L.orc_ode_solve.restype = C.c_long
         L.orc_ionic_tissue_step.argtypes = [C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long,
                                             C.c_double, C.c_double, _dp, _dp, C.c_int, C.c_int,
@@ -175,6 +177,20 @@ def model_eval(model, tissue, t, c, y, solve_vm=False, stim=None):
 
 
 EULER = 2   # integrator / method value of OpenFOAM's adaptive Euler (1: RKF45, 0: RL)
+
+
+def model_algebraic(model, tissue, t, y, stim=None, consts=None):
+    """ALGEBRAIC variables (reference order) of BuenoOrovio / AlievPanfilov at model time t;
+    None for a model without a restated list"""
+    c, _ = model_init(model, tissue)
+    if consts is not None:
+        c = np.array(consts, dtype=np.float64)
+    yy = np.zeros(MAX_STATES)
+    yy[:len(y)] = y
+    a = np.zeros(64)
+    st = stim if stim is not None else make_stim()
+    n = lib().orc_model_algebraic(model, t, c, yy, C.byref(st), a)
+    return None if n < 0 else a[:n]
 
 
 def ode_solve(method, model, tissue, y, x0, x1, dx_try, abs_tol, rel_tol, solve_vm=True,

@@ -1272,6 +1272,45 @@ int orc_model_const_index(int model, const char* name)
     return -1;
 }
 
+/* ALGEBRAIC variables of the models that have them in the reference's order
+ * (BuenoOrovio_2008.H:30-43 names, :166-225 values; AlievPanfilov_1996.H:28-32, :94-120);
+ * returns their count, or -1 for a model whose algebraic list is not restated here.
+ * Parity pins (tests/test_oracle_algebraic.py): no reference output holds these columns;
+ * they are tied to the right-hand sides the golden traces pin (total current, stimulus,
+ * and the Rush-Larsen (x_inf, tau) pairs of each Heaviside region). */
+int orc_model_algebraic(int model, double t, const double* c, const double* y,
+                        const orc_stim_t* st, double* a)
+{
+    if (model == ORC_BO) {
+        const double u = y[0], v = y[1], w = y[2], s = y[3];
+        const double Hv = H(u - c[BO_thv]), Hw = H(u - c[BO_thw]);
+        const double Ho = H(u - c[BO_tho]), Hvm = H(u - c[BO_thvm]);
+        a[5] = c[BO_tso1] + (c[BO_tso2] - c[BO_tso1]) *
+               (1.0 + tanh(c[BO_kso] * (u - c[BO_uso]))) / 2.0;              /* tauSo */
+        a[6] = (1.0 - Ho) * c[BO_to1] + Ho * c[BO_to2];                      /* tauO */
+        a[0] = -v * Hv * (u - c[BO_thv]) * (c[BO_uu] - u) / c[BO_tfi];       /* Jfi */
+        a[1] = (u - c[BO_uo]) * (1.0 - Hw) / a[6] + Hw / a[5];               /* Jso */
+        a[2] = -Hw * w * s / c[BO_tsi];                                      /* Jsi */
+        a[3] = a[0] + a[1] + a[2];                                           /* Jion */
+        a[7] = (1.0 - Hvm) * c[BO_tv1m] + Hvm * c[BO_tv2m];                  /* tauVMinus */
+        a[8] = c[BO_tw1m] + (c[BO_tw2m] - c[BO_tw1m]) *
+               (1.0 + tanh(c[BO_kwm] * (u - c[BO_uwm]))) / 2.0;              /* tauWMinus */
+        a[9] = (1.0 - Hw) * c[BO_ts1] + Hw * c[BO_ts2];                      /* tauS */
+        a[10] = H(c[BO_thvm] - u);                                           /* vInfty */
+        a[11] = (1.0 - Ho) * (1.0 - u / c[BO_twinf]) + Ho * c[BO_winfstar];  /* wInfty */
+        a[4] = orc_stimulus(t, st);                                          /* Istim */
+        return 12;
+    }
+    if (model == ORC_AP) {
+        const double u = y[0], rr = y[1];
+        a[0] = c[3] + c[5] * rr / (c[6] + u);                               /* AV_eps */
+        a[1] = orc_stimulus(t, st);                                          /* Istim */
+        a[2] = c[4] * u * (u - c[7]) * (u - 1.0) + rr * u;                   /* Iion_cm */
+        return 3;
+    }
+    return -1;
+}
+
 void orc_model_eval(int model, int tissue, double t, const double* c, const double* y,
                     int solve_vm, const orc_stim_t* st, double* r, int* g, double* xi,
                     double* ta, double* iion, double* istim)

@@ -74,6 +74,11 @@ void orc_model_eval(int model, int tissue, double t, const double* c, const doub
 /* stimulus protocol, stimulusIO.C:238-286 */
 double orc_stimulus(double t, const orc_stim_t* s);
 
+/* ALGEBRAIC variables in the reference's order (BuenoOrovio: 12, AlievPanfilov: 3) at
+ * model time t; returns their count, or -1 for a model without a restated list. */
+int orc_model_algebraic(int model, double t, const double* c, const double* y,
+                        const orc_stim_t* stim, double* alg);
+
 /* ----------------------------------------------------------- integrators */
 /* One Rush-Larsen / forward-Euler step of dt (model units) on one cell. */
 void orc_rl_step(int model, int tissue, double t, double dt, const double* c, double* y,

@@ -81,3 +81,18 @@ def test_adjoint_stale_backward_check():
     adj.generation = -1                       # a forward run that failed
     with pytest.raises(RuntimeError):
         adj.check_backward(3)
+
+
+def test_model_units_follow_the_reference():
+    """cg_model_units: BuenoOrovio.C:119-122 / BuenoOrovio.H:78-81 ((Vm*1000 + 84)/85.7,
+    85.7 u - 84, ms), AlievPanfilov.C:107-122 / AlievPanfilov.H:78-81 ((Vm*1000 + 80)/100,
+    100 u - 80, t*1000/12.9), the mV models (Vm*1000, ms) and the manufactured model (V =
+    Vm, s: monodomainFDAManufactured.C:98)."""
+    from cardiacfoam_amd.ionic import model_units
+    assert model_units("BuenoOrovio") == (1000.0, 84.0, 85.7, 1000.0)
+    assert model_units("AlievPanfilov") == (1000.0, 80.0, 100.0, 1000.0 / 12.9)
+    for key in ("TNNP", "ORd", "Courtemanche", "PerisYague_2022", "Stewart"):
+        assert model_units(key) == (1000.0, 0.0, 1.0, 1000.0), key
+    assert model_units("monodomainFDAManufactured") == (1.0, 0.0, 1.0, 1.0)
+    with pytest.raises(_capi.CgError):
+        model_units(8)

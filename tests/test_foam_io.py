@@ -186,8 +186,17 @@ def test_protocol_suffix_and_trace_plan(fio):
     full = fio.trace_plan("BuenoOrovio", names, None)
     assert [c[0] for c in full] == ["Vm", "v", "w", "s", "RATES_u", "RATES_v", "RATES_w",
                                     "RATES_s"]
+    # without the model's algebraic list an unresolved name may be one: refused
     with pytest.raises(KeyError):
-        fio.trace_plan("BuenoOrovio", names, ["Vm", "Jsi"])
+        fio.trace_plan("TNNP", ["V", "K_i"], ["V", "i_Na"])
+    # with it, an unknown name is dropped with a warning (ionicModelIO.C:388-419)
+    alg = ["Jfi", "Jso", "Jsi", "Jion", "Istim", "tauSo", "tauO", "tauVMinus", "tauWMinus",
+           "tauS", "vInfty", "wInfty"]
+    with pytest.warns(UserWarning, match="nothing"):
+        plan = fio.trace_plan("BuenoOrovio", names, ["Vm", "Jsi", "nothing", "J_ion"], alg)
+    assert plan == [("Vm", "vm", 0), ("Jsi", "alg", 2), ("J_ion", "alg", 3)]
+    full = fio.trace_plan("BuenoOrovio", names, None, alg)
+    assert [c[0] for c in full][4:16] == alg
 
 
 def test_trace_file_layout_and_regression_check(fio, tmp_path):
