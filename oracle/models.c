@@ -5,13 +5,16 @@
  * and the single-cell stimulus.
  *
  * Parity pins (see tests/test_oracle_*.py):
- *   BuenoOrovio  - singleCell.reference golden (RKF45 mode), NiedererEtAl2012.reference
- *                  golden (tissue), hand-computed RHS values.
- *   AlievPanfilov- closed-form RHS (cubic), hand-computed values.
- *   TNNP, ORd    - no golden in the reference; pinned by invariants: rest stability,
- *                  RL exactness, published action-potential shape (APD90, peak,
- *                  resting potential), charge/concentration conservation.  The exact
- *                  generated expressions remain "parity unpinned" beyond that.
+ *   every model  - the reference's own generated expressions: each model's initConsts and
+ *                  the function its derivatives() calls, evaluated from the header text
+ *                  at seeded states (tests/golden/make_model_vectors.py); initial states
+ *                  exact, rates in tissue and single-cell mode to 1e-13
+ *                  (tests/test_oracle_model_vectors.py).
+ *   BuenoOrovio  - also the singleCell.reference golden (RKF45 mode) and the
+ *                  NiedererEtAl2012.reference golden (tissue).
+ *   AlievPanfilov- also the reference's trace AlievPanfilov_myocyte_S1_1000.txt.
+ *   TNNP, ORd    - also invariants: rest stability, RL exactness, published
+ *                  action-potential shape, charge/concentration conservation.
  */
 #include <math.h>
 #include <string.h>
@@ -1034,8 +1037,8 @@ static void pys_eval(double t, const double* c, const double* y, int solve_vm,
  * Parity pins (tests/test_oracle_models.py): charge conservation over the cytosolic,
  * subspace and SR compartments; the GHK quotient of ICaL continuous through V = 15 mV
  * from both sides; Rush-Larsen and RKF45 agreeing on the automatic rhythm from the
- * reference's initial state.  Gate kinetics: parity unpinned beyond device-vs-oracle
- * agreement (no published trace for the reference's constants). */
+ * reference's initial state; the reference's generated expressions at seeded states
+ * (tests/test_oracle_model_vectors.py). */
 enum { SW_V, SW_y, SW_Xr1, SW_Xr2, SW_Xs, SW_m, SW_h, SW_j, SW_d, SW_f, SW_f2, SW_fCass,
        SW_s, SW_r, SW_Cai, SW_CaSR, SW_Cass, SW_Rp, SW_Nai, SW_Ki, SW_NS };
 enum { SC_KpCa, SC_gpCa, SC_gCaL, SC_gbca, SC_Bufc, SC_Bufsr, SC_Bufss, SC_Cao, SC_EC,
