@@ -71,6 +71,8 @@ class IonicModel:
         self.tissue = L.cg_tissue_flag(tissue.encode())
         self.n = int(n)
         self.solve_vm = bool(solve_vm)
+        # tissue handles: callable giving the Vm (V) the last step was fed, or None
+        self._vm_fed = None
         self.precision = precision
         prec = {"fp64": _capi.FP64, "mixed": _capi.MIXED}[precision]
         h = C.c_void_p()
@@ -126,6 +128,14 @@ class IonicModel:
         if len(Vm) != self.n or len(Im) != self.n:
             raise ValueError("field size %d/%d != nIntegrationPoints %d" % (len(Vm), len(Im), self.n))
         check(lib().cg_ionic_step(self.h, float(stepStartTime), float(deltaT), pv, pi, fv))
+        # the V state of the step (BuenoOrovio.C:119-122) for the io hooks: a copy, since
+        # the caller's Vm moves on with the diffusion step
+        if fv:
+            v = Vm.detach().clone()
+            self._vm_fed = lambda v=v: v.cpu().numpy()
+        else:
+            v = np.array(Vm, dtype=np.float64, copy=True)
+            self._vm_fed = lambda v=v: v
 
     ODE_SOLVERS = {"RushLarsen": _capi.ODE_RUSH_LARSEN, "RKF45": _capi.ODE_RKF45,
                    "Euler": _capi.ODE_EULER}
@@ -277,6 +287,69 @@ class IonicModel:
             check(lib().cg_model_algebraic_name(self.model, i, buf, 64))
             out.append(buf.value.decode())
         return out
+
+    def io_states(self):
+        """STATES_ [nstates, n] as the reference holds them after a step (its ioStatesPtr,
+        ionicModel.H:112-118): the device states, with a tissue cell's V the one its last
+        step was fed ((Vm vmul + voff)/vscale, BuenoOrovio.C:119-122)"""
+        S = self.states()
+        vm = self._vm_fed() if (self._vm_fed is not None and not self.solve_vm) else None
+        if vm is not None:
+            S[0] = self.model_v(vm)
+        return S
+
+    def stateVariableValues(self, integrationPtI=0):
+        """ionicModel.H:391-406: the states of one integration point"""
+        if not 0 <= integrationPtI < self.n:
+            raise FatalIOError("Requested integration point %d but valid range is [0, %d]."
+                               % (integrationPtI, self.n - 1))
+        return self.io_states()[:, integrationPtI].copy()
+
+    def algebraicVariableNames(self):
+        return self.algebraic_names()
+
+    def vmRest(self):
+        """ionicModel.H:305-316: cell 0's V in volts through the model's Vm transform (its
+        initial state until the first step)"""
+        _, voff, vscale, _ = self.units()
+        return float(self.io_states()[0, 0] * vscale - voff) * 1e-3
+
+    def _cai_index(self):
+        # ionicVariableCompatibility.C:154-181 findCaiStateIndex
+        from .foam_io import _canonical
+        for i, nm in enumerate(self.stateVariableNames()):
+            c = _canonical(nm)
+            if c in ("cai", "calciumcai", "intracellularcai", "intracellularcalcium") or \
+                    "cai" in c:
+                return i
+        return -1
+
+    def hasSignal(self, s):
+        """ionicModel.C:120-148 for the coupling signals "VM" (every device model: V is
+        state 0 under the model's Vm transform) and "CAI" (a state named like Ca_i)"""
+        if s == "VM":
+            return True
+        if s == "CAI":
+            return self._cai_index() >= 0
+        return False
+
+    def signals(self, s):
+        """ionicModel.C:150-204 signal(i, s) of every integration point: VM in mV, CAI in
+        mM; a signal the model does not provide is a FatalError"""
+        if not self.hasSignal(s):
+            raise FatalIOError("Requested coupling signal %s from ionicModel, but this ionic "
+                               "model does not provide it." % s)
+        S = self.io_states()
+        if s == "VM":
+            _, voff, vscale, _ = self.units()
+            return S[0] * vscale - voff
+        return S[self._cai_index()]
+
+    def signal(self, i, s):
+        if not 0 <= i < self.n:
+            raise FatalIOError("Requested coupling signal index i=%d but valid "
+                               "integration-point range is [0, %d]." % (i, self.n - 1))
+        return float(self.signals(s)[i])
 
     def units(self):
         """(vmul, voff, vscale, t_per_s) of the model (cg_model_units): the V state fed

@@ -7,7 +7,9 @@ electroProperties keys as MyocardiumDomain / MonodomainSolver
 (myocardiumDomain.C:180-277, monodomainSolver.C:48-100, stimulusIO.C:305-390).
 """
 import ctypes as C
+import functools
 import os
+import weakref
 
 import numpy as np
 
@@ -256,6 +258,13 @@ def fv_schemes_selection(fv_schemes, general_mesh=False):
     return {"ddt": ddt, "grad": grad, "laplacian": lap, "interpolation": interp}
 
 
+def _vm_fed_by(domain_ref):
+    d = domain_ref()
+    if d is None or d.time[1] == 0:
+        return None
+    return d.field(_capi.FIELD_VM_OLD)
+
+
 class Monodomain:
     """Device monodomain tissue: MyocardiumDomain state + MonodomainSolver operator."""
 
@@ -301,6 +310,9 @@ class Monodomain:
             raise ValueError("dt is required for the implicit algorithm")
         self.dt = float(dt)
         self.ionic = IonicModel.New(coeffs, mesh.n, self.dt, False, precision=precision)
+        # the tissue feeds its cells V^n each step: the ionic io hooks read it back (a weak
+        # reference, so the handles are freed with the domain)
+        self.ionic._vm_fed = functools.partial(_vm_fed_by, weakref.ref(self))
         if coeffs.get("singleCellStimulus"):
             raise FatalIOError("Detected active ionic-model stimulus protocol while using a "
                                "reaction-diffusion myocardium domain.")
@@ -493,9 +505,7 @@ class Monodomain:
             return []
         plan = foam_io.trace_plan(self.ionic.model_key, self.ionic.stateVariableNames(), export,
                                   self.ionic.algebraic_names())
-        S = self.ionic.states()
-        if self.time[1] > 0:
-            S[0] = self.ionic.model_v(self.field(_capi.FIELD_VM_OLD))
+        S = self.ionic.io_states()
         srcs = {src for _, src, _ in plan}
         R = self.ionic.eval("rates", S) if "rate" in srcs else None
         A = self.ionic.eval("algebraic", S) if "alg" in srcs else None

@@ -185,3 +185,38 @@ def test_tissue_exported_fields(gpu, orc, tmp_path):
             assert "dimensions      [0 0 0 0 0 0 0];" in fh.read()
     assert np.max(np.abs(vm - vm_old)) > 0.0           # the tissue moved over the last step
     assert (u > 0.5).sum() > 0                         # some cells are activated
+
+
+def test_coupling_signals_and_io_states(gpu, orc):
+    """ionicModel::hasSignal / signal (ionicModel.C:120-204) on the host mirror: VM in mV
+    through the model's Vm transform of the V state the last step was fed, CAI from the
+    state named like Ca_i (TNNP: Ca_i; none for BuenoOrovio); vmRest from the initial
+    state; stateVariableValues of one integration point."""
+    _say("coupling signals")
+    n = 64
+    Vm = np.linspace(-0.086, 0.02, n)
+    Im = np.zeros(n)
+    bo = gpu.IonicModel("BuenoOrovio", "epicardialCells", n)
+    assert bo.vmRest() == pytest.approx(-0.084, abs=1e-15)      # u = 0: 85.7*0 - 84 mV
+    bo.solveODE(0.0, 1e-5, Vm, Im)
+    Vm_next = Vm + 1e-3                  # the caller's Vm moves on; the fed copy does not
+    np.testing.assert_allclose(bo.signals("VM"), Vm * 1000.0, rtol=0, atol=1e-12)
+    assert bo.signal(5, "VM") == pytest.approx(Vm[5] * 1000.0, abs=1e-12)
+    assert bo.hasSignal("VM") and not bo.hasSignal("CAI")
+    from cardiacfoam_amd import FatalIOError
+    with pytest.raises(FatalIOError):
+        bo.signals("CAI")
+    with pytest.raises(FatalIOError):
+        bo.signal(n, "VM")
+    tn = gpu.IonicModel("TNNP", "epicardialCells", n)
+    assert tn.hasSignal("CAI")
+    c, y0 = orc.model_init(orc.TNNP, orc.EPI)
+    assert tn.vmRest() == pytest.approx(y0[0] * 1e-3, rel=1e-15)
+    tn.solveODE(0.0, 2e-5, Vm_next, Im)
+    S = tn.states()
+    names = tn.stateVariableNames()
+    np.testing.assert_array_equal(tn.signals("CAI"), S[names.index("Ca_i")])
+    np.testing.assert_allclose(tn.signals("VM"), Vm_next * 1000.0, rtol=1e-15)
+    v = tn.stateVariableValues(7)
+    assert v.shape == (len(names),) and v[0] == pytest.approx(Vm_next[7] * 1000.0, rel=1e-15)
+    np.testing.assert_array_equal(v[1:], S[1:, 7])
