@@ -640,33 +640,44 @@ def run_model_leg(cf, torch, w, sigma, model, steps=100, warmup=100, prof_steps=
     return out
 
 
-def run_full_window(cf, torch, w, sigma, steps=10000, every=500, control_tol=1e-11):
+def run_full_window(cf, torch, w, sigma, steps=10000, every=500, control_tol=1e-11,
+                    band_v=5e-4):
     """The whole simulated window of the workload (10,000 steps = 200 ms at dt 0.02 ms)
     on both paths side by side: Vm L-inf of mixed against FP64 at every `every` steps
     (max over the checkpoints), activation times of all cells at the end, and each path's
-    wall time over the window (checkpoint downloads excluded).  A third, FP64 run at a
-    tighter solver tolerance is the control: cells whose activation time moves by more
-    than 0.05 ms between the two FP64 runs are ill-conditioned (their upstroke stalls at
-    0 mV); the largest differences are listed with their cells."""
+    wall time over the window (checkpoint downloads excluded).
+
+    The activation gate is strict against a control: FP64 at the tighter solver tolerance
+    control_tol, run three times with the activation threshold at 0 (the reference's) and
+    at -band_v and +band_v (band_v = 0.5 mV, the Vm gate).  A trajectory within band_v of
+    the control crosses 0 V somewhere in [t(-band_v), t(+band_v)] of the control.  Where
+    that interval is within 0.05 ms of t(0) the cell is conditioned and mixed must agree
+    with the control to 0.05 ms; elsewhere the crossing grazes 0 V (Vm peaks at ~0 mV,
+    dips, and crosses later) and mixed must fall inside the interval, +-0.05 ms.  A cell
+    activated in one run only must have a control that never exceeds +band_v in the
+    window.  The FP64 path at the working tolerance is classified the same way."""
+    runs = (("fp64", "fp64", w["tol"] or 1e-8, 0.0), ("mixed", "mixed", w["tol"] or 1e-8, 0.0),
+            ("control", "fp64", control_tol, 0.0), ("control_lo", "fp64", control_tol, -band_v),
+            ("control_hi", "fp64", control_tol, band_v))
     doms = {}
-    for prec, tol in (("fp64", w["tol"] or 1e-8), ("mixed", w["tol"] or 1e-8),
-                      ("control", control_tol)):
-        doms[prec] = cf.Monodomain(cf.HexMesh(*w["shape"], w["h"]), coeffs_for(w, sigma),
-                                   dt=w["dt"], precision="fp64" if prec == "control" else prec,
-                                   tol=tol, max_iter=20000)
+    for name, prec, tol, thr in runs:
+        doms[name] = cf.Monodomain(cf.HexMesh(*w["shape"], w["h"]), coeffs_for(w, sigma),
+                                   dt=w["dt"], precision=prec, tol=tol, max_iter=20000)
+        if thr:
+            doms[name].set_activation_threshold(thr)
     wall = {"fp64": 0.0, "mixed": 0.0}
     worst, worst_at, trace = 0.0, 0, []
     done = 0
     while done < steps:
         k = min(every, steps - done)
-        for prec, dom in doms.items():
+        for name, dom in doms.items():
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             dom.step(k)
             dom.sync()
             torch.cuda.synchronize()
-            if prec in wall:
-                wall[prec] += time.perf_counter() - t0
+            if name in wall:
+                wall[name] += time.perf_counter() - t0
         done += k
         dv = float(np.max(np.abs(doms["fp64"].Vm - doms["mixed"].Vm)) * 1e3)
         trace.append([done, round(dv, 5)])
@@ -674,47 +685,69 @@ def run_full_window(cf, torch, w, sigma, steps=10000, every=500, control_tol=1e-
             worst, worst_at = dv, done
         print("[bench] full window: %d steps, Vm Linf %.4f mV" % (done, dv), file=sys.stderr,
               flush=True)
-    a64, a32, ac = (doms[p].activationTime for p in ("fp64", "mixed", "control"))
+    act = {name: dom.activationTime for name, dom in doms.items()}
+    vm_ctrl = {p: float(np.max(np.abs(doms[p].Vm - doms["control"].Vm)) * 1e3)
+               for p in ("fp64", "mixed")}
+    del doms
+    a64, a32, ac, lo, hi = (act[p] for p in ("fp64", "mixed", "control", "control_lo",
+                                              "control_hi"))
+    t_end = steps * w["dt"]
+    tol_t = 5e-5
     both = (a64 > 0) & (a32 > 0)
     d = np.where(both, np.abs(a64 - a32), 0.0)
-    dc = np.where((a64 > 0) & (ac > 0), np.abs(a64 - ac), 0.0)
-    ill = dc > 5e-5
-    miss = (a64 > 0) != (a32 > 0)
-    t_end = steps * w["dt"]
-    first_miss = float(np.min(np.maximum(a64, a32)[miss]) * 1e3) if miss.any() else None
+    # conditioning of each cell's activation in the control
+    cond = (ac > 0) & (lo > 0) & (hi > 0) & (hi - ac <= tol_t) & (ac - lo <= tol_t)
+
+    def gate_vs_control(a):
+        act_both = (a > 0) & (ac > 0)
+        err = np.where(act_both, np.abs(a - ac), 0.0)
+        bad_cond = act_both & cond & (err > tol_t)
+        inside = (a >= lo - tol_t) & ((hi <= 0) | (a <= hi + tol_t))
+        bad_graze = act_both & ~cond & ~inside
+        only_a = (a > 0) & (ac <= 0)
+        only_c = (a <= 0) & (ac > 0)
+        bad_miss = (only_a & ~((lo > 0) & (a >= lo - tol_t))) | (only_c & (hi > 0))
+        out = {"activation_linf_ms": float(err.max() * 1e3),
+               "conditioned_activation_linf_ms": float(np.where(cond, err, 0.0).max() * 1e3),
+               "cells_over_0.05ms": int(np.count_nonzero(err > tol_t)),
+               "grazing_cells_over_0.05ms": int(np.count_nonzero(act_both & ~cond &
+                                                                 (err > tol_t))),
+               "grazing_cells_outside_band": int(np.count_nonzero(bad_graze)),
+               "conditioned_cells_over_0.05ms": int(np.count_nonzero(bad_cond)),
+               "activation_set_mismatch": int(np.count_nonzero(only_a | only_c)),
+               "mismatch_outside_band": int(np.count_nonzero(bad_miss))}
+        out["passed"] = bool(not bad_cond.any() and not bad_graze.any() and
+                             not bad_miss.any())
+        return out, err
+
+    g32, e32 = gate_vs_control(a32)
+    g64, e64 = gate_vs_control(a64)
     n = int(np.prod(w["shape"]))
+    nx, ny = w["shape"][0], w["shape"][1]
     out = {"steps": steps, "simulated_ms": t_end * 1e3, "checkpoint_every": every,
            "vm_linf_max_mV": worst, "vm_linf_max_at_step": worst_at,
            "vm_linf_trace_mV": trace,
-           "activated_cells": {p: int(np.count_nonzero(a > 0)) for p, a in (("fp64", a64), ("mixed", a32))},
-           "activation_linf_ms": float(d.max() * 1e3),
-           "activation_p999_ms": float(np.percentile(d[both], 99.9) * 1e3) if both.any() else 0.0,
-           "activation_cells_over_0.05ms": int(np.count_nonzero(d > 5e-5)),
-           "control_fp64_tol": control_tol,
-           "control_cells_over_0.05ms": int(np.count_nonzero(ill)),
-           "control_activation_linf_ms": float(dc.max() * 1e3),
-           "activation_set_mismatch": int(np.count_nonzero(miss)),
-           "mismatch_earliest_activation_ms": first_miss,
+           "vm_linf_final_vs_control_mV": vm_ctrl,
+           "activated_cells": {p: int(np.count_nonzero(act[p] > 0)) for p in ("fp64", "mixed",
+                                                                             "control")},
+           "activation_linf_vs_fp64_ms": float(d.max() * 1e3),
+           "control_fp64_tol": control_tol, "band_mV": band_v * 1e3,
+           "control_conditioned_cells": int(np.count_nonzero(cond)),
+           "control_grazing_cells": int(np.count_nonzero((ac > 0) & ~cond)),
+           "mixed_vs_control": g32, "fp64_vs_control": g64,
            "cell_steps_per_s": {p: n * steps / wall[p] for p in wall}}
-    nx, ny = w["shape"][0], w["shape"][1]
     out["activation_outliers"] = [
         {"ijk": [int(i % nx), int(i // nx % ny), int(i // (nx * ny))],
-         "fp64_ms": round(float(a64[i]) * 1e3, 4), "mixed_ms": round(float(a32[i]) * 1e3, 4),
-         "control_ms": round(float(ac[i]) * 1e3, 4)}
-        for i in np.argsort(d)[::-1][:16] if d[i] > 5e-5]
-    # The strict bound asks every activated cell to agree within 0.05 ms.  A cell whose
-    # upstroke stalls at 0 mV (its FP64 activation time then falls within 1e-4 ms of a
-    # step boundary) crosses the threshold a step-count later or earlier under any
-    # perturbation, the control's solver tolerance included; the robust bound asks it of
-    # 99.9 % of the cells.  Mismatched cells must activate within the window's last 0.1 ms.
-    edge_ok = out["activation_set_mismatch"] == 0 or first_miss >= (t_end - 1e-4) * 1e3
-    out["gate"] = ("Vm Linf <= 0.5 mV over the window; activation error <= 0.05 ms "
-                   "(strict: every cell; robust: 99.9th percentile); mismatches only in "
-                   "the last 0.1 ms")
-    out["gate_passed_strict"] = bool(out["vm_linf_max_mV"] <= 0.5 and
-                                     out["activation_linf_ms"] <= 0.05 and edge_ok)
-    out["gate_passed"] = bool(out["vm_linf_max_mV"] <= 0.5 and
-                              out["activation_p999_ms"] <= 0.05 and edge_ok)
+         "mixed_ms": round(float(a32[i]) * 1e3, 4), "fp64_ms": round(float(a64[i]) * 1e3, 4),
+         "control_ms": round(float(ac[i]) * 1e3, 4),
+         "control_band_ms": [round(float(lo[i]) * 1e3, 4), round(float(hi[i]) * 1e3, 4)],
+         "conditioned": bool(cond[i])}
+        for i in np.argsort(e32)[::-1][:16] if e32[i] > tol_t]
+    out["gate"] = ("Vm Linf <= 0.5 mV over the window; every cell's activation time within "
+                   "0.05 ms of the FP64 control where the control's crossing is conditioned "
+                   "(its -0.5 mV and +0.5 mV crossings within 0.05 ms), else inside the "
+                   "control's [-0.5 mV, +0.5 mV] crossing interval +-0.05 ms")
+    out["gate_passed_strict"] = bool(worst <= 0.5 and g32["passed"])
     return out
 
 def run_reference_semantics(cf, wname, w, sigma, cpu_steps, with_cpu, ord_steps=20):
@@ -901,6 +934,9 @@ def main():
                     help="only the 200 ms mixed-vs-FP64 comparison of the workload")
     ap.add_argument("--models-only", action="store_true",
                     help="only the other ionic models on the workload's mesh (profiling runs)")
+    ap.add_argument("--legs", default="",
+                    help="only these secondary legs, comma-separated: c0 (explicit "
+                         "BuenoOrovio), c1, c2 (batches), c4 (adjoint); profiling runs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 10 times (HBM counter scale)")
     args = ap.parse_args()
@@ -936,6 +972,16 @@ def main():
     if args.full_window_only:
         print(json.dumps({"workload": args.workload, "cells": n,
                           "full_window": run_full_window(cf, torch, w, sigma)}))
+        return
+    if args.legs:
+        fns = {"c0": lambda: run_explicit_c0(cf, torch), "c1": lambda: run_batch_c1(cf),
+               "c2": lambda: run_batch_c2(cf), "c4": lambda: run_adjoint_c4(cf, torch)}
+        legs = [x.strip() for x in args.legs.split(",") if x.strip()]
+        bad = [x for x in legs if x not in fns]
+        if bad:
+            print("unknown legs %s (choose from %s)" % (bad, sorted(fns)), file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps({x: fns[x]() for x in legs}))
         return
     if args.models_only:
         print(json.dumps({"workload": args.workload, "cells": n,
@@ -1017,14 +1063,18 @@ def main():
         except Exception as e:  # reported, never silently replaced
             cpu = {"error": repr(e)}
 
+    # the mixed path counts when its Vm stays within 0.5 mV of FP64 (the metric's bound),
+    # over the timed steps and, where it ran, over the whole 200 ms window
+    vm_ok = vm_err_mv <= 0.5 and (full is None or full["vm_linf_max_mV"] <= 0.5)
+    ok = ok and vm_ok
     line = {
         "metric": "cell-steps/s mixed vs FP64 at <=0.5 mV Vm Linf error; adjoint/forward cost",
-        "value": thr["mixed"] if ok else thr["fp64"],
+        "value": thr["mixed"] if vm_ok else thr["fp64"],
         "unit": "cell-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": (res["mixed"] if ok else res["fp64"])["elapsed"] / args.steps * 1e3,
+        "ms_per_step": (res["mixed"] if vm_ok else res["fp64"])["elapsed"] / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32+fp64" if ok else "f64",
+        "dtype": "fp32+fp64" if vm_ok else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "cells": n, "shape": w["shape"], "h_m": w["h"],
                    "model": w["model"], "algorithm": w["algorithm"], "dt_s": w["dt"],
@@ -1033,7 +1083,12 @@ def main():
         "mixed_cell_steps_per_s": thr["mixed"],
         "mixed_over_fp64": thr["mixed"] / thr["fp64"],
         "mixed_vm_linf_mV": vm_err_mv, "mixed_activation_linf_ms": act_err_ms,
-        "mixed_activation_set_mismatch": act_mismatch, "mixed_gate_passed": ok,
+        "mixed_activation_set_mismatch": act_mismatch,
+        # the timed window's gate, and the 200 ms window's strict gate against the control
+        "mixed_gate_passed_timed_window": ok,
+        "mixed_gate_passed": (full["gate_passed_strict"] if full is not None else ok),
+        "full_window_cell_steps_per_s": (full["cell_steps_per_s"] if full is not None
+                                         else None),
         "cg_iterations_per_step": {p: res[p]["iters"] / args.steps for p in res},
         "roofline": roof,
         "kernels": kernels_out,

@@ -59,6 +59,7 @@ def lib():
     L.cg_model_nstates.argtypes = [C.c_int]
     L.cg_model_state_name.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_size_t]
     L.cg_model_units.argtypes = [C.c_int, dp, dp, dp, dp]
+    L.cg_tissue_set_activation_threshold.argtypes = [vp, C.c_double]
     L.cg_ionic_create.argtypes = [C.c_int, C.c_int, ip64, C.c_int, C.c_int, C.POINTER(vp)]
     L.cg_ionic_destroy.argtypes = [vp]
     L.cg_ionic_nstates.argtypes = [vp]

@@ -16,6 +16,14 @@
 // The derivatives of the gates and currents are written out by hand from the model
 // equations (BuenoOrovio_2008.H:166-225); Heaviside switches contribute zero derivative.
 //
+// Implicit (backward-Euler) diffusion, monodomainSolver.C:121-136 with ddt Euler and
+// fvm::laplacian: M V' = V - dt I + a src, M = 1 - a L, a = dt/chiCm (M symmetric
+// positive definite).  Adjoint: lambda = M^-1 mu, then the cell VJP with lJ = -dt 85.7
+// lambda and dLoss/dV^n = lambda + (1000/85.7) lu; the kappa gradient is the same face sum
+// with V^{n+1} and lambda in place of V^n and mu.  Both solves are Jacobi PCG to a relative
+// residual of 1e-14 (tight, so that the forward map the gradient differentiates is the
+// solved one).
+//
 // Checkpoints: the forward pass stores (V, v, w, s) every K steps; the backward pass
 // re-runs each segment from its checkpoint into a (K+1)-slot store, then sweeps it in
 // reverse.
@@ -114,12 +122,17 @@ CG_DEV double bo_step(double u, double& v, double& w, double& s, double dt, cons
 // One step from slot `in` into slot `out` (planes SL_V..SL_s, [plane][n]); when aux is
 // not null (planes SL_Ev.. of the input slot) the step's factors are stored there; Vm at
 // probes into probe_row.
+// IMPL: the diffusion is implicit; instead of V^{n+1} the kernel writes the right-hand
+// side V - dt I + a src of M V^{n+1} into bvec and V^n (the solve's initial guess) into the
+// output V plane, and records no probes (the solve's result is V^{n+1}).
+template <bool IMPL>
 __global__ __launch_bounds__(ABS) void k_adj_forward(AdjGeom G, double t0,
                                                      const double* __restrict__ in,
                                                      double* __restrict__ out,
                                                      double* __restrict__ aux,
                                                      const int* __restrict__ probe_slot,
-                                                     double* __restrict__ probe_row)
+                                                     double* __restrict__ probe_row,
+                                                     double* __restrict__ bvec)
 {
     const HexGeom& g = G.g;
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
@@ -142,9 +155,14 @@ __global__ __launch_bounds__(ABS) void k_adj_forward(AdjGeom G, double t0,
         aux[(SL_ths - SL_Ev) * n + c] = f.ths;
         aux[(SL_thso - SL_Ev) * n + c] = f.thso;
     }
+    const double src = box_src(q.i, q.j, q.k, g, *G.boxes, t0);
+    if constexpr (IMPL) {
+        bvec[c] = vn + G.dt / G.chiCm * (src - G.chiCm * iion);
+        out[SL_V * n + c] = vn;
+        return;
+    }
     const HexCoefs hc{0, 0, 0, G.fx, G.fy, G.fz};
     const double lap = hex_laplacian<false>(in + SL_V * n, c, q, g, hc);
-    const double src = box_src(q.i, q.j, q.k, g, *G.boxes, t0);
     const double vnew = (1.0 * vn + G.dt / G.chiCm * (lap - G.chiCm * iion + src)) / 1.0;
     out[SL_V * n + c] = vnew;
     if (probe_row) {
@@ -243,8 +261,12 @@ template <int NV> CG_DEV void block_sum_a(double (&v)[NV])
 // seed of V^n from seed_row when not null) and ly = dLoss/dy^n, adds into gk and into the
 // per-block parameter partials.  Grid-stride over cells with AG blocks (block b always
 // accumulates partial b).
+// IMPL: mu is lambda = M^-1 dLoss/dV^{n+1}; the diffusion term is inside the solve, and
+// the kappa gradient takes V^{n+1} (the next slot's V plane, vnext).
+template <bool IMPL>
 __global__ __launch_bounds__(ABS) void k_adj_backward(AdjGeom G, double gxh, double gyh,
                                                       double gzh, const double* __restrict__ sl,
+                                                      const double* __restrict__ vnext,
                                                       const double* __restrict__ mu,
                                                       const double* __restrict__ ly1,
                                                       double* __restrict__ lV,
@@ -284,21 +306,26 @@ __global__ __launch_bounds__(ABS) void k_adj_backward(AdjGeom G, double gxh, dou
         acc[0] += r.dtfi;
         acc[1] += r.dtsi;
         // diffusion adjoint (L symmetric) and the kappa gradient, one pass over the faces
-        const double fxp = G.fx[c], fxm = G.fx[b.xm], fyp = G.fy[c], fym = G.fy[b.ym];
-        const double fzp = G.fz[c], fzm = G.fz[b.zm];
         const double mxp = mu[b.xp], mxm = mu[b.xm], myp = mu[b.yp], mym = mu[b.ym];
         const double mzp = mu[b.zp], mzm = mu[b.zm];
-        const double Lmu = fxp * (mxp - muc) + fxm * (mxm - muc) + fyp * (myp - muc) +
-                           fym * (mym - muc) + fzp * (mzp - muc) + fzm * (mzm - muc);
-        double lv = muc + a * Lmu + du_dV * r.lu;
+        double lv = muc + du_dV * r.lu;
+        if constexpr (!IMPL) {
+            const double fxp = G.fx[c], fxm = G.fx[b.xm], fyp = G.fy[c], fym = G.fy[b.ym];
+            const double fzp = G.fz[c], fzm = G.fz[b.zm];
+            const double Lmu = fxp * (mxp - muc) + fxm * (mxm - muc) + fyp * (myp - muc) +
+                               fym * (mym - muc) + fzp * (mzp - muc) + fzm * (mzm - muc);
+            lv += a * Lmu;
+        }
         if (seed_row) {
             const int slot = probe_slot[c];
             if (slot >= 0) lv += seed_row[slot];
         }
         lV[c] = lv;
-        const double dk = gxh * ((Vn[b.xp] - vc) * (muc - mxp) + (Vn[b.xm] - vc) * (muc - mxm)) +
-                          gyh * ((Vn[b.yp] - vc) * (muc - myp) + (Vn[b.ym] - vc) * (muc - mym)) +
-                          gzh * ((Vn[b.zp] - vc) * (muc - mzp) + (Vn[b.zm] - vc) * (muc - mzm));
+        const double* __restrict__ Vx = IMPL ? vnext : Vn;  // the field L acted on
+        const double xc = Vx[c];
+        const double dk = gxh * ((Vx[b.xp] - xc) * (muc - mxp) + (Vx[b.xm] - xc) * (muc - mxm)) +
+                          gyh * ((Vx[b.yp] - xc) * (muc - myp) + (Vx[b.ym] - xc) * (muc - mym)) +
+                          gzh * ((Vx[b.zp] - xc) * (muc - mzp) + (Vx[b.zm] - xc) * (muc - mzm));
         gk[c] += a * dk;
     }
     block_sum_a<2>(acc);
@@ -306,6 +333,117 @@ __global__ __launch_bounds__(ABS) void k_adj_backward(AdjGeom G, double gxh, dou
         gtheta[blockIdx.x * 2 + 0] += acc[0];
         gtheta[blockIdx.x * 2 + 1] += acc[1];
     }
+}
+
+// ---------------------------------------------------------- implicit solves
+// Jacobi PCG on M = 1 - a L with the adjoint's face coefficients (kappa-scaled, wall faces
+// 0, wall neighbours clamped to the cell as in hex_laplacian).  p and q are updated as
+// p = z + beta p, q = M z + beta q (q = M p without a second stencil pass), so an
+// iteration is two kernels; the host sums the per-block partials in a fixed order.
+CG_DEV double m_apply(const AdjGeom& G, double a, const double* __restrict__ x, int64_t c,
+                      const IJK& q)
+{
+    const Nbr b = nbr_of(c, q, G.g);
+    const double xc = x[c];
+    const double L = G.fx[c] * (x[b.xp] - xc) + G.fx[b.xm] * (x[b.xm] - xc) +
+                     G.fy[c] * (x[b.yp] - xc) + G.fy[b.ym] * (x[b.ym] - xc) +
+                     G.fz[c] * (x[b.zp] - xc) + G.fz[b.zm] * (x[b.zm] - xc);
+    return xc - a * L;
+}
+
+// 1/diag of M
+__global__ __launch_bounds__(ABS) void k_adj_dinv(AdjGeom G, double a, double* __restrict__ dinv)
+{
+    const HexGeom& g = G.g;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t c = (int64_t)blockIdx.x * ABS + threadIdx.x;
+    if (c >= n) return;
+    const IJK q = ijk_of((uint32_t)c, g);
+    const Nbr b = nbr_of(c, q, g);
+    const double s = G.fx[c] + (q.i > 0 ? G.fx[b.xm] : 0.0) + G.fy[c] +
+                     (q.j > 0 ? G.fy[b.ym] : 0.0) + G.fz[c] + (q.k > 0 ? G.fz[b.zm] : 0.0);
+    dinv[c] = 1.0 / (1.0 + a * s);
+}
+
+// r = b - M x, z = D^-1 r; partials (r, z), (r, r), (b, b)
+__global__ __launch_bounds__(ABS) void k_adj_cg0(AdjGeom G, double a, const double* __restrict__ bv,
+                                                 const double* __restrict__ x,
+                                                 const double* __restrict__ dinv,
+                                                 double* __restrict__ r, double* __restrict__ z,
+                                                 double* __restrict__ part)
+{
+    const HexGeom& g = G.g;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t c = (int64_t)blockIdx.x * ABS + threadIdx.x; c < n; c += (int64_t)gridDim.x * ABS) {
+        const IJK q = ijk_of((uint32_t)c, g);
+        const double rc = bv[c] - m_apply(G, a, x, c, q);
+        const double zc = dinv[c] * rc;
+        r[c] = rc;
+        z[c] = zc;
+        acc[0] += rc * zc;
+        acc[1] += rc * rc;
+        acc[2] += bv[c] * bv[c];
+    }
+    block_sum_a<3>(acc);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) part[k * AG + blockIdx.x] = acc[k];
+}
+
+// p = z + beta p, q = M z + beta q (first: p = z, q = M z); partial (p, q)
+__global__ __launch_bounds__(ABS) void k_adj_cg1(AdjGeom G, double a, double beta, int first,
+                                                 const double* __restrict__ z,
+                                                 double* __restrict__ p, double* __restrict__ qv,
+                                                 double* __restrict__ part)
+{
+    const HexGeom& g = G.g;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    double acc[1] = {0.0};
+    for (int64_t c = (int64_t)blockIdx.x * ABS + threadIdx.x; c < n; c += (int64_t)gridDim.x * ABS) {
+        const IJK q = ijk_of((uint32_t)c, g);
+        const double mz = m_apply(G, a, z, c, q);
+        const double pc = first ? z[c] : z[c] + beta * p[c];
+        const double qc = first ? mz : mz + beta * qv[c];
+        p[c] = pc;
+        qv[c] = qc;
+        acc[0] += pc * qc;
+    }
+    block_sum_a<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// x += alpha p, r -= alpha q, z = D^-1 r; partials (r, z), (r, r)
+__global__ __launch_bounds__(ABS) void k_adj_cg2(int64_t n, double alpha,
+                                                 const double* __restrict__ p,
+                                                 const double* __restrict__ qv,
+                                                 const double* __restrict__ dinv,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 double* __restrict__ z, double* __restrict__ part)
+{
+    double acc[2] = {0.0, 0.0};
+    for (int64_t c = (int64_t)blockIdx.x * ABS + threadIdx.x; c < n; c += (int64_t)gridDim.x * ABS) {
+        x[c] += alpha * p[c];
+        const double rc = r[c] - alpha * qv[c];
+        const double zc = dinv[c] * rc;
+        r[c] = rc;
+        z[c] = zc;
+        acc[0] += rc * zc;
+        acc[1] += rc * rc;
+    }
+    block_sum_a<2>(acc);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 2; ++k) part[k * AG + blockIdx.x] = acc[k];
+}
+
+// Vm at the probes from a solved V plane
+__global__ __launch_bounds__(ABS) void k_adj_probe(int64_t n, const double* __restrict__ v,
+                                                   const int* __restrict__ probe_slot,
+                                                   double* __restrict__ probe_row)
+{
+    const int64_t c = (int64_t)blockIdx.x * ABS + threadIdx.x;
+    if (c >= n) return;
+    const int slot = probe_slot[c];
+    if (slot >= 0) probe_row[slot] = v[c];
 }
 
 // lV = seed of V^N (the last probe row) at probes, 0 elsewhere; ly = 0
@@ -356,6 +494,10 @@ struct Adjoint {
     bool store_all = false;     // last forward kept every step's slot (no recompute)
     double *lV[2] = {nullptr, nullptr}, *ly[2] = {nullptr, nullptr};
     double* gtheta = nullptr;   // [AG][2]
+    bool implicit = false;      // backward-Euler diffusion (solves in both passes)
+    double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_q = nullptr;
+    double *dinv = nullptr, *bvec = nullptr, *lam = nullptr, *part = nullptr;  // part [3][AG]
+    std::vector<double> h_part;
     int last_steps = 0;
     std::vector<double> t0;     // start time of every step of the last forward
     hipStream_t stream;
@@ -364,9 +506,10 @@ struct Adjoint {
 void adjoint_destroy(Adjoint* A)
 {
     if (!A) return;
-    void* p[] = {A->d_prm, A->d_boxes, A->d_slot, A->d_V0, A->d_y0,  A->fx,    A->fy,
+    void* p[] = {A->d_prm, A->d_boxes, A->d_slot, A->d_V0,  A->d_y0,  A->fx,   A->fy,
                  A->fz,    A->ck,      A->seg,    A->lV[0], A->lV[1], A->ly[0], A->ly[1],
-                 A->gtheta,  A->d_mask};
+                 A->gtheta, A->d_mask, A->cg_r,   A->cg_z,  A->cg_p,  A->cg_q,  A->dinv,
+                 A->bvec,  A->lam,     A->part};
     for (void* q : p)
         if (q) hipFree(q);
     delete A;
@@ -375,9 +518,10 @@ void adjoint_destroy(Adjoint* A)
 int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, double dt,
                    double chiCm, const ModelParams& P, const double* V0, const void* y0_states,
                    const ActiveBoxesAll& boxes, int K, int n_probes, const int64_t* probes,
-                   hipStream_t stream, std::string* err)
+                   bool implicit, hipStream_t stream, std::string* err)
 {
     Adjoint* A = new Adjoint();
+    A->implicit = implicit;
     A->g = g;
     A->n = (int64_t)g.nx * g.ny * g.nz;
     A->gx = sigma0_diag[0] / (g.hx * g.hx);
@@ -416,6 +560,12 @@ int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, d
     M((void**)&A->fx, 8 * n); M((void**)&A->fy, 8 * n); M((void**)&A->fz, 8 * n);
     for (int i = 0; i < 2; ++i) { M((void**)&A->lV[i], 8 * n); M((void**)&A->ly[i], 8 * 3 * n); }
     M((void**)&A->gtheta, 8 * 2 * AG);
+    if (implicit) {
+        M((void**)&A->cg_r, 8 * n); M((void**)&A->cg_z, 8 * n); M((void**)&A->cg_p, 8 * n);
+        M((void**)&A->cg_q, 8 * n); M((void**)&A->dinv, 8 * n); M((void**)&A->bvec, 8 * n);
+        M((void**)&A->lam, 8 * n); M((void**)&A->part, 8 * 3 * AG);
+        A->h_part.resize(3 * AG);
+    }
     if (!ok) {
         adjoint_destroy(A);
         *err = "adjoint allocation failed";
@@ -455,6 +605,85 @@ static AdjGeom geom_of(const Adjoint* A)
     return G;
 }
 
+// the first k partial sets [k][AG] summed in a fixed order (host)
+static bool read_partials(Adjoint* A, int k, double* out)
+{
+    if (hipMemcpyAsync(A->h_part.data(), A->part, 8 * (size_t)k * AG, hipMemcpyDeviceToHost,
+                       A->stream) != hipSuccess ||
+        hipStreamSynchronize(A->stream) != hipSuccess)
+        return false;
+    for (int j = 0; j < k; ++j) {
+        double t = 0.0;
+        for (int b = 0; b < AG; ++b) t += A->h_part[(size_t)j * AG + b];
+        out[j] = t;
+    }
+    return true;
+}
+
+// x = M^-1 b by Jacobi PCG from the x given, to ||r|| <= 1e-14 ||b||
+static int adj_solve(Adjoint* A, const AdjGeom& G, const double* b, double* x,
+                     std::string* err)
+{
+    constexpr double TOL = 1e-14;
+    constexpr int MAX_IT = 1000;
+    const double a = A->dt / A->chiCm;
+    double s3[3];
+    hipLaunchKernelGGL(k_adj_cg0, dim3(AG), dim3(ABS), 0, A->stream, G, a, b,
+                       (const double*)x, (const double*)A->dinv, A->cg_r, A->cg_z, A->part);
+    if (!read_partials(A, 3, s3)) {
+        *err = "adjoint solve: partial read-back failed";
+        return 4;
+    }
+    double rz = s3[0], rr = s3[1], beta = 0.0;
+    const double lim = TOL * TOL * s3[2];
+    for (int it = 0; rr > lim; ++it) {
+        if (it >= MAX_IT) {
+            *err = "adjoint solve did not converge";
+            return 4;
+        }
+        double pq, s2[2];
+        hipLaunchKernelGGL(k_adj_cg1, dim3(AG), dim3(ABS), 0, A->stream, G, a, beta,
+                           it == 0 ? 1 : 0, (const double*)A->cg_z, A->cg_p, A->cg_q, A->part);
+        if (!read_partials(A, 1, &pq)) {
+            *err = "adjoint solve: partial read-back failed";
+            return 4;
+        }
+        const double alpha = rz / pq;
+        hipLaunchKernelGGL(k_adj_cg2, dim3(AG), dim3(ABS), 0, A->stream, A->n, alpha,
+                           (const double*)A->cg_p, (const double*)A->cg_q,
+                           (const double*)A->dinv, x, A->cg_r, A->cg_z, A->part);
+        if (!read_partials(A, 2, s2)) {
+            *err = "adjoint solve: partial read-back failed";
+            return 4;
+        }
+        beta = s2[0] / rz;
+        rz = s2[0];
+        rr = s2[1];
+    }
+    return 0;
+}
+
+// one forward step from slot `in` into slot `outp` (factors into aux, probes into
+// probe_row when not null)
+static int adj_step(Adjoint* A, const AdjGeom& G, double t0, const double* in, double* outp,
+                    double* aux, double* probe_row, std::string* err)
+{
+    const unsigned nb = blocks_of(A->n);
+    if (!A->implicit) {
+        hipLaunchKernelGGL(k_adj_forward<false>, dim3(nb), dim3(ABS), 0, A->stream, G, t0, in,
+                           outp, aux, (const int*)A->d_slot, probe_row, (double*)nullptr);
+        return 0;
+    }
+    hipLaunchKernelGGL(k_adj_forward<true>, dim3(nb), dim3(ABS), 0, A->stream, G, t0, in,
+                       outp, aux, (const int*)A->d_slot, (double*)nullptr, A->bvec);
+    if (int rc = adj_solve(A, G, A->bvec, outp + SL_V * A->n, err)) return rc;
+    if (probe_row)
+        hipLaunchKernelGGL(k_adj_probe, dim3(nb), dim3(ABS), 0, A->stream, A->n,
+                           (const double*)(outp + SL_V * A->n), (const int*)A->d_slot,
+                           probe_row);
+    return 0;
+}
+
 static bool ensure(double** buf, int* have, int want, size_t per)
 {
     if (want <= *have) return true;
@@ -490,6 +719,9 @@ int adjoint_forward(Adjoint* A, const double* kappa, const double* theta, const 
                    A->stream);
     hipLaunchKernelGGL(k_adj_faces, dim3(nb), dim3(ABS), 0, A->stream, A->g, kappa, A->gx,
                        A->gy, A->gz, A->fx, A->fy, A->fz);
+    if (A->implicit)
+        hipLaunchKernelGGL(k_adj_dinv, dim3(nb), dim3(ABS), 0, A->stream, geom_of(A),
+                           A->dt / A->chiCm, A->dinv);
     hipMemcpyAsync(A->seg + SL_V * n, A->d_V0, 8 * n, hipMemcpyDeviceToDevice, A->stream);
     hipMemcpyAsync(A->seg + SL_v * n, A->d_y0, 8 * 3 * n, hipMemcpyDeviceToDevice, A->stream);
     const AdjGeom G = geom_of(A);
@@ -508,9 +740,9 @@ int adjoint_forward(Adjoint* A, const double* kappa, const double* theta, const 
                 hipMemcpyAsync(A->ck + (size_t)(st / A->K) * 4 * n, in, 8 * 4 * n,
                                hipMemcpyDeviceToDevice, A->stream);
         }
-        hipLaunchKernelGGL(k_adj_forward, dim3(nb), dim3(ABS), 0, A->stream, G, t0[st],
-                           (const double*)in, outp, aux, (const int*)A->d_slot,
-                           probes + (size_t)st * A->n_probes);
+        if (int rc = adj_step(A, G, t0[st], in, outp, aux, probes + (size_t)st * A->n_probes,
+                              err))
+            return rc;
     }
     A->last_steps = n_steps;
     A->t0 = t0;
@@ -555,19 +787,30 @@ int adjoint_backward(Adjoint* A, const double* dprobes, double* dkappa, double* 
                            hipMemcpyDeviceToDevice, A->stream);
             for (int st = s0; st < s1; ++st) {
                 double* in = A->seg + (size_t)(st - s0) * SN;
-                hipLaunchKernelGGL(k_adj_forward, dim3(nb), dim3(ABS), 0, A->stream, G,
-                                   A->t0[st], (const double*)in, in + SN, in + SL_Ev * n,
-                                   (const int*)A->d_slot, (double*)nullptr);
+                if (int rc = adj_step(A, G, A->t0[st], in, in + SN, in + SL_Ev * n, nullptr,
+                                      err))
+                    return rc;
             }
         }
         for (int st = s1 - 1; st >= s0; --st) {
             const double* sl = base + (size_t)(st - s0) * SN;
             const int nxt = cur ^ 1;
             const double* seed = st >= 1 ? dprobes + (size_t)(st - 1) * A->n_probes : nullptr;
-            hipLaunchKernelGGL(k_adj_backward, dim3(AG), dim3(ABS), 0, A->stream, G, gxh, gyh,
-                               gzh, sl, (const double*)A->lV[cur], (const double*)A->ly[cur],
-                               A->lV[nxt], A->ly[nxt], (const int*)A->d_slot, seed, dkappa,
-                               A->gtheta);
+            if (A->implicit) {
+                // lambda = M^-1 dLoss/dV^{n+1}, from lambda = 0
+                hipLaunchKernelGGL(k_fill, dim3(nb), dim3(ABS), 0, A->stream, n, A->lam, 0.0);
+                if (int rc = adj_solve(A, G, A->lV[cur], A->lam, err)) return rc;
+                hipLaunchKernelGGL(k_adj_backward<true>, dim3(AG), dim3(ABS), 0, A->stream, G,
+                                   gxh, gyh, gzh, sl, sl + SN, (const double*)A->lam,
+                                   (const double*)A->ly[cur], A->lV[nxt], A->ly[nxt],
+                                   (const int*)A->d_slot, seed, dkappa, A->gtheta);
+            } else {
+                hipLaunchKernelGGL(k_adj_backward<false>, dim3(AG), dim3(ABS), 0, A->stream, G,
+                                   gxh, gyh, gzh, sl, (const double*)nullptr,
+                                   (const double*)A->lV[cur], (const double*)A->ly[cur],
+                                   A->lV[nxt], A->ly[nxt], (const int*)A->d_slot, seed, dkappa,
+                                   A->gtheta);
+            }
             cur = nxt;
         }
     }

@@ -251,6 +251,7 @@ struct cg_tissue {
     uint64_t boxes_ver = 1, mask_boxes_ver = 0, mask_centres_ver = 0;
     double t = 0.0;
     int64_t steps = 0;
+    double act_v = 0.0;  // activation threshold (V)
     cg::ImplicitSolver* implicit = nullptr;
     cg::Profiler prof;
     // pseudo-ECG buffers: electrode positions, per-block partial sums, results
@@ -1266,7 +1267,8 @@ int cg_tissue_create(cg_mesh* m, cg_ionic* ion, const cg_tissue_config* cfg, cg_
 int cg_tissue_destroy(cg_tissue* t)
 {
     if (!t) return CG_OK;
-    if (t->ion && t->ion->stream) hipStreamSynchronize(t->ion->stream);
+    // the ionic handle may already be gone: a destroy touches its own handle only
+    hipDeviceSynchronize();
     for (int q = 0; q < 3; ++q) hipFree(t->d_V[q]);
     hipFree(t->d_Iion); hipFree(t->d_flag); hipFree(t->d_tact);
     if (t->d_box_mask) hipFree(t->d_box_mask);
@@ -1367,7 +1369,7 @@ static int general_advance(cg_tissue* t, cg::GeneralMesh* G, int n_steps, bool i
         const double t0 = tn - dt;
         const bool bdf2 = t->cfg.ddt == CG_DDT_BACKWARD && t->steps >= 1;
         cg::GeneralStep st{};
-        st.dt = dt; st.chiCm = chiCm; st.t_old = t0;
+        st.dt = dt; st.chiCm = chiCm; st.t_old = t0; st.act_v = t->act_v;
         st.c = bdf2 ? 1.5 : 1.0; st.c0 = bdf2 ? 2.0 : 1.0; st.c00 = bdf2 ? 0.5 : 0.0;
         st.boxes = active_boxes(t, t0);
         st.src = src_dev;
@@ -1429,7 +1431,7 @@ static int general_advance(cg_tissue* t, cg::GeneralMesh* G, int n_steps, bool i
                 if (last) break;
             }
             t->g_solves = solves;
-            HIPCHK(cg::general_activation(G, Vn, Vnew, t->d_flag, t->d_tact, t0, dt, s));
+            HIPCHK(cg::general_activation(G, Vn, Vnew, t->d_flag, t->d_tact, t0, dt, t->act_v, s));
         }
         t->cur = (t->cur + 1) % 3;
         t->t = tn;
@@ -1495,6 +1497,7 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
         st.dt = dt; st.dt_model = dtm; st.chiCm = chiCm;
         st.c = bdf2 ? 1.5 : 1.0; st.c0 = bdf2 ? 2.0 : 1.0; st.c00 = bdf2 ? 0.5 : 0.0;
         st.t_old = t0;
+        st.act_v = t->act_v;
         st.boxes = active_boxes(t, t0);
         st.od = m->d_od;
         st.od_mask = m->od_mask;
@@ -1532,6 +1535,14 @@ int cg_tissue_step(cg_tissue* t, int n_steps)
         t->t = tn;
         t->steps++;
     }
+    return CG_OK;
+}
+
+int cg_tissue_set_activation_threshold(cg_tissue* t, double v_threshold)
+{
+    if (!t) return fail(CG_ERR_ARG, "NULL handle");
+    if (!(std::fabs(v_threshold) < 1.0)) return fail(CG_ERR_ARG, "threshold out of range");
+    t->act_v = v_threshold;
     return CG_OK;
 }
 
@@ -1871,8 +1882,8 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
     cg_ionic* ion = t->ion;
     cg_mesh* m = t->mesh;
     if (m->gm) return fail(CG_ERR_STATE, "the adjoint covers the hex block");
-    if (t->cfg.algorithm != CG_EXPLICIT || t->cfg.ddt != CG_DDT_EULER)
-        return fail(CG_ERR_STATE, "the adjoint covers the explicit Euler tissue step");
+    if (t->cfg.ddt != CG_DDT_EULER)
+        return fail(CG_ERR_STATE, "the adjoint covers the Euler-ddt tissue step");
     if (ion->model != cg::MODEL_BO || ion->prec != CG_FP64)
         return fail(CG_ERR_STATE, "the adjoint covers BuenoOrovio in FP64");
     if (t->ion->ode_kind != CG_ODE_RUSH_LARSEN)
@@ -1898,7 +1909,8 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
     cg::Adjoint* A = nullptr;
     const int rc = cg::adjoint_create(&A, m->g, diag, t->cfg.dt, t->cfg.chi * t->cfg.cm, ion->P,
                                       t->d_V[t->cur], ion->d_states, bx, checkpoint_every,
-                                      n_probes, probe_cells, ion->stream, &err);
+                                      n_probes, probe_cells, t->cfg.algorithm == CG_IMPLICIT,
+                                      ion->stream, &err);
     if (rc) return fail(rc, "%s", err.c_str());
     cg_adjoint* a = new cg_adjoint{t, A, t->t, n_probes};
     *out = a;
@@ -1908,7 +1920,7 @@ int cg_adjoint_create(cg_tissue* t, int n_probes, const int64_t* probe_cells,
 int cg_adjoint_destroy(cg_adjoint* a)
 {
     if (!a) return CG_OK;
-    hipStreamSynchronize(a->t->ion->stream);
+    hipDeviceSynchronize();   // the tissue and ionic handles may already be gone
     cg::adjoint_destroy(a->A);
     delete a;
     return CG_OK;

@@ -16,11 +16,12 @@ struct ActiveBoxesAll {
 };
 
 // Snapshot of the initial state (V0 [n], ionic states [4][n] of BuenoOrovio, FP64) and the
-// kappa = 1 conductivity diagonal; probes: cell ids.  Returns 0 or an error code.
+// kappa = 1 conductivity diagonal; probes: cell ids; implicit: backward-Euler diffusion
+// (Euler ddt) instead of the explicit update.  Returns 0 or an error code.
 int adjoint_create(Adjoint** out, const HexGeom& g, const double* sigma0_diag, double dt,
                    double chiCm, const ModelParams& P, const double* V0, const void* y0_states,
                    const ActiveBoxesAll& boxes, int checkpoint_every, int n_probes,
-                   const int64_t* probes, hipStream_t s, std::string* err);
+                   const int64_t* probes, bool implicit, hipStream_t s, std::string* err);
 void adjoint_destroy(Adjoint* A);
 // n_steps from the initial state with conductivity kappa * sigma0 (device [n]) and the
 // constants theta[k] at indices theta_index[k]; Vm after each step at the probes into

@@ -32,6 +32,7 @@ int64_t general_faces(const GeneralMesh* G);
 // constants of one step (ddt coefficients c V^{n+1} = c0 V^n - c00 V^{n-1} + ...)
 struct GeneralStep {
     double dt, chiCm, c, c0, c00, t_old;
+    double act_v;       // activation threshold (V), 0 the reference's
     ActiveBoxes boxes;
     const double* src;  // source field [n] (device, A/m^3) in place of the boxes, or null
 };
@@ -63,7 +64,7 @@ hipError_t general_guess(GeneralMesh* G, const double* Vn, const double* Vm1, do
 // activation update after an implicit solve (myocardiumDomain.C:345-374)
 hipError_t general_activation(GeneralMesh* G, const double* Vn, const double* Vnew,
                               uint8_t* flag, double* tact, double t_old, double dt,
-                              hipStream_t s);
+                              double vthr, hipStream_t s);
 // pseudo-ECG (pseudoECGSolver.C:36-83) of V at n_el electrodes pos [E][3] (device): the
 // conductivity per cell sig ([6][n] when sym, else [9][n]; device) or the uniform s9 (host);
 // partial [E][nblocks] scratch, out [E] (device).  Asynchronous on s.

@@ -73,6 +73,9 @@ struct ExplicitStep {
     uint8_t* act_flag;
     double* act_tact;
     double act_t_old, act_dt;
+    // activation threshold (V): the first crossing of V > act_v + 1e-15, interpolated
+    // linearly to act_v; 0 is the reference's (myocardiumDomain.C:362-371)
+    double act_v = 0.0;
 };
 
 // Model kernels, one set per arithmetic type: _fp64 (IEEE double, model_fp64.hip) and

@@ -170,9 +170,9 @@ __global__ __launch_bounds__(GBS) void k_gexplicit(GView g, const double* __rest
     if (st.c00 != 0.0) rhs -= st.c00 * Vold[c];
     const double vnew = rhs / st.c;
     Vnew[c] = vnew;
-    if (flag[c] && vnew > 1e-15) {
+    if (flag[c] && vnew > st.act_v + 1e-15) {
         flag[c] = 0;
-        tact[c] = st.t_old + (0.0 - vn) / (vnew - vn) * st.dt;
+        tact[c] = st.t_old + (st.act_v - vn) / (vnew - vn) * st.dt;
     }
 }
 
@@ -220,15 +220,15 @@ __global__ __launch_bounds__(GBS) void k_gactivation(int64_t n, const double* __
                                                      const double* __restrict__ Vnew,
                                                      uint8_t* __restrict__ flag,
                                                      double* __restrict__ tact, double t_old,
-                                                     double dt)
+                                                     double dt, double vthr)
 {
     const int64_t c = (int64_t)blockIdx.x * GBS + threadIdx.x;
     if (c >= n) return;
     const double vnew = Vnew[c];
-    if (flag[c] && vnew > 1e-15) {
+    if (flag[c] && vnew > vthr + 1e-15) {
         const double vn = Vn[c];
         flag[c] = 0;
-        tact[c] = t_old + (0.0 - vn) / (vnew - vn) * dt;
+        tact[c] = t_old + (vthr - vn) / (vnew - vn) * dt;
     }
 }
 
@@ -669,10 +669,10 @@ hipError_t general_guess(GeneralMesh* G, const double* Vn, const double* Vm1, do
 
 hipError_t general_activation(GeneralMesh* G, const double* Vn, const double* Vnew,
                               uint8_t* flag, double* tact, double t_old, double dt,
-                              hipStream_t s)
+                              double vthr, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gactivation, dim3(grid_of(G->n)), dim3(GBS), 0, s, G->n, Vn, Vnew,
-                       flag, tact, t_old, dt);
+                       flag, tact, t_old, dt, vthr);
     return hipGetLastError();
 }
 

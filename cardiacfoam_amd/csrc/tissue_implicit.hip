@@ -579,18 +579,18 @@ __global__ __launch_bounds__(BS) void k_activation(HexGeom g, const double* __re
                                                    double* Vnew,
                                                    uint8_t* __restrict__ flag,
                                                    double* __restrict__ tact, double t_old,
-                                                   double dt, const long long* gate,
-                                                   long long gate_val)
+                                                   double dt, double vthr,
+                                                   const long long* gate, long long gate_val)
 {
     if (gate && *gate != gate_val) return;
     CELL_INDEX
     (void)cc;
     const double vnew = x[c];
     if (x != Vnew) Vnew[c] = vnew;
-    if (flag[c] && vnew > 1e-15) {
+    if (flag[c] && vnew > vthr + 1e-15) {
         const double vn = Vn[c];
         flag[c] = 0;
-        tact[c] = t_old + (0.0 - vn) / (vnew - vn) * dt;
+        tact[c] = t_old + (vthr - vn) / (vnew - vn) * dt;
     }
 }
 
@@ -1003,7 +1003,7 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
         if (last && !act_in_next) {
             int t = prof ? prof->begin(K_ACTIVATION) : -1;
             hipLaunchKernelGGL(k_activation, dim3(grid), dim3(BS), 0, S->stream, S->g, Vn, x,
-                               Vnew, flag, tact, st.t_old, st.dt, conv_gen, g);
+                               Vnew, flag, tact, st.t_old, st.dt, st.act_v, conv_gen, g);
             if (prof) prof->end(t);
         }
         hipEventRecord(S->read_done, S->stream);
@@ -1017,6 +1017,7 @@ int implicit_step(ImplicitSolver* S, int model, int prec, int tissue, void* stat
                 sn.act_tact = tact;
                 sn.act_t_old = st.t_old;
                 sn.act_dt = st.dt;
+                sn.act_v = st.act_v;
             }
             int t = prof ? prof->begin(K_IONIC) : -1;
             launch_ionic_rhs(model, prec, tissue, S->g, states, Vnew, Vn,

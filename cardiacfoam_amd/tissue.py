@@ -553,6 +553,11 @@ class Monodomain:
             got.append(name)
         return got
 
+    def set_activation_threshold(self, v):
+        """Activation threshold in V for the steps still to run (0, the default, is the
+        reference's, myocardiumDomain.C:362-371)."""
+        check(lib().cg_tissue_set_activation_threshold(self.t, float(v)))
+
     def solver_report(self):
         """last step: (linear solves, initial residual of the last corrector, final residual)"""
         n = C.c_int()

@@ -12,7 +12,9 @@
  *
  * Every entry returns CG_OK (0) or an error code; cg_last_error() gives a thread-local
  * message the adapter turns into FatalErrorInFunction.  Each handle owns its device
- * memory and one HIP stream; calls on one handle must not overlap.
+ * memory and one HIP stream; calls on one handle must not overlap.  Handles may be
+ * destroyed in any order (a destroy touches its own handle only, after a device-wide
+ * synchronize); every other call needs the handles it was created from still alive.
  */
 #ifndef CARDIACGPU_H
 #define CARDIACGPU_H
@@ -306,6 +308,14 @@ int cg_tissue_set_stimulus_boxes(cg_tissue* t, int n, const double* box_min,
 int cg_tissue_step(cg_tissue* t, int n_steps);
 int cg_tissue_sync(cg_tissue* t);
 
+/* Activation threshold in V: a cell's activation time is the first step whose new Vm
+ * exceeds v_threshold (+1e-15), interpolated linearly to v_threshold.  The default, 0, is
+ * the reference's (myocardiumDomain.C:362-371).  No reference counterpart otherwise: other
+ * thresholds measure how well each cell's activation time is conditioned (a crossing
+ * that grazes 0 V moves far under a small Vm perturbation).  Applies to steps not yet
+ * run; |v_threshold| < 1. */
+int cg_tissue_set_activation_threshold(cg_tissue* t, double v_threshold);
+
 enum { CG_FIELD_VM = 0, CG_FIELD_IION = 1, CG_FIELD_ACTIVATION = 2, CG_FIELD_VM_OLD = 3 };
 /* copies a field [n] to host doubles (synchronous) */
 int cg_tissue_get_field(cg_tissue* t, int which, double* host);
@@ -382,12 +392,14 @@ int cg_ldu_amul_timed(cg_ldu* L, const double* x, double* y, float* kernel_ms);
 int cg_mesh_laplacian(cg_mesh* m, const double* V_dev, double* out_dev);
 
 /* ------------------------------------------------------------------ adjoint
- * Reverse-mode gradients of Vm at probe cells over n steps of an explicit, Euler-ddt,
- * BuenoOrovio FP64 tissue with respect to a per-cell scale kappa of the (uniform,
- * diagonal) conductivity and the model constants tau_fi, tau_si.  No reference
+ * Reverse-mode gradients of Vm at probe cells over n steps of an Euler-ddt, BuenoOrovio
+ * FP64 tissue, explicit or implicit, with respect to a per-cell scale kappa of the
+ * (uniform, diagonal) conductivity and the model constants tau_fi, tau_si.  No reference
  * counterpart (cardiacFoam has no derivatives); the forward map is the reference's step
- * (myocardiumDomain.C:487-529, monodomainSolver.C:103-118, BuenoOrovio.C:107-152) with
- * sigma_c = kappa_c sigma. */
+ * (myocardiumDomain.C:487-529, monodomainSolver.C:103-136, BuenoOrovio.C:107-152) with
+ * sigma_c = kappa_c sigma.  Implicit tissues solve M V^{n+1} = V^n - dt Iion + dt/(chi Cm)
+ * src, M = 1 - dt/(chi Cm) L, by Jacobi PCG to a relative residual of 1e-14 in both
+ * passes (the adjoint solves M lambda = dLoss/dV^{n+1}). */
 typedef struct cg_adjoint cg_adjoint;
 /* Snapshots the tissue's current Vm, ionic states, stimulus boxes and time as the start of
  * every forward run.  The forward run checkpoints the state every checkpoint_every steps

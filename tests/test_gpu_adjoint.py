@@ -1,7 +1,7 @@
-"""Adjoint of the explicit monodomain step (BuenoOrovio, Euler ddt, FP64): the adjoint's
-forward run equals cg_tissue_step, and its gradients with respect to a per-cell
-conductivity scale and tau_fi / tau_si equal central finite differences of the loss
-computed by the CPU oracle's independent tissue run."""
+"""Adjoint of the monodomain step (BuenoOrovio, Euler ddt, FP64), explicit and implicit:
+the adjoint's forward run equals cg_tissue_step, and its gradients with respect to a
+per-cell conductivity scale and tau_fi / tau_si equal central finite differences of the
+loss computed by the CPU oracle's independent tissue run."""
 import sys
 
 import numpy as np
@@ -20,13 +20,13 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def _dom(gpu, shape, h):
+def _dom(gpu, shape, h, algorithm="explicit", tol=1e-8):
     mesh = gpu.HexMesh(*shape, h)
     coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": 1.4e5,
-              "cm": 0.01, "solutionAlgorithm": "explicit",
+              "cm": 0.01, "solutionAlgorithm": algorithm,
               "conductivity": [SIG0[0], 0, 0, 0, SIG0[1], 0, 0, 0, SIG0[2]],
               "externalStimulus": BOX}
-    return mesh, gpu.Monodomain(mesh, coeffs, dt=DT)
+    return mesh, gpu.Monodomain(mesh, coeffs, dt=DT, tol=tol)
 
 
 def test_adjoint_forward_equals_tissue_step(gpu, orc):
@@ -48,12 +48,12 @@ def test_adjoint_forward_equals_tissue_step(gpu, orc):
     assert err < 1e-12
 
 
-def _oracle_loss(orc, shape, h, kappa, theta, steps, cells, target):
+def _oracle_loss(orc, shape, h, kappa, theta, steps, cells, target, implicit=0):
     nx, ny, nz = shape
     sig = np.zeros((kappa.size, 9))
     sig[:, 0], sig[:, 4], sig[:, 8] = kappa * SIG0[0], kappa * SIG0[1], kappa * SIG0[2]
     T = orc.OracleTissue(nx, ny, nz, h, sig, orc.BO, orc.EPI, DT, 1.4e5, 0.01, integrator=0,
-                         ddt=0, implicit=0, nthreads=8)
+                         ddt=0, implicit=implicit, pcg_tol=1e-14, nthreads=8)
     T.set_const(orc.const_index(orc.BO, "tauFi"), theta[0])
     T.set_const(orc.const_index(orc.BO, "tauSi"), theta[1])
     bx = orc.make_boxes(BOX["stimulusLocationMin"], BOX["stimulusLocationMax"], [0.0], [2e-3],
@@ -157,3 +157,93 @@ def test_adjoint_backward_of_a_replaced_forward_raises(gpu, orc):
         assert rc == 9, (gen, n, rc)          # CG_ERR_STATE
     (t2 ** 2).sum().backward()
     assert torch.isfinite(k2.grad).all()
+
+
+def test_implicit_adjoint_forward_equals_tissue_step(gpu, orc):
+    """Implicit (backward-Euler) tissue: the adjoint's forward run (PCG to 1e-14) equals
+    cg_tissue_step (PCG to 1e-13).  Each step's two solves stop at different points within
+    their tolerances, so the traces agree to about steps x 1e-13 of Vm's range, not to
+    rounding as in the explicit case."""
+    import torch
+    _say("implicit adjoint forward vs tissue step")
+    shape, h, steps = (20, 12, 8), 2e-4, 300
+    mesh, dom = _dom(gpu, shape, h, "implicit", tol=1e-13)
+    n = mesh.n
+    cells = np.random.default_rng(1).choice(n, 16, replace=False)
+    adj = gpu.TissueAdjoint(dom, cells, checkpoint_every=64)
+    kappa = torch.ones(n, dtype=torch.float64, device="cuda")
+    tr = gpu.probe_traces(adj, kappa, None, steps).cpu().numpy()
+    ref = np.empty((steps, len(cells)))
+    for s in range(steps):
+        dom.step(1)
+        ref[s] = dom.Vm[cells]
+    err = np.max(np.abs(tr - ref)) / np.max(np.abs(ref))
+    _say("  trace rel err %.3e" % err)
+    assert err < steps * 2e-13
+
+
+def test_handles_destroyed_in_any_order(gpu):
+    """Python's cyclic collector calls finalizers in no fixed order (a failed test's
+    traceback makes its locals such garbage), so the C-ABI lets handles go in any order:
+    here each owner goes before what depends on it -- ionic, mesh, tissue, then the
+    adjoint made from them."""
+    import torch
+    from cardiacfoam_amd import _capi
+    _say("handles destroyed owner first")
+    mesh, dom = _dom(gpu, (8, 6, 4), 2e-4, "implicit", tol=1e-10)
+    dom.step(3)
+    adj = gpu.TissueAdjoint(dom, [0, 5, 17], checkpoint_every=4)
+    kappa = torch.ones(mesh.n, dtype=torch.float64, device="cuda")
+    tr = gpu.probe_traces(adj, kappa, None, 10).cpu().numpy()
+    assert np.all(np.isfinite(tr))
+    L = _capi.lib()
+    for obj, attr, destroy in ((dom.ionic, "h", L.cg_ionic_destroy),
+                               (mesh, "m", L.cg_mesh_destroy),
+                               (dom, "t", L.cg_tissue_destroy),
+                               (adj, "a", L.cg_adjoint_destroy)):
+        h = getattr(obj, attr)
+        setattr(obj, attr, None)
+        assert destroy(h) == 0
+
+
+def test_implicit_adjoint_gradient_vs_oracle_finite_differences(gpu, orc):
+    """The implicit tissue's gradient (three random kappa directions, tau_fi, tau_si)
+    against central differences of the oracle's implicit loss (PCG to 1e-14); 240 steps with
+    checkpoints every 50 (partial last segment)."""
+    import torch
+    _say("implicit adjoint gradient vs oracle FD")
+    shape, h, steps = (12, 10, 6), 2e-4, 240
+    mesh, dom = _dom(gpu, shape, h, "implicit", tol=1e-13)
+    n = mesh.n
+    rng = np.random.default_rng(8)
+    cells = rng.choice(n, 24, replace=False)
+    theta0 = np.array([0.11, 1.8875])
+    kappa0 = rng.uniform(0.8, 1.25, n)
+    target = -0.084 + 0.1 * rng.random((steps, cells.size))
+    adj = gpu.TissueAdjoint(dom, cells, checkpoint_every=50)
+    kappa = torch.tensor(kappa0, dtype=torch.float64, device="cuda", requires_grad=True)
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    tr = gpu.probe_traces(adj, kappa, theta, steps)
+    loss = ((tr - torch.tensor(target, device="cuda")) ** 2).sum()
+    loss.backward()
+    gk = kappa.grad.cpu().numpy()
+    gt = theta.grad.numpy()
+    l_or = _oracle_loss(orc, shape, h, kappa0, theta0, steps, cells, target, implicit=1)
+    _say("  loss gpu %.12e oracle %.12e" % (loss.item(), l_or))
+    assert abs(loss.item() - l_or) <= 1e-10 * abs(l_or)
+    eps = 1e-6
+    for k in range(3):
+        d = rng.standard_normal(n)
+        fp = _oracle_loss(orc, shape, h, kappa0 + eps * d, theta0, steps, cells, target, 1)
+        fm = _oracle_loss(orc, shape, h, kappa0 - eps * d, theta0, steps, cells, target, 1)
+        fd, ad = (fp - fm) / (2 * eps), float(gk @ d)
+        _say("  kappa direction %d: adjoint %.9e fd %.9e" % (k, ad, fd))
+        assert abs(ad - fd) <= 1e-5 * abs(fd)
+    for i in range(2):
+        e = np.zeros(2)
+        e[i] = eps * theta0[i]
+        fp = _oracle_loss(orc, shape, h, kappa0, theta0 + e, steps, cells, target, 1)
+        fm = _oracle_loss(orc, shape, h, kappa0, theta0 - e, steps, cells, target, 1)
+        fd = (fp - fm) / (2 * e[i])
+        _say("  theta %d: adjoint %.9e fd %.9e" % (i, gt[i], fd))
+        assert abs(gt[i] - fd) <= 1e-5 * abs(fd)

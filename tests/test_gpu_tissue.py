@@ -222,3 +222,48 @@ def test_implicit_operator(gpu, orc, precision, coefft, shape):
     _say("  operator rel err %.3e" % err)
     # fp32: operands rounded to fp32 (2^-24 each), six-term sums with cancellation
     assert err <= (1e-13 if precision == "fp64" else 2e-5)
+
+
+@pytest.mark.parametrize("algorithm", ["explicit", "implicit"])
+def test_activation_threshold(gpu, orc, algorithm):
+    """cg_tissue_set_activation_threshold: with thresholds -0.5, 0 and +0.5 mV each cell's
+    activation time is the crossing of that level interpolated from the Vm of the two
+    steps around it (recomputed on the host from every step's Vm), and 0 reproduces the
+    default run bit for bit."""
+    _say("activation threshold", algorithm)
+    shape, h, dt, n_steps = (16, 10, 6), 2e-4, 2e-5, 160
+    sig = np.asarray(rotating_fibre_sigma(*shape)).reshape(-1, 9)
+    coeffs = {"ionicModel": "TNNP", "tissue": "epicardialCells", "chi": 1.4e5, "cm": 0.01,
+              "solutionAlgorithm": algorithm, "conductivity": sig,
+              "externalStimulus": {"stimulusLocationMin": [0, 0, 0],
+                                   "stimulusLocationMax": [6e-4, 6e-4, 6e-4],
+                                   "stimulusDuration": 2e-3, "stimulusIntensity": 1e6,
+                                   "stimulusStartTime": 0.0}}
+
+    def run(thr):
+        dom = gpu.Monodomain(gpu.HexMesh(*shape, h), coeffs, dt=dt, ddt="backward", tol=1e-12)
+        if thr is not None:
+            dom.set_activation_threshold(thr)
+        vs = [dom.Vm]
+        for _ in range(n_steps):
+            dom.step(1)
+            vs.append(dom.Vm)
+        return dom.activationTime, np.array(vs)
+
+    a_def, V = run(None)
+    assert (a_def > 0).sum() > 50
+    for thr in (-5e-4, 0.0, 5e-4):
+        a, _ = run(thr)
+        if thr == 0.0:
+            np.testing.assert_array_equal(a, a_def)
+        want = np.zeros(V.shape[1])
+        for c in range(V.shape[1]):
+            k = np.nonzero(V[1:, c] > thr + 1e-15)[0]
+            if k.size:
+                s = k[0]            # step s -> s+1 crosses
+                v0, v1 = V[s, c], V[s + 1, c]
+                want[c] = s * dt + (thr - v0) / (v1 - v0) * dt
+        _say("  threshold %+.1e V: %d activated, max |device - host| %.2e s" %
+             (thr, (a > 0).sum(), np.max(np.abs(a - want))))
+        np.testing.assert_array_equal(a > 0, want > 0)
+        assert np.max(np.abs(a - want)) <= 1e-12
