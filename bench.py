@@ -38,9 +38,9 @@ WORKLOADS = {
 
 # algorithmic bytes per cell (row) of one launch, by kernel and precision
 NSTATES = {"ORd": 41, "BuenoOrovio": 4, "TNNP": 17, "AlievPanfilov": 2, "Courtemanche": 21,
-           "PerisYague_2022": 22, "Stewart": 20}
+           "PerisYague_2022": 22, "Stewart": 20, "Grandi": 41}
 # the ionic models run on the workload's mesh beside its own model
-OTHER_MODELS = ("Courtemanche", "PerisYague_2022", "Stewart")
+OTHER_MODELS = ("Courtemanche", "PerisYague_2022", "Stewart", "Grandi")
 
 
 def algo_bytes_per_cell(kernel, prec, model, algorithm, od_planes=1):
@@ -934,6 +934,8 @@ def main():
                     help="only the 200 ms mixed-vs-FP64 comparison of the workload")
     ap.add_argument("--models-only", action="store_true",
                     help="only the other ionic models on the workload's mesh (profiling runs)")
+    ap.add_argument("--models", default="",
+                    help="with --models-only: these models, comma-separated (default all)")
     ap.add_argument("--legs", default="",
                     help="only these secondary legs, comma-separated: c0 (explicit "
                          "BuenoOrovio), c1, c2 (batches), c4 (adjoint); profiling runs")
@@ -984,10 +986,15 @@ def main():
         print(json.dumps({x: fns[x]() for x in legs}))
         return
     if args.models_only:
+        models = [m for m in args.models.split(",") if m] or list(OTHER_MODELS)
+        bad = [m for m in models if m not in OTHER_MODELS]
+        if bad:
+            print("unknown models %s (choose from %s)" % (bad, OTHER_MODELS), file=sys.stderr)
+            sys.exit(2)
         print(json.dumps({"workload": args.workload, "cells": n,
                           "models_2M": {m: run_model_leg(cf, torch, w, sigma, m, steps=args.steps,
                                                          warmup=args.warmup, prof_steps=0)
-                                        for m in OTHER_MODELS}}))
+                                        for m in models}}))
         return
     if args.only != "both":
         r = run_precision(cf, torch, w, sigma, args.only, args.steps, args.warmup,

@@ -40,10 +40,10 @@ int fail(int code, const char* fmt, ...)
             return fail(CG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));           \
     } while (0)
 
-constexpr int kNumModels = 8;
+constexpr int kNumModels = 9;
 const char* const kModelKeys[kNumModels] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
                                             "monodomainFDAManufactured", "Courtemanche",
-                                            "PerisYague_2022", "Stewart"};
+                                            "PerisYague_2022", "Stewart", "Grandi"};
 
 // state names, order of the *_STATES_NAMES tables / Names.H enums
 const char* const kBONames[4] = {"u", "v", "w", "s"};
@@ -76,6 +76,13 @@ const char* const kSTWNames[20] = {
     "IL_type_Ca_current_f2_gate_f2", "IL_type_Ca_current_fCass_gate_fCass",
     "Itransient_outward_current_s_gate_s", "Itransient_outward_current_r_gate_r", "Ca_i",
     "Ca_SR", "Ca_ss", "R_prime", "Na_i", "K_i"};
+const char* const kGRDNames[41] = {
+    "membrane_V", "Ca_i", "Ca_jn", "Ca_sl", "Ca_sr", "Csqn", "Ical_d", "Ical_f",
+    "Ical_fCaB_jn", "Ical_fCaB_sl", "Ikr_xr", "Iks_xs", "Ikur_ikur_r", "Ikur_s", "Ina_h",
+    "Ina_j", "Ina_m", "Inal_hl", "Inal_ml", "Ito_x", "Ito_y", "K_i", "ryr_i", "ryr_o",
+    "ryr_ryr_r", "Na_i", "Na_jn", "Na_sl", "buffCa_CaM", "buffCa_Myoc", "buffCa_Myom",
+    "buffCa_SLH_jn", "buffCa_SLH_sl", "buffCa_SLL_jn", "buffCa_SLL_sl", "buffCa_SRB",
+    "buffCa_TnCHc", "buffCa_TnCHm", "buffCa_TnCL", "buffNa_NaB_jn", "buffNa_NaB_sl"};
 
 struct ModelInfo {
     int ns, np;
@@ -111,6 +118,9 @@ bool model_info(int model, ModelInfo* mi)
     case cg::MODEL_STW:
         *mi = {cg::Stewart::NS, cg::Stewart::NP, 1.0, 0.0, 1.0, 1000.0};
         return true;
+    case cg::MODEL_GRD:
+        *mi = {cg::Grandi::NS, cg::Grandi::NP, 1.0, 0.0, 1.0, 1000.0};
+        return true;
     default: return false;
     }
 }
@@ -126,6 +136,7 @@ bool model_host_init(int model, int tissue, double* p, double* y)
     case cg::MODEL_CRN: return cg::Courtemanche::host_init(tissue, p, y);
     case cg::MODEL_PYS: return cg::PerisYague::host_init(tissue, p, y);
     case cg::MODEL_STW: return cg::Stewart::host_init(tissue, p, y);
+    case cg::MODEL_GRD: return cg::Grandi::host_init(tissue, p, y);
     default: return false;
     }
 }
@@ -141,6 +152,7 @@ double state_ref(int model, int q)
     case cg::MODEL_CRN: r = cg::StateRef<cg::Courtemanche>::at(q); break;
     case cg::MODEL_PYS: r = cg::StateRef<cg::PerisYague>::at(q); break;
     case cg::MODEL_STW: r = cg::StateRef<cg::Stewart>::at(q); break;
+    case cg::MODEL_GRD: r = cg::StateRef<cg::Grandi>::at(q); break;
     default: break;
     }
     return (double)(float)r;
@@ -373,7 +385,7 @@ int cg_model_state_name(int model, int i, char* buf, size_t len)
     const char* nm = model == 0 ? kBONames[i] : model == 1 ? kTNNPNames[i]
                    : model == 2 ? kORdNames[i] : model == 3 ? kAPNames[i]
                    : model == 4 ? kFDANames[i] : model == 5 ? kCRNNames[i]
-                   : model == 6 ? kPYSNames[i] : kSTWNames[i];
+                   : model == 6 ? kPYSNames[i] : model == 7 ? kSTWNames[i] : kGRDNames[i];
     snprintf(buf, len, "%s", nm);
     return CG_OK;
 }

@@ -89,6 +89,8 @@ template <class T> struct RlStep {
         y[i] = rl(y[i], xinf, tau, dt);
     }
     CG_HD void lin(int i, T a, T b) { gate(i, a / b, T(1) / b); }
+    // the same step for a rate the model writes in its own form r = a - b x
+    CG_HD void lin(int i, T a, T b, T) { gate(i, a / b, T(1) / b); }
     CG_HD void rate(int i, T r)
     {
         if constexpr (is_fp32<T>()) {
@@ -105,6 +107,7 @@ template <class T> struct Deriv {
     T* d;
     CG_HD void gate(int i, T xinf, T tau) { d[i] = (xinf - y[i]) / tau; }
     CG_HD void lin(int i, T a, T b) { d[i] = a - y[i] * b; }
+    CG_HD void lin(int i, T, T, T r) { d[i] = r; }
     CG_HD void rate(int i, T r) { d[i] = r; }
 };
 

@@ -16,11 +16,12 @@
 #include "model_crn.hpp"
 #include "model_pys.hpp"
 #include "model_stw.hpp"
+#include "model_grd.hpp"
 
 namespace cg {
 
 enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3, MODEL_FDA = 4,
-               MODEL_CRN = 5, MODEL_PYS = 6, MODEL_STW = 7 };
+               MODEL_CRN = 5, MODEL_PYS = 6, MODEL_STW = 7, MODEL_GRD = 8 };
 
 constexpr int MAX_PARAMS = 48;
 struct ModelParams {
@@ -81,6 +82,17 @@ template <> struct StateRef<Stewart> {
     {
         return q == Stewart::Nai ? 8.8042029 : q == Stewart::Ki ? 136.7818942
              : q == Stewart::CaSR ? 3.1083689 : 0.0;
+    }
+};
+
+template <> struct StateRef<Grandi> {
+    static constexpr bool any = true;
+    CG_HD static constexpr double at(int q)
+    {
+        return (q == Grandi::Nai || q == Grandi::Najn || q == Grandi::Nasl) ? 9.1519968
+             : q == Grandi::Ki ? 120.0 : q == Grandi::Casr ? 0.50230583
+             : q == Grandi::Csqn ? 1.1333754 : q == Grandi::NaBjn ? 3.6139606
+             : q == Grandi::NaBsl ? 0.78860779 : 0.0;
     }
 };
 
@@ -429,6 +441,55 @@ template <> struct Ops<Stewart> {
                              T istim, const Stewart::VTerms<T>& q)
     {
         Stewart::deriv(y, d, p, gs, solve_v, istim, q);
+    }
+};
+
+template <> struct Ops<Grandi> {
+    template <class T>
+    CG_HD static void cell_step_off(T* y, T* s, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = Grandi::vterms(y[0], p);
+        cell_step_offsets<Grandi>(
+            y, s, dt, [&](RlStep<T>& out) { Grandi::rates(y, out, p, gs, true, istim, q); });
+    }
+    template <class T>
+    CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = Grandi::vterms(v, p);
+        return tissue_step_offsets<Grandi>(
+            y, s, dt, [&](RlStep<T>& out) { Grandi::rates(y, out, p, gs, false, T(0), q); },
+            [&]() { return Grandi::currents(v, y, p, q, gs).total(); });
+    }
+    template <class T>
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = Grandi::vterms(v, p);
+        Grandi::step(y, dt, p, gs, false, T(0), q);
+        return Grandi::currents(v, y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = Grandi::vterms(y[0], p);
+        Grandi::step(y, dt, p, gs, true, istim, q);
+    }
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T* gs)
+    {
+        const auto q = Grandi::vterms(y[0], p);
+        return Grandi::currents(y[0], y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static Grandi::VTerms<T> vt(T v, const T* p)
+    {
+        return Grandi::vterms(v, p);
+    }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int, const T* gs, bool solve_v,
+                             T istim, const Grandi::VTerms<T>& q)
+    {
+        Grandi::deriv(y, d, p, gs, solve_v, istim, q);
     }
 };
 

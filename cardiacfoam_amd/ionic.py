@@ -15,7 +15,7 @@ from . import _capi
 from ._capi import check, lib
 
 MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov", "monodomainFDAManufactured",
-              "Courtemanche", "PerisYague_2022", "Stewart"]
+              "Courtemanche", "PerisYague_2022", "Stewart", "Grandi"]
 SUPPORTED_TISSUES = {
     "BuenoOrovio": ["endocardialCells", "mCells", "epicardialCells"],
     "TNNP": ["endocardialCells", "mCells", "epicardialCells"],
@@ -24,6 +24,7 @@ SUPPORTED_TISSUES = {
     "Courtemanche": ["myocyte"],  # Courtemanche.C:92-95
     "PerisYague_2022": ["myocyte"],  # PerisYague.C:92-95
     "Stewart": ["myocyte"],  # Stewart.C:93-96
+    "Grandi": ["myocyte"],  # Grandi.C:93-96
     # the manufactured model's supportedDimensions() (monodomainFDAManufactured.C:85-88)
     "monodomainFDAManufactured": ["1D", "2D", "3D"],
 }

@@ -1193,6 +1193,310 @@ static void stw_eval(double t, const double* c, const double* y, int solve_vm,
     *iion = Iion; *istim = Ist;
 }
 
+/* ================================================================== Grandi
+ * Grandi et al. 2011 human atrial myocyte (41 states): constants and initial state of
+ * Grandi_2011.H:414-657 (GrandiinitConsts, with AF = ISO = RA = 0 as it sets them), right-
+ * hand side of :660-931, state order of its STATES_INDEX enum.  Tissue "myocyte" only
+ * (Grandi.C:93-96); V in mV (Grandi.C:121-124), Im = Iion_cm.  K_i is constant (its rate
+ * is 0).  In tissue mode dV/dt is 0.
+ * Rush-Larsen classification: the Hodgkin-Huxley gates (d f xr xs ikur_r ikur_s h j m hl
+ * x y) as gates, the first-order kinetics linear in their own state (ml, fCaB_jn/sl, the
+ * Ca and Na buffers, Csqn) as linear gates with the other states frozen, Ca_sr, Ca_jn and
+ * Ca_sl linearised in themselves (below), the rest forward Euler.
+ * Parity pins: the reference's own expressions at seeded states
+ * (tests/test_oracle_model_vectors.py). */
+enum { GR_V, GR_Cai, GR_Cajn, GR_Casl, GR_Casr, GR_Csqn, GR_d, GR_f, GR_fCaBjn, GR_fCaBsl,
+       GR_xr, GR_xs, GR_kur_r, GR_kur_s, GR_h, GR_j, GR_m, GR_hl, GR_ml, GR_x, GR_y, GR_Ki,
+       GR_ri, GR_ro, GR_rr, GR_Nai, GR_Najn, GR_Nasl, GR_CaM, GR_Myoc, GR_Myom, GR_SLHjn,
+       GR_SLHsl, GR_SLLjn, GR_SLLsl, GR_SRB, GR_TnCHc, GR_TnCHm, GR_TnCL, GR_NaBjn, GR_NaBsl,
+       GR_NS };
+/* runtime constants (scalable per cell: gNa, gKr, pCa, gKs) and the geometry and
+ * buffer constants the rates use */
+enum { GC_gNa, GC_gKr, GC_pCa, GC_gKs, GC_gK1, GC_gto, GC_gKur, GC_gCaB, GC_gNaB, GC_gClB,
+       GC_GClCa, GC_IbarNaK, GC_IbarNCX, GC_IbarSLCaP, GC_VmaxSRCaP, GC_ks, GC_pK, GC_pNa,
+       GC_gKp, GC_gNaL, GC_Vjn, GC_Vsl, GC_Vmyo, GC_Vsr, GC_BmaxSLHj, GC_BmaxSLHsl,
+       GC_BmaxSLLj, GC_BmaxSLLsl, GC_BmaxCsqn, GC_sigma, GC_ECl, GC_FRT, GC_NC };
+
+static void grd_init(double* c, double* y)
+{
+    const double pi = 3.14159265358979312, len = 100.0, rad = 10.25;
+    const double Vcell = pi * pow(rad, 2.0) * len * 1e-15;
+    const double F = 96485.0, R = 8314.0, T = 310.0;
+    c[GC_Vjn] = 0.0539 * 0.01 * Vcell;
+    c[GC_Vmyo] = 0.65 * Vcell;
+    c[GC_Vsl] = 0.02 * Vcell;
+    c[GC_Vsr] = 0.035 * Vcell;
+    c[GC_FRT] = F / R / T;
+    c[GC_BmaxSLLj] = 0.0046 * c[GC_Vmyo] / c[GC_Vjn] * 0.1;
+    c[GC_BmaxSLLsl] = c[GC_Vmyo] / c[GC_Vsl] * 0.0374;
+    c[GC_BmaxSLHj] = 0.00165 * c[GC_Vmyo] / c[GC_Vjn] * 0.1;
+    c[GC_BmaxSLHsl] = c[GC_Vmyo] / c[GC_Vsl] * 0.0134;
+    c[GC_BmaxCsqn] = c[GC_Vmyo] / c[GC_Vsr] * 0.14;
+    c[GC_sigma] = (exp(140.0 / 67.3) - 1.0) / 7.0;
+    c[GC_ECl] = 1.0 / c[GC_FRT] * log(15.0 / 150.0);
+    c[GC_gNa] = 23.0;
+    c[GC_gKr] = 0.035 * sqrt(5.4 / 5.4);
+    c[GC_pCa] = 0.00027;
+    c[GC_gKs] = 0.0035;
+    c[GC_gK1] = sqrt(5.4 / 5.4) * 0.0525;
+    c[GC_gto] = 0.165;
+    c[GC_gKur] = 0.045;
+    c[GC_gCaB] = 6.06430000000000033e-04;
+    c[GC_gNaB] = 0.000597;
+    c[GC_gClB] = 0.009;
+    c[GC_GClCa] = 0.0548;
+    c[GC_IbarNaK] = 1.26;
+    c[GC_IbarNCX] = 3.15;
+    c[GC_IbarSLCaP] = 0.0471;
+    c[GC_VmaxSRCaP] = 0.0053114;
+    c[GC_ks] = 25.0;
+    c[GC_pK] = 1.35e-07;
+    c[GC_pNa] = 7.5e-09;
+    c[GC_gKp] = 0.002;
+    c[GC_gNaL] = 0.0;                       /* gNaL_max * AF, AF = 0 */
+    static const double y0[GR_NS] = {
+        -7.34336366728778671e+01, 2.10808768153058460e-04, 3.25814677291117296e-04,
+        2.33018340557575125e-04, 5.02305826642838293e-01, 1.13337536953687845e+00,
+        2.16850216379767157e-05, 9.98384427312367095e-01, 4.49572164109603364e-02,
+        3.28512098597005947e-02, 1.31290096227093382e-03, 7.49436760722081534e-03,
+        3.93548562883350357e-04, 9.58234428284286399e-01, 3.15482710277587786e-01,
+        2.48034071360795916e-01, 1.89326933812916480e-02, 3.79829335413739144e-02,
+        1.01974216400706526e-02, 1.37939236359928058e-03, 9.45874848392074696e-01, 120.0,
+        5.01323282772066123e-07, 2.01567245823636694e-06, 8.00819151705148946e-01,
+        9.15199678386256998e+00, 9.15153381546177336e+00, 9.15182798281732346e+00,
+        7.02128101897185673e-04, 3.94923428392655786e-03, 1.35538532457244482e-01,
+        1.03674364292988680e-01, 1.90759804527589089e-01, 1.35640688636079511e-02,
+        2.14063418881809235e-02, 4.45327242854324807e-03, 1.27856586024588575e-01,
+        5.69999505293381902e-03, 1.83143535034222225e-02, 3.61396062660070427e+00,
+        7.88607791910409195e-01};
+    for (int i = 0; i < GR_NS; ++i) y[i] = y0[i];
+}
+
+static void grd_eval(double t, const double* c, const double* y, int solve_vm,
+                     const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                     double* iion, double* istim)
+{
+    const double V = y[GR_V], FRT = c[GC_FRT], F = 96485.0, Cm = 1.1e-10;
+    const double Fjn = 0.11, Fsl = 1.0 - 0.11, FjnL = 0.9, FslL = 1.0 - 0.9;
+    const double Ko = 5.4, Nao = 140.0, Cao = 1.8, Mgi = 1.0;
+    const double Vjn = c[GC_Vjn], Vsl = c[GC_Vsl], Vmyo = c[GC_Vmyo], Vsr = c[GC_Vsr];
+    for (int i = 0; i < GR_NS; ++i) g[i] = 0;
+#define GR_GATE(idx, inf, tau) do { const double i_ = (inf), t_ = (tau); \
+        r[idx] = (i_ - y[idx]) / t_; g[idx] = 1; xi[idx] = i_; ta[idx] = t_; } while (0)
+    /* linear in its own state: dx/dt = a - b x; the rate itself as the reference writes it */
+#define GR_LIN(idx, a, b, rate) do { const double a_ = (a), b_ = (b); \
+        r[idx] = (rate); g[idx] = 1; xi[idx] = a_ / b_; ta[idx] = 1.0 / b_; } while (0)
+
+    /* voltage-gated kinetics */
+    GR_GATE(GR_xr, 1.0 / (1.0 + exp(-(V + 10.0) / 5.0)),
+            550.0 / (1.0 + exp((-22.0 - V) / 9.0)) * 6.0 / (1.0 + exp((V + 11.0) / 9.0)) +
+                230.0 / (1.0 + exp((V + 40.0) / 20.0)));
+    GR_GATE(GR_kur_r, 1.0 / (1.0 + exp((V + 6.0) / -8.6)),
+            9.0 / (1.0 + exp((V + 5.0) / 12.0)) + 0.5);
+    GR_GATE(GR_kur_s, 1.0 / (1.0 + exp((V + 7.5) / 10.0)),
+            590.0 / (1.0 + exp((V + 60.0) / 10.0)) + 3050.0);
+    const double hjinf = 1.0 / pow(1.0 + exp((V + 71.55) / 7.43), 2.0);
+    const double ah = V >= -40.0 ? 0.0 : 0.057 * exp(-(V + 80.0) / 6.8);
+    const double bh = V >= -40.0 ? 0.77 / (0.13 * (1.0 + exp(-(V + 10.66) / 11.1)))
+                                 : 2.7 * exp(0.079 * V) + 3.1 * pow(10.0, 5.0) * exp(0.3485 * V);
+    GR_GATE(GR_h, hjinf, 1.0 / (ah + bh));
+    const double aj = V >= -40.0 ? 0.0
+        : (-2.5428 * pow(10.0, 4.0) * exp(0.2444 * V) - 6.948 * pow(10.0, -6.0) * exp(-0.04391 * V)) *
+              (V + 37.78) / (1.0 + exp(0.311 * (V + 79.23)));
+    const double bj = V >= -40.0 ? 0.6 * exp(0.057 * V) / (1.0 + exp(-0.1 * (V + 32.0)))
+                                 : 0.02424 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)));
+    GR_GATE(GR_j, hjinf, 1.0 / (aj + bj));
+    GR_GATE(GR_m, 1.0 / pow(1.0 + exp(-(56.86 + V) / 9.03), 2.0),
+            0.1292 * exp(-pow((V + 45.79) / 15.54, 2.0)) + 0.06487 * exp(-pow((V - 4.823) / 51.12, 2.0)));
+    GR_GATE(GR_hl, 1.0 / (1.0 + exp((V + 91.0) / 6.1)), 600.0);
+    const double aml = 0.32 * (V + 47.13) / (1.0 - exp(-0.1 * (V + 47.13)));
+    const double bml = 0.08 * exp(-V / 11.0);
+    GR_LIN(GR_ml, aml, aml + bml, aml * (1.0 - y[GR_ml]) - bml * y[GR_ml]);
+    GR_GATE(GR_x, 1.0 / (1.0 + exp(-(V + 1.0) / 11.0)), 3.5 * exp(-pow(V / 30.0, 2.0)) + 1.5);
+    GR_GATE(GR_y, 1.0 / (1.0 + exp((V + 40.5) / 11.5)),
+            25.635 * exp(-pow((V + 52.45) / 15.8827, 2.0)) + 24.14);
+    const double dinf = 1.0 / (1.0 + exp(-(V + 9.0) / 6.0));
+    GR_GATE(GR_d, dinf, dinf * (1.0 - exp(-(V + 9.0) / 6.0)) / (0.035 * (V + 9.0)));
+    GR_GATE(GR_f, 1.0 / (1.0 + exp((V + 30.0) / 7.0)) + 0.2 / (1.0 + exp((50.0 - V) / 20.0)),
+            1.0 / (0.0197 * exp(-pow(0.0337 * (V + 25.0), 2.0)) + 0.02));
+    GR_GATE(GR_xs, 1.0 / (1.0 + exp(-(V + 3.8) / 14.25)),
+            990.1 / (1.0 + exp(-(V + 2.436) / 14.12)));
+    GR_LIN(GR_fCaBjn, 1.7 * y[GR_Cajn], 1.7 * y[GR_Cajn] + 0.0119,
+           (1.7 * y[GR_Cajn] * (1.0 - y[GR_fCaBjn]) - 0.0119 * y[GR_fCaBjn]) * 1.0);
+    GR_LIN(GR_fCaBsl, 1.7 * y[GR_Casl], 1.7 * y[GR_Casl] + 0.0119,
+           (1.7 * y[GR_Casl] * (1.0 - y[GR_fCaBsl]) - 0.0119 * y[GR_fCaBsl]) * 1.0);
+
+    /* buffers: kon C (Bmax - B) - koff B, linear in B (a second bound form frozen) */
+    const double dNaBjn = 0.0001 * y[GR_Najn] * (7.561 - y[GR_NaBjn]) - 0.001 * y[GR_NaBjn];
+    const double dNaBsl = 0.0001 * y[GR_Nasl] * (1.65 - y[GR_NaBsl]) - 0.001 * y[GR_NaBsl];
+    GR_LIN(GR_NaBjn, 0.0001 * y[GR_Najn] * 7.561, 0.0001 * y[GR_Najn] + 0.001, dNaBjn);
+    GR_LIN(GR_NaBsl, 0.0001 * y[GR_Nasl] * 1.65, 0.0001 * y[GR_Nasl] + 0.001, dNaBsl);
+    const double koff_tncl = 0.0196;
+    const double dCaM = 34.0 * y[GR_Cai] * (0.024 - y[GR_CaM]) - 0.238 * y[GR_CaM];
+    const double dMyoc = 13.8 * y[GR_Cai] * (0.14 - y[GR_Myoc] - y[GR_Myom]) - 0.00046 * y[GR_Myoc];
+    const double dMyom = 0.0157 * Mgi * (0.14 - y[GR_Myoc] - y[GR_Myom]) - 5.7e-05 * y[GR_Myom];
+    const double dSLHjn = 100.0 * y[GR_Cajn] * (c[GC_BmaxSLHj] - y[GR_SLHjn]) - 0.03 * y[GR_SLHjn];
+    const double dSLHsl = 100.0 * y[GR_Casl] * (c[GC_BmaxSLHsl] - y[GR_SLHsl]) - 0.03 * y[GR_SLHsl];
+    const double dSLLjn = 100.0 * y[GR_Cajn] * (c[GC_BmaxSLLj] - y[GR_SLLjn]) - 1.3 * y[GR_SLLjn];
+    const double dSLLsl = 100.0 * y[GR_Casl] * (c[GC_BmaxSLLsl] - y[GR_SLLsl]) - 1.3 * y[GR_SLLsl];
+    const double dSRB = 100.0 * y[GR_Cai] * (0.0171 - y[GR_SRB]) - 0.06 * y[GR_SRB];
+    const double dTnCHc = 2.37 * y[GR_Cai] * (0.14 - y[GR_TnCHc] - y[GR_TnCHm]) - 3.2e-05 * y[GR_TnCHc];
+    const double dTnCHm = 0.003 * Mgi * (0.14 - y[GR_TnCHc] - y[GR_TnCHm]) - 0.00333 * y[GR_TnCHm];
+    const double dTnCL = 32.7 * y[GR_Cai] * (0.07 - y[GR_TnCL]) - koff_tncl * y[GR_TnCL];
+    GR_LIN(GR_CaM, 34.0 * y[GR_Cai] * 0.024, 34.0 * y[GR_Cai] + 0.238, dCaM);
+    GR_LIN(GR_Myoc, 13.8 * y[GR_Cai] * (0.14 - y[GR_Myom]), 13.8 * y[GR_Cai] + 0.00046, dMyoc);
+    GR_LIN(GR_Myom, 0.0157 * Mgi * (0.14 - y[GR_Myoc]), 0.0157 * Mgi + 5.7e-05, dMyom);
+    GR_LIN(GR_SLHjn, 100.0 * y[GR_Cajn] * c[GC_BmaxSLHj], 100.0 * y[GR_Cajn] + 0.03, dSLHjn);
+    GR_LIN(GR_SLHsl, 100.0 * y[GR_Casl] * c[GC_BmaxSLHsl], 100.0 * y[GR_Casl] + 0.03, dSLHsl);
+    GR_LIN(GR_SLLjn, 100.0 * y[GR_Cajn] * c[GC_BmaxSLLj], 100.0 * y[GR_Cajn] + 1.3, dSLLjn);
+    GR_LIN(GR_SLLsl, 100.0 * y[GR_Casl] * c[GC_BmaxSLLsl], 100.0 * y[GR_Casl] + 1.3, dSLLsl);
+    GR_LIN(GR_SRB, 100.0 * y[GR_Cai] * 0.0171, 100.0 * y[GR_Cai] + 0.06, dSRB);
+    GR_LIN(GR_TnCHc, 2.37 * y[GR_Cai] * (0.14 - y[GR_TnCHm]), 2.37 * y[GR_Cai] + 3.2e-05, dTnCHc);
+    GR_LIN(GR_TnCHm, 0.003 * Mgi * (0.14 - y[GR_TnCHc]), 0.003 * Mgi + 0.00333, dTnCHm);
+    GR_LIN(GR_TnCL, 32.7 * y[GR_Cai] * 0.07, 32.7 * y[GR_Cai] + koff_tncl, dTnCL);
+    const double JCaBcyt = dTnCL + dTnCHc + dTnCHm + dCaM + dMyoc + dMyom + dSRB;
+    const double JCaBjn = dSLLjn + dSLHjn;
+    const double JCaBsl = dSLLsl + dSLHsl;
+
+    /* reversal potentials */
+    const double ECajn = 1.0 / FRT / 2.0 * log(Cao / y[GR_Cajn]);
+    const double ECasl = 1.0 / FRT / 2.0 * log(Cao / y[GR_Casl]);
+    const double EK = 1.0 / FRT * log(Ko / y[GR_Ki]);
+    const double ENajn = 1.0 / FRT * log(Nao / y[GR_Najn]);
+    const double ENasl = 1.0 / FRT * log(Nao / y[GR_Nasl]);
+    const double EKs = 1.0 / FRT * log((Ko + 0.01833 * Nao) / (y[GR_Ki] + 0.01833 * y[GR_Nai]));
+
+    /* INaK */
+    const double fnak = 1.0 / (1.0 + 0.1245 * exp(-0.1 * V * FRT) + 0.0365 * c[GC_sigma] * exp(-V * FRT));
+    const double KmNaip = 11.0;
+    const double INaKjn = Fjn * c[GC_IbarNaK] * fnak * Ko / (1.0 + pow(KmNaip / y[GR_Najn], 4.0)) / (Ko + 1.5);
+    const double INaKsl = Fsl * c[GC_IbarNaK] * fnak * Ko / (1.0 + pow(KmNaip / y[GR_Nasl], 4.0)) / (Ko + 1.5);
+    const double INaK = INaKjn + INaKsl;
+
+    /* RyR and SR fluxes */
+    const double RI = 1.0 - y[GR_rr] - y[GR_ro] - y[GR_ri];
+    const double JSRCarel = c[GC_ks] * y[GR_ro] * (y[GR_Casr] - y[GR_Cajn]);
+    const double JSRleak = (y[GR_Casr] - y[GR_Cajn]) * 5.348e-06;
+    const double Kmf = 2.5 * 0.000246, Kmr = 1.7, hill = 1.787;
+    const double fi = pow(y[GR_Cai] / Kmf, hill), fr = pow(y[GR_Casr] / Kmr, hill);
+    const double Jserca = pow(2.6, 0.0) * c[GC_VmaxSRCaP] * (fi - fr) / (1.0 + fi + fr);
+    const double kCaSR = 15.0 - (15.0 - 1.0) / (1.0 + pow(0.45 / y[GR_Casr], 2.5));
+    const double kiSRCa = 0.5 * kCaSR, koSRCa = 10.0 / kCaSR;
+    const double cj2 = pow(y[GR_Cajn], 2.0);
+    r[GR_ri] = kiSRCa * y[GR_Cajn] * y[GR_ro] - 0.005 * y[GR_ri] - (0.06 * y[GR_ri] - koSRCa * cj2 * RI);
+    r[GR_ro] = koSRCa * cj2 * y[GR_rr] - 0.06 * y[GR_ro] - (kiSRCa * y[GR_Cajn] * y[GR_ro] - 0.005 * y[GR_ri]);
+    r[GR_rr] = 0.005 * RI - kiSRCa * y[GR_Cajn] * y[GR_rr] - (koSRCa * cj2 * y[GR_rr] - 0.06 * y[GR_ro]);
+
+    /* background and L-type currents */
+    const double ICaBjn = Fjn * c[GC_gCaB] * (V - ECajn), ICaBsl = Fsl * c[GC_gCaB] * (V - ECasl);
+    const double e2 = exp(2.0 * V * FRT), e1 = exp(V * FRT);
+    const double ibarcajn = c[GC_pCa] * 4.0 * (V * F * FRT) * (0.341 * y[GR_Cajn] * e2 - 0.341 * Cao) / (e2 - 1.0);
+    const double ibarcasl = c[GC_pCa] * 4.0 * (V * F * FRT) * (0.341 * y[GR_Casl] * e2 - 0.341 * Cao) / (e2 - 1.0);
+    const double ibark = c[GC_pK] * V * F * FRT * (0.75 * y[GR_Ki] * e1 - 0.75 * Ko) / (e1 - 1.0);
+    const double ibarnajn = c[GC_pNa] * V * F * FRT * (0.75 * y[GR_Najn] * e1 - 0.75 * Nao) / (e1 - 1.0);
+    const double ibarnasl = c[GC_pNa] * V * F * FRT * (0.75 * y[GR_Nasl] * e1 - 0.75 * Nao) / (e1 - 1.0);
+    const double q10 = pow(1.8, 0.0) * 0.45, df = y[GR_d] * y[GR_f];
+    const double ICaLCajn = FjnL * ibarcajn * df * (1.0 - y[GR_fCaBjn]) * q10;
+    const double ICaLCasl = FslL * ibarcasl * df * (1.0 - y[GR_fCaBsl]) * q10;
+    const double ICaLK = ibark * df * (FjnL * (1.0 - y[GR_fCaBjn]) + FslL * (1.0 - y[GR_fCaBsl])) * q10;
+    const double ICaLNajn = FjnL * ibarnajn * df * (1.0 - y[GR_fCaBjn]) * q10;
+    const double ICaLNasl = FslL * ibarnasl * df * (1.0 - y[GR_fCaBsl]) * q10;
+
+    /* chloride */
+    const double IClB = c[GC_gClB] * (V - c[GC_ECl]);
+    const double IClCa = Fjn * c[GC_GClCa] / (1.0 + 0.1 / y[GR_Cajn]) * (V - c[GC_ECl]) +
+                         Fsl * c[GC_GClCa] / (1.0 + 0.1 / y[GR_Casl]) * (V - c[GC_ECl]);
+
+    /* potassium */
+    const double k1a = 1.02 / (1.0 + exp(0.2385 * (V - EK - 59.215)));
+    const double k1b = (0.49124 * exp(0.08032 * (V + 5.476 - EK)) + exp(0.06175 * (V - EK - 594.31))) /
+                       (1.0 + exp(-0.5143 * (V - EK + 4.753)));
+    const double IK1 = c[GC_gK1] * (k1a / (k1a + k1b)) * (V - EK);
+    const double kp = 1.0 / (1.0 + exp(7.488 - V / 5.98));
+    const double IKp = Fjn * c[GC_gKp] * kp * (V - EK) + Fsl * c[GC_gKp] * kp * (V - EK);
+    const double IKr = c[GC_gKr] * y[GR_xr] * (1.0 / (1.0 + exp((V + 74.0) / 24.0))) * (V - EK);
+    const double xs2 = pow(y[GR_xs], 2.0);
+    const double IKs = Fjn * c[GC_gKs] * xs2 * (V - EKs) + Fsl * c[GC_gKs] * xs2 * (V - EKs);
+    const double IKur = c[GC_gKur] * y[GR_kur_r] * y[GR_kur_s] * (V - EK);
+    const double Ito = c[GC_gto] * y[GR_x] * y[GR_y] * (V - EK);
+
+    /* sodium */
+    const double m3hj = pow(y[GR_m], 3.0) * y[GR_h] * y[GR_j];
+    const double INajn = Fjn * c[GC_gNa] * m3hj * (V - ENajn);
+    const double INasl = Fsl * c[GC_gNa] * m3hj * (V - ENasl);
+    const double INaBjn = Fjn * c[GC_gNaB] * (V - ENajn), INaBsl = Fsl * c[GC_gNaB] * (V - ENasl);
+    const double ml3 = pow(y[GR_ml], 3.0);
+    const double INaLjn = Fjn * c[GC_gNaL] * ml3 * y[GR_hl] * (V - ENajn);
+    const double INaLsl = Fsl * c[GC_gNaL] * ml3 * y[GR_hl] * (V - ENasl);
+
+    /* NCX and sarcolemmal Ca pump */
+    const double nu = 0.35, KmCai = 0.00359, KmCao = 1.3, KmNai = 12.29, KmNao = 87.5;
+    const double Kajn = 1.0 / (1.0 + pow(0.000384 / y[GR_Cajn], 2.0));
+    const double Kasl = 1.0 / (1.0 + pow(0.000384 / y[GR_Casl], 2.0));
+    const double en = exp(nu * V * FRT), en1 = exp((nu - 1.0) * V * FRT);
+    const double Nao3 = pow(Nao, 3.0), KmNao3 = pow(KmNao, 3.0);
+    const double nj3 = pow(y[GR_Najn], 3.0), ns3 = pow(y[GR_Nasl], 3.0);
+    const double s1jn = en * nj3 * Cao, s2jn = en1 * Nao3 * y[GR_Cajn];
+    const double s3jn = KmCai * Nao3 * (1.0 + pow(y[GR_Najn] / KmNai, 3.0)) +
+                        KmNao3 * y[GR_Cajn] * (1.0 + y[GR_Cajn] / KmCai) + KmCao * nj3 +
+                        nj3 * Cao + Nao3 * y[GR_Cajn];
+    const double s1sl = en * ns3 * Cao, s2sl = en1 * Nao3 * y[GR_Casl];
+    const double s3sl = KmCai * Nao3 * (1.0 + pow(y[GR_Nasl] / KmNai, 3.0)) +
+                        KmNao3 * y[GR_Casl] * (1.0 + y[GR_Casl] / KmCai) + KmCao * ns3 +
+                        ns3 * Cao + Nao3 * y[GR_Casl];
+    const double ncx = c[GC_IbarNCX] * pow(1.57, 0.0);
+    const double INaCajn = Fjn * ncx * Kajn * (s1jn - s2jn) / s3jn / (1.0 + 0.27 * en1);
+    const double INaCasl = Fsl * ncx * Kasl * (s1sl - s2sl) / s3sl / (1.0 + 0.27 * en1);
+    const double pb = pow(0.0005 * 1.0, 1.6);
+    const double paj = pow(y[GR_Cajn] * 1.0, 1.6), pas = pow(y[GR_Casl] * 1.0, 1.6);
+    const double IpCajn = Fjn * pow(2.35, 0.0) * c[GC_IbarSLCaP] * paj / (paj + pb);
+    const double IpCasl = Fsl * pow(2.35, 0.0) * c[GC_IbarSLCaP] * pas / (pas + pb);
+
+    /* totals */
+    const double ICatjn = ICaLCajn + ICaBjn + IpCajn - 2.0 * INaCajn;
+    const double ICatsl = ICaLCasl + ICaBsl + IpCasl - 2.0 * INaCasl;
+    const double INatjn = INajn + INaBjn + 3.0 * INaCajn + 3.0 * INaKjn + ICaLNajn + INaLjn;
+    const double INatsl = INasl + INaBsl + 3.0 * INaCasl + 3.0 * INaKsl + ICaLNasl + INaLsl;
+    const double IKt = Ito + IKr + IKs + IK1 - 2.0 * INaK + ICaLK + IKp + IKur;
+    const double Iion = (INatjn + INatsl) + (IClCa + IClB) + (ICatjn + ICatsl) + IKt;
+
+    /* concentrations */
+    const double dCsqn = 100.0 * y[GR_Casr] * (c[GC_BmaxCsqn] - y[GR_Csqn]) - 65.0 * y[GR_Csqn];
+    GR_LIN(GR_Csqn, 100.0 * y[GR_Casr] * c[GC_BmaxCsqn], 100.0 * y[GR_Casr] + 65.0, dCsqn);
+    const double JCaslmyo = 3.72425607984805052e-12, JCajnsl = 8.24130542277896849e-13;
+    const double JNaslmyo = 1.63862792221979433e-12, JNajnsl = 1.83127823220607955e-14;
+    r[GR_Cai] = -Jserca * Vsr / Vmyo - JCaBcyt + JCaslmyo / Vmyo * (y[GR_Casl] - y[GR_Cai]);
+    r[GR_Nai] = JNaslmyo / Vmyo * (y[GR_Nasl] - y[GR_Nai]);
+    /* SR, junctional and subsarcolemmal Ca: stiff at dt ~ 10 us (release, the small
+     * junction volume and the buffer on-rates give -d(rate)/dx ~ 1e2..1e3 /ms), so the
+     * Rush-Larsen step treats each as linear in itself with the diagonal coefficient b of
+     * those terms frozen: x' = a - b x, a = rate + b x (the same rate) */
+    const double rCasr = Jserca - (JSRleak * Vmyo / Vsr + JSRCarel) - dCsqn;
+    const double bCasr = c[GC_ks] * y[GR_ro] + 5.348e-06 * Vmyo / Vsr +
+                         100.0 * (c[GC_BmaxCsqn] - y[GR_Csqn]);
+    GR_LIN(GR_Casr, rCasr + bCasr * y[GR_Casr], bCasr, rCasr);
+    const double rCajn = -ICatjn * Cm / (Vjn * 2.0 * F) + JCajnsl / Vjn * (y[GR_Casl] - y[GR_Cajn]) -
+                         JCaBjn + JSRCarel * Vsr / Vjn + JSRleak * Vmyo / Vjn;
+    const double bCajn = c[GC_ks] * y[GR_ro] * Vsr / Vjn + 5.348e-06 * Vmyo / Vjn + JCajnsl / Vjn +
+                         100.0 * ((c[GC_BmaxSLLj] - y[GR_SLLjn]) + (c[GC_BmaxSLHj] - y[GR_SLHjn]));
+    GR_LIN(GR_Cajn, rCajn + bCajn * y[GR_Cajn], bCajn, rCajn);
+    const double rCasl = -ICatsl * Cm / (Vsl * 2.0 * F) + JCajnsl / Vsl * (y[GR_Cajn] - y[GR_Casl]) +
+                         JCaslmyo / Vsl * (y[GR_Cai] - y[GR_Casl]) - JCaBsl;
+    const double bCasl = JCajnsl / Vsl + JCaslmyo / Vsl +
+                         100.0 * ((c[GC_BmaxSLLsl] - y[GR_SLLsl]) + (c[GC_BmaxSLHsl] - y[GR_SLHsl]));
+    GR_LIN(GR_Casl, rCasl + bCasl * y[GR_Casl], bCasl, rCasl);
+    r[GR_Najn] = -INatjn * Cm / (Vjn * F) + JNajnsl / Vjn * (y[GR_Nasl] - y[GR_Najn]) - dNaBjn;
+    r[GR_Nasl] = -INatsl * Cm / (Vsl * F) + JNajnsl / Vsl * (y[GR_Najn] - y[GR_Nasl]) +
+                 JNaslmyo / Vsl * (y[GR_Nai] - y[GR_Nasl]) - dNaBsl;
+    r[GR_Ki] = 0.0;
+#undef GR_GATE
+#undef GR_LIN
+    const double Ist = orc_stimulus(t, st);
+    r[GR_V] = solve_vm ? -Iion - Ist : 0.0;
+    *iion = Iion;
+    *istim = Ist;
+}
+
 /* ============================================================== dispatch */
 int orc_model_nstates(int model)
 {
@@ -1205,6 +1509,7 @@ int orc_model_nstates(int model)
     case ORC_CRN: return CR_NS;
     case ORC_PYS: return PY_NS;
     case ORC_STW: return SW_NS;
+    case ORC_GRD: return GR_NS;
     default: return -1;
     }
 }
@@ -1237,6 +1542,9 @@ int orc_model_init(int model, int tissue, double* c, double* y)
     case ORC_STW:
         if (tissue != ORC_MYOCYTE) return -1;
         stw_init(c, y); return 0;
+    case ORC_GRD:
+        if (tissue != ORC_MYOCYTE) return -1;
+        grd_init(c, y); return 0;
     default: return -1;
     }
 }
@@ -1268,6 +1576,11 @@ int orc_model_const_index(int model, const char* name)
         if (!strcmp(name, "g_Kr")) return SC_gKr;
         if (!strcmp(name, "g_CaL")) return SC_gCaL;
         if (!strcmp(name, "g_Ks")) return SC_gKs;
+    } else if (model == ORC_GRD) {
+        if (!strcmp(name, "g_Na")) return GC_gNa;
+        if (!strcmp(name, "g_Kr")) return GC_gKr;
+        if (!strcmp(name, "g_CaL")) return GC_pCa;
+        if (!strcmp(name, "g_Ks")) return GC_gKs;
     } else if (model == ORC_BO) {
         if (!strcmp(name, "tauFi")) return BO_tfi;
         if (!strcmp(name, "tauSi")) return BO_tsi;
@@ -1327,6 +1640,7 @@ void orc_model_eval(int model, int tissue, double t, const double* c, const doub
     case ORC_CRN: crn_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_PYS: pys_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_STW: stw_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_GRD: grd_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     default: break;
     }
 }

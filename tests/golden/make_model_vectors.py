@@ -51,6 +51,7 @@ MODELS = {
     "PerisYague_2022": ("PerisYague/PerisYague_2022.H", "PerisYague_2022", "computeVariables",
                         (4,), 6),
     "Stewart": ("Stewart/Stewart_2009.H", "Stewart", "computeVariables", (4,), 7),
+    "Grandi": ("Grandi/Grandi_2011.H", "Grandi", "computeVariables", (4,), 8),
 }
 
 # ORd-2011 INaK and SERCA constants left unassigned by ORdinitConsts (published values)
@@ -392,7 +393,7 @@ def input_states(key, tissue, n=24, seed=0):
     import oracle as orc
     m = MODELS[key][4]
     c, y = orc.model_init(m, tissue)
-    amp = {0: 0.4, 1: 52.0, 2: 80.0, 3: 0.5, 5: 60.0, 6: 40.0, 7: 40.0}[m]
+    amp = {0: 0.4, 1: 52.0, 2: 80.0, 3: 0.5, 5: 60.0, 6: 40.0, 7: 40.0, 8: 40.0}[m]
     st = orc.make_stim(start=2.0, period_s1=1000.0, duration=1.0, amplitude=amp, nstim1=1)
     dt = 0.01 * (1.0 / 12.9 if m == 3 else 1.0)
     yy = np.zeros(64)

@@ -12,6 +12,7 @@ MODELS = {  # name -> (oracle id, tissue flag, library key, tissue word, stimulu
     "CRN": (5, 4, "Courtemanche", "myocyte", 60.0),
     "PYS": (6, 4, "PerisYague_2022", "myocyte", 40.0),
     "STW": (7, 4, "Stewart", "myocyte", 40.0),
+    "GRD": (8, 4, "Grandi", "myocyte", 40.0),
 }
 
 

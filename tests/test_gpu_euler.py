@@ -60,7 +60,8 @@ def test_restitution_single_cell_euler(gpu, orc, tmp_path):
     np.testing.assert_allclose(got, y, rtol=1e-9, atol=1e-12)
 
 
-@pytest.mark.parametrize("name", ["BO_epi", "TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW"])
+@pytest.mark.parametrize("name", ["BO_epi", "TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW",
+                                  "GRD"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_ionic_tissue_step_euler(gpu, orc, name, precision):
     """cg_ionic_step with Euler over 20 steps from states along an action potential; the

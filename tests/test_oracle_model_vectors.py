@@ -4,7 +4,7 @@ models' headers: initConsts and the function each model's derivatives() calls, a
 states along an action potential).
 
 Pins every model the oracle restates -- BuenoOrovio, TNNP (3 tissues), ORd, AlievPanfilov,
-Courtemanche, PerisYague_2022 and Stewart -- to the reference's text: the initial states
+Courtemanche, PerisYague_2022, Stewart and Grandi -- to the reference's text: the initial states
 exactly, and RATES in tissue mode (V frozen) and in single-cell mode (dV/dt = -Iion) to
 1e-13 of each rate's range over the samples (the same libm; the two differ only in the
 order of a few operations).  ORd carries the documented substitution of the published
@@ -19,7 +19,7 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURE = os.path.join(HERE, "golden", "model_vectors.json")
 IDS = {"BuenoOrovio": 0, "TNNP": 1, "ORd": 2, "AlievPanfilov": 3, "Courtemanche": 5,
-       "PerisYague_2022": 6, "Stewart": 7}
+       "PerisYague_2022": 6, "Stewart": 7, "Grandi": 8}
 TOL = 1e-13
 
 
@@ -79,7 +79,8 @@ def test_oracle_against_reference_expressions(orc, key, tissue):
     assert np.all(np.array(v["rates_tissue"])[:, 0] == 0.0)
 
 
-@pytest.mark.parametrize("key", ["TNNP", "ORd", "Courtemanche", "PerisYague_2022", "Stewart"])
+@pytest.mark.parametrize("key", ["TNNP", "ORd", "Courtemanche", "PerisYague_2022", "Stewart",
+                                 "Grandi"])
 def test_vectors_see_a_constant_off_by_one_part_in_a_million(orc, key):
     """Planted in the oracle's constants on the CPU: each constant that some rate depends
     on at these states, scaled by 1 + 1e-6, moves a rate beyond the pin's tolerance."""

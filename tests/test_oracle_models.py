@@ -13,6 +13,9 @@ PerisYague_2022: the charge balance with chloride, its guard values and its ungu
 quotients at their singular voltages.  Its gate kinetics have no published trace to pin
 against (the reference's h-gate beta differs from Courtemanche's): parity unpinned there
 beyond the device-vs-oracle agreement.
+Grandi: the Na+/Ca2+ charge balance over its three compartments and their buffers (K_i
+and Cl_i are fixed in that model), and first-order convergence of the Rush-Larsen split
+to the adaptive RKF45 on one paced beat.
 AlievPanfilov: the cubic's roots 0, a, 1 and the recovery nullcline.
 Stimulus: stimulusIO.C:238-286 cases, including the S2 placement at the end of S1.
 """
@@ -145,7 +148,7 @@ def test_aliev_panfilov_cubic(orc):
 
 def test_tissue_support_matches_reference(orc):
     # BuenoOrovio.C:92-96, TNNP.C:90-94, ORd.C:93-96, AlievPanfilov.C:92-95,
-    # Courtemanche.C:92-95, PerisYague.C:92-95, Stewart.C:93-96
+    # Courtemanche.C:92-95, PerisYague.C:92-95, Stewart.C:93-96, Grandi.C:93-96
     for tissue in (1, 2, 3):
         orc.model_init(orc.BO, tissue)
         orc.model_init(orc.TNNP, tissue)
@@ -157,10 +160,13 @@ def test_tissue_support_matches_reference(orc):
             orc.model_init(orc.PYS, tissue)
         with pytest.raises(ValueError):
             orc.model_init(orc.STW, tissue)
+        with pytest.raises(ValueError):
+            orc.model_init(orc.GRD, tissue)
     orc.model_init(orc.ORD, 4)
     orc.model_init(orc.CRN, 4)
     orc.model_init(orc.PYS, 4)
     orc.model_init(orc.STW, 4)
+    orc.model_init(orc.GRD, 4)
     orc.model_init(orc.AP, 4)
     with pytest.raises(ValueError):
         orc.model_init(orc.BO, 4)
@@ -369,3 +375,72 @@ def test_stw_ical_ghk_limit(orc):
     assert np.isnan(iion(15.0))
     left, right = 2 * iion(15.0 - 1e-4) - iion(15.0 - 2e-4), 2 * iion(15.0 + 1e-4) - iion(15.0 + 2e-4)
     assert left == pytest.approx(right, rel=1e-7)
+
+
+# Grandi oracle constant slots (the GC_* order of oracle/models.c): conductances and
+# permeabilities that carry K+ or Cl-, INaK (which moves K+), and the compartment volumes
+GRD_K_CL_PATHS = (1, 3, 4, 5, 6, 9, 10, 11, 16, 18)   # gKr gKs gK1 gto gKur gClB GClCa IbarNaK pK gKp
+GRD_VJN, GRD_VSL, GRD_VMYO, GRD_VSR = 20, 21, 22, 23
+
+
+def test_grd_charge_conservation(orc):
+    """Grandi's K_i and Cl_i are fixed (K_i's rate is 0), so the charge balance holds for
+    the Na+ and Ca2+ currents: with the K+ and Cl- paths and INaK off, F times the
+    volume-weighted change of total Na (free + NaB, three compartments) and twice that of
+    total Ca (free + the buffers each compartment's balance subtracts: SLL/SLH at the
+    membrane, TnC/CaM/Myo/SRB in the cytosol, Csqn in the SR) equals -Cm Iion.  Internal
+    fluxes (SR uptake, leak, release, diffusion between compartments) must cancel."""
+    F, Cm = 96485.0, 1.1e-10
+    c0, states = _states_along_ap(orc, orc.GRD, orc.MYOCYTE, 40.0)
+    c = c0.copy()
+    c[list(GRD_K_CL_PATHS)] = 0.0
+    Vjn, Vsl, Vmyo, Vsr = c[GRD_VJN], c[GRD_VSL], c[GRD_VMYO], c[GRD_VSR]
+    (Cai, Cajn, Casl, Casr, Csqn, Nai, Najn, Nasl, CaM, Myoc, Myom, SLHjn, SLHsl, SLLjn,
+     SLLsl, SRB, TnCHc, TnCHm, TnCL, NaBjn, NaBsl) = (1, 2, 3, 4, 5, 25, 26, 27, 28, 29, 30,
+                                                      31, 32, 33, 34, 35, 36, 37, 38, 39, 40)
+    st = orc.make_stim(start=0.0, period_s1=1000.0, duration=2.0, amplitude=40.0, nstim1=1)
+    seen = 0.0
+    for y in states:
+        for t, stim in ((500.0, None), (1.0, st)):
+            r = orc.model_eval(orc.GRD, orc.MYOCYTE, t, c, y, solve_vm=True, stim=stim)
+            d = r["rates"]
+            na = Vjn * (d[Najn] + d[NaBjn]) + Vsl * (d[Nasl] + d[NaBsl]) + Vmyo * d[Nai]
+            ca = (Vjn * (d[Cajn] + d[SLLjn] + d[SLHjn]) + Vsl * (d[Casl] + d[SLLsl] + d[SLHsl]) +
+                  Vmyo * (d[Cai] + d[TnCL] + d[TnCHc] + d[TnCHm] + d[CaM] + d[Myoc] + d[Myom] +
+                          d[SRB]) + Vsr * (d[Casr] + d[Csqn]))
+            assert F * (na + 2 * ca) == pytest.approx(-Cm * r["iion"], rel=1e-8,
+                                                      abs=1e-9 * Cm * abs(r["iion"]) + 1e-30)
+            assert d[0] == pytest.approx(-(r["iion"] + r["istim"]), rel=1e-14)
+            seen = max(seen, abs(r["iion"]))
+    assert seen > 1.0          # the balance was tested with currents flowing
+    # with every path on, K_i stays fixed
+    r = orc.model_eval(orc.GRD, orc.MYOCYTE, 500.0, c0, states[3], solve_vm=True)
+    assert r["rates"][21] == 0.0
+
+
+def test_grd_rush_larsen_converges_to_rkf45(orc):
+    """The Rush-Larsen split (gates, linear buffer kinetics, and SR/junctional/
+    subsarcolemmal Ca linearised in themselves, the other states frozen over a step) is a
+    first-order discretisation of the same right-hand side: one paced atrial beat's APD90
+    approaches the adaptive RKF45's (rest -73.4 mV, APD90 ~ 327 ms) with an error
+    proportional to dt.  The error constant is large -- the fast Ca-buffer exchanges
+    (Csqn with SR Ca, rate ~115 /ms; SLL/SLH with junctional Ca) are split -- so at 0.01 ms
+    APD90 is ~40 ms long; the reference itself integrates this model with its adaptive ODE
+    solvers, which the device runs as well.  The split stays stable at 0.02 ms."""
+    st = orc.make_stim(start=10.0, period_s1=1000.0, duration=1.0, amplitude=40.0, nstim1=1)
+
+    def apd(integ, dt, **kw):
+        t, v, y = orc.single_cell_run(orc.GRD, orc.MYOCYTE, st, dt, int(0.6 / dt),
+                                      integrator=integ, **kw)
+        b = orc.detect_beats(t, v)
+        assert len(b) == 1 and np.all(np.isfinite(y))
+        return (b[0]["t_repol90"] - b[0]["t_up"]) * 1e3, b[0]["v_peak"]
+
+    a_rk, p_rk = apd(1, 1e-5, abs_tol=1e-10, rel_tol=1e-8)
+    assert 300.0 < a_rk < 360.0 and p_rk > 0.0
+    errs = [apd(0, dt)[0] - a_rk for dt in (1e-5, 2e-6, 5e-7)]
+    assert all(e > 0 for e in errs)
+    assert 3.5 < errs[0] / errs[1] < 6.5          # dt ratio 5
+    assert 3.0 < errs[1] / errs[2] < 5.0          # dt ratio 4
+    assert errs[2] < 3.0
+    apd(0, 2e-5)                                  # stable at the tissue step
