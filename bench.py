@@ -38,9 +38,10 @@ WORKLOADS = {
 
 # algorithmic bytes per cell (row) of one launch, by kernel and precision
 NSTATES = {"ORd": 41, "BuenoOrovio": 4, "TNNP": 17, "AlievPanfilov": 2, "Courtemanche": 21,
-           "PerisYague_2022": 22, "Stewart": 20, "Grandi": 41}
-# the ionic models run on the workload's mesh beside its own model
-OTHER_MODELS = ("Courtemanche", "PerisYague_2022", "Stewart", "Grandi")
+           "PerisYague_2022": 22, "Stewart": 20, "Grandi": 41, "ToRORd_dynCl": 45}
+# the ionic models run on the workload's mesh beside its own model, with their tissue word
+OTHER_MODELS = ("Courtemanche", "PerisYague_2022", "Stewart", "Grandi", "ToRORd_dynCl")
+MODEL_TISSUE = {"ToRORd_dynCl": "endocardialCells"}
 
 
 def algo_bytes_per_cell(kernel, prec, model, algorithm, od_planes=1):
@@ -609,7 +610,7 @@ def run_model_leg(cf, torch, w, sigma, model, steps=100, warmup=100, prof_steps=
     """The workload's implicit tissue step with another ionic model (same mesh, fibres,
     stimulus and solver): FP64 and mixed throughput, the mixed error gate, and the ionic
     kernel's HBM rate at its algorithmic bytes."""
-    wm = dict(w, model=model, tissue="myocyte")
+    wm = dict(w, model=model, tissue=MODEL_TISSUE.get(model, "myocyte"))
     n = int(np.prod(wm["shape"]))
     out = {"model": model, "steps": steps}
     res = {}

@@ -272,6 +272,7 @@ template <class T> struct BatchLaunch {
         case MODEL_PYS: return BatchLaunch<T>::variant<PerisYague>(variant, tissue, states, P, gscale, a, s); \
         case MODEL_STW: return BatchLaunch<T>::variant<Stewart>(variant, tissue, states, P, gscale, a, s); \
         case MODEL_GRD: return BatchLaunch<T>::variant<Grandi>(variant, tissue, states, P, gscale, a, s); \
+        case MODEL_TOR: return BatchLaunch<T>::variant<ToRORd>(variant, tissue, states, P, gscale, a, s); \
         default: return hipErrorInvalidValue;                                              \
         }                                                                                  \
     }

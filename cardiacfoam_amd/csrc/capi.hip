@@ -40,10 +40,11 @@ int fail(int code, const char* fmt, ...)
             return fail(CG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));           \
     } while (0)
 
-constexpr int kNumModels = 9;
+constexpr int kNumModels = 10;
 const char* const kModelKeys[kNumModels] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
                                             "monodomainFDAManufactured", "Courtemanche",
-                                            "PerisYague_2022", "Stewart", "Grandi"};
+                                            "PerisYague_2022", "Stewart", "Grandi",
+                                            "ToRORd_dynCl"};
 
 // state names, order of the *_STATES_NAMES tables / Names.H enums
 const char* const kBONames[4] = {"u", "v", "w", "s"};
@@ -83,6 +84,13 @@ const char* const kGRDNames[41] = {
     "ryr_ryr_r", "Na_i", "Na_jn", "Na_sl", "buffCa_CaM", "buffCa_Myoc", "buffCa_Myom",
     "buffCa_SLH_jn", "buffCa_SLH_sl", "buffCa_SLL_jn", "buffCa_SLL_sl", "buffCa_SRB",
     "buffCa_TnCHc", "buffCa_TnCHm", "buffCa_TnCL", "buffNa_NaB_jn", "buffNa_NaB_sl"};
+const char* const kTORNames[45] = {
+    "V", "CaMKt", "Nai", "Nass", "Ki", "Kss", "Cass", "Cansr", "Cajsr", "Cai", "Cli", "Clss",
+    "INa_m", "INa_h", "INa_j", "INa_hp", "INa_jp", "INaL_mL", "INaL_hL", "INaL_hLp", "Ito_a",
+    "Ito_iF", "Ito_iS", "Ito_ap", "Ito_iFp", "Ito_iSp", "ICaL_d", "ICaL_ff", "ICaL_fs",
+    "ICaL_fcaf", "ICaL_fcas", "ICaL_jca", "ICaL_ffp", "ICaL_fcafp", "ICaL_nca_ss",
+    "ICaL_nca_i", "IKr_C1", "IKr_C2", "IKr_C3", "IKr_I", "IKr_O", "IKs_xs1", "IKs_xs2",
+    "Jrel_np", "Jrel_p"};
 
 struct ModelInfo {
     int ns, np;
@@ -121,6 +129,9 @@ bool model_info(int model, ModelInfo* mi)
     case cg::MODEL_GRD:
         *mi = {cg::Grandi::NS, cg::Grandi::NP, 1.0, 0.0, 1.0, 1000.0};
         return true;
+    case cg::MODEL_TOR:
+        *mi = {cg::ToRORd::NS, cg::ToRORd::NP, 1.0, 0.0, 1.0, 1000.0};
+        return true;
     default: return false;
     }
 }
@@ -137,6 +148,7 @@ bool model_host_init(int model, int tissue, double* p, double* y)
     case cg::MODEL_PYS: return cg::PerisYague::host_init(tissue, p, y);
     case cg::MODEL_STW: return cg::Stewart::host_init(tissue, p, y);
     case cg::MODEL_GRD: return cg::Grandi::host_init(tissue, p, y);
+    case cg::MODEL_TOR: return cg::ToRORd::host_init(tissue, p, y);
     default: return false;
     }
 }
@@ -153,6 +165,7 @@ double state_ref(int model, int q)
     case cg::MODEL_PYS: r = cg::StateRef<cg::PerisYague>::at(q); break;
     case cg::MODEL_STW: r = cg::StateRef<cg::Stewart>::at(q); break;
     case cg::MODEL_GRD: r = cg::StateRef<cg::Grandi>::at(q); break;
+    case cg::MODEL_TOR: r = cg::StateRef<cg::ToRORd>::at(q); break;
     default: break;
     }
     return (double)(float)r;
@@ -385,7 +398,8 @@ int cg_model_state_name(int model, int i, char* buf, size_t len)
     const char* nm = model == 0 ? kBONames[i] : model == 1 ? kTNNPNames[i]
                    : model == 2 ? kORdNames[i] : model == 3 ? kAPNames[i]
                    : model == 4 ? kFDANames[i] : model == 5 ? kCRNNames[i]
-                   : model == 6 ? kPYSNames[i] : model == 7 ? kSTWNames[i] : kGRDNames[i];
+                   : model == 6 ? kPYSNames[i] : model == 7 ? kSTWNames[i]
+                   : model == 8 ? kGRDNames[i] : kTORNames[i];
     snprintf(buf, len, "%s", nm);
     return CG_OK;
 }

@@ -17,11 +17,13 @@
 #include "model_pys.hpp"
 #include "model_stw.hpp"
 #include "model_grd.hpp"
+#include "model_tor.hpp"
 
 namespace cg {
 
 enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3, MODEL_FDA = 4,
-               MODEL_CRN = 5, MODEL_PYS = 6, MODEL_STW = 7, MODEL_GRD = 8 };
+               MODEL_CRN = 5, MODEL_PYS = 6, MODEL_STW = 7, MODEL_GRD = 8,
+               MODEL_TOR = 9 };
 
 constexpr int MAX_PARAMS = 48;
 struct ModelParams {
@@ -93,6 +95,17 @@ template <> struct StateRef<Grandi> {
              : q == Grandi::Ki ? 120.0 : q == Grandi::Casr ? 0.50230583
              : q == Grandi::Csqn ? 1.1333754 : q == Grandi::NaBjn ? 3.6139606
              : q == Grandi::NaBsl ? 0.78860779 : 0.0;
+    }
+};
+
+template <> struct StateRef<ToRORd> {
+    static constexpr bool any = true;
+    CG_HD static constexpr double at(int q)
+    {
+        return (q == ToRORd::Nai || q == ToRORd::Nass) ? 13.4
+             : (q == ToRORd::Ki || q == ToRORd::Kss) ? 152.4
+             : (q == ToRORd::Cli || q == ToRORd::Clss) ? 34.3
+             : (q == ToRORd::Cansr || q == ToRORd::Cajsr) ? 1.8 : 0.0;
     }
 };
 
@@ -490,6 +503,57 @@ template <> struct Ops<Grandi> {
                              T istim, const Grandi::VTerms<T>& q)
     {
         Grandi::deriv(y, d, p, gs, solve_v, istim, q);
+    }
+};
+
+template <> struct Ops<ToRORd> {
+    template <class T>
+    CG_HD static void cell_step_off(T* y, T* s, T dt, T istim, const T* p, int tissue,
+                                    const T* gs)
+    {
+        const auto q = ToRORd::vterms(y[0], p);
+        cell_step_offsets<ToRORd>(y, s, dt, [&](RlStep<T>& out) {
+            ToRORd::rates(y, out, p, tissue, gs, true, istim, q);
+        });
+    }
+    template <class T>
+    CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int tissue, const T* gs)
+    {
+        y[0] = v;
+        const auto q = ToRORd::vterms(v, p);
+        return tissue_step_offsets<ToRORd>(
+            y, s, dt,
+            [&](RlStep<T>& out) { ToRORd::rates(y, out, p, tissue, gs, false, T(0), q); },
+            [&]() { return ToRORd::currents(v, y, p, q, gs).total(); });
+    }
+    template <class T>
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int tissue, const T* gs)
+    {
+        y[0] = v;
+        const auto q = ToRORd::vterms(v, p);
+        ToRORd::step(y, dt, p, tissue, gs, false, T(0), q);
+        return ToRORd::currents(v, y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int tissue, const T* gs)
+    {
+        const auto q = ToRORd::vterms(y[0], p);
+        ToRORd::step(y, dt, p, tissue, gs, true, istim, q);
+    }
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T* gs)
+    {
+        const auto q = ToRORd::vterms(y[0], p);
+        return ToRORd::currents(y[0], y, p, q, gs).total();
+    }
+    template <class T> CG_HD static ToRORd::VTerms<T> vt(T v, const T* p)
+    {
+        return ToRORd::vterms(v, p);
+    }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int tissue, const T* gs, bool solve_v,
+                             T istim, const ToRORd::VTerms<T>& q)
+    {
+        ToRORd::deriv(y, d, p, tissue, gs, solve_v, istim, q);
     }
 };
 

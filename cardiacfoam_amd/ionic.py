@@ -15,7 +15,7 @@ from . import _capi
 from ._capi import check, lib
 
 MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov", "monodomainFDAManufactured",
-              "Courtemanche", "PerisYague_2022", "Stewart", "Grandi"]
+              "Courtemanche", "PerisYague_2022", "Stewart", "Grandi", "ToRORd_dynCl"]
 SUPPORTED_TISSUES = {
     "BuenoOrovio": ["endocardialCells", "mCells", "epicardialCells"],
     "TNNP": ["endocardialCells", "mCells", "epicardialCells"],
@@ -25,6 +25,7 @@ SUPPORTED_TISSUES = {
     "PerisYague_2022": ["myocyte"],  # PerisYague.C:92-95
     "Stewart": ["myocyte"],  # Stewart.C:93-96
     "Grandi": ["myocyte"],  # Grandi.C:93-96
+    "ToRORd_dynCl": ["endocardialCells", "mCells", "epicardialCells"],  # ToRORd_dynCl.C:93-96
     # the manufactured model's supportedDimensions() (monodomainFDAManufactured.C:85-88)
     "monodomainFDAManufactured": ["1D", "2D", "3D"],
 }

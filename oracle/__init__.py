@@ -14,14 +14,16 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
 _lib = None
 
-BO, TNNP, ORD, AP, FDA, CRN, PYS, STW, GRD = 0, 1, 2, 3, 4, 5, 6, 7, 8
+BO, TNNP, ORD, AP, FDA, CRN, PYS, STW, GRD, TOR = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
 EPI, MCELL, ENDO, MYOCYTE = 1, 2, 3, 4
 DIM1, DIM2, DIM3 = 11, 12, 13          # dimension flags of the manufactured model
-MAX_STATES, MAX_CONSTS = 41, 160
-NSTATES = {BO: 4, TNNP: 17, ORD: 41, AP: 2, FDA: 4, CRN: 21, PYS: 22, STW: 20, GRD: 41}
+MAX_STATES, MAX_CONSTS = 48, 160
+NSTATES = {BO: 4, TNNP: 17, ORD: 41, AP: 2, FDA: 4, CRN: 21, PYS: 22, STW: 20, GRD: 41,
+           TOR: 45}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
@@ -134,6 +136,10 @@ def lib():
         L.orc_tissue_step.argtypes = [C.POINTER(Tissue), C.POINTER(Boxes)]
         L.orc_detect_beats.argtypes = [_dp, _dp, C.c_long, C.POINTER(Beat), C.c_int, C.c_double,
                                        C.c_double, C.c_double]
+        L.orc_bo_tangent.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int,
+                                     C.c_double, C.c_double, C.c_double, C.POINTER(Boxes), _dp,
+                                     _dp, C.c_int, _dp, _dp, C.c_long, C.c_int, _lp, _dp,
+                                     C.c_int, C.POINTER(C.c_double), _dp, _dp]
         _lib = L
     return _lib
 
@@ -368,6 +374,38 @@ def make_boxes(box_min, box_max, t_start, duration, intensity, centres=None):
     b = Boxes(len(arrs[2]), *ptrs)
     b._keep = arrs
     return b
+
+
+def bo_tangent(shape, h, sig0, tissue, dt, chi, cm, boxes, kappa, theta, dkappa, dtheta,
+               n_steps, probes, target, nthreads=8):
+    """Loss and its directional derivatives along each row of (dkappa [K, n], dtheta
+    [K, 2]) from the tangent-linear BuenoOrovio tissue run (orc_bo_tangent), eight
+    directions per pass.  Returns (loss, dloss [K], Vm after the last step)."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    dk = np.ascontiguousarray(np.reshape(dkappa, (-1, n)), dtype=np.float64)
+    dth = np.ascontiguousarray(np.reshape(dtheta, (-1, 2)), dtype=np.float64)
+    assert dk.shape[0] == dth.shape[0]
+    args = (np.ascontiguousarray(kappa, dtype=np.float64),
+            np.ascontiguousarray(theta, dtype=np.float64))
+    pr = np.ascontiguousarray(probes, dtype=np.int64)
+    tg = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+    assert tg.size == n_steps * pr.size
+    s0 = np.ascontiguousarray(sig0, dtype=np.float64)
+    out, loss, v = [], None, np.zeros(n)
+    for a in range(0, max(dk.shape[0], 1), 8):
+        kk = dk[a:a + 8]
+        L, d = C.c_double(), np.zeros(8)
+        rc = lib().orc_bo_tangent(nx, ny, nz, h, s0, tissue, dt, chi, cm, C.byref(boxes),
+                                  args[0], args[1], kk.shape[0],
+                                  np.ascontiguousarray(kk if kk.size else np.zeros((1, n))),
+                                  np.ascontiguousarray(dth[a:a + 8] if kk.size else np.zeros((1, 2))),
+                                  int(n_steps), pr.size, pr, tg, nthreads, C.byref(L), d, v)
+        if rc != 0:
+            raise ValueError("orc_bo_tangent failed (%d)" % rc)
+        loss = L.value
+        out.append(d[:kk.shape[0]])
+    return loss, np.concatenate(out) if out else np.zeros(0), v
 
 
 class OracleTissue:

@@ -47,11 +47,9 @@ double orc_stimulus(double t, const orc_stim_t* s)
 }
 
 /* =============================================================== BuenoOrovio
- * BuenoOrovio_2008.H:116-225; constant order follows BuenoOrovio_2008Names.H. */
-enum { BO_uo, BO_uu, BO_thv, BO_thw, BO_thvm, BO_tho, BO_tv1m, BO_tv2m, BO_tvp,
-       BO_tw1m, BO_tw2m, BO_kwm, BO_uwm, BO_twp, BO_tfi, BO_to1, BO_to2, BO_tso1,
-       BO_tso2, BO_kso, BO_uso, BO_ts1, BO_ts2, BO_ks, BO_us, BO_tsi, BO_twinf,
-       BO_winfstar, BO_NC };
+ * BuenoOrovio_2008.H:116-225; constant order (bo_consts.h) follows
+ * BuenoOrovio_2008Names.H. */
+#include "bo_consts.h"
 
 static void bo_init(int tf, double* c, double* y)
 {
@@ -1497,6 +1495,389 @@ static void grd_eval(double t, const double* c, const double* y, int solve_vm,
     *istim = Ist;
 }
 
+/* ============================================================ ToRORd_dynCl
+ * ToR-ORd with dynamic chloride (Tomek et al. 2019, 2023 revision), 45 states:
+ * constants and initial states of ToRORd_dynCl_2023.H:571-846 (ToRORd_dynClinitConsts,
+ * three tissue columns), right-hand side of :848-1177, state order of its STATES_INDEX
+ * enum.  Tissues endocardialCells, mCells, epicardialCells (ToRORd_dynCl.C:93-96).  V in
+ * mV, Im = Iion_cm.  The K_i rate omits the stimulus, as the reference does inside
+ * derivatives() (ALGEBRAIC[Istim] is read before it is assigned, :1119 vs :1172, on a
+ * zeroed scratch).  I_katp is 0 (fkatp = 0 in initConsts).
+ * Rush-Larsen classification: the Hodgkin-Huxley gates and the RyR release fluxes as
+ * gates; nca_i, nca_ss and the five IKr Markov states as linear in their own state with
+ * the others frozen over the step; the rest forward Euler.
+ * Parity pins: the reference's own expressions at seeded states
+ * (tests/test_oracle_model_vectors.py). */
+enum { TR_V, TR_CaMKt, TR_Nai, TR_Nass, TR_Ki, TR_Kss, TR_Cass, TR_Cansr, TR_Cajsr, TR_Cai,
+       TR_Cli, TR_Clss, TR_m, TR_h, TR_j, TR_hp, TR_jp, TR_mL, TR_hL, TR_hLp, TR_a, TR_iF,
+       TR_iS, TR_ap, TR_iFp, TR_iSp, TR_d, TR_ff, TR_fs, TR_fcaf, TR_fcas, TR_jca, TR_ffp,
+       TR_fcafp, TR_nca_ss, TR_nca_i, TR_C1, TR_C2, TR_C3, TR_I, TR_O, TR_xs1, TR_xs2,
+       TR_Jrelnp, TR_Jrelp, TR_NS };
+/* tissue-dependent and rescalable constants (the tissue flag itself, for delta_epi of Ito
+ * and the M-cell release factor, comes with each evaluation) */
+enum { RC_GNa, RC_GKr, RC_PCa, RC_GKs, RC_GK1, RC_GKb, RC_GNaL, RC_Gto, RC_Gncx, RC_Pnak,
+       RC_cmdnmax, RC_upScale, RC_GpCa, RC_GClCa, RC_GClb, RC_PCab, RC_PNab, RC_Jrel_b,
+       RC_Jup_b, RC_bt, RC_gkatp, RC_fkatp, RC_NC };
+
+static void tor_init(int tf, double* c, double* y)
+{
+    /* initConsts' tissue selections (:618-769): flag 1 epi, 2 M, 3 endo */
+    const int epi = tf == 1, mid = tf == 2;
+    const double PCa_b = 8.37570000000000046e-05;
+    c[RC_GNa] = 11.7802;
+    c[RC_GKr] = epi ? 0.0321 * 1.3 : mid ? 0.0321 * 0.8 : 0.0321;
+    c[RC_PCa] = epi ? PCa_b * 1.2 : mid ? PCa_b * 2.0 : PCa_b;
+    c[RC_GKs] = epi ? 0.0011 * 1.4 : 0.0011;
+    c[RC_GK1] = epi ? 0.6992 * 1.2 : mid ? 0.6992 * 1.3 : 0.6992;
+    c[RC_GKb] = epi ? 0.0189 * 0.6 : 0.0189;
+    c[RC_GNaL] = epi ? 0.0279 * 0.6 : 0.0279;
+    c[RC_Gto] = (epi || mid) ? 0.16 * 2.0 : 0.16;
+    c[RC_Gncx] = epi ? 0.0034 * 1.1 : mid ? 0.0034 * 1.4 : 0.0034;
+    c[RC_Pnak] = epi ? 15.4509 * 0.9 : mid ? 15.4509 * 0.7 : 15.4509;
+    c[RC_cmdnmax] = epi ? 0.05 * 1.3 : 0.05;
+    c[RC_upScale] = epi ? 1.3 : 1.0;
+    c[RC_GpCa] = 0.0005;
+    c[RC_GClCa] = 0.2843;
+    c[RC_GClb] = 0.00198;
+    c[RC_PCab] = 5.91940000000000005e-08;
+    c[RC_PNab] = 1.9239e-09;
+    c[RC_Jrel_b] = 1.5378;
+    c[RC_Jup_b] = 1.0;
+    c[RC_bt] = 4.75;
+    c[RC_gkatp] = 4.3195;
+    c[RC_fkatp] = 0.0;
+    /* initial states (:782-846), columns endo / M / epi */
+    static const double y0[3][TR_NS] = {
+        {-89.74808, 1.10e-02, 12.39736, 12.3977, 147.7115, 147.7114, 6.50e-05, 1.528001,
+         1.525693, 7.45e-05, 29.20698, 29.20696, 6.52e-04, 0.8473267, 0.8471657, 0.7018454,
+         0.8469014, 1.35e-04, 0.5566017, 0.3115491, 8.90e-04, 0.9996716, 0.5988908,
+         4.53e-04, 0.9996716, 0.6620692, 1.59e-31, 1.0, 0.9401791, 1.0, 0.9999014,
+         0.9999846, 1.0, 1.0, 4.90e-04, 8.33e-04, 0.9982511, 7.94e-04, 6.53e-04, 9.80e-06,
+         2.92e-04, 0.243959, 1.59e-04, 1.81e-22, 4.36e-21},
+        {-91.33918, 2.01882e-02, 15.94867, 15.94922, 156.7131, 156.7130, 6.64e-05, 2.012225,
+         2.016415, 8.30e-05, 48.91277, 48.91274, 4.62e-04, 0.8739077, 0.8737841, 0.7478972,
+         0.8735375, 9.99e-05, 0.5986118, 0.3339899, 7.99e-04, 0.9997514, 0.5702538,
+         4.07e-04, 0.9997514, 0.6351927, -8.33e-30, 1.0, 0.9183587, 1.0, 0.9997540,
+         0.9999743, 1.0, 1.0, 5.34e-04, 1.26e-03, 0.9983451, 7.09e-04, 5.91e-04, 1.06e-05,
+         3.45e-04, 0.2642293, 1.33e-04, -1.30e-21, -7.61e-20},
+        {-90.74563, 1.27e-02, 13.40062, 13.40094, 152.3639, 152.3638, 5.75e-05, 1.806794,
+         1.805047, 6.62e-05, 34.31721, 34.31719, 5.25e-04, 0.8645148, 0.8644571, 0.7313656,
+         0.8643527, 1.12e-04, 0.5916536, 0.3476812, 8.32e-04, 0.9997242, 0.9997235,
+         4.24e-04, 0.9997242, 0.9997241, -2.49e-36, 1.0, 0.9510602, 1.0, 0.9999377,
+         0.9999886, 1.0, 1.0, 3.05e-04, 5.27e-04, 0.9984733, 7.39e-04, 6.03e-04, 5.68e-06,
+         1.79e-04, 0.2233584, 1.42e-04, 6.78e-25, -1.58e-23}};
+    memcpy(y, y0[tf == 3 ? 0 : mid ? 1 : 2], sizeof y0[0]);
+}
+
+static void tor_eval(int tf, double t, const double* c, const double* y, int solve_vm,
+                     const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                     double* iion, double* istim)
+{
+    /* fixed constants of initConsts (:573-770) */
+    const double L = 0.01, rad = 0.0011;
+    const double Ageo = 2.0 * 3.14 * rad * rad + 2.0 * 3.14 * rad * L;
+    const double vcell = 1000.0 * 3.14 * rad * rad * L;
+    const double Acap = 2.0 * Ageo;
+    const double vjsr = 0.0048 * vcell, vmyo = 0.68 * vcell, vnsr = 0.0552 * vcell,
+                 vss = 0.02 * vcell;
+    const double cao = 1.8, clo = 150.0, ko = 5.0, nao = 140.0;
+    const double F = 96485.0, R = 8314.0, T = 310.0;
+    const double KmCaMK = 0.15, aCaMK = 0.05, bCaMK = 0.00068, CaMKo = 0.05, KmCaM = 0.0015;
+
+    const double V = y[TR_V], Cai = y[TR_Cai], Cass = y[TR_Cass], Nai = y[TR_Nai],
+                 Nass = y[TR_Nass], Ki = y[TR_Ki], Kss = y[TR_Kss], Cli = y[TR_Cli],
+                 Clss = y[TR_Clss];
+    for (int i = 0; i < TR_NS; ++i) g[i] = 0;
+#define TR_GATE(idx, inf, tau) do { const double i_ = (inf), t_ = (tau); \
+        r[idx] = (i_ - y[idx]) / t_; g[idx] = 1; xi[idx] = i_; ta[idx] = t_; } while (0)
+    /* dx/dt = a - b x written as the reference writes it (rate) */
+#define TR_LIN(idx, a, b, rate) do { const double a_ = (a), b_ = (b); \
+        r[idx] = (rate); g[idx] = 1; xi[idx] = a_ / b_; ta[idx] = 1.0 / b_; } while (0)
+
+    /* CaMK, IpCa, diffusion, transfer, SERCA (:850-877) */
+    const double CaMKb = CaMKo * (1.0 - y[TR_CaMKt]) / (1.0 + KmCaM / Cass);
+    const double CaMKa = CaMKb + y[TR_CaMKt];
+    r[TR_CaMKt] = aCaMK * CaMKb * (CaMKb + y[TR_CaMKt]) - bCaMK * y[TR_CaMKt];
+    const double fcamk = 1.0 / (1.0 + KmCaMK / CaMKa);
+    const double IpCa = c[RC_GpCa] * Cai / (0.0005 + Cai);
+    const double Jdiff = (Cass - Cai) / 0.2, JdiffCl = (Clss - Cli) / 2.0,
+                 JdiffK = (Kss - Ki) / 2.0, JdiffNa = (Nass - Nai) / 2.0;
+    const double Jtr = (y[TR_Cansr] - y[TR_Cajsr]) / 60.0;
+    const double Jleak = 0.0048825 * y[TR_Cansr] / 15.0;
+    const double Jupnp = c[RC_upScale] * 0.005425 * Cai / (Cai + 0.00092);
+    const double Jupp = c[RC_upScale] * 2.75 * 0.005425 * Cai / (Cai + 0.00092 - 0.00017);
+    const double Jup = c[RC_Jup_b] * ((1.0 - fcamk) * Jupnp + fcamk * Jupp - Jleak);
+
+    /* reversal potentials (:879-884); z_Cl = -1 */
+    const double RTF = R * T / F;
+    const double ECl = R * T / (-1.0 * F) * log(clo / Cli);
+    const double EClss = R * T / (-1.0 * F) * log(clo / Clss);
+    const double EK = RTF * log(ko / Ki);
+    const double ENa = RTF * log(nao / Nai);
+    const double EKs = RTF * log((ko + 0.01833 * nao) / (Ki + 0.01833 * Nai));
+
+    /* chloride (:886-891); Fjunc = 1 leaves the subsarcolemmal Ca-activated part at 0 */
+    const double IClCa_junc = 1.0 * c[RC_GClCa] / (1.0 + 0.1 / Cass) * (V - EClss);
+    const double IClCa_sl = (1.0 - 1.0) * c[RC_GClCa] / (1.0 + 0.1 / Cai) * (V - ECl);
+    const double IClb = c[RC_GClb] * (V - ECl);
+    const double IClCa = IClCa_junc + IClCa_sl;
+
+    /* IK1, IKb, IKs (:893-906) */
+    const double dvk = V - EK;
+    const double aK1 = 4.094 / (1.0 + exp(0.1217 * (dvk - 49.934)));
+    const double bK1 = (15.72 * exp(0.0674 * (dvk - 3.257)) + exp(0.0618 * (dvk - 594.31))) /
+                       (1.0 + exp(-0.1629 * (dvk + 14.207)));
+    const double IK1 = c[RC_GK1] * sqrt(ko / 5.0) * (aK1 / (aK1 + bK1)) * dvk;
+    const double IKb = c[RC_GKb] * (1.0 / (1.0 + exp(-(V - 10.8968) / 23.9871))) * dvk;
+    const double KsCa = 1.0 + 0.6 / (1.0 + pow(3.8e-05 / Cai, 1.4));
+    const double xs1ss = 1.0 / (1.0 + exp(-(V + 11.6) / 8.932));
+    TR_GATE(TR_xs1, xs1ss, 817.3 + 1.0 / (0.0002326 * exp((V + 48.28) / 17.8) +
+                                          0.001292 * exp(-(V + 210.0) / 230.0)));
+    TR_GATE(TR_xs2, xs1ss, 1.0 / (0.01 * exp((V - 50.0) / 20.0) + 0.0193 * exp(-(V + 66.54) / 31.0)));
+    const double IKs = c[RC_GKs] * KsCa * y[TR_xs1] * y[TR_xs2] * (V - EKs);
+
+    /* INa (:908-930) */
+    const double ah = V >= -40.0 ? 0.0 : 0.057 * exp(-(V + 80.0) / 6.8);
+    const double aj = V >= -40.0 ? 0.0
+        : (-25428.0 * exp(0.2444 * V) - 6.948e-06 * exp(-0.04391 * V)) * (V + 37.78) /
+              (1.0 + exp(0.311 * (V + 79.23)));
+    const double bh = V >= -40.0 ? 0.77 / (0.13 * (1.0 + exp(-(V + 10.66) / 11.1)))
+                                 : 2.7 * exp(0.079 * V) + 310000.0 * exp(0.3485 * V);
+    const double bj = V >= -40.0 ? 0.6 * exp(0.057 * V) / (1.0 + exp(-0.1 * (V + 32.0)))
+                                 : 0.02424 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)));
+    const double hss = 1.0 / pow(1.0 + exp((V + 71.55) / 7.43), 2.0);
+    const double hssp = 1.0 / pow(1.0 + exp((V + 77.55) / 7.43), 2.0);
+    const double mss = 1.0 / pow(1.0 + exp(-(V + 56.86) / 9.03), 2.0);
+    const double tm = 0.1292 * exp(-pow((V + 45.79) / 15.54, 2.0)) +
+                      0.06487 * exp(-pow((V - 4.823) / 51.12, 2.0));
+    const double INa = c[RC_GNa] * (V - ENa) * pow(y[TR_m], 3.0) *
+                       ((1.0 - fcamk) * y[TR_h] * y[TR_j] + fcamk * y[TR_hp] * y[TR_jp]);
+    const double th = 1.0 / (ah + bh), tj = 1.0 / (aj + bj);
+    TR_GATE(TR_m, mss, tm);
+    TR_GATE(TR_h, hss, th);
+    TR_GATE(TR_hp, hssp, th);
+    TR_GATE(TR_j, hss, tj);
+    TR_GATE(TR_jp, hss, 1.46 * tj);
+
+    /* INaL (:932-941) */
+    const double INaL = c[RC_GNaL] * (V - ENa) * y[TR_mL] *
+                        ((1.0 - fcamk) * y[TR_hL] + fcamk * y[TR_hLp]);
+    TR_GATE(TR_hL, 1.0 / (1.0 + exp((V + 87.61) / 7.488)), 200.0);
+    TR_GATE(TR_mL, 1.0 / (1.0 + exp(-(V + 42.85) / 5.264)), tm);
+    TR_GATE(TR_hLp, 1.0 / (1.0 + exp((V + 93.81) / 7.488)), 3.0 * 200.0);
+
+    /* I_katp (:943-944): akik = (ko/5)^0.24, bkik = 1/(1 + (2/0.25)^2) */
+    const double Ikatp = c[RC_fkatp] * c[RC_gkatp] * pow(ko / 5.0, 0.24) *
+                         (1.0 / (1.0 + pow(2.0 / 0.25, 2.0))) * dvk;
+
+    /* Ito (:946-973); EKshift = 0 */
+    const double AiF = 1.0 / (1.0 + exp((V - 213.6) / 151.2));
+    const double delta_epi = tf == 1 ? 1.0 - 0.95 / (1.0 + exp((V + 70.0) / 5.0)) : 1.0;
+    const double dti = (1.354 + 0.0001 / (exp((V - 167.4) / 15.89) + exp(-(V - 12.23) / 0.2154))) *
+                       (1.0 - 0.5 / (1.0 + exp((V + 70.0) / 20.0)));
+    const double iss = 1.0 / (1.0 + exp((V + 43.94) / 5.711));
+    const double tau_a = 1.0515 / (1.0 / (1.2089 * (1.0 + exp(-(V - 18.4099) / 29.3814))) +
+                                   3.5 / (1.0 + exp((V + 100.0) / 29.3814)));
+    const double tiF = (4.562 + 1.0 / (0.3933 * exp(-(V + 100.0) / 100.0) +
+                                       0.08004 * exp((V + 50.0) / 16.59))) * delta_epi;
+    const double tiS = (23.62 + 1.0 / (0.001416 * exp(-(V + 96.52) / 59.05) +
+                                       1.78e-08 * exp((V + 114.1) / 8.079))) * delta_epi;
+    const double AiS = 1.0 - AiF;
+    const double ii = AiF * y[TR_iF] + AiS * y[TR_iS];
+    const double ip = AiF * y[TR_iFp] + AiS * y[TR_iSp];
+    const double Ito = c[RC_Gto] * dvk * ((1.0 - fcamk) * y[TR_a] * ii + fcamk * y[TR_ap] * ip);
+    TR_GATE(TR_a, 1.0 / (1.0 + exp(-(V - 14.34) / 14.82)), tau_a);
+    TR_GATE(TR_ap, 1.0 / (1.0 + exp(-(V - 24.34) / 14.82)), tau_a);
+    TR_GATE(TR_iF, iss, tiF);
+    TR_GATE(TR_iS, iss, tiS);
+    TR_GATE(TR_iFp, iss, dti * tiF);
+    TR_GATE(TR_iSp, iss, dti * tiS);
+
+    /* ICaL gates and activity coefficients (:975-1008) */
+    const double Afcaf = 0.3 + 0.6 / (1.0 + exp((V - 10.0) / 10.0)), Afcas = 1.0 - Afcaf;
+    const double Ii = 0.5 * (Nai + Ki + Cli + 4.0 * Cai) / 1000.0;
+    const double Iss = 0.5 * (Nass + Kss + Clss + 4.0 * Cass) / 1000.0;
+    const double dss = V >= 31.4978 ? 1.0 : 1.0763 * exp(-1.007 * exp(-0.0829 * V));
+    const double fss = 1.0 / (1.0 + exp((V + 19.58) / 3.696));
+    const double km2n = y[TR_jca] * 1.0;
+    const double tfcaf = 7.0 + 1.0 / (0.04 * exp(-(V - 4.0) / 7.0) + 0.04 * exp((V - 4.0) / 7.0));
+    const double tff = 7.0 + 1.0 / (0.0045 * exp(-(V + 20.0) / 10.0) + 0.0045 * exp((V + 20.0) / 10.0));
+    const double k2n = 500.0, Kmn = 0.002, Aff = 0.6, Afs = 1.0 - Aff;
+    const double anca_i = 1.0 / (k2n / km2n + pow(1.0 + Kmn / Cai, 4.0));
+    const double anca_ss = 1.0 / (k2n / km2n + pow(1.0 + Kmn / Cass, 4.0));
+    const double f = Aff * y[TR_ff] + Afs * y[TR_fs];
+    const double fca = Afcaf * y[TR_fcaf] + Afcas * y[TR_fcas];
+    const double fcap = Afcaf * y[TR_fcafp] + Afcas * y[TR_fcas];
+    const double fp = Aff * y[TR_ffp] + Afs * y[TR_fs];
+    const double constA = 1820000.0 * pow(74.0 * T, -1.5);
+    const double Io = 0.5 * (nao + ko + clo + 4.0 * cao) / 1000.0;
+    const double sIi = sqrt(Ii), sIss = sqrt(Iss), sIo = sqrt(Io);
+    const double lgi = sIi / (1.0 + sIi) - 0.3 * Ii, lgss = sIss / (1.0 + sIss) - 0.3 * Iss,
+                 lgo = sIo / (1.0 + sIo) - 0.3 * Io;
+    const double gamma_cai = exp(-constA * 4.0 * lgi), gamma_cass = exp(-constA * 4.0 * lgss);
+    const double gamma_ki = exp(-constA * 1.0 * lgi), gamma_kss = exp(-constA * 1.0 * lgss);
+    const double gamma_cao = exp(-constA * 4.0 * lgo), gamma_o1 = exp(-constA * 1.0 * lgo);
+    TR_GATE(TR_d, dss, 0.6 + 1.0 / (exp(-0.05 * (V + 6.0)) + exp(0.09 * (V + 14.0))));
+    TR_GATE(TR_ff, fss, tff);
+    TR_GATE(TR_fs, fss, 1000.0 + 1.0 / (3.5e-05 * exp(-(V + 5.0) / 4.0) + 3.5e-05 * exp((V + 5.0) / 6.0)));
+    TR_GATE(TR_jca, 1.0 / (1.0 + exp((V + 18.08) / 2.7916)), 72.5);
+    TR_GATE(TR_fcaf, fss, tfcaf);
+    TR_GATE(TR_fcafp, fss, 2.5 * tfcaf);
+    TR_GATE(TR_fcas, fss, 100.0 + 1.0 / (0.00012 * exp(-V / 3.0) + 0.00012 * exp(V / 7.0)));
+    TR_GATE(TR_ffp, fss, 2.5 * tff);
+    TR_LIN(TR_nca_i, anca_i * k2n, km2n, anca_i * k2n - y[TR_nca_i] * km2n);
+    TR_LIN(TR_nca_ss, anca_ss * k2n, km2n, anca_ss * k2n - y[TR_nca_ss] * km2n);
+
+    /* IKr Markov chain (:1011, :1063-1109) */
+    const double vfrt = V * F / (R * T), vffrt = V * F * F / (R * T);
+    const double IKr = c[RC_GKr] * sqrt(ko / 5.0) * y[TR_O] * dvk;
+    const double al = 0.1161 * exp(0.299 * vfrt), be = 0.2442 * exp(-1.604 * vfrt);
+    const double al2 = 0.0578 * exp(0.971 * vfrt), be2 = 0.000349 * exp(-1.062 * vfrt);
+    const double ali = 0.2533 * exp(0.5953 * vfrt), bei = 0.06525 * exp(-0.8209 * vfrt);
+    const double alCI = 5.2e-05 * exp(1.525 * vfrt);
+    const double beIC = be2 * bei * alCI / (al2 * ali);
+    const double al1 = 0.154375, be1 = 0.1911;
+    const double C1 = y[TR_C1], C2 = y[TR_C2], C3 = y[TR_C3], Ik = y[TR_I], O = y[TR_O];
+    TR_LIN(TR_C3, be * C2, al, be * C2 - al * C3);
+    TR_LIN(TR_C2, al * C3 + be1 * C1, be + al1, al * C3 + be1 * C1 - (be + al1) * C2);
+    TR_LIN(TR_C1, al1 * C2 + be2 * O + beIC * Ik, be1 + al2 + alCI,
+           al1 * C2 + be2 * O + beIC * Ik - (be1 + al2 + alCI) * C1);
+    TR_LIN(TR_O, al2 * C1 + bei * Ik, be2 + ali, al2 * C1 + bei * Ik - (be2 + ali) * O);
+    TR_LIN(TR_I, alCI * C1 + ali * O, beIC + bei, alCI * C1 + ali * O - (beIC + bei) * Ik);
+
+    /* INaCa, i and ss compartments (:1013-1023, :1065-1166) */
+    const double kna1 = 15.0, kna2 = 5.0, kna3 = 88.12, kcaon = 1500000.0, kcaoff = 5000.0,
+                 wca = 60000.0, wna = 60000.0, wnaca = 5000.0;
+    const double h10 = 12.5 + 1.0 + nao / kna1 * (1.0 + nao / kna2);
+    const double h11 = nao * nao / (h10 * kna1 * kna2), h12 = 1.0 / h10;
+    const double k1 = h12 * cao * kcaon, k2 = kcaoff, k5 = kcaoff;
+    const double hca = exp(0.167 * vfrt), hna = exp(0.5224 * vfrt);
+    double INaCa[2];
+    for (int comp = 0; comp < 2; ++comp) {
+        const double na = comp == 0 ? Nai : Nass, ca = comp == 0 ? Cai : Cass;
+        const double allo = 1.0 / (1.0 + pow(0.00015 / ca, 2.0));
+        const double h4 = 1.0 + na / kna1 * (1.0 + na / kna2);
+        const double h5 = na * na / (h4 * kna1 * kna2), h6 = 1.0 / h4;
+        const double h1 = 1.0 + na / kna3 * (1.0 + hna);
+        const double h7 = 1.0 + nao / kna3 * (1.0 + 1.0 / hna);
+        const double h2 = na * hna / (kna3 * h1), h3 = 1.0 / h1;
+        const double h8 = nao / (kna3 * hna * h7), h9 = 1.0 / h7;
+        const double k6 = h6 * ca * kcaon;
+        const double k3pp = h8 * wnaca, k4pp = h2 * wnaca;
+        const double k3 = h9 * wca + k3pp, k4 = h3 * wca / hca + k4pp;
+        const double k7 = h5 * h2 * wna, k8 = h8 * h11 * wna;
+        const double x1 = k2 * k4 * (k7 + k6) + k5 * k7 * (k2 + k3);
+        const double x2 = k1 * k7 * (k4 + k5) + k4 * k6 * (k1 + k8);
+        const double x3 = k1 * k3 * (k7 + k6) + k8 * k6 * (k2 + k3);
+        const double x4 = k2 * k8 * (k4 + k5) + k3 * k5 * (k1 + k8);
+        const double sx = x1 + x2 + x3 + x4;
+        const double E1 = x1 / sx, E2 = x2 / sx, E3 = x3 / sx, E4 = x4 / sx;
+        const double JCa = E2 * k2 - E1 * k1;
+        const double JNa = 3.0 * (E4 * k7 - E1 * k8) + E3 * k4pp - E2 * k3pp;
+        INaCa[comp] = (comp == 0 ? 1.0 - 0.35 : 0.35) * c[RC_Gncx] * allo * (1.0 * JNa + 2.0 * JCa);
+    }
+
+    /* INaK (:1025-1027, :1081-1139) */
+    const double Knai = 9.073 * exp(-0.155 * vfrt / 3.0);
+    const double Knao = 27.78 * exp((1.0 - -0.155) * vfrt / 3.0);
+    const double Pp = 4.2 / (1.0 + 1e-07 / 1.698e-07 + Nai / 224.0 + Ki / 292.0);
+    const double dnk = pow(1.0 + Nai / Knai, 3.0) + pow(1.0 + Ki / 0.5, 2.0) - 1.0;
+    const double dno = pow(1.0 + nao / Knao, 3.0) + pow(1.0 + ko / 0.3582, 2.0) - 1.0;
+    const double a1 = 949.5 * pow(Nai / Knai, 3.0) / dnk;
+    const double b1 = 182.4 * 0.05, a2 = 687.2;
+    const double b2 = 39.4 * pow(nao / Knao, 3.0) / dno;
+    const double a3 = 1899.0 * pow(ko / 0.3582, 2.0) / dno;
+    const double b3 = 79300.0 * Pp * 1e-07 / (1.0 + 9.8 / 1.698e-07);
+    const double a4 = 639.0 * 9.8 / 1.698e-07 / (1.0 + 9.8 / 1.698e-07);
+    const double b4 = 40.0 * pow(Ki / 0.5, 2.0) / dnk;
+    const double x1 = a4 * a1 * a2 + b2 * b4 * b3 + a2 * b4 * b3 + b3 * a1 * a2;
+    const double x2 = b2 * b1 * b4 + a1 * a2 * a3 + a3 * b1 * b4 + a2 * a3 * b4;
+    const double x3 = a2 * a3 * a4 + b3 * b2 * b1 + b2 * b1 * a4 + a3 * a4 * b1;
+    const double x4 = b4 * b3 * b2 + a3 * a4 * a1 + b2 * a4 * a1 + b3 * b2 * a1;
+    const double sx = x1 + x2 + x3 + x4;
+    const double JnakK = 2.0 * (x4 / sx * b1 - x3 / sx * a1);
+    const double JnakNa = 3.0 * (x1 / sx * a3 - x2 / sx * b3);
+    const double INaK = c[RC_Pnak] * (1.0 * JnakNa + 1.0 * JnakK);
+
+    /* GHK fluxes and L-type currents (:1047-1062) */
+    const double e1 = exp(1.0 * vfrt), e2 = exp(2.0 * vfrt);
+    const double PhiCaK_i = 1.0 * vffrt * (gamma_ki * Ki * e1 - gamma_o1 * ko) / (e1 - 1.0);
+    const double PhiCaK_ss = 1.0 * vffrt * (gamma_kss * Kss * e1 - gamma_o1 * ko) / (e1 - 1.0);
+    const double PhiCaL_i = 4.0 * vffrt * (gamma_cai * Cai * e2 - gamma_cao * cao) / (e2 - 1.0);
+    const double PhiCaL_ss = 4.0 * vffrt * (gamma_cass * Cass * e2 - gamma_cao * cao) / (e2 - 1.0);
+    const double PhiCaNa_i = 1.0 * vffrt * (gamma_ki * Nai * e1 - gamma_o1 * nao) / (e1 - 1.0);
+    const double PhiCaNa_ss = 1.0 * vffrt * (gamma_kss * Nass * e1 - gamma_o1 * nao) / (e1 - 1.0);
+    const double ICab = c[RC_PCab] * 4.0 * vffrt * (gamma_cai * Cai * e2 - gamma_cao * cao) / (e2 - 1.0);
+    const double INab = c[RC_PNab] * vffrt * (Nai * exp(vfrt) - nao) / (exp(vfrt) - 1.0);
+    const double PCa = c[RC_PCa], PCap = 1.1 * PCa;
+    const double gi = f * (1.0 - y[TR_nca_i]) + y[TR_jca] * fca * y[TR_nca_i];
+    const double gpi = fp * (1.0 - y[TR_nca_i]) + y[TR_jca] * fcap * y[TR_nca_i];
+    const double gs = f * (1.0 - y[TR_nca_ss]) + y[TR_jca] * fca * y[TR_nca_ss];
+    const double gps = fp * (1.0 - y[TR_nca_ss]) + y[TR_jca] * fcap * y[TR_nca_ss];
+    const double dd = y[TR_d];
+    /* fractionSS = 0.8 of the L-type flux faces the subspace */
+    const double fi = 1.0 - 0.8, fs = 0.8;
+    const double ICaL_i = fi * ((1.0 - fcamk) * PCa * PhiCaL_i * dd * gi + fcamk * PCap * PhiCaL_i * dd * gpi);
+    const double ICaL_ss = fs * ((1.0 - fcamk) * PCa * PhiCaL_ss * dd * gs + fcamk * PCap * PhiCaL_ss * dd * gps);
+    const double ICaK_i = fi * ((1.0 - fcamk) * (0.0003574 * PCa) * PhiCaK_i * dd * gi +
+                                fcamk * (0.0003574 * PCap) * PhiCaK_i * dd * gpi);
+    const double ICaK_ss = fs * ((1.0 - fcamk) * (0.0003574 * PCa) * PhiCaK_ss * dd * gs +
+                                 fcamk * (0.0003574 * PCap) * PhiCaK_ss * dd * gps);
+    const double ICaNa_i = fi * ((1.0 - fcamk) * (0.00125 * PCa) * PhiCaNa_i * dd * gi +
+                                 fcamk * (0.00125 * PCap) * PhiCaNa_i * dd * gpi);
+    const double ICaNa_ss = fs * ((1.0 - fcamk) * (0.00125 * PCa) * PhiCaNa_ss * dd * gs +
+                                  fcamk * (0.00125 * PCap) * PhiCaNa_ss * dd * gps);
+
+    /* buffers and RyR (:1032-1045) */
+    const double Bcajsr = 1.0 / (1.0 + 10.0 * 0.8 / pow(0.8 + y[TR_Cajsr], 2.0));
+    const double Bcass = 1.0 / (1.0 + 0.047 * 0.00087 / pow(0.00087 + Cass, 2.0) +
+                                1.124 * 0.0087 / pow(0.0087 + Cass, 2.0));
+    const double Bcai = 1.0 / (1.0 + c[RC_cmdnmax] * 0.00238 / pow(0.00238 + Cai, 2.0) +
+                               0.07 * 0.0005 / pow(0.0005 + Cai, 2.0));
+    const double bt = c[RC_bt], btp = 1.25 * bt;
+    const double a_rel = 0.5 * bt / 1.0, a_relp = 0.5 * btp / 1.0;
+    const double Jrel = c[RC_Jrel_b] * ((1.0 - fcamk) * y[TR_Jrelnp] + fcamk * y[TR_Jrelp]);
+    double trel = bt / (1.0 + 0.0123 / y[TR_Cajsr]);
+    if (trel < 0.001) trel = 0.001;
+    double trelp = btp / (1.0 + 0.0123 / y[TR_Cajsr]);
+    if (trelp < 0.001) trelp = 0.001;
+    const double p8 = 1.0 + pow(1.7 / y[TR_Cajsr], 8.0);
+    double Jrel_inf = -a_rel * ICaL_ss / 1.0 / p8;
+    double Jrel_infp = -a_relp * ICaL_ss / 1.0 / p8;
+    if (tf == 2) {
+        Jrel_inf *= 1.7;
+        Jrel_infp *= 1.7;
+    }
+    TR_GATE(TR_Jrelnp, Jrel_inf, trel);
+    TR_GATE(TR_Jrelp, Jrel_infp, trelp);
+#undef TR_GATE
+#undef TR_LIN
+
+    /* concentrations (:1029-1031, :1119-1165) */
+    r[TR_Cansr] = Jup - Jtr * vjsr / vnsr;
+    r[TR_Cli] = (IClb + IClCa_sl) * Acap / (F * vmyo) + JdiffCl * vss / vmyo;
+    r[TR_Clss] = IClCa_junc * Acap / (F * vss) - JdiffCl;
+    r[TR_Cajsr] = Bcajsr * (Jtr - Jrel);
+    r[TR_Ki] = -(Ito + IKr + IKs + IK1 + IKb + Ikatp - 2.0 * INaK + ICaK_i) * Acap / (F * vmyo) +
+               JdiffK * vss / vmyo;
+    r[TR_Kss] = -ICaK_ss * Acap / (F * vss) - JdiffK;
+    r[TR_Cai] = Bcai * (-(ICaL_i + IpCa + ICab - 2.0 * INaCa[0]) * Acap / (2.0 * F * vmyo) -
+                        Jup * vnsr / vmyo + Jdiff * vss / vmyo);
+    r[TR_Cass] = Bcass * (-(ICaL_ss - 2.0 * INaCa[1]) * Acap / (2.0 * F * vss) +
+                          Jrel * vjsr / vss - Jdiff);
+    r[TR_Nai] = -(INa + INaL + 3.0 * INaCa[0] + ICaNa_i + 3.0 * INaK + INab) * Acap / (F * vmyo) +
+                JdiffNa * vss / vmyo;
+    r[TR_Nass] = -(ICaNa_ss + 3.0 * INaCa[1]) * Acap / (F * vss) - JdiffNa;
+
+    const double Iion = INa + INaL + Ito + (ICaL_ss + ICaL_i) + (ICaNa_ss + ICaNa_i) +
+                        (ICaK_ss + ICaK_i) + IKr + IKs + IK1 + INaCa[0] + INaCa[1] + INaK + INab +
+                        IKb + IpCa + ICab + IClCa + IClb + Ikatp;
+    const double Ist = orc_stimulus(t, st);
+    r[TR_V] = solve_vm ? -Iion - Ist : 0.0;
+    *iion = Iion;
+    *istim = Ist;
+}
+
 /* ============================================================== dispatch */
 int orc_model_nstates(int model)
 {
@@ -1510,6 +1891,7 @@ int orc_model_nstates(int model)
     case ORC_PYS: return PY_NS;
     case ORC_STW: return SW_NS;
     case ORC_GRD: return GR_NS;
+    case ORC_TOR: return TR_NS;
     default: return -1;
     }
 }
@@ -1545,6 +1927,9 @@ int orc_model_init(int model, int tissue, double* c, double* y)
     case ORC_GRD:
         if (tissue != ORC_MYOCYTE) return -1;
         grd_init(c, y); return 0;
+    case ORC_TOR:
+        if (tissue < 1 || tissue > 3) return -1;
+        tor_init(tissue, c, y); return 0;
     default: return -1;
     }
 }
@@ -1576,6 +1961,11 @@ int orc_model_const_index(int model, const char* name)
         if (!strcmp(name, "g_Kr")) return SC_gKr;
         if (!strcmp(name, "g_CaL")) return SC_gCaL;
         if (!strcmp(name, "g_Ks")) return SC_gKs;
+    } else if (model == ORC_TOR) {
+        if (!strcmp(name, "GNa")) return RC_GNa;
+        if (!strcmp(name, "GKr")) return RC_GKr;
+        if (!strcmp(name, "PCa")) return RC_PCa;
+        if (!strcmp(name, "GKs")) return RC_GKs;
     } else if (model == ORC_GRD) {
         if (!strcmp(name, "g_Na")) return GC_gNa;
         if (!strcmp(name, "g_Kr")) return GC_gKr;
@@ -1641,6 +2031,7 @@ void orc_model_eval(int model, int tissue, double t, const double* c, const doub
     case ORC_PYS: pys_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_STW: stw_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_GRD: grd_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_TOR: tor_eval(tissue, t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     default: break;
     }
 }

@@ -36,13 +36,13 @@ extern "C" {
 
 /* model ids (same numbering as the product C-ABI) */
 enum { ORC_BO = 0, ORC_TNNP = 1, ORC_ORD = 2, ORC_AP = 3, ORC_FDA = 4, ORC_CRN = 5,
-       ORC_PYS = 6, ORC_STW = 7, ORC_GRD = 8 };
+       ORC_PYS = 6, ORC_STW = 7, ORC_GRD = 8, ORC_TOR = 9 };
 /* dimension flags of the manufactured model (its `dimension` word 1D/2D/3D) */
 enum { ORC_DIM1 = 11, ORC_DIM2 = 12, ORC_DIM3 = 13 };
 /* tissue flags, ionicSelector.C:60-66 */
 enum { ORC_EPI = 1, ORC_MCELL = 2, ORC_ENDO = 3, ORC_MYOCYTE = 4 };
 
-#define ORC_MAX_STATES 41
+#define ORC_MAX_STATES 48
 #define ORC_MAX_CONSTS 160
 
 typedef struct {
@@ -241,6 +241,18 @@ int orc_tissue_create(orc_tissue_t* T, int nx, int ny, int nz, double hx, double
 void orc_tissue_free(orc_tissue_t* T);
 /* one step: t += dt; t0 = t - dt; stimulus; ionic; diffusion; activation */
 int orc_tissue_step(orc_tissue_t* T, const orc_boxes_t* boxes);
+
+/* Tangent-linear run (tangent.cpp) of the BuenoOrovio tissue from rest: explicit, Euler
+ * ddt, Rush-Larsen, sigma_c = kappa_c sig0[3] per cell, h-cube cells.  Loss = sum over
+ * n_steps and probes of (Vm - target[step][probe])^2; dloss[k] its derivative along
+ * direction k = (dkappa[k][n], dtheta[k][2]) of (kappa, tau_fi, tau_si), n_dir <= 8.
+ * v_final [n] (may be NULL): Vm after the last step.  Returns 0, or < 0 on bad input. */
+int orc_bo_tangent(int nx, int ny, int nz, double h, const double* sig0, int tissue,
+                   double dt, double chi, double cm, const orc_boxes_t* boxes,
+                   const double* kappa, const double* theta, int n_dir, const double* dkappa,
+                   const double* dtheta, long n_steps, int n_probes, const long* probes,
+                   const double* target, int nthreads, double* loss, double* dloss,
+                   double* v_final);
 
 /* ------------------------------------------------------------ APD90 */
 typedef struct {

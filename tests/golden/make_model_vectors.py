@@ -11,7 +11,8 @@ they are written in:
 
 The subset is assignments to STATES / CONSTANTS / RATES / ALGEBRAIC entries and local
 doubles, if / else, `?:`, the arithmetic, relational and logical operators, and
-exp / pow / log / sqrt / fabs / tanh / heaviside (heaviside.H: 1 above 0, else 0).
+exp / pow / log / sqrt / fabs / tanh / heaviside (heaviside.H: 1 above 0, else 0) and
+ToRORd's tissueSelect (its endo / M / epi column).
 `computeIstim` is 0: the vectors carry no stimulus.
 
 Inputs are seeded states along each model's paced action potential (from the oracle:
@@ -52,6 +53,8 @@ MODELS = {
                         (4,), 6),
     "Stewart": ("Stewart/Stewart_2009.H", "Stewart", "computeVariables", (4,), 7),
     "Grandi": ("Grandi/Grandi_2011.H", "Grandi", "computeVariables", (4,), 8),
+    "ToRORd_dynCl": ("ToRORd_dynCl/ToRORd_dynCl_2023.H", "ToRORd_dynCl", "computeVariables",
+                     (1, 2, 3), 9),
 }
 
 # ORd-2011 INaK and SERCA constants left unassigned by ORdinitConsts (published values)
@@ -276,7 +279,9 @@ class Interp:
 
     FUNCS = {"exp": math.exp, "log": math.log, "sqrt": math.sqrt, "fabs": math.fabs,
              "tanh": math.tanh, "pow": math.pow,
-             "heaviside": lambda x: 1 if x > 0 else 0, "computeIstim": lambda *a: 0.0}
+             "heaviside": lambda x: 1 if x > 0 else 0, "computeIstim": lambda *a: 0.0,
+             # ToRORd_dynCl_2023.H:558-568: the endo, M, epi column of an initial value
+             "tissueSelect": lambda f, v1, v2, v3: v1 if f == 3 else (v2 if f == 2 else v3)}
 
     def primary(self, live):
         kind, v = self.t[self.i]
@@ -393,7 +398,7 @@ def input_states(key, tissue, n=24, seed=0):
     import oracle as orc
     m = MODELS[key][4]
     c, y = orc.model_init(m, tissue)
-    amp = {0: 0.4, 1: 52.0, 2: 80.0, 3: 0.5, 5: 60.0, 6: 40.0, 7: 40.0, 8: 40.0}[m]
+    amp = {0: 0.4, 1: 52.0, 2: 80.0, 3: 0.5, 5: 60.0, 6: 40.0, 7: 40.0, 8: 40.0, 9: 53.0}[m]
     st = orc.make_stim(start=2.0, period_s1=1000.0, duration=1.0, amplitude=amp, nstim1=1)
     dt = 0.01 * (1.0 / 12.9 if m == 3 else 1.0)
     yy = np.zeros(64)

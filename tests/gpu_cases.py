@@ -13,6 +13,9 @@ MODELS = {  # name -> (oracle id, tissue flag, library key, tissue word, stimulu
     "PYS": (6, 4, "PerisYague_2022", "myocyte", 40.0),
     "STW": (7, 4, "Stewart", "myocyte", 40.0),
     "GRD": (8, 4, "Grandi", "myocyte", 40.0),
+    "TOR_epi": (9, 1, "ToRORd_dynCl", "epicardialCells", 53.0),
+    "TOR_M": (9, 2, "ToRORd_dynCl", "mCells", 53.0),
+    "TOR_endo": (9, 3, "ToRORd_dynCl", "endocardialCells", 53.0),
 }
 
 
@@ -26,7 +29,7 @@ def physiological_states(orc, name, n, seed=0, dt_ms=0.01, span_ms=400.0):
     nsamp = 64
     every = max(1, int(span_ms / dt_ms / nsamp))
     samples = []
-    yy = np.zeros(41)
+    yy = np.zeros(orc.MAX_STATES)
     yy[:len(y)] = y
     for k in range(nsamp * every):
         orc.lib().orc_rl_step(m, tis, k * dt_ms * tscale, dt_ms * tscale, c, yy, 1, st)

@@ -20,12 +20,13 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def _dom(gpu, shape, h, algorithm="explicit", tol=1e-8):
+def _dom(gpu, shape, h, algorithm="explicit", tol=1e-8, box_max=None):
     mesh = gpu.HexMesh(*shape, h)
+    box = dict(BOX) if box_max is None else dict(BOX, stimulusLocationMax=list(box_max))
     coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": 1.4e5,
               "cm": 0.01, "solutionAlgorithm": algorithm,
               "conductivity": [SIG0[0], 0, 0, 0, SIG0[1], 0, 0, 0, SIG0[2]],
-              "externalStimulus": BOX}
+              "externalStimulus": box}
     return mesh, gpu.Monodomain(mesh, coeffs, dt=DT, tol=tol)
 
 
@@ -180,6 +181,59 @@ def test_implicit_adjoint_forward_equals_tissue_step(gpu, orc):
     err = np.max(np.abs(tr - ref)) / np.max(np.abs(ref))
     _say("  trace rel err %.3e" % err)
     assert err < steps * 2e-13
+
+
+def _tangent_check(gpu, orc, shape, h, steps, n_probes, box_max, n_kdirs, seed, ckpt):
+    """The adjoint's loss and gradient against the oracle's tangent-linear run over the
+    whole horizon: the loss, n_kdirs random kappa directions and tau_fi, tau_si.  Returns
+    the worst relative differences (loss, directions)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    mesh, dom = _dom(gpu, shape, h, box_max=box_max)
+    cells = rng.choice(n, n_probes, replace=False)
+    adj = gpu.TissueAdjoint(dom, cells, checkpoint_every=ckpt)
+    theta0 = np.array([0.11, 1.8875])
+    # target: the device's own run at a smooth conductivity perturbation
+    x = (np.arange(n) % shape[0]) / shape[0]
+    kt = torch.tensor(1.0 + 0.2 * np.sin(2 * np.pi * x), device="cuda")
+    with torch.no_grad():
+        target = gpu.probe_traces(adj, kt, torch.tensor(theta0), steps).clone()
+    kappa = torch.ones(n, dtype=torch.float64, device="cuda", requires_grad=True)
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    loss = ((gpu.probe_traces(adj, kappa, theta, steps) - target) ** 2).sum()
+    loss.backward()
+    gk, gt = kappa.grad.cpu().numpy(), theta.grad.numpy()
+    dk = np.zeros((n_kdirs + 2, n))
+    dk[:n_kdirs] = rng.standard_normal((n_kdirs, n))
+    dth = np.zeros((n_kdirs + 2, 2))
+    dth[n_kdirs, 0], dth[n_kdirs + 1, 1] = 1.0, 1.0
+    _say("  tangent-linear oracle run: %d cells, %d steps, %d directions" %
+         (n, steps, n_kdirs + 2))
+    L, d, _ = orc.bo_tangent(shape, h, SIG0, orc.EPI, DT, 1.4e5, 0.01,
+                             orc.make_boxes(BOX["stimulusLocationMin"],
+                                            BOX["stimulusLocationMax"] if box_max is None
+                                            else box_max, [0.0], [2e-3], [5e4]),
+                             np.ones(n), theta0, dk, dth, steps, cells,
+                             target.cpu().numpy(), nthreads=16)
+    adj_d = np.concatenate([dk[:n_kdirs] @ gk, gt])
+    rel = np.abs(adj_d - d) / np.maximum(np.abs(d), 1e-300)
+    _say("  loss device %.12e oracle %.12e" % (loss.item(), L))
+    for k in range(n_kdirs + 2):
+        _say("  direction %d: adjoint %.12e tangent %.12e rel %.2e" % (k, adj_d[k], d[k], rel[k]))
+    return abs(loss.item() - L) / abs(L), float(np.max(rel))
+
+
+def test_adjoint_full_horizon_vs_tangent_linear(gpu, orc):
+    """5000 steps (50 ms: stimulus, the wave across the block, the plateau and the start of
+    repolarisation), where central differences fail (a perturbation moves threshold
+    crossings by whole steps): the gradient against the oracle's tangent-linear run of the
+    same discrete map, which differentiates the branch each step takes as the adjoint
+    does.  Six kappa directions, tau_fi and tau_si."""
+    _say("adjoint full horizon vs tangent-linear oracle")
+    lrel, drel = _tangent_check(gpu, orc, (16, 12, 6), 2e-4, 5000, 32, None, 6, 11, 250)
+    assert lrel <= 1e-10
+    assert drel <= 1e-7
 
 
 def test_handles_destroyed_in_any_order(gpu):

@@ -50,7 +50,7 @@ def test_ionic_tissue_step_parity(gpu, orc, name, precision):
 
 
 @pytest.mark.parametrize("name", ["BO_endo", "TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW",
-                                  "GRD"])
+                                  "GRD", "TOR_epi", "TOR_M", "TOR_endo"])
 def test_single_cell_run_parity(gpu, orc, name):
     """cg_ionic_run_cells (Vm inside the ODE, stimulus protocol) vs the oracle's RL
     single cell, stimulus edges placed off the step grid."""
@@ -66,7 +66,7 @@ def test_single_cell_run_parity(gpu, orc, name):
     got = ion.states()[:, 0]
     c, y = orc.model_init(m, tis)
     st = orc.make_stim(start=10.0025, period_s1=500.0, duration=1.0, amplitude=amp, nstim1=2)
-    yy = np.zeros(41)
+    yy = np.zeros(orc.MAX_STATES)
     yy[:len(y)] = y
     dtm = dt * 1000.0 / 12.9 if m == 3 else dt * 1000.0
     for k in range(n_steps):

@@ -61,7 +61,8 @@ def test_single_cell_golden_on_device(gpu, orc):
 
 
 @pytest.mark.parametrize("tol", [(1e-6, 1e-4, 1e-3), (1e-10, 1e-8, 1e-6)])
-@pytest.mark.parametrize("name", ["TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW", "GRD"])
+@pytest.mark.parametrize("name", ["TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW", "GRD",
+                                  "TOR_M"])
 def test_single_cell_rkf45_vs_oracle(gpu, orc, name, tol):
     """cg_ionic_run_cells with RKF45 (stimulus evaluated at each stage time) vs the
     oracle's single-cell RKF45 over one paced beat.  The stimulus switches on inside a
@@ -91,7 +92,7 @@ def test_single_cell_rkf45_vs_oracle(gpu, orc, name, tol):
 
 
 @pytest.mark.parametrize("name", ["BO_epi", "TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW",
-                                  "GRD"])
+                                  "GRD", "TOR_endo"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_ionic_tissue_step_rkf45(gpu, orc, name, precision):
     """cg_ionic_step with RKF45 over 20 steps from states along an action potential; the

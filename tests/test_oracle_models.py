@@ -16,6 +16,8 @@ beyond the device-vs-oracle agreement.
 Grandi: the Na+/Ca2+ charge balance over its three compartments and their buffers (K_i
 and Cl_i are fixed in that model), and first-order convergence of the Rush-Larsen split
 to the adaptive RKF45 on one paced beat.
+ToRORd_dynCl: the charge balance of Na, K, Cl and Ca with its rapid buffers (three tissues),
+and the Rush-Larsen split against RKF45 on the published action potential.
 AlievPanfilov: the cubic's roots 0, a, 1 and the recovery nullcline.
 Stimulus: stimulusIO.C:238-286 cases, including the S2 placement at the end of S1.
 """
@@ -37,8 +39,13 @@ def _states_along_ap(orc, model, tissue, amp, n=12):
     return c, out
 
 
+def orc_max_states():
+    import oracle
+    return oracle.MAX_STATES
+
+
 def _pad(y):
-    a = np.zeros(41)
+    a = np.zeros(orc_max_states())
     a[:len(y)] = y
     return a
 
@@ -148,7 +155,8 @@ def test_aliev_panfilov_cubic(orc):
 
 def test_tissue_support_matches_reference(orc):
     # BuenoOrovio.C:92-96, TNNP.C:90-94, ORd.C:93-96, AlievPanfilov.C:92-95,
-    # Courtemanche.C:92-95, PerisYague.C:92-95, Stewart.C:93-96, Grandi.C:93-96
+    # Courtemanche.C:92-95, PerisYague.C:92-95, Stewart.C:93-96, Grandi.C:93-96,
+    # ToRORd_dynCl.C:93-96
     for tissue in (1, 2, 3):
         orc.model_init(orc.BO, tissue)
         orc.model_init(orc.TNNP, tissue)
@@ -162,11 +170,14 @@ def test_tissue_support_matches_reference(orc):
             orc.model_init(orc.STW, tissue)
         with pytest.raises(ValueError):
             orc.model_init(orc.GRD, tissue)
+        orc.model_init(orc.TOR, tissue)
     orc.model_init(orc.ORD, 4)
     orc.model_init(orc.CRN, 4)
     orc.model_init(orc.PYS, 4)
     orc.model_init(orc.STW, 4)
     orc.model_init(orc.GRD, 4)
+    with pytest.raises(ValueError):
+        orc.model_init(orc.TOR, 4)
     orc.model_init(orc.AP, 4)
     with pytest.raises(ValueError):
         orc.model_init(orc.BO, 4)
@@ -343,7 +354,7 @@ def test_stw_limit_cycle_returns_to_reference_state(orc):
     import ctypes as C
     c, y0 = orc.model_init(orc.STW, orc.MYOCYTE)
     st = orc.make_stim(start=0.0, period_s1=1000.0, duration=0.0, amplitude=0.0, nstim1=0)
-    yy = np.zeros(41)
+    yy = np.zeros(orc.MAX_STATES)
     yy[:20] = y0
     step = C.c_double(1e-3)
     t, dt, cross = 0.0, 0.05, []
@@ -444,3 +455,60 @@ def test_grd_rush_larsen_converges_to_rkf45(orc):
     assert 3.0 < errs[1] / errs[2] < 5.0          # dt ratio 4
     assert errs[2] < 3.0
     apd(0, 2e-5)                                  # stable at the tissue step
+
+
+@pytest.mark.parametrize("tissue", ["EPI", "MCELL", "ENDO"])
+def test_tor_charge_conservation(orc, tissue):
+    """ToRORd_dynCl: every ion is dynamic, chloride included (charge -1), so F times the
+    volume-weighted change of Na, K, Cl and twice that of total Ca (the rapid-buffer
+    factors undone, the SR pools included) equals -Acap Iion (Cm = 1 uF/cm^2)."""
+    L, rad = 0.01, 0.0011
+    Ageo = 2 * 3.14 * rad * rad + 2 * 3.14 * rad * L
+    vcell = 1000 * 3.14 * rad * rad * L
+    Acap, vmyo, vss, vnsr, vjsr = 2 * Ageo, 0.68 * vcell, 0.02 * vcell, 0.0552 * vcell, 0.0048 * vcell
+    F = 96485.0
+    tf = getattr(orc, tissue)
+    cmdnmax = 0.05 * 1.3 if tf == orc.EPI else 0.05
+    kmcmdn, trpnmax, kmtrpn = 0.00238, 0.07, 0.0005
+    BSRmax, KmBSR, BSLmax, KmBSL = 0.047, 0.00087, 1.124, 0.0087
+    csqnmax, kmcsqn = 10.0, 0.8
+    c, states = _states_along_ap(orc, orc.TOR, tf, 53.0)
+    I = dict(Nai=2, Nass=3, Ki=4, Kss=5, Cass=6, Cansr=7, Cajsr=8, Cai=9, Cli=10, Clss=11)
+    seen = 0.0
+    for y in states:
+        r = orc.model_eval(orc.TOR, tf, 500.0, c, y, solve_vm=True)
+        d = r["rates"]
+        Cai, Cass, Cajsr = y[I["Cai"]], y[I["Cass"]], y[I["Cajsr"]]
+        dCai_t = d[I["Cai"]] * (1 + cmdnmax * kmcmdn / (kmcmdn + Cai) ** 2 + trpnmax * kmtrpn / (kmtrpn + Cai) ** 2)
+        dCass_t = d[I["Cass"]] * (1 + BSRmax * KmBSR / (KmBSR + Cass) ** 2 + BSLmax * KmBSL / (KmBSL + Cass) ** 2)
+        dCajsr_t = d[I["Cajsr"]] * (1 + csqnmax * kmcsqn / (kmcsqn + Cajsr) ** 2)
+        lhs = F * (vmyo * (d[I["Nai"]] + d[I["Ki"]] - d[I["Cli"]]) +
+                   vss * (d[I["Nass"]] + d[I["Kss"]] - d[I["Clss"]]) +
+                   2 * (vmyo * dCai_t + vss * dCass_t + vnsr * d[I["Cansr"]] + vjsr * dCajsr_t))
+        assert lhs == pytest.approx(-Acap * r["iion"], rel=1e-9, abs=1e-12)
+        seen = max(seen, abs(r["iion"]))
+    assert seen > 0.5          # tested with currents flowing
+
+
+@pytest.mark.parametrize("tissue", ["EPI", "MCELL", "ENDO"])
+def test_tor_rush_larsen_converges_to_rkf45(orc, tissue):
+    """The Rush-Larsen split of ToRORd_dynCl (gates; nca and the IKr Markov states linear
+    in themselves) against the adaptive RKF45 on one paced beat from the reference's
+    initial state: the published ToR-ORd action potential (rest about -89..-91 mV, APD90
+    237 / 351 / 273 ms for epi / M / endo) and an APD90 error that shrinks with dt
+    (measured 0.76 ms at 0.01 ms and 0.16 ms at 0.002 ms in M cells, below 0.15 ms in the
+    others)."""
+    tf = getattr(orc, tissue)
+    st = orc.make_stim(start=10.0, period_s1=1000.0, duration=1.0, amplitude=53.0, nstim1=1)
+
+    def run(integ, dt, **kw):
+        t, v, y = orc.single_cell_run(orc.TOR, tf, st, dt, int(0.6 / dt), integrator=integ, **kw)
+        b = orc.detect_beats(t, v)
+        assert len(b) == 1 and np.all(np.isfinite(y))
+        return (b[0]["t_repol90"] - b[0]["t_up"]) * 1e3, b[0]
+
+    a_rk, b_rk = run(1, 1e-5, abs_tol=1e-10, rel_tol=1e-8)
+    assert 220.0 < a_rk < 370.0 and -92.0 < b_rk["v_base"] < -88.0 and b_rk["v_peak"] > 20.0
+    e1 = abs(run(0, 1e-5)[0] - a_rk)
+    e2 = abs(run(0, 2e-6)[0] - a_rk)
+    assert e1 < 5.0 and (e2 < 0.3 * e1 or e2 < 0.05)
