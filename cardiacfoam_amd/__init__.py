@@ -1,6 +1,6 @@
 """cardiacfoam_amd -- MI355X (gfx950) implementation of cardiacFoam's monodomain tissue
 step: per-cell ionic update (BuenoOrovio, TNNP, ORd, AlievPanfilov, Courtemanche,
-PerisYague_2022, Stewart, Grandi, ToRORd_dynCl) and transmembrane
+PerisYague_2022, Stewart, Grandi, ToRORd_dynCl, Gaur) and transmembrane
 diffusion (explicit face-flux update, or implicit assembly + Jacobi-PCG), behind the
 C-ABI in include/cardiacgpu.h.
 

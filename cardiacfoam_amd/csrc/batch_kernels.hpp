@@ -273,6 +273,7 @@ template <class T> struct BatchLaunch {
         case MODEL_STW: return BatchLaunch<T>::variant<Stewart>(variant, tissue, states, P, gscale, a, s); \
         case MODEL_GRD: return BatchLaunch<T>::variant<Grandi>(variant, tissue, states, P, gscale, a, s); \
         case MODEL_TOR: return BatchLaunch<T>::variant<ToRORd>(variant, tissue, states, P, gscale, a, s); \
+        case MODEL_GAUR: return BatchLaunch<T>::variant<Gaur>(variant, tissue, states, P, gscale, a, s); \
         default: return hipErrorInvalidValue;                                              \
         }                                                                                  \
     }

@@ -40,11 +40,11 @@ int fail(int code, const char* fmt, ...)
             return fail(CG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));           \
     } while (0)
 
-constexpr int kNumModels = 10;
+constexpr int kNumModels = 11;
 const char* const kModelKeys[kNumModels] = {"BuenoOrovio", "TNNP", "ORd", "AlievPanfilov",
                                             "monodomainFDAManufactured", "Courtemanche",
                                             "PerisYague_2022", "Stewart", "Grandi",
-                                            "ToRORd_dynCl"};
+                                            "ToRORd_dynCl", "Gaur"};
 
 // state names, order of the *_STATES_NAMES tables / Names.H enums
 const char* const kBONames[4] = {"u", "v", "w", "s"};
@@ -91,6 +91,11 @@ const char* const kTORNames[45] = {
     "ICaL_fcaf", "ICaL_fcas", "ICaL_jca", "ICaL_ffp", "ICaL_fcafp", "ICaL_nca_ss",
     "ICaL_nca_i", "IKr_C1", "IKr_C2", "IKr_C3", "IKr_I", "IKr_O", "IKs_xs1", "IKs_xs2",
     "Jrel_np", "Jrel_p"};
+const char* const kGAURNames[29] = {
+    "cell_v", "nai", "nass", "ki", "kss", "cai", "cai2", "cass", "cansr", "cajsr", "cacsr",
+    "I_Na_m", "I_Na_h", "I_Na_j", "INaL_ml", "INaL_hl", "ICaL_d", "ICaL_fca", "IKr_xr",
+    "IKs_xs1", "IKs_xs2", "ITo_aa", "CICR_Jrel2", "CICR_Jrel1", "CaMK_CaMKt", "CICR_tjsrol",
+    "CICR_A", "ICaL_fs", "ICaL_ff"};
 
 struct ModelInfo {
     int ns, np;
@@ -132,6 +137,9 @@ bool model_info(int model, ModelInfo* mi)
     case cg::MODEL_TOR:
         *mi = {cg::ToRORd::NS, cg::ToRORd::NP, 1.0, 0.0, 1.0, 1000.0};
         return true;
+    case cg::MODEL_GAUR:
+        *mi = {cg::Gaur::NS, cg::Gaur::NP, 1.0, 0.0, 1.0, 1000.0};
+        return true;
     default: return false;
     }
 }
@@ -149,6 +157,7 @@ bool model_host_init(int model, int tissue, double* p, double* y)
     case cg::MODEL_STW: return cg::Stewart::host_init(tissue, p, y);
     case cg::MODEL_GRD: return cg::Grandi::host_init(tissue, p, y);
     case cg::MODEL_TOR: return cg::ToRORd::host_init(tissue, p, y);
+    case cg::MODEL_GAUR: return cg::Gaur::host_init(tissue, p, y);
     default: return false;
     }
 }
@@ -166,6 +175,7 @@ double state_ref(int model, int q)
     case cg::MODEL_STW: r = cg::StateRef<cg::Stewart>::at(q); break;
     case cg::MODEL_GRD: r = cg::StateRef<cg::Grandi>::at(q); break;
     case cg::MODEL_TOR: r = cg::StateRef<cg::ToRORd>::at(q); break;
+    case cg::MODEL_GAUR: r = cg::StateRef<cg::Gaur>::at(q); break;
     default: break;
     }
     return (double)(float)r;
@@ -399,7 +409,8 @@ int cg_model_state_name(int model, int i, char* buf, size_t len)
                    : model == 2 ? kORdNames[i] : model == 3 ? kAPNames[i]
                    : model == 4 ? kFDANames[i] : model == 5 ? kCRNNames[i]
                    : model == 6 ? kPYSNames[i] : model == 7 ? kSTWNames[i]
-                   : model == 8 ? kGRDNames[i] : kTORNames[i];
+                   : model == 8 ? kGRDNames[i] : model == 9 ? kTORNames[i]
+                   : kGAURNames[i];
     snprintf(buf, len, "%s", nm);
     return CG_OK;
 }
@@ -553,6 +564,14 @@ int cg_ionic_set_integrator(cg_ionic* h, int kind, double abs_tol, double rel_to
         (!(abs_tol >= 0.0) || !(rel_tol >= 0.0) || !(abs_tol + rel_tol > 0.0) || max_steps < 1))
         return fail(CG_ERR_ARG, "%s needs absTol, relTol >= 0 (not both 0) and maxSteps >= 1",
                     kind == CG_ODE_EULER ? "Euler" : "RKF45");
+    // Gaur's FP64 tissue-mode RKF45 kernel (29 states, every stage array live) returns
+    // non-finite states on physiological inputs in the gfx950 build, while the same source
+    // compiled for the host (also under address and undefined-behaviour sanitizers) and the
+    // FP32 and single-cell FP64 kernels agree with the oracle; the result depends on how
+    // that one kernel is compiled, so it stays switched off until the cause is known.
+    if (kind == CG_ODE_RKF45 && h->model == cg::MODEL_GAUR && h->prec == CG_FP64 && !h->solve_vm)
+        return fail(CG_ERR_STATE, "RKF45 in tissue mode is not available for Gaur on FP64 "
+                                  "handles (use Rush-Larsen, Euler or precision mixed)");
     HIPCHK(hipStreamSynchronize(h->stream));
     if (adaptive && !h->d_step) {
         if (hipMalloc(&h->d_step, sizeof(double) * h->n) != hipSuccess ||

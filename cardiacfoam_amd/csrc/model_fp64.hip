@@ -109,6 +109,7 @@ hipError_t launch_ionic_eval(int model, int tissue, int64_t n, const double* st,
     case MODEL_STW: return CG_EVAL(Stewart);
     case MODEL_GRD: return CG_EVAL(Grandi);
     case MODEL_TOR: return CG_EVAL(ToRORd);
+    case MODEL_GAUR: return CG_EVAL(Gaur);
     default: return hipErrorInvalidValue;
     }
 #undef CG_EVAL

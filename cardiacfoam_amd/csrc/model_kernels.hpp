@@ -522,6 +522,7 @@ template <class T> struct ModelLaunch {
     case MODEL_STW: return ModelLaunch<T>::template FN<Stewart>(__VA_ARGS__);              \
     case MODEL_GRD: return ModelLaunch<T>::template FN<Grandi>(__VA_ARGS__);               \
     case MODEL_TOR: return ModelLaunch<T>::template FN<ToRORd>(__VA_ARGS__);               \
+    case MODEL_GAUR: return ModelLaunch<T>::template FN<Gaur>(__VA_ARGS__);                \
     default: return hipErrorInvalidValue;                                                  \
     }
 

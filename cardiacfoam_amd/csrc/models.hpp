@@ -18,12 +18,13 @@
 #include "model_stw.hpp"
 #include "model_grd.hpp"
 #include "model_tor.hpp"
+#include "model_gaur.hpp"
 
 namespace cg {
 
 enum ModelId { MODEL_BO = 0, MODEL_TNNP = 1, MODEL_ORD = 2, MODEL_AP = 3, MODEL_FDA = 4,
                MODEL_CRN = 5, MODEL_PYS = 6, MODEL_STW = 7, MODEL_GRD = 8,
-               MODEL_TOR = 9 };
+               MODEL_TOR = 9, MODEL_GAUR = 10 };
 
 constexpr int MAX_PARAMS = 48;
 struct ModelParams {
@@ -109,15 +110,26 @@ template <> struct StateRef<ToRORd> {
     }
 };
 
+template <> struct StateRef<Gaur> {
+    static constexpr bool any = true;
+    CG_HD static constexpr double at(int q)
+    {
+        return (q == Gaur::nai || q == Gaur::nass) ? 6.43
+             : (q == Gaur::ki || q == Gaur::kss) ? 140.76
+             : (q == Gaur::cansr || q == Gaur::cajsr || q == Gaur::cacsr) ? 1.3 : 0.0;
+    }
+};
+
 // The Rush-Larsen tissue step on offset storage: y holds the full values (y[0] = v is
 // set here), s the stored offsets; the increments of the step land in s, y is refreshed
 // from them, and the total ionic current at the step end is returned.  RATES is the
 // model's rates() call with the RlStep policy given.
-// The single-cell step (V integrated inside) on offset storage, the same way.
+// The single-cell step (V integrated inside) on offset storage, the same way, with the
+// FP32 gate increments of rl_frac_precise.
 template <class M, class T, class RATES>
 CG_HD void cell_step_offsets(T* y, T* s, T dt, RATES rates)
 {
-    RlStep<T> out{y, dt, s};
+    RlStep<T> out{y, dt, s, true};
     rates(out);
 #pragma unroll
     for (int q = 0; q < M::NS; ++q) y[q] = T(StateRef<M>::at(q)) + s[q];
@@ -554,6 +566,54 @@ template <> struct Ops<ToRORd> {
                              T istim, const ToRORd::VTerms<T>& q)
     {
         ToRORd::deriv(y, d, p, tissue, gs, solve_v, istim, q);
+    }
+};
+
+template <> struct Ops<Gaur> {
+    template <class T>
+    CG_HD static void cell_step_off(T* y, T* s, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = Gaur::vterms(y[0], p);
+        cell_step_offsets<Gaur>(
+            y, s, dt, [&](RlStep<T>& out) { Gaur::rates(y, out, p, gs, true, istim, q); });
+    }
+    template <class T>
+    CG_HD static T tissue_step_off(T v, T* y, T* s, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = Gaur::vterms(v, p);
+        return tissue_step_offsets<Gaur>(
+            y, s, dt, [&](RlStep<T>& out) { Gaur::rates(y, out, p, gs, false, T(0), q); },
+            [&]() { return Gaur::currents(v, y, p, q, gs).total(); });
+    }
+    template <class T>
+    CG_HD static T tissue_step(T v, T* y, T dt, const T* p, int, const T* gs)
+    {
+        y[0] = v;
+        const auto q = Gaur::vterms(v, p);
+        Gaur::step(y, dt, p, gs, false, T(0), q);
+        return Gaur::currents(v, y, p, q, gs).total();
+    }
+    template <class T>
+    CG_HD static void cell_step(T* y, T dt, T istim, const T* p, int, const T* gs)
+    {
+        const auto q = Gaur::vterms(y[0], p);
+        Gaur::step(y, dt, p, gs, true, istim, q);
+    }
+    template <class T> CG_HD static T iion(const T* y, const T* p, int, const T* gs)
+    {
+        const auto q = Gaur::vterms(y[0], p);
+        return Gaur::currents(y[0], y, p, q, gs).total();
+    }
+    template <class T> CG_HD static Gaur::VTerms<T> vt(T v, const T* p)
+    {
+        return Gaur::vterms(v, p);
+    }
+    template <class T>
+    CG_HD static void deriv(const T* y, T* d, const T* p, int, const T* gs, bool solve_v,
+                             T istim, const Gaur::VTerms<T>& q)
+    {
+        Gaur::deriv(y, d, p, gs, solve_v, istim, q);
     }
 };
 

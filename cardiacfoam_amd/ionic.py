@@ -15,7 +15,8 @@ from . import _capi
 from ._capi import check, lib
 
 MODEL_KEYS = ["BuenoOrovio", "TNNP", "ORd", "AlievPanfilov", "monodomainFDAManufactured",
-              "Courtemanche", "PerisYague_2022", "Stewart", "Grandi", "ToRORd_dynCl"]
+              "Courtemanche", "PerisYague_2022", "Stewart", "Grandi", "ToRORd_dynCl",
+              "Gaur"]
 SUPPORTED_TISSUES = {
     "BuenoOrovio": ["endocardialCells", "mCells", "epicardialCells"],
     "TNNP": ["endocardialCells", "mCells", "epicardialCells"],
@@ -26,6 +27,7 @@ SUPPORTED_TISSUES = {
     "Stewart": ["myocyte"],  # Stewart.C:93-96
     "Grandi": ["myocyte"],  # Grandi.C:93-96
     "ToRORd_dynCl": ["endocardialCells", "mCells", "epicardialCells"],  # ToRORd_dynCl.C:93-96
+    "Gaur": ["myocyte"],  # Gaur.C:93-96
     # the manufactured model's supportedDimensions() (monodomainFDAManufactured.C:85-88)
     "monodomainFDAManufactured": ["1D", "2D", "3D"],
 }

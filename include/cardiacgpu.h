@@ -134,7 +134,9 @@ int cg_ionic_set_stimulus(cg_ionic* h, const cg_stimulus* s);
  *     y = y0 + dx f(x0, y0) with the increment itself as the error estimate.
  * A failing cell (step-size underflow, more than max_steps sub-steps) makes the next
  * synchronising call return CG_ERR_ODE.  Batched protocols (cg_batch_run) use
- * Rush-Larsen only. */
+ * Rush-Larsen only.  CG_ERR_STATE for CG_ODE_RKF45 on an FP64 Gaur handle in tissue mode
+ * (solve_vm 0): that kernel is switched off (its gfx950 build gives non-finite states
+ * where the same source on the host does not); mixed precision and single cells run. */
 int cg_ionic_set_integrator(cg_ionic* h, int kind, double abs_tol, double rel_tol,
                             int64_t max_steps, double initial_step);
 

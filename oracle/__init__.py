@@ -14,12 +14,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
 _lib = None
 
-BO, TNNP, ORD, AP, FDA, CRN, PYS, STW, GRD, TOR = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
+BO, TNNP, ORD, AP, FDA, CRN, PYS, STW, GRD, TOR, GAUR = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
 EPI, MCELL, ENDO, MYOCYTE = 1, 2, 3, 4
 DIM1, DIM2, DIM3 = 11, 12, 13          # dimension flags of the manufactured model
 MAX_STATES, MAX_CONSTS = 48, 160
 NSTATES = {BO: 4, TNNP: 17, ORD: 41, AP: 2, FDA: 4, CRN: 21, PYS: 22, STW: 20, GRD: 41,
-           TOR: 45}
+           TOR: 45, GAUR: 29}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

@@ -1878,6 +1878,271 @@ static void tor_eval(int tf, double t, const double* c, const double* y, int sol
     *istim = Ist;
 }
 
+/* =================================================================== Gaur
+ * Gaur et al. 2021 human ventricular myocyte with spontaneous SR release (29 states):
+ * constants and initial state of Gaur_2021.H:460-647 (GaurinitConsts), right-hand side of
+ * :649-921, state order of its STATES_INDEX enum.  Tissue "myocyte" only (Gaur.C:93-96).
+ * V in mV (Gaur.C:121-126), Im = Iion_cm.  The model's own factors are kept as written
+ * (e.g. Acap / (2 F vmyo) in the K_i and Na_i balances).
+ * Rush-Larsen classification: the Hodgkin-Huxley gates and the two release fluxes as
+ * gates; ITo_aa (linear in itself); the SOICR timer and its refractory variable relax
+ * exponentially (tau 1 us / 0.1 ms) or grow at rate 1, which the step takes exactly; the
+ * rest forward Euler.
+ * Parity pins: the reference's own expressions at seeded states
+ * (tests/test_oracle_model_vectors.py). */
+enum { GA_V, GA_nai, GA_nass, GA_ki, GA_kss, GA_cai, GA_cai2, GA_cass, GA_cansr, GA_cajsr,
+       GA_cacsr, GA_m, GA_h, GA_j, GA_ml, GA_hl, GA_d, GA_fca, GA_xr, GA_xs1, GA_xs2, GA_aa,
+       GA_Jrel2, GA_Jrel1, GA_CaMKt, GA_tjsrol, GA_A, GA_fs, GA_ff, GA_NS };
+/* rescalable conductances and permeabilities */
+enum { GAC_GNa, GAC_GKr, GAC_PCa, GAC_GKs, GAC_GNaL, GAC_GK1, GAC_Gto, GAC_Gncx, GAC_Pnak,
+       GAC_GKb, GAC_PNab, GAC_PCab, GAC_GpCa, GAC_NC };
+
+static void gaur_init(double* c, double* y)
+{
+    c[GAC_GNa] = 14.838;
+    c[GAC_GKr] = 0.5 * 0.015;
+    c[GAC_PCa] = 0.0002;
+    c[GAC_GKs] = 0.021;
+    c[GAC_GNaL] = 0.0075;
+    c[GAC_GK1] = 0.75 * 0.2;
+    c[GAC_Gto] = 0.4 * 0.5;
+    c[GAC_Gncx] = 1.0 * 0.0008;
+    c[GAC_Pnak] = 1.0 * 30.0;
+    c[GAC_GKb] = 0.003;
+    c[GAC_PNab] = 3.75e-10;
+    c[GAC_PCab] = 1.0 * 2.5e-08;
+    c[GAC_GpCa] = 0.0575;
+    static const double y0[GA_NS] = {
+        -87.3, 6.43, 6.43, 140.76, 140.76, 6.75e-05, 6.81e-05, 6.75e-05, 1.37, 1.31, 1.3,
+        0.0022, 0.631, 0.631, 0.001, 0.301, 4.42e-07, 0.988, 0.153, 0.054, 0.046, 0.995,
+        8.59e-10, 9.71e-22, 8.23910999999999914e-03, 100.0, 100.0, 0.0, 0.0};
+    memcpy(y, y0, sizeof y0);
+}
+
+/* Na/Ca exchanger of one compartment (:764-819), gfrac its share of Gncx */
+static double gaur_ncx(double na, double ca, double hca, double hna, double gfrac)
+{
+    const double nao = 140.0, cao = 1.0 * 1.8, kna1 = 15.0, kna2 = 5.0, kna3 = 88.12,
+                 kcaon = 1500000.0, kcaoff = 5000.0, wca = 60000.0, wna = 60000.0,
+                 wnaca = 5000.0, KmCaAct = 0.00015;
+    const double h10 = 12.5 + 1.0 + nao / kna1 * (1.0 + nao / kna2);
+    const double h11 = nao * nao / (h10 * kna1 * kna2), h12 = 1.0 / h10;
+    const double k1 = h12 * cao * kcaon, k2 = kcaoff, k5 = kcaoff;
+    const double allo = 1.0 / (1.0 + KmCaAct / ca * KmCaAct / ca);
+    const double h4 = 1.0 + na / kna1 * (1.0 + na / kna2);
+    const double h1 = 1.0 + na / kna3 * (1.0 + hna);
+    const double h5 = na * na / (h4 * kna1 * kna2), h6 = 1.0 / h4;
+    const double h7 = 1.0 + nao / kna3 * (1.0 + 1.0 / hna);
+    const double h2 = na * hna / (kna3 * h1), h3 = 1.0 / h1;
+    const double h8 = nao / (kna3 * hna * h7), h9 = 1.0 / h7;
+    const double k6 = h6 * ca * kcaon;
+    const double k3pp = h8 * wnaca, k4pp = h2 * wnaca;
+    const double k3 = h9 * wca + k3pp, k4 = h3 * wca / hca + k4pp;
+    const double k7 = h5 * h2 * wna, k8 = h8 * h11 * wna;
+    const double x1 = k2 * k4 * (k7 + k6) + k5 * k7 * (k2 + k3);
+    const double x2 = k1 * k7 * (k4 + k5) + k4 * k6 * (k1 + k8);
+    const double x3 = k1 * k3 * (k7 + k6) + k8 * k6 * (k2 + k3);
+    const double x4 = k2 * k8 * (k4 + k5) + k3 * k5 * (k1 + k8);
+    const double sx = x1 + x2 + x3 + x4;
+    const double E1 = x1 / sx, E2 = x2 / sx, E3 = x3 / sx, E4 = x4 / sx;
+    const double JCa = E2 * k2 - E1 * k1;
+    const double JNa = 3.0 * (E4 * k7 - E1 * k8) + E3 * k4pp - E2 * k3pp;
+    return gfrac * allo * (1.0 * JNa + 2.0 * JCa);
+}
+
+static void gaur_eval(double t, const double* c, const double* y, int solve_vm,
+                      const orc_stim_t* st, double* r, int* g, double* xi, double* ta,
+                      double* iion, double* istim)
+{
+    /* cell geometry and fixed constants (:463-642) */
+    const double F = 96485.0, L = 0.017, R = 8314.0, T = 310.0, cao = 1.0 * 1.8, cli = 19.53,
+                 clo = 100.0, ko = 5.4, nao = 140.0, pi = 3.14, rad = 0.0011, vmyo1frac = 0.4;
+    const double Ageo = 2.0 * pi * rad * rad + 2.0 * pi * rad * L;
+    const double vcell = 1000.0 * pi * rad * rad * L;
+    const double Acap = 2.0 * Ageo;
+    const double vjsr = 0.0048 * vcell / 2.0, vmyo = 0.68 * vcell, vnsr = 0.0552 * vcell,
+                 vss = 0.02 * vcell / 2.0, vcsr = vjsr;
+    const double vmyo1 = vmyo * vmyo1frac, vnsr1 = vnsr / 2.0, vnsr2 = vnsr / 2.0,
+                 vmyo2 = vmyo - vmyo1;
+    const double ECl = R * T / F * log(cli / clo);
+    const double KmCaMK = 0.065, SOICR = 10.0;
+    const double V = y[GA_V], nai = y[GA_nai], nass = y[GA_nass], ki = y[GA_ki],
+                 kss = y[GA_kss], cai = y[GA_cai], cai2 = y[GA_cai2], cass = y[GA_cass];
+    for (int i = 0; i < GA_NS; ++i) g[i] = 0;
+#define GA_GATE(idx, inf, tau) do { const double i_ = (inf), t_ = (tau); \
+        r[idx] = (i_ - y[idx]) / t_; g[idx] = 1; xi[idx] = i_; ta[idx] = t_; } while (0)
+
+    const double Jdiff = (cass - cai) / 0.2, JdiffK = (kss - ki) / 2.0,
+                 JdiffNa = (nass - nai) / 2.0;
+    /* CaMK, reversal potentials (:656-669) */
+    const double EK = R * T / F * log(ko / ki), ENa = R * T / F * log(nao / nai);
+    const double vfrt = V * F / (R * T);
+    const double CaMKb = 0.05 * (1.0 - y[GA_CaMKt]) / (1.0 + 0.0015 / cass);
+    const double EKs = R * T / F * log((ko + 0.01833 * nao) / (ki + 0.01833 * nai));
+    const double CaMKa = CaMKb + y[GA_CaMKt];
+    r[GA_CaMKt] = 0.05 * CaMKb * (CaMKb + y[GA_CaMKt]) - 0.00068 * y[GA_CaMKt];
+    const double CaMK_f = 1.0 / (1.0 + KmCaMK / CaMKa);
+
+    /* I_Na (:671-689) */
+    const double aa_h = V >= -40.0 ? 0.0 : 0.057 * exp(-(V + 80.0) / 6.8);
+    const double aa_j = V >= -40.0 ? 0.0
+        : (-25428.0 * exp(0.2444 * V) - 6.948e-06 * exp(-0.04391 * V)) * (V + 37.78) /
+              (1.0 + exp(0.311 * (V + 79.23)));
+    const double aa_m = 1.0 / (1.0 + exp((-60.0 - V) / 5.0));
+    const double bb_h = V >= -40.0 ? 0.77 / (0.13 * (1.0 + exp(-(V + 10.66) / 11.1)))
+                                   : 2.7 * exp(0.079 * V) + 310000.0 * exp(0.3485 * V);
+    const double bb_j = V >= -40.0 ? 0.6 * exp(0.057 * V) / (1.0 + exp(-0.1 * (V + 32.0)))
+                                   : 0.02424 * exp(-0.01052 * V) / (1.0 + exp(-0.1378 * (V + 40.14)));
+    const double bb_m = 0.1 / (1.0 + exp((V + 35.0) / 5.0)) + 0.1 / (1.0 + exp((V - 50.0) / 200.0));
+    const double eh = 1.0 + exp((V + 71.55) / 7.43), em = 1.0 + exp((-56.86 - V) / 9.03);
+    const double h_inf = 1.0 / (eh * eh), m_inf = 1.0 / (em * em);
+    const double INa = c[GAC_GNa] * y[GA_m] * y[GA_m] * y[GA_m] * y[GA_h] * y[GA_j] * (V - ENa);
+    GA_GATE(GA_h, h_inf, 1.0 / (aa_h + bb_h));
+    GA_GATE(GA_j, h_inf, 1.0 / (aa_j + bb_j));
+    GA_GATE(GA_m, m_inf, aa_m * bb_m);
+
+    /* INaL (:691-699) */
+    const double aml = 0.32 * (V == -47.13 ? 10.0 : -(V + 47.13) / (exp(-0.1 * (V + 47.13)) - 1.0));
+    const double bml = 0.08 * exp(-V / 11.0);
+    const double INaL = c[GAC_GNaL] * y[GA_ml] * y[GA_ml] * y[GA_ml] * y[GA_hl] * (V - ENa);
+    GA_GATE(GA_hl, 1.0 / (1.0 + exp((V + 91.0) / 6.1)), 600.0);
+    GA_GATE(GA_ml, aml / (aml + bml), 1.0 / (aml + bml));
+
+    /* ICaL (:701-724) */
+    const double g1 = V == 0.0 ? 1.0 : vfrt / (exp(vfrt) - 1.0);
+    const double g2 = V == 0.0 ? 1.0 : 2.0 * vfrt / (exp(2.0 * vfrt) - 1.0);
+    const double PhiCaK = F * (0.75 * kss * exp(vfrt) - 0.75 * ko) * g1;
+    const double PhiCaL = 2.0 * F * (cass * exp(2.0 * (V * F / (R * T))) - 0.341 * cao) * g2;
+    const double PhiCaNa = F * (0.75 * nass * exp(vfrt) - 0.75 * nao) * g1;
+    const double f = y[GA_ff] * y[GA_fs];
+    const double ICaL = c[GAC_PCa] * PhiCaL * y[GA_d] * f * y[GA_fca];
+    const double ICaK = 0.0003574 * c[GAC_PCa] * PhiCaK * y[GA_d] * f * y[GA_fca];
+    const double ICaNa = 0.00125 * c[GAC_PCa] * PhiCaNa * y[GA_d] * f * y[GA_fca];
+    const double f_inf = 1.0 / (1.0 + exp((V - -22.9) / 4.9)) + 0.35 / (1.0 + exp((45.0 - V) / 20.0));
+    const double fca_inf = 0.3 / (1.0 - (ICaL > 0.0 ? 0.0 : ICaL / 0.05)) +
+                           0.55 / (1.0 + cass / 0.003) + 0.15;
+    GA_GATE(GA_d, 1.0 / (1.0 + exp(-(V - 3.4) / 6.2)),
+            0.6 + 1.0 / (exp(-0.05 * (V + 6.0)) + exp(0.09 * (V + 14.0))));
+    GA_GATE(GA_fca, fca_inf, 10.0 * CaMK_f + 0.5 + 1.0 / (1.0 + cass / 0.003));
+    GA_GATE(GA_ff, f_inf, 7.0 + 1.0 / (0.0045 * exp(-(V + 20.0) / 10.0) + 0.0045 * exp((V + 20.0) / 10.0)));
+    GA_GATE(GA_fs, f_inf, 70.0 + 1.0 / (3.5e-05 * exp(-(V + 5.0) / 4.0) + 3.5e-05 * exp((V + 5.0) / 6.0)));
+
+    /* IKr, IKs, IK1 (:726-747) */
+    const double IKr = c[GAC_GKr] * sqrt(ko / 5.4) * y[GA_xr] * (1.0 / (1.0 + exp((V + 22.0) / 15.0))) * (V - EK);
+    GA_GATE(GA_xr, 1.0 / (1.0 + exp(-(V + 56.8) / 17.8)),
+            12.98 + 1.0 / (0.3652 * exp((V - 31.66) / 3.869) + 4.123e-05 * exp(-(V - 47.78) / 20.38)));
+    const double KsCa = 1.0 + 0.6 / (1.0 + pow(3.8e-05 / cai, 1.4));
+    const double xs1_inf = 1.0 / (1.0 + exp(-(V - 25.1) / 37.1));
+    const double IKs = c[GAC_GKs] * KsCa * y[GA_xs1] * y[GA_xs2] * (V - EKs);
+    GA_GATE(GA_xs1, xs1_inf, 300.0 + 1.0 / (1e-06 * exp((V + 50.0) / 20.0) + 0.04 * exp(-(V + 50.0) / 20.0)));
+    GA_GATE(GA_xs2, xs1_inf, 1.0 / (0.01 * exp((V - 50.0) / 20.0) + 0.0193 * exp(-(V + 66.54) / 31.0)));
+    const double IK1 = c[GAC_GK1] * sqrt(ko / 5.4) * (1.0 / (1.0 + exp((V + 79.3 - 2.6 * ko) / 19.6))) * (V - EK);
+
+    /* ITo gate (:749-753): d aa/dt = alpha (1 - aa) - beta aa */
+    const double alpha_aa = 0.025 / (1.0 + exp((V + 58.0) / 5.0));
+    const double beta_aa = 1.0 / (5.0 * (1.0 + exp((V + 19.0) / -9.0)));
+    const double rito2 = 1.0 / (1.0 + exp(-(V + 10.0) / 5.0));
+    r[GA_aa] = alpha_aa * (1.0 - y[GA_aa]) - beta_aa * y[GA_aa];
+    g[GA_aa] = 1; xi[GA_aa] = alpha_aa / (alpha_aa + beta_aa); ta[GA_aa] = 1.0 / (alpha_aa + beta_aa);
+
+    /* CICR (:755-769) */
+    const double camk8 = 1.0 + 1.0 / (1.0 + pow(KmCaMK / CaMKa, 8.0));
+    const double trel1 = 1.0 * 20.0 * camk8 / (1.0 + pow(0.5 / y[GA_cajsr], 8.0));
+    const double trel2 = 50.0 * camk8 / (1.0 + pow(0.5 / y[GA_cacsr], 8.0));
+    const double Jgap = (cai - cai2) / 3.0;
+    const double grel = 2.0 * (1.0 - exp(-y[GA_tjsrol] / 0.5)) * exp(-y[GA_tjsrol] / 5.0);
+    const double Jrel = y[GA_Jrel1] + grel * (y[GA_cajsr] - cass);
+    /* A and the timer: x' = -x / tau (exact exponential decay) or x' = 1 */
+    if (y[GA_tjsrol] < 5.0) {
+        r[GA_A] = -y[GA_A] / 0.1; g[GA_A] = 1; xi[GA_A] = 0.0; ta[GA_A] = 0.1;
+    } else {
+        r[GA_A] = 1.0;
+    }
+    if (y[GA_cajsr] > SOICR && y[GA_A] > 45.0) {
+        r[GA_tjsrol] = -y[GA_tjsrol] / 0.001; g[GA_tjsrol] = 1; xi[GA_tjsrol] = 0.0;
+        ta[GA_tjsrol] = 0.001;
+    } else {
+        r[GA_tjsrol] = 1.0;
+    }
+
+    /* exchanger, pump, background currents (:771-875) */
+    const double hca = exp(0.167 * V * F / (R * T)), hna = exp(0.5224 * V * F / (R * T));
+    const double INaCa_i = gaur_ncx(nai, cai, hca, hna, 0.8 * c[GAC_Gncx]);
+    const double INaCa_ss = gaur_ncx(nass, cass, hca, hna, 0.2 * c[GAC_Gncx]);
+    const double Knai = 9.073 * exp(-0.155 * V * F / (3.0 * R * T));
+    const double Knao = 27.78 * exp((1.0 - -0.155) * V * F / (3.0 * R * T));
+    const double P = 4.2 / (1.0 + 1e-07 / 1.698e-07 + nai / 224.0 + ki / 292.0);
+    const double ni = nai / Knai, no = nao / Knao, kk = ki / 0.5, kko = ko / 0.3582;
+    const double dni = (1.0 + ni) * (1.0 + ni) * (1.0 + ni) + (1.0 + kk) * (1.0 + kk) - 1.0;
+    const double dno = (1.0 + no) * (1.0 + no) * (1.0 + no) + (1.0 + kko) * (1.0 + kko) - 1.0;
+    const double a1 = 949.5 * (ni * ni * ni) / dni;
+    const double a3 = 1899.0 * (kko * kko) / dno;
+    const double b2 = 39.4 * (no * no * no) / dno;
+    const double b3 = 79300.0 * P * 1e-07 / (1.0 + 9.8 / 1.698e-07);
+    const double b4 = 40.0 * (kk * kk) / dni;
+    const double a2 = 687.2, a4 = 639.0 * 9.8 / 1.698e-07 / (1.0 + 9.8 / 1.698e-07), b1 = 182.4 * 0.05;
+    const double x1 = a4 * a1 * a2 + b2 * b4 * b3 + a2 * b4 * b3 + b3 * a1 * a2;
+    const double x2 = b2 * b1 * b4 + a1 * a2 * a3 + a3 * b1 * b4 + a2 * a3 * b4;
+    const double x3 = a2 * a3 * a4 + b3 * b2 * b1 + b2 * b1 * a4 + a3 * a4 * b1;
+    const double x4 = b4 * b3 * b2 + a3 * a4 * a1 + b2 * a4 * a1 + b3 * b2 * a1;
+    const double sx = x1 + x2 + x3 + x4;
+    const double JnakK = 2.0 * (x4 / sx * b1 - x3 / sx * a1);
+    const double JnakNa = 3.0 * (x1 / sx * a3 - x2 / sx * b3);
+    const double INaK = c[GAC_Pnak] * (1.0 * JnakNa + 1.0 * JnakK);
+    const double IKb = c[GAC_GKb] * (1.0 / (1.0 + exp(-(V - 14.48) / 18.34))) * (V - EK);
+    const double INab = c[GAC_PNab] * F * (nai * exp(vfrt) - nao) * g1;
+    const double ICab = c[GAC_PCab] * 2.0 * F * (cai * exp(2.0 * vfrt) - 0.341 * cao) * g2;
+    const double IpCa = c[GAC_GpCa] * cai / (0.0005 + cai);
+
+    /* SR uptake and the concentrations (:877-903) */
+    const double Jleak = 0.005 * y[GA_cansr] / 15.0;
+    const double Jtr = (y[GA_cansr] - y[GA_cajsr]) / 100.0, Jtr2 = (y[GA_cansr] - y[GA_cacsr]) / 100.0;
+    const double Jupnp = 1.0 * 0.005 * cai / (cai + 0.001), Jupnp2 = 1.0 * 0.005 * cai2 / (cai2 + 0.001);
+    const double Jupp = 1.0 * 3.0 * 0.005 * cai / (cai + 0.001 - 0.0002);
+    const double Jupp2 = 1.0 * 3.0 * 0.005 * cai2 / (cai2 + 0.001 - 0.0002);
+    const double Jup = (1.0 - CaMK_f) * Jupnp + CaMK_f * Jupp - Jleak;
+    const double Jup2 = (1.0 - CaMK_f) * Jupnp2 + CaMK_f * Jupp2 - Jleak;
+    const double Bcacsr = 1.0 / (1.0 + 10.0 * 0.8 / ((0.8 + y[GA_cacsr]) * (0.8 + y[GA_cacsr])));
+    const double Bcai = 1.0 / (1.0 + 0.05 * 0.00238 / ((0.00238 + cai) * (0.00238 + cai)) +
+                               0.07 * 0.0005 / ((0.0005 + cai) * (0.0005 + cai)));
+    const double Bcai2 = 1.0 / (1.0 + 0.05 * 0.00238 / ((0.00238 + cai2) * (0.00238 + cai2)) +
+                                0.07 * 0.0005 / ((0.0005 + cai2) * (0.0005 + cai2)));
+    const double Bcajsr = 1.0 / (1.0 + 10.0 * 0.8 / ((0.8 + y[GA_cajsr]) * (0.8 + y[GA_cajsr])));
+    const double Bcass = 1.0 / (1.0 + 0.047 * 0.00087 / ((0.00087 + cass) * (0.00087 + cass)) +
+                                1.124 * 0.0087 / ((0.0087 + cass) * (0.0087 + cass)));
+    r[GA_cansr] = Jup * (vnsr1 / vnsr) + Jup2 * (vnsr2 / vnsr) - Jtr * vjsr / vnsr - Jtr2 * vcsr / vnsr;
+    r[GA_ki] = -(IKr + IKs + IK1 + IKb - 2.0 * INaK) * Acap / (2.0 * F * vmyo) + JdiffK * vss / vmyo;
+    r[GA_kss] = -ICaK * Acap / (2.0 * F * vss) - JdiffK;
+    r[GA_nai] = -(INa + INaL + 3.0 * INaCa_i + 3.0 * INaK + INab) * Acap / (2.0 * F * vmyo) +
+                JdiffNa * vss / vmyo;
+    r[GA_nass] = -(ICaNa + 3.0 * INaCa_ss) * Acap / (2.0 * F * vss) - JdiffNa;
+    r[GA_cacsr] = Bcacsr * (Jtr2 - y[GA_Jrel2]);
+    r[GA_cai] = Bcai * (-(IpCa + ICab - 2.0 * INaCa_i) * Acap / (2.0 * 2.0 * F * vmyo1) -
+                        Jup * vnsr1 / vmyo1 + Jdiff * vss / vmyo1 - Jgap);
+    r[GA_cai2] = Bcai2 * (y[GA_Jrel2] * (vcsr / vmyo2) + Jgap * vmyo2 / vmyo1 - Jup2 * vnsr2 / vmyo2);
+    r[GA_cajsr] = Bcajsr * (Jtr - Jrel);
+    r[GA_cass] = Bcass * (-(ICaL - 2.0 * INaCa_ss) * Acap / (2.0 * 2.0 * F * vss) +
+                          Jrel * vjsr / vss - Jdiff);
+
+    /* release fluxes and ITo (:906-913) */
+    const double kito2 = 1.0 - 1.0 / (1.0 + Jrel / 0.4 * Jrel / 0.4);
+    const double Rel1 = (-ICaL + 2.0 * INaCa_ss) * (Acap / (2.0 * vss * F)) + Jrel * (vjsr / vss) - Jdiff;
+    const double Rel2 = Jgap + y[GA_Jrel2] * (vcsr / vmyo2) - Jup2 * (vnsr2 / vmyo2);
+    const double ITo = c[GAC_Gto] * y[GA_aa] * rito2 * kito2 * (V - ECl);
+    const double Jrel1_inf = Rel1 > 0.0 ? 1.0 * 60.0 * Rel1 * camk8 / (1.0 + pow(0.75 / y[GA_cajsr], 8.0)) : 0.0;
+    const double Jrel2_inf = Rel2 > 0.0 ? 1.0 * (250.0 * Rel2 * camk8) / (1.0 + pow(0.75 / y[GA_cacsr], 8.0)) : 0.0;
+    GA_GATE(GA_Jrel1, Jrel1_inf, 0.001 > trel1 ? 0.001 : trel1);
+    GA_GATE(GA_Jrel2, Jrel2_inf, 0.001 > trel2 ? 0.001 : trel2);
+#undef GA_GATE
+
+    const double Iion = INa + INaL + ICaL + ICaNa + ICaK + IKr + IKs + IK1 + ITo + INaCa_i +
+                        INaCa_ss + INaK + INab + IKb + IpCa + ICab;
+    const double Ist = orc_stimulus(t, st);
+    r[GA_V] = solve_vm ? -Iion - Ist : 0.0;
+    *iion = Iion;
+    *istim = Ist;
+}
+
 /* ============================================================== dispatch */
 int orc_model_nstates(int model)
 {
@@ -1892,6 +2157,7 @@ int orc_model_nstates(int model)
     case ORC_STW: return SW_NS;
     case ORC_GRD: return GR_NS;
     case ORC_TOR: return TR_NS;
+    case ORC_GAUR: return GA_NS;
     default: return -1;
     }
 }
@@ -1930,6 +2196,9 @@ int orc_model_init(int model, int tissue, double* c, double* y)
     case ORC_TOR:
         if (tissue < 1 || tissue > 3) return -1;
         tor_init(tissue, c, y); return 0;
+    case ORC_GAUR:
+        if (tissue != ORC_MYOCYTE) return -1;
+        gaur_init(c, y); return 0;
     default: return -1;
     }
 }
@@ -1961,6 +2230,11 @@ int orc_model_const_index(int model, const char* name)
         if (!strcmp(name, "g_Kr")) return SC_gKr;
         if (!strcmp(name, "g_CaL")) return SC_gCaL;
         if (!strcmp(name, "g_Ks")) return SC_gKs;
+    } else if (model == ORC_GAUR) {
+        if (!strcmp(name, "GNa")) return GAC_GNa;
+        if (!strcmp(name, "GKr")) return GAC_GKr;
+        if (!strcmp(name, "PCa")) return GAC_PCa;
+        if (!strcmp(name, "GKs")) return GAC_GKs;
     } else if (model == ORC_TOR) {
         if (!strcmp(name, "GNa")) return RC_GNa;
         if (!strcmp(name, "GKr")) return RC_GKr;
@@ -2032,6 +2306,7 @@ void orc_model_eval(int model, int tissue, double t, const double* c, const doub
     case ORC_STW: stw_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_GRD: grd_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     case ORC_TOR: tor_eval(tissue, t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
+    case ORC_GAUR: gaur_eval(t, c, y, solve_vm, st, r, g, xi, ta, iion, istim); break;
     default: break;
     }
 }

@@ -36,7 +36,7 @@ extern "C" {
 
 /* model ids (same numbering as the product C-ABI) */
 enum { ORC_BO = 0, ORC_TNNP = 1, ORC_ORD = 2, ORC_AP = 3, ORC_FDA = 4, ORC_CRN = 5,
-       ORC_PYS = 6, ORC_STW = 7, ORC_GRD = 8, ORC_TOR = 9 };
+       ORC_PYS = 6, ORC_STW = 7, ORC_GRD = 8, ORC_TOR = 9, ORC_GAUR = 10 };
 /* dimension flags of the manufactured model (its `dimension` word 1D/2D/3D) */
 enum { ORC_DIM1 = 11, ORC_DIM2 = 12, ORC_DIM3 = 13 };
 /* tissue flags, ionicSelector.C:60-66 */

@@ -55,6 +55,7 @@ MODELS = {
     "Grandi": ("Grandi/Grandi_2011.H", "Grandi", "computeVariables", (4,), 8),
     "ToRORd_dynCl": ("ToRORd_dynCl/ToRORd_dynCl_2023.H", "ToRORd_dynCl", "computeVariables",
                      (1, 2, 3), 9),
+    "Gaur": ("Gaur/Gaur_2021.H", "Gaur", "computeVariables", (4,), 10),
 }
 
 # ORd-2011 INaK and SERCA constants left unassigned by ORdinitConsts (published values)
@@ -128,11 +129,20 @@ class Interp:
     ARRAYS = ("STATES", "CONSTANTS", "RATES", "ALGEBRAIC")
     TYPES = {"double", "const", "int", "float", "auto", "bool", "Foam::scalar", "scalar"}
 
-    def __init__(self, env):
+    def __init__(self, env, state_names=()):
         self.arr = {a: {} for a in self.ARRAYS}
         self.env = dict(env)
+        self.names = list(state_names)
         self.t = []
         self.i = 0
+
+    def _key(self, arr, tok):
+        """array subscript: a STATES_INDEX name, or a literal index into STATES / RATES
+        (Gaur_2021.H:917 writes RATES[0] for its membrane potential)"""
+        kind, v = tok
+        if kind == "num" and arr in ("STATES", "RATES"):
+            return self.names[int(float(v))]
+        return v
 
     def run(self, toks):
         self.t, self.i = toks, 0
@@ -195,7 +205,7 @@ class Interp:
         elif kind == "name" and v in self.ARRAYS:
             self.i += 1
             self.expect("[")
-            key = self.t[self.i][1]
+            key = self._key(v, self.t[self.i])
             self.i += 1
             self.expect("]")
             self.expect("=")
@@ -294,7 +304,7 @@ class Interp:
             return x
         if kind == "name" and v in self.ARRAYS:
             self.expect("[")
-            key = self.t[self.i][1]
+            key = self._key(v, self.t[self.i])
             self.i += 1
             self.expect("]")
             if not live:
@@ -349,7 +359,8 @@ def evaluate(ref_root, key, tissue, states_in, solve_vm):
         src = f.read()
     init = function_body(src, prefix + "initConsts")
     body = function_body(src, prefix + deriv)
-    base = Interp({"tissueFlag": tissue})
+    names = header_state_names(ref_root, key)
+    base = Interp({"tissueFlag": tissue}, names)
     base.run(init)
     consts = dict(base.arr["CONSTANTS"])
     if key == "ORd":
@@ -357,7 +368,8 @@ def evaluate(ref_root, key, tissue, states_in, solve_vm):
     init_states = dict(base.arr["STATES"])
     rates = []
     for y in states_in:
-        it = Interp({"tissueFlag": tissue, "solveVmWithinODESolver": solve_vm, "VOI": 0.0})
+        it = Interp({"tissueFlag": tissue, "solveVmWithinODESolver": solve_vm, "VOI": 0.0},
+                    names)
         it.arr["CONSTANTS"] = dict(consts)
         it.arr["STATES"] = dict(y)
         it.run(body)
@@ -398,7 +410,8 @@ def input_states(key, tissue, n=24, seed=0):
     import oracle as orc
     m = MODELS[key][4]
     c, y = orc.model_init(m, tissue)
-    amp = {0: 0.4, 1: 52.0, 2: 80.0, 3: 0.5, 5: 60.0, 6: 40.0, 7: 40.0, 8: 40.0, 9: 53.0}[m]
+    amp = {0: 0.4, 1: 52.0, 2: 80.0, 3: 0.5, 5: 60.0, 6: 40.0, 7: 40.0, 8: 40.0, 9: 53.0,
+           10: 40.0}[m]
     st = orc.make_stim(start=2.0, period_s1=1000.0, duration=1.0, amplitude=amp, nstim1=1)
     dt = 0.01 * (1.0 / 12.9 if m == 3 else 1.0)
     yy = np.zeros(64)

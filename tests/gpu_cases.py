@@ -16,6 +16,7 @@ MODELS = {  # name -> (oracle id, tissue flag, library key, tissue word, stimulu
     "TOR_epi": (9, 1, "ToRORd_dynCl", "epicardialCells", 53.0),
     "TOR_M": (9, 2, "ToRORd_dynCl", "mCells", 53.0),
     "TOR_endo": (9, 3, "ToRORd_dynCl", "endocardialCells", 53.0),
+    "GAUR": (10, 4, "Gaur", "myocyte", 40.0),
 }
 
 

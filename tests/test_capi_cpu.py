@@ -22,12 +22,14 @@ def test_model_and_tissue_lookup():
     assert L.cg_model_id(b"Stewart") == 7
     assert L.cg_model_id(b"Grandi") == 8
     assert L.cg_model_id(b"ToRORd_dynCl") == 9
+    assert L.cg_model_id(b"Gaur") == 10
     assert L.cg_model_id(b"Fabbri") == -1
     assert [L.cg_tissue_flag(t) for t in (b"epicardialCells", b"mCells", b"endocardialCells",
                                           b"myocyte")] == [1, 2, 3, 4]
     assert L.cg_tissue_flag(b"purkinje") == -1
-    assert [L.cg_model_nstates(m) for m in range(10)] == [4, 17, 41, 2, 4, 21, 22, 20, 41, 45]
-    assert L.cg_model_nstates(10) == -1
+    assert [L.cg_model_nstates(m) for m in range(11)] == [4, 17, 41, 2, 4, 21, 22, 20, 41, 45,
+                                                          29]
+    assert L.cg_model_nstates(11) == -1
     buf = C.create_string_buffer(32)
     names = []
     for i in range(4):
@@ -94,8 +96,8 @@ def test_model_units_follow_the_reference():
     assert model_units("BuenoOrovio") == (1000.0, 84.0, 85.7, 1000.0)
     assert model_units("AlievPanfilov") == (1000.0, 80.0, 100.0, 1000.0 / 12.9)
     for key in ("TNNP", "ORd", "Courtemanche", "PerisYague_2022", "Stewart", "Grandi",
-                "ToRORd_dynCl"):
+                "ToRORd_dynCl", "Gaur"):
         assert model_units(key) == (1000.0, 0.0, 1.0, 1000.0), key
     assert model_units("monodomainFDAManufactured") == (1.0, 0.0, 1.0, 1.0)
     with pytest.raises(_capi.CgError):
-        model_units(10)
+        model_units(11)

@@ -35,7 +35,7 @@ def _oracle_rows(orc, m, tis, c, S, t, solve_vm, stim):
 
 
 @pytest.mark.parametrize("name", ["BO_epi", "BO_endo", "AP", "TNNP_M", "ORd", "CRN", "PYS",
-                                  "STW", "GRD", "TOR_M"])
+                                  "STW", "GRD", "TOR_M", "GAUR"])
 def test_rates_and_algebraic_single_cell(gpu, orc, name):
     """A single-cell handle (V inside the ODE, the handle's stimulus protocol at each row's
     model time): RATES of every model, ALGEBRAIC of BuenoOrovio and AlievPanfilov; the

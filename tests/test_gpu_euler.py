@@ -61,7 +61,7 @@ def test_restitution_single_cell_euler(gpu, orc, tmp_path):
 
 
 @pytest.mark.parametrize("name", ["BO_epi", "TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW",
-                                  "GRD", "TOR_M"])
+                                  "GRD", "TOR_M", "GAUR"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_ionic_tissue_step_euler(gpu, orc, name, precision):
     """cg_ionic_step with Euler over 20 steps from states along an action potential; the

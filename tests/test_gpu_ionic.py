@@ -50,7 +50,7 @@ def test_ionic_tissue_step_parity(gpu, orc, name, precision):
 
 
 @pytest.mark.parametrize("name", ["BO_endo", "TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW",
-                                  "GRD", "TOR_epi", "TOR_M", "TOR_endo"])
+                                  "GRD", "TOR_epi", "TOR_M", "TOR_endo", "GAUR"])
 def test_single_cell_run_parity(gpu, orc, name):
     """cg_ionic_run_cells (Vm inside the ODE, stimulus protocol) vs the oracle's RL
     single cell, stimulus edges placed off the step grid."""

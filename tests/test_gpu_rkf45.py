@@ -62,7 +62,7 @@ def test_single_cell_golden_on_device(gpu, orc):
 
 @pytest.mark.parametrize("tol", [(1e-6, 1e-4, 1e-3), (1e-10, 1e-8, 1e-6)])
 @pytest.mark.parametrize("name", ["TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW", "GRD",
-                                  "TOR_M"])
+                                  "TOR_M", "GAUR"])
 def test_single_cell_rkf45_vs_oracle(gpu, orc, name, tol):
     """cg_ionic_run_cells with RKF45 (stimulus evaluated at each stage time) vs the
     oracle's single-cell RKF45 over one paced beat.  The stimulus switches on inside a
@@ -92,13 +92,19 @@ def test_single_cell_rkf45_vs_oracle(gpu, orc, name, tol):
 
 
 @pytest.mark.parametrize("name", ["BO_epi", "TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW",
-                                  "GRD", "TOR_endo"])
+                                  "GRD", "TOR_endo", "GAUR"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_ionic_tissue_step_rkf45(gpu, orc, name, precision):
     """cg_ionic_step with RKF45 over 20 steps from states along an action potential; the
     per-cell trial steps persist between calls on both sides."""
     _say("rkf45 ionic step", name, precision)
     m, tis, key, word, _ = MODELS[name]
+    if name == "GAUR" and precision == "fp64":
+        ion = gpu.IonicModel(key, word, 4, precision=precision)
+        with pytest.raises(gpu._capi.CgError, match="not available for Gaur"):
+            ion.set_integrator("RKF45", 1e-6, 1e-4, 100000)
+        pytest.skip("Gaur FP64 tissue-mode RKF45 is switched off: its gfx950 kernel gives "
+                    "non-finite states where the same source on the host does not")
     n = 2048
     c, states, Vm = physiological_states(orc, name, n, seed=3)
     ion = gpu.IonicModel(key, word, n, precision=precision)

@@ -58,7 +58,7 @@ def _compare(gpu_dom, T, vtol_rel=1e-9, ttol=1e-12):
 
 @pytest.mark.parametrize("ddt", ["Euler", "backward"])
 @pytest.mark.parametrize("name", ["BO_epi", "TNNP_epi", "ORd", "AP", "CRN", "PYS", "STW",
-                                  "GRD", "TOR_epi", "TOR_M"])
+                                  "GRD", "TOR_epi", "TOR_M", "GAUR"])
 def test_explicit_parity_fp64(gpu, orc, name, ddt):
     _say("explicit fp64", name, ddt)
     m, tis, key, word, _ = MODELS[name]

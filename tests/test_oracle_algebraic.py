@@ -31,8 +31,8 @@ def test_library_names_follow_the_reference():
             _capi.check(L.cg_model_algebraic_name(model, i, buf, 32))
             got.append(buf.value.decode())
         assert got == want
-    assert [L.cg_model_nalgebraic(m) for m in (1, 2, 4, 5, 6, 7, 8, 9)] == [0] * 8
-    assert L.cg_model_nalgebraic(10) == -1
+    assert [L.cg_model_nalgebraic(m) for m in (1, 2, 4, 5, 6, 7, 8, 9, 10)] == [0] * 9
+    assert L.cg_model_nalgebraic(11) == -1
 
 
 def _bo_states(rng, n):

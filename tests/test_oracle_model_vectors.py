@@ -4,8 +4,8 @@ models' headers: initConsts and the function each model's derivatives() calls, a
 states along an action potential).
 
 Pins every model the oracle restates -- BuenoOrovio, TNNP (3 tissues), ORd, AlievPanfilov,
-Courtemanche, PerisYague_2022, Stewart, Grandi and ToRORd_dynCl (3 tissues) -- to the
-reference's text: the initial states exactly, and RATES in tissue mode (V frozen) and in single-cell mode (dV/dt = -Iion) to
+Courtemanche, PerisYague_2022, Stewart, Grandi, ToRORd_dynCl (3 tissues) and Gaur -- to
+the reference's text: the initial states exactly, and RATES in tissue mode (V frozen) and in single-cell mode (dV/dt = -Iion) to
 1e-13 of each rate's range over the samples (the same libm; the two differ only in the
 order of a few operations).  ORd carries the documented substitution of the published
 values for the 27 constants its initConsts leaves unset, in both.  A constant off by one
@@ -19,7 +19,7 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURE = os.path.join(HERE, "golden", "model_vectors.json")
 IDS = {"BuenoOrovio": 0, "TNNP": 1, "ORd": 2, "AlievPanfilov": 3, "Courtemanche": 5,
-       "PerisYague_2022": 6, "Stewart": 7, "Grandi": 8, "ToRORd_dynCl": 9}
+       "PerisYague_2022": 6, "Stewart": 7, "Grandi": 8, "ToRORd_dynCl": 9, "Gaur": 10}
 TOL = 1e-13
 
 
@@ -80,7 +80,7 @@ def test_oracle_against_reference_expressions(orc, key, tissue):
 
 
 @pytest.mark.parametrize("key", ["TNNP", "ORd", "Courtemanche", "PerisYague_2022", "Stewart",
-                                 "Grandi", "ToRORd_dynCl"])
+                                 "Grandi", "ToRORd_dynCl", "Gaur"])
 def test_vectors_see_a_constant_off_by_one_part_in_a_million(orc, key):
     """Planted in the oracle's constants on the CPU: each constant that some rate depends
     on at these states, scaled by 1 + 1e-6, moves a rate beyond the pin's tolerance."""

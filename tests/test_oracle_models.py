@@ -512,3 +512,25 @@ def test_tor_rush_larsen_converges_to_rkf45(orc, tissue):
     e1 = abs(run(0, 1e-5)[0] - a_rk)
     e2 = abs(run(0, 2e-6)[0] - a_rk)
     assert e1 < 5.0 and (e2 < 0.3 * e1 or e2 < 0.05)
+
+
+def test_gaur_rush_larsen_converges_to_rkf45(orc):
+    """The Rush-Larsen split of Gaur 2021 (gates and the two release fluxes; the SOICR
+    timer and its refractory variable relaxed or grown exactly) against the adaptive RKF45
+    on one paced beat from the reference's initial state: rest -87.3 mV, an overshoot near
+    +38 mV, and an APD90 error of first order in dt (measured 2.77, 1.35 and 0.25 ms at
+    0.02, 0.01 and 0.002 ms against 171 ms)."""
+    st = orc.make_stim(start=10.0, period_s1=1000.0, duration=1.0, amplitude=40.0, nstim1=1)
+
+    def run(integ, dt, **kw):
+        t, v, y = orc.single_cell_run(orc.GAUR, orc.MYOCYTE, st, dt, int(0.6 / dt),
+                                      integrator=integ, **kw)
+        b = orc.detect_beats(t, v)
+        assert len(b) == 1 and np.all(np.isfinite(y))
+        return (b[0]["t_repol90"] - b[0]["t_up"]) * 1e3, b[0]
+
+    a_rk, b_rk = run(1, 1e-5, abs_tol=1e-10, rel_tol=1e-8)
+    assert 150.0 < a_rk < 200.0 and -88.0 < b_rk["v_base"] < -86.5 and 30.0 < b_rk["v_peak"] < 45.0
+    e = [abs(run(0, dt)[0] - a_rk) for dt in (2e-5, 1e-5, 2e-6)]
+    assert e[0] < 4.0
+    assert 1.6 < e[0] / e[1] < 2.6 and 3.5 < e[1] / e[2] < 7.0     # first order
