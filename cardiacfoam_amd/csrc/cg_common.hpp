@@ -53,8 +53,8 @@ template <class T> CG_HD T cube(T x) { return x * x * x; }
 // That rounding (absolute 6e-8 near 1) still biases the increments of slow gates
 // (e = dt/tau ~ 1e-5) by up to 0.6 %; a series for small e removes it (measured on the
 // 2.1M-cell ORd run over 200 ms: Vm L-inf against FP64 0.20 -> 0.06 mV) but costs 21 % of
+// the ionic kernel's time, and both stay well inside the 0.5 mV bound, so the plain
 // exponential is kept in the tissue kernels.
-// exponential is kept.
 CG_HD float rl_frac(float e) { return 1.0f - cexp(-e); }
 // The same with FP32 rounding relative to the result: below e = 0.05 the series
 // e (1 - e/2 (1 - e/3 (1 - e/4))), whose truncation is < 6e-8 relative there.  The
