@@ -774,17 +774,13 @@ def run_reference_semantics(cf, wname, w, sigma, cpu_steps, with_cpu, ord_steps=
     c = cases.NIEDERER
     nx, ny, nz = c["shape"]
     b = c["box"]
-    box = {"stimulusLocationMin": b["min"], "stimulusLocationMax": b["max"],
-           "stimulusDuration": b["duration"], "stimulusIntensity": b["intensity"],
-           "stimulusStartTime": 0.0}
-    coeffs = {"ionicModel": "BuenoOrovio", "tissue": "epicardialCells", "chi": c["chi"],
-              "cm": c["cm"], "solutionAlgorithm": "explicit", "conductivity": c["sigma9"],
-              "externalStimulus": box, "solver": "RKF45", "maxSteps": 1000000000}
-    mesh = cf.HexMesh(nx, ny, nz, c["h"])
-    warm = cf.Monodomain(mesh, coeffs, max_co=c["max_co"], ddt="backward")
+    # the case's mesh with the cell centres its blockMesh writes (the box test is a
+    # comparison on those coordinates)
+    from tests.gpu_cases import niederer_domain
+    warm = niederer_domain(cf, solver="RKF45", maxSteps=1000000000)
     warm.step(5)
     warm.sync()
-    dom = cf.Monodomain(mesh, coeffs, max_co=c["max_co"], ddt="backward")
+    dom = niederer_domain(cf, solver="RKF45", maxSteps=1000000000)
     steps = 0
     while steps * dom.dt < c["end_time"] - 0.5 * dom.dt:
         steps += 1
