@@ -794,7 +794,7 @@ def run_reference_semantics(cf, wname, w, sigma, cpu_steps, with_cpu, ord_steps=
         i, j, k = p["cell"]
         e = float(abs(ta[k, j, i] - p["expected"]))
         errs.append(e)
-        ok = ok and e <= p.get("tol_here", p["tol"])
+        ok = ok and e <= p["tol"]
     n = nx * ny * nz
     out = {"niederer_rkf45": {"cells": n, "steps": steps, "dt_s": dom.dt, "gpu_s": el,
                               "cell_steps_per_s": n * steps / el, "probes_within_golden": ok,

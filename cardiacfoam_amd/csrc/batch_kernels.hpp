@@ -194,10 +194,53 @@ template <class T> struct Detector {
     }
 };
 
+// One adaptive step of a paced cell over model times [x0, x1] (RKF45 for INTEG 1,
+// OpenFOAM's Euler for 2), the stimulus evaluated at every stage time.  FP64 integrates
+// the full values y; FP32 integrates the step's increments from 0 against the step-start
+// values (rates and error norm on base + d) and adds them to the stored words once, so
+// increments below half an ulp of a state still count.  Returns false when the solver
+// failed (step-size underflow or more than max_steps sub-steps).
+template <class M, class T, int INTEG>
+CG_DEV bool protocol_adaptive_step(T (&y)[M::NS], T (&sw)[M::NS], double x0, double x1,
+                                   double& step, const BatchArgs& a, const Stimulus& stim,
+                                   const T* p, int tissue, const T* gs)
+{
+    constexpr bool f32 = sizeof(T) == 4;
+    T base[M::NS], d[M::NS];
+    auto f = [&](double t, const T* ss, T* r) __attribute__((always_inline)) {
+        T yy[M::NS];
+#pragma unroll
+        for (int q = 0; q < M::NS; ++q) yy[q] = f32 ? base[q] + ss[q] : ss[q];
+        Ops<M>::deriv(yy, r, p, tissue, gs, true, T(stimulus_current(t, stim)),
+                      Ops<M>::vt(yy[0], p));
+    };
+    bool ok;
+    if constexpr (f32) {
+#pragma unroll
+        for (int q = 0; q < M::NS; ++q) {
+            base[q] = y[q];
+            d[q] = T(0);
+        }
+        ok = ode_solve<INTEG, M::NS>(f, d, x0, x1, step, T(a.abs_tol), T(a.rel_tol),
+                                     a.max_steps, base);
+#pragma unroll
+        for (int q = 0; q < M::NS; ++q) {
+            sw[q] = sw[q] + d[q];
+            y[q] = offsets_stored<M, T>() ? T(StateRef<M>::at(q)) + sw[q] : sw[q];
+        }
+    } else {
+        ok = ode_solve<INTEG, M::NS>(f, y, x0, x1, step, T(a.abs_tol), T(a.rel_tol),
+                                     a.max_steps);
+#pragma unroll
+        for (int q = 0; q < M::NS; ++q) sw[q] = y[q];
+    }
+    return ok;
+}
+
 // One launch: lanes advance by up to a.chunk steps (stopping at their run length).
 // BLOCK threads per block; WPE = minimum waves per SIMD the register budget must allow
-// (0: the compiler's choice).
-template <class M, class T, int BLOCK, int WPE>
+// (0: the compiler's choice); INTEG 0 Rush-Larsen, 1 RKF45.
+template <class M, class T, int BLOCK, int WPE, int INTEG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_cells_protocol(T* __restrict__ st, KParams<T> P, const T* __restrict__ gsc, int tissue,
                       BatchArgs a)
@@ -220,10 +263,24 @@ void k_cells_protocol(T* __restrict__ st, KParams<T> P, const T* __restrict__ gs
     const Stimulus stim = a.prot ? a.prot[cell] : a.stim;
     Detector<T> det{a, lane, cell, (T*)a.ring};
     if (a.det) det.load();
-    for (int64_t s = a.s0; s < s1; ++s) {
-        cell_step_stored<M>(y, sw, T(a.dt), T(stimulus_current((double)s * a.dt, stim)), p,
-                            tissue, gs);
-        if (a.det) det.push(s, (double)y[0] * a.vscale - a.voff);
+    if constexpr (INTEG == 0) {
+        for (int64_t s = a.s0; s < s1; ++s) {
+            cell_step_stored<M>(y, sw, T(a.dt), T(stimulus_current((double)s * a.dt, stim)),
+                                p, tissue, gs);
+            if (a.det) det.push(s, (double)y[0] * a.vscale - a.voff);
+        }
+    } else {
+        double step = a.step[cell];
+        bool ok = true;
+        for (int64_t s = a.s0; s < s1 && ok; ++s) {
+            if (!(step < a.dt)) step = a.dt;
+            ok = protocol_adaptive_step<M, T, INTEG>(y, sw, (double)s * a.dt,
+                                                     (double)(s + 1) * a.dt, step, a, stim, p,
+                                                     tissue, gs);
+            if (a.det) det.push(s, (double)y[0] * a.vscale - a.voff);
+        }
+        a.step[cell] = step;
+        if (!ok) *a.fail = 1;
     }
     store_states(sw, st, n, cell, 0);
     if (a.det) {
@@ -233,19 +290,23 @@ void k_cells_protocol(T* __restrict__ st, KParams<T> P, const T* __restrict__ gs
 }
 
 template <class T> struct BatchLaunch {
-    template <class M, int BLOCK, int WPE>
+    template <class M, int BLOCK, int WPE, int INTEG = 0>
     static hipError_t go(int tissue, void* states, const ModelParams& P, const void* gscale,
                          const BatchArgs& a, hipStream_t s)
     {
         const unsigned grid = (unsigned)((a.n + BLOCK - 1) / BLOCK);
-        hipLaunchKernelGGL((k_cells_protocol<M, T, BLOCK, WPE>), dim3(grid), dim3(BLOCK), 0, s,
-                           (T*)states, kparams<T>(P), (const T*)gscale, tissue, a);
+        hipLaunchKernelGGL((k_cells_protocol<M, T, BLOCK, WPE, INTEG>), dim3(grid), dim3(BLOCK),
+                           0, s, (T*)states, kparams<T>(P), (const T*)gscale, tissue, a);
         return hipGetLastError();
     }
+    // the adaptive integrators run one shape, one wave per block (the lanes of a wave take
+    // different numbers of sub-steps)
     template <class M>
     static hipError_t variant(int v, int tissue, void* states, const ModelParams& P,
                               const void* gscale, const BatchArgs& a, hipStream_t s)
     {
+        if (a.method == 1) return go<M, 64, 0, 1>(tissue, states, P, gscale, a, s);
+        if (a.method != 0) return hipErrorInvalidValue;   // Euler batches are refused
         switch (v) {
         case 0: return go<M, 256, 0>(tissue, states, P, gscale, a, s);
         case 1: return go<M, 64, 0>(tissue, states, P, gscale, a, s);

@@ -766,6 +766,14 @@ hipError_t batch_launch(cg_ionic* h, double dtm, int64_t s0, int chunk, hipStrea
         a.vscale = h->mi.vscale;
         a.voff = h->mi.voff;
     }
+    if (h->ode_kind != CG_ODE_RUSH_LARSEN) {
+        a.method = h->ode_kind;
+        a.abs_tol = h->abs_tol;
+        a.rel_tol = h->rel_tol;
+        a.max_steps = h->max_steps;
+        a.step = h->d_step;
+        a.fail = h->d_fail;
+    }
     return h->prec == CG_FP64
         ? cg::launch_cells_protocol_fp64(h->model, h->tissue, h->variant, h->d_states, h->P,
                                          h->d_gscale, a, s)
@@ -783,9 +791,17 @@ int batch_check(cg_ionic* const* hs, int nh, double dt_s)
         if (!hs[i]->lens_set) return fail(CG_ERR_STATE, "handle %d: run lengths not set", i);
         if (hs[i]->model == cg::MODEL_FDA)
             return fail(CG_ERR_STATE, "handle %d: the manufactured model has no pacing protocol", i);
-        if (hs[i]->ode_kind != CG_ODE_RUSH_LARSEN)
-            return fail(CG_ERR_STATE, "handle %d: batched protocols integrate with Rush-Larsen; "
-                                      "use cg_ionic_run_cells for RKF45", i);
+        // Euler batches exceeded maxSteps = 1e5 sub-steps in a 0.01 ms protocol step on
+        // TNNP, ORd and Courtemanche cells that the oracle's Euler integrates in about a
+        // second, for a reason not yet found; Gaur's FP64 RKF45 batch kernel returns wrong
+        // states as its tissue-mode twin does.  Both stay refused.
+        if (hs[i]->ode_kind == CG_ODE_EULER)
+            return fail(CG_ERR_STATE, "handle %d: batched protocols run Rush-Larsen or RKF45; "
+                                      "use cg_ionic_run_cells for Euler", i);
+        if (hs[i]->ode_kind == CG_ODE_RKF45 && hs[i]->model == cg::MODEL_GAUR &&
+            hs[i]->prec == CG_FP64)
+            return fail(CG_ERR_STATE, "handle %d: RKF45 batches are not available for Gaur on "
+                                      "FP64 handles", i);
         for (int j = 0; j < i; ++j)
             if (hs[j] == hs[i]) return fail(CG_ERR_ARG, "handle %d given twice", i);
     }
@@ -799,10 +815,13 @@ int cg_batch_run(cg_ionic* const* hs, int nh, double dt_s, int chunk)
     int rc = batch_check(hs, nh, dt_s);
     if (rc) return rc;
     if (chunk <= 0) return fail(CG_ERR_ARG, "chunk must be positive");
-    // beat detection state, cleared outside the graph
+    // beat detection state and the adaptive solvers' first trial steps, set outside the
+    // graph
     for (int i = 0; i < nh; ++i) {
         cg_ionic* h = hs[i];
         HIPCHK(hipStreamSynchronize(h->stream));
+        cg::OdeArgs o;
+        if (int rc2 = ode_args(h, 0.0, dt_s, &o)) return rc2;
         if (h->max_beats <= 0) continue;
         const double dtm = model_dt(h->model, dt_s);
         const int need = (int)std::ceil(h->beat_base_s * model_time(h->model, 1.0) / dtm) + 4;
@@ -855,6 +874,8 @@ int cg_batch_run(cg_ionic* const* hs, int nh, double dt_s, int chunk)
     if (err == hipSuccess) err = hipStreamSynchronize(s0);
     cleanup();
     if (err != hipSuccess) return fail(CG_ERR_HIP, "batched run: %s", hipGetErrorString(err));
+    for (int i = 0; i < nh; ++i)
+        if (int rc2 = check_ode(hs[i])) return rc2;
     return CG_OK;
 }
 
@@ -864,6 +885,10 @@ int cg_batch_autotune(cg_ionic* const* hs, int nh, double dt_s, int trial_steps,
     int rc = batch_check(hs, nh, dt_s);
     if (rc) return rc;
     if (trial_steps <= 0) return fail(CG_ERR_ARG, "trial_steps must be positive");
+    for (int i = 0; i < nh; ++i)
+        if (hs[i]->ode_kind != CG_ODE_RUSH_LARSEN)
+            return fail(CG_ERR_STATE, "handle %d: the adaptive solvers run one kernel shape; "
+                                      "autotuning is for Rush-Larsen handles", i);
     std::string rep;
     for (int i = 0; i < nh; ++i) {
         cg_ionic* h = hs[i];

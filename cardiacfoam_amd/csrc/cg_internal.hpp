@@ -151,6 +151,13 @@ struct BatchArgs {
     int max_beats;
     double thr, baseline, peak_window;  // mV per model time unit, model time units
     double vscale, voff;                // V (mV) = y[0] * vscale - voff
+    // integrator: 0 Rush-Larsen; 1 RKF45, 2 OpenFOAM's Euler, each step [s dt, (s+1) dt]
+    // from the cell's kept trial step (clamped to dt), as cg_ionic_run_cells does
+    int method;
+    double abs_tol, rel_tol;
+    long long max_steps;
+    double* step;          // [cell] kept trial steps (model time units)
+    int* fail;             // set to 1 by a cell whose integration failed
 };
 
 // batched single-cell protocol launch (batch_kernels.hpp); variant = kernel shape
