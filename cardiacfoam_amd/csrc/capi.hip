@@ -796,8 +796,8 @@ int batch_check(cg_ionic* const* hs, int nh, double dt_s)
         // second, for a reason not yet found; Gaur's FP64 RKF45 batch kernel returns wrong
         // states as its tissue-mode twin does.  Both stay refused.
         if (hs[i]->ode_kind == CG_ODE_EULER)
-            return fail(CG_ERR_STATE, "handle %d: batched protocols run Rush-Larsen or RKF45; "
-                                      "use cg_ionic_run_cells for Euler", i);
+            return fail(CG_ERR_STATE, "handle %d: batched protocols run Rush-Larsen or RKF45, "
+                                      "not Euler", i);
         if (hs[i]->ode_kind == CG_ODE_RKF45 && hs[i]->model == cg::MODEL_GAUR &&
             hs[i]->prec == CG_FP64)
             return fail(CG_ERR_STATE, "handle %d: RKF45 batches are not available for Gaur on "
