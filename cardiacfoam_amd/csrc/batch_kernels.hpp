@@ -239,7 +239,7 @@ CG_DEV bool protocol_adaptive_step(T (&y)[M::NS], T (&sw)[M::NS], double x0, dou
 
 // One launch: lanes advance by up to a.chunk steps (stopping at their run length).
 // BLOCK threads per block; WPE = minimum waves per SIMD the register budget must allow
-// (0: the compiler's choice); INTEG 0 Rush-Larsen, 1 RKF45.
+// (0: the compiler's choice); INTEG 0 Rush-Larsen, 1 RKF45, 2 Euler.
 template <class M, class T, int BLOCK, int WPE, int INTEG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_cells_protocol(T* __restrict__ st, KParams<T> P, const T* __restrict__ gsc, int tissue,
@@ -306,7 +306,7 @@ template <class T> struct BatchLaunch {
                               const void* gscale, const BatchArgs& a, hipStream_t s)
     {
         if (a.method == 1) return go<M, 64, 0, 1>(tissue, states, P, gscale, a, s);
-        if (a.method != 0) return hipErrorInvalidValue;   // Euler batches are refused
+        if (a.method == 2) return go<M, 64, 0, 2>(tissue, states, P, gscale, a, s);
         switch (v) {
         case 0: return go<M, 256, 0>(tissue, states, P, gscale, a, s);
         case 1: return go<M, 64, 0>(tissue, states, P, gscale, a, s);

@@ -791,13 +791,9 @@ int batch_check(cg_ionic* const* hs, int nh, double dt_s)
         if (!hs[i]->lens_set) return fail(CG_ERR_STATE, "handle %d: run lengths not set", i);
         if (hs[i]->model == cg::MODEL_FDA)
             return fail(CG_ERR_STATE, "handle %d: the manufactured model has no pacing protocol", i);
-        // Euler batches exceeded maxSteps = 1e5 sub-steps in a 0.01 ms protocol step on
-        // TNNP, ORd and Courtemanche cells that the oracle's Euler integrates in about a
-        // second, for a reason not yet found; Gaur's FP64 RKF45 batch kernel returns wrong
-        // states as its tissue-mode twin does.  Both stay refused.
-        if (hs[i]->ode_kind == CG_ODE_EULER)
-            return fail(CG_ERR_STATE, "handle %d: batched protocols run Rush-Larsen or RKF45, "
-                                      "not Euler", i);
+        // Gaur's FP64 RKF45 batch kernel failed (maxSteps) on cells whose oracle run
+        // takes ordinary steps, as its tissue-mode twin returns non-finite states (see
+        // cg_ionic_set_integrator); it stays refused
         if (hs[i]->ode_kind == CG_ODE_RKF45 && hs[i]->model == cg::MODEL_GAUR &&
             hs[i]->prec == CG_FP64)
             return fail(CG_ERR_STATE, "handle %d: RKF45 batches are not available for Gaur on "

@@ -133,10 +133,12 @@ int cg_ionic_set_stimulus(cg_ionic* h, const cg_stimulus* s);
  *   CG_ODE_EULER: OpenFOAM's `Euler` ODESolver under the same controller and kept steps:
  *     y = y0 + dx f(x0, y0) with the increment itself as the error estimate.
  * A failing cell (step-size underflow, more than max_steps sub-steps) makes the next
- * synchronising call return CG_ERR_ODE.  Batched protocols (cg_batch_run) run Rush-Larsen
- * and RKF45 (cg_batch_autotune only Rush-Larsen), each RKF45 protocol step [s dt, (s+1) dt]
- * from the cell's kept trial step; Euler handles, and RKF45 on FP64 Gaur handles, get
- * CG_ERR_STATE there.  CG_ERR_STATE for CG_ODE_RKF45 on an FP64 Gaur handle in tissue mode
+ * synchronising call return CG_ERR_ODE.  Batched protocols (cg_batch_run) run all three
+ * (cg_batch_autotune only Rush-Larsen), each adaptive protocol step [s dt, (s+1) dt] from
+ * the cell's kept trial step; RKF45 on FP64 Gaur handles gets CG_ERR_STATE there.  Euler
+ * takes sub-steps of at most abs_tol / |dV/dt| while V crosses 0 mV (the relative part of
+ * its tolerance vanishes there): millions of them per crossing at abs_tol 1e-6, so
+ * max_steps must allow for that.  CG_ERR_STATE for CG_ODE_RKF45 on an FP64 Gaur handle in tissue mode
  * (solve_vm 0): that kernel is switched off (its gfx950 build gives non-finite states
  * where the same source on the host does not); mixed precision and single cells run. */
 int cg_ionic_set_integrator(cg_ionic* h, int kind, double abs_tol, double rel_tol,

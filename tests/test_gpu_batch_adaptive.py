@@ -1,12 +1,14 @@
-"""Batched single cells (cg_batch_run) under RKF45 per protocol step, each cell keeping
-its trial step, against the oracle's single-cell run with the same integrator,
-tolerances and protocol; Euler batches and FP64 Gaur RKF45 batches are refused.
+"""Batched single cells (cg_batch_run) under RKF45 and OpenFOAM's Euler per protocol step,
+each cell keeping its trial step, against the oracle's single-cell run with the same
+integrator, tolerances and protocol; FP64 Gaur RKF45 batches are refused.
 
 The device's protocol steps start at s*dt (model units) and the oracle's at the
 accumulated time ((t + dt) - dt in seconds); the ulp-level difference moves the accepted
 sub-steps next to a stimulus edge, after which two correct implementations agree to about
-relTol (the single-cell RKF45 tests measure the same), so the bound is 10 relTol at
-absTol 1e-10, relTol 1e-8."""
+relTol (the single-cell RKF45 tests measure the same), so the bound is 10 (relTol |y| +
+absTol): RKF45 at absTol 1e-10, relTol 1e-8; Euler at absTol 1e-3, relTol 1e-4 (Euler's
+error estimate is the whole increment, so while V crosses 0 mV its sub-steps are at most
+absTol / |dV/dt|)."""
 import sys
 
 import numpy as np
@@ -25,7 +27,7 @@ def _say(*a):
     sys.stdout.flush()
 
 
-TOL = {"RKF45": (1e-10, 1e-8)}
+TOL = {"RKF45": (1e-10, 1e-8), "Euler": (1e-3, 1e-4)}
 
 
 def _case(gpu, name, precision, solver, n, seed, tol):
@@ -38,16 +40,16 @@ def _case(gpu, name, precision, solver, n, seed, tol):
     ion = gpu.IonicModel(key, word, n, solve_vm=True, precision=precision)
     ion.set_protocols(prots)
     ion.set_run_lengths(np.asarray(lens, dtype=np.int64))
-    ion.set_integrator(solver, tol[0], tol[1], 100000)
+    ion.set_integrator(solver, tol[0], tol[1], 1000000)
     ion.enable_beats(max_beats=4, dvdt_threshold=5.0, baseline_s=2e-3, peak_window_s=2.0)
     return ion, prots, lens
 
 
-@pytest.mark.parametrize("solver", ["RKF45"])
+@pytest.mark.parametrize("solver", ["RKF45", "Euler"])
 @pytest.mark.parametrize("name", ["TNNP_epi", "ORd", "CRN"])
 def test_batch_adaptive_fp64_vs_oracle(gpu, orc, name, solver):
     """FP64: each cell's final states equal the oracle's single-cell run with the same
-    integrator to 10 relTol; one beat per cell."""
+    integrator to 10 (relTol |y| + absTol); one beat per cell."""
     _say("batch adaptive fp64", name, solver)
     m, tis = MODELS[name][0], MODELS[name][1]
     n = 3
@@ -65,10 +67,11 @@ def test_batch_adaptive_fp64_vs_oracle(gpu, orc, name, solver):
         _, _, y = orc.single_cell_run(m, tis, st, DT_MS * 1e-3, int(lens[i]), integrator=integ,
                                       abs_tol=at, rel_tol=rt)
         rel = np.max(np.abs(got[:, i] - y) / (np.abs(y) + 1e-12))
+        excess = np.max(np.abs(got[:, i] - y) / (rt * np.abs(y) + at))
         _say("  cell %d: %d steps, %d beats, Vm %.9f (oracle %.9f), worst state rel %.3e"
              % (i, lens[i], cnt[i], got[0, i], y[0], rel))
         assert cnt[i] == 1
-        assert rel <= 10 * rt
+        assert excess <= 10.0, excess
 
 
 @pytest.mark.parametrize("name", ["TNNP_epi", "ORd", "CRN"])
@@ -102,9 +105,8 @@ def test_batch_autotune_refuses_adaptive(gpu):
         gpu.batch_autotune([ion], DT_MS * 1e-3, trial_steps=10)
 
 
-def test_batch_refuses_euler_and_fp64_gaur_rkf45(gpu):
-    for key, word, solver, prec, msg in (("TNNP", "epicardialCells", "Euler", "fp64", "Euler"),
-                                         ("Gaur", "myocyte", "RKF45", "fp64", "Gaur")):
+def test_batch_refuses_fp64_gaur_rkf45(gpu):
+    for key, word, solver, prec, msg in (("Gaur", "myocyte", "RKF45", "fp64", "Gaur"),):
         ion = gpu.IonicModel(key, word, 4, solve_vm=True, precision=prec)
         ion.set_protocols([{"stim_start": 5.0, "stim_period_S1": 1000.0,
                             "stim_duration": 1.0, "stim_amplitude": 40.0, "nstim1": 1}] * 4)
