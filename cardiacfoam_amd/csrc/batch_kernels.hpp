@@ -206,23 +206,21 @@ CG_DEV bool protocol_adaptive_step(T (&y)[M::NS], T (&sw)[M::NS], double x0, dou
                                    const T* p, int tissue, const T* gs)
 {
     constexpr bool f32 = sizeof(T) == 4;
-    T base[M::NS], d[M::NS];
+    T d[M::NS];
+    // FP32: y holds the step-start values, unchanged until the solve ends
     auto f = [&](double t, const T* ss, T* r) __attribute__((always_inline)) {
         T yy[M::NS];
 #pragma unroll
-        for (int q = 0; q < M::NS; ++q) yy[q] = f32 ? base[q] + ss[q] : ss[q];
+        for (int q = 0; q < M::NS; ++q) yy[q] = f32 ? y[q] + ss[q] : ss[q];
         Ops<M>::deriv(yy, r, p, tissue, gs, true, T(stimulus_current(t, stim)),
                       Ops<M>::vt(yy[0], p));
     };
     bool ok;
     if constexpr (f32) {
 #pragma unroll
-        for (int q = 0; q < M::NS; ++q) {
-            base[q] = y[q];
-            d[q] = T(0);
-        }
+        for (int q = 0; q < M::NS; ++q) d[q] = T(0);
         ok = ode_solve<INTEG, M::NS>(f, d, x0, x1, step, T(a.abs_tol), T(a.rel_tol),
-                                     a.max_steps, base);
+                                     a.max_steps, y);
 #pragma unroll
         for (int q = 0; q < M::NS; ++q) {
             sw[q] = sw[q] + d[q];

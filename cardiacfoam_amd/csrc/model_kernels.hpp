@@ -99,28 +99,25 @@ CG_DEV T tissue_advance(T v, T (&y)[M::NS], T (&s)[M::NS], T dt, const T* p, int
         // step-start values (rates at base + d, the norm on those values), then add d to
         // each stored word once: a sub-step's increment below half an FP32 ulp of the state
         // would otherwise be rounded away (many short Euler sub-steps leave slow states
-        // frozen), while d resolves it
-        T base[M::NS], d[M::NS];
+        // frozen), while d resolves it.  y itself is the step-start base (V set to v): it
+        // is not written until the solve ends, so no second copy is kept live.
+        T d[M::NS];
 #pragma unroll
-        for (int q = 0; q < M::NS; ++q) {
-            base[q] = y[q];
-            d[q] = T(0);
-        }
-        base[0] = v;
+        for (int q = 0; q < M::NS; ++q) d[q] = T(0);
+        y[0] = v;
         const auto qv = Ops<M>::vt(v, p);
         auto f = [&](double, const T* dd, T* r) __attribute__((always_inline)) {
             T yy[M::NS];
 #pragma unroll
-            for (int q = 0; q < M::NS; ++q) yy[q] = base[q] + dd[q];
+            for (int q = 0; q < M::NS; ++q) yy[q] = y[q] + dd[q];
             Ops<M>::deriv(yy, r, p, tissue, gs, false, T(0), qv);
         };
         double step = o.step[c];
         if (!(step < o.dt_clamp)) step = o.dt_clamp;
         const bool ok = ode_solve<INTEG, M::NS>(f, d, o.t0, o.t1, step, T(o.abs_tol),
-                                                T(o.rel_tol), o.max_steps, base);
+                                                T(o.rel_tol), o.max_steps, y);
         o.step[c] = step;
         if (!ok) *o.fail = 1;
-        y[0] = v;
         s[0] = v;
 #pragma unroll
         for (int q = 1; q < M::NS; ++q) {
@@ -225,12 +222,13 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
         // each step integrates its increments d from 0 against the step-start values
         // (rates at base + d, norm on those values) and adds d to the stored words once,
         // so sub-step increments below half an ulp of a state still count.
+        // (FP32: y holds the step-start values, unchanged until that step's solve ends)
         constexpr bool f32 = sizeof(T) == 4;
-        T base[M::NS], d[M::NS];
+        T d[M::NS];
         auto f = [&](double t, const T* ss, T* r) __attribute__((always_inline)) {
             T yy[M::NS];
 #pragma unroll
-            for (int q = 0; q < M::NS; ++q) yy[q] = f32 ? base[q] + ss[q] : ss[q];
+            for (int q = 0; q < M::NS; ++q) yy[q] = f32 ? y[q] + ss[q] : ss[q];
             const auto q = Ops<M>::vt(yy[0], p);
             Ops<M>::deriv(yy, r, p, tissue, gs, true, T(stimulus_current(t, stim)), q);
         };
@@ -250,12 +248,9 @@ __global__ __launch_bounds__(MBS) void k_ionic_cells(T* __restrict__ st, int64_t
             if (!(step < ode.dt_clamp)) step = ode.dt_clamp;
             if constexpr (f32) {
 #pragma unroll
-                for (int q = 0; q < M::NS; ++q) {
-                    base[q] = y[q];
-                    d[q] = T(0);
-                }
+                for (int q = 0; q < M::NS; ++q) d[q] = T(0);
                 ok = ode_solve<INTEG, M::NS>(f, d, x0, x1, step, T(ode.abs_tol),
-                                             T(ode.rel_tol), ode.max_steps, base);
+                                             T(ode.rel_tol), ode.max_steps, y);
 #pragma unroll
                 for (int q = 0; q < M::NS; ++q) {
                     sw[q] = sw[q] + d[q];
